@@ -1,0 +1,15 @@
+"""MI355X (gfx950) smoke solver: a drop-in for the reference's `pub mod fluid`
+(src/lib.rs:4, src/fluid.rs).  The compute path is the HIP library
+``libaether_fluid_hip.so`` behind include/aether_fluid.h; this package is its
+host-side mirror."""
+from ._lib import (AetherFluidError, FIELDS, FP32, FP32_IR64, FP64, SOLVER_CG, SOLVER_RBGS,
+                   SOURCE_JETS, SOURCE_NONE, SOURCE_SPHERE, LIB_PATH, SIGNATURES, build, lib)
+from .solver import FluidSolver3D, FluidTelemetryFrame, PoissonSolverKind, rng_fill
+from .scenarios import seeded_config_state, SMOKE_CONFIGS
+
+__all__ = [
+    "AetherFluidError", "FIELDS", "FP32", "FP32_IR64", "FP64", "SOLVER_CG", "SOLVER_RBGS",
+    "SOURCE_JETS", "SOURCE_NONE", "SOURCE_SPHERE", "LIB_PATH", "SIGNATURES", "build", "lib",
+    "FluidSolver3D", "FluidTelemetryFrame", "PoissonSolverKind", "rng_fill",
+    "seeded_config_state", "SMOKE_CONFIGS",
+]

@@ -1,0 +1,272 @@
+"""Python mirror of the reference's ``FluidSolver3D`` (src/fluid.rs:23-673)
+over the C ABI of include/aether_fluid.h.
+
+Names, argument meaning and error behaviour follow the reference:
+``FluidSolver3D(nx, ny, nz)`` raises for dimensions below 6 (the reference
+asserts, src/fluid.rs:54); the pub knobs ``vorticity_eps``, ``buoyancy_beta``,
+``gravity_alpha``, ``solver_kind`` and ``solver_iters`` are plain attributes
+(src/fluid.rs:45-49) pushed to the device before the next call; the pub
+arrays ``u, v, w, density, temperature, pressure, divergence`` are
+properties that download (get) or upload (set) flat f64 arrays indexed by
+``idx(i, j, k) = (k*ny + j)*nx + i``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import enum
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib as L
+
+
+class PoissonSolverKind(enum.IntEnum):
+    """src/fluid.rs:17-21"""
+    ConjugateGradient = L.SOLVER_CG
+    RedBlackGaussSeidel = L.SOLVER_RBGS
+
+
+@dataclass
+class FluidTelemetryFrame:
+    """src/fluid.rs:6-15"""
+    time_sec: float
+    kinetic_energy: float
+    enstrophy: float
+    peak_vorticity: float
+    total_mass: float
+    center_of_mass: tuple
+
+    @classmethod
+    def _from(cls, t: L.Telemetry) -> "FluidTelemetryFrame":
+        return cls(t.time_sec, t.kinetic_energy, t.enstrophy, t.peak_vorticity, t.total_mass,
+                   tuple(t.center_of_mass))
+
+    def as_array(self) -> np.ndarray:
+        return np.array([self.time_sec, self.kinetic_energy, self.enstrophy,
+                         self.peak_vorticity, self.total_mass, *self.center_of_mass])
+
+
+_PARAM_NAMES = ("vorticity_eps", "buoyancy_beta", "gravity_alpha", "solver_kind",
+                "solver_iters", "sor_omega", "cg_rel_tol", "dissipation_velocity",
+                "dissipation_density", "dissipation_temperature", "source_kind",
+                "source_center", "source_radius", "source_density", "source_temperature",
+                "ir_inner_rel_tol", "ir_div_tol", "ir_max_outer")
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class FluidSolver3D:
+    """Device-resident smoke solver; see the module docstring."""
+
+    def __init__(self, nx: int, ny: int, nz: int, h: float = 0.0, precision: str = "fp32"):
+        prec = {"fp32": L.FP32, "fp64": L.FP64, "fp32_ir64": L.FP32_IR64}[precision]
+        self._lib = L.lib()
+        handle = C.c_void_p()
+        L.check(self._lib.aether_fluid_create(nx, ny, nz, float(h), prec, C.byref(handle)))
+        self._h = handle
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.n = nx * ny * nz
+        hh = C.c_double()
+        L.check(self._lib.aether_fluid_dims(self._h, None, None, None, C.byref(hh), None))
+        self.h = hh.value
+        self.precision = precision
+        p = L.Params()
+        L.check(self._lib.aether_fluid_default_params(C.byref(p)))
+        for name in _PARAM_NAMES:
+            val = getattr(p, name)
+            if name == "source_center":
+                val = tuple(val)
+            object.__setattr__(self, name, val)
+        self._pushed = None
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.aether_fluid_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ----------------------------------------------------------------- params
+    def _params(self) -> L.Params:
+        p = L.Params()
+        for name in _PARAM_NAMES:
+            val = getattr(self, name)
+            if name == "source_center":
+                p.source_center = (C.c_double * 3)(*[float(x) for x in val])
+            elif name in ("solver_kind", "solver_iters", "source_kind", "ir_max_outer"):
+                setattr(p, name, int(val))
+            else:
+                setattr(p, name, float(val))
+        return p
+
+    def _sync_params(self) -> None:
+        key = tuple(getattr(self, n) for n in _PARAM_NAMES)
+        if key != self._pushed:
+            p = self._params()
+            L.check(self._lib.aether_fluid_set_params(self._h, C.byref(p)))
+            self._pushed = key
+
+    def set_sphere_source(self, center, radius: float, density: float = 1.0,
+                          temperature: float = 1.0) -> None:
+        """Sphere source that sets density and temperature inside d^2 < r^2."""
+        self.source_kind = L.SOURCE_SPHERE
+        self.source_center = tuple(float(c) for c in center)
+        self.source_radius = float(radius)
+        self.source_density = float(density)
+        self.source_temperature = float(temperature)
+
+    # ----------------------------------------------------------------- fields
+    def idx(self, i: int, j: int, k: int) -> int:
+        """src/fluid.rs:88-90"""
+        return (k * self.ny + j) * self.nx + i
+
+    def download(self, name: str) -> np.ndarray:
+        out = np.empty(self.n, dtype=np.float64)
+        L.check(self._lib.aether_fluid_download(self._h, L.FIELDS[name], _dptr(out), self.n))
+        return out
+
+    def upload(self, name: str, values) -> None:
+        a = np.ascontiguousarray(np.asarray(values, dtype=np.float64).reshape(-1))
+        L.check(self._lib.aether_fluid_upload(self._h, L.FIELDS[name], _dptr(a), a.size))
+
+    def _field_prop(name):  # noqa: N805 - property factory
+        return property(lambda self: self.download(name),
+                        lambda self, val: self.upload(name, val))
+
+    u = _field_prop("u")
+    v = _field_prop("v")
+    w = _field_prop("w")
+    density = _field_prop("density")
+    temperature = _field_prop("temperature")
+    pressure = _field_prop("pressure")
+    divergence = _field_prop("divergence")
+    del _field_prop
+
+    # ---------------------------------------------------------------- methods
+    def step(self, sim_dt: float, timeline_sec: float) -> FluidTelemetryFrame:
+        """src/fluid.rs:598-611"""
+        self._sync_params()
+        t = L.Telemetry()
+        L.check(self._lib.aether_fluid_step(self._h, sim_dt, timeline_sec, C.byref(t)))
+        return FluidTelemetryFrame._from(t)
+
+    def run(self, sim_dt: float, times) -> list:
+        """Many steps with one synchronisation (the src/main.rs:89-94 loop)."""
+        self._sync_params()
+        ts = np.ascontiguousarray(np.asarray(times, dtype=np.float64))
+        out = (L.Telemetry * max(len(ts), 1))()
+        L.check(self._lib.aether_fluid_run(self._h, sim_dt, _dptr(ts), len(ts), out))
+        return [FluidTelemetryFrame._from(out[a]) for a in range(len(ts))]
+
+    def advect_maccormack_in_place(self, field_kind: int, dt: float, dissipation: float):
+        """src/fluid.rs:170-234"""
+        L.check(self._lib.aether_fluid_advect_maccormack_in_place(self._h, field_kind, dt,
+                                                                  dissipation))
+
+    def enforce_wall_velocity_bc(self) -> None:
+        """src/fluid.rs:236-253"""
+        L.check(self._lib.aether_fluid_enforce_wall_velocity_bc(self._h))
+
+    def apply_buoyancy_and_vorticity_confinement(self, dt: float) -> None:
+        """src/fluid.rs:256-330"""
+        self._sync_params()
+        L.check(self._lib.aether_fluid_apply_buoyancy_and_vorticity_confinement(self._h, dt))
+
+    def compute_divergence_norm(self) -> float:
+        """src/fluid.rs:333-353"""
+        out = C.c_double()
+        L.check(self._lib.aether_fluid_compute_divergence_norm(self._h, C.byref(out)))
+        return out.value
+
+    def divergence_max(self) -> float:
+        out = C.c_double()
+        L.check(self._lib.aether_fluid_divergence_max(self._h, C.byref(out)))
+        return out.value
+
+    def project(self) -> None:
+        """src/fluid.rs:496-529"""
+        self._sync_params()
+        L.check(self._lib.aether_fluid_project(self._h))
+
+    def solve_poisson_conjugate_gradient(self, max_iters: int) -> None:
+        """src/fluid.rs:445-493 (on the stored divergence and pressure)"""
+        self._sync_params()
+        L.check(self._lib.aether_fluid_solve_poisson_conjugate_gradient(self._h, max_iters))
+
+    def solve_poisson_gauss_seidel(self, iterations: int) -> None:
+        """src/fluid.rs:397-442"""
+        self._sync_params()
+        L.check(self._lib.aether_fluid_solve_poisson_gauss_seidel(self._h, iterations))
+
+    def inject_sources(self, time_sec: float, dt: float) -> None:
+        self._sync_params()
+        L.check(self._lib.aether_fluid_inject_sources(self._h, time_sec, dt))
+
+    def inject_cyberpunk_sources(self, time_sec: float, dt: float) -> None:
+        """src/fluid.rs:532-595 (the reference's jets, whatever source_kind says)"""
+        saved = self.source_kind
+        self.source_kind = L.SOURCE_JETS
+        try:
+            self.inject_sources(time_sec, dt)
+        finally:
+            self.source_kind = saved
+
+    def compute_telemetry(self, timeline_sec: float) -> FluidTelemetryFrame:
+        """src/fluid.rs:613-672"""
+        t = L.Telemetry()
+        L.check(self._lib.aether_fluid_compute_telemetry(self._h, timeline_sec, C.byref(t)))
+        return FluidTelemetryFrame._from(t)
+
+    def sample_trilinear(self, field: str, x, y=None, z=None):
+        """src/fluid.rs:94-128; scalar (x, y, z) or an (n, 3) array of points."""
+        pts = np.asarray([[x, y, z]] if y is not None else x, dtype=np.float64).reshape(-1, 3)
+        pts = np.ascontiguousarray(pts)
+        out = np.empty(len(pts), dtype=np.float64)
+        L.check(self._lib.aether_fluid_sample_trilinear(self._h, L.FIELDS[field], _dptr(pts),
+                                                        len(pts), _dptr(out)))
+        return float(out[0]) if y is not None else out
+
+    def max_density(self) -> float:
+        out = C.c_double()
+        L.check(self._lib.aether_fluid_max_density(self._h, C.byref(out)))
+        return out.value
+
+    def last_solve_stats(self) -> dict:
+        s = L.SolveStats()
+        L.check(self._lib.aether_fluid_last_solve_stats(self._h, C.byref(s)))
+        return {"iters": s.iterations, "converged": bool(s.converged), "b2": s.b_norm2,
+                "r2_init": s.r_norm2_initial, "r2": s.r_norm2_final,
+                "outer": s.outer_iterations, "div_max": s.div_max_final}
+
+    def set_kernel_timing(self, enable: bool) -> None:
+        L.check(self._lib.aether_fluid_set_kernel_timing(self._h, int(enable)))
+
+    def kernel_timing(self, which: int) -> tuple:
+        avg = C.c_double()
+        cnt = C.c_long()
+        L.check(self._lib.aether_fluid_kernel_timing(self._h, which, C.byref(avg),
+                                                     C.byref(cnt)))
+        return avg.value, cnt.value
+
+
+def rng_fill(seed_or_state, lo: float, hi: float, n: int):
+    """Draw n range_f64(lo, hi) values from the reference's seeded stream
+    (src/rng.rs:21-33, 86-88).  Returns (values, state) so that draws can
+    continue from the same stream."""
+    lib = L.lib()
+    st = (C.c_uint64 * 4)()
+    if isinstance(seed_or_state, int):
+        L.check(lib.aether_rng_seed(C.c_uint64(seed_or_state), st))
+    else:
+        for a in range(4):
+            st[a] = seed_or_state[a]
+    out = np.empty(n, dtype=np.float64)
+    L.check(lib.aether_rng_fill_range(st, lo, hi, _dptr(out), n))
+    return out, [st[a] for a in range(4)]

@@ -1,0 +1,944 @@
+// fluid_api.hip -- host side of the MI355X smoke solver behind the C ABI of
+// include/aether_fluid.h.
+//
+// One handle = one grid, one HIP stream and its device buffers.  All state
+// stays resident in HBM; fields cross to the host only through
+// upload/download.  The stage sequence of step() follows
+// FluidSolver3D::step (src/fluid.rs:598-611); the CG driver keeps every
+// scalar (alpha, beta, r.r, the stop flag) on the device, so a solve is a
+// fixed sequence of launches with no host round trip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "aether_fluid.h"
+#include "fluid_kernels.hpp"
+
+namespace aether {
+namespace {
+
+thread_local std::string g_last_error;
+
+struct Failure {
+    int code;
+    std::string msg;
+};
+
+[[noreturn]] void raise(int code, const std::string &msg) { throw Failure{code, msg}; }
+
+#define HIPT(expr)                                                                        \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            raise(e_ == hipErrorOutOfMemory ? AETHER_ENOMEM : AETHER_EHIP,                \
+                  std::string(#expr) + ": " + hipGetErrorString(e_) + " (line " +         \
+                      std::to_string(__LINE__) + ")");                                    \
+    } while (0)
+
+#define LAUNCH_CHECK() HIPT(hipGetLastError())
+
+template <typename F>
+int guarded(F &&f) {
+    try {
+        f();
+        return AETHER_OK;
+    } catch (const Failure &e) {
+        g_last_error = e.msg;
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        g_last_error = "host allocation failed";
+        return AETHER_ENOMEM;
+    }
+}
+
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t count) { alloc(count); }
+    void alloc(size_t count) {
+        release();
+        if (count) HIPT(hipMalloc((void **)&p, count * sizeof(T)));
+        n = count;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DevBuf() { release(); }
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+};
+
+aether_fluid_params default_params() {
+    aether_fluid_params p;
+    std::memset(&p, 0, sizeof(p));
+    p.vorticity_eps = 5.5;  // src/fluid.rs:79-83
+    p.buoyancy_beta = 6.8;
+    p.gravity_alpha = 0.45;
+    p.solver_kind = AETHER_SOLVER_CG;
+    p.solver_iters = 32;
+    p.sor_omega = 1.65;  // src/fluid.rs:398
+    p.cg_rel_tol = 0.0;
+    p.dissipation_velocity = 0.996;  // src/fluid.rs:599-607
+    p.dissipation_density = 0.996;
+    p.dissipation_temperature = 0.985;  // src/fluid.rs:608
+    p.source_kind = AETHER_SOURCE_JETS;
+    p.source_radius = 0.0;
+    p.source_density = 1.0;
+    p.source_temperature = 1.0;
+    p.ir_inner_rel_tol = 1e-3;
+    p.ir_div_tol = 1e-7;
+    p.ir_max_outer = 30;
+    return p;
+}
+
+void check_params(const aether_fluid_params &p) {
+    if (p.solver_kind != AETHER_SOLVER_CG && p.solver_kind != AETHER_SOLVER_RBGS)
+        raise(AETHER_EINVAL, "solver_kind must be AETHER_SOLVER_CG or AETHER_SOLVER_RBGS");
+    if (p.solver_iters < 0) raise(AETHER_EINVAL, "solver_iters must be >= 0");
+    if (p.source_kind < AETHER_SOURCE_JETS || p.source_kind > AETHER_SOURCE_NONE)
+        raise(AETHER_EINVAL, "unknown source_kind");
+    if (!(p.cg_rel_tol >= 0.0)) raise(AETHER_EINVAL, "cg_rel_tol must be >= 0");
+}
+
+// Kernel timing accumulator (HIP events on the solver's stream).
+struct KernelTimer {
+    bool enabled = false;
+    std::vector<hipEvent_t> pool;
+    struct Rec {
+        int which, step, it;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> recs;
+    size_t used = 0;
+    double sum_ms[2] = {0, 0};
+    long count[2] = {0, 0};
+    hipEvent_t get() {
+        if (used == pool.size()) {
+            hipEvent_t e;
+            HIPT(hipEventCreate(&e));
+            pool.push_back(e);
+        }
+        return pool[used++];
+    }
+    void reset() {
+        recs.clear();
+        used = 0;
+        sum_ms[0] = sum_ms[1] = 0;
+        count[0] = count[1] = 0;
+    }
+    ~KernelTimer() {
+        for (auto e : pool) (void)hipEventDestroy(e);
+    }
+};
+
+class SolverBase {
+   public:
+    virtual ~SolverBase() = default;
+    virtual void advect(int kind, double dt, double diss) = 0;
+    virtual void enforce_bc() = 0;
+    virtual void buoyancy_confinement(double dt) = 0;
+    virtual double divergence_norm() = 0;
+    virtual double divergence_max() = 0;
+    virtual void project() = 0;
+    virtual void solve_cg(int iters) = 0;
+    virtual void solve_rbgs(int sweeps) = 0;
+    virtual void inject_sources(double t, double dt) = 0;
+    virtual void telemetry(double t, aether_fluid_telemetry *out) = 0;
+    virtual void run(double dt, const double *times, int nsteps, aether_fluid_telemetry *out) = 0;
+    virtual void upload(int field, const double *src) = 0;
+    virtual void download(int field, double *dst) = 0;
+    virtual void sample(int field, const double *xyz, size_t npts, double *out) = 0;
+    virtual double max_density() = 0;
+    virtual void stats(aether_fluid_solve_stats *out) = 0;
+    aether_fluid_params params = default_params();
+    KernelTimer timer;
+};
+
+template <typename T>
+class Solver final : public SolverBase {
+   public:
+    Solver(int nx, int ny, int nz, double h, hipStream_t stream) : h_(h), st_(stream) {
+        g_.nx = nx;
+        g_.ny = ny;
+        g_.nz = nz;
+        g_.sxy = nx * ny;
+        n_ = (size_t)nx * ny * nz;
+        cell_blk_ = dim3(kBX, kBY, 1);
+        const int gx = (nx + kBX - 1) / kBX, gy = (ny + kBY - 1) / kBY;
+        cell_grid_ = dim3(gx, gy, nz);
+        // reduction launches: each workgroup marches a chunk of z planes so
+        // that a solve sees ~1k partial sums at most
+        const int cols = gx * gy;
+        int nch = std::max(1, std::min(nz, (1024 + cols - 1) / cols));
+        zchunk_ = (nz + nch - 1) / nch;
+        nch = (nz + zchunk_ - 1) / zchunk_;
+        red_grid_ = dim3(gx, gy, nch);
+        nb_ = cols * nch;
+        for (int a = 0; a < 5; ++a) {
+            f_[a].alloc(n_);
+            alt_[a].alloc(n_);
+            cur_[a] = f_[a].p;
+            nxt_[a] = alt_[a].p;
+        }
+        fwd_[0].alloc(n_);
+        fwd_[1].alloc(n_);
+        curl_.alloc(n_);
+        p_.alloc(n_);
+        b_.alloc(n_);
+        r_.alloc(n_);
+        d_[0].alloc(n_);
+        d_[1].alloc(n_);
+        partA_.alloc(nb_);
+        partB_.alloc(nb_);
+        partC_.alloc(nb_);
+        part_tel_.alloc((size_t)nb_ * kTelParts);
+        scalar_.alloc(8);
+        tel_.alloc(8);
+        cg_.alloc(1);
+        iters_log_.alloc(1);
+        for (int a = 0; a < 5; ++a) zero(cur_[a]);
+        zero(p_.p);
+        zero(b_.p);
+        zero(r_.p);
+        zero(d_[0].p);
+        zero(d_[1].p);
+        HIPT(hipMemsetAsync(cg_.p, 0, sizeof(CgState), st_));
+        sync();
+    }
+
+    void zero(T *x) {
+        k_fill<T><<<blocks_for(n_), kThreads, 0, st_>>>(x, (long)n_, T(0));
+        LAUNCH_CHECK();
+    }
+    static unsigned blocks_for(size_t n) {
+        size_t b = (n + kThreads - 1) / kThreads;
+        return (unsigned)std::min<size_t>(std::max<size_t>(b, 1), 4096);
+    }
+    void sync() { HIPT(hipStreamSynchronize(st_)); }
+
+    // --- stages ----------------------------------------------------------
+    void advect(int kind, double dt, double diss) override {
+        if (kind < 0 || kind > 4) raise(AETHER_EINVAL, "field_kind must be 0..4");
+        enqueue_advect1(kind, dt, diss);
+        sync();
+    }
+    void enqueue_advect1(int kind, double dt, double diss) {
+        const T s = (T)(dt / h_);  // src/fluid.rs:154
+        T *src = cur_[kind];
+        k_sl_forward<T, 1><<<cell_grid_, cell_blk_, 0, st_>>>(g_, s, cur_[0], cur_[1], cur_[2],
+                                                              src, nullptr, fwd_[0].p, nullptr);
+        LAUNCH_CHECK();
+        k_mac_correct<T, 1><<<cell_grid_, cell_blk_, 0, st_>>>(
+            g_, s, cur_[0], cur_[1], cur_[2], src, fwd_[0].p, nxt_[kind], (T)diss, nullptr,
+            nullptr, nullptr, T(0), kind >= 3 ? 1 : 0);
+        LAUNCH_CHECK();
+        std::swap(cur_[kind], nxt_[kind]);
+    }
+    // density and temperature share one velocity: one fused pass pair
+    void enqueue_advect_scalars(double dt) {
+        const T s = (T)(dt / h_);
+        k_sl_forward<T, 2><<<cell_grid_, cell_blk_, 0, st_>>>(
+            g_, s, cur_[0], cur_[1], cur_[2], cur_[3], cur_[4], fwd_[0].p, fwd_[1].p);
+        LAUNCH_CHECK();
+        k_mac_correct<T, 2><<<cell_grid_, cell_blk_, 0, st_>>>(
+            g_, s, cur_[0], cur_[1], cur_[2], cur_[3], fwd_[0].p, nxt_[3],
+            (T)params.dissipation_density, cur_[4], fwd_[1].p, nxt_[4],
+            (T)params.dissipation_temperature, 1);
+        LAUNCH_CHECK();
+        std::swap(cur_[3], nxt_[3]);
+        std::swap(cur_[4], nxt_[4]);
+    }
+
+    void enforce_bc() override {
+        k_enforce_bc<T><<<cell_grid_, cell_blk_, 0, st_>>>(g_, cur_[0], cur_[1], cur_[2]);
+        LAUNCH_CHECK();
+        sync();
+    }
+
+    void buoyancy_confinement(double dt) override {
+        enqueue_buoyancy_confinement(dt);
+        sync();
+    }
+    void enqueue_buoyancy_confinement(double dt) {
+        const double inv_2h = 0.5 / h_;                    // src/fluid.rs:257
+        const double eps_h = params.vorticity_eps * h_;   // src/fluid.rs:284
+        k_curl_mag<T><<<cell_grid_, cell_blk_, 0, st_>>>(g_, (T)inv_2h, cur_[0], cur_[1],
+                                                          cur_[2], curl_.p);
+        LAUNCH_CHECK();
+        k_confine_buoy_bc<T><<<cell_grid_, cell_blk_, 0, st_>>>(
+            g_, (T)dt, (T)inv_2h, (T)eps_h, (T)params.buoyancy_beta, (T)params.gravity_alpha,
+            cur_[0], cur_[1], cur_[2], curl_.p, cur_[3], cur_[4], nxt_[0], nxt_[1], nxt_[2]);
+        LAUNCH_CHECK();
+        for (int a = 0; a < 3; ++a) std::swap(cur_[a], nxt_[a]);
+    }
+
+    double read_scalar() {
+        double v = 0;
+        HIPT(hipMemcpyAsync(&v, scalar_.p, sizeof(double), hipMemcpyDeviceToHost, st_));
+        sync();
+        return v;
+    }
+
+    double divergence_norm() override {
+        // src/fluid.rs:333-353: stores b and returns the interior RMS of div u
+        k_divergence<T><<<red_grid_, cell_blk_, 0, st_>>>(g_, zchunk_, (T)h_, (T)(1.0 / h_), 0,
+                                                           cur_[0], cur_[1], cur_[2], b_.p,
+                                                           partA_.p);
+        LAUNCH_CHECK();
+        k_partials_final<<<1, kThreads, 0, st_>>>(partA_.p, nb_, 0, scalar_.p);
+        LAUNCH_CHECK();
+        const double sum_sq = read_scalar();
+        const double interior_n = (double)((size_t)(g_.nx - 2) * (g_.ny - 2) * (g_.nz - 2));
+        return std::sqrt(sum_sq / interior_n);
+    }
+
+    double divergence_max() override {
+        k_div_max<T><<<red_grid_, cell_blk_, 0, st_>>>(g_, zchunk_, (T)(1.0 / h_), cur_[0],
+                                                        cur_[1], cur_[2], partA_.p);
+        LAUNCH_CHECK();
+        k_partials_final<<<1, kThreads, 0, st_>>>(partA_.p, nb_, 1, scalar_.p);
+        LAUNCH_CHECK();
+        return read_scalar();
+    }
+
+    double rel_tol2() const {
+        return params.cg_rel_tol > 0.0 ? params.cg_rel_tol * params.cg_rel_tol : 0.0;
+    }
+
+    // CG iterations (src/fluid.rs:463-487) as two launches each; every
+    // launch returns at once after the device stop flag is set.
+    void enqueue_cg_loop(int max_iters, int step_index) {
+        const double rel2 = rel_tol2();
+        for (int it = 0; it < max_iters; ++it) {
+            hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+            if (timer.enabled) {
+                e0 = timer.get();
+                HIPT(hipEventRecord(e0, st_));
+            }
+            k_cg_dir<T><<<red_grid_, cell_blk_, 0, st_>>>(g_, zchunk_, it, rel2, partB_.p,
+                                                          partC_.p, nb_, cg_.p, r_.p,
+                                                          d_[(it + 1) & 1].p, d_[it & 1].p,
+                                                          partA_.p);
+            LAUNCH_CHECK();
+            if (timer.enabled) {
+                e1 = timer.get();
+                HIPT(hipEventRecord(e1, st_));
+            }
+            k_cg_update<T><<<red_grid_, cell_blk_, 0, st_>>>(g_, zchunk_, it, partA_.p, nb_,
+                                                             cg_.p, d_[it & 1].p, p_.p, r_.p,
+                                                             partB_.p);
+            LAUNCH_CHECK();
+            if (timer.enabled) {
+                e2 = timer.get();
+                HIPT(hipEventRecord(e2, st_));
+                timer.recs.push_back({0, step_index, it, e0, e1});
+                timer.recs.push_back({1, step_index, it, e1, e2});
+            }
+        }
+        k_cg_finalize<<<1, cell_blk_, 0, st_>>>(max_iters, rel2, partB_.p, partC_.p, nb_, cg_.p,
+                                                 step_index >= 0 ? iters_dev_.p + step_index
+                                                                 : iters_log_.p);
+        LAUNCH_CHECK();
+        last_kind_ = AETHER_SOLVER_CG;
+    }
+
+    void enqueue_rbgs(int sweeps) {
+        for (int it = 0; it < sweeps; ++it)
+            for (int color = 0; color < 2; ++color) {
+                k_rbgs<T><<<cell_grid_, cell_blk_, 0, st_>>>(g_, color, (T)params.sor_omega,
+                                                             b_.p, p_.p);
+                LAUNCH_CHECK();
+            }
+        last_kind_ = AETHER_SOLVER_RBGS;
+        last_sweeps_ = sweeps;
+    }
+
+    void enqueue_project(int step_index) {
+        // src/fluid.rs:496-529; the opening wall BC is applied through masked
+        // reads in the divergence, the closing one in the gradient kernel
+        if (params.solver_kind == AETHER_SOLVER_CG) {
+            k_div_cginit<T><<<red_grid_, cell_blk_, 0, st_>>>(g_, zchunk_, (T)h_, 1, cur_[0],
+                                                              cur_[1], cur_[2], p_.p, b_.p, r_.p,
+                                                              partB_.p, partC_.p, cg_.p);
+            LAUNCH_CHECK();
+            enqueue_cg_loop(params.solver_iters, step_index);
+        } else {
+            k_divergence<T><<<red_grid_, cell_blk_, 0, st_>>>(g_, zchunk_, (T)h_, (T)(1.0 / h_),
+                                                               1, cur_[0], cur_[1], cur_[2],
+                                                               b_.p, partA_.p);
+            LAUNCH_CHECK();
+            enqueue_rbgs(params.solver_iters);
+        }
+        k_gradient_bc<T><<<cell_grid_, cell_blk_, 0, st_>>>(g_, (T)(1.0 / h_), p_.p, cur_[0],
+                                                             cur_[1], cur_[2]);
+        LAUNCH_CHECK();
+    }
+
+    void project() override {
+        enqueue_project(-1);
+        sync();
+    }
+
+    void solve_cg(int iters) override {
+        if (iters < 0) raise(AETHER_EINVAL, "iterations must be >= 0");
+        k_div_cginit<T><<<red_grid_, cell_blk_, 0, st_>>>(g_, zchunk_, (T)h_, 0, cur_[0],
+                                                          cur_[1], cur_[2], p_.p, b_.p, r_.p,
+                                                          partB_.p, partC_.p, cg_.p);
+        LAUNCH_CHECK();
+        enqueue_cg_loop(iters, -1);
+        sync();
+    }
+
+    void solve_rbgs(int sweeps) override {
+        if (sweeps < 0) raise(AETHER_EINVAL, "sweeps must be >= 0");
+        enqueue_rbgs(sweeps);
+        sync();
+    }
+
+    void enqueue_sources(double t, double dt) {
+        if (params.source_kind == AETHER_SOURCE_JETS) {
+            // scalar set-up of src/fluid.rs:533-563 on the host (libm sin/cos)
+            const double TAU = 6.28318530717958647692528676655900577;
+            const double cx = 0.5 * (double)g_.nx, cz = 0.5 * (double)g_.nz;
+            const int base_j = std::min(3, g_.ny - 3);
+            const double pulse = 0.65 + 0.35 * std::sin(t * TAU * 0.55) +
+                                 0.25 * std::fmax(std::cos(t * TAU * 1.35), 0.0);
+            const double angle = t * 1.85;
+            const double orbit_r = (double)g_.nx * 0.14;
+            JetParams jp;
+            const double ca = std::cos(angle), sa = std::sin(angle);
+            const double j0[6] = {cx + orbit_r * ca, (double)base_j + 1.5, cz + orbit_r * sa,
+                                  -2.6 * sa, 7.2 * pulse, 2.6 * ca};
+            const double j1[6] = {cx - orbit_r * ca, (double)base_j + 2.0, cz - orbit_r * sa,
+                                  2.6 * sa, 6.4 * pulse, -2.6 * ca};
+            std::memcpy(jp.jet[0], j0, sizeof(j0));
+            std::memcpy(jp.jet[1], j1, sizeof(j1));
+            const double r_cells = std::fmax((double)g_.nx * 0.105, 2.2);
+            jp.r2 = r_cells * r_cells;
+            jp.pulse = pulse;
+            jp.dt = dt;
+            int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-1, -1, -1};
+            const int nn[3] = {g_.nx, g_.ny, g_.nz};
+            for (int q = 0; q < 2; ++q) {
+                int qlo[3], qhi[3];
+                bool empty = false;
+                for (int d = 0; d < 3; ++d) {
+                    const double c = jp.jet[q][d];
+                    qlo[d] = (int)std::fmax(std::floor(c - r_cells - 1.0), 1.0);
+                    qhi[d] = (int)std::fmin(std::ceil(c + r_cells + 1.0), (double)(nn[d] - 2));
+                    if (qlo[d] > qhi[d]) empty = true;
+                }
+                if (empty) continue;
+                for (int d = 0; d < 3; ++d) {
+                    lo[d] = std::min(lo[d], qlo[d]);
+                    hi[d] = std::max(hi[d], qhi[d]);
+                }
+            }
+            if (hi[0] < lo[0] || hi[1] < lo[1] || hi[2] < lo[2]) return;
+            const int ni = hi[0] - lo[0] + 1, nj = hi[1] - lo[1] + 1, nk = hi[2] - lo[2] + 1;
+            const unsigned nbk = (unsigned)(((long)ni * nj * nk + kThreads - 1) / kThreads);
+            k_jets<T><<<nbk, cell_blk_, 0, st_>>>(g_, jp, lo[0], lo[1], lo[2], ni, nj, nk,
+                                                  cur_[0], cur_[1], cur_[2], cur_[3], cur_[4]);
+            LAUNCH_CHECK();
+        } else if (params.source_kind == AETHER_SOURCE_SPHERE) {
+            const double r = params.source_radius;
+            if (!(r > 0.0)) return;
+            const int nn[3] = {g_.nx, g_.ny, g_.nz};
+            int lo[3], hi[3];
+            for (int d = 0; d < 3; ++d) {
+                const double c = params.source_center[d];
+                lo[d] = std::max(1, (int)std::floor(c - r));
+                hi[d] = std::min(nn[d] - 2, (int)std::ceil(c + r));
+                if (hi[d] < lo[d]) return;
+            }
+            const int ni = hi[0] - lo[0] + 1, nj = hi[1] - lo[1] + 1, nk = hi[2] - lo[2] + 1;
+            const unsigned nbk = (unsigned)(((long)ni * nj * nk + kThreads - 1) / kThreads);
+            k_sphere<T><<<nbk, cell_blk_, 0, st_>>>(
+                g_, params.source_center[0], params.source_center[1], params.source_center[2],
+                r * r, params.source_density, params.source_temperature, lo[0], lo[1], lo[2], ni,
+                nj, nk, cur_[3], cur_[4]);
+            LAUNCH_CHECK();
+        }
+    }
+
+    void inject_sources(double t, double dt) override {
+        enqueue_sources(t, dt);
+        sync();
+    }
+
+    void enqueue_telemetry(double t, double *dst_dev) {
+        k_telemetry_part<T><<<red_grid_, cell_blk_, 0, st_>>>(g_, zchunk_, h_, cur_[0], cur_[1],
+                                                              cur_[2], cur_[3], part_tel_.p);
+        LAUNCH_CHECK();
+        k_telemetry_final<<<1, cell_blk_, 0, st_>>>(part_tel_.p, nb_, t, dst_dev);
+        LAUNCH_CHECK();
+    }
+
+    static void unpack(const double *raw, aether_fluid_telemetry *out) {
+        out->time_sec = raw[0];
+        out->kinetic_energy = raw[1];
+        out->enstrophy = raw[2];
+        out->peak_vorticity = raw[3];
+        out->total_mass = raw[4];
+        out->center_of_mass[0] = raw[5];
+        out->center_of_mass[1] = raw[6];
+        out->center_of_mass[2] = raw[7];
+    }
+
+    void telemetry(double t, aether_fluid_telemetry *out) override {
+        enqueue_telemetry(t, tel_.p);
+        double raw[8];
+        HIPT(hipMemcpyAsync(raw, tel_.p, sizeof(raw), hipMemcpyDeviceToHost, st_));
+        sync();
+        if (out) unpack(raw, out);
+    }
+
+    // src/fluid.rs:598-611
+    void enqueue_step(double dt, double t, double *tel_dst, int step_index) {
+        enqueue_advect1(0, dt, params.dissipation_velocity);
+        enqueue_advect1(1, dt, params.dissipation_velocity);
+        enqueue_advect1(2, dt, params.dissipation_velocity);
+        enqueue_sources(t, dt);
+        enqueue_buoyancy_confinement(dt);
+        enqueue_project(step_index);
+        enqueue_advect_scalars(dt);
+        enqueue_telemetry(t, tel_dst);
+    }
+
+    void run(double dt, const double *times, int nsteps, aether_fluid_telemetry *out) override {
+        if (nsteps < 0) raise(AETHER_EINVAL, "nsteps must be >= 0");
+        if (nsteps == 0) return;
+        if (tel_ring_.n < (size_t)nsteps * 8) tel_ring_.alloc((size_t)nsteps * 8);
+        if (iters_dev_.n < (size_t)nsteps) iters_dev_.alloc((size_t)nsteps);
+        if (timer.enabled) timer.reset();
+        for (int s = 0; s < nsteps; ++s)
+            enqueue_step(dt, times[s], tel_ring_.p + (size_t)s * 8, s);
+        std::vector<double> raw((size_t)nsteps * 8);
+        HIPT(hipMemcpyAsync(raw.data(), tel_ring_.p, raw.size() * sizeof(double),
+                            hipMemcpyDeviceToHost, st_));
+        std::vector<int> iters(nsteps);
+        HIPT(hipMemcpyAsync(iters.data(), iters_dev_.p, nsteps * sizeof(int),
+                            hipMemcpyDeviceToHost, st_));
+        sync();
+        if (out)
+            for (int s = 0; s < nsteps; ++s) unpack(raw.data() + (size_t)s * 8, out + s);
+        last_run_iters_ = iters;
+        if (timer.enabled) {
+            // count only launches that did work: iterations < iters of that step
+            for (const auto &r : timer.recs) {
+                if (r.it >= iters[r.step]) continue;
+                float ms = 0;
+                HIPT(hipEventElapsedTime(&ms, r.a, r.b));
+                timer.sum_ms[r.which] += ms;
+                timer.count[r.which] += 1;
+            }
+        }
+    }
+
+    T *field_ptr(int field) {
+        switch (field) {
+        case AETHER_FIELD_U:
+        case AETHER_FIELD_V:
+        case AETHER_FIELD_W:
+        case AETHER_FIELD_DENSITY:
+        case AETHER_FIELD_TEMPERATURE:
+            return cur_[field];
+        case AETHER_FIELD_PRESSURE:
+            return p_.p;
+        case AETHER_FIELD_DIVERGENCE:
+            return b_.p;
+        default:
+            raise(AETHER_EINVAL, "unknown field id " + std::to_string(field));
+        }
+    }
+
+    void upload(int field, const double *src) override {
+        T *dst = field_ptr(field);
+        if constexpr (sizeof(T) == 8) {
+            HIPT(hipMemcpyAsync(dst, src, n_ * sizeof(double), hipMemcpyHostToDevice, st_));
+        } else {
+            if (!staging_.p) staging_.alloc(n_);
+            HIPT(hipMemcpyAsync(staging_.p, src, n_ * sizeof(double), hipMemcpyHostToDevice,
+                                st_));
+            k_convert<double, T><<<blocks_for(n_), kThreads, 0, st_>>>(staging_.p, (long)n_,
+                                                                         dst);
+            LAUNCH_CHECK();
+        }
+        sync();
+    }
+
+    void download(int field, double *dst) override {
+        T *src = field_ptr(field);
+        if constexpr (sizeof(T) == 8) {
+            HIPT(hipMemcpyAsync(dst, src, n_ * sizeof(double), hipMemcpyDeviceToHost, st_));
+        } else {
+            if (!staging_.p) staging_.alloc(n_);
+            k_convert<T, double><<<blocks_for(n_), kThreads, 0, st_>>>(src, (long)n_,
+                                                                         staging_.p);
+            LAUNCH_CHECK();
+            HIPT(hipMemcpyAsync(dst, staging_.p, n_ * sizeof(double), hipMemcpyDeviceToHost,
+                                st_));
+        }
+        sync();
+    }
+
+    void sample(int field, const double *xyz, size_t npts, double *out) override {
+        T *f = field_ptr(field);
+        if (npts == 0) return;
+        DevBuf<double> dxyz(npts * 3), dout(npts);
+        HIPT(hipMemcpyAsync(dxyz.p, xyz, npts * 3 * sizeof(double), hipMemcpyHostToDevice, st_));
+        k_sample_points<T><<<(unsigned)((npts + kThreads - 1) / kThreads), cell_blk_, 0, st_>>>(
+            g_, f, dxyz.p, (long)npts, dout.p);
+        LAUNCH_CHECK();
+        HIPT(hipMemcpyAsync(out, dout.p, npts * sizeof(double), hipMemcpyDeviceToHost, st_));
+        sync();
+    }
+
+    double max_density() override {
+        // src/fluid.rs:704-707: max(0, all density entries)
+        const unsigned nbk = std::min<unsigned>(blocks_for(n_), (unsigned)nb_);
+        k_max_part<T><<<nbk, cell_blk_, 0, st_>>>(cur_[3], (long)n_, partA_.p);
+        LAUNCH_CHECK();
+        k_partials_final<<<1, kThreads, 0, st_>>>(partA_.p, (int)nbk, 1, scalar_.p);
+        LAUNCH_CHECK();
+        return read_scalar();
+    }
+
+    void stats(aether_fluid_solve_stats *out) override {
+        std::memset(out, 0, sizeof(*out));
+        if (last_kind_ == AETHER_SOLVER_RBGS) {
+            out->iterations = last_sweeps_;
+            out->converged = 1;
+            return;
+        }
+        CgState cs;
+        HIPT(hipMemcpyAsync(&cs, cg_.p, sizeof(cs), hipMemcpyDeviceToHost, st_));
+        sync();
+        out->iterations = cs.iters;
+        out->converged = cs.converged;
+        out->b_norm2 = cs.b2;
+        out->r_norm2_initial = cs.r2_init;
+        out->r_norm2_final = cs.r2_final;
+    }
+
+    const std::vector<int> &last_run_iters() const { return last_run_iters_; }
+
+   private:
+    Dims g_;
+    double h_;
+    hipStream_t st_;
+    size_t n_;
+    dim3 cell_blk_, cell_grid_, red_grid_;
+    int zchunk_, nb_;
+    DevBuf<T> f_[5], alt_[5];
+    T *cur_[5], *nxt_[5];  // u, v, w, density, temperature (current / scratch)
+    DevBuf<T> fwd_[2], curl_, p_, b_, r_, d_[2];
+    DevBuf<double> partA_, partB_, partC_, part_tel_, scalar_, tel_, tel_ring_, staging_;
+    DevBuf<CgState> cg_;
+    DevBuf<int> iters_log_, iters_dev_;
+    int last_kind_ = AETHER_SOLVER_CG;
+    int last_sweeps_ = 0;
+    std::vector<int> last_run_iters_;
+};
+
+}  // namespace
+}  // namespace aether
+
+using namespace aether;
+
+struct aether_fluid_s {
+    int nx, ny, nz, precision;
+    double h;
+    int device;
+    hipStream_t stream = nullptr;
+    std::unique_ptr<SolverBase> impl;
+    ~aether_fluid_s() {
+        impl.reset();
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+aether_fluid_s *checked(aether_fluid_t s) {
+    if (!s) raise(AETHER_EINVAL, "null solver handle");
+    HIPT(hipSetDevice(s->device));
+    return s;
+}
+size_t cells(const aether_fluid_s *s) { return (size_t)s->nx * s->ny * s->nz; }
+
+// SplitMix64 seeding and Xoshiro256++ (src/rng.rs:5-64).
+uint64_t splitmix64(uint64_t &st) {
+    st += 0x9E3779B97F4A7C15ull;
+    uint64_t z = st;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+inline uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+uint64_t xoshiro_next(uint64_t s[4]) {
+    const uint64_t result = rotl(s[0] + s[3], 23) + s[0];
+    const uint64_t t = s[1] << 17;
+    s[2] ^= s[0];
+    s[3] ^= s[1];
+    s[1] ^= s[2];
+    s[0] ^= s[3];
+    s[2] ^= t;
+    s[3] = rotl(s[3], 45);
+    return result;
+}
+}  // namespace
+
+extern "C" {
+
+int aether_fluid_abi_version(void) { return AETHER_FLUID_ABI_VERSION; }
+
+const char *aether_fluid_last_error(void) { return g_last_error.c_str(); }
+
+int aether_fluid_device_count(int *count) {
+    return guarded([&] {
+        if (!count) raise(AETHER_EINVAL, "null count pointer");
+        int c = 0;
+        hipError_t e = hipGetDeviceCount(&c);
+        *count = e == hipSuccess ? c : 0;
+    });
+}
+
+int aether_fluid_create(int nx, int ny, int nz, double h, int precision, aether_fluid_t *out) {
+    return guarded([&] {
+        if (!out) raise(AETHER_EINVAL, "null output handle pointer");
+        *out = nullptr;
+        if (nx < 6 || ny < 6 || nz < 6)
+            raise(AETHER_EINVAL, "grid dimensions must be >= 6 (src/fluid.rs:54)");
+        if ((double)nx * ny * nz >= 2147483647.0)
+            raise(AETHER_EINVAL, "grid has more than 2^31-1 cells");
+        if (precision != AETHER_FP32 && precision != AETHER_FP64)
+            raise(AETHER_EINVAL, "precision must be AETHER_FP32 or AETHER_FP64");
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+            raise(AETHER_ENODEV, "no HIP device available");
+        auto s = std::make_unique<aether_fluid_s>();
+        s->nx = nx;
+        s->ny = ny;
+        s->nz = nz;
+        s->precision = precision;
+        // src/fluid.rs:56
+        s->h = h > 0.0 ? h : 1.0 / (double)std::max(nx, std::max(ny, nz));
+        HIPT(hipGetDevice(&s->device));
+        HIPT(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+        if (precision == AETHER_FP64)
+            s->impl = std::make_unique<Solver<double>>(nx, ny, nz, s->h, s->stream);
+        else
+            s->impl = std::make_unique<Solver<float>>(nx, ny, nz, s->h, s->stream);
+        *out = s.release();
+    });
+}
+
+int aether_fluid_destroy(aether_fluid_t s) {
+    return guarded([&] {
+        if (!s) return;
+        (void)hipSetDevice(s->device);
+        delete s;
+    });
+}
+
+int aether_fluid_dims(aether_fluid_t s, int *nx, int *ny, int *nz, double *h, int *precision) {
+    return guarded([&] {
+        if (!s) raise(AETHER_EINVAL, "null solver handle");
+        if (nx) *nx = s->nx;
+        if (ny) *ny = s->ny;
+        if (nz) *nz = s->nz;
+        if (h) *h = s->h;
+        if (precision) *precision = s->precision;
+    });
+}
+
+int aether_fluid_default_params(aether_fluid_params *out) {
+    return guarded([&] {
+        if (!out) raise(AETHER_EINVAL, "null params pointer");
+        *out = default_params();
+    });
+}
+
+int aether_fluid_set_params(aether_fluid_t s, const aether_fluid_params *p) {
+    return guarded([&] {
+        checked(s);
+        if (!p) raise(AETHER_EINVAL, "null params pointer");
+        check_params(*p);
+        s->impl->params = *p;
+    });
+}
+
+int aether_fluid_get_params(aether_fluid_t s, aether_fluid_params *p) {
+    return guarded([&] {
+        checked(s);
+        if (!p) raise(AETHER_EINVAL, "null params pointer");
+        *p = s->impl->params;
+    });
+}
+
+int aether_fluid_step(aether_fluid_t s, double dt, double time_sec, aether_fluid_telemetry *out) {
+    return guarded([&] {
+        checked(s);
+        aether_fluid_telemetry tmp;
+        s->impl->run(dt, &time_sec, 1, &tmp);
+        if (out) *out = tmp;
+    });
+}
+
+int aether_fluid_run(aether_fluid_t s, double dt, const double *times, int nsteps,
+                     aether_fluid_telemetry *out) {
+    return guarded([&] {
+        checked(s);
+        if (nsteps > 0 && !times) raise(AETHER_EINVAL, "null times array");
+        s->impl->run(dt, times, nsteps, out);
+    });
+}
+
+int aether_fluid_advect_maccormack_in_place(aether_fluid_t s, int field_kind, double dt, double dissipation) {
+    return guarded([&] { checked(s)->impl->advect(field_kind, dt, dissipation); });
+}
+
+int aether_fluid_enforce_wall_velocity_bc(aether_fluid_t s) {
+    return guarded([&] { checked(s)->impl->enforce_bc(); });
+}
+
+int aether_fluid_apply_buoyancy_and_vorticity_confinement(aether_fluid_t s, double dt) {
+    return guarded([&] { checked(s)->impl->buoyancy_confinement(dt); });
+}
+
+int aether_fluid_compute_divergence_norm(aether_fluid_t s, double *rms) {
+    return guarded([&] {
+        checked(s);
+        if (!rms) raise(AETHER_EINVAL, "null output pointer");
+        *rms = s->impl->divergence_norm();
+    });
+}
+
+int aether_fluid_divergence_max(aether_fluid_t s, double *out) {
+    return guarded([&] {
+        checked(s);
+        if (!out) raise(AETHER_EINVAL, "null output pointer");
+        *out = s->impl->divergence_max();
+    });
+}
+
+int aether_fluid_project(aether_fluid_t s) {
+    return guarded([&] { checked(s)->impl->project(); });
+}
+
+int aether_fluid_solve_poisson_conjugate_gradient(aether_fluid_t s, int max_iters) {
+    return guarded([&] { checked(s)->impl->solve_cg(max_iters); });
+}
+
+int aether_fluid_solve_poisson_gauss_seidel(aether_fluid_t s, int sweeps) {
+    return guarded([&] { checked(s)->impl->solve_rbgs(sweeps); });
+}
+
+int aether_fluid_inject_sources(aether_fluid_t s, double time_sec, double dt) {
+    return guarded([&] { checked(s)->impl->inject_sources(time_sec, dt); });
+}
+
+int aether_fluid_compute_telemetry(aether_fluid_t s, double time_sec, aether_fluid_telemetry *out) {
+    return guarded([&] {
+        checked(s);
+        if (!out) raise(AETHER_EINVAL, "null output pointer");
+        s->impl->telemetry(time_sec, out);
+    });
+}
+
+int aether_fluid_sample_trilinear(aether_fluid_t s, int field, const double *xyz, size_t npts,
+                                  double *out) {
+    return guarded([&] {
+        checked(s);
+        if (npts && (!xyz || !out)) raise(AETHER_EINVAL, "null point or output array");
+        s->impl->sample(field, xyz, npts, out);
+    });
+}
+
+int aether_fluid_last_solve_stats(aether_fluid_t s, aether_fluid_solve_stats *out) {
+    return guarded([&] {
+        checked(s);
+        if (!out) raise(AETHER_EINVAL, "null output pointer");
+        s->impl->stats(out);
+    });
+}
+
+int aether_fluid_upload(aether_fluid_t s, int field, const double *src, size_t n) {
+    return guarded([&] {
+        checked(s);
+        if (n != cells(s))
+            raise(AETHER_EINVAL, "upload size " + std::to_string(n) + " != nx*ny*nz = " +
+                                     std::to_string(cells(s)));
+        if (!src) raise(AETHER_EINVAL, "null source array");
+        s->impl->upload(field, src);
+    });
+}
+
+int aether_fluid_download(aether_fluid_t s, int field, double *dst, size_t n) {
+    return guarded([&] {
+        checked(s);
+        if (n != cells(s))
+            raise(AETHER_EINVAL, "download size " + std::to_string(n) + " != nx*ny*nz = " +
+                                     std::to_string(cells(s)));
+        if (!dst) raise(AETHER_EINVAL, "null destination array");
+        s->impl->download(field, dst);
+    });
+}
+
+int aether_fluid_max_density(aether_fluid_t s, double *out) {
+    return guarded([&] {
+        checked(s);
+        if (!out) raise(AETHER_EINVAL, "null output pointer");
+        *out = s->impl->max_density();
+    });
+}
+
+int aether_rng_seed(uint64_t seed, uint64_t state[4]) {
+    return guarded([&] {
+        if (!state) raise(AETHER_EINVAL, "null state");
+        uint64_t sm = seed ^ 0xA5A55A5AF0F00F0Full;  // src/rng.rs:22
+        for (int a = 0; a < 4; ++a) state[a] = splitmix64(sm);
+        if (!state[0] && !state[1] && !state[2] && !state[3]) state[0] = 1;
+    });
+}
+
+int aether_rng_fill_range(uint64_t state[4], double lo, double hi, double *out, size_t n) {
+    return guarded([&] {
+        if (!state || (n && !out)) raise(AETHER_EINVAL, "null state or output");
+        for (size_t a = 0; a < n; ++a) {
+            // src/rng.rs:74-77, 86-88
+            const double f = (double)(xoshiro_next(state) >> 11) * (1.0 / 9007199254740992.0);
+            out[a] = lo + (hi - lo) * f;
+        }
+    });
+}
+
+int aether_fluid_set_kernel_timing(aether_fluid_t s, int enable) {
+    return guarded([&] {
+        checked(s);
+        s->impl->timer.enabled = enable != 0;
+        s->impl->timer.reset();
+    });
+}
+
+int aether_fluid_kernel_timing(aether_fluid_t s, int which, double *avg_ms, long *launches) {
+    return guarded([&] {
+        checked(s);
+        if (which < 0 || which > 1) raise(AETHER_EINVAL, "which must be 0 (cg_dir) or 1 (cg_update)");
+        const auto &t = s->impl->timer;
+        if (avg_ms) *avg_ms = t.count[which] ? t.sum_ms[which] / t.count[which] : 0.0;
+        if (launches) *launches = t.count[which];
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,962 @@
+// fluid_kernels.hpp -- CDNA4 (gfx950) kernels of the smoke step.
+//
+// Every kernel is a template on the storage type T (float or double).  The
+// double instantiation evaluates each expression in the order of the
+// reference (src/fluid.rs), and the library is compiled with
+// -ffp-contract=off, so that FP64 results agree with the reference to the
+// last bit wherever no reduction is involved.  The float instantiation keeps
+// the same algorithm but uses forms that lose less precision in FP32: the
+// back-trace is carried as (cell index, FP32 offset) instead of an absolute
+// coordinate, and the Laplacian is summed as differences (x_c - x_nb).
+// Reductions (CG dot products, telemetry) always accumulate in FP64, in a
+// fixed tree order, so every result is run-to-run deterministic.
+//
+// Layout: one array of nx*ny*nz values per field, idx = (k*ny + j)*nx + i
+// (src/fluid.rs:88-90), x fastest.  u/v/w hold the reference's dual-use
+// velocity: cell-centred for advection/curl, faces at i-1/2 for the MAC
+// divergence and gradient.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace aether {
+
+constexpr int kBX = 64;  // one wavefront spans 64 consecutive x cells
+constexpr int kBY = 4;   // four wavefronts per workgroup
+constexpr int kThreads = kBX * kBY;
+
+struct Dims {
+    int nx, ny, nz;
+    int sxy;  // nx*ny
+};
+
+// Device state of one pressure solve (written only by workgroup 0 of a
+// kernel, read by later kernels on the same stream).
+struct CgState {
+    double delta[2];  // r.r of the current iteration, double-buffered by iteration parity
+    double b2;        // ||b||^2
+    double r2_init;   // ||b - A p0||^2
+    double r2_final;  // recursive residual at exit
+    int done;
+    int iters;
+    int converged;
+    int pad;
+};
+
+__device__ __forceinline__ int tid_lin() { return threadIdx.y * kBX + threadIdx.x; }
+
+__device__ __forceinline__ double wave_sum(double v) {
+    // xor butterfly: every lane ends with the bit-identical total
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// Sum NV values over the workgroup; every thread receives the totals.
+// lds must hold 4*NV doubles.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double *lds) {
+    const int t = tid_lin(), lane = t & 63, wv = t >> 6;
+#pragma unroll
+    for (int a = 0; a < NV; ++a) v[a] = wave_sum(v[a]);
+    if (lane == 0) {
+#pragma unroll
+        for (int a = 0; a < NV; ++a) lds[a * 4 + wv] = v[a];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < NV; ++a)
+        v[a] = ((lds[a * 4 + 0] + lds[a * 4 + 1]) + lds[a * 4 + 2]) + lds[a * 4 + 3];
+    __syncthreads();
+}
+
+__device__ __forceinline__ double block_max(double v, double *lds) {
+    const int t = tid_lin(), lane = t & 63, wv = t >> 6;
+    v = wave_max(v);
+    if (lane == 0) lds[wv] = v;
+    __syncthreads();
+    v = fmax(fmax(lds[0], lds[1]), fmax(lds[2], lds[3]));
+    __syncthreads();
+    return v;
+}
+
+// Reads a device flag once per workgroup so that every wave takes the same
+// branch even if another workgroup updates the flag meanwhile.
+__device__ __forceinline__ bool block_flag(const int *flag, int *lds_i) {
+    if (tid_lin() == 0) lds_i[0] = __atomic_load_n(flag, __ATOMIC_RELAXED);
+    __syncthreads();
+    const bool f = lds_i[0] != 0;
+    return f;
+}
+
+// Sum of nb per-workgroup partials, in the same order in every workgroup.
+__device__ __forceinline__ double sum_partials(const double *part, int nb, double *lds) {
+    double acc[1] = {0.0};
+    for (int a = tid_lin(); a < nb; a += kThreads) acc[0] += part[a];
+    block_sum<1>(acc, lds);
+    return acc[0];
+}
+
+__device__ __forceinline__ bool is_interior(const Dims &g, int i, int j, int k) {
+    return i >= 1 && i <= g.nx - 2 && j >= 1 && j <= g.ny - 2 && k >= 1 && k <= g.nz - 2;
+}
+
+// Wall masks of src/fluid.rs:241-249.
+__device__ __forceinline__ bool bc_u(const Dims &g, int i, int j, int k) {
+    return i <= 1 || i + 1 >= g.nx || j == 0 || j + 1 >= g.ny || k == 0 || k + 1 >= g.nz;
+}
+__device__ __forceinline__ bool bc_v(const Dims &g, int i, int j, int k) {
+    return j <= 1 || j + 1 >= g.ny || i == 0 || i + 1 >= g.nx || k == 0 || k + 1 >= g.nz;
+}
+__device__ __forceinline__ bool bc_w(const Dims &g, int i, int j, int k) {
+    return k <= 1 || k + 1 >= g.nz || i == 0 || i + 1 >= g.nx || j == 0 || j + 1 >= g.ny;
+}
+
+// ---------------------------------------------------------------------------
+// Back-trace location along one axis: position x = i - sv (sv = signed
+// dt/h * velocity), clamped to [0.5, n - 1.500001] (src/fluid.rs:95-108),
+// plus the near-wall test of src/fluid.rs:196-201.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void axis_loc(int i, double sv, int n, int &i0, double &t, bool &nw) {
+    double x = (double)i - sv;
+    nw = x < 1.5 || x > (double)n - 2.5;
+    double xc = x;
+    if (xc < 0.5) xc = 0.5;
+    if (xc > (double)n - 1.500001) xc = (double)n - 1.500001;
+    double f = floor(xc);
+    i0 = (int)f;
+    t = xc - f;
+}
+
+__device__ __forceinline__ void axis_loc(int i, float sv, int n, int &i0, float &t, bool &nw) {
+    // x = i + o with o = -sv kept separately: comparisons against the clamp
+    // and near-wall bounds are exact rewrites of the reference's tests.
+    const float o = -sv;
+    const float fi = (float)i;
+    nw = (o < 1.5f - fi) || (o > ((float)n - 2.5f) - fi);
+    if (o < 0.5f - fi) {
+        i0 = 0;
+        t = 0.5f;
+    } else if (o > (float)(n - 1 - i) - 0.500001f) {
+        i0 = n - 2;
+        t = 0.499999f;
+    } else {
+        const float f = floorf(o);
+        i0 = i + (int)f;
+        t = o - f;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ T lerp3(T c000, T c100, T c010, T c110, T c001, T c101, T c011,
+                                   T c111, T tx, T ty, T tz) {
+    // src/fluid.rs:119-127, same association
+    const T one = T(1);
+    T c00 = c000 * (one - tx) + c100 * tx;
+    T c10 = c010 * (one - tx) + c110 * tx;
+    T c01 = c001 * (one - tx) + c101 * tx;
+    T c11 = c011 * (one - tx) + c111 * tx;
+    T c0 = c00 * (one - ty) + c10 * ty;
+    T c1 = c01 * (one - ty) + c11 * ty;
+    return c0 * (one - tz) + c1 * tz;
+}
+
+template <typename T>
+__device__ __forceinline__ T gather_lerp(const Dims &g, const T *__restrict__ f, int i0, int j0,
+                                         int k0, T tx, T ty, T tz) {
+    const int b = (k0 * g.ny + j0) * g.nx + i0;
+    return lerp3<T>(f[b], f[b + 1], f[b + g.nx], f[b + g.nx + 1], f[b + g.sxy],
+                    f[b + g.sxy + 1], f[b + g.sxy + g.nx], f[b + g.sxy + g.nx + 1], tx, ty, tz);
+}
+
+template <typename T>
+__device__ __forceinline__ void gather_minmax(const Dims &g, const T *__restrict__ f, int i0,
+                                              int j0, int k0, T &lo, T &hi) {
+    const int b = (k0 * g.ny + j0) * g.nx + i0;
+    T c[8] = {f[b], f[b + 1], f[b + g.nx], f[b + g.nx + 1], f[b + g.sxy], f[b + g.sxy + 1],
+              f[b + g.sxy + g.nx], f[b + g.sxy + g.nx + 1]};
+    lo = c[0];
+    hi = c[0];
+#pragma unroll
+    for (int a = 1; a < 8; ++a) {
+        lo = fmin(lo, c[a]);
+        hi = fmax(hi, c[a]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Advection (src/fluid.rs:153-234)
+// ---------------------------------------------------------------------------
+
+// Forward semi-Lagrangian pass of NF fields sharing one velocity:
+// fwd = SL(src, +dt) on interior cells, 0 on the boundary layer.
+template <typename T, int NF>
+__global__ __launch_bounds__(kThreads) void k_sl_forward(Dims g, T s, const T *__restrict__ u,
+                                                         const T *__restrict__ v,
+                                                         const T *__restrict__ w,
+                                                         const T *__restrict__ src0,
+                                                         const T *__restrict__ src1,
+                                                         T *__restrict__ fwd0,
+                                                         T *__restrict__ fwd1) {
+    const int i = blockIdx.x * kBX + threadIdx.x;
+    const int j = blockIdx.y * kBY + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= g.nx || j >= g.ny) return;
+    const int id = (k * g.ny + j) * g.nx + i;
+    if (!is_interior(g, i, j, k)) {
+        fwd0[id] = T(0);
+        if (NF == 2) fwd1[id] = T(0);
+        return;
+    }
+    int i0, j0, k0;
+    T tx, ty, tz;
+    bool nw;
+    axis_loc(i, s * u[id], g.nx, i0, tx, nw);
+    axis_loc(j, s * v[id], g.ny, j0, ty, nw);
+    axis_loc(k, s * w[id], g.nz, k0, tz, nw);
+    fwd0[id] = gather_lerp<T>(g, src0, i0, j0, k0, tx, ty, tz);
+    if (NF == 2) fwd1[id] = gather_lerp<T>(g, src1, i0, j0, k0, tx, ty, tz);
+}
+
+// MacCormack correction with the 8-cell extrema clamp (src/fluid.rs:183-228):
+// out = clamp(fwd + (src - bwd)/2, min8, max8) * diss away from walls,
+// fwd * diss near walls, bwd = SL(fwd, -dt) evaluated at this cell only.
+template <typename T>
+__device__ __forceinline__ T mac_correct_one(const Dims &g, const T *__restrict__ src,
+                                             const T *__restrict__ fwd, int id, bool near_wall,
+                                             int i0, int j0, int k0, int fi0, int fj0, int fk0,
+                                             T ftx, T fty, T ftz, T diss, bool clamp0) {
+    T val;
+    const T f = fwd[id];
+    if (near_wall) {
+        val = f;
+    } else {
+        const T bwd = gather_lerp<T>(g, fwd, fi0, fj0, fk0, ftx, fty, ftz);
+        const T mc = f + T(0.5) * (src[id] - bwd);
+        T lo, hi;
+        gather_minmax<T>(g, src, i0, j0, k0, lo, hi);
+        val = mc;
+        if (val < lo) val = lo;
+        if (val > hi) val = hi;
+    }
+    T o = val * diss;
+    if (clamp0) o = fmax(o, T(0));
+    return o;
+}
+
+template <typename T, int NF>
+__global__ __launch_bounds__(kThreads) void k_mac_correct(
+    Dims g, T s, const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ w,
+    const T *__restrict__ src0, const T *__restrict__ fwd0, T *__restrict__ out0, T diss0,
+    const T *__restrict__ src1, const T *__restrict__ fwd1, T *__restrict__ out1, T diss1,
+    int clamp0) {
+    const int i = blockIdx.x * kBX + threadIdx.x;
+    const int j = blockIdx.y * kBY + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= g.nx || j >= g.ny) return;
+    const int id = (k * g.ny + j) * g.nx + i;
+    if (!is_interior(g, i, j, k)) {
+        out0[id] = T(0);
+        if (NF == 2) out1[id] = T(0);
+        return;
+    }
+    const T su = s * u[id], sv = s * v[id], sw = s * w[id];
+    int i0, j0, k0, fi0, fj0, fk0;
+    T tx, ty, tz, ftx, fty, ftz;
+    bool nwx, nwy, nwz, dummy;
+    axis_loc(i, su, g.nx, i0, tx, nwx);
+    axis_loc(j, sv, g.ny, j0, ty, nwy);
+    axis_loc(k, sw, g.nz, k0, tz, nwz);
+    const bool near_wall = nwx || nwy || nwz;
+    // forward trace for bwd = SL(fwd, -dt): position i + s*u
+    axis_loc(i, -su, g.nx, fi0, ftx, dummy);
+    axis_loc(j, -sv, g.ny, fj0, fty, dummy);
+    axis_loc(k, -sw, g.nz, fk0, ftz, dummy);
+    out0[id] = mac_correct_one<T>(g, src0, fwd0, id, near_wall, i0, j0, k0, fi0, fj0, fk0, ftx,
+                                  fty, ftz, diss0, clamp0 != 0);
+    if (NF == 2)
+        out1[id] = mac_correct_one<T>(g, src1, fwd1, id, near_wall, i0, j0, k0, fi0, fj0, fk0,
+                                      ftx, fty, ftz, diss1, clamp0 != 0);
+}
+
+// ---------------------------------------------------------------------------
+// Walls, vorticity confinement, buoyancy (src/fluid.rs:236-330)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_enforce_bc(Dims g, T *__restrict__ u,
+                                                         T *__restrict__ v, T *__restrict__ w) {
+    const int i = blockIdx.x * kBX + threadIdx.x;
+    const int j = blockIdx.y * kBY + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= g.nx || j >= g.ny) return;
+    const int id = (k * g.ny + j) * g.nx + i;
+    if (bc_u(g, i, j, k)) u[id] = T(0);
+    if (bc_v(g, i, j, k)) v[id] = T(0);
+    if (bc_w(g, i, j, k)) w[id] = T(0);
+}
+
+template <typename T>
+__device__ __forceinline__ void curl_at(const Dims &g, const T *__restrict__ u,
+                                        const T *__restrict__ v, const T *__restrict__ w, int id,
+                                        T inv_2h, T &ox, T &oy, T &oz) {
+    // src/fluid.rs:265-274
+    const T dw_dy = (w[id + g.nx] - w[id - g.nx]) * inv_2h;
+    const T dv_dz = (v[id + g.sxy] - v[id - g.sxy]) * inv_2h;
+    const T du_dz = (u[id + g.sxy] - u[id - g.sxy]) * inv_2h;
+    const T dw_dx = (w[id + 1] - w[id - 1]) * inv_2h;
+    const T dv_dx = (v[id + 1] - v[id - 1]) * inv_2h;
+    const T du_dy = (u[id + g.nx] - u[id - g.nx]) * inv_2h;
+    ox = dw_dy - dv_dz;
+    oy = du_dz - dw_dx;
+    oz = dv_dx - du_dy;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_curl_mag(Dims g, T inv_2h, const T *__restrict__ u,
+                                                       const T *__restrict__ v,
+                                                       const T *__restrict__ w,
+                                                       T *__restrict__ cmag) {
+    const int i = blockIdx.x * kBX + threadIdx.x;
+    const int j = blockIdx.y * kBY + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= g.nx || j >= g.ny) return;
+    const int id = (k * g.ny + j) * g.nx + i;
+    if (!is_interior(g, i, j, k)) {
+        cmag[id] = T(0);
+        return;
+    }
+    T ox, oy, oz;
+    curl_at<T>(g, u, v, w, id, inv_2h, ox, oy, oz);
+    cmag[id] = sqrt(ox * ox + oy * oy + oz * oz);
+}
+
+// Confinement force eps*h*(N x w) on cells [2, n-3], then Boussinesq lift on
+// interior cells, then the wall masks; writes new velocity arrays.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_confine_buoy_bc(
+    Dims g, T dt, T inv_2h, T eps_h, T beta, T alpha, const T *__restrict__ u,
+    const T *__restrict__ v, const T *__restrict__ w, const T *__restrict__ cmag,
+    const T *__restrict__ rho, const T *__restrict__ temp, T *__restrict__ uo,
+    T *__restrict__ vo, T *__restrict__ wo) {
+    const int i = blockIdx.x * kBX + threadIdx.x;
+    const int j = blockIdx.y * kBY + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= g.nx || j >= g.ny) return;
+    const int id = (k * g.ny + j) * g.nx + i;
+    T uu = u[id], vv = v[id], ww = w[id];
+    if (i >= 2 && i <= g.nx - 3 && j >= 2 && j <= g.ny - 3 && k >= 2 && k <= g.nz - 3) {
+        const T gx = (cmag[id + 1] - cmag[id - 1]) * inv_2h;
+        const T gy = (cmag[id + g.nx] - cmag[id - g.nx]) * inv_2h;
+        const T gz = (cmag[id + g.sxy] - cmag[id - g.sxy]) * inv_2h;
+        const T glen = sqrt(gx * gx + gy * gy + gz * gz) + T(1e-10);
+        const T nxv = gx / glen, nyv = gy / glen, nzv = gz / glen;
+        T ox, oy, oz;
+        curl_at<T>(g, u, v, w, id, inv_2h, ox, oy, oz);
+        const T fx = eps_h * (nyv * oz - nzv * oy);
+        const T fy = eps_h * (nzv * ox - nxv * oz);
+        const T fz = eps_h * (nxv * oy - nyv * ox);
+        uu += dt * fx;
+        vv += dt * fy;
+        ww += dt * fz;
+    }
+    if (is_interior(g, i, j, k)) {
+        const T lift = beta * temp[id] - alpha * rho[id];
+        vv += dt * lift;
+    }
+    if (bc_u(g, i, j, k)) uu = T(0);
+    if (bc_v(g, i, j, k)) vv = T(0);
+    if (bc_w(g, i, j, k)) ww = T(0);
+    uo[id] = uu;
+    vo[id] = vv;
+    wo[id] = ww;
+}
+
+// ---------------------------------------------------------------------------
+// Sources (src/fluid.rs:532-595 and the sphere extension).  Evaluated in
+// FP64 for both storage types.
+// ---------------------------------------------------------------------------
+struct JetParams {
+    double jet[2][6];  // x, y, z, vx, vy, vz per jet
+    double pulse, r2, dt;
+};
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_jets(Dims g, JetParams jp, int i_lo, int j_lo,
+                                                   int k_lo, int ni, int nj, int nk,
+                                                   T *__restrict__ u, T *__restrict__ v,
+                                                   T *__restrict__ w, T *__restrict__ rho,
+                                                   T *__restrict__ temp) {
+    const int a = blockIdx.x * kThreads + tid_lin();
+    if (a >= ni * nj * nk) return;
+    const int i = i_lo + a % ni, j = j_lo + (a / ni) % nj, k = k_lo + a / (ni * nj);
+    const int id = (k * g.ny + j) * g.nx + i;
+    double dd = (double)rho[id], tt = (double)temp[id], uu = (double)u[id], vv = (double)v[id],
+           ww = (double)w[id];
+    bool hit = false;
+    for (int q = 0; q < 2; ++q) {  // jet order of src/fluid.rs:565
+        const double dx = (double)i - jp.jet[q][0];
+        const double dy = (double)j - jp.jet[q][1];
+        const double dz = (double)k - jp.jet[q][2];
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        if (d2 < jp.r2) {
+            hit = true;
+            const double wf = exp(-2.2 * d2 / jp.r2);
+            dd = fmin(dd + jp.dt * 9.5 * jp.pulse * wf, 3.8);
+            tt = fmin(tt + jp.dt * 8.2 * jp.pulse * wf, 4.5);
+            uu += jp.dt * jp.jet[q][3] * wf * 4.2;
+            vv += jp.dt * jp.jet[q][4] * wf * 4.2;
+            ww += jp.dt * jp.jet[q][5] * wf * 4.2;
+        }
+    }
+    if (hit) {
+        rho[id] = (T)dd;
+        temp[id] = (T)tt;
+        u[id] = (T)uu;
+        v[id] = (T)vv;
+        w[id] = (T)ww;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_sphere(Dims g, double cx, double cy, double cz,
+                                                     double r2, double rho_val, double t_val,
+                                                     int i_lo, int j_lo, int k_lo, int ni, int nj,
+                                                     int nk, T *__restrict__ rho,
+                                                     T *__restrict__ temp) {
+    const int a = blockIdx.x * kThreads + tid_lin();
+    if (a >= ni * nj * nk) return;
+    const int i = i_lo + a % ni, j = j_lo + (a / ni) % nj, k = k_lo + a / (ni * nj);
+    const double dx = (double)i - cx, dy = (double)j - cy, dz = (double)k - cz;
+    if (dx * dx + dy * dy + dz * dz < r2) {
+        const int id = (k * g.ny + j) * g.nx + i;
+        rho[id] = (T)rho_val;
+        temp[id] = (T)t_val;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Projection (src/fluid.rs:333-529)
+// ---------------------------------------------------------------------------
+
+// Velocity read through the wall mask: the value enforce_wall_velocity_bc
+// would leave (src/fluid.rs:497 precedes the divergence).
+template <typename T>
+__device__ __forceinline__ void div_terms(const Dims &g, const T *__restrict__ u,
+                                          const T *__restrict__ v, const T *__restrict__ w,
+                                          int i, int j, int k, int id, bool masked, T &du, T &dv,
+                                          T &dw) {
+    T u1 = u[id + 1], u0 = u[id], v1 = v[id + g.nx], v0 = v[id], w1 = w[id + g.sxy], w0 = w[id];
+    if (masked) {
+        if (bc_u(g, i + 1, j, k)) u1 = T(0);
+        if (bc_u(g, i, j, k)) u0 = T(0);
+        if (bc_v(g, i, j + 1, k)) v1 = T(0);
+        if (bc_v(g, i, j, k)) v0 = T(0);
+        if (bc_w(g, i, j, k + 1)) w1 = T(0);
+        if (bc_w(g, i, j, k)) w0 = T(0);
+    }
+    du = u1 - u0;
+    dv = v1 - v0;
+    dw = w1 - w0;
+}
+
+// A x at an interior cell (src/fluid.rs:356-394): pure-Neumann 7-point
+// operator, a neighbour counts only if it is interior.  X(idx) supplies x.
+template <typename T, typename X>
+__device__ __forceinline__ T lap_at(const Dims &g, int i, int j, int k, int id, X x) {
+    if constexpr (sizeof(T) == 8) {
+        T diag = T(0), nb = T(0);
+        if (i > 1) { diag += T(1); nb += x(id - 1); }
+        if (i + 2 < g.nx) { diag += T(1); nb += x(id + 1); }
+        if (j > 1) { diag += T(1); nb += x(id - g.nx); }
+        if (j + 2 < g.ny) { diag += T(1); nb += x(id + g.nx); }
+        if (k > 1) { diag += T(1); nb += x(id - g.sxy); }
+        if (k + 2 < g.nz) { diag += T(1); nb += x(id + g.sxy); }
+        return diag * x(id) - nb;
+    } else {
+        const T xc = x(id);
+        T acc = T(0);
+        if (i > 1) acc += xc - x(id - 1);
+        if (i + 2 < g.nx) acc += xc - x(id + 1);
+        if (j > 1) acc += xc - x(id - g.nx);
+        if (j + 2 < g.ny) acc += xc - x(id + g.nx);
+        if (k > 1) acc += xc - x(id - g.sxy);
+        if (k + 2 < g.nz) acc += xc - x(id + g.sxy);
+        return acc;
+    }
+}
+
+// b = -h*div(u) (src/fluid.rs:346) and, for the norm, sum of div^2
+// (src/fluid.rs:344-347); one partial per workgroup.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_divergence(Dims g, int zchunk, T h, T inv_h,
+                                                         int masked, const T *__restrict__ u,
+                                                         const T *__restrict__ v,
+                                                         const T *__restrict__ w,
+                                                         T *__restrict__ b,
+                                                         double *__restrict__ part) {
+    __shared__ double lds[8];
+    const int i = blockIdx.x * kBX + threadIdx.x;
+    const int j = blockIdx.y * kBY + threadIdx.y;
+    const int kb = blockIdx.z * zchunk;
+    const int ke = min(kb + zchunk, g.nz);
+    double acc[1] = {0.0};
+    if (i < g.nx && j < g.ny) {
+        for (int k = kb; k < ke; ++k) {
+            const int id = (k * g.ny + j) * g.nx + i;
+            T bv = T(0);
+            if (is_interior(g, i, j, k)) {
+                T du, dv, dw;
+                div_terms<T>(g, u, v, w, i, j, k, id, masked != 0, du, dv, dw);
+                const T dsum = du + dv + dw;
+                const double dval = (double)(dsum * inv_h);
+                bv = -h * dsum;
+                acc[0] += dval * dval;
+            }
+            b[id] = bv;
+        }
+    }
+    block_sum<1>(acc, lds);
+    if (tid_lin() == 0)
+        part[(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = acc[0];
+}
+
+// max |div u| over interior cells, one partial per workgroup.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_div_max(Dims g, int zchunk, T inv_h,
+                                                      const T *__restrict__ u,
+                                                      const T *__restrict__ v,
+                                                      const T *__restrict__ w,
+                                                      double *__restrict__ part) {
+    __shared__ double lds[8];
+    const int i = blockIdx.x * kBX + threadIdx.x;
+    const int j = blockIdx.y * kBY + threadIdx.y;
+    const int kb = blockIdx.z * zchunk;
+    const int ke = min(kb + zchunk, g.nz);
+    double m = 0.0;
+    if (i < g.nx && j < g.ny) {
+        for (int k = kb; k < ke; ++k) {
+            if (!is_interior(g, i, j, k)) continue;
+            const int id = (k * g.ny + j) * g.nx + i;
+            T du, dv, dw;
+            div_terms<T>(g, u, v, w, i, j, k, id, false, du, dv, dw);
+            m = fmax(m, fabs((double)((du + dv + dw) * inv_h)));
+        }
+    }
+    m = block_max(m, lds);
+    if (tid_lin() == 0)
+        part[(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = m;
+}
+
+// Divergence + CG initialisation (src/fluid.rs:497-498, 450-460):
+// b = -h div(masked u), r = b - A p; partials of r.r and b.b.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_div_cginit(Dims g, int zchunk, T h, int with_div,
+                                                         const T *__restrict__ u,
+                                                         const T *__restrict__ v,
+                                                         const T *__restrict__ w,
+                                                         const T *__restrict__ p,
+                                                         T *__restrict__ b, T *__restrict__ r,
+                                                         double *__restrict__ part_rr,
+                                                         double *__restrict__ part_bb,
+                                                         CgState *st) {
+    __shared__ double lds[16];
+    const int i = blockIdx.x * kBX + threadIdx.x;
+    const int j = blockIdx.y * kBY + threadIdx.y;
+    const int kb = blockIdx.z * zchunk;
+    const int ke = min(kb + zchunk, g.nz);
+    double acc[2] = {0.0, 0.0};
+    if (i < g.nx && j < g.ny) {
+        for (int k = kb; k < ke; ++k) {
+            const int id = (k * g.ny + j) * g.nx + i;
+            T bv = T(0), rv = T(0);
+            if (is_interior(g, i, j, k)) {
+                if (with_div) {
+                    T du, dv, dw;
+                    div_terms<T>(g, u, v, w, i, j, k, id, true, du, dv, dw);
+                    bv = -h * (du + dv + dw);
+                } else {
+                    bv = b[id];
+                }
+                const T q = lap_at<T>(g, i, j, k, id, [&](int a) { return p[a]; });
+                rv = bv - q;
+            }
+            if (with_div) b[id] = bv;
+            r[id] = rv;
+            acc[0] += (double)rv * (double)rv;
+            acc[1] += (double)bv * (double)bv;
+        }
+    }
+    block_sum<2>(acc, lds);
+    const int bid = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+    if (tid_lin() == 0) {
+        part_rr[bid] = acc[0];
+        part_bb[bid] = acc[1];
+        if (bid == 0) {
+            st->done = 0;
+            st->iters = 0;
+            st->converged = 0;
+        }
+    }
+}
+
+// CG kernel 1 of iteration `it` (src/fluid.rs:463-469, 482-486):
+// finish r.r of the previous iteration, test the stopping rules, form
+// d = r + beta d_old (d = r at it = 0), q = A d on the fly, and the partial
+// d.q.  d is double-buffered so that neighbours can recompute it.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_cg_dir(Dims g, int zchunk, int it, double rel_tol2,
+                                                     const double *__restrict__ part_rr,
+                                                     const double *__restrict__ part_bb, int nb,
+                                                     CgState *st, const T *__restrict__ r,
+                                                     const T *__restrict__ dold,
+                                                     T *__restrict__ dnew,
+                                                     double *__restrict__ part_dq) {
+    __shared__ double lds[8];
+    __shared__ int flag_lds[1];
+    if (block_flag(&st->done, flag_lds)) return;
+    const bool lead = (blockIdx.x | blockIdx.y | blockIdx.z) == 0 && tid_lin() == 0;
+    double beta = 0.0;
+    const double rr = sum_partials(part_rr, nb, lds);
+    if (it == 0) {
+        const double bb = sum_partials(part_bb, nb, lds);
+        const bool stop = !(rr > 1e-24) || (rel_tol2 > 0.0 && rr <= rel_tol2 * bb);
+        if (lead) {
+            st->b2 = bb;
+            st->r2_init = rr;
+            st->delta[0] = rr;
+            if (stop) {
+                st->done = 1;
+                st->iters = 0;
+                st->r2_final = rr;
+                st->converged = 1;
+            }
+        }
+        if (stop) return;
+    } else {
+        const bool stop = rr < 1e-20 || (rel_tol2 > 0.0 && rr <= rel_tol2 * st->b2);
+        if (lead) {
+            if (stop) {
+                st->done = 1;
+                st->iters = it;
+                st->r2_final = rr;
+                st->converged = 1;
+            } else {
+                st->delta[it & 1] = rr;
+            }
+        }
+        if (stop) return;
+        beta = rr / st->delta[(it - 1) & 1];
+    }
+    const T tb = (T)beta;
+    const bool first = it == 0;
+    auto dval = [&](int a) -> T { return first ? r[a] : r[a] + tb * dold[a]; };
+    const int i = blockIdx.x * kBX + threadIdx.x;
+    const int j = blockIdx.y * kBY + threadIdx.y;
+    const int kb = blockIdx.z * zchunk;
+    const int ke = min(kb + zchunk, g.nz);
+    double acc[1] = {0.0};
+    if (i < g.nx && j < g.ny) {
+        for (int k = kb; k < ke; ++k) {
+            const int id = (k * g.ny + j) * g.nx + i;
+            const T dn = dval(id);
+            if (is_interior(g, i, j, k)) {
+                const T q = lap_at<T>(g, i, j, k, id, dval);
+                acc[0] += (double)dn * (double)q;
+            }
+            dnew[id] = dn;
+        }
+    }
+    block_sum<1>(acc, lds);
+    if (tid_lin() == 0)
+        part_dq[(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = acc[0];
+}
+
+// CG kernel 2 of iteration `it` (src/fluid.rs:466-478): finish d.q, alpha,
+// p += alpha d, r -= alpha A d (A d recomputed), partial r.r.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_cg_update(Dims g, int zchunk, int it,
+                                                        const double *__restrict__ part_dq,
+                                                        int nb, CgState *st,
+                                                        const T *__restrict__ d,
+                                                        T *__restrict__ p, T *__restrict__ r,
+                                                        double *__restrict__ part_rr) {
+    __shared__ double lds[8];
+    __shared__ int flag_lds[1];
+    if (block_flag(&st->done, flag_lds)) return;
+    const bool lead = (blockIdx.x | blockIdx.y | blockIdx.z) == 0 && tid_lin() == 0;
+    const double dq = sum_partials(part_dq, nb, lds);
+    if (fabs(dq) < 1e-28) {
+        if (lead) {
+            st->done = 1;
+            st->iters = it;
+            st->converged = 1;
+            st->r2_final = st->delta[it & 1];
+        }
+        return;
+    }
+    const double alpha = st->delta[it & 1] / dq;
+    const T ta = (T)alpha;
+    const int i = blockIdx.x * kBX + threadIdx.x;
+    const int j = blockIdx.y * kBY + threadIdx.y;
+    const int kb = blockIdx.z * zchunk;
+    const int ke = min(kb + zchunk, g.nz);
+    double acc[1] = {0.0};
+    if (i < g.nx && j < g.ny) {
+        for (int k = kb; k < ke; ++k) {
+            const int id = (k * g.ny + j) * g.nx + i;
+            const T dc = d[id];
+            T q = T(0);
+            if (is_interior(g, i, j, k)) q = lap_at<T>(g, i, j, k, id, [&](int a) { return d[a]; });
+            p[id] += ta * dc;
+            const T rn = r[id] - ta * q;
+            r[id] = rn;
+            acc[0] += (double)rn * (double)rn;
+        }
+    }
+    block_sum<1>(acc, lds);
+    if (tid_lin() == 0)
+        part_rr[(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = acc[0];
+}
+
+// Closes a solve that ran out of iterations (or had max_iters == 0).
+__global__ __launch_bounds__(kThreads) void k_cg_finalize(int max_iters, double rel_tol2,
+                                                          const double *__restrict__ part_rr,
+                                                          const double *__restrict__ part_bb,
+                                                          int nb, CgState *st,
+                                                          int *__restrict__ iters_log) {
+    __shared__ double lds[8];
+    __shared__ int flag_lds[1];
+    if (!block_flag(&st->done, flag_lds)) {
+        const double rr = sum_partials(part_rr, nb, lds);
+        double bb = 0.0;
+        if (max_iters == 0) bb = sum_partials(part_bb, nb, lds);
+        if (tid_lin() == 0) {
+            if (max_iters == 0) {
+                st->b2 = bb;
+                st->r2_init = rr;
+            }
+            st->iters = max_iters;
+            st->r2_final = rr;
+            st->converged = max_iters > 0 && (rr < 1e-20 ||
+                                              (rel_tol2 > 0.0 && rr <= rel_tol2 * st->b2));
+            st->done = 1;
+        }
+    }
+    __syncthreads();
+    if (iters_log && tid_lin() == 0) *iters_log = st->iters;
+}
+
+// Red-black SOR sweep of one colour (src/fluid.rs:400-437).  Same-colour
+// cells are independent, so this equals the reference's sequential loop.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_rbgs(Dims g, int color, T omega,
+                                                   const T *__restrict__ b, T *__restrict__ p) {
+    const int i = blockIdx.x * kBX + threadIdx.x;
+    const int j = blockIdx.y * kBY + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= g.nx || j >= g.ny) return;
+    if (!is_interior(g, i, j, k) || ((i + j + k) & 1) != color) return;
+    const int id = (k * g.ny + j) * g.nx + i;
+    T diag = T(0), nb = T(0);
+    if (i > 1) { diag += T(1); nb += p[id - 1]; }
+    if (i + 2 < g.nx) { diag += T(1); nb += p[id + 1]; }
+    if (j > 1) { diag += T(1); nb += p[id - g.nx]; }
+    if (j + 2 < g.ny) { diag += T(1); nb += p[id + g.nx]; }
+    if (k > 1) { diag += T(1); nb += p[id - g.sxy]; }
+    if (k + 2 < g.nz) { diag += T(1); nb += p[id + g.sxy]; }
+    const T gs = (b[id] + nb) / diag;
+    p[id] = (T(1) - omega) * p[id] + omega * gs;
+}
+
+// Gradient subtraction and the closing wall masks (src/fluid.rs:509-528).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_gradient_bc(Dims g, T inv_h,
+                                                          const T *__restrict__ p,
+                                                          T *__restrict__ u, T *__restrict__ v,
+                                                          T *__restrict__ w) {
+    const int i = blockIdx.x * kBX + threadIdx.x;
+    const int j = blockIdx.y * kBY + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= g.nx || j >= g.ny) return;
+    const int id = (k * g.ny + j) * g.nx + i;
+    T uu = u[id], vv = v[id], ww = w[id];
+    if (is_interior(g, i, j, k)) {
+        const T pc = p[id];
+        if (i > 1) uu -= (pc - p[id - 1]) * inv_h;
+        if (j > 1) vv -= (pc - p[id - g.nx]) * inv_h;
+        if (k > 1) ww -= (pc - p[id - g.sxy]) * inv_h;
+    }
+    if (bc_u(g, i, j, k)) uu = T(0);
+    if (bc_v(g, i, j, k)) vv = T(0);
+    if (bc_w(g, i, j, k)) ww = T(0);
+    u[id] = uu;
+    v[id] = vv;
+    w[id] = ww;
+}
+
+// ---------------------------------------------------------------------------
+// Telemetry (src/fluid.rs:613-672), FP64 accumulation.
+// Partial layout per workgroup: ke, enstrophy, mass, cx, cy, cz, peak.
+// ---------------------------------------------------------------------------
+constexpr int kTelParts = 7;
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_telemetry_part(Dims g, int zchunk, double h,
+                                                             const T *__restrict__ u,
+                                                             const T *__restrict__ v,
+                                                             const T *__restrict__ w,
+                                                             const T *__restrict__ rho,
+                                                             double *__restrict__ part) {
+    __shared__ double lds[32];
+    const double dv = h * h * h;
+    const double inv_2h = 0.5 / h;
+    const int i = blockIdx.x * kBX + threadIdx.x;
+    const int j = blockIdx.y * kBY + threadIdx.y;
+    const int kb = blockIdx.z * zchunk;
+    const int ke = min(kb + zchunk, g.nz);
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    double peak = 0.0;
+    if (i < g.nx && j < g.ny) {
+        for (int k = kb; k < ke; ++k) {
+            if (!is_interior(g, i, j, k)) continue;
+            const int id = (k * g.ny + j) * g.nx + i;
+            const double uc = u[id], vc = v[id], wc = w[id];
+            const double u2 = uc * uc + vc * vc + wc * wc;
+            acc[0] += 0.5 * u2 * dv;
+            const double dw_dy = ((double)w[id + g.nx] - (double)w[id - g.nx]) * inv_2h;
+            const double dv_dz = ((double)v[id + g.sxy] - (double)v[id - g.sxy]) * inv_2h;
+            const double du_dz = ((double)u[id + g.sxy] - (double)u[id - g.sxy]) * inv_2h;
+            const double dw_dx = ((double)w[id + 1] - (double)w[id - 1]) * inv_2h;
+            const double dv_dx = ((double)v[id + 1] - (double)v[id - 1]) * inv_2h;
+            const double du_dy = ((double)u[id + g.nx] - (double)u[id - g.nx]) * inv_2h;
+            const double ox = dw_dy - dv_dz, oy = du_dz - dw_dx, oz = dv_dx - du_dy;
+            const double w2 = ox * ox + oy * oy + oz * oz;
+            acc[1] += 0.5 * w2 * dv;
+            peak = fmax(peak, sqrt(w2));
+            const double r = rho[id];
+            if (r > 1e-8) {
+                const double m = r * dv;
+                acc[2] += m;
+                acc[3] += (2.0 * ((double)i / (double)(g.nx - 1)) - 1.0) * m;
+                acc[4] += (2.0 * ((double)j / (double)(g.ny - 1)) - 1.0) * m;
+                acc[5] += (2.0 * ((double)k / (double)(g.nz - 1)) - 1.0) * m;
+            }
+        }
+    }
+    block_sum<6>(acc, lds);
+    peak = block_max(peak, lds);
+    if (tid_lin() == 0) {
+        const int bid = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+        double *o = part + (size_t)bid * kTelParts;
+        for (int a = 0; a < 6; ++a) o[a] = acc[a];
+        o[6] = peak;
+    }
+}
+
+// One workgroup: fixed-order combination of the partials into a
+// FluidTelemetryFrame {time, ke, enstrophy, peak, mass, com xyz}.
+__global__ __launch_bounds__(kThreads) void k_telemetry_final(const double *__restrict__ part,
+                                                              int nb, double time_sec,
+                                                              double *__restrict__ out) {
+    __shared__ double lds[32];
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    double peak = 0.0;
+    for (int a = tid_lin(); a < nb; a += kThreads) {
+        const double *o = part + (size_t)a * kTelParts;
+        for (int c = 0; c < 6; ++c) acc[c] += o[c];
+        peak = fmax(peak, o[6]);
+    }
+    block_sum<6>(acc, lds);
+    peak = block_max(peak, lds);
+    if (tid_lin() == 0) {
+        double mass = acc[2], cx = acc[3], cy = acc[4], cz = acc[5];
+        if (mass > 1e-12) {
+            const double inv = 1.0 / mass;  // Vec3 /= f64 (math.rs:226-239)
+            cx = cx * inv;
+            cy = cy * inv;
+            cz = cz * inv;
+        }
+        out[0] = time_sec;
+        out[1] = acc[0];
+        out[2] = acc[1];
+        out[3] = peak;
+        out[4] = mass;
+        out[5] = fmin(fmax(cx, -1.0), 1.0);
+        out[6] = fmin(fmax(cy, -1.0), 1.0);
+        out[7] = fmin(fmax(cz, -1.0), 1.0);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Small utilities
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void k_partials_final(const double *__restrict__ part,
+                                                             int nb, int op,
+                                                             double *__restrict__ out) {
+    __shared__ double lds[8];
+    if (op == 0) {
+        const double s = sum_partials(part, nb, lds);
+        if (tid_lin() == 0) *out = s;
+    } else {
+        double m = 0.0;
+        for (int a = tid_lin(); a < nb; a += kThreads) m = fmax(m, part[a]);
+        m = block_max(m, lds);
+        if (tid_lin() == 0) *out = m;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_max_part(const T *__restrict__ x, long n,
+                                                       double *__restrict__ part) {
+    __shared__ double lds[8];
+    double m = 0.0;
+    for (long a = (long)blockIdx.x * kThreads + tid_lin(); a < n; a += (long)gridDim.x * kThreads)
+        m = fmax(m, (double)x[a]);
+    m = block_max(m, lds);
+    if (tid_lin() == 0) part[blockIdx.x] = m;
+}
+
+template <typename Tin, typename Tout>
+__global__ __launch_bounds__(kThreads) void k_convert(const Tin *__restrict__ in, long n,
+                                                      Tout *__restrict__ out) {
+    for (long a = (long)blockIdx.x * kThreads + tid_lin(); a < n; a += (long)gridDim.x * kThreads)
+        out[a] = (Tout)in[a];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_fill(T *__restrict__ x, long n, T val) {
+    for (long a = (long)blockIdx.x * kThreads + tid_lin(); a < n; a += (long)gridDim.x * kThreads)
+        x[a] = val;
+}
+
+// Point sampling of src/fluid.rs:94-128, evaluated in FP64.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_sample_points(Dims g, const T *__restrict__ f,
+                                                            const double *__restrict__ xyz,
+                                                            long npts, double *__restrict__ out) {
+    const long a = (long)blockIdx.x * kThreads + tid_lin();
+    if (a >= npts) return;
+    double c[3];
+    int i0[3];
+    const int n[3] = {g.nx, g.ny, g.nz};
+    for (int d = 0; d < 3; ++d) {
+        double x = xyz[3 * a + d];
+        if (x < 0.5) x = 0.5;
+        if (x > (double)n[d] - 1.500001) x = (double)n[d] - 1.500001;
+        const double fl = floor(x);
+        i0[d] = (int)fl;
+        c[d] = x - fl;
+    }
+    const int b = (i0[2] * g.ny + i0[1]) * g.nx + i0[0];
+    out[a] = lerp3<double>(f[b], f[b + 1], f[b + g.nx], f[b + g.nx + 1], f[b + g.sxy],
+                           f[b + g.sxy + 1], f[b + g.sxy + g.nx], f[b + g.sxy + g.nx + 1], c[0],
+                           c[1], c[2]);
+}
+
+}  // namespace aether

@@ -1,0 +1,182 @@
+/*
+ * aether_fluid.h -- C ABI of the MI355X (gfx950) smoke solver.
+ *
+ * Drop-in boundary for the reference's `pub mod fluid` (src/lib.rs:4):
+ * every entry point below replaces one public item of `FluidSolver3D`
+ * (src/fluid.rs:23-673), cited beside it.  Plain C types only: opaque handle,
+ * ints, doubles, host pointers and sizes.  Fields cross the boundary as f64
+ * host arrays in the reference's layout, idx = (k*ny + j)*nx + i
+ * (src/fluid.rs:88-90); the library keeps them on the device in FP32 or FP64.
+ *
+ * Every function returns 0 on success or a negative AETHER_E* code; the text
+ * of the last error on the calling thread is returned by
+ * aether_fluid_last_error().  The library never aborts the process.
+ *
+ * A handle is not thread-safe (the reference's methods take &mut self); two
+ * handles may be driven concurrently from different threads.  Every call is
+ * synchronous at return, like the reference's blocking methods.
+ */
+#ifndef AETHER_FLUID_H
+#define AETHER_FLUID_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AETHER_FLUID_ABI_VERSION 1
+
+/* status codes */
+#define AETHER_OK 0
+#define AETHER_EINVAL (-1)   /* bad argument: dims < 6, null handle, bad field id, size mismatch */
+#define AETHER_EHIP (-2)     /* HIP runtime error */
+#define AETHER_ENOMEM (-3)   /* device allocation failed */
+#define AETHER_ENODEV (-4)   /* no usable gfx950 device */
+
+/* precision of the device state */
+#define AETHER_FP32 0        /* FP32 fields and stencils, FP64 CG dot products */
+#define AETHER_FP64 1        /* FP64 everywhere (parity instantiation) */
+#define AETHER_FP32_IR64 2   /* FP32 fields; FP64 pressure/rhs with FP32 inner CG inside FP64 iterative refinement */
+
+/* Poisson solver, src/fluid.rs:17-21 PoissonSolverKind */
+#define AETHER_SOLVER_CG 0
+#define AETHER_SOLVER_RBGS 1
+
+/* source kinds */
+#define AETHER_SOURCE_JETS 0   /* src/fluid.rs:532-595 inject_cyberpunk_sources */
+#define AETHER_SOURCE_SPHERE 1 /* sphere (cell-index coords, d^2<r^2, interior cells) that sets rho and T */
+#define AETHER_SOURCE_NONE 2
+
+/* field ids: the reference's pub arrays (src/fluid.rs:28-34) */
+#define AETHER_FIELD_U 0
+#define AETHER_FIELD_V 1
+#define AETHER_FIELD_W 2
+#define AETHER_FIELD_DENSITY 3
+#define AETHER_FIELD_TEMPERATURE 4
+#define AETHER_FIELD_PRESSURE 5
+#define AETHER_FIELD_DIVERGENCE 6
+
+typedef struct aether_fluid_s *aether_fluid_t;
+
+/* Solver knobs.  The first five are the reference's pub fields
+ * (src/fluid.rs:45-49, defaults src/fluid.rs:79-83); the rest are constants
+ * the reference hard-codes, exposed with the reference's values as defaults. */
+typedef struct {
+    double vorticity_eps;           /* 5.5 */
+    double buoyancy_beta;           /* 6.8 */
+    double gravity_alpha;           /* 0.45 */
+    int32_t solver_kind;            /* AETHER_SOLVER_CG */
+    int32_t solver_iters;           /* 32: CG max iterations / RBGS sweeps */
+    double sor_omega;               /* 1.65 (src/fluid.rs:398) */
+    double cg_rel_tol;              /* 0: reference absolute guards only (src/fluid.rs:462,469,479);
+                                       >0: also stop when ||r||2 <= tol*||b||2 */
+    double dissipation_velocity;    /* 0.996 (src/fluid.rs:599-601) */
+    double dissipation_density;     /* 0.996 (src/fluid.rs:607) */
+    double dissipation_temperature; /* 0.985 (src/fluid.rs:608) */
+    int32_t source_kind;            /* AETHER_SOURCE_JETS */
+    int32_t reserved0;
+    double source_center[3];        /* sphere source centre, cell-index coordinates */
+    double source_radius;           /* sphere radius in cells */
+    double source_density;          /* value set inside the sphere */
+    double source_temperature;      /* value set inside the sphere */
+    double ir_inner_rel_tol;        /* AETHER_FP32_IR64: inner FP32 CG tolerance (1e-3) */
+    double ir_div_tol;              /* AETHER_FP32_IR64: stop when max|div u| <= this (1e-7) */
+    int32_t ir_max_outer;           /* AETHER_FP32_IR64: max refinement sweeps (30) */
+    int32_t reserved1;
+} aether_fluid_params;
+
+/* src/fluid.rs:6-15 FluidTelemetryFrame (center_of_mass flattened) */
+typedef struct {
+    double time_sec;
+    double kinetic_energy;
+    double enstrophy;
+    double peak_vorticity;
+    double total_mass;
+    double center_of_mass[3];
+} aether_fluid_telemetry;
+
+/* Statistics of the last pressure solve (no counterpart in the reference,
+ * whose CG returns nothing: src/fluid.rs:445). */
+typedef struct {
+    int32_t iterations;      /* CG iterations that updated p (RBGS: sweeps) */
+    int32_t converged;       /* 1 if a stopping test fired before max iterations */
+    double b_norm2;          /* ||b||_2^2 of the right-hand side */
+    double r_norm2_initial;  /* ||b - A p0||_2^2 */
+    double r_norm2_final;    /* recursive residual ||r||_2^2 at exit */
+    int32_t outer_iterations;/* AETHER_FP32_IR64 refinement sweeps (else 0) */
+    int32_t reserved;
+    double div_max_final;    /* AETHER_FP32_IR64: max|div u| reached (else 0) */
+} aether_fluid_solve_stats;
+
+/* Library / device information. */
+int aether_fluid_abi_version(void);
+const char *aether_fluid_last_error(void);
+int aether_fluid_device_count(int *count);
+
+/* src/fluid.rs:53-85 FluidSolver3D::new.  h <= 0 selects the reference rule
+ * h = 1/max(nx,ny,nz) (src/fluid.rs:56); nx,ny,nz >= 6 (src/fluid.rs:54). */
+int aether_fluid_create(int nx, int ny, int nz, double h, int precision, aether_fluid_t *out);
+int aether_fluid_destroy(aether_fluid_t s);
+int aether_fluid_dims(aether_fluid_t s, int *nx, int *ny, int *nz, double *h, int *precision);
+
+/* Reference defaults for every knob (src/fluid.rs:79-83, 398, 599-608). */
+int aether_fluid_default_params(aether_fluid_params *out);
+/* Writes of the pub fields vorticity_eps, buoyancy_beta, gravity_alpha,
+ * solver_kind, solver_iters (src/fluid.rs:45-49). */
+int aether_fluid_set_params(aether_fluid_t s, const aether_fluid_params *p);
+int aether_fluid_get_params(aether_fluid_t s, aether_fluid_params *p);
+
+/* src/fluid.rs:598-611 FluidSolver3D::step.  `out` may be NULL. */
+int aether_fluid_step(aether_fluid_t s, double dt, double time_sec, aether_fluid_telemetry *out);
+/* Batched driver of the src/main.rs:89-94 loop: runs `nsteps` steps, step i
+ * at time times[i], writing one telemetry frame per step into `out` (may be
+ * NULL), with a single host synchronisation at the end. */
+int aether_fluid_run(aether_fluid_t s, double dt, const double *times, int nsteps,
+                     aether_fluid_telemetry *out);
+
+/* Per-stage methods (the calls made by src/verify.rs:231-238 and the
+ * fluid.rs tests). */
+int aether_fluid_advect_maccormack_in_place(aether_fluid_t s, int field_kind, double dt, double dissipation); /* fluid.rs:170 */
+int aether_fluid_enforce_wall_velocity_bc(aether_fluid_t s);                            /* fluid.rs:236 */
+int aether_fluid_apply_buoyancy_and_vorticity_confinement(aether_fluid_t s, double dt);                     /* fluid.rs:256 */
+int aether_fluid_compute_divergence_norm(aether_fluid_t s, double *rms);                        /* fluid.rs:333 */
+int aether_fluid_project(aether_fluid_t s);                                             /* fluid.rs:496 */
+int aether_fluid_solve_poisson_conjugate_gradient(aether_fluid_t s, int max_iters);                             /* fluid.rs:445 */
+int aether_fluid_solve_poisson_gauss_seidel(aether_fluid_t s, int sweeps);                              /* fluid.rs:397 */
+int aether_fluid_inject_sources(aether_fluid_t s, double time_sec, double dt);          /* fluid.rs:532 */
+int aether_fluid_compute_telemetry(aether_fluid_t s, double time_sec, aether_fluid_telemetry *out); /* fluid.rs:613 */
+/* fluid.rs:94 sample_trilinear, for npts points (x,y,z interleaved). */
+int aether_fluid_sample_trilinear(aether_fluid_t s, int field, const double *xyz, size_t npts,
+                                  double *out);
+/* Extension: max |div u| over interior cells (same stencil as fluid.rs:341-344). */
+int aether_fluid_divergence_max(aether_fluid_t s, double *out);
+int aether_fluid_last_solve_stats(aether_fluid_t s, aether_fluid_solve_stats *out);
+
+/* Field access (the reference's pub Vec<f64> arrays, src/fluid.rs:28-34).
+ * n must equal nx*ny*nz. */
+int aether_fluid_upload(aether_fluid_t s, int field, const double *src, size_t n);
+int aether_fluid_download(aether_fluid_t s, int field, double *dst, size_t n);
+
+/* src/fluid.rs:703-708 FluidMedium::from_solver: max density over all cells
+ * (reduced on the device) and the f64 density/temperature snapshots. */
+int aether_fluid_max_density(aether_fluid_t s, double *out);
+
+/* Seeded inputs: the reference's SplitMix64-seeded Xoshiro256++ stream
+ * (src/rng.rs:21-33, 48-64, 74-88).  Fills out[0..n) with range_f64(lo,hi)
+ * draws continuing from *state (4 words), which is advanced. */
+int aether_rng_seed(uint64_t seed, uint64_t state[4]);
+int aether_rng_fill_range(uint64_t state[4], double lo, double hi, double *out, size_t n);
+
+/* Measurement hooks used by the benchmark driver. */
+/* Average device time (ms) per launch of the CG iteration kernels during the
+ * last aether_fluid_run with timing enabled (HIP events on the solver's
+ * stream); *launches counts the kernels timed. */
+int aether_fluid_set_kernel_timing(aether_fluid_t s, int enable);
+int aether_fluid_kernel_timing(aether_fluid_t s, int which, double *avg_ms, long *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AETHER_FLUID_H */

@@ -1,0 +1,709 @@
+/*
+ * fluid_oracle.c -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C, f64 restatement of the reference smoke solver `FluidSolver3D`
+ * (src/fluid.rs:23-673) and of its seeded RNG (src/rng.rs:5-88).  It is the
+ * parity checker for the HIP library and the CPU baseline timed by bench.py.
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load it; the product library never links or calls it.
+ *
+ * Fidelity rules (so that the f64 results follow the Rust code operation for
+ * operation): same loop nests and iteration order, same in-place sequencing,
+ * same left-to-right evaluation of every expression, no FMA contraction
+ * (build with -ffp-contract=off), sequential reductions in index order.
+ * Rust's f64::clamp/min/max are restated as rclamp/fmin/fmax (identical for
+ * non-NaN input).
+ *
+ * Extensions (all default to the reference's behaviour):
+ *   h override            (fluid.rs:56 rule when h <= 0)
+ *   sor_omega             (fluid.rs:398 hard-codes 1.65)
+ *   cg_rel_tol            (stop when ||r||2 <= tol*||b||2; 0 = reference guards only)
+ *   dissipation triple    (fluid.rs:599-608 hard-codes 0.996/0.996/0.985)
+ *   source_kind           0 = reference jets (fluid.rs:532-595),
+ *                         1 = sphere that SETS density/temperature,
+ *                         2 = no source
+ *   CG statistics         iterations run, ||b||^2, final ||r||^2
+ *
+ * Parity pins (tests/test_oracle.py): README.md:54 known answer of the
+ * verify.rs:229-250 scenario, the fluid.rs:918-968 tests, closed forms
+ * (trilinear exactness on affine fields, integer shifts, Laplacian symmetry
+ * and null space, brute-force dense Poisson solve, curl of linear fields,
+ * telemetry of a single unit mass), and the published Xoshiro256++ /
+ * SplitMix64 vectors.
+ *
+ * Built twice by oracle/Makefile: sequential (the faithful stand-in) and with
+ * -fopenmp (the multi-core CPU baseline; its reductions are re-associated by
+ * OpenMP, so it is used for timing only, never as the checker).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#define OF_PRAGMA(x) _Pragma(#x)
+#define OMP_FOR OF_PRAGMA(omp parallel for schedule(static))
+#define OMP_FOR_SUM1(a) OF_PRAGMA(omp parallel for schedule(static) reduction(+ : a))
+#else
+#define OMP_FOR
+#define OMP_FOR_SUM1(a)
+#endif
+
+#define TAU_ 6.28318530717958647692528676655900577
+
+typedef struct {
+    size_t nx, ny, nz, n;
+    double h;
+    double *u, *v, *w, *density, *temperature, *pressure, *divergence;
+    double *curl_x, *curl_y, *curl_z, *curl_mag;
+    double *tmp_fwd, *tmp_bwd, *tmp_out, *snap;
+    double *cg_r, *cg_d, *cg_q;
+    double vorticity_eps, buoyancy_beta, gravity_alpha;
+    int solver_kind; /* 0 = CG, 1 = red-black Gauss-Seidel/SOR */
+    long solver_iters;
+    double sor_omega, cg_rel_tol;
+    double diss_vel, diss_rho, diss_temp;
+    int source_kind;
+    double src_cx, src_cy, src_cz, src_r, src_rho, src_temp;
+    /* statistics of the last pressure solve */
+    long stat_iters;
+    double stat_b2, stat_r2_init, stat_r2;
+    int stat_converged;
+} OFluid;
+
+/* Field ids shared with the product ABI. */
+enum { F_U = 0, F_V, F_W, F_DENSITY, F_TEMPERATURE, F_PRESSURE, F_DIVERGENCE,
+       F_CURL_MAG, F_CG_R };
+
+static inline size_t IX(const OFluid *s, size_t i, size_t j, size_t k) {
+    return (k * s->ny + j) * s->nx + i; /* fluid.rs:88-90 */
+}
+
+static inline double rclamp(double x, double lo, double hi) {
+    if (x < lo) x = lo;
+    if (x > hi) x = hi;
+    return x;
+}
+
+/* ------------------------------------------------------------------ */
+/* construction                                                        */
+/* ------------------------------------------------------------------ */
+
+/* fluid.rs:53-85 (h_override > 0 replaces the 1/max(n) rule of fluid.rs:56) */
+OFluid *of_create(long nx, long ny, long nz, double h_override) {
+    if (nx < 6 || ny < 6 || nz < 6) return NULL; /* fluid.rs:54 */
+    OFluid *s = (OFluid *)calloc(1, sizeof(OFluid));
+    s->nx = (size_t)nx; s->ny = (size_t)ny; s->nz = (size_t)nz;
+    s->n = s->nx * s->ny * s->nz;
+    size_t m = s->nx;
+    if (s->ny > m) m = s->ny;
+    if (s->nz > m) m = s->nz;
+    s->h = h_override > 0.0 ? h_override : 1.0 / (double)m;
+    double **arrs[] = {&s->u, &s->v, &s->w, &s->density, &s->temperature, &s->pressure,
+                       &s->divergence, &s->curl_x, &s->curl_y, &s->curl_z, &s->curl_mag,
+                       &s->tmp_fwd, &s->tmp_bwd, &s->tmp_out, &s->snap, &s->cg_r, &s->cg_d,
+                       &s->cg_q};
+    for (size_t a = 0; a < sizeof(arrs) / sizeof(arrs[0]); ++a)
+        *arrs[a] = (double *)calloc(s->n, sizeof(double));
+    s->vorticity_eps = 5.5; /* fluid.rs:79-83 */
+    s->buoyancy_beta = 6.8;
+    s->gravity_alpha = 0.45;
+    s->solver_kind = 0;
+    s->solver_iters = 32;
+    s->sor_omega = 1.65;  /* fluid.rs:398 */
+    s->cg_rel_tol = 0.0;
+    s->diss_vel = 0.996;  /* fluid.rs:599-607 */
+    s->diss_rho = 0.996;
+    s->diss_temp = 0.985; /* fluid.rs:608 */
+    s->source_kind = 0;
+    return s;
+}
+
+void of_destroy(OFluid *s) {
+    if (!s) return;
+    double *arrs[] = {s->u, s->v, s->w, s->density, s->temperature, s->pressure,
+                      s->divergence, s->curl_x, s->curl_y, s->curl_z, s->curl_mag,
+                      s->tmp_fwd, s->tmp_bwd, s->tmp_out, s->snap, s->cg_r, s->cg_d, s->cg_q};
+    for (size_t a = 0; a < sizeof(arrs) / sizeof(arrs[0]); ++a) free(arrs[a]);
+    free(s);
+}
+
+void of_set_params(OFluid *s, double eps, double beta, double alpha, int solver_kind,
+                   long solver_iters, double omega, double rel_tol, double diss_vel,
+                   double diss_rho, double diss_temp) {
+    s->vorticity_eps = eps;
+    s->buoyancy_beta = beta;
+    s->gravity_alpha = alpha;
+    s->solver_kind = solver_kind;
+    s->solver_iters = solver_iters;
+    s->sor_omega = omega;
+    s->cg_rel_tol = rel_tol;
+    s->diss_vel = diss_vel;
+    s->diss_rho = diss_rho;
+    s->diss_temp = diss_temp;
+}
+
+void of_set_source(OFluid *s, int kind, double cx, double cy, double cz, double r,
+                   double rho, double temp) {
+    s->source_kind = kind;
+    s->src_cx = cx; s->src_cy = cy; s->src_cz = cz;
+    s->src_r = r; s->src_rho = rho; s->src_temp = temp;
+}
+
+double *of_field(OFluid *s, int id) {
+    switch (id) {
+    case F_U: return s->u;
+    case F_V: return s->v;
+    case F_W: return s->w;
+    case F_DENSITY: return s->density;
+    case F_TEMPERATURE: return s->temperature;
+    case F_PRESSURE: return s->pressure;
+    case F_DIVERGENCE: return s->divergence;
+    case F_CURL_MAG: return s->curl_mag;
+    case F_CG_R: return s->cg_r;
+    default: return NULL;
+    }
+}
+
+double of_h(const OFluid *s) { return s->h; }
+
+void of_stats(const OFluid *s, long *iters, double *b2, double *r2_init, double *r2,
+              int *converged) {
+    *iters = s->stat_iters;
+    *b2 = s->stat_b2;
+    *r2_init = s->stat_r2_init;
+    *r2 = s->stat_r2;
+    *converged = s->stat_converged;
+}
+
+/* ------------------------------------------------------------------ */
+/* sampling and advection                                              */
+/* ------------------------------------------------------------------ */
+
+/* fluid.rs:94-128 */
+static double sample_trilinear(const OFluid *s, const double *f, double x, double y, double z) {
+    double xc = rclamp(x, 0.5, (double)s->nx - 1.500001);
+    double yc = rclamp(y, 0.5, (double)s->ny - 1.500001);
+    double zc = rclamp(z, 0.5, (double)s->nz - 1.500001);
+    size_t i0 = (size_t)floor(xc), j0 = (size_t)floor(yc), k0 = (size_t)floor(zc);
+    size_t i1 = i0 + 1, j1 = j0 + 1, k1 = k0 + 1;
+    double tx = xc - (double)i0, ty = yc - (double)j0, tz = zc - (double)k0;
+    double c000 = f[IX(s, i0, j0, k0)], c100 = f[IX(s, i1, j0, k0)];
+    double c010 = f[IX(s, i0, j1, k0)], c110 = f[IX(s, i1, j1, k0)];
+    double c001 = f[IX(s, i0, j0, k1)], c101 = f[IX(s, i1, j0, k1)];
+    double c011 = f[IX(s, i0, j1, k1)], c111 = f[IX(s, i1, j1, k1)];
+    double c00 = c000 * (1.0 - tx) + c100 * tx;
+    double c10 = c010 * (1.0 - tx) + c110 * tx;
+    double c01 = c001 * (1.0 - tx) + c101 * tx;
+    double c11 = c011 * (1.0 - tx) + c111 * tx;
+    double c0 = c00 * (1.0 - ty) + c10 * ty;
+    double c1 = c01 * (1.0 - ty) + c11 * ty;
+    return c0 * (1.0 - tz) + c1 * tz;
+}
+
+double of_sample_trilinear(const OFluid *s, int field, double x, double y, double z) {
+    return sample_trilinear(s, of_field((OFluid *)s, field), x, y, z);
+}
+
+/* fluid.rs:131-150 */
+static void stencil_minmax(const OFluid *s, const double *f, double x, double y, double z,
+                           double *lo_out, double *hi_out) {
+    double xc = rclamp(x, 0.5, (double)s->nx - 1.500001);
+    double yc = rclamp(y, 0.5, (double)s->ny - 1.500001);
+    double zc = rclamp(z, 0.5, (double)s->nz - 1.500001);
+    size_t i0 = (size_t)floor(xc), j0 = (size_t)floor(yc), k0 = (size_t)floor(zc);
+    double lo = INFINITY, hi = -INFINITY;
+    for (size_t dk = 0; dk <= 1; ++dk)
+        for (size_t dj = 0; dj <= 1; ++dj)
+            for (size_t di = 0; di <= 1; ++di) {
+                double v = f[IX(s, i0 + di, j0 + dj, k0 + dk)];
+                lo = fmin(lo, v);
+                hi = fmax(hi, v);
+            }
+    *lo_out = lo;
+    *hi_out = hi;
+}
+
+/* fluid.rs:153-167 */
+static void advect_semi_lagrangian(const OFluid *s, const double *src, double *dst, double dt) {
+    double scale = dt / s->h;
+    memset(dst, 0, s->n * sizeof(double));
+    OMP_FOR
+    for (size_t k = 1; k < s->nz - 1; ++k)
+        for (size_t j = 1; j < s->ny - 1; ++j)
+            for (size_t i = 1; i < s->nx - 1; ++i) {
+                size_t id = IX(s, i, j, k);
+                double bx = (double)i - scale * s->u[id];
+                double by = (double)j - scale * s->v[id];
+                double bz = (double)k - scale * s->w[id];
+                dst[id] = sample_trilinear(s, src, bx, by, bz);
+            }
+}
+
+/* fluid.rs:170-234 */
+void of_advect(OFluid *s, int kind, double dt, double dissipation) {
+    double *field = kind == 0 ? s->u : kind == 1 ? s->v : kind == 2 ? s->w
+                  : kind == 3 ? s->density : s->temperature;
+    const double *src = s->snap;
+    memcpy(s->snap, field, s->n * sizeof(double));
+    advect_semi_lagrangian(s, src, s->tmp_fwd, dt);
+    advect_semi_lagrangian(s, s->tmp_fwd, s->tmp_bwd, -dt);
+    double *out = s->tmp_out;
+    memset(out, 0, s->n * sizeof(double));
+    double scale = dt / s->h;
+    OMP_FOR
+    for (size_t k = 1; k < s->nz - 1; ++k)
+        for (size_t j = 1; j < s->ny - 1; ++j)
+            for (size_t i = 1; i < s->nx - 1; ++i) {
+                size_t id = IX(s, i, j, k);
+                double bx = (double)i - scale * s->u[id];
+                double by = (double)j - scale * s->v[id];
+                double bz = (double)k - scale * s->w[id];
+                int near_wall = bx < 1.5 || bx > (double)s->nx - 2.5 || by < 1.5 ||
+                                by > (double)s->ny - 2.5 || bz < 1.5 || bz > (double)s->nz - 2.5;
+                double val;
+                if (near_wall) {
+                    val = s->tmp_fwd[id];
+                } else {
+                    double mc = s->tmp_fwd[id] + 0.5 * (src[id] - s->tmp_bwd[id]);
+                    double lo, hi;
+                    stencil_minmax(s, src, bx, by, bz, &lo, &hi);
+                    val = rclamp(mc, lo, hi);
+                }
+                out[id] = val * dissipation;
+            }
+    if (kind <= 2) {
+        memcpy(field, out, s->n * sizeof(double));
+    } else {
+        OMP_FOR
+        for (size_t a = 0; a < s->n; ++a) field[a] = fmax(out[a], 0.0);
+    }
+}
+
+/* fluid.rs:236-253 */
+void of_enforce_bc(OFluid *s) {
+    size_t nx = s->nx, ny = s->ny, nz = s->nz;
+    OMP_FOR
+    for (size_t k = 0; k < nz; ++k)
+        for (size_t j = 0; j < ny; ++j)
+            for (size_t i = 0; i < nx; ++i) {
+                size_t id = IX(s, i, j, k);
+                if (i <= 1 || i + 1 >= nx || j == 0 || j + 1 >= ny || k == 0 || k + 1 >= nz)
+                    s->u[id] = 0.0;
+                if (j <= 1 || j + 1 >= ny || i == 0 || i + 1 >= nx || k == 0 || k + 1 >= nz)
+                    s->v[id] = 0.0;
+                if (k <= 1 || k + 1 >= nz || i == 0 || i + 1 >= nx || j == 0 || j + 1 >= ny)
+                    s->w[id] = 0.0;
+            }
+}
+
+/* fluid.rs:256-330 */
+void of_buoyancy_confinement(OFluid *s, double dt) {
+    size_t nx = s->nx, ny = s->ny, nz = s->nz;
+    double inv_2h = 0.5 / s->h;
+    memset(s->curl_mag, 0, s->n * sizeof(double));
+    OMP_FOR
+    for (size_t k = 1; k < nz - 1; ++k)
+        for (size_t j = 1; j < ny - 1; ++j)
+            for (size_t i = 1; i < nx - 1; ++i) {
+                size_t id = IX(s, i, j, k);
+                double dw_dy = (s->w[IX(s, i, j + 1, k)] - s->w[IX(s, i, j - 1, k)]) * inv_2h;
+                double dv_dz = (s->v[IX(s, i, j, k + 1)] - s->v[IX(s, i, j, k - 1)]) * inv_2h;
+                double du_dz = (s->u[IX(s, i, j, k + 1)] - s->u[IX(s, i, j, k - 1)]) * inv_2h;
+                double dw_dx = (s->w[IX(s, i + 1, j, k)] - s->w[IX(s, i - 1, j, k)]) * inv_2h;
+                double dv_dx = (s->v[IX(s, i + 1, j, k)] - s->v[IX(s, i - 1, j, k)]) * inv_2h;
+                double du_dy = (s->u[IX(s, i, j + 1, k)] - s->u[IX(s, i, j - 1, k)]) * inv_2h;
+                double ox = dw_dy - dv_dz, oy = du_dz - dw_dx, oz = dv_dx - du_dy;
+                s->curl_x[id] = ox;
+                s->curl_y[id] = oy;
+                s->curl_z[id] = oz;
+                s->curl_mag[id] = sqrt(ox * ox + oy * oy + oz * oz);
+            }
+    double eps_h = s->vorticity_eps * s->h;
+    OMP_FOR
+    for (size_t k = 2; k < nz - 2; ++k)
+        for (size_t j = 2; j < ny - 2; ++j)
+            for (size_t i = 2; i < nx - 2; ++i) {
+                size_t id = IX(s, i, j, k);
+                const double *cm = s->curl_mag;
+                double gx = (cm[IX(s, i + 1, j, k)] - cm[IX(s, i - 1, j, k)]) * inv_2h;
+                double gy = (cm[IX(s, i, j + 1, k)] - cm[IX(s, i, j - 1, k)]) * inv_2h;
+                double gz = (cm[IX(s, i, j, k + 1)] - cm[IX(s, i, j, k - 1)]) * inv_2h;
+                double glen = sqrt(gx * gx + gy * gy + gz * gz) + 1e-10;
+                double nxv = gx / glen, nyv = gy / glen, nzv = gz / glen;
+                double ox = s->curl_x[id], oy = s->curl_y[id], oz = s->curl_z[id];
+                double fx = eps_h * (nyv * oz - nzv * oy);
+                double fy = eps_h * (nzv * ox - nxv * oz);
+                double fz = eps_h * (nxv * oy - nyv * ox);
+                s->u[id] += dt * fx;
+                s->v[id] += dt * fy;
+                s->w[id] += dt * fz;
+            }
+    OMP_FOR
+    for (size_t k = 1; k < nz - 1; ++k)
+        for (size_t j = 1; j < ny - 1; ++j)
+            for (size_t i = 1; i < nx - 1; ++i) {
+                size_t id = IX(s, i, j, k);
+                double lift = s->buoyancy_beta * s->temperature[id] -
+                              s->gravity_alpha * s->density[id];
+                s->v[id] += dt * lift;
+            }
+    of_enforce_bc(s);
+}
+
+/* fluid.rs:333-353 */
+double of_divergence_norm(OFluid *s) {
+    size_t nx = s->nx, ny = s->ny, nz = s->nz;
+    double inv_h = 1.0 / s->h;
+    double sum_sq = 0.0;
+    memset(s->divergence, 0, s->n * sizeof(double));
+    OMP_FOR_SUM1(sum_sq)
+    for (size_t k = 1; k < nz - 1; ++k)
+        for (size_t j = 1; j < ny - 1; ++j)
+            for (size_t i = 1; i < nx - 1; ++i) {
+                size_t id = IX(s, i, j, k);
+                double du = s->u[IX(s, i + 1, j, k)] - s->u[id];
+                double dv = s->v[IX(s, i, j + 1, k)] - s->v[id];
+                double dw = s->w[IX(s, i, j, k + 1)] - s->w[id];
+                double div_val = (du + dv + dw) * inv_h;
+                s->divergence[id] = -s->h * (du + dv + dw);
+                sum_sq += div_val * div_val;
+            }
+    double interior_n = (double)((nx - 2) * (ny - 2) * (nz - 2));
+    return sqrt(sum_sq / interior_n);
+}
+
+/* Extension: max |div u| over the interior (same stencil as fluid.rs:341-344). */
+double of_divergence_max(const OFluid *s) {
+    size_t nx = s->nx, ny = s->ny, nz = s->nz;
+    double inv_h = 1.0 / s->h, m = 0.0;
+    for (size_t k = 1; k < nz - 1; ++k)
+        for (size_t j = 1; j < ny - 1; ++j)
+            for (size_t i = 1; i < nx - 1; ++i) {
+                size_t id = IX(s, i, j, k);
+                double du = s->u[IX(s, i + 1, j, k)] - s->u[id];
+                double dv = s->v[IX(s, i, j + 1, k)] - s->v[id];
+                double dw = s->w[IX(s, i, j, k + 1)] - s->w[id];
+                m = fmax(m, fabs((du + dv + dw) * inv_h));
+            }
+    return m;
+}
+
+/* fluid.rs:356-394 */
+static void apply_spd_laplacian(const OFluid *s, const double *x, double *out) {
+    size_t nx = s->nx, ny = s->ny, nz = s->nz;
+    memset(out, 0, s->n * sizeof(double));
+    OMP_FOR
+    for (size_t k = 1; k < nz - 1; ++k)
+        for (size_t j = 1; j < ny - 1; ++j)
+            for (size_t i = 1; i < nx - 1; ++i) {
+                size_t id = IX(s, i, j, k);
+                double diag = 0.0, nb_sum = 0.0;
+                if (i > 1) { diag += 1.0; nb_sum += x[IX(s, i - 1, j, k)]; }
+                if (i + 2 < nx) { diag += 1.0; nb_sum += x[IX(s, i + 1, j, k)]; }
+                if (j > 1) { diag += 1.0; nb_sum += x[IX(s, i, j - 1, k)]; }
+                if (j + 2 < ny) { diag += 1.0; nb_sum += x[IX(s, i, j + 1, k)]; }
+                if (k > 1) { diag += 1.0; nb_sum += x[IX(s, i, j, k - 1)]; }
+                if (k + 2 < nz) { diag += 1.0; nb_sum += x[IX(s, i, j, k + 1)]; }
+                out[id] = diag * x[id] - nb_sum;
+            }
+}
+
+/* Exposed for the operator tests: out = A x with A the fluid.rs:356-394 matrix. */
+void of_apply_laplacian(const OFluid *s, const double *x, double *out) {
+    apply_spd_laplacian(s, x, out);
+}
+
+/* fluid.rs:397-442 (omega parameterised; the reference constant is 1.65) */
+void of_solve_rbgs(OFluid *s, long iterations) {
+    size_t nx = s->nx, ny = s->ny, nz = s->nz;
+    double omega = s->sor_omega;
+    double *p = s->pressure;
+    const double *b = s->divergence;
+    for (long it = 0; it < iterations; ++it)
+        for (size_t color = 0; color < 2; ++color) {
+            OMP_FOR
+            for (size_t k = 1; k < nz - 1; ++k)
+                for (size_t j = 1; j < ny - 1; ++j)
+                    for (size_t i = 1; i < nx - 1; ++i) {
+                        if ((i + j + k) % 2 != color) continue;
+                        size_t id = IX(s, i, j, k);
+                        double diag = 0.0, nb_sum = 0.0;
+                        if (i > 1) { diag += 1.0; nb_sum += p[IX(s, i - 1, j, k)]; }
+                        if (i + 2 < nx) { diag += 1.0; nb_sum += p[IX(s, i + 1, j, k)]; }
+                        if (j > 1) { diag += 1.0; nb_sum += p[IX(s, i, j - 1, k)]; }
+                        if (j + 2 < ny) { diag += 1.0; nb_sum += p[IX(s, i, j + 1, k)]; }
+                        if (k > 1) { diag += 1.0; nb_sum += p[IX(s, i, j, k - 1)]; }
+                        if (k + 2 < nz) { diag += 1.0; nb_sum += p[IX(s, i, j, k + 1)]; }
+                        double gs_target = (b[id] + nb_sum) / diag;
+                        p[id] = (1.0 - omega) * p[id] + omega * gs_target;
+                    }
+        }
+    s->stat_iters = iterations;
+    s->stat_converged = 1;
+}
+
+
+/* fluid.rs:445-493, plus an iteration counter and the optional relative
+ * tolerance ||r||2 <= tol*||b||2, tested at the same points as the
+ * reference's absolute guards (fluid.rs:462, 479). */
+void of_solve_cg(OFluid *s, long max_iters) {
+    size_t n = s->n;
+    double *r = s->cg_r, *d = s->cg_d, *q = s->cg_q, *p = s->pressure;
+    const double *b = s->divergence;
+    apply_spd_laplacian(s, p, q); /* fluid.rs:450-453 warm start */
+    double delta_new = 0.0;
+    for (size_t a = 0; a < n; ++a) {
+        double ri = b[a] - q[a];
+        r[a] = ri;
+        d[a] = ri;
+        delta_new += ri * ri;
+    }
+    double b2 = 0.0;
+    for (size_t a = 0; a < n; ++a) b2 += b[a] * b[a];
+    double tol2 = s->cg_rel_tol * s->cg_rel_tol * b2;
+    int use_rel = s->cg_rel_tol > 0.0;
+    long iters = 0;
+    int converged = 0;
+    s->stat_b2 = b2;
+    s->stat_r2_init = delta_new;
+    if (delta_new > 1e-24 && !(use_rel && delta_new <= tol2)) {
+        for (long it = 0; it < max_iters; ++it) {
+            apply_spd_laplacian(s, d, q);
+            double dq = 0.0;
+            OMP_FOR_SUM1(dq)
+            for (size_t a = 0; a < n; ++a) dq += d[a] * q[a];
+            if (fabs(dq) < 1e-28) { converged = 1; break; }
+            double alpha = delta_new / dq;
+            double delta_next = 0.0;
+            OMP_FOR_SUM1(delta_next)
+            for (size_t a = 0; a < n; ++a) {
+                p[a] += alpha * d[a];
+                r[a] -= alpha * q[a];
+                delta_next += r[a] * r[a];
+            }
+            ++iters;
+            if (delta_next < 1e-20 || (use_rel && delta_next <= tol2)) {
+                delta_new = delta_next;
+                converged = 1;
+                break;
+            }
+            double beta = delta_next / delta_new;
+            OMP_FOR
+            for (size_t a = 0; a < n; ++a) d[a] = r[a] + beta * d[a];
+            delta_new = delta_next;
+        }
+    } else {
+        converged = 1;
+    }
+    s->stat_iters = iters;
+    s->stat_r2 = delta_new;
+    s->stat_converged = converged;
+}
+
+/* fluid.rs:496-529 */
+void of_project(OFluid *s) {
+    size_t nx = s->nx, ny = s->ny, nz = s->nz;
+    of_enforce_bc(s);
+    of_divergence_norm(s);
+    if (s->solver_kind == 0) of_solve_cg(s, s->solver_iters);
+    else of_solve_rbgs(s, s->solver_iters);
+    double inv_h = 1.0 / s->h;
+    const double *p = s->pressure;
+    OMP_FOR
+    for (size_t k = 1; k < nz - 1; ++k)
+        for (size_t j = 1; j < ny - 1; ++j)
+            for (size_t i = 1; i < nx - 1; ++i) {
+                size_t id = IX(s, i, j, k);
+                double pc = p[id];
+                if (i > 1) s->u[id] -= (pc - p[IX(s, i - 1, j, k)]) * inv_h;
+                if (j > 1) s->v[id] -= (pc - p[IX(s, i, j - 1, k)]) * inv_h;
+                if (k > 1) s->w[id] -= (pc - p[IX(s, i, j, k - 1)]) * inv_h;
+            }
+    of_enforce_bc(s);
+}
+
+/* fluid.rs:532-595 */
+static void inject_jets(OFluid *s, double time_sec, double dt) {
+    double cx = 0.5 * (double)s->nx;
+    double cz = 0.5 * (double)s->nz;
+    size_t base_j = 3 < s->ny - 3 ? 3 : s->ny - 3;
+    double pulse = 0.65 + 0.35 * sin(time_sec * TAU_ * 0.55) +
+                   0.25 * fmax(cos(time_sec * TAU_ * 1.35), 0.0);
+    double angle = time_sec * 1.85;
+    double orbit_r = (double)s->nx * 0.14;
+    double jets[2][6] = {
+        {cx + orbit_r * cos(angle), (double)base_j + 1.5, cz + orbit_r * sin(angle),
+         -2.6 * sin(angle), 7.2 * pulse, 2.6 * cos(angle)},
+        {cx - orbit_r * cos(angle), (double)base_j + 2.0, cz - orbit_r * sin(angle),
+         2.6 * sin(angle), 6.4 * pulse, -2.6 * cos(angle)},
+    };
+    double r_cells = fmax((double)s->nx * 0.105, 2.2);
+    double r2 = r_cells * r_cells;
+    for (int q = 0; q < 2; ++q) {
+        double jx = jets[q][0], jy = jets[q][1], jz = jets[q][2];
+        double vx = jets[q][3], vy = jets[q][4], vz = jets[q][5];
+        size_t k_min = (size_t)fmax(floor(jz - r_cells - 1.0), 1.0);
+        size_t k_max = (size_t)fmin(ceil(jz + r_cells + 1.0), (double)(s->nz - 2));
+        size_t j_min = (size_t)fmax(floor(jy - r_cells - 1.0), 1.0);
+        size_t j_max = (size_t)fmin(ceil(jy + r_cells + 1.0), (double)(s->ny - 2));
+        size_t i_min = (size_t)fmax(floor(jx - r_cells - 1.0), 1.0);
+        size_t i_max = (size_t)fmin(ceil(jx + r_cells + 1.0), (double)(s->nx - 2));
+        for (size_t k = k_min; k <= k_max; ++k)
+            for (size_t j = j_min; j <= j_max; ++j)
+                for (size_t i = i_min; i <= i_max; ++i) {
+                    double dx = (double)i - jx, dy = (double)j - jy, dz = (double)k - jz;
+                    double d2 = dx * dx + dy * dy + dz * dz;
+                    if (d2 < r2) {
+                        double w_falloff = exp(-2.2 * d2 / r2);
+                        size_t id = IX(s, i, j, k);
+                        s->density[id] = fmin(s->density[id] + dt * 9.5 * pulse * w_falloff, 3.8);
+                        s->temperature[id] =
+                            fmin(s->temperature[id] + dt * 8.2 * pulse * w_falloff, 4.5);
+                        s->u[id] += dt * vx * w_falloff * 4.2;
+                        s->v[id] += dt * vy * w_falloff * 4.2;
+                        s->w[id] += dt * vz * w_falloff * 4.2;
+                    }
+                }
+    }
+}
+
+/* Extension: a sphere (cell-index coordinates, d^2 < r^2) whose interior
+ * cells [1, n-2] get density and temperature SET to fixed values. */
+static void inject_sphere(OFluid *s) {
+    double r2 = s->src_r * s->src_r;
+    for (size_t k = 1; k < s->nz - 1; ++k)
+        for (size_t j = 1; j < s->ny - 1; ++j)
+            for (size_t i = 1; i < s->nx - 1; ++i) {
+                double dx = (double)i - s->src_cx, dy = (double)j - s->src_cy,
+                       dz = (double)k - s->src_cz;
+                if (dx * dx + dy * dy + dz * dz < r2) {
+                    size_t id = IX(s, i, j, k);
+                    s->density[id] = s->src_rho;
+                    s->temperature[id] = s->src_temp;
+                }
+            }
+}
+
+void of_inject_sources(OFluid *s, double time_sec, double dt) {
+    if (s->source_kind == 0) inject_jets(s, time_sec, dt);
+    else if (s->source_kind == 1) inject_sphere(s);
+}
+
+/* fluid.rs:613-672.  out = {time, KE, enstrophy, peak |w|, mass, com x, y, z} */
+void of_telemetry(const OFluid *s, double time_sec, double *out) {
+    size_t nx = s->nx, ny = s->ny, nz = s->nz;
+    double dv = s->h * s->h * s->h;
+    double inv_2h = 0.5 / s->h;
+    double ke = 0.0, enstrophy = 0.0, peak = 0.0, mass = 0.0;
+    double cx = 0.0, cy = 0.0, cz = 0.0;
+    const double *u = s->u, *v = s->v, *w = s->w;
+    for (size_t k = 1; k < nz - 1; ++k)
+        for (size_t j = 1; j < ny - 1; ++j)
+            for (size_t i = 1; i < nx - 1; ++i) {
+                size_t id = IX(s, i, j, k);
+                double u2 = u[id] * u[id] + v[id] * v[id] + w[id] * w[id];
+                ke += 0.5 * u2 * dv;
+                double dw_dy = (w[IX(s, i, j + 1, k)] - w[IX(s, i, j - 1, k)]) * inv_2h;
+                double dv_dz = (v[IX(s, i, j, k + 1)] - v[IX(s, i, j, k - 1)]) * inv_2h;
+                double du_dz = (u[IX(s, i, j, k + 1)] - u[IX(s, i, j, k - 1)]) * inv_2h;
+                double dw_dx = (w[IX(s, i + 1, j, k)] - w[IX(s, i - 1, j, k)]) * inv_2h;
+                double dv_dx = (v[IX(s, i + 1, j, k)] - v[IX(s, i - 1, j, k)]) * inv_2h;
+                double du_dy = (u[IX(s, i, j + 1, k)] - u[IX(s, i, j - 1, k)]) * inv_2h;
+                double ox = dw_dy - dv_dz, oy = du_dz - dw_dx, oz = dv_dx - du_dy;
+                double w2 = ox * ox + oy * oy + oz * oz;
+                enstrophy += 0.5 * w2 * dv;
+                peak = fmax(peak, sqrt(w2));
+                double rho = s->density[id];
+                if (rho > 1e-8) {
+                    mass += rho * dv;
+                    double nxn = 2.0 * ((double)i / (double)(nx - 1)) - 1.0;
+                    double nyn = 2.0 * ((double)j / (double)(ny - 1)) - 1.0;
+                    double nzn = 2.0 * ((double)k / (double)(nz - 1)) - 1.0;
+                    double m = rho * dv;
+                    cx += nxn * m; /* Vec3 * f64 then += (math.rs:167-210) */
+                    cy += nyn * m;
+                    cz += nzn * m;
+                }
+            }
+    if (mass > 1e-12) { /* Vec3 /= f64 multiplies by 1/rhs (math.rs:226-239) */
+        double inv = 1.0 / mass;
+        cx = cx * inv; cy = cy * inv; cz = cz * inv;
+    }
+    out[0] = time_sec;
+    out[1] = ke;
+    out[2] = enstrophy;
+    out[3] = peak;
+    out[4] = mass;
+    out[5] = rclamp(cx, -1.0, 1.0);
+    out[6] = rclamp(cy, -1.0, 1.0);
+    out[7] = rclamp(cz, -1.0, 1.0);
+}
+
+/* fluid.rs:598-611 (dissipation factors parameterised) */
+void of_step(OFluid *s, double dt, double time_sec, double *telemetry_out) {
+    of_advect(s, 0, dt, s->diss_vel);
+    of_advect(s, 1, dt, s->diss_vel);
+    of_advect(s, 2, dt, s->diss_vel);
+    of_inject_sources(s, time_sec, dt);
+    of_buoyancy_confinement(s, dt);
+    of_project(s);
+    of_advect(s, 3, dt, s->diss_rho);
+    of_advect(s, 4, dt, s->diss_temp);
+    if (telemetry_out) of_telemetry(s, time_sec, telemetry_out);
+}
+
+int of_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+/* ------------------------------------------------------------------ */
+/* seeded RNG: SplitMix64 -> Xoshiro256++ (rng.rs:5-88)                */
+/* ------------------------------------------------------------------ */
+
+typedef struct { uint64_t s[4]; } ORng;
+
+static uint64_t splitmix64(uint64_t *state) { /* rng.rs:5-11 */
+    *state += 0x9E3779B97F4A7C15ull;
+    uint64_t z = *state;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+static inline uint64_t rotl64(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+
+void orng_seed(ORng *r, uint64_t seed) { /* rng.rs:21-33 */
+    uint64_t sm = seed ^ 0xA5A55A5AF0F00F0Full;
+    for (int a = 0; a < 4; ++a) r->s[a] = splitmix64(&sm);
+    if (!r->s[0] && !r->s[1] && !r->s[2] && !r->s[3]) r->s[0] = 1;
+}
+
+void orng_set_state(ORng *r, const uint64_t *st) { memcpy(r->s, st, sizeof(r->s)); }
+
+uint64_t orng_next_u64(ORng *r) { /* rng.rs:48-64 */
+    uint64_t result = rotl64(r->s[0] + r->s[3], 23) + r->s[0];
+    uint64_t t = r->s[1] << 17;
+    r->s[2] ^= r->s[0];
+    r->s[3] ^= r->s[1];
+    r->s[1] ^= r->s[2];
+    r->s[0] ^= r->s[3];
+    r->s[2] ^= t;
+    r->s[3] = rotl64(r->s[3], 45);
+    return result;
+}
+
+double orng_next_f64(ORng *r) { /* rng.rs:74-77 */
+    return (double)(orng_next_u64(r) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+double orng_range_f64(ORng *r, double lo, double hi) { /* rng.rs:86-88 */
+    return lo + (hi - lo) * orng_next_f64(r);
+}
+
+uint64_t of_splitmix64_next(uint64_t *state) { return splitmix64(state); }

@@ -1,0 +1,224 @@
+"""ctypes wrapper of the CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+The oracle is the f64 C restatement of the reference `FluidSolver3D`
+(src/fluid.rs:23-673) in ``oracle/fluid_oracle.c``.  Only ``tests/``,
+``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` may
+import this module; the product package never does.
+
+``OracleFluid`` mirrors the reference's public surface: the constructor, the
+parameter fields, ``step``, ``project``, ``compute_divergence_norm``,
+``inject_sources``, ``apply_buoyancy_and_vorticity_confinement``,
+``enforce_wall_velocity_bc``, ``advect_maccormack_in_place``,
+``compute_telemetry`` and ``sample_trilinear``; fields are exposed as numpy
+views shaped (nz, ny, nx) over the oracle's own buffers.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIELDS = {"u": 0, "v": 1, "w": 2, "density": 3, "temperature": 4, "pressure": 5,
+          "divergence": 6, "curl_mag": 7, "cg_r": 8}
+
+_libs: dict = {}
+
+
+def build() -> None:
+    """Compile both oracle builds with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def _load(omp: bool = False) -> C.CDLL:
+    name = "liboracle_fluid_omp.so" if omp else "liboracle_fluid.so"
+    if name in _libs:
+        return _libs[name]
+    path = os.path.join(HERE, name)
+    if not os.path.exists(path):
+        build()
+    lib = C.CDLL(path)
+    vp, d, l, i = C.c_void_p, C.c_double, C.c_long, C.c_int
+    pd = C.POINTER(C.c_double)
+    sig = {
+        "of_create": (vp, [l, l, l, d]),
+        "of_destroy": (None, [vp]),
+        "of_set_params": (None, [vp, d, d, d, i, l, d, d, d, d, d]),
+        "of_set_source": (None, [vp, i, d, d, d, d, d, d]),
+        "of_field": (pd, [vp, i]),
+        "of_h": (d, [vp]),
+        "of_stats": (None, [vp, C.POINTER(l), pd, pd, pd, C.POINTER(i)]),
+        "of_sample_trilinear": (d, [vp, i, d, d, d]),
+        "of_advect": (None, [vp, i, d, d]),
+        "of_enforce_bc": (None, [vp]),
+        "of_buoyancy_confinement": (None, [vp, d]),
+        "of_divergence_norm": (d, [vp]),
+        "of_divergence_max": (d, [vp]),
+        "of_apply_laplacian": (None, [vp, pd, pd]),
+        "of_solve_rbgs": (None, [vp, l]),
+        "of_solve_cg": (None, [vp, l]),
+        "of_project": (None, [vp]),
+        "of_inject_sources": (None, [vp, d, d]),
+        "of_telemetry": (None, [vp, d, pd]),
+        "of_step": (None, [vp, d, d, pd]),
+        "of_num_threads": (i, []),
+        "orng_seed": (None, [vp, C.c_uint64]),
+        "orng_set_state": (None, [vp, C.POINTER(C.c_uint64)]),
+        "orng_next_u64": (C.c_uint64, [vp]),
+        "orng_next_f64": (d, [vp]),
+        "orng_range_f64": (d, [vp, d, d]),
+        "of_splitmix64_next": (C.c_uint64, [C.POINTER(C.c_uint64)]),
+    }
+    for fn, (res, args) in sig.items():
+        f = getattr(lib, fn)
+        f.restype = res
+        f.argtypes = args
+    _libs[name] = lib
+    return lib
+
+
+def _ptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class OracleRng:
+    """SplitMix64-seeded Xoshiro256++ (src/rng.rs:5-88)."""
+
+    def __init__(self, seed: int | None = None, state=None):
+        self._lib = _load()
+        self._buf = C.create_string_buffer(32)
+        if state is not None:
+            arr = (C.c_uint64 * 4)(*state)
+            self._lib.orng_set_state(self._buf, arr)
+        else:
+            self._lib.orng_seed(self._buf, C.c_uint64(seed))
+
+    def next_u64(self) -> int:
+        return self._lib.orng_next_u64(self._buf)
+
+    def next_f64(self) -> float:
+        return self._lib.orng_next_f64(self._buf)
+
+    def range_f64(self, lo: float, hi: float) -> float:
+        return self._lib.orng_range_f64(self._buf, lo, hi)
+
+
+def splitmix64_stream(seed: int, n: int) -> list:
+    lib = _load()
+    st = C.c_uint64(seed)
+    return [lib.of_splitmix64_next(C.byref(st)) for _ in range(n)]
+
+
+class OracleFluid:
+    """The f64 oracle solver; constructor and fields follow src/fluid.rs:23-85."""
+
+    def __init__(self, nx: int, ny: int, nz: int, h: float = 0.0, omp: bool = False):
+        self._lib = _load(omp)
+        self._h = self._lib.of_create(nx, ny, nz, h)
+        if not self._h:
+            raise ValueError("grid dimensions must be >= 6 (src/fluid.rs:54)")
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.n = nx * ny * nz
+        self.h = self._lib.of_h(self._h)
+        self.vorticity_eps = 5.5
+        self.buoyancy_beta = 6.8
+        self.gravity_alpha = 0.45
+        self.solver_kind = 0
+        self.solver_iters = 32
+        self.sor_omega = 1.65
+        self.cg_rel_tol = 0.0
+        self.dissipation = (0.996, 0.996, 0.985)
+        self._push()
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._lib.of_destroy(self._h)
+            self._h = None
+
+    def _push(self):
+        dv, dr, dt = self.dissipation
+        self._lib.of_set_params(self._h, self.vorticity_eps, self.buoyancy_beta,
+                                self.gravity_alpha, int(self.solver_kind),
+                                int(self.solver_iters), self.sor_omega, self.cg_rel_tol,
+                                dv, dr, dt)
+
+    def set_source(self, kind: int, center=(0.0, 0.0, 0.0), radius=0.0, rho=1.0, temp=1.0):
+        self._lib.of_set_source(self._h, kind, float(center[0]), float(center[1]),
+                                float(center[2]), float(radius), float(rho), float(temp))
+
+    def field(self, name: str) -> np.ndarray:
+        p = self._lib.of_field(self._h, FIELDS[name])
+        return np.ctypeslib.as_array(p, shape=(self.nz, self.ny, self.nx))
+
+    def __getattr__(self, name):
+        if name in FIELDS:
+            return self.field(name)
+        raise AttributeError(name)
+
+    # --- reference methods -------------------------------------------------
+    def step(self, dt: float, t: float) -> np.ndarray:
+        self._push()
+        out = np.zeros(8)
+        self._lib.of_step(self._h, dt, t, _ptr(out))
+        return out
+
+    def advect_maccormack_in_place(self, kind: int, dt: float, dissipation: float):
+        self._lib.of_advect(self._h, kind, dt, dissipation)
+
+    def enforce_wall_velocity_bc(self):
+        self._lib.of_enforce_bc(self._h)
+
+    def apply_buoyancy_and_vorticity_confinement(self, dt: float):
+        self._push()
+        self._lib.of_buoyancy_confinement(self._h, dt)
+
+    def compute_divergence_norm(self) -> float:
+        return self._lib.of_divergence_norm(self._h)
+
+    def divergence_max(self) -> float:
+        return self._lib.of_divergence_max(self._h)
+
+    def project(self):
+        self._push()
+        self._lib.of_project(self._h)
+
+    def solve_poisson_conjugate_gradient(self, iters: int):
+        self._push()
+        self._lib.of_solve_cg(self._h, iters)
+
+    def solve_poisson_gauss_seidel(self, iters: int):
+        self._push()
+        self._lib.of_solve_rbgs(self._h, iters)
+
+    def inject_sources(self, t: float, dt: float):
+        self._lib.of_inject_sources(self._h, t, dt)
+
+    def compute_telemetry(self, t: float) -> np.ndarray:
+        out = np.zeros(8)
+        self._lib.of_telemetry(self._h, t, _ptr(out))
+        return out
+
+    def sample_trilinear(self, field: str, x: float, y: float, z: float) -> float:
+        return self._lib.of_sample_trilinear(self._h, FIELDS[field], x, y, z)
+
+    def apply_laplacian(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(self.nz, self.ny, self.nx)
+        out = np.zeros_like(x)
+        self._lib.of_apply_laplacian(self._h, _ptr(x), _ptr(out))
+        return out
+
+    def stats(self) -> dict:
+        it = C.c_long()
+        b2, r0, r2 = C.c_double(), C.c_double(), C.c_double()
+        conv = C.c_int()
+        self._lib.of_stats(self._h, C.byref(it), C.byref(b2), C.byref(r0), C.byref(r2),
+                           C.byref(conv))
+        return {"iters": it.value, "b2": b2.value, "r2_init": r0.value, "r2": r2.value,
+                "converged": bool(conv.value)}
+
+    @staticmethod
+    def num_threads(omp: bool = False) -> int:
+        return _load(omp).of_num_threads()
