@@ -245,6 +245,15 @@ class FluidSolver3D:
                 "r2_init": s.r_norm2_initial, "r2": s.r_norm2_final,
                 "outer": s.outer_iterations, "div_max": s.div_max_final}
 
+    def run_iters(self) -> list:
+        """Per-step CG iterations (RBGS: sweeps) of the last run/step."""
+        cnt = C.c_int()
+        L.check(self._lib.aether_fluid_last_run_iterations(self._h, None, 0, C.byref(cnt)))
+        buf = (C.c_int32 * max(cnt.value, 1))()
+        L.check(self._lib.aether_fluid_last_run_iterations(self._h, buf, cnt.value,
+                                                           C.byref(cnt)))
+        return [buf[a] for a in range(cnt.value)]
+
     def set_kernel_timing(self, enable: bool) -> None:
         L.check(self._lib.aether_fluid_set_kernel_timing(self._h, int(enable)))
 

@@ -108,6 +108,12 @@ void check_params(const aether_fluid_params &p) {
     if (p.source_kind < AETHER_SOURCE_JETS || p.source_kind > AETHER_SOURCE_NONE)
         raise(AETHER_EINVAL, "unknown source_kind");
     if (!(p.cg_rel_tol >= 0.0)) raise(AETHER_EINVAL, "cg_rel_tol must be >= 0");
+    if (p.source_kind == AETHER_SOURCE_SPHERE &&
+        !(std::isfinite(p.source_center[0]) && std::isfinite(p.source_center[1]) &&
+          std::isfinite(p.source_center[2]) && std::isfinite(p.source_radius) &&
+          std::fabs(p.source_center[0]) < 1e9 && std::fabs(p.source_center[1]) < 1e9 &&
+          std::fabs(p.source_center[2]) < 1e9 && p.source_radius < 1e9))
+        raise(AETHER_EINVAL, "sphere source centre/radius must be finite");
 }
 
 // Kernel timing accumulator (HIP events on the solver's stream).
@@ -160,6 +166,7 @@ class SolverBase {
     virtual void sample(int field, const double *xyz, size_t npts, double *out) = 0;
     virtual double max_density() = 0;
     virtual void stats(aether_fluid_solve_stats *out) = 0;
+    virtual const std::vector<int> &last_run_iters() const = 0;
     aether_fluid_params params = default_params();
     KernelTimer timer;
 };
@@ -532,6 +539,8 @@ class Solver final : public SolverBase {
         sync();
         if (out)
             for (int s = 0; s < nsteps; ++s) unpack(raw.data() + (size_t)s * 8, out + s);
+        if (params.solver_kind == AETHER_SOLVER_RBGS)
+            std::fill(iters.begin(), iters.end(), params.solver_iters);
         last_run_iters_ = iters;
         if (timer.enabled) {
             // count only launches that did work: iterations < iters of that step
@@ -631,7 +640,7 @@ class Solver final : public SolverBase {
         out->r_norm2_final = cs.r2_final;
     }
 
-    const std::vector<int> &last_run_iters() const { return last_run_iters_; }
+    const std::vector<int> &last_run_iters() const override { return last_run_iters_; }
 
    private:
     Dims g_;
@@ -870,6 +879,15 @@ int aether_fluid_last_solve_stats(aether_fluid_t s, aether_fluid_solve_stats *ou
         checked(s);
         if (!out) raise(AETHER_EINVAL, "null output pointer");
         s->impl->stats(out);
+    });
+}
+
+int aether_fluid_last_run_iterations(aether_fluid_t s, int32_t *out, int n, int *count) {
+    return guarded([&] {
+        checked(s);
+        const auto &v = s->impl->last_run_iters();
+        if (count) *count = (int)v.size();
+        for (int a = 0; a < n && a < (int)v.size(); ++a) out[a] = v[a];
     });
 }
 

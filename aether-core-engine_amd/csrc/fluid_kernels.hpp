@@ -132,6 +132,8 @@ __device__ __forceinline__ void axis_loc(int i, double sv, int n, int &i0, doubl
     double f = floor(xc);
     i0 = (int)f;
     t = xc - f;
+    // a NaN velocity passes both clamps; keep the gather inside the grid
+    i0 = min(max(i0, 0), n - 2);
 }
 
 __device__ __forceinline__ void axis_loc(int i, float sv, int n, int &i0, float &t, bool &nw) {
@@ -151,6 +153,7 @@ __device__ __forceinline__ void axis_loc(int i, float sv, int n, int &i0, float 
         i0 = i + (int)f;
         t = o - f;
     }
+    i0 = min(max(i0, 0), n - 2);  // NaN guard, as above
 }
 
 template <typename T>
@@ -950,7 +953,7 @@ __global__ __launch_bounds__(kThreads) void k_sample_points(Dims g, const T *__r
         if (x < 0.5) x = 0.5;
         if (x > (double)n[d] - 1.500001) x = (double)n[d] - 1.500001;
         const double fl = floor(x);
-        i0[d] = (int)fl;
+        i0[d] = min(max((int)fl, 0), n[d] - 2);
         c[d] = x - fl;
     }
     const int b = (i0[2] * g.ny + i0[1]) * g.nx + i0[0];

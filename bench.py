@@ -1,0 +1,190 @@
+"""Benchmark: FP32 smoke step at 128^3 on one MI355X.
+
+Workload: 128^3 MAC grid, h = 1/128, dt = 0.01, seeded strongly divergent
+initial velocity (u, v, w i.i.d. uniform[-1, 1]) and density block [32, 96)^3,
+sphere source r = 12 at (64, 10, 64), Boussinesq alpha = 0.05, beta = 1,
+confinement eps = 0.3, MacCormack with the 8-cell clamp, CG projection to
+relative residual 1e-5 (max 200 iterations).  A "step" is one full
+FluidSolver3D::step (src/fluid.rs:598-611) of that grid.
+
+Prints one JSON line: voxel-steps/s (value), the HBM roofline of the
+dominant kernel (CG update) measured with HIP events on the solver's stream,
+and the CPU oracle timed on this host's cores (cpu_baseline).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (ROOT, os.path.join(ROOT, "aether-core-engine_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+import numpy as np  # noqa: E402
+
+METRIC = ("smoke voxel-steps/s at 128³ (fwd, fwd+adjoint), % HBM roofline, vs host CPU")
+HBM_PEAK_GBPS = 8000.0
+WORKLOAD = "plain_128"
+
+
+def algorithmic_bytes_per_step(nvox: int, cg_iters: int) -> float:
+    """FP32 array passes of one step with k CG iterations: 4 B * N * (59 + 8k)."""
+    return 4.0 * nvox * (59 + 8 * cg_iters)
+
+
+def make_solver(solver_kind: str):
+    import aether_fluid as af
+    from aether_fluid.scenarios import SMOKE_CONFIGS, configure, seeded_config_state
+    cfg = SMOKE_CONFIGS[WORKLOAD]
+    s = af.FluidSolver3D(*cfg["dims"], h=cfg["h"], precision="fp32")
+    configure(s, WORKLOAD, solver_kind)
+    for name, arr in seeded_config_state(WORKLOAD).items():
+        s.upload(name, arr)
+    return cfg, s
+
+
+def time_gpu(solver_kind: str, steps: int, warmup: int):
+    cfg, s = make_solver(solver_kind)
+    dt = cfg["dt"]
+    t_axis = [dt * i for i in range(warmup + steps)]
+    if warmup:
+        s.run(dt, t_axis[:warmup])
+    t0 = time.perf_counter()
+    frames = s.run(dt, t_axis[warmup:])  # returns after the stream has drained
+    elapsed = time.perf_counter() - t0
+    return cfg, s, elapsed, frames, s.run_iters()
+
+
+def kernel_roofline(solver_kind: str, nvox: int, steps: int = 2):
+    """Average device time of the two CG kernels over `steps` timed steps
+    (HIP events around each launch on the solver's stream)."""
+    cfg, s = make_solver(solver_kind)
+    s.set_kernel_timing(True)
+    s.run(cfg["dt"], [cfg["dt"] * i for i in range(steps)])
+    dir_ms, dir_n = s.kernel_timing(0)
+    upd_ms, upd_n = s.kernel_timing(1)
+    s.set_kernel_timing(False)
+    # algorithmic bytes per launch: cg_dir reads r, d_old, writes d (3 FP32
+    # passes); cg_update reads p, d, r and writes p, r (5 FP32 passes)
+    kern = {
+        "cg_dir": {"avg_ms": dir_ms, "launches": dir_n, "bytes": 3 * 4.0 * nvox},
+        "cg_update": {"avg_ms": upd_ms, "launches": upd_n, "bytes": 5 * 4.0 * nvox},
+    }
+    for k in kern.values():
+        k["total_ms"] = k["avg_ms"] * k["launches"]
+        k["gbps"] = k["bytes"] / (k["avg_ms"] * 1e-3) / 1e9 if k["avg_ms"] > 0 else 0.0
+    dom = max(kern, key=lambda k: kern[k]["total_ms"])
+    return dom, kern
+
+
+def load_traffic(kernel: str):
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            data = json.load(f)
+        return data.get(kernel, {}).get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(seconds_budget: float = 20.0):
+    """The f64 oracle (C restatement of src/fluid.rs) on this host: the
+    OpenMP build on every core, and the single-thread build, on the first
+    steps of the same workload."""
+    from aether_fluid.scenarios import SMOKE_CONFIGS, seeded_config_state
+    from oracle.oracle import OracleFluid
+    cfg = SMOKE_CONFIGS[WORKLOAD]
+    state = seeded_config_state(WORKLOAD)
+    nvox = int(np.prod(cfg["dims"]))
+    out = {}
+    for omp in (True, False):
+        o = OracleFluid(*cfg["dims"], h=cfg["h"], omp=omp)
+        o.vorticity_eps, o.buoyancy_beta, o.gravity_alpha = cfg["eps"], cfg["beta"], cfg["alpha"]
+        o.solver_kind, o.solver_iters, o.cg_rel_tol = 0, cfg["cg_max"], cfg["cg_tol"]
+        o.set_source(1, cfg["source"]["center"], cfg["source"]["radius"], 1.0, 1.0)
+        for name, arr in state.items():
+            o.field(name)[...] = arr.reshape(o.nz, o.ny, o.nx)
+        done, iters, t0 = 0, [], time.perf_counter()
+        while True:
+            o.step(cfg["dt"], cfg["dt"] * done)
+            iters.append(o.stats()["iters"])
+            done += 1
+            el = time.perf_counter() - t0
+            if el > seconds_budget / 2 or done >= 5:
+                break
+        out["omp" if omp else "single"] = {
+            "value": nvox * done / el, "steps": done, "seconds": el,
+            "threads": OracleFluid.num_threads(omp), "cg_iters": iters}
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only; the smoke step is not "
+              "sharded across GPUs (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    cfg, s, elapsed, frames, run_iters = time_gpu("cg", args.steps, args.warmup)
+    nvox = int(np.prod(cfg["dims"]))
+    stats = s.last_solve_stats()
+    value = nvox * args.steps / elapsed
+    ms = elapsed * 1e3 / args.steps
+
+    # RBGS variant of the same workload (omega 1.8, 60 sweeps)
+    _, _, el_rb, _, _ = time_gpu("rbgs", args.steps, args.warmup)
+
+    dom, kern = kernel_roofline("cg", nvox)
+    kd = kern[dom]
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": kd["gbps"], "peak": HBM_PEAK_GBPS,
+                "unit": "GB/s", "frac": kd["gbps"] / HBM_PEAK_GBPS,
+                "traffic": load_traffic(dom), "avg_launch_ms": kd["avg_ms"],
+                "algorithmic_bytes_per_launch": kd["bytes"], "launches_timed": kd["launches"],
+                "other_kernels": {k: {"avg_ms": v["avg_ms"], "gbps": v["gbps"]}
+                                  for k, v in kern.items() if k != dom}}
+
+    line = {
+        "metric": METRIC, "value": value, "unit": "voxel-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic (seeded Xoshiro256++ initial state)",
+        "config": {"workload": "plain smoke 128^3 FP32, CG rel tol 1e-5 (max 200)",
+                   "grid": list(cfg["dims"]), "h": cfg["h"], "dt": cfg["dt"],
+                   "solver": "cg", "parallelism": "single GPU"},
+        "cg_iters_last_step": stats["iters"],
+        "rbgs": {"value": nvox * args.steps / el_rb, "ms_per_step": el_rb * 1e3 / args.steps,
+                 "sweeps": cfg["rbgs_sweeps"], "omega": cfg["sor_omega"]},
+        "roofline": roofline,
+        "fwd_adjoint": None,
+    }
+    if run_iters:
+        line["cg_iters_mean"] = float(np.mean(run_iters))
+        line["step_algorithmic_gbps"] = (sum(algorithmic_bytes_per_step(nvox, k)
+                                             for k in run_iters) / elapsed / 1e9)
+    if not args.no_cpu:
+        cb = cpu_baseline(args.cpu_seconds)
+        omp = cb["omp"]
+        line["cpu_baseline"] = {
+            "value": omp["value"], "unit": "voxel-steps/s", "cores": omp["threads"],
+            "kind": "port",
+            "sample": (f"f64 C oracle (OpenMP) steps 1-{omp['steps']} of the same 128^3 "
+                       f"workload, CG iters {omp['cg_iters']}"),
+            "single_thread": {"value": cb["single"]["value"], "cores": 1,
+                              "steps": cb["single"]["steps"]},
+        }
+        line["gpu_over_cpu"] = value / omp["value"]
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

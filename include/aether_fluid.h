@@ -153,6 +153,9 @@ int aether_fluid_sample_trilinear(aether_fluid_t s, int field, const double *xyz
 /* Extension: max |div u| over interior cells (same stencil as fluid.rs:341-344). */
 int aether_fluid_divergence_max(aether_fluid_t s, double *out);
 int aether_fluid_last_solve_stats(aether_fluid_t s, aether_fluid_solve_stats *out);
+/* CG iterations of each step of the last aether_fluid_run / aether_fluid_step
+ * (RBGS: sweeps); copies min(n, steps) values, *count = steps. */
+int aether_fluid_last_run_iterations(aether_fluid_t s, int32_t *out, int n, int *count);
 
 /* Field access (the reference's pub Vec<f64> arrays, src/fluid.rs:28-34).
  * n must equal nx*ny*nz. */

@@ -1,0 +1,76 @@
+"""CPU tier: the C-ABI library loads, exports every symbol that
+include/aether_fluid.h declares, and rejects bad arguments before touching a
+device.  No compute calls (this container has no GPU)."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "aether_fluid.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*(aether_\w+)\s*\(", text, re.M)))
+
+
+def test_header_symbols_exported_and_bound():
+    import aether_fluid as af
+    lib = af.lib()
+    names = declared_symbols()
+    assert len(names) >= 30
+    out = subprocess.run(["nm", "-D", "--defined-only", af.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = set(re.findall(r"\bT (aether_\w+)", out))
+    for n in names:
+        assert n in exported, n
+        assert n in af.SIGNATURES, n
+        getattr(lib, n)
+    assert set(af.SIGNATURES) == set(names)
+
+
+def test_abi_version():
+    import aether_fluid as af
+    assert af.lib().aether_fluid_abi_version() == 1
+
+
+def test_bad_dimensions_rejected_before_device_use():
+    import aether_fluid as af
+    h = C.c_void_p()
+    rc = af.lib().aether_fluid_create(5, 10, 10, 0.0, 0, C.byref(h))
+    assert rc == -1 and not h.value
+    assert b"must be >= 6" in af.lib().aether_fluid_last_error()
+    with pytest.raises(af.AetherFluidError) as e:
+        af.FluidSolver3D(6, 4, 6)
+    assert e.value.code == -1
+
+
+def test_null_handle_rejected():
+    import aether_fluid as af
+    assert af.lib().aether_fluid_project(None) == -1
+    assert af.lib().aether_fluid_destroy(None) == 0
+
+
+def test_default_params_are_reference_values():
+    import aether_fluid as af
+    p = af._lib.Params()
+    assert af.lib().aether_fluid_default_params(C.byref(p)) == 0
+    # src/fluid.rs:79-83, 398, 599-608
+    assert (p.vorticity_eps, p.buoyancy_beta, p.gravity_alpha) == (5.5, 6.8, 0.45)
+    assert (p.solver_kind, p.solver_iters, p.sor_omega) == (0, 32, 1.65)
+    assert (p.dissipation_velocity, p.dissipation_density,
+            p.dissipation_temperature) == (0.996, 0.996, 0.985)
+    assert p.cg_rel_tol == 0.0 and p.source_kind == 0
+
+
+def test_no_device_here_reports_enodev():
+    import aether_fluid as af
+    from conftest import gpu_available
+    if gpu_available():
+        pytest.skip("a device is present")
+    with pytest.raises(af.AetherFluidError) as e:
+        af.FluidSolver3D(8, 8, 8)
+    assert e.value.code == -4

@@ -1,0 +1,268 @@
+"""GPU parity: the HIP library (through the C ABI) against the f64 oracle.
+
+Bars:
+* FP64 instantiation, stages without reductions (advection, wall BC,
+  confinement/buoyancy, gradient, RBGS): bit-exact against the oracle.
+* FP64 stages with reductions or device libm (CG, jets' exp, telemetry):
+  relative error <= 1e-11 (fixed-order tree sums vs sequential sums).
+* FP32 instantiation: per-field relative L2 <= 1e-5 after one step of the
+  plain-smoke workloads, CG iteration count within +-1 of the oracle's.
+"""
+import numpy as np
+import pytest
+
+from pairs import compare_fields, load_state, make_pair, random_state, rel_l2, set_knobs
+
+pytestmark = pytest.mark.gpu
+
+FP64_TIGHT = 1e-11
+
+
+def _announce(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2, 3, 4])
+def test_fp64_advection_bitexact(kind):
+    _announce(f"fp64 advect kind={kind} 18x22x18")
+    st = random_state(18, 22, 18, seed=10 + kind, vel=1.5)
+    gpu, orc = make_pair(18, 22, 18, state=st)
+    diss = 0.985 if kind == 4 else 0.996
+    gpu.advect_maccormack_in_place(kind, 0.06, diss)
+    orc.advect_maccormack_in_place(kind, 0.06, diss)
+    name = ["u", "v", "w", "density", "temperature"][kind]
+    np.testing.assert_array_equal(gpu.download(name), orc.field(name).ravel())
+
+
+def test_fp64_wall_bc_and_confinement_bitexact():
+    _announce("fp64 bc+confinement 16x20x16")
+    st = random_state(16, 20, 16, seed=3)
+    gpu, orc = make_pair(16, 20, 16, state=st)
+    set_knobs(gpu, orc)
+    gpu.enforce_wall_velocity_bc()
+    orc.enforce_wall_velocity_bc()
+    for n in ("u", "v", "w"):
+        np.testing.assert_array_equal(gpu.download(n), orc.field(n).ravel())
+    gpu.apply_buoyancy_and_vorticity_confinement(0.08)
+    orc.apply_buoyancy_and_vorticity_confinement(0.08)
+    for n in ("u", "v", "w"):
+        np.testing.assert_array_equal(gpu.download(n), orc.field(n).ravel())
+
+
+def test_fp64_jets_and_divergence():
+    _announce("fp64 jets+divergence 16x20x16")
+    st = random_state(16, 20, 16, seed=4)
+    gpu, orc = make_pair(16, 20, 16, state=st)
+    set_knobs(gpu, orc)
+    gpu.inject_sources(0.5, 0.1)
+    orc.inject_sources(0.5, 0.1)
+    for n in ("u", "v", "w", "density", "temperature"):
+        assert rel_l2(gpu.download(n), orc.field(n)) <= 1e-15
+    g = gpu.compute_divergence_norm()
+    o = orc.compute_divergence_norm()
+    assert abs(g - o) <= FP64_TIGHT * o
+    np.testing.assert_array_equal(gpu.download("divergence"), orc.field("divergence").ravel())
+
+
+def test_known_answer_readme_fp64():
+    """README.md:54 known answer of the verify.rs:229-250 scenario."""
+    _announce("known answer 16x20x16 fp64")
+    gpu, orc = make_pair(16, 20, 16)
+    set_knobs(gpu, orc, kind=0, iters=40)
+    gpu.inject_cyberpunk_sources(0.5, 0.1)
+    gpu.apply_buoyancy_and_vorticity_confinement(0.1)
+    pre = gpu.compute_divergence_norm()
+    gpu.project()
+    post = gpu.compute_divergence_norm()
+    assert f"{pre:.4e}" == "2.5751e+00"
+    assert f"{post:.4e}" == "6.7289e-03"
+    assert gpu.last_solve_stats()["iters"] == 40
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_projection_cg_and_rbgs_ratio(precision):
+    """src/fluid.rs:918-956 on the GPU."""
+    _announce(f"fluid.rs burst test {precision}")
+    import aether_fluid as af
+    for kind, iters in ((0, 45), (1, 90)):
+        s = af.FluidSolver3D(18, 22, 18, precision=precision)
+        s.solver_kind, s.solver_iters = kind, iters
+        n = 18 * 22 * 18
+        u, v, w = np.zeros(n), np.zeros(n), np.zeros(n)
+        for k in range(6, 12):
+            for j in range(5, 11):
+                for i in range(6, 12):
+                    idx = s.idx(i, j, k)
+                    u[idx] = (i - 8.5) * 3.0
+                    v[idx] = (j - 7.5) * 4.5
+                    w[idx] = (k - 8.5) * 2.5
+        s.u, s.v, s.w = u, v, w
+        s.enforce_wall_velocity_bc()
+        before = s.compute_divergence_norm()
+        assert before > 1.0
+        s.project()
+        after = s.compute_divergence_norm()
+        assert after < before * 0.02, (kind, before, after)
+
+
+def test_fp64_cg_projection_parity():
+    _announce("fp64 CG projection 18x22x18")
+    st = random_state(18, 22, 18, seed=5)
+    gpu, orc = make_pair(18, 22, 18, state=st)
+    set_knobs(gpu, orc, kind=0, iters=60)
+    gpu.project()
+    orc.project()
+    errs = compare_fields(gpu, orc, ("u", "v", "w", "pressure", "divergence"))
+    assert max(errs.values()) <= FP64_TIGHT, errs
+    gs, os_ = gpu.last_solve_stats(), orc.stats()
+    assert gs["iters"] == os_["iters"]
+    assert abs(gs["r2"] - os_["r2"]) <= 1e-8 * os_["r2"]
+
+
+def test_fp64_cg_rel_tol_iteration_count():
+    _announce("fp64 CG rel tol 24x24x24")
+    st = random_state(24, 24, 24, seed=6)
+    gpu, orc = make_pair(24, 24, 24, state=st)
+    set_knobs(gpu, orc, kind=0, iters=200, rel_tol=1e-5)
+    gpu.project()
+    orc.project()
+    gs, os_ = gpu.last_solve_stats(), orc.stats()
+    assert os_["converged"] and gs["converged"]
+    assert abs(gs["iters"] - os_["iters"]) <= 1
+    errs = compare_fields(gpu, orc, ("u", "v", "w"))
+    assert max(errs.values()) <= 1e-9, errs
+
+
+def test_cg_zero_rhs_does_nothing():
+    _announce("CG zero rhs")
+    import aether_fluid as af
+    s = af.FluidSolver3D(12, 12, 12, precision="fp64")
+    s.solver_iters = 50
+    s.project()
+    st = s.last_solve_stats()
+    assert st["iters"] == 0 and st["converged"]
+
+
+def test_fp64_rbgs_bitexact():
+    _announce("fp64 RBGS 18x22x18")
+    st = random_state(18, 22, 18, seed=7)
+    gpu, orc = make_pair(18, 22, 18, state=st)
+    set_knobs(gpu, orc, kind=1, iters=30, omega=1.7)
+    gpu.project()
+    orc.project()
+    for n in ("u", "v", "w", "pressure", "divergence"):
+        np.testing.assert_array_equal(gpu.download(n), orc.field(n).ravel())
+
+
+def test_fp64_full_step_reference_defaults():
+    """Two reference-default steps (jets, CG 32) as in src/fluid.rs:959-968."""
+    _announce("fp64 full step 16x20x16 x2")
+    gpu, orc = make_pair(16, 20, 16)
+    set_knobs(gpu, orc)
+    for t in (0.2, 0.4):
+        tg = gpu.step(0.08, t).as_array()
+        to = orc.step(0.08, t)
+        np.testing.assert_allclose(tg, to, rtol=1e-10, atol=1e-300)
+    errs = compare_fields(gpu, orc)
+    assert max(errs.values()) <= 1e-10, errs
+    d = gpu.download("density")
+    assert np.all(np.isfinite(d)) and np.all(d >= 0)
+
+
+def test_fp64_main_rs_stage_20_steps():
+    """The src/main.rs:82-94 loop shape (eps 6.5, 28 iters, dt 0.08) at
+    20x28x20 for 20 steps (tests/engine_integration.rs:18-22 grid)."""
+    _announce("fp64 20 steps 20x28x20")
+    gpu, orc = make_pair(20, 28, 20)
+    set_knobs(gpu, orc, eps=6.5, iters=28)
+    times = [15.0 * (i / 19.0) for i in range(20)]
+    tg = np.array([f.as_array() for f in gpu.run(0.08, times)])
+    to = np.array([orc.step(0.08, t) for t in times])
+    np.testing.assert_allclose(tg, to, rtol=1e-8, atol=1e-14)
+    errs = compare_fields(gpu, orc)
+    assert max(errs.values()) <= 1e-8, errs
+
+
+def _plain_pair(name, dims, precision):
+    from aether_fluid.scenarios import SMOKE_CONFIGS, seeded_config_state
+    cfg = SMOKE_CONFIGS[name]
+    state = seeded_config_state(name, dims=dims if dims != cfg["dims"] else None)
+    gpu, orc = make_pair(*dims, h=cfg["h"] * cfg["dims"][0] / dims[0], precision=precision,
+                         state=state)
+    return cfg, gpu, orc
+
+
+@pytest.mark.parametrize("solver", ["cg", "rbgs"])
+def test_fp32_plain_smoke_one_step(solver):
+    """FP32 device step vs the f64 oracle on the seeded 128^3 recipe at 48^3."""
+    _announce(f"fp32 plain smoke 48^3 {solver}")
+    cfg, gpu, orc = _plain_pair("plain_128", (48, 48, 48), "fp32")
+    src = dict(center=(24.0, 4.0, 24.0), radius=4.5)
+    if solver == "cg":
+        set_knobs(gpu, orc, eps=0.3, beta=1.0, alpha=0.05, kind=0, iters=200, rel_tol=1e-5,
+                  source=src)
+    else:
+        set_knobs(gpu, orc, eps=0.3, beta=1.0, alpha=0.05, kind=1, iters=60, omega=1.8,
+                  source=src)
+    gpu.step(0.01, 0.0)
+    orc.step(0.01, 0.0)
+    errs = compare_fields(gpu, orc, ("u", "v", "w", "density", "temperature"))
+    print("rel L2", errs)
+    assert max(errs.values()) <= 1e-5, errs
+    if solver == "cg":
+        assert abs(gpu.last_solve_stats()["iters"] - orc.stats()["iters"]) <= 1
+
+
+def test_fp32_cfg0_few_steps():
+    """32x48x32, h = 1/32, zero start, sphere source: 5 FP32 steps."""
+    _announce("fp32 32x48x32 5 steps")
+    from aether_fluid.scenarios import SMOKE_CONFIGS
+    cfg = SMOKE_CONFIGS["plain_32x48x32"]
+    gpu, orc = make_pair(*cfg["dims"], h=cfg["h"], precision="fp32")
+    set_knobs(gpu, orc, eps=0.3, beta=1.0, alpha=0.05, kind=0, iters=200, rel_tol=1e-5,
+              source=cfg["source"])
+    for s in range(5):
+        gpu.step(cfg["dt"], 0.0)
+        orc.step(cfg["dt"], 0.0)
+        assert abs(gpu.last_solve_stats()["iters"] - orc.stats()["iters"]) <= 1
+    errs = compare_fields(gpu, orc, ("u", "v", "w", "density", "temperature"))
+    print("rel L2", errs)
+    assert max(errs.values()) <= 1e-4, errs
+
+
+def test_determinism_bitwise():
+    _announce("determinism 32^3")
+    import aether_fluid as af
+    outs = []
+    for _ in range(2):
+        s = af.FluidSolver3D(32, 32, 32, precision="fp32")
+        st = random_state(32, 32, 32, seed=9)
+        for n, a in st.items():
+            s.upload(n, a)
+        s.cg_rel_tol, s.solver_iters = 1e-5, 200
+        s.run(0.02, [0.0, 0.1, 0.2])
+        outs.append(np.concatenate([s.download(n) for n in ("u", "v", "w", "density",
+                                                             "pressure")]))
+    np.testing.assert_array_equal(outs[0], outs[1])
+
+
+def test_sample_trilinear_and_max_density():
+    _announce("sample + max density")
+    st = random_state(12, 14, 10, seed=11)
+    gpu, orc = make_pair(12, 14, 10, state=st)
+    pts = np.random.default_rng(2).uniform(-1, 16, size=(64, 3))
+    got = gpu.sample_trilinear("density", pts)
+    exp = np.array([orc.sample_trilinear("density", *p) for p in pts])
+    np.testing.assert_array_equal(got, exp)
+    assert gpu.max_density() == max(0.0, st["density"].max())
+
+
+def test_invalid_arguments_raise():
+    import aether_fluid as af
+    with pytest.raises(af.AetherFluidError):
+        af.FluidSolver3D(5, 10, 10)
+    s = af.FluidSolver3D(8, 8, 8)
+    with pytest.raises(af.AetherFluidError):
+        s.upload("u", np.zeros(7))
+    with pytest.raises(af.AetherFluidError):
+        s.advect_maccormack_in_place(7, 0.1, 1.0)

@@ -1,0 +1,373 @@
+"""CPU tier: pins of the f64 oracle (oracle/fluid_oracle.c).
+
+Each test checks the oracle against something it does not restate: the
+reference's own known answer and ratio tests, published RNG vectors, or a
+property fixed by the mathematics (exactness on affine fields, integer
+shifts, an independently assembled graph Laplacian, a dense least-squares
+solve, the post-projection residual identity, closed-form telemetry).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle.oracle import OracleFluid, OracleRng, splitmix64_stream
+
+
+def _interior(nx, ny, nz):
+    m = np.zeros((nz, ny, nx), dtype=bool)
+    m[1:-1, 1:-1, 1:-1] = True
+    return m
+
+
+# ---------------------------------------------------------------- reference pins
+def test_readme_known_answer():
+    """README.md:54: verify.rs:229-250 prints 2.5751e0 -> 6.7289e-3 (99.7%)."""
+    s = OracleFluid(16, 20, 16)
+    s.solver_kind, s.solver_iters = 0, 40
+    s.inject_sources(0.5, 0.1)
+    s.apply_buoyancy_and_vorticity_confinement(0.1)
+    pre = s.compute_divergence_norm()
+    s.project()
+    post = s.compute_divergence_norm()
+    assert f"{pre:.4e}" == "2.5751e+00"
+    assert f"{post:.4e}" == "6.7289e-03"
+    assert f"{(1 - post / pre) * 100:.1f}" == "99.7"
+    assert post <= 0.03 * pre  # verify.rs:239
+
+
+@pytest.mark.parametrize("kind,iters", [(0, 45), (1, 90)])
+def test_fluid_rs_projection_ratio(kind, iters):
+    """src/fluid.rs:918-956."""
+    s = OracleFluid(18, 22, 18)
+    s.solver_kind, s.solver_iters = kind, iters
+    u, v, w = s.field("u"), s.field("v"), s.field("w")
+    for k in range(6, 12):
+        for j in range(5, 11):
+            for i in range(6, 12):
+                u[k, j, i] = (i - 8.5) * 3.0
+                v[k, j, i] = (j - 7.5) * 4.5
+                w[k, j, i] = (k - 8.5) * 2.5
+    s.enforce_wall_velocity_bc()
+    before = s.compute_divergence_norm()
+    assert before > 1.0
+    s.project()
+    assert s.compute_divergence_norm() < before * 0.02
+
+
+def test_fluid_rs_positivity_two_steps():
+    """src/fluid.rs:959-968."""
+    s = OracleFluid(16, 20, 16)
+    f1 = s.step(0.08, 0.2)
+    f2 = s.step(0.08, 0.4)
+    assert f1[1] > 0.0 and f2[2] > 0.0
+    d = s.field("density")
+    assert np.all(np.isfinite(d)) and np.all(d >= 0.0)
+
+
+# ---------------------------------------------------------------- RNG
+def test_xoshiro256pp_published_vector():
+    """Xoshiro256++ from state {1,2,3,4}: the published first ten outputs."""
+    r = OracleRng(state=[1, 2, 3, 4])
+    got = [r.next_u64() for _ in range(10)]
+    assert got == [41943041, 58720359, 3588806011781223, 3591011842654386,
+                   9228616714210784205, 9973669472204895162, 14011001112246962877,
+                   12406186145184390807, 15849039046786891736, 10450023813501588000]
+
+
+def test_splitmix64_published_vector():
+    assert splitmix64_stream(0, 3)[0] == 0xE220A8397B1DCDAF
+
+
+def test_next_f64_scaling():
+    # next_f64 = (x >> 11) * 2^-53: for state {1,2,3,4} the first draw is 41943041 >> 11
+    r = OracleRng(state=[1, 2, 3, 4])
+    assert r.next_f64() == (41943041 >> 11) * 2.0 ** -53
+
+
+def test_product_rng_matches_oracle_rng():
+    import aether_fluid as af
+    vals, st = af.rng_fill(0xAE7E0001, -1.0, 1.0, 1000)
+    r = OracleRng(0xAE7E0001)
+    exp = np.array([r.range_f64(-1.0, 1.0) for _ in range(1000)])
+    np.testing.assert_array_equal(vals, exp)
+    more, _ = af.rng_fill(st, 0.0, 1.0, 10)
+    np.testing.assert_array_equal(more, [r.next_f64() for _ in range(10)])
+
+
+# ---------------------------------------------------------------- sampling / advection
+def test_trilinear_exact_on_affine_field():
+    s = OracleFluid(9, 11, 13)
+    k, j, i = np.meshgrid(np.arange(13), np.arange(11), np.arange(9), indexing="ij")
+    s.field("density")[...] = 0.25 + 1.5 * i - 0.75 * j + 2.0 * k
+    rng = np.random.default_rng(0)
+    for _ in range(200):
+        x, y, z = rng.uniform(0.5, 9 - 1.5), rng.uniform(0.5, 11 - 1.5), rng.uniform(0.5, 13 - 1.5)
+        exp = 0.25 + 1.5 * x - 0.75 * y + 2.0 * z
+        assert abs(s.sample_trilinear("density", x, y, z) - exp) < 1e-12
+
+
+def test_trilinear_clamps_outside():
+    s = OracleFluid(8, 8, 8)
+    f = s.field("density")
+    f[...] = np.random.default_rng(1).uniform(size=f.shape)
+    a = s.sample_trilinear("density", -5.0, 3.2, 100.0)
+    b = s.sample_trilinear("density", 0.5, 3.2, 8 - 1.500001)
+    assert a == b
+
+
+def test_maccormack_integer_shift_is_exact():
+    """Uniform velocity moving exactly one cell per step: interior cells away
+    from the walls receive the upwind neighbour's value times dissipation."""
+    nx = ny = nz = 14
+    s = OracleFluid(nx, ny, nz, h=0.1)
+    dt = 0.1  # dt/h * 1.0 = one cell
+    s.field("u")[...] = 1.0
+    s.field("v")[...] = 0.0
+    s.field("w")[...] = 0.0
+    rho = np.random.default_rng(2).uniform(size=(nz, ny, nx))
+    s.field("density")[...] = rho
+    s.advect_maccormack_in_place(3, dt, 0.9)
+    out = s.field("density")
+    # back-trace x = i - 1 must stay >= 1.5 and forward trace i + 1 <= n - 2.5
+    sl = (slice(3, nz - 3), slice(3, ny - 3), slice(3, nx - 3))
+    np.testing.assert_array_equal(out[sl], 0.9 * rho[sl[0], sl[1], slice(2, nx - 4)])
+
+
+def test_maccormack_constant_field():
+    s = OracleFluid(10, 12, 9)
+    rng = np.random.default_rng(3)
+    for n in ("u", "v", "w"):
+        s.field(n)[...] = rng.uniform(-1, 1, size=(9, 12, 10))
+    s.field("temperature")[...] = 0.7
+    s.advect_maccormack_in_place(4, 0.05, 0.985)
+    t = s.field("temperature")
+    inside = _interior(10, 12, 9)
+    np.testing.assert_allclose(t[inside], 0.7 * 0.985, rtol=1e-15)
+    assert np.all(t[~inside] == 0.0)
+
+
+def test_maccormack_stays_within_clamp_and_nonnegative():
+    s = OracleFluid(16, 16, 16)
+    rng = np.random.default_rng(4)
+    for n in ("u", "v", "w"):
+        s.field(n)[...] = rng.uniform(-3, 3, size=(16, 16, 16))
+    rho = rng.uniform(0, 1, size=(16, 16, 16))
+    s.field("density")[...] = rho
+    s.advect_maccormack_in_place(3, 0.1, 1.0)
+    d = s.field("density")
+    assert d.min() >= 0.0 and d.max() <= rho.max()
+
+
+# ---------------------------------------------------------------- Poisson operator
+def _dense_operator(s):
+    n = s.n
+    A = np.zeros((n, n))
+    e = np.zeros(n)
+    for c in range(n):
+        e[:] = 0.0
+        e[c] = 1.0
+        A[:, c] = s.apply_laplacian(e).ravel()
+    return A
+
+
+def _graph_laplacian(nx, ny, nz):
+    """L = D - Adj of the graph whose nodes are interior cells and whose edges
+    join face neighbours (the pure-Neumann 7-point Laplacian)."""
+    n = nx * ny * nz
+    L = np.zeros((n, n))
+    idx = lambda i, j, k: (k * ny + j) * nx + i  # noqa: E731
+    for k in range(1, nz - 1):
+        for j in range(1, ny - 1):
+            for i in range(1, nx - 1):
+                a = idx(i, j, k)
+                for di, dj, dk in ((1, 0, 0), (0, 1, 0), (0, 0, 1)):
+                    ii, jj, kk = i + di, j + dj, k + dk
+                    if 1 <= ii <= nx - 2 and 1 <= jj <= ny - 2 and 1 <= kk <= nz - 2:
+                        b = idx(ii, jj, kk)
+                        L[a, a] += 1
+                        L[b, b] += 1
+                        L[a, b] -= 1
+                        L[b, a] -= 1
+    return L
+
+
+def test_laplacian_is_interior_graph_laplacian():
+    s = OracleFluid(6, 7, 8)
+    A = _dense_operator(s)
+    np.testing.assert_array_equal(A, _graph_laplacian(6, 7, 8))
+    assert np.allclose(A, A.T)
+    ones = _interior(6, 7, 8).ravel().astype(float)
+    assert np.abs(A @ ones).max() == 0.0
+    assert np.linalg.eigvalsh(A).min() > -1e-12
+
+
+def test_cg_matches_dense_least_squares():
+    nx, ny, nz = 7, 6, 8
+    s = OracleFluid(nx, ny, nz)
+    inside = _interior(nx, ny, nz).ravel()
+    rng = np.random.default_rng(5)
+    b = np.zeros(s.n)
+    b[inside] = rng.normal(size=inside.sum())
+    b[inside] -= b[inside].mean()  # compatible with the constant null space
+    s.field("divergence")[...] = b.reshape(nz, ny, nx)
+    s.solve_poisson_conjugate_gradient(500)
+    p = s.field("pressure").ravel()[inside]
+    L = _graph_laplacian(nx, ny, nz)[np.ix_(inside, inside)]
+    ref, *_ = np.linalg.lstsq(L, b[inside], rcond=None)
+    np.testing.assert_allclose(p - p.mean(), ref - ref.mean(), atol=1e-8)
+    st = s.stats()
+    assert st["converged"] and st["r2"] < 1e-20
+
+
+def test_cg_zero_rhs_no_iterations():
+    s = OracleFluid(10, 10, 10)
+    s.solver_iters = 50
+    s.project()
+    assert s.stats()["iters"] == 0
+
+
+def test_cg_rel_tol_stops_early():
+    s = OracleFluid(16, 16, 16)
+    rng = np.random.default_rng(6)
+    for n in ("u", "v", "w"):
+        s.field(n)[...] = rng.uniform(-1, 1, size=(16, 16, 16))
+    s.solver_iters, s.cg_rel_tol = 500, 1e-4
+    s.project()
+    st = s.stats()
+    assert st["converged"] and st["r2"] <= 1e-8 * st["b2"] and 0 < st["iters"] < 500
+
+
+def test_projection_residual_identity():
+    """After project(), -h * div(u) equals the CG residual cell by cell:
+    D(u - G p/h) = D u + A p / h, so -h D u_new = b - A p = r."""
+    s = OracleFluid(12, 14, 10)
+    rng = np.random.default_rng(7)
+    for n in ("u", "v", "w"):
+        s.field(n)[...] = rng.uniform(-1, 1, size=(10, 14, 12))
+    s.solver_iters = 15  # stop early so that r is far from zero
+    s.project()
+    r = s.field("cg_r").copy()
+    s.compute_divergence_norm()  # stores -h*div(u) of the projected field
+    b_new = s.field("divergence")
+    assert np.abs(r).max() > 1e-3
+    np.testing.assert_allclose(b_new, r, atol=1e-12)
+
+
+def test_rbgs_parallel_equals_sequential():
+    """Red-black colouring: the OpenMP build (parallel over k within a
+    colour) gives the sequential result bit for bit."""
+    outs = []
+    for omp in (False, True):
+        s = OracleFluid(14, 12, 16, omp=omp)
+        rng = np.random.default_rng(8)
+        for n in ("u", "v", "w"):
+            s.field(n)[...] = rng.uniform(-1, 1, size=(16, 12, 14))
+        s.solver_kind, s.solver_iters = 1, 25
+        s.project()
+        outs.append(s.field("pressure").copy())
+    np.testing.assert_array_equal(outs[0], outs[1])
+
+
+# ---------------------------------------------------------------- forces
+def test_linear_velocity_has_no_confinement():
+    """Curl of a linear field is constant, so grad|w| = 0 and the
+    confinement force vanishes; only buoyancy and the wall BC act."""
+    nx, ny, nz = 10, 11, 12
+    s = OracleFluid(nx, ny, nz)
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    # dyadic coefficients: every difference is exact, so |w| is bit-identical
+    # in all cells and the 1e-10 guard of src/fluid.rs:298 makes N = 0
+    s.field("u")[...] = 0.25 * j + 0.125 * k
+    s.field("v")[...] = -0.5 * k + 0.0625 * i
+    s.field("w")[...] = 0.375 * i
+    rng = np.random.default_rng(9)
+    T = rng.uniform(size=(nz, ny, nx))
+    R = rng.uniform(size=(nz, ny, nx))
+    s.field("temperature")[...] = T
+    s.field("density")[...] = R
+    u0, v0, w0 = (s.field(n).copy() for n in ("u", "v", "w"))
+    s.vorticity_eps, s.buoyancy_beta, s.gravity_alpha = 50.0, 2.0, 0.5
+    dt = 0.1
+    s.apply_buoyancy_and_vorticity_confinement(dt)
+    inside = _interior(nx, ny, nz)
+    ev = v0 + np.where(inside, dt * (2.0 * T - 0.5 * R), 0.0)
+    # wall masks of src/fluid.rs:241-249
+    mu = np.ones_like(inside)
+    mu[:, :, :2] = mu[:, :, -1:] = mu[:, :1, :] = mu[:, -1:, :] = mu[:1] = mu[-1:] = False
+    mv = np.ones_like(inside)
+    mv[:, :2, :] = mv[:, -1:, :] = mv[:, :, :1] = mv[:, :, -1:] = mv[:1] = mv[-1:] = False
+    mw = np.ones_like(inside)
+    mw[:2] = mw[-1:] = mw[:, :, :1] = mw[:, :, -1:] = mw[:, :1, :] = mw[:, -1:, :] = False
+    np.testing.assert_allclose(s.field("u"), np.where(mu, u0, 0.0), atol=1e-13)
+    np.testing.assert_allclose(s.field("v"), np.where(mv, ev, 0.0), atol=1e-13)
+    np.testing.assert_allclose(s.field("w"), np.where(mw, w0, 0.0), atol=1e-13)
+
+
+def test_sphere_source_sets_values_inside_only():
+    s = OracleFluid(20, 20, 20)
+    s.set_source(1, (10.0, 3.0, 9.5), 4.0, 1.0, 2.0)
+    s.inject_sources(0.0, 0.1)
+    k, j, i = np.meshgrid(np.arange(20), np.arange(20), np.arange(20), indexing="ij")
+    inside = ((i - 10.0) ** 2 + (j - 3.0) ** 2 + (k - 9.5) ** 2 < 16.0) & _interior(20, 20, 20)
+    np.testing.assert_array_equal(s.field("density"), np.where(inside, 1.0, 0.0))
+    np.testing.assert_array_equal(s.field("temperature"), np.where(inside, 2.0, 0.0))
+
+
+def test_jets_at_time_zero_closed_form():
+    """At t = 0: pulse = 0.65 + 0.35 sin 0 + 0.25 max(cos 0, 0) = 0.9, angle
+    = 0, so jet 1 sits at (cx + 0.14 nx, base_j + 1.5, cz) pushing
+    (0, 7.2*0.9, 2.6) and jet 2 at (cx - 0.14 nx, base_j + 2, cz) pushing
+    (0, 6.4*0.9, -2.6) (src/fluid.rs:538-560)."""
+    nx, ny, nz = 40, 30, 40
+    s = OracleFluid(nx, ny, nz)
+    dt = 0.1
+    s.inject_sources(0.0, dt)
+    r = nx * 0.105
+    jets = [((20 + 0.14 * nx, 4.5, 20.0), (0.0, 7.2 * 0.9, 2.6)),
+            ((20 - 0.14 * nx, 5.0, 20.0), (0.0, 6.4 * 0.9, -2.6))]
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    rho = np.zeros((nz, ny, nx))
+    vz = np.zeros((nz, ny, nx))
+    for (cx, cy, cz), vel in jets:
+        d2 = (i - cx) ** 2 + (j - cy) ** 2 + (k - cz) ** 2
+        wf = np.where(d2 < r * r, np.exp(-2.2 * d2 / (r * r)), 0.0)
+        rho = np.minimum(rho + dt * 9.5 * 0.9 * wf, 3.8)
+        vz = vz + dt * vel[2] * wf * 4.2
+    np.testing.assert_allclose(s.field("density"), rho, rtol=1e-13, atol=1e-15)
+    np.testing.assert_allclose(s.field("w"), vz, rtol=1e-13, atol=1e-15)
+
+
+# ---------------------------------------------------------------- telemetry
+def test_telemetry_uniform_velocity_and_unit_mass():
+    nx, ny, nz = 9, 8, 10
+    s = OracleFluid(nx, ny, nz)
+    h = s.h
+    s.field("u")[...] = 0.3
+    s.field("v")[...] = -0.4
+    s.field("w")[...] = 1.2
+    s.field("density")[4, 3, 6] = 2.0  # (i, j, k) = (6, 3, 4)
+    t = s.compute_telemetry(1.25)
+    interior_n = (nx - 2) * (ny - 2) * (nz - 2)
+    assert t[0] == 1.25
+    assert math.isclose(t[1], 0.5 * (0.09 + 0.16 + 1.44) * h ** 3 * interior_n, rel_tol=1e-12)
+    assert t[2] == 0.0 and t[3] == 0.0
+    assert math.isclose(t[4], 2.0 * h ** 3, rel_tol=1e-15)
+    exp = [2 * 6 / (nx - 1) - 1, 2 * 3 / (ny - 1) - 1, 2 * 4 / (nz - 1) - 1]
+    np.testing.assert_allclose(t[5:8], exp, rtol=1e-14)
+
+
+def test_telemetry_solid_body_rotation_enstrophy():
+    """u = -c*(y), v = c*(x) (about a vertical axis): curl = (0, 0, 2c) in
+    index units /h... with central differences exact on linear fields."""
+    nx, ny, nz = 10, 10, 10
+    s = OracleFluid(nx, ny, nz)
+    h = s.h
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    c = 0.7
+    s.field("u")[...] = -c * j * h
+    s.field("v")[...] = c * i * h
+    t = s.compute_telemetry(0.0)
+    interior_n = (nx - 2) * (ny - 2) * (nz - 2)
+    assert math.isclose(t[3], 2 * c, rel_tol=1e-12)
+    assert math.isclose(t[2], 0.5 * (2 * c) ** 2 * h ** 3 * interior_n, rel_tol=1e-12)
