@@ -4,12 +4,13 @@
 host-side mirror."""
 from ._lib import (AetherFluidError, FIELDS, FP32, FP32_IR64, FP64, SOLVER_CG, SOLVER_RBGS,
                    SOURCE_JETS, SOURCE_NONE, SOURCE_SPHERE, LIB_PATH, SIGNATURES, build, lib)
-from .solver import FluidSolver3D, FluidTelemetryFrame, PoissonSolverKind, rng_fill
+from .solver import (FluidSolver3D, FluidTelemetryFrame, PoissonSolverKind, rng_fill,
+                     stream_copy_ms)
 from .scenarios import seeded_config_state, SMOKE_CONFIGS
 
 __all__ = [
     "AetherFluidError", "FIELDS", "FP32", "FP32_IR64", "FP64", "SOLVER_CG", "SOLVER_RBGS",
     "SOURCE_JETS", "SOURCE_NONE", "SOURCE_SPHERE", "LIB_PATH", "SIGNATURES", "build", "lib",
-    "FluidSolver3D", "FluidTelemetryFrame", "PoissonSolverKind", "rng_fill",
+    "FluidSolver3D", "FluidTelemetryFrame", "PoissonSolverKind", "rng_fill", "stream_copy_ms",
     "seeded_config_state", "SMOKE_CONFIGS",
 ]

@@ -11,7 +11,8 @@ import os
 import subprocess
 
 PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(PKG_ROOT, "lib", "libaether_fluid_hip.so")
+LIB_PATH = os.environ.get("AETHER_FLUID_LIB") or os.path.join(PKG_ROOT, "lib",
+                                                                "libaether_fluid_hip.so")
 
 # status codes and enums (include/aether_fluid.h)
 AETHER_OK = 0
@@ -115,6 +116,7 @@ SIGNATURES = {
     "aether_rng_seed": (C.c_int, [C.c_uint64, _pu64]),
     "aether_rng_fill_range": (C.c_int, [_pu64, C.c_double, C.c_double, _pd, C.c_size_t]),
     "aether_fluid_set_kernel_timing": (C.c_int, [_vp, C.c_int]),
+    "aether_fluid_stream_copy": (C.c_int, [C.c_size_t, C.c_int, _pd]),
     "aether_fluid_kernel_timing": (C.c_int, [_vp, C.c_int, _pd, C.POINTER(C.c_long)]),
 }
 

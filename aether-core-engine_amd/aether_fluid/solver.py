@@ -279,3 +279,10 @@ def rng_fill(seed_or_state, lo: float, hi: float, n: int):
     out = np.empty(n, dtype=np.float64)
     L.check(lib.aether_rng_fill_range(st, lo, hi, _dptr(out), n))
     return out, [st[a] for a in range(4)]
+
+
+def stream_copy_ms(nbytes: int, reps: int = 20) -> float:
+    """Average device time (ms) of a plain nbytes copy kernel."""
+    out = C.c_double()
+    L.check(L.lib().aether_fluid_stream_copy(nbytes, reps, C.byref(out)))
+    return out.value

@@ -103,6 +103,29 @@ __device__ __forceinline__ double sum_partials(const double *part, int nb, doubl
     return acc[0];
 }
 
+// Partial sums of the previous kernel, loaded early (before the data loads of
+// the caller) so that both memory round trips overlap.  Up to 8 per thread
+// are held in registers; any excess is summed in a loop afterwards.
+struct EarlyPartials {
+    double v[8];
+    __device__ __forceinline__ void load(const double *__restrict__ part, int nb) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int a = tid_lin() + q * kThreads;
+            const double x = part[min(a, nb - 1)];  // always a valid address
+            v[q] = a < nb ? x : 0.0;
+        }
+    }
+    // fixed-order total, identical in every workgroup
+    __device__ __forceinline__ double total(const double *__restrict__ part, int nb,
+                                            double *lds) const {
+        double acc[1] = {((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]))};
+        for (int a = tid_lin() + 8 * kThreads; a < nb; a += kThreads) acc[0] += part[a];
+        block_sum<1>(acc, lds);
+        return acc[0];
+    }
+};
+
 __device__ __forceinline__ bool is_interior(const Dims &g, int i, int j, int k) {
     return i >= 1 && i <= g.nx - 2 && j >= 1 && j <= g.ny - 2 && k >= 1 && k <= g.nz - 2;
 }
@@ -556,10 +579,106 @@ __global__ __launch_bounds__(kThreads) void k_div_max(Dims g, int zchunk, T inv_
         part[(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = m;
 }
 
-// Divergence + CG initialisation (src/fluid.rs:497-498, 450-460):
-// b = -h div(masked u), r = b - A p; partials of r.r and b.b.
+// ---------------------------------------------------------------------------
+// CG kernels.  Workgroup = 64 x 4 columns of cells; every thread owns ZU
+// consecutive z cells of its column.  All loads of a thread are issued before
+// the workgroup combines the previous kernel's partial sums (a barrier on
+// gfx950 does not wait for outstanding global loads), so the fixed-order
+// reduction overlaps the memory traffic.  Workgroups are numbered so that
+// each of the 8 XCDs receives one contiguous slab of tiles (block b runs on
+// XCD b % 8), which keeps y/z halo reads inside one L2.
+// ---------------------------------------------------------------------------
+struct Tiling {
+    int gx, gy, gz;  // tiles of 64 x, 4 y and ZU z cells
+    int nb;          // gx * gy * gz workgroups (= partial sums)
+    int remap;       // 1 when nb % 8 == 0: XCD-contiguous tile order
+};
+
+__device__ __forceinline__ void tile_coords(const Tiling &t, int &bx, int &by, int &bz) {
+    int b = blockIdx.x;
+    if (t.remap) b = (b & 7) * (t.nb >> 3) + (b >> 3);
+    bx = b % t.gx;
+    const int rest = b / t.gx;
+    by = rest % t.gy;
+    bz = rest / t.gy;
+}
+
+// A x at an interior cell from its 7 stencil values (src/fluid.rs:356-394);
+// v* say whether each neighbour is an interior cell.
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_div_cginit(Dims g, int zchunk, T h, int with_div,
+__device__ __forceinline__ T lap7(T xc, T xm, T xp, T ym, T yp, T zm, T zp, bool vxm, bool vxp,
+                                  bool vym, bool vyp, bool vzm, bool vzp) {
+    if constexpr (sizeof(T) == 8) {
+        T diag = T(0), nb = T(0);
+        if (vxm) { diag += T(1); nb += xm; }
+        if (vxp) { diag += T(1); nb += xp; }
+        if (vym) { diag += T(1); nb += ym; }
+        if (vyp) { diag += T(1); nb += yp; }
+        if (vzm) { diag += T(1); nb += zm; }
+        if (vzp) { diag += T(1); nb += zp; }
+        return diag * xc - nb;
+    } else {
+        T acc = T(0);
+        if (vxm) acc += xc - xm;
+        if (vxp) acc += xc - xp;
+        if (vym) acc += xc - ym;
+        if (vyp) acc += xc - yp;
+        if (vzm) acc += xc - zm;
+        if (vzp) acc += xc - zp;
+        return acc;
+    }
+}
+
+// Store for CG vectors.  -DAETHER_NT_STORES=1 selects streaming
+// (non-temporal) stores; measured at 128^3 they are ~2% slower than plain
+// stores, which stay the default.
+#ifndef AETHER_NT_STORES
+#define AETHER_NT_STORES 0
+#endif
+template <typename T>
+__device__ __forceinline__ void st_stream(T *p, T v) {
+#if AETHER_NT_STORES
+    __builtin_nontemporal_store(v, p);
+#else
+    *p = v;
+#endif
+}
+
+// Column geometry shared by the CG kernels.
+struct Col {
+    int i, j, k0;
+    int ic, jc;          // clamped column (always a valid address)
+    int xm, xp, ym, yp;  // clamped in-plane neighbour offsets from (ic, jc)
+    bool valid;
+};
+
+template <int ZU>
+__device__ __forceinline__ Col col_of(const Dims &g, const Tiling &tl) {
+    int bx, by, bz;
+    tile_coords(tl, bx, by, bz);
+    Col c;
+    c.i = bx * kBX + threadIdx.x;
+    c.j = by * kBY + threadIdx.y;
+    c.k0 = bz * ZU;
+    c.valid = c.i < g.nx && c.j < g.ny;
+    c.ic = min(c.i, g.nx - 1);
+    c.jc = min(c.j, g.ny - 1);
+    c.xm = (c.ic > 0 ? -1 : 0);
+    c.xp = (c.ic < g.nx - 1 ? 1 : 0);
+    c.ym = (c.jc > 0 ? -g.nx : 0);
+    c.yp = (c.jc < g.ny - 1 ? g.nx : 0);
+    return c;
+}
+
+__device__ __forceinline__ int col_id(const Dims &g, const Col &c, int k) {
+    k = min(max(k, 0), g.nz - 1);
+    return (k * g.ny + c.jc) * g.nx + c.ic;
+}
+
+// Divergence + CG initialisation (src/fluid.rs:497-498, 450-460):
+// b = -h div(masked u) (or the stored b), r = b - A p; partials of r.r, b.b.
+template <typename T, int ZU>
+__global__ __launch_bounds__(kThreads) void k_div_cginit(Dims g, Tiling tl, T h, int with_div,
                                                          const T *__restrict__ u,
                                                          const T *__restrict__ v,
                                                          const T *__restrict__ w,
@@ -569,24 +688,26 @@ __global__ __launch_bounds__(kThreads) void k_div_cginit(Dims g, int zchunk, T h
                                                          double *__restrict__ part_bb,
                                                          CgState *st) {
     __shared__ double lds[16];
-    const int i = blockIdx.x * kBX + threadIdx.x;
-    const int j = blockIdx.y * kBY + threadIdx.y;
-    const int kb = blockIdx.z * zchunk;
-    const int ke = min(kb + zchunk, g.nz);
+    const Col c = col_of<ZU>(g, tl);
     double acc[2] = {0.0, 0.0};
-    if (i < g.nx && j < g.ny) {
-        for (int k = kb; k < ke; ++k) {
-            const int id = (k * g.ny + j) * g.nx + i;
+    if (c.valid) {
+#pragma unroll
+        for (int a = 0; a < ZU; ++a) {
+            const int k = c.k0 + a;
+            if (k >= g.nz) break;
+            const int id = (k * g.ny + c.j) * g.nx + c.i;
             T bv = T(0), rv = T(0);
-            if (is_interior(g, i, j, k)) {
+            if (is_interior(g, c.i, c.j, k)) {
                 if (with_div) {
                     T du, dv, dw;
-                    div_terms<T>(g, u, v, w, i, j, k, id, true, du, dv, dw);
+                    div_terms<T>(g, u, v, w, c.i, c.j, k, id, true, du, dv, dw);
                     bv = -h * (du + dv + dw);
                 } else {
                     bv = b[id];
                 }
-                const T q = lap_at<T>(g, i, j, k, id, [&](int a) { return p[a]; });
+                const T q = lap7<T>(p[id], p[id - 1], p[id + 1], p[id - g.nx], p[id + g.nx],
+                                    p[id - g.sxy], p[id + g.sxy], c.i > 1, c.i + 2 < g.nx,
+                                    c.j > 1, c.j + 2 < g.ny, k > 1, k + 2 < g.nz);
                 rv = bv - q;
             }
             if (with_div) b[id] = bv;
@@ -596,11 +717,10 @@ __global__ __launch_bounds__(kThreads) void k_div_cginit(Dims g, int zchunk, T h
         }
     }
     block_sum<2>(acc, lds);
-    const int bid = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
     if (tid_lin() == 0) {
-        part_rr[bid] = acc[0];
-        part_bb[bid] = acc[1];
-        if (bid == 0) {
+        part_rr[blockIdx.x] = acc[0];
+        part_bb[blockIdx.x] = acc[1];
+        if (blockIdx.x == 0) {
             st->done = 0;
             st->iters = 0;
             st->converged = 0;
@@ -608,26 +728,58 @@ __global__ __launch_bounds__(kThreads) void k_div_cginit(Dims g, int zchunk, T h
     }
 }
 
-// CG kernel 1 of iteration `it` (src/fluid.rs:463-469, 482-486):
-// finish r.r of the previous iteration, test the stopping rules, form
-// d = r + beta d_old (d = r at it = 0), q = A d on the fly, and the partial
-// d.q.  d is double-buffered so that neighbours can recompute it.
-template <typename T>
-__global__ __launch_bounds__(kThreads) void k_cg_dir(Dims g, int zchunk, int it, double rel_tol2,
+// CG kernel 1 of iteration `it` (src/fluid.rs:463-469, 482-486): finish
+// r.r of the previous iteration, apply the stopping rules, form
+// d = r + beta d_old (d = r at it = 0) on the fly, q = A d, partial d.q.
+// d is double-buffered so that neighbouring columns recompute it.
+template <typename T, int ZU, bool FIRST>
+__global__ __launch_bounds__(kThreads) void k_cg_dir(Dims g, Tiling tl, int it, double rel_tol2,
                                                      const double *__restrict__ part_rr,
-                                                     const double *__restrict__ part_bb, int nb,
+                                                     const double *__restrict__ part_bb,
                                                      CgState *st, const T *__restrict__ r,
                                                      const T *__restrict__ dold,
                                                      T *__restrict__ dnew,
-                                                     double *__restrict__ part_dq) {
+                                                     double *__restrict__ part_dq,
+                                                     int *done_mirror, int solve_id) {
     __shared__ double lds[8];
     __shared__ int flag_lds[1];
-    if (block_flag(&st->done, flag_lds)) return;
-    const bool lead = (blockIdx.x | blockIdx.y | blockIdx.z) == 0 && tid_lin() == 0;
+    const Col c = col_of<ZU>(g, tl);
+    constexpr bool first = FIRST;
+    // 1) scalars and partial sums of the previous kernel, then every data
+    //    load of this thread: one memory round trip for all of them.  The
+    //    stop flag was written by an earlier kernel, so a plain load sees it.
+    const int flag_reg = st->done;
+    EarlyPartials prr, pbb;
+    prr.load(part_rr, tl.nb);
+    if (first) pbb.load(part_bb, tl.nb);
+    const double delta_prev = first ? 0.0 : st->delta[(it - 1) & 1];
+    const double b2_prev = first ? 0.0 : st->b2;
+    T rz[ZU + 2], dz[ZU + 2], rn[ZU][4], dn[ZU][4];
+#pragma unroll
+    for (int a = 0; a < ZU + 2; ++a) {
+        const int id = col_id(g, c, c.k0 - 1 + a);
+        rz[a] = r[id];
+        dz[a] = first ? T(0) : dold[id];
+    }
+#pragma unroll
+    for (int a = 0; a < ZU; ++a) {
+        const int id = col_id(g, c, c.k0 + a);
+        const int o[4] = {c.xm, c.xp, c.ym, c.yp};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            rn[a][q] = r[id + o[q]];
+            dn[a][q] = first ? T(0) : dold[id + o[q]];
+        }
+    }
+    // 2) stopping rules (every workgroup reaches the same decision); thread
+    //    0's flag is broadcast so that all waves take the same branch
+    const bool lead = blockIdx.x == 0 && tid_lin() == 0;
     double beta = 0.0;
-    const double rr = sum_partials(part_rr, nb, lds);
-    if (it == 0) {
-        const double bb = sum_partials(part_bb, nb, lds);
+    if (tid_lin() == 0) flag_lds[0] = flag_reg;
+    const double rr = prr.total(part_rr, tl.nb, lds);  // its barrier publishes flag_lds
+    if (flag_lds[0]) return;
+    if (first) {
+        const double bb = pbb.total(part_bb, tl.nb, lds);
         const bool stop = !(rr > 1e-24) || (rel_tol2 > 0.0 && rr <= rel_tol2 * bb);
         if (lead) {
             st->b2 = bb;
@@ -638,93 +790,118 @@ __global__ __launch_bounds__(kThreads) void k_cg_dir(Dims g, int zchunk, int it,
                 st->iters = 0;
                 st->r2_final = rr;
                 st->converged = 1;
+                *done_mirror = solve_id;
             }
         }
         if (stop) return;
     } else {
-        const bool stop = rr < 1e-20 || (rel_tol2 > 0.0 && rr <= rel_tol2 * st->b2);
+        const bool stop = rr < 1e-20 || (rel_tol2 > 0.0 && rr <= rel_tol2 * b2_prev);
         if (lead) {
             if (stop) {
                 st->done = 1;
                 st->iters = it;
                 st->r2_final = rr;
                 st->converged = 1;
+                *done_mirror = solve_id;
             } else {
                 st->delta[it & 1] = rr;
             }
         }
         if (stop) return;
-        beta = rr / st->delta[(it - 1) & 1];
+        beta = rr / delta_prev;
     }
+    // 3) d, A d, d.(A d)
     const T tb = (T)beta;
-    const bool first = it == 0;
-    auto dval = [&](int a) -> T { return first ? r[a] : r[a] + tb * dold[a]; };
-    const int i = blockIdx.x * kBX + threadIdx.x;
-    const int j = blockIdx.y * kBY + threadIdx.y;
-    const int kb = blockIdx.z * zchunk;
-    const int ke = min(kb + zchunk, g.nz);
     double acc[1] = {0.0};
-    if (i < g.nx && j < g.ny) {
-        for (int k = kb; k < ke; ++k) {
-            const int id = (k * g.ny + j) * g.nx + i;
-            const T dn = dval(id);
-            if (is_interior(g, i, j, k)) {
-                const T q = lap_at<T>(g, i, j, k, id, dval);
-                acc[0] += (double)dn * (double)q;
+    T dcol[ZU + 2];
+#pragma unroll
+    for (int a = 0; a < ZU + 2; ++a) dcol[a] = first ? rz[a] : rz[a] + tb * dz[a];
+#pragma unroll
+    for (int a = 0; a < ZU; ++a) {
+        const int k = c.k0 + a;
+        if (c.valid && k < g.nz) {
+            const int id = (k * g.ny + c.j) * g.nx + c.i;
+            T x4[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) x4[q] = first ? rn[a][q] : rn[a][q] + tb * dn[a][q];
+            const T xc = dcol[a + 1];
+            if (is_interior(g, c.i, c.j, k)) {
+                const T q = lap7<T>(xc, x4[0], x4[1], x4[2], x4[3], dcol[a], dcol[a + 2],
+                                    c.i > 1, c.i + 2 < g.nx, c.j > 1, c.j + 2 < g.ny, k > 1,
+                                    k + 2 < g.nz);
+                acc[0] += (double)xc * (double)q;
             }
-            dnew[id] = dn;
+            st_stream(dnew + id, xc);
         }
     }
     block_sum<1>(acc, lds);
-    if (tid_lin() == 0)
-        part_dq[(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = acc[0];
+    if (tid_lin() == 0) part_dq[blockIdx.x] = acc[0];
 }
 
-// CG kernel 2 of iteration `it` (src/fluid.rs:466-478): finish d.q, alpha,
-// p += alpha d, r -= alpha A d (A d recomputed), partial r.r.
-template <typename T>
-__global__ __launch_bounds__(kThreads) void k_cg_update(Dims g, int zchunk, int it,
+// CG kernel 2 of iteration `it` (src/fluid.rs:466-478): finish d.q,
+// alpha, p += alpha d, r -= alpha A d (A d recomputed), partial r.r.
+template <typename T, int ZU>
+__global__ __launch_bounds__(kThreads) void k_cg_update(Dims g, Tiling tl, int it,
                                                         const double *__restrict__ part_dq,
-                                                        int nb, CgState *st,
-                                                        const T *__restrict__ d,
+                                                        CgState *st, const T *__restrict__ d,
                                                         T *__restrict__ p, T *__restrict__ r,
-                                                        double *__restrict__ part_rr) {
+                                                        double *__restrict__ part_rr,
+                                                        int *done_mirror, int solve_id) {
     __shared__ double lds[8];
     __shared__ int flag_lds[1];
-    if (block_flag(&st->done, flag_lds)) return;
-    const bool lead = (blockIdx.x | blockIdx.y | blockIdx.z) == 0 && tid_lin() == 0;
-    const double dq = sum_partials(part_dq, nb, lds);
+    const Col c = col_of<ZU>(g, tl);
+    const int flag_reg = st->done;
+    EarlyPartials pdq;
+    pdq.load(part_dq, tl.nb);
+    const double delta = st->delta[it & 1];
+    T dz[ZU + 2], dn[ZU][4], pc[ZU], rc[ZU];
+#pragma unroll
+    for (int a = 0; a < ZU + 2; ++a) dz[a] = d[col_id(g, c, c.k0 - 1 + a)];
+#pragma unroll
+    for (int a = 0; a < ZU; ++a) {
+        const int id = col_id(g, c, c.k0 + a);
+        dn[a][0] = d[id + c.xm];
+        dn[a][1] = d[id + c.xp];
+        dn[a][2] = d[id + c.ym];
+        dn[a][3] = d[id + c.yp];
+        pc[a] = p[id];
+        rc[a] = r[id];
+    }
+    const bool lead = blockIdx.x == 0 && tid_lin() == 0;
+    if (tid_lin() == 0) flag_lds[0] = flag_reg;
+    const double dq = pdq.total(part_dq, tl.nb, lds);
+    if (flag_lds[0]) return;
     if (fabs(dq) < 1e-28) {
         if (lead) {
             st->done = 1;
             st->iters = it;
             st->converged = 1;
-            st->r2_final = st->delta[it & 1];
+            st->r2_final = delta;
+            *done_mirror = solve_id;
         }
         return;
     }
-    const double alpha = st->delta[it & 1] / dq;
-    const T ta = (T)alpha;
-    const int i = blockIdx.x * kBX + threadIdx.x;
-    const int j = blockIdx.y * kBY + threadIdx.y;
-    const int kb = blockIdx.z * zchunk;
-    const int ke = min(kb + zchunk, g.nz);
+    const T ta = (T)(delta / dq);
     double acc[1] = {0.0};
-    if (i < g.nx && j < g.ny) {
-        for (int k = kb; k < ke; ++k) {
-            const int id = (k * g.ny + j) * g.nx + i;
-            const T dc = d[id];
+#pragma unroll
+    for (int a = 0; a < ZU; ++a) {
+        const int k = c.k0 + a;
+        if (c.valid && k < g.nz) {
+            const int id = (k * g.ny + c.j) * g.nx + c.i;
+            const T dc = dz[a + 1];
             T q = T(0);
-            if (is_interior(g, i, j, k)) q = lap_at<T>(g, i, j, k, id, [&](int a) { return d[a]; });
-            p[id] += ta * dc;
-            const T rn = r[id] - ta * q;
-            r[id] = rn;
+            if (is_interior(g, c.i, c.j, k))
+                q = lap7<T>(dc, dn[a][0], dn[a][1], dn[a][2], dn[a][3], dz[a], dz[a + 2],
+                            c.i > 1, c.i + 2 < g.nx, c.j > 1, c.j + 2 < g.ny, k > 1,
+                            k + 2 < g.nz);
+            st_stream(p + id, pc[a] + ta * dc);
+            const T rn = rc[a] - ta * q;
+            st_stream(r + id, rn);
             acc[0] += (double)rn * (double)rn;
         }
     }
     block_sum<1>(acc, lds);
-    if (tid_lin() == 0)
-        part_rr[(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = acc[0];
+    if (tid_lin() == 0) part_rr[blockIdx.x] = acc[0];
 }
 
 // Closes a solve that ran out of iterations (or had max_iters == 0).
@@ -930,6 +1107,14 @@ __global__ __launch_bounds__(kThreads) void k_convert(const Tin *__restrict__ in
                                                       Tout *__restrict__ out) {
     for (long a = (long)blockIdx.x * kThreads + tid_lin(); a < n; a += (long)gridDim.x * kThreads)
         out[a] = (Tout)in[a];
+}
+
+// Plain streaming copy (16-byte vectors), the reference point for measured
+// bandwidth and for scaling the memory-traffic counters.
+__global__ __launch_bounds__(kThreads) void k_stream_copy(const float4 *__restrict__ in, long n4,
+                                                          float4 *__restrict__ out) {
+    for (long a = (long)blockIdx.x * kThreads + tid_lin(); a < n4; a += (long)gridDim.x * kThreads)
+        out[a] = in[a];
 }
 
 template <typename T>

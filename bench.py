@@ -27,6 +27,8 @@ import numpy as np  # noqa: E402
 METRIC = ("smoke voxel-steps/s at 128³ (fwd, fwd+adjoint), % HBM roofline, vs host CPU")
 HBM_PEAK_GBPS = 8000.0
 WORKLOAD = "plain_128"
+SMOKE_DT = 0.01
+COPY_BYTES = 256 << 20
 
 
 def algorithmic_bytes_per_step(nvox: int, cg_iters: int) -> float:
@@ -127,11 +129,22 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--probe", action="store_true",
+                    help="counter-collection mode: 2 steps of the workload and a 256 MiB "
+                         "stream copy, no timing output")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only; the smoke step is not "
               "sharded across GPUs (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
+
+    import aether_fluid as af
+    if args.probe:
+        _, s = make_solver("cg")
+        s.run(SMOKE_DT, [0.0, SMOKE_DT])
+        af.stream_copy_ms(COPY_BYTES, 5)
+        print("probe done", flush=True)
+        return
 
     cfg, s, elapsed, frames, run_iters = time_gpu("cg", args.steps, args.warmup)
     nvox = int(np.prod(cfg["dims"]))
@@ -150,6 +163,9 @@ def main():
                 "algorithmic_bytes_per_launch": kd["bytes"], "launches_timed": kd["launches"],
                 "other_kernels": {k: {"avg_ms": v["avg_ms"], "gbps": v["gbps"]}
                                   for k, v in kern.items() if k != dom}}
+
+    copy_ms = af.stream_copy_ms(COPY_BYTES, 20)
+    roofline["copy_gbps"] = 2.0 * COPY_BYTES / (copy_ms * 1e-3) / 1e9
 
     line = {
         "metric": METRIC, "value": value, "unit": "voxel-steps/s", "n_gpus": 1,

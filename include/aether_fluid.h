@@ -177,6 +177,9 @@ int aether_rng_fill_range(uint64_t state[4], double lo, double hi, double *out, 
  * last aether_fluid_run with timing enabled (HIP events on the solver's
  * stream); *launches counts the kernels timed. */
 int aether_fluid_set_kernel_timing(aether_fluid_t s, int enable);
+/* Plain device-to-device copy of nbytes (multiple of 16), `reps` launches
+ * timed with HIP events: the measured-bandwidth reference point. */
+int aether_fluid_stream_copy(size_t nbytes, int reps, double *avg_ms);
 int aether_fluid_kernel_timing(aether_fluid_t s, int which, double *avg_ms, long *launches);
 
 #ifdef __cplusplus
