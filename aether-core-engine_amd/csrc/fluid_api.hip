@@ -20,6 +20,7 @@
 
 #include "aether_fluid.h"
 #include "fluid_kernels.hpp"
+#include "cg_resident.hpp"
 
 namespace aether {
 namespace {
@@ -146,8 +147,8 @@ struct KernelTimer {
     };
     std::vector<Rec> recs;
     size_t used = 0;
-    double sum_ms[2] = {0, 0};
-    long count[2] = {0, 0};
+    double sum_ms[3] = {0, 0, 0};  // cg_dir, cg_update, cg_resident
+    long count[3] = {0, 0, 0};
     hipEvent_t get() {
         if (used == pool.size()) {
             hipEvent_t e;
@@ -159,8 +160,10 @@ struct KernelTimer {
     void reset() {
         recs.clear();
         used = 0;
-        sum_ms[0] = sum_ms[1] = 0;
-        count[0] = count[1] = 0;
+        for (int a = 0; a < 3; ++a) {
+            sum_ms[a] = 0;
+            count[a] = 0;
+        }
     }
     ~KernelTimer() {
         for (auto e : pool) (void)hipEventDestroy(e);
@@ -253,6 +256,7 @@ class Solver final : public SolverBase {
         partA_.alloc(npart);
         partB_.alloc(npart);
         partC_.alloc(npart);
+        setup_resident(gx, gy, nz);
         part_tel_.alloc((size_t)nb_ * kTelParts);
         scalar_.alloc(8);
         tel_.alloc(8);
@@ -266,6 +270,93 @@ class Solver final : public SolverBase {
         zero(d_[1].p);
         HIPT(hipMemsetAsync(cg_.p, 0, sizeof(CgState), st_));
         sync();
+    }
+
+    template <int ZT>
+    bool resident_fits(int nb) {
+        int per_cu = 0, cus = 0, dev = 0;
+        HIPT(hipGetDevice(&dev));
+        HIPT(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        HIPT(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cg_resident<T, ZT, 0>,
+                                                          kThreads, 0));
+        return per_cu > 0 && (long)per_cu * cus >= nb;
+    }
+
+    // The resident CG keeps every cell in registers: usable when all tiles
+    // can be co-resident on the device (about 2-4 M cells on MI355X).  It is
+    // opt-in (AETHER_CG_RESIDENT=1): on MI355X a grid-wide barrier over 1024
+    // workgroups measured ~46 us (per-workgroup agent-scope fences), against
+    // ~10 us for a whole streaming CG kernel, so the two-kernel streaming
+    // iteration is the faster default.
+    void setup_resident(int gx, int gy, int nz) {
+        const char *e = std::getenv("AETHER_CG_RESIDENT");
+        if (!e || std::atoi(e) == 0) return;
+        if constexpr (sizeof(T) != 4) return;  // FP64 keeps the streaming kernels
+        for (int zt : {8, 16}) {
+            Tiling t;
+            t.gx = gx;
+            t.gy = gy;
+            t.gz = (nz + zt - 1) / zt;
+            t.nb = gx * gy * t.gz;
+            t.remap = (t.nb % 8 == 0) ? 1 : 0;
+            const bool fits = zt == 8 ? resident_fits<8>(t.nb) : resident_fits<16>(t.nb);
+            if (fits) {
+                tl_res_ = t;
+                res_zt_ = zt;
+                break;
+            }
+        }
+        if (!res_zt_) return;
+        if (const char *e = std::getenv("AETHER_GRID_BARRIER")) barrier_mode_ = std::atoi(e) ? 1 : 0;
+        if ((int)partA_.n < tl_res_.nb) partA_.alloc(tl_res_.nb);
+        partD_.alloc(tl_res_.nb);
+        gsync_.alloc(1);
+        HIPT(hipMemsetAsync(gsync_.p, 0, sizeof(GridSync), st_));
+    }
+
+    void enqueue_cg_resident(int max_iters, int step_index) {
+        Dims g = g_;
+        Tiling tl = tl_res_;
+        int mi = max_iters;
+        double rel2 = rel_tol2();
+        const double *p0 = partB_.p, *b0 = partC_.p;
+        int nb0 = tl_.nb;
+        double *pa = partA_.p, *pd = partD_.p;
+        T *pg = p_.p, *rg = r_.p, *dg = d_[0].p;
+        CgState *st = cg_.p;
+        int *log = step_index >= 0 ? iters_dev_.p + step_index : iters_log_.p;
+        GridSync *gs = gsync_.p;
+        void *args[] = {&g, &tl, &mi, &rel2, &p0, &b0, &nb0, &pa, &pd, &pg, &rg, &dg, &st, &log,
+                        &gs};
+        k_grid_sync_reset<<<1, 64, 0, st_>>>(gsync_.p);
+        LAUNCH_CHECK();
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (timer.enabled) {
+            e0 = timer.get();
+            HIPT(hipEventRecord(e0, st_));
+        }
+        const void *fn;
+        if (barrier_mode_ == 1)
+            fn = res_zt_ == 8 ? (const void *)k_cg_resident<T, 8, 1>
+                              : (const void *)k_cg_resident<T, 16, 1>;
+        else
+            fn = res_zt_ == 8 ? (const void *)k_cg_resident<T, 8, 0>
+                              : (const void *)k_cg_resident<T, 16, 0>;
+        HIPT(hipLaunchCooperativeKernel(fn, dim3(tl.nb), cell_blk_, args, 0, st_));
+        if (timer.enabled) {
+            e1 = timer.get();
+            HIPT(hipEventRecord(e1, st_));
+            timer.recs.push_back({2, step_index, 0, e0, e1});
+        }
+        last_kind_ = AETHER_SOLVER_CG;
+    }
+
+    void check_grid_sync() {
+        if (!res_zt_) return;
+        GridSync gs;
+        HIPT(hipMemcpyAsync(&gs, gsync_.p, sizeof(gs), hipMemcpyDeviceToHost, st_));
+        sync();
+        if (gs.error) raise(AETHER_EHIP, "resident CG: grid barrier timed out");
     }
 
     ~Solver() override {
@@ -462,7 +553,10 @@ class Solver final : public SolverBase {
                                   g_, tl_, (T)h_, 1, cur_[0], cur_[1], cur_[2], p_.p, b_.p,
                                   r_.p, partB_.p, partC_.p, cg_.p)));
             LAUNCH_CHECK();
-            enqueue_cg_loop(params.solver_iters, step_index);
+            if (res_zt_)
+                enqueue_cg_resident(params.solver_iters, step_index);
+            else
+                enqueue_cg_loop(params.solver_iters, step_index);
         } else {
             k_divergence<T><<<red_grid_, cell_blk_, 0, st_>>>(g_, zchunk_, (T)h_, (T)(1.0 / h_),
                                                                1, cur_[0], cur_[1], cur_[2],
@@ -478,6 +572,7 @@ class Solver final : public SolverBase {
     void project() override {
         enqueue_project(-1);
         sync();
+        check_grid_sync();
     }
 
     void solve_cg(int iters) override {
@@ -486,8 +581,12 @@ class Solver final : public SolverBase {
                               g_, tl_, (T)h_, 0, cur_[0], cur_[1], cur_[2], p_.p, b_.p, r_.p,
                               partB_.p, partC_.p, cg_.p)));
         LAUNCH_CHECK();
-        enqueue_cg_loop(iters, -1);
+        if (res_zt_)
+            enqueue_cg_resident(iters, -1);
+        else
+            enqueue_cg_loop(iters, -1);
         sync();
+        check_grid_sync();
     }
 
     void solve_rbgs(int sweeps) override {
@@ -623,13 +722,14 @@ class Solver final : public SolverBase {
         sync();
         if (out)
             for (int s = 0; s < nsteps; ++s) unpack(raw.data() + (size_t)s * 8, out + s);
+        check_grid_sync();
         if (params.solver_kind == AETHER_SOLVER_RBGS)
             std::fill(iters.begin(), iters.end(), params.solver_iters);
         last_run_iters_ = iters;
         if (timer.enabled) {
             // count only launches that did work: iterations < iters of that step
             for (const auto &r : timer.recs) {
-                if (r.it >= iters[r.step]) continue;
+                if (r.which < 2 && r.it >= iters[r.step]) continue;
                 float ms = 0;
                 HIPT(hipEventElapsedTime(&ms, r.a, r.b));
                 timer.sum_ms[r.which] += ms;
@@ -740,6 +840,12 @@ class Solver final : public SolverBase {
     int *done_host_ = nullptr; // mapped pinned word the device sets to the finished solve id
     int *done_dev_ = nullptr;
     std::vector<hipEvent_t> chunk_events_;
+    // resident (single cooperative launch) CG: tiling, eligibility
+    Tiling tl_res_;
+    int res_zt_ = 0;  // 0: not usable on this grid/device, else cells per thread
+    int barrier_mode_ = 0;  // grid barrier of the resident CG (see cg_resident.hpp)
+    DevBuf<double> partD_;
+    DevBuf<GridSync> gsync_;
     DevBuf<T> f_[5], alt_[5];
     T *cur_[5], *nxt_[5];  // u, v, w, density, temperature (current / scratch)
     DevBuf<T> fwd_[2], curl_, p_, b_, r_, d_[2];
@@ -1081,7 +1187,8 @@ int aether_fluid_set_kernel_timing(aether_fluid_t s, int enable) {
 int aether_fluid_kernel_timing(aether_fluid_t s, int which, double *avg_ms, long *launches) {
     return guarded([&] {
         checked(s);
-        if (which < 0 || which > 1) raise(AETHER_EINVAL, "which must be 0 (cg_dir) or 1 (cg_update)");
+        if (which < 0 || which > 2)
+            raise(AETHER_EINVAL, "which must be 0 (cg_dir), 1 (cg_update) or 2 (cg_resident)");
         const auto &t = s->impl->timer;
         if (avg_ms) *avg_ms = t.count[which] ? t.sum_ms[which] / t.count[which] : 0.0;
         if (launches) *launches = t.count[which];

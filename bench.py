@@ -60,25 +60,31 @@ def time_gpu(solver_kind: str, steps: int, warmup: int):
 
 
 def kernel_roofline(solver_kind: str, nvox: int, steps: int = 2):
-    """Average device time of the two CG kernels over `steps` timed steps
-    (HIP events around each launch on the solver's stream)."""
+    """Average device time of the CG kernels over `steps` steps, from HIP
+    events recorded around each launch on the solver's stream.
+
+    Algorithmic bytes (FP32 array passes of N cells): cg_dir reads r, d_old
+    and writes d (3 passes); cg_update reads p, d, r and writes p, r (5
+    passes); the register-resident solve does the same work in one launch,
+    8 passes per CG iteration it runs."""
     cfg, s = make_solver(solver_kind)
     s.set_kernel_timing(True)
     s.run(cfg["dt"], [cfg["dt"] * i for i in range(steps)])
-    dir_ms, dir_n = s.kernel_timing(0)
-    upd_ms, upd_n = s.kernel_timing(1)
+    iters = s.run_iters()
+    kern = {}
+    for which, name, passes in ((0, "cg_dir", 3), (1, "cg_update", 5), (2, "cg_resident", None)):
+        avg, n = s.kernel_timing(which)
+        if n == 0:
+            continue
+        if passes is None:
+            bytes_ = 8 * 4.0 * nvox * (sum(iters) / n)  # mean iterations per launch
+        else:
+            bytes_ = passes * 4.0 * nvox
+        kern[name] = {"avg_ms": avg, "launches": n, "bytes": bytes_, "total_ms": avg * n,
+                      "gbps": bytes_ / (avg * 1e-3) / 1e9 if avg > 0 else 0.0}
     s.set_kernel_timing(False)
-    # algorithmic bytes per launch: cg_dir reads r, d_old, writes d (3 FP32
-    # passes); cg_update reads p, d, r and writes p, r (5 FP32 passes)
-    kern = {
-        "cg_dir": {"avg_ms": dir_ms, "launches": dir_n, "bytes": 3 * 4.0 * nvox},
-        "cg_update": {"avg_ms": upd_ms, "launches": upd_n, "bytes": 5 * 4.0 * nvox},
-    }
-    for k in kern.values():
-        k["total_ms"] = k["avg_ms"] * k["launches"]
-        k["gbps"] = k["bytes"] / (k["avg_ms"] * 1e-3) / 1e9 if k["avg_ms"] > 0 else 0.0
     dom = max(kern, key=lambda k: kern[k]["total_ms"])
-    return dom, kern
+    return dom, kern, iters
 
 
 def load_traffic(kernel: str):
@@ -155,12 +161,13 @@ def main():
     # RBGS variant of the same workload (omega 1.8, 60 sweeps)
     _, _, el_rb, _, _ = time_gpu("rbgs", args.steps, args.warmup)
 
-    dom, kern = kernel_roofline("cg", nvox)
+    dom, kern, timed_iters = kernel_roofline("cg", nvox)
     kd = kern[dom]
     roofline = {"bound": "hbm", "kernel": dom, "achieved": kd["gbps"], "peak": HBM_PEAK_GBPS,
                 "unit": "GB/s", "frac": kd["gbps"] / HBM_PEAK_GBPS,
                 "traffic": load_traffic(dom), "avg_launch_ms": kd["avg_ms"],
                 "algorithmic_bytes_per_launch": kd["bytes"], "launches_timed": kd["launches"],
+                "cg_iters_timed": timed_iters,
                 "other_kernels": {k: {"avg_ms": v["avg_ms"], "gbps": v["gbps"]}
                                   for k, v in kern.items() if k != dom}}
 

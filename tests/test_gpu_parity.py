@@ -273,3 +273,54 @@ def test_invalid_arguments_raise():
         s.upload("u", np.zeros(7))
     with pytest.raises(af.AetherFluidError):
         s.advect_maccormack_in_place(7, 0.1, 1.0)
+
+
+def _step_128(resident: bool, solver="cg"):
+    import os
+    import aether_fluid as af
+    from aether_fluid.scenarios import SMOKE_CONFIGS, configure, seeded_config_state
+    os.environ["AETHER_CG_RESIDENT"] = "1" if resident else "0"
+    try:
+        cfg = SMOKE_CONFIGS["plain_128"]
+        s = af.FluidSolver3D(*cfg["dims"], h=cfg["h"], precision="fp32")
+    finally:
+        os.environ.pop("AETHER_CG_RESIDENT", None)
+    configure(s, "plain_128", solver)
+    for n, a in seeded_config_state("plain_128").items():
+        s.upload(n, a)
+    s.run(cfg["dt"], [0.0, cfg["dt"]])
+    return s
+
+
+def test_resident_cg_matches_streaming_cg_128():
+    """The single-launch register-resident CG and the two-kernel streaming CG
+    run the same iterations on the full 128^3 workload (2 steps)."""
+    _announce("resident vs streaming CG 128^3")
+    a = _step_128(True)
+    b = _step_128(False)
+    assert a.run_iters() == b.run_iters() or all(
+        abs(x - y) <= 1 for x, y in zip(a.run_iters(), b.run_iters()))
+    for n in ("u", "v", "w", "pressure", "density"):
+        err = rel_l2(a.download(n), b.download(n))
+        assert err <= 1e-5, (n, err)
+    sa, sb = a.last_solve_stats(), b.last_solve_stats()
+    assert abs(sa["r2"] - sb["r2"]) <= 1e-3 * sb["r2"]
+
+
+def test_full_size_128_projection_properties():
+    """Size-independent checks at 128^3: the recursive residual reached the
+    reported value, the projected field's divergence equals -r/h^2 per cell
+    (so max|div u| = max|r|/h^2), density stays in [0, 1], and a repeat run
+    is bitwise identical."""
+    _announce("128^3 properties")
+    s = _step_128(True)
+    st = s.last_solve_stats()
+    h = s.h
+    div_max = s.divergence_max()
+    assert st["iters"] > 0 and st["r2"] < st["r2_init"]
+    # |div u|_max <= |r|_2 / h^2 (a cell value cannot exceed the 2-norm)
+    assert div_max <= (st["r2"] ** 0.5) / (h * h) * 1.0001 + 1e-6
+    d = s.download("density")
+    assert d.min() >= 0.0 and d.max() <= 1.0 + 1e-6
+    t = _step_128(True)
+    np.testing.assert_array_equal(s.download("pressure"), t.download("pressure"))
