@@ -7,6 +7,7 @@
 // FluidSolver3D::step (src/fluid.rs:598-611); the CG driver keeps every
 // scalar (alpha, beta, r.r, the stop flag) on the device, so a solve is a
 // fixed sequence of launches with no host round trip.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -466,11 +467,31 @@ class Solver final : public SolverBase {
 
     // CG iterations (src/fluid.rs:463-487) as two launches each; every
     // launch returns at once after the device stop flag is set.
+    // Launch with start/stop events stamped by the dispatch itself (kernel
+    // execution only, no queueing gap): the kernel-timing mode.  Every
+    // argument must have exactly the kernel parameter's type.
+    template <typename... Args>
+    void launch_stamped(const void *fn, hipEvent_t e0, hipEvent_t e1, Args... args) {
+        void *ptrs[] = {(void *)&args...};
+        HIPT(hipExtLaunchKernel(fn, dim3(tl_.nb), cell_blk_, ptrs, 0, st_, e0, e1, 0));
+    }
+
     void enqueue_cg_iteration(int it, double rel2, int solve_id, int step_index) {
-        hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+        const double *pB = partB_.p, *pC = partC_.p, *pA = partA_.p;
+        const T *dold = d_[(it + 1) & 1].p, *dcur = d_[it & 1].p, *rr = r_.p;
         if (timer.enabled) {
-            e0 = timer.get();
-            HIPT(hipEventRecord(e0, st_));
+            hipEvent_t e0 = timer.get(), e1 = timer.get(), e2 = timer.get(), e3 = timer.get();
+            ZU_DISPATCH(zu_, (launch_stamped(
+                                 it == 0 ? (const void *)k_cg_dir<T, ZU, true>
+                                         : (const void *)k_cg_dir<T, ZU, false>,
+                                 e0, e1, g_, tl_, it, rel2, pB, pC, cg_.p, rr, dold,
+                                 d_[it & 1].p, partA_.p, done_dev_, solve_id)));
+            ZU_DISPATCH(zu_, (launch_stamped((const void *)k_cg_update<T, ZU>, e2, e3, g_, tl_,
+                                             it, pA, cg_.p, dcur, p_.p, r_.p, partB_.p,
+                                             done_dev_, solve_id)));
+            timer.recs.push_back({0, step_index, it, e0, e1});
+            timer.recs.push_back({1, step_index, it, e2, e3});
+            return;
         }
         if (it == 0)
             ZU_DISPATCH(zu_, (k_cg_dir<T, ZU, true><<<tl_.nb, cell_blk_, 0, st_>>>(
@@ -483,20 +504,10 @@ class Solver final : public SolverBase {
                                   d_[(it + 1) & 1].p, d_[it & 1].p, partA_.p, done_dev_,
                                   solve_id)));
         LAUNCH_CHECK();
-        if (timer.enabled) {
-            e1 = timer.get();
-            HIPT(hipEventRecord(e1, st_));
-        }
         ZU_DISPATCH(zu_, (k_cg_update<T, ZU><<<tl_.nb, cell_blk_, 0, st_>>>(
                               g_, tl_, it, partA_.p, cg_.p, d_[it & 1].p, p_.p, r_.p, partB_.p,
                               done_dev_, solve_id)));
         LAUNCH_CHECK();
-        if (timer.enabled) {
-            e2 = timer.get();
-            HIPT(hipEventRecord(e2, st_));
-            timer.recs.push_back({0, step_index, it, e0, e1});
-            timer.recs.push_back({1, step_index, it, e1, e2});
-        }
     }
 
     // CG iterations (src/fluid.rs:463-487), two launches each.  Every scalar
