@@ -198,7 +198,8 @@ class SolverBase {
 template <typename T>
 class Solver final : public SolverBase {
    public:
-    Solver(int nx, int ny, int nz, double h, hipStream_t stream) : h_(h), st_(stream) {
+    Solver(int nx, int ny, int nz, double h, hipStream_t stream, bool ir64 = false)
+        : h_(h), st_(stream), ir64_(ir64) {
         g_.nx = nx;
         g_.ny = ny;
         g_.nz = nz;
@@ -257,12 +258,19 @@ class Solver final : public SolverBase {
         partA_.alloc(npart);
         partB_.alloc(npart);
         partC_.alloc(npart);
+        partD64_.alloc(npart);
         setup_resident(gx, gy, nz);
         part_tel_.alloc((size_t)nb_ * kTelParts);
         scalar_.alloc(8);
         tel_.alloc(8);
         cg_.alloc(1);
         iters_log_.alloc(1);
+        if (ir64_) {
+            p64_.alloc(n_);
+            b64_.alloc(n_);
+            fill64(p64_.p);
+            fill64(b64_.p);
+        }
         for (int a = 0; a < 5; ++a) zero(cur_[a]);
         zero(p_.p);
         zero(b_.p);
@@ -366,6 +374,11 @@ class Solver final : public SolverBase {
         if (done_host_) (void)hipHostFree(done_host_);
     }
 
+    void fill64(double *x) {
+        k_fill<double><<<blocks_for(n_), kThreads, 0, st_>>>(x, (long)n_, 0.0);
+        LAUNCH_CHECK();
+    }
+
     void zero(T *x) {
         k_fill<T><<<blocks_for(n_), kThreads, 0, st_>>>(x, (long)n_, T(0));
         LAUNCH_CHECK();
@@ -462,7 +475,8 @@ class Solver final : public SolverBase {
     }
 
     double rel_tol2() const {
-        return params.cg_rel_tol > 0.0 ? params.cg_rel_tol * params.cg_rel_tol : 0.0;
+        const double t = rel_override_ >= 0.0 ? rel_override_ : params.cg_rel_tol;
+        return t > 0.0 ? t * t : 0.0;
     }
 
     // CG iterations (src/fluid.rs:463-487) as two launches each; every
@@ -556,7 +570,54 @@ class Solver final : public SolverBase {
         last_sweeps_ = sweeps;
     }
 
+    // Mixed-precision projection: b and p in FP64, the FP32 CG solves the
+    // correction equation A e = r to ir_inner_rel_tol, p += e, until the FP64
+    // residual gives max|div u| = max|b - A p| / h^2 <= ir_div_tol.  One
+    // host synchronisation per refinement sweep reads max|r|.
+    void project_ir(int step_index) {
+        k_div64<T><<<cell_grid_, cell_blk_, 0, st_>>>(g_, h_, cur_[0], cur_[1], cur_[2], b64_.p);
+        LAUNCH_CHECK();
+        const double inv_h2 = 1.0 / (h_ * h_);
+        int outer = 0, inner = 0;
+        double rmax = 0.0;
+        for (;; ++outer) {
+            k_residual64<T><<<red_grid_, cell_blk_, 0, st_>>>(g_, zchunk_, b64_.p, p64_.p, b_.p,
+                                                              partA_.p, partD64_.p);
+            LAUNCH_CHECK();
+            k_partials_final<<<1, kThreads, 0, st_>>>(partA_.p, nb_, 1, scalar_.p);
+            LAUNCH_CHECK();
+            rmax = read_scalar();
+            if (rmax * inv_h2 <= params.ir_div_tol || outer >= params.ir_max_outer) break;
+            zero(p_.p);  // inner unknown e starts at 0
+            rel_override_ = params.ir_inner_rel_tol;
+            ZU_DISPATCH(zu_, (k_div_cginit<T, ZU><<<tl_.nb, cell_blk_, 0, st_>>>(
+                                  g_, tl_, (T)h_, 0, cur_[0], cur_[1], cur_[2], p_.p, b_.p, r_.p,
+                                  partB_.p, partC_.p, cg_.p)));
+            LAUNCH_CHECK();
+            enqueue_cg_loop(params.solver_iters, -1);
+            rel_override_ = -1.0;
+            CgState cs;
+            HIPT(hipMemcpyAsync(&cs, cg_.p, sizeof(cs), hipMemcpyDeviceToHost, st_));
+            k_add64<T><<<blocks_for(n_), kThreads, 0, st_>>>(p_.p, (long)n_, p64_.p);
+            LAUNCH_CHECK();
+            sync();
+            inner += cs.iters;
+        }
+        k_gradient64_bc<T><<<cell_grid_, cell_blk_, 0, st_>>>(g_, 1.0 / h_, p64_.p, cur_[0],
+                                                               cur_[1], cur_[2]);
+        LAUNCH_CHECK();
+        ir_outer_ = outer;
+        ir_inner_ = inner;
+        ir_div_ = rmax * inv_h2;
+        if (step_index >= 0) ir_log_.push_back(inner);
+        last_kind_ = AETHER_SOLVER_CG;
+    }
+
     void enqueue_project(int step_index) {
+        if (ir64_) {
+            project_ir(step_index);
+            return;
+        }
         // src/fluid.rs:496-529; the opening wall BC is applied through masked
         // reads in the divergence, the closing one in the gradient kernel
         if (params.solver_kind == AETHER_SOLVER_CG) {
@@ -722,6 +783,7 @@ class Solver final : public SolverBase {
         if (tel_ring_.n < (size_t)nsteps * 8) tel_ring_.alloc((size_t)nsteps * 8);
         if (iters_dev_.n < (size_t)nsteps) iters_dev_.alloc((size_t)nsteps);
         if (timer.enabled) timer.reset();
+        ir_log_.clear();
         for (int s = 0; s < nsteps; ++s)
             enqueue_step(dt, times[s], tel_ring_.p + (size_t)s * 8, s);
         std::vector<double> raw((size_t)nsteps * 8);
@@ -734,6 +796,7 @@ class Solver final : public SolverBase {
         if (out)
             for (int s = 0; s < nsteps; ++s) unpack(raw.data() + (size_t)s * 8, out + s);
         check_grid_sync();
+        if (ir64_) iters.assign(ir_log_.begin(), ir_log_.end());
         if (params.solver_kind == AETHER_SOLVER_RBGS)
             std::fill(iters.begin(), iters.end(), params.solver_iters);
         last_run_iters_ = iters;
@@ -766,7 +829,19 @@ class Solver final : public SolverBase {
         }
     }
 
+    double *field64(int field) {
+        if (!ir64_) return nullptr;
+        if (field == AETHER_FIELD_PRESSURE) return p64_.p;
+        if (field == AETHER_FIELD_DIVERGENCE) return b64_.p;
+        return nullptr;
+    }
+
     void upload(int field, const double *src) override {
+        if (double *d64 = field64(field)) {
+            HIPT(hipMemcpyAsync(d64, src, n_ * sizeof(double), hipMemcpyHostToDevice, st_));
+            sync();
+            return;
+        }
         T *dst = field_ptr(field);
         if constexpr (sizeof(T) == 8) {
             HIPT(hipMemcpyAsync(dst, src, n_ * sizeof(double), hipMemcpyHostToDevice, st_));
@@ -782,6 +857,11 @@ class Solver final : public SolverBase {
     }
 
     void download(int field, double *dst) override {
+        if (double *d64 = field64(field)) {
+            HIPT(hipMemcpyAsync(dst, d64, n_ * sizeof(double), hipMemcpyDeviceToHost, st_));
+            sync();
+            return;
+        }
         T *src = field_ptr(field);
         if constexpr (sizeof(T) == 8) {
             HIPT(hipMemcpyAsync(dst, src, n_ * sizeof(double), hipMemcpyDeviceToHost, st_));
@@ -833,6 +913,12 @@ class Solver final : public SolverBase {
         out->b_norm2 = cs.b2;
         out->r_norm2_initial = cs.r2_init;
         out->r_norm2_final = cs.r2_final;
+        if (ir64_) {
+            out->iterations = ir_inner_;
+            out->outer_iterations = ir_outer_;
+            out->div_max_final = ir_div_;
+            out->converged = ir_div_ <= params.ir_div_tol;
+        }
     }
 
     const std::vector<int> &last_run_iters() const override { return last_run_iters_; }
@@ -846,6 +932,12 @@ class Solver final : public SolverBase {
     int zchunk_, nb_;
     Tiling tl_;
     int zu_ = 4;
+    bool ir64_ = false;           // mixed-precision projection (AETHER_FP32_IR64)
+    DevBuf<double> p64_, b64_, partD64_;
+    double rel_override_ = -1.0;  // inner-solve tolerance while refining
+    int ir_outer_ = 0, ir_inner_ = 0;
+    double ir_div_ = 0.0;
+    std::vector<int> ir_log_;
     int cg_chunk_ = 8;         // CG iterations per host look-ahead chunk (0: no look-ahead)
     int solve_seq_ = 0;        // id of the current pressure solve
     int *done_host_ = nullptr; // mapped pinned word the device sets to the finished solve id
@@ -938,8 +1030,8 @@ int aether_fluid_create(int nx, int ny, int nz, double h, int precision, aether_
             raise(AETHER_EINVAL, "grid dimensions must be >= 6 (src/fluid.rs:54)");
         if ((double)nx * ny * nz >= 2147483647.0)
             raise(AETHER_EINVAL, "grid has more than 2^31-1 cells");
-        if (precision != AETHER_FP32 && precision != AETHER_FP64)
-            raise(AETHER_EINVAL, "precision must be AETHER_FP32 or AETHER_FP64");
+        if (precision != AETHER_FP32 && precision != AETHER_FP64 && precision != AETHER_FP32_IR64)
+            raise(AETHER_EINVAL, "precision must be AETHER_FP32, AETHER_FP64 or AETHER_FP32_IR64");
         int ndev = 0;
         if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
             raise(AETHER_ENODEV, "no HIP device available");
@@ -955,7 +1047,8 @@ int aether_fluid_create(int nx, int ny, int nz, double h, int precision, aether_
         if (precision == AETHER_FP64)
             s->impl = std::make_unique<Solver<double>>(nx, ny, nz, s->h, s->stream);
         else
-            s->impl = std::make_unique<Solver<float>>(nx, ny, nz, s->h, s->stream);
+            s->impl = std::make_unique<Solver<float>>(nx, ny, nz, s->h, s->stream,
+                                                      precision == AETHER_FP32_IR64);
         *out = s.release();
     });
 }

@@ -981,6 +981,107 @@ __global__ __launch_bounds__(kThreads) void k_gradient_bc(Dims g, T inv_h,
 }
 
 // ---------------------------------------------------------------------------
+// Mixed-precision projection: FP64 pressure and right-hand side around an
+// FP32 inner CG (iterative refinement).  Velocities stay FP32.
+// ---------------------------------------------------------------------------
+
+// b64 = -h div(masked u) in FP64 (src/fluid.rs:346 with the BC of :497).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_div64(Dims g, double h, const T *__restrict__ u,
+                                                    const T *__restrict__ v,
+                                                    const T *__restrict__ w,
+                                                    double *__restrict__ b64) {
+    const int i = blockIdx.x * kBX + threadIdx.x;
+    const int j = blockIdx.y * kBY + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= g.nx || j >= g.ny) return;
+    const int id = (k * g.ny + j) * g.nx + i;
+    double b = 0.0;
+    if (is_interior(g, i, j, k)) {
+        // differences of the stored values are formed in FP64
+        const double u1 = bc_u(g, i + 1, j, k) ? 0.0 : (double)u[id + 1];
+        const double u0 = bc_u(g, i, j, k) ? 0.0 : (double)u[id];
+        const double v1 = bc_v(g, i, j + 1, k) ? 0.0 : (double)v[id + g.nx];
+        const double v0 = bc_v(g, i, j, k) ? 0.0 : (double)v[id];
+        const double w1 = bc_w(g, i, j, k + 1) ? 0.0 : (double)w[id + g.sxy];
+        const double w0 = bc_w(g, i, j, k) ? 0.0 : (double)w[id];
+        b = -h * ((u1 - u0) + (v1 - v0) + (w1 - w0));
+    }
+    b64[id] = b;
+}
+
+// r = b64 - A p64 in FP64; r32 = (float) r feeds the inner FP32 CG; one
+// partial max|r| and one partial r.r per workgroup.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_residual64(Dims g, int zchunk,
+                                                         const double *__restrict__ b64,
+                                                         const double *__restrict__ p64,
+                                                         T *__restrict__ r32,
+                                                         double *__restrict__ part_max,
+                                                         double *__restrict__ part_sum) {
+    __shared__ double lds[8];
+    const int i = blockIdx.x * kBX + threadIdx.x;
+    const int j = blockIdx.y * kBY + threadIdx.y;
+    const int kb = blockIdx.z * zchunk;
+    const int ke = min(kb + zchunk, g.nz);
+    double m = 0.0, acc[1] = {0.0};
+    if (i < g.nx && j < g.ny) {
+        for (int k = kb; k < ke; ++k) {
+            const int id = (k * g.ny + j) * g.nx + i;
+            double r = 0.0;
+            if (is_interior(g, i, j, k))
+                r = b64[id] - lap_at<double>(g, i, j, k, id, [&](int a) { return p64[a]; });
+            r32[id] = (T)r;
+            m = fmax(m, fabs(r));
+            acc[0] += r * r;
+        }
+    }
+    m = block_max(m, lds);
+    block_sum<1>(acc, lds);
+    if (tid_lin() == 0) {
+        const int bid = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+        part_max[bid] = m;
+        part_sum[bid] = acc[0];
+    }
+}
+
+// p64 += e (the inner solve's correction)
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_add64(const T *__restrict__ e, long n,
+                                                    double *__restrict__ p64) {
+    for (long a = (long)blockIdx.x * kThreads + tid_lin(); a < n; a += (long)gridDim.x * kThreads)
+        p64[a] += (double)e[a];
+}
+
+// Gradient subtraction with the FP64 pressure, rounded once to the storage
+// type, then the closing wall masks (src/fluid.rs:509-528).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_gradient64_bc(Dims g, double inv_h,
+                                                            const double *__restrict__ p,
+                                                            T *__restrict__ u,
+                                                            T *__restrict__ v,
+                                                            T *__restrict__ w) {
+    const int i = blockIdx.x * kBX + threadIdx.x;
+    const int j = blockIdx.y * kBY + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= g.nx || j >= g.ny) return;
+    const int id = (k * g.ny + j) * g.nx + i;
+    T uu = u[id], vv = v[id], ww = w[id];
+    if (is_interior(g, i, j, k)) {
+        const double pc = p[id];
+        if (i > 1) uu = (T)((double)uu - (pc - p[id - 1]) * inv_h);
+        if (j > 1) vv = (T)((double)vv - (pc - p[id - g.nx]) * inv_h);
+        if (k > 1) ww = (T)((double)ww - (pc - p[id - g.sxy]) * inv_h);
+    }
+    if (bc_u(g, i, j, k)) uu = T(0);
+    if (bc_v(g, i, j, k)) vv = T(0);
+    if (bc_w(g, i, j, k)) ww = T(0);
+    u[id] = uu;
+    v[id] = vv;
+    w[id] = ww;
+}
+
+// ---------------------------------------------------------------------------
 // Telemetry (src/fluid.rs:613-672), FP64 accumulation.
 // Partial layout per workgroup: ke, enstrophy, mass, cx, cy, cz, peak.
 // ---------------------------------------------------------------------------

@@ -324,3 +324,65 @@ def test_full_size_128_projection_properties():
     assert d.min() >= 0.0 and d.max() <= 1.0 + 1e-6
     t = _step_128(True)
     np.testing.assert_array_equal(s.download("pressure"), t.download("pressure"))
+
+
+def _mixed_state(dims, seed=0xAE7E0002):
+    from aether_fluid.scenarios import seeded_config_state
+    return seeded_config_state("mixed_256", dims=dims, seed=seed)
+
+
+def test_mixed_precision_projection_vs_oracle():
+    """FP32 fields + FP64 iterative refinement (inner FP32 CG at 1e-3)
+    reaches max|div u| <= 1e-7 in the FP64 residual and matches the f64
+    oracle's tightly converged projection."""
+    _announce("mixed precision projection 40^3")
+    dims = (40, 40, 40)
+    st = _mixed_state(dims)
+    gpu, orc = make_pair(*dims, h=1.0 / 40, precision="fp32_ir64")
+    for n, a in st.items():
+        gpu.upload(n, a)
+        orc.field(n)[...] = a.reshape(40, 40, 40)
+    gpu.solver_kind, gpu.solver_iters = 0, 400
+    gpu.ir_inner_rel_tol, gpu.ir_div_tol, gpu.ir_max_outer = 1e-3, 1e-7, 30
+    orc.solver_kind, orc.solver_iters, orc.cg_rel_tol = 0, 5000, 1e-15
+    gpu.project()
+    orc.project()
+    s = gpu.last_solve_stats()
+    print("ir stats", s)
+    assert s["converged"] and s["div_max"] <= 1e-7 and s["outer"] >= 2
+    errs = compare_fields(gpu, orc, ("u", "v", "w"))
+    assert max(errs.values()) <= 1e-6, errs
+    p_g, p_o = gpu.download("pressure"), orc.field("pressure").ravel()
+    inside = np.zeros((40, 40, 40), bool)
+    inside[1:-1, 1:-1, 1:-1] = True
+    inside = inside.ravel()
+    dg, do = p_g[inside] - p_g[inside].mean(), p_o[inside] - p_o[inside].mean()
+    # p is fixed by the solve to ~ max|r| relative to |b|: a few 1e-8
+    assert rel_l2(dg, do) <= 1e-7
+
+
+def test_mixed_precision_matches_fp64_projection_steps():
+    """Three steps: the refined FP32 path against the all-FP64 GPU path run
+    to the same divergence (FP64 CG at relative tolerance 1e-14)."""
+    _announce("mixed vs fp64 steps 32^3")
+    import aether_fluid as af
+    dims = (32, 32, 32)
+    st = _mixed_state(dims, seed=7)
+    runs = {}
+    for prec in ("fp32_ir64", "fp64"):
+        s = af.FluidSolver3D(*dims, h=1.0 / 32, precision=prec)
+        for n, a in st.items():
+            s.upload(n, a)
+        s.gravity_alpha, s.buoyancy_beta, s.vorticity_eps = 0.05, 1.0, 0.3
+        s.set_sphere_source((16.0, 3.0, 16.0), 3.0)
+        s.solver_kind, s.solver_iters = 0, 2000
+        if prec == "fp64":
+            s.cg_rel_tol = 1e-14
+        else:
+            s.ir_inner_rel_tol, s.ir_div_tol = 1e-3, 1e-7
+        s.run(0.005, [0.0, 0.005, 0.01])
+        runs[prec] = (s, s.last_solve_stats())
+    a, b = runs["fp32_ir64"][0], runs["fp64"][0]
+    assert runs["fp32_ir64"][1]["div_max"] <= 1e-7
+    for n in ("u", "v", "w", "density"):
+        assert rel_l2(a.download(n), b.download(n)) <= 1e-5, n
