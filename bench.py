@@ -128,6 +128,50 @@ def cpu_baseline(seconds_budget: float = 20.0):
     return out
 
 
+def bench_mixed(steps: int, warmup: int) -> dict:
+    """256^3, dt 0.005: FP32 advection/stencils with the FP64-refined FP32
+    CG projection (inner tol 1e-3, refine to max|div u| <= 1e-7), against
+    the all-FP64 GPU projection (FP64 CG to relative tolerance 1e-14)."""
+    import aether_fluid as af
+    from aether_fluid.scenarios import SMOKE_CONFIGS, seeded_config_state
+    cfg = SMOKE_CONFIGS["mixed_256"]
+    nvox = int(np.prod(cfg["dims"]))
+    state = seeded_config_state("mixed_256")
+    out = {"workload": "mixed-precision projection 256^3", "steps": steps, "warmup": warmup}
+    for prec in ("fp32_ir64", "fp64"):
+        s = af.FluidSolver3D(*cfg["dims"], h=cfg["h"], precision=prec)
+        for n, a in state.items():
+            s.upload(n, a)
+        s.gravity_alpha, s.buoyancy_beta, s.vorticity_eps = cfg["alpha"], cfg["beta"], cfg["eps"]
+        s.set_sphere_source(cfg["source"]["center"], cfg["source"]["radius"])
+        s.solver_kind, s.solver_iters = 0, 4000
+        if prec == "fp64":
+            s.cg_rel_tol = 1e-14
+        else:
+            s.solver_iters = 1000
+            s.ir_inner_rel_tol, s.ir_div_tol, s.ir_max_outer = 1e-3, 1e-7, 30
+        dt = cfg["dt"]
+        times = [dt * i for i in range(warmup + steps)]
+        if warmup:
+            s.run(dt, times[:warmup])
+        t0 = time.perf_counter()
+        s.run(dt, times[warmup:])
+        el = time.perf_counter() - t0
+        st = s.last_solve_stats()
+        h = cfg["h"]
+        res_div = st["div_max"] if prec != "fp64" else (st["r2"] ** 0.5) / (h * h)
+        out[prec] = {"voxel_steps_per_s": nvox * steps / el, "ms_per_step": el * 1e3 / steps,
+                     "cg_iters_per_step": s.run_iters(), "outer_sweeps_last": st["outer"],
+                     "solve_div_max": res_div, "velocity_div_max": s.divergence_max()}
+        out[prec + "_state"] = [s.download(n) for n in ("u", "v", "w", "density")]
+    a, b = out.pop("fp32_ir64_state"), out.pop("fp64_state")
+    out["rel_l2_mixed_vs_fp64"] = {n: float(np.linalg.norm(x - y) / max(np.linalg.norm(y), 1e-300))
+                                   for n, x, y in zip(("u", "v", "w", "density"), a, b)}
+    out["speedup_mixed_over_fp64"] = (out["fp32_ir64"]["voxel_steps_per_s"] /
+                                      out["fp64"]["voxel_steps_per_s"])
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -135,6 +179,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--workload", default="plain_128", choices=["plain_128", "mixed_256"],
+                    help="plain_128: the headline line; mixed_256: mixed-precision "
+                         "projection at 256^3 against the all-FP64 projection")
     ap.add_argument("--probe", action="store_true",
                     help="counter-collection mode: 2 steps of the workload and a 256 MiB "
                          "stream copy, no timing output")
@@ -145,6 +192,9 @@ def main():
         sys.exit(2)
 
     import aether_fluid as af
+    if args.workload == "mixed_256":
+        print(json.dumps(bench_mixed(args.steps, args.warmup)), flush=True)
+        return
     if args.probe:
         _, s = make_solver("cg")
         s.run(SMOKE_DT, [0.0, SMOKE_DT])
