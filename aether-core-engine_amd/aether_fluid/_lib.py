@@ -77,6 +77,17 @@ class SolveStats(C.Structure):
     ]
 
 
+class InstanceDesc(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("reserved", C.c_int32),
+        ("h", C.c_double),
+        ("vorticity_eps", C.c_double), ("buoyancy_beta", C.c_double),
+        ("gravity_alpha", C.c_double),
+        ("source_center", C.c_double * 3), ("source_radius", C.c_double),
+        ("source_density", C.c_double), ("source_temperature", C.c_double),
+    ]
+
+
 # every symbol declared in include/aether_fluid.h: name -> (restype, argtypes)
 _vp = C.c_void_p
 _pd = C.POINTER(C.c_double)
@@ -113,6 +124,14 @@ SIGNATURES = {
     "aether_fluid_upload": (C.c_int, [_vp, C.c_int, _pd, C.c_size_t]),
     "aether_fluid_download": (C.c_int, [_vp, C.c_int, _pd, C.c_size_t]),
     "aether_fluid_max_density": (C.c_int, [_vp, _pd]),
+    "aether_fluid_batch_create": (C.c_int, [C.POINTER(InstanceDesc), C.c_int, C.c_int,
+                                            C.POINTER(Params), C.POINTER(_vp)]),
+    "aether_fluid_batch_destroy": (C.c_int, [_vp]),
+    "aether_fluid_batch_upload": (C.c_int, [_vp, C.c_int, C.c_int, _pd, C.c_size_t]),
+    "aether_fluid_batch_download": (C.c_int, [_vp, C.c_int, C.c_int, _pd, C.c_size_t]),
+    "aether_fluid_batch_run": (C.c_int, [_vp, C.c_double, _pd, C.c_int, C.POINTER(Telemetry)]),
+    "aether_fluid_batch_iterations": (C.c_int, [_vp, C.POINTER(C.c_int32), C.c_int,
+                                                C.POINTER(C.c_int)]),
     "aether_rng_seed": (C.c_int, [C.c_uint64, _pu64]),
     "aether_rng_fill_range": (C.c_int, [_pu64, C.c_double, C.c_double, _pd, C.c_size_t]),
     "aether_fluid_set_kernel_timing": (C.c_int, [_vp, C.c_int]),

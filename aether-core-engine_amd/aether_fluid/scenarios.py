@@ -81,3 +81,27 @@ def configure(solver, name: str, solver_kind: str = "cg") -> None:
         solver.solver_kind = 1
         solver.solver_iters = cfg["rbgs_sweeps"]
         solver.sor_omega = cfg["sor_omega"]
+
+
+BATCH_SIZES = ((32, 32, 32), (48, 48, 48), (64, 64, 64), (96, 96, 96), (128, 96, 128))
+
+
+def batched_instances(count: int = 64, seed: int = 0xAE7E0004):
+    """The batched workload: `count` grids with sizes drawn from BATCH_SIZES,
+    a sphere source (radius max(2, nx/10)) at a uniform position in the
+    bottom quarter, alpha ~ U[0.02, 0.1], eps ~ U[0.1, 0.5], beta = 1,
+    h = 1/max(n), all drawn from the reference's seeded stream."""
+    draws, _ = rng_fill(seed, 0.0, 1.0, 6 * count)
+    out = []
+    for q in range(count):
+        u = draws[6 * q:6 * q + 6]
+        nx, ny, nz = BATCH_SIZES[min(int(u[0] * len(BATCH_SIZES)), len(BATCH_SIZES) - 1)]
+        r = max(2.0, nx / 10.0)
+        cx = (r + 1) + u[1] * (nx - 2 * r - 3)
+        cy = 1.0 + u[2] * (ny / 4.0 - 1.0)
+        cz = (r + 1) + u[3] * (nz - 2 * r - 3)
+        out.append(dict(nx=nx, ny=ny, nz=nz, h=0.0, vorticity_eps=0.1 + 0.4 * u[5],
+                        buoyancy_beta=1.0, gravity_alpha=0.02 + 0.08 * u[4],
+                        source_center=(cx, cy, cz), source_radius=r,
+                        source_density=1.0, source_temperature=1.0))
+    return out

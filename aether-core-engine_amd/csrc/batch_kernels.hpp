@@ -1,0 +1,228 @@
+// batch_kernels.hpp -- many independent smoke grids of different sizes in one
+// launch per stage.
+//
+// Every batched kernel reads a work list of (instance, tile) items; the
+// instance table gives that grid's dimensions, buffers and forcing.  The
+// per-tile work is the same device body as the single-grid kernels, so a
+// batched step computes exactly what the instances would compute one at a
+// time (tested bit for bit).  Each instance keeps its own CG scalars,
+// partial sums and stop flag: a converged instance's tiles return at once
+// while the others keep iterating.
+#pragma once
+#include "fluid_kernels.hpp"
+
+namespace aether {
+
+struct BItem {
+    int inst;       // instance index
+    int bx, by, bz; // tile coordinates inside the instance
+    int local;      // tile index inside the instance (its partial-sum slot)
+    int pad[3];
+};
+
+template <typename T>
+struct BInst {
+    Dims g;
+    double h;
+    T *f[2][5];  // [slot][u, v, w, density, temperature]
+    T *fwd0, *fwd1, *curl, *p, *b, *r, *d[2];
+    double *pa, *pb, *pc, *ptel;  // this instance's partial-sum segments
+    CgState *st;
+    int nb;                // tiles of the instance in the CG work list
+    int pad;
+    double eps, beta, alpha;              // forcing
+    double sc[3], sr2, srho, stemp;       // sphere source (cell-index coordinates)
+    int slo[3], sn[3];                    // sphere bounding box, interior-clamped
+    double *tel;                          // telemetry ring, 8 doubles per step
+    int *iters;                           // CG iterations per step
+};
+
+struct Slots {
+    int u, v, w, rho, t;  // slot (0/1) holding the current value of each field
+};
+
+template <typename T>
+__device__ __forceinline__ void cell_of(const BItem &it, int &i, int &j, int &k) {
+    i = it.bx * kBX + threadIdx.x;
+    j = it.by * kBY + threadIdx.y;
+    k = it.bz;
+}
+
+// fwd = SL(field, +dt) with the current velocity, for one field (NF = 1,
+// field id f0) or density and temperature together (NF = 2).
+template <typename T, int NF>
+__global__ __launch_bounds__(kThreads) void kb_sl_forward(const BInst<T> *__restrict__ I,
+                                                          const BItem *__restrict__ items,
+                                                          double dt, Slots sl, int f0) {
+    const BItem it = items[blockIdx.x];
+    const BInst<T> &in = I[it.inst];
+    int i, j, k;
+    cell_of<T>(it, i, j, k);
+    const int slots[5] = {sl.u, sl.v, sl.w, sl.rho, sl.t};
+    const T *src0 = in.f[slots[f0]][f0];
+    const T *src1 = NF == 2 ? in.f[sl.t][4] : nullptr;
+    body_sl_forward<T, NF>(i, j, k, in.g, (T)(dt / in.h), in.f[sl.u][0], in.f[sl.v][1],
+                           in.f[sl.w][2], src0, src1, in.fwd0, in.fwd1);
+}
+
+// MacCormack correction into the other slot of field f0 (and of
+// temperature when NF = 2).
+template <typename T, int NF>
+__global__ __launch_bounds__(kThreads) void kb_mac_correct(const BInst<T> *__restrict__ I,
+                                                           const BItem *__restrict__ items,
+                                                           double dt, Slots sl, int f0,
+                                                           double diss0, double diss1,
+                                                           int clamp0) {
+    const BItem it = items[blockIdx.x];
+    const BInst<T> &in = I[it.inst];
+    int i, j, k;
+    cell_of<T>(it, i, j, k);
+    const int slots[5] = {sl.u, sl.v, sl.w, sl.rho, sl.t};
+    const int s0 = slots[f0];
+    body_mac_correct<T, NF>(i, j, k, in.g, (T)(dt / in.h), in.f[sl.u][0], in.f[sl.v][1],
+                            in.f[sl.w][2], in.f[s0][f0], in.fwd0, in.f[1 - s0][f0], (T)diss0,
+                            NF == 2 ? in.f[sl.t][4] : nullptr, in.fwd1,
+                            NF == 2 ? in.f[1 - sl.t][4] : nullptr, (T)diss1, clamp0);
+}
+
+// Sphere source: one workgroup per 256 cells of an instance's bounding box.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void kb_sphere(const BInst<T> *__restrict__ I,
+                                                      const int2 *__restrict__ items, Slots sl) {
+    const int2 it = items[blockIdx.x];  // (instance, first box cell)
+    const BInst<T> &in = I[it.x];
+    const int a = it.y + tid_lin();
+    const int ni = in.sn[0], nj = in.sn[1], nk = in.sn[2];
+    if (a >= ni * nj * nk) return;
+    const int i = in.slo[0] + a % ni, j = in.slo[1] + (a / ni) % nj, k = in.slo[2] + a / (ni * nj);
+    const double dx = (double)i - in.sc[0], dy = (double)j - in.sc[1], dz = (double)k - in.sc[2];
+    if (dx * dx + dy * dy + dz * dz < in.sr2) {
+        const int id = (k * in.g.ny + j) * in.g.nx + i;
+        in.f[sl.rho][3][id] = (T)in.srho;
+        in.f[sl.t][4][id] = (T)in.stemp;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void kb_curl_mag(const BInst<T> *__restrict__ I,
+                                                        const BItem *__restrict__ items,
+                                                        Slots sl) {
+    const BItem it = items[blockIdx.x];
+    const BInst<T> &in = I[it.inst];
+    int i, j, k;
+    cell_of<T>(it, i, j, k);
+    body_curl_mag<T>(i, j, k, in.g, (T)(0.5 / in.h), in.f[sl.u][0], in.f[sl.v][1],
+                     in.f[sl.w][2], in.curl);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void kb_confine_buoy_bc(const BInst<T> *__restrict__ I,
+                                                               const BItem *__restrict__ items,
+                                                               double dt, Slots sl) {
+    const BItem it = items[blockIdx.x];
+    const BInst<T> &in = I[it.inst];
+    int i, j, k;
+    cell_of<T>(it, i, j, k);
+    body_confine_buoy_bc<T>(i, j, k, in.g, (T)dt, (T)(0.5 / in.h), (T)(in.eps * in.h),
+                            (T)in.beta, (T)in.alpha, in.f[sl.u][0], in.f[sl.v][1], in.f[sl.w][2],
+                            in.curl, in.f[sl.rho][3], in.f[sl.t][4], in.f[1 - sl.u][0],
+                            in.f[1 - sl.v][1], in.f[1 - sl.w][2]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void kb_gradient_bc(const BInst<T> *__restrict__ I,
+                                                           const BItem *__restrict__ items,
+                                                           Slots sl) {
+    const BItem it = items[blockIdx.x];
+    const BInst<T> &in = I[it.inst];
+    int i, j, k;
+    cell_of<T>(it, i, j, k);
+    body_gradient_bc<T>(i, j, k, in.g, (T)(1.0 / in.h), in.p, in.f[sl.u][0], in.f[sl.v][1],
+                        in.f[sl.w][2]);
+}
+
+template <typename T, int ZU>
+__global__ __launch_bounds__(kThreads) void kb_div_cginit(const BInst<T> *__restrict__ I,
+                                                          const BItem *__restrict__ items,
+                                                          Slots sl) {
+    const BItem it = items[blockIdx.x];
+    const BInst<T> &in = I[it.inst];
+    body_div_cginit<T, ZU>(col_at<ZU>(in.g, it.bx, it.by, it.bz), it.local, in.g, (T)in.h, 1,
+                           in.f[sl.u][0], in.f[sl.v][1], in.f[sl.w][2], in.p, in.b, in.r, in.pb,
+                           in.pc, in.st);
+}
+
+template <typename T, int ZU, bool FIRST>
+__global__ __launch_bounds__(kThreads) void kb_cg_dir(const BInst<T> *__restrict__ I,
+                                                      const BItem *__restrict__ items, int iter,
+                                                      double rel_tol2, int *done_mirror,
+                                                      int solve_id) {
+    const BItem it = items[blockIdx.x];
+    const BInst<T> &in = I[it.inst];
+    body_cg_dir<T, ZU, FIRST>(col_at<ZU>(in.g, it.bx, it.by, it.bz), it.local, in.nb, in.g, iter,
+                              rel_tol2, in.pb, in.pc, in.st, in.r, in.d[(iter + 1) & 1],
+                              in.d[iter & 1], in.pa, done_mirror + it.inst, solve_id);
+}
+
+template <typename T, int ZU>
+__global__ __launch_bounds__(kThreads) void kb_cg_update(const BInst<T> *__restrict__ I,
+                                                         const BItem *__restrict__ items,
+                                                         int iter, int *done_mirror,
+                                                         int solve_id) {
+    const BItem it = items[blockIdx.x];
+    const BInst<T> &in = I[it.inst];
+    body_cg_update<T, ZU>(col_at<ZU>(in.g, it.bx, it.by, it.bz), it.local, in.nb, in.g, iter,
+                          in.pa, in.st, in.d[iter & 1], in.p, in.r, in.pb,
+                          done_mirror + it.inst, solve_id);
+}
+
+// One workgroup per instance: closes solves that ran out of iterations and
+// logs the iteration count of step `step`.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void kb_cg_finalize(const BInst<T> *__restrict__ I,
+                                                           int max_iters, double rel_tol2,
+                                                           int step) {
+    const BInst<T> &in = I[blockIdx.x];
+    __shared__ double lds[8];
+    __shared__ int flag_lds[1];
+    CgState *st = in.st;
+    if (!block_flag(&st->done, flag_lds)) {
+        const double rr = sum_partials(in.pb, in.nb, lds);
+        double bb = 0.0;
+        if (max_iters == 0) bb = sum_partials(in.pc, in.nb, lds);
+        if (tid_lin() == 0) {
+            if (max_iters == 0) {
+                st->b2 = bb;
+                st->r2_init = rr;
+            }
+            st->iters = max_iters;
+            st->r2_final = rr;
+            st->converged = max_iters > 0 && (rr < 1e-20 ||
+                                              (rel_tol2 > 0.0 && rr <= rel_tol2 * st->b2));
+            st->done = 1;
+        }
+    }
+    __syncthreads();
+    if (tid_lin() == 0) in.iters[step] = st->iters;
+}
+
+template <typename T, int ZU>
+__global__ __launch_bounds__(kThreads) void kb_telemetry_part(const BInst<T> *__restrict__ I,
+                                                              const BItem *__restrict__ items,
+                                                              Slots sl) {
+    const BItem it = items[blockIdx.x];
+    const BInst<T> &in = I[it.inst];
+    const int kb = it.bz * ZU;
+    body_telemetry_part<T>(it.bx * kBX + threadIdx.x, it.by * kBY + threadIdx.y, kb,
+                           min(kb + ZU, in.g.nz), it.local, in.g, in.h, in.f[sl.u][0],
+                           in.f[sl.v][1], in.f[sl.w][2], in.f[sl.rho][3], in.ptel);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void kb_telemetry_final(const BInst<T> *__restrict__ I,
+                                                               int step, double time_sec) {
+    const BInst<T> &in = I[blockIdx.x];
+    body_telemetry_final(in.ptel, in.nb, time_sec, in.tel + (size_t)step * 8);
+}
+
+}  // namespace aether

@@ -22,6 +22,7 @@
 #include "aether_fluid.h"
 #include "fluid_kernels.hpp"
 #include "cg_resident.hpp"
+#include "batch_kernels.hpp"
 
 namespace aether {
 namespace {
@@ -960,10 +961,379 @@ class Solver final : public SolverBase {
     std::vector<int> last_run_iters_;
 };
 
+
+// ---------------------------------------------------------------------------
+// Batched smoke grids: every stage is one launch over the tiles of all
+// instances (batch_kernels.hpp).  Instances share dt, the solver knobs and
+// the dissipation; each has its own grid, h, forcing and sphere source.
+// ---------------------------------------------------------------------------
+constexpr int kBatchZU = 8;  // z cells per thread of the batched CG tiling
+
+class BatchBase {
+   public:
+    virtual ~BatchBase() = default;
+    virtual void upload(int inst, int field, const double *src, size_t n) = 0;
+    virtual void download(int inst, int field, double *dst, size_t n) = 0;
+    virtual void run(double dt, const double *times, int nsteps, aether_fluid_telemetry *out) = 0;
+    virtual const std::vector<int> &iters() const = 0;
+    virtual int count() const = 0;
+};
+
+template <typename T>
+class Batch final : public BatchBase {
+    struct Inst {
+        Dims g;
+        double h;
+        size_t n;
+        int gx, gy, gz, nb;
+        aether_fluid_instance_desc d;
+        DevBuf<T> f[2][5], fwd0, fwd1, curl, p, b, r, dd[2];
+        DevBuf<double> pa, pb, pc, ptel, tel, staging;
+        DevBuf<CgState> st;
+        DevBuf<int> iters;
+    };
+
+   public:
+    Batch(const aether_fluid_instance_desc *desc, int count, const aether_fluid_params &shared,
+          hipStream_t stream)
+        : params_(shared), st_(stream) {
+        std::vector<BItem> cells, cg;
+        std::vector<int2> src;
+        for (int q = 0; q < count; ++q) {
+            auto in = std::make_unique<Inst>();
+            const aether_fluid_instance_desc &d = desc[q];
+            in->d = d;
+            in->g.nx = d.nx;
+            in->g.ny = d.ny;
+            in->g.nz = d.nz;
+            in->g.sxy = d.nx * d.ny;
+            in->h = d.h > 0.0 ? d.h : 1.0 / (double)std::max(d.nx, std::max(d.ny, d.nz));
+            in->n = (size_t)d.nx * d.ny * d.nz;
+            in->gx = (d.nx + kBX - 1) / kBX;
+            in->gy = (d.ny + kBY - 1) / kBY;
+            in->gz = (d.nz + kBatchZU - 1) / kBatchZU;
+            in->nb = in->gx * in->gy * in->gz;
+            for (int a = 0; a < 2; ++a)
+                for (int f = 0; f < 5; ++f) in->f[a][f].alloc(in->n);
+            for (DevBuf<T> *b : {&in->fwd0, &in->fwd1, &in->curl, &in->p, &in->b, &in->r,
+                                 &in->dd[0], &in->dd[1]})
+                b->alloc(in->n);
+            in->pa.alloc(in->nb);
+            in->pb.alloc(in->nb);
+            in->pc.alloc(in->nb);
+            in->ptel.alloc((size_t)in->nb * kTelParts);
+            in->st.alloc(1);
+            for (int a = 0; a < 2; ++a)
+                for (int f = 0; f < 5; ++f) fill(in->f[a][f].p, in->n);
+            for (DevBuf<T> *b : {&in->fwd0, &in->fwd1, &in->curl, &in->p, &in->b, &in->r,
+                                 &in->dd[0], &in->dd[1]})
+                fill(b->p, in->n);
+            HIPT(hipMemsetAsync(in->st.p, 0, sizeof(CgState), st_));
+            // cell work items: one per (64 x 4 columns, z plane)
+            for (int k = 0; k < d.nz; ++k)
+                for (int by = 0; by < in->gy; ++by)
+                    for (int bx = 0; bx < in->gx; ++bx) cells.push_back({q, bx, by, k, 0, {0, 0, 0}});
+            // CG tiles, numbered like the single-grid launch (same XCD-slab
+            // order), so each tile's partial sum lands in the same slot
+            const bool remap = in->nb % 8 == 0;
+            for (int b = 0; b < in->nb; ++b) {
+                const int t = remap ? (b & 7) * (in->nb >> 3) + (b >> 3) : b;
+                const int bx = t % in->gx, rest = t / in->gx;
+                cg.push_back({q, bx, rest % in->gy, rest / in->gy, b, {0, 0, 0}});
+            }
+            // sphere source box (interior cells)
+            if (d.source_radius > 0.0) {
+                const int nn[3] = {d.nx, d.ny, d.nz};
+                int lo[3], hi[3];
+                bool empty = false;
+                for (int a = 0; a < 3; ++a) {
+                    lo[a] = std::max(1, (int)std::floor(d.source_center[a] - d.source_radius));
+                    hi[a] = std::min(nn[a] - 2, (int)std::ceil(d.source_center[a] + d.source_radius));
+                    if (hi[a] < lo[a]) empty = true;
+                }
+                if (!empty) {
+                    const int cnt = (hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1);
+                    for (int a = 0; a < cnt; a += kThreads) src.push_back(make_int2(q, a));
+                    for (int a = 0; a < 3; ++a) {
+                        box_lo_.push_back(lo[a]);
+                        box_n_.push_back(hi[a] - lo[a] + 1);
+                    }
+                } else {
+                    for (int a = 0; a < 3; ++a) {
+                        box_lo_.push_back(1);
+                        box_n_.push_back(0);
+                    }
+                }
+            } else {
+                for (int a = 0; a < 3; ++a) {
+                    box_lo_.push_back(1);
+                    box_n_.push_back(0);
+                }
+            }
+            inst_.push_back(std::move(in));
+        }
+        ncell_ = (int)cells.size();
+        ncg_ = (int)cg.size();
+        nsrc_ = (int)src.size();
+        cell_items_.alloc(ncell_);
+        cg_items_.alloc(ncg_);
+        HIPT(hipMemcpyAsync(cell_items_.p, cells.data(), cells.size() * sizeof(BItem),
+                            hipMemcpyHostToDevice, st_));
+        HIPT(hipMemcpyAsync(cg_items_.p, cg.data(), cg.size() * sizeof(BItem),
+                            hipMemcpyHostToDevice, st_));
+        if (nsrc_) {
+            src_items_.alloc(nsrc_);
+            HIPT(hipMemcpyAsync(src_items_.p, src.data(), src.size() * sizeof(int2),
+                                hipMemcpyHostToDevice, st_));
+        }
+        table_.alloc(count);
+        HIPT(hipHostMalloc((void **)&done_host_, sizeof(int) * count, hipHostMallocMapped));
+        for (int q = 0; q < count; ++q) done_host_[q] = 0;
+        HIPT(hipHostGetDevicePointer((void **)&done_dev_, done_host_, 0));
+        HIPT(hipStreamSynchronize(st_));
+    }
+
+    ~Batch() override {
+        (void)hipStreamSynchronize(st_);
+        for (auto e : chunk_events_) (void)hipEventDestroy(e);
+        if (done_host_) (void)hipHostFree(done_host_);
+    }
+
+    int count() const override { return (int)inst_.size(); }
+    const std::vector<int> &iters() const override { return iters_; }
+
+    void fill(T *x, size_t n) {
+        const unsigned b = (unsigned)std::min<size_t>((n + kThreads - 1) / kThreads, 4096);
+        k_fill<T><<<std::max(b, 1u), kThreads, 0, st_>>>(x, (long)n, T(0));
+        LAUNCH_CHECK();
+    }
+
+    T *field(Inst &in, int field) {
+        const int slots[5] = {sl_.u, sl_.v, sl_.w, sl_.rho, sl_.t};
+        if (field >= 0 && field < 5) return in.f[slots[field]][field].p;
+        if (field == AETHER_FIELD_PRESSURE) return in.p.p;
+        if (field == AETHER_FIELD_DIVERGENCE) return in.b.p;
+        raise(AETHER_EINVAL, "unknown field id " + std::to_string(field));
+    }
+
+    Inst &inst(int q) {
+        if (q < 0 || q >= count()) raise(AETHER_EINVAL, "instance index out of range");
+        return *inst_[q];
+    }
+
+    void upload(int q, int f, const double *src, size_t n) override {
+        Inst &in = inst(q);
+        if (n != in.n) raise(AETHER_EINVAL, "upload size does not match the instance grid");
+        T *dst = field(in, f);
+        if (!in.staging.p) in.staging.alloc(in.n);
+        HIPT(hipMemcpyAsync(in.staging.p, src, n * sizeof(double), hipMemcpyHostToDevice, st_));
+        k_convert<double, T><<<4096, kThreads, 0, st_>>>(in.staging.p, (long)n, dst);
+        LAUNCH_CHECK();
+        HIPT(hipStreamSynchronize(st_));
+    }
+
+    void download(int q, int f, double *dst, size_t n) override {
+        Inst &in = inst(q);
+        if (n != in.n) raise(AETHER_EINVAL, "download size does not match the instance grid");
+        T *src = field(in, f);
+        if (!in.staging.p) in.staging.alloc(in.n);
+        k_convert<T, double><<<4096, kThreads, 0, st_>>>(src, (long)n, in.staging.p);
+        LAUNCH_CHECK();
+        HIPT(hipMemcpyAsync(dst, in.staging.p, n * sizeof(double), hipMemcpyDeviceToHost, st_));
+        HIPT(hipStreamSynchronize(st_));
+    }
+
+    void write_table() {
+        std::vector<BInst<T>> tab(count());
+        for (int q = 0; q < count(); ++q) {
+            Inst &in = *inst_[q];
+            BInst<T> &t = tab[q];
+            std::memset(&t, 0, sizeof(t));
+            t.g = in.g;
+            t.h = in.h;
+            for (int a = 0; a < 2; ++a)
+                for (int f = 0; f < 5; ++f) t.f[a][f] = in.f[a][f].p;
+            t.fwd0 = in.fwd0.p;
+            t.fwd1 = in.fwd1.p;
+            t.curl = in.curl.p;
+            t.p = in.p.p;
+            t.b = in.b.p;
+            t.r = in.r.p;
+            t.d[0] = in.dd[0].p;
+            t.d[1] = in.dd[1].p;
+            t.pa = in.pa.p;
+            t.pb = in.pb.p;
+            t.pc = in.pc.p;
+            t.ptel = in.ptel.p;
+            t.st = in.st.p;
+            t.nb = in.nb;
+            t.eps = in.d.vorticity_eps;
+            t.beta = in.d.buoyancy_beta;
+            t.alpha = in.d.gravity_alpha;
+            for (int a = 0; a < 3; ++a) {
+                t.sc[a] = in.d.source_center[a];
+                t.slo[a] = box_lo_[3 * q + a];
+                t.sn[a] = box_n_[3 * q + a];
+            }
+            t.sr2 = in.d.source_radius * in.d.source_radius;
+            t.srho = in.d.source_density;
+            t.stemp = in.d.source_temperature;
+            t.tel = in.tel.p;
+            t.iters = in.iters.p;
+        }
+        HIPT(hipMemcpyAsync(table_.p, tab.data(), tab.size() * sizeof(BInst<T>),
+                            hipMemcpyHostToDevice, st_));
+    }
+
+    void advect1(int f, double dt, double diss) {
+        kb_sl_forward<T, 1><<<ncell_, dim3(kBX, kBY), 0, st_>>>(table_.p, cell_items_.p, dt, sl_, f);
+        LAUNCH_CHECK();
+        kb_mac_correct<T, 1><<<ncell_, dim3(kBX, kBY), 0, st_>>>(table_.p, cell_items_.p, dt, sl_,
+                                                                f, diss, 0.0, 0);
+        LAUNCH_CHECK();
+        int *slot[5] = {&sl_.u, &sl_.v, &sl_.w, &sl_.rho, &sl_.t};
+        *slot[f] ^= 1;
+    }
+
+    void cg_solve(int step) {
+        const int max_iters = params_.solver_iters;
+        const double rel2 = params_.cg_rel_tol > 0 ? params_.cg_rel_tol * params_.cg_rel_tol : 0.0;
+        const int solve_id = ++solve_seq_;
+        const int chunk = 8;
+        const int nchunks = (max_iters + chunk - 1) / chunk;
+        while ((int)chunk_events_.size() < nchunks) {
+            hipEvent_t e;
+            HIPT(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            chunk_events_.push_back(e);
+        }
+        const dim3 blk(kBX, kBY);
+        for (int c = 0; c < nchunks; ++c) {
+            if (c >= 2) {
+                HIPT(hipEventSynchronize(chunk_events_[c - 2]));
+                bool all = true;
+                for (int q = 0; q < count() && all; ++q)
+                    all = __atomic_load_n(done_host_ + q, __ATOMIC_ACQUIRE) == solve_id;
+                if (all) break;
+            }
+            for (int it = c * chunk; it < std::min(max_iters, (c + 1) * chunk); ++it) {
+                if (it == 0)
+                    kb_cg_dir<T, kBatchZU, true><<<ncg_, blk, 0, st_>>>(
+                        table_.p, cg_items_.p, it, rel2, done_dev_, solve_id);
+                else
+                    kb_cg_dir<T, kBatchZU, false><<<ncg_, blk, 0, st_>>>(
+                        table_.p, cg_items_.p, it, rel2, done_dev_, solve_id);
+                LAUNCH_CHECK();
+                kb_cg_update<T, kBatchZU><<<ncg_, blk, 0, st_>>>(table_.p, cg_items_.p, it,
+                                                                 done_dev_, solve_id);
+                LAUNCH_CHECK();
+            }
+            HIPT(hipEventRecord(chunk_events_[c], st_));
+        }
+        kb_cg_finalize<T><<<count(), blk, 0, st_>>>(table_.p, max_iters, rel2, step);
+        LAUNCH_CHECK();
+    }
+
+    // src/fluid.rs:598-611 for every instance at once
+    void step(double dt, double t, int s) {
+        (void)t;
+        const dim3 blk(kBX, kBY);
+        advect1(0, dt, params_.dissipation_velocity);
+        advect1(1, dt, params_.dissipation_velocity);
+        advect1(2, dt, params_.dissipation_velocity);
+        if (nsrc_) {
+            kb_sphere<T><<<nsrc_, blk, 0, st_>>>(table_.p, src_items_.p, sl_);
+            LAUNCH_CHECK();
+        }
+        kb_curl_mag<T><<<ncell_, blk, 0, st_>>>(table_.p, cell_items_.p, sl_);
+        LAUNCH_CHECK();
+        kb_confine_buoy_bc<T><<<ncell_, blk, 0, st_>>>(table_.p, cell_items_.p, dt, sl_);
+        LAUNCH_CHECK();
+        sl_.u ^= 1;
+        sl_.v ^= 1;
+        sl_.w ^= 1;
+        kb_div_cginit<T, kBatchZU><<<ncg_, blk, 0, st_>>>(table_.p, cg_items_.p, sl_);
+        LAUNCH_CHECK();
+        cg_solve(s);
+        kb_gradient_bc<T><<<ncell_, blk, 0, st_>>>(table_.p, cell_items_.p, sl_);
+        LAUNCH_CHECK();
+        kb_sl_forward<T, 2><<<ncell_, blk, 0, st_>>>(table_.p, cell_items_.p, dt, sl_, 3);
+        LAUNCH_CHECK();
+        kb_mac_correct<T, 2><<<ncell_, blk, 0, st_>>>(table_.p, cell_items_.p, dt, sl_, 3,
+                                                      params_.dissipation_density,
+                                                      params_.dissipation_temperature, 1);
+        LAUNCH_CHECK();
+        sl_.rho ^= 1;
+        sl_.t ^= 1;
+        kb_telemetry_part<T, kBatchZU><<<ncg_, blk, 0, st_>>>(table_.p, cg_items_.p, sl_);
+        LAUNCH_CHECK();
+        kb_telemetry_final<T><<<count(), blk, 0, st_>>>(table_.p, s, t);
+        LAUNCH_CHECK();
+    }
+
+    void run(double dt, const double *times, int nsteps, aether_fluid_telemetry *out) override {
+        if (nsteps <= 0) return;
+        for (auto &in : inst_) {
+            if (in->tel.n < (size_t)nsteps * 8) in->tel.alloc((size_t)nsteps * 8);
+            if (in->iters.n < (size_t)nsteps) in->iters.alloc(nsteps);
+        }
+        write_table();
+        for (int s = 0; s < nsteps; ++s) step(dt, times[s], s);
+        const int cnt = count();
+        std::vector<double> raw((size_t)nsteps * 8);
+        iters_.assign((size_t)nsteps * cnt, 0);
+        std::vector<int> it(nsteps);
+        for (int q = 0; q < cnt; ++q) {
+            HIPT(hipMemcpyAsync(raw.data(), inst_[q]->tel.p, raw.size() * sizeof(double),
+                                hipMemcpyDeviceToHost, st_));
+            HIPT(hipMemcpyAsync(it.data(), inst_[q]->iters.p, nsteps * sizeof(int),
+                                hipMemcpyDeviceToHost, st_));
+            HIPT(hipStreamSynchronize(st_));
+            for (int s = 0; s < nsteps; ++s) {
+                iters_[(size_t)s * cnt + q] = it[s];
+                if (out) {
+                    aether_fluid_telemetry &o = out[(size_t)s * cnt + q];
+                    const double *r = raw.data() + (size_t)s * 8;
+                    o.time_sec = r[0];
+                    o.kinetic_energy = r[1];
+                    o.enstrophy = r[2];
+                    o.peak_vorticity = r[3];
+                    o.total_mass = r[4];
+                    for (int a = 0; a < 3; ++a) o.center_of_mass[a] = r[5 + a];
+                }
+            }
+        }
+    }
+
+   private:
+    aether_fluid_params params_;
+    hipStream_t st_;
+    std::vector<std::unique_ptr<Inst>> inst_;
+    std::vector<int> box_lo_, box_n_;
+    DevBuf<BInst<T>> table_;
+    DevBuf<BItem> cell_items_, cg_items_;
+    DevBuf<int2> src_items_;
+    int ncell_ = 0, ncg_ = 0, nsrc_ = 0;
+    Slots sl_ = {0, 0, 0, 0, 0};
+    int *done_host_ = nullptr, *done_dev_ = nullptr;
+    int solve_seq_ = 0;
+    std::vector<hipEvent_t> chunk_events_;
+    std::vector<int> iters_;
+};
+
 }  // namespace
 }  // namespace aether
 
 using namespace aether;
+
+struct aether_fluid_batch_s {
+    int device;
+    hipStream_t stream = nullptr;
+    std::unique_ptr<BatchBase> impl;
+    ~aether_fluid_batch_s() {
+        impl.reset();
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
 
 struct aether_fluid_s {
     int nx, ny, nz, precision;
@@ -1219,6 +1589,93 @@ int aether_fluid_max_density(aether_fluid_t s, double *out) {
         checked(s);
         if (!out) raise(AETHER_EINVAL, "null output pointer");
         *out = s->impl->max_density();
+    });
+}
+
+int aether_fluid_batch_create(const aether_fluid_instance_desc *desc, int count, int precision,
+                              const aether_fluid_params *shared, aether_fluid_batch_t *out) {
+    return guarded([&] {
+        if (!out || !desc || !shared) raise(AETHER_EINVAL, "null argument");
+        *out = nullptr;
+        if (count < 1) raise(AETHER_EINVAL, "count must be >= 1");
+        if (precision != AETHER_FP32 && precision != AETHER_FP64)
+            raise(AETHER_EINVAL, "batched precision must be AETHER_FP32 or AETHER_FP64");
+        check_params(*shared);
+        if (shared->solver_kind != AETHER_SOLVER_CG)
+            raise(AETHER_EINVAL, "batched instances use the CG projection");
+        for (int q = 0; q < count; ++q) {
+            const auto &d = desc[q];
+            if (d.nx < 6 || d.ny < 6 || d.nz < 6)
+                raise(AETHER_EINVAL, "instance " + std::to_string(q) + ": dimensions must be >= 6");
+            if ((double)d.nx * d.ny * d.nz >= 2147483647.0)
+                raise(AETHER_EINVAL, "instance grid has more than 2^31-1 cells");
+            if (!(std::isfinite(d.source_center[0]) && std::isfinite(d.source_center[1]) &&
+                  std::isfinite(d.source_center[2]) && std::isfinite(d.source_radius) &&
+                  std::fabs(d.source_radius) < 1e6))
+                raise(AETHER_EINVAL, "instance source must be finite");
+        }
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+            raise(AETHER_ENODEV, "no HIP device available");
+        auto b = std::make_unique<aether_fluid_batch_s>();
+        HIPT(hipGetDevice(&b->device));
+        HIPT(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+        if (precision == AETHER_FP64)
+            b->impl = std::make_unique<Batch<double>>(desc, count, *shared, b->stream);
+        else
+            b->impl = std::make_unique<Batch<float>>(desc, count, *shared, b->stream);
+        *out = b.release();
+    });
+}
+
+int aether_fluid_batch_destroy(aether_fluid_batch_t b) {
+    return guarded([&] {
+        if (!b) return;
+        (void)hipSetDevice(b->device);
+        delete b;
+    });
+}
+
+namespace {
+aether_fluid_batch_s *checked_batch(aether_fluid_batch_t b) {
+    if (!b) raise(AETHER_EINVAL, "null batch handle");
+    HIPT(hipSetDevice(b->device));
+    return b;
+}
+}  // namespace
+
+int aether_fluid_batch_upload(aether_fluid_batch_t b, int inst, int field, const double *src,
+                              size_t n) {
+    return guarded([&] {
+        if (!src) raise(AETHER_EINVAL, "null source array");
+        checked_batch(b)->impl->upload(inst, field, src, n);
+    });
+}
+
+int aether_fluid_batch_download(aether_fluid_batch_t b, int inst, int field, double *dst,
+                                size_t n) {
+    return guarded([&] {
+        if (!dst) raise(AETHER_EINVAL, "null destination array");
+        checked_batch(b)->impl->download(inst, field, dst, n);
+    });
+}
+
+int aether_fluid_batch_run(aether_fluid_batch_t b, double dt, const double *times, int nsteps,
+                           aether_fluid_telemetry *out) {
+    return guarded([&] {
+        checked_batch(b);
+        if (nsteps < 0) raise(AETHER_EINVAL, "nsteps must be >= 0");
+        if (nsteps > 0 && !times) raise(AETHER_EINVAL, "null times array");
+        b->impl->run(dt, times, nsteps, out);
+    });
+}
+
+int aether_fluid_batch_iterations(aether_fluid_batch_t b, int32_t *out, int n, int *count) {
+    return guarded([&] {
+        checked_batch(b);
+        const auto &v = b->impl->iters();
+        if (count) *count = (int)v.size();
+        for (int a = 0; a < n && a < (int)v.size(); ++a) out[a] = v[a];
     });
 }
 

@@ -223,16 +223,13 @@ __device__ __forceinline__ void gather_minmax(const Dims &g, const T *__restrict
 // Forward semi-Lagrangian pass of NF fields sharing one velocity:
 // fwd = SL(src, +dt) on interior cells, 0 on the boundary layer.
 template <typename T, int NF>
-__global__ __launch_bounds__(kThreads) void k_sl_forward(Dims g, T s, const T *__restrict__ u,
+__device__ __forceinline__ void body_sl_forward(int i, int j, int k, Dims g, T s, const T *__restrict__ u,
                                                          const T *__restrict__ v,
                                                          const T *__restrict__ w,
                                                          const T *__restrict__ src0,
                                                          const T *__restrict__ src1,
                                                          T *__restrict__ fwd0,
                                                          T *__restrict__ fwd1) {
-    const int i = blockIdx.x * kBX + threadIdx.x;
-    const int j = blockIdx.y * kBY + threadIdx.y;
-    const int k = blockIdx.z;
     if (i >= g.nx || j >= g.ny) return;
     const int id = (k * g.ny + j) * g.nx + i;
     if (!is_interior(g, i, j, k)) {
@@ -248,6 +245,16 @@ __global__ __launch_bounds__(kThreads) void k_sl_forward(Dims g, T s, const T *_
     axis_loc(k, s * w[id], g.nz, k0, tz, nw);
     fwd0[id] = gather_lerp<T>(g, src0, i0, j0, k0, tx, ty, tz);
     if (NF == 2) fwd1[id] = gather_lerp<T>(g, src1, i0, j0, k0, tx, ty, tz);
+}
+template <typename T, int NF>
+__global__ __launch_bounds__(kThreads) void k_sl_forward(Dims g, T s, const T *__restrict__ u,
+                                                         const T *__restrict__ v,
+                                                         const T *__restrict__ w,
+                                                         const T *__restrict__ src0,
+                                                         const T *__restrict__ src1,
+                                                         T *__restrict__ fwd0,
+                                                         T *__restrict__ fwd1) {
+    body_sl_forward<T, NF>(blockIdx.x * kBX + threadIdx.x, blockIdx.y * kBY + threadIdx.y, blockIdx.z, g, s, u, v, w, src0, src1, fwd0, fwd1);
 }
 
 // MacCormack correction with the 8-cell extrema clamp (src/fluid.rs:183-228):
@@ -277,14 +284,11 @@ __device__ __forceinline__ T mac_correct_one(const Dims &g, const T *__restrict_
 }
 
 template <typename T, int NF>
-__global__ __launch_bounds__(kThreads) void k_mac_correct(
+__device__ __forceinline__ void body_mac_correct(int i, int j, int k, 
     Dims g, T s, const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ w,
     const T *__restrict__ src0, const T *__restrict__ fwd0, T *__restrict__ out0, T diss0,
     const T *__restrict__ src1, const T *__restrict__ fwd1, T *__restrict__ out1, T diss1,
     int clamp0) {
-    const int i = blockIdx.x * kBX + threadIdx.x;
-    const int j = blockIdx.y * kBY + threadIdx.y;
-    const int k = blockIdx.z;
     if (i >= g.nx || j >= g.ny) return;
     const int id = (k * g.ny + j) * g.nx + i;
     if (!is_interior(g, i, j, k)) {
@@ -310,21 +314,31 @@ __global__ __launch_bounds__(kThreads) void k_mac_correct(
         out1[id] = mac_correct_one<T>(g, src1, fwd1, id, near_wall, i0, j0, k0, fi0, fj0, fk0,
                                       ftx, fty, ftz, diss1, clamp0 != 0);
 }
+template <typename T, int NF>
+__global__ __launch_bounds__(kThreads) void k_mac_correct(
+    Dims g, T s, const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ w,
+    const T *__restrict__ src0, const T *__restrict__ fwd0, T *__restrict__ out0, T diss0,
+    const T *__restrict__ src1, const T *__restrict__ fwd1, T *__restrict__ out1, T diss1,
+    int clamp0) {
+    body_mac_correct<T, NF>(blockIdx.x * kBX + threadIdx.x, blockIdx.y * kBY + threadIdx.y, blockIdx.z, g, s, u, v, w, src0, fwd0, out0, diss0, src1, fwd1, out1, diss1, clamp0);
+}
 
 // ---------------------------------------------------------------------------
 // Walls, vorticity confinement, buoyancy (src/fluid.rs:236-330)
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_enforce_bc(Dims g, T *__restrict__ u,
+__device__ __forceinline__ void body_enforce_bc(int i, int j, int k, Dims g, T *__restrict__ u,
                                                          T *__restrict__ v, T *__restrict__ w) {
-    const int i = blockIdx.x * kBX + threadIdx.x;
-    const int j = blockIdx.y * kBY + threadIdx.y;
-    const int k = blockIdx.z;
     if (i >= g.nx || j >= g.ny) return;
     const int id = (k * g.ny + j) * g.nx + i;
     if (bc_u(g, i, j, k)) u[id] = T(0);
     if (bc_v(g, i, j, k)) v[id] = T(0);
     if (bc_w(g, i, j, k)) w[id] = T(0);
+}
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_enforce_bc(Dims g, T *__restrict__ u,
+                                                         T *__restrict__ v, T *__restrict__ w) {
+    body_enforce_bc<T>(blockIdx.x * kBX + threadIdx.x, blockIdx.y * kBY + threadIdx.y, blockIdx.z, g, u, v, w);
 }
 
 template <typename T>
@@ -344,13 +358,10 @@ __device__ __forceinline__ void curl_at(const Dims &g, const T *__restrict__ u,
 }
 
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_curl_mag(Dims g, T inv_2h, const T *__restrict__ u,
+__device__ __forceinline__ void body_curl_mag(int i, int j, int k, Dims g, T inv_2h, const T *__restrict__ u,
                                                        const T *__restrict__ v,
                                                        const T *__restrict__ w,
                                                        T *__restrict__ cmag) {
-    const int i = blockIdx.x * kBX + threadIdx.x;
-    const int j = blockIdx.y * kBY + threadIdx.y;
-    const int k = blockIdx.z;
     if (i >= g.nx || j >= g.ny) return;
     const int id = (k * g.ny + j) * g.nx + i;
     if (!is_interior(g, i, j, k)) {
@@ -361,18 +372,22 @@ __global__ __launch_bounds__(kThreads) void k_curl_mag(Dims g, T inv_2h, const T
     curl_at<T>(g, u, v, w, id, inv_2h, ox, oy, oz);
     cmag[id] = sqrt(ox * ox + oy * oy + oz * oz);
 }
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_curl_mag(Dims g, T inv_2h, const T *__restrict__ u,
+                                                       const T *__restrict__ v,
+                                                       const T *__restrict__ w,
+                                                       T *__restrict__ cmag) {
+    body_curl_mag<T>(blockIdx.x * kBX + threadIdx.x, blockIdx.y * kBY + threadIdx.y, blockIdx.z, g, inv_2h, u, v, w, cmag);
+}
 
 // Confinement force eps*h*(N x w) on cells [2, n-3], then Boussinesq lift on
 // interior cells, then the wall masks; writes new velocity arrays.
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_confine_buoy_bc(
+__device__ __forceinline__ void body_confine_buoy_bc(int i, int j, int k, 
     Dims g, T dt, T inv_2h, T eps_h, T beta, T alpha, const T *__restrict__ u,
     const T *__restrict__ v, const T *__restrict__ w, const T *__restrict__ cmag,
     const T *__restrict__ rho, const T *__restrict__ temp, T *__restrict__ uo,
     T *__restrict__ vo, T *__restrict__ wo) {
-    const int i = blockIdx.x * kBX + threadIdx.x;
-    const int j = blockIdx.y * kBY + threadIdx.y;
-    const int k = blockIdx.z;
     if (i >= g.nx || j >= g.ny) return;
     const int id = (k * g.ny + j) * g.nx + i;
     T uu = u[id], vv = v[id], ww = w[id];
@@ -401,6 +416,14 @@ __global__ __launch_bounds__(kThreads) void k_confine_buoy_bc(
     uo[id] = uu;
     vo[id] = vv;
     wo[id] = ww;
+}
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_confine_buoy_bc(
+    Dims g, T dt, T inv_2h, T eps_h, T beta, T alpha, const T *__restrict__ u,
+    const T *__restrict__ v, const T *__restrict__ w, const T *__restrict__ cmag,
+    const T *__restrict__ rho, const T *__restrict__ temp, T *__restrict__ uo,
+    T *__restrict__ vo, T *__restrict__ wo) {
+    body_confine_buoy_bc<T>(blockIdx.x * kBX + threadIdx.x, blockIdx.y * kBY + threadIdx.y, blockIdx.z, g, dt, inv_2h, eps_h, beta, alpha, u, v, w, cmag, rho, temp, uo, vo, wo);
 }
 
 // ---------------------------------------------------------------------------
@@ -653,9 +676,7 @@ struct Col {
 };
 
 template <int ZU>
-__device__ __forceinline__ Col col_of(const Dims &g, const Tiling &tl) {
-    int bx, by, bz;
-    tile_coords(tl, bx, by, bz);
+__device__ __forceinline__ Col col_at(const Dims &g, int bx, int by, int bz) {
     Col c;
     c.i = bx * kBX + threadIdx.x;
     c.j = by * kBY + threadIdx.y;
@@ -670,6 +691,13 @@ __device__ __forceinline__ Col col_of(const Dims &g, const Tiling &tl) {
     return c;
 }
 
+template <int ZU>
+__device__ __forceinline__ Col col_of(const Dims &g, const Tiling &tl) {
+    int bx, by, bz;
+    tile_coords(tl, bx, by, bz);
+    return col_at<ZU>(g, bx, by, bz);
+}
+
 __device__ __forceinline__ int col_id(const Dims &g, const Col &c, int k) {
     k = min(max(k, 0), g.nz - 1);
     return (k * g.ny + c.jc) * g.nx + c.ic;
@@ -678,17 +706,13 @@ __device__ __forceinline__ int col_id(const Dims &g, const Col &c, int k) {
 // Divergence + CG initialisation (src/fluid.rs:497-498, 450-460):
 // b = -h div(masked u) (or the stored b), r = b - A p; partials of r.r, b.b.
 template <typename T, int ZU>
-__global__ __launch_bounds__(kThreads) void k_div_cginit(Dims g, Tiling tl, T h, int with_div,
-                                                         const T *__restrict__ u,
-                                                         const T *__restrict__ v,
-                                                         const T *__restrict__ w,
-                                                         const T *__restrict__ p,
-                                                         T *__restrict__ b, T *__restrict__ r,
-                                                         double *__restrict__ part_rr,
-                                                         double *__restrict__ part_bb,
-                                                         CgState *st) {
+__device__ __forceinline__ void body_div_cginit(const Col &c, int slot, Dims g, T h, int with_div,
+                                                const T *__restrict__ u, const T *__restrict__ v,
+                                                const T *__restrict__ w, const T *__restrict__ p,
+                                                T *__restrict__ b, T *__restrict__ r,
+                                                double *__restrict__ part_rr,
+                                                double *__restrict__ part_bb, CgState *st) {
     __shared__ double lds[16];
-    const Col c = col_of<ZU>(g, tl);
     double acc[2] = {0.0, 0.0};
     if (c.valid) {
 #pragma unroll
@@ -718,9 +742,9 @@ __global__ __launch_bounds__(kThreads) void k_div_cginit(Dims g, Tiling tl, T h,
     }
     block_sum<2>(acc, lds);
     if (tid_lin() == 0) {
-        part_rr[blockIdx.x] = acc[0];
-        part_bb[blockIdx.x] = acc[1];
-        if (blockIdx.x == 0) {
+        part_rr[slot] = acc[0];
+        part_bb[slot] = acc[1];
+        if (slot == 0) {
             st->done = 0;
             st->iters = 0;
             st->converged = 0;
@@ -728,30 +752,41 @@ __global__ __launch_bounds__(kThreads) void k_div_cginit(Dims g, Tiling tl, T h,
     }
 }
 
+template <typename T, int ZU>
+__global__ __launch_bounds__(kThreads) void k_div_cginit(Dims g, Tiling tl, T h, int with_div,
+                                                         const T *__restrict__ u,
+                                                         const T *__restrict__ v,
+                                                         const T *__restrict__ w,
+                                                         const T *__restrict__ p,
+                                                         T *__restrict__ b, T *__restrict__ r,
+                                                         double *__restrict__ part_rr,
+                                                         double *__restrict__ part_bb,
+                                                         CgState *st) {
+    body_div_cginit<T, ZU>(col_of<ZU>(g, tl), blockIdx.x, g, h, with_div, u, v, w, p, b, r,
+                           part_rr, part_bb, st);
+}
+
 // CG kernel 1 of iteration `it` (src/fluid.rs:463-469, 482-486): finish
 // r.r of the previous iteration, apply the stopping rules, form
 // d = r + beta d_old (d = r at it = 0) on the fly, q = A d, partial d.q.
 // d is double-buffered so that neighbouring columns recompute it.
 template <typename T, int ZU, bool FIRST>
-__global__ __launch_bounds__(kThreads) void k_cg_dir(Dims g, Tiling tl, int it, double rel_tol2,
-                                                     const double *__restrict__ part_rr,
-                                                     const double *__restrict__ part_bb,
-                                                     CgState *st, const T *__restrict__ r,
-                                                     const T *__restrict__ dold,
-                                                     T *__restrict__ dnew,
-                                                     double *__restrict__ part_dq,
-                                                     int *done_mirror, int solve_id) {
+__device__ __forceinline__ void body_cg_dir(const Col &c, int slot, int nb, Dims g, int it,
+                                            double rel_tol2, const double *__restrict__ part_rr,
+                                            const double *__restrict__ part_bb, CgState *st,
+                                            const T *__restrict__ r, const T *__restrict__ dold,
+                                            T *__restrict__ dnew, double *__restrict__ part_dq,
+                                            int *done_mirror, int solve_id) {
     __shared__ double lds[8];
     __shared__ int flag_lds[1];
-    const Col c = col_of<ZU>(g, tl);
     constexpr bool first = FIRST;
     // 1) scalars and partial sums of the previous kernel, then every data
     //    load of this thread: one memory round trip for all of them.  The
     //    stop flag was written by an earlier kernel, so a plain load sees it.
     const int flag_reg = st->done;
     EarlyPartials prr, pbb;
-    prr.load(part_rr, tl.nb);
-    if (first) pbb.load(part_bb, tl.nb);
+    prr.load(part_rr, nb);
+    if (first) pbb.load(part_bb, nb);
     const double delta_prev = first ? 0.0 : st->delta[(it - 1) & 1];
     const double b2_prev = first ? 0.0 : st->b2;
     T rz[ZU + 2], dz[ZU + 2], rn[ZU][4], dn[ZU][4];
@@ -773,13 +808,13 @@ __global__ __launch_bounds__(kThreads) void k_cg_dir(Dims g, Tiling tl, int it, 
     }
     // 2) stopping rules (every workgroup reaches the same decision); thread
     //    0's flag is broadcast so that all waves take the same branch
-    const bool lead = blockIdx.x == 0 && tid_lin() == 0;
+    const bool lead = slot == 0 && tid_lin() == 0;
     double beta = 0.0;
     if (tid_lin() == 0) flag_lds[0] = flag_reg;
-    const double rr = prr.total(part_rr, tl.nb, lds);  // its barrier publishes flag_lds
+    const double rr = prr.total(part_rr, nb, lds);  // its barrier publishes flag_lds
     if (flag_lds[0]) return;
     if (first) {
-        const double bb = pbb.total(part_bb, tl.nb, lds);
+        const double bb = pbb.total(part_bb, nb, lds);
         const bool stop = !(rr > 1e-24) || (rel_tol2 > 0.0 && rr <= rel_tol2 * bb);
         if (lead) {
             st->b2 = bb;
@@ -835,24 +870,35 @@ __global__ __launch_bounds__(kThreads) void k_cg_dir(Dims g, Tiling tl, int it, 
         }
     }
     block_sum<1>(acc, lds);
-    if (tid_lin() == 0) part_dq[blockIdx.x] = acc[0];
+    if (tid_lin() == 0) part_dq[slot] = acc[0];
+}
+
+template <typename T, int ZU, bool FIRST>
+__global__ __launch_bounds__(kThreads) void k_cg_dir(Dims g, Tiling tl, int it, double rel_tol2,
+                                                     const double *__restrict__ part_rr,
+                                                     const double *__restrict__ part_bb,
+                                                     CgState *st, const T *__restrict__ r,
+                                                     const T *__restrict__ dold,
+                                                     T *__restrict__ dnew,
+                                                     double *__restrict__ part_dq,
+                                                     int *done_mirror, int solve_id) {
+    body_cg_dir<T, ZU, FIRST>(col_of<ZU>(g, tl), blockIdx.x, tl.nb, g, it, rel_tol2, part_rr,
+                              part_bb, st, r, dold, dnew, part_dq, done_mirror, solve_id);
 }
 
 // CG kernel 2 of iteration `it` (src/fluid.rs:466-478): finish d.q,
 // alpha, p += alpha d, r -= alpha A d (A d recomputed), partial r.r.
 template <typename T, int ZU>
-__global__ __launch_bounds__(kThreads) void k_cg_update(Dims g, Tiling tl, int it,
-                                                        const double *__restrict__ part_dq,
-                                                        CgState *st, const T *__restrict__ d,
-                                                        T *__restrict__ p, T *__restrict__ r,
-                                                        double *__restrict__ part_rr,
-                                                        int *done_mirror, int solve_id) {
+__device__ __forceinline__ void body_cg_update(const Col &c, int slot, int nb, Dims g, int it,
+                                               const double *__restrict__ part_dq, CgState *st,
+                                               const T *__restrict__ d, T *__restrict__ p,
+                                               T *__restrict__ r, double *__restrict__ part_rr,
+                                               int *done_mirror, int solve_id) {
     __shared__ double lds[8];
     __shared__ int flag_lds[1];
-    const Col c = col_of<ZU>(g, tl);
     const int flag_reg = st->done;
     EarlyPartials pdq;
-    pdq.load(part_dq, tl.nb);
+    pdq.load(part_dq, nb);
     const double delta = st->delta[it & 1];
     T dz[ZU + 2], dn[ZU][4], pc[ZU], rc[ZU];
 #pragma unroll
@@ -867,9 +913,9 @@ __global__ __launch_bounds__(kThreads) void k_cg_update(Dims g, Tiling tl, int i
         pc[a] = p[id];
         rc[a] = r[id];
     }
-    const bool lead = blockIdx.x == 0 && tid_lin() == 0;
+    const bool lead = slot == 0 && tid_lin() == 0;
     if (tid_lin() == 0) flag_lds[0] = flag_reg;
-    const double dq = pdq.total(part_dq, tl.nb, lds);
+    const double dq = pdq.total(part_dq, nb, lds);
     if (flag_lds[0]) return;
     if (fabs(dq) < 1e-28) {
         if (lead) {
@@ -901,7 +947,18 @@ __global__ __launch_bounds__(kThreads) void k_cg_update(Dims g, Tiling tl, int i
         }
     }
     block_sum<1>(acc, lds);
-    if (tid_lin() == 0) part_rr[blockIdx.x] = acc[0];
+    if (tid_lin() == 0) part_rr[slot] = acc[0];
+}
+
+template <typename T, int ZU>
+__global__ __launch_bounds__(kThreads) void k_cg_update(Dims g, Tiling tl, int it,
+                                                        const double *__restrict__ part_dq,
+                                                        CgState *st, const T *__restrict__ d,
+                                                        T *__restrict__ p, T *__restrict__ r,
+                                                        double *__restrict__ part_rr,
+                                                        int *done_mirror, int solve_id) {
+    body_cg_update<T, ZU>(col_of<ZU>(g, tl), blockIdx.x, tl.nb, g, it, part_dq, st, d, p, r,
+                          part_rr, done_mirror, solve_id);
 }
 
 // Closes a solve that ran out of iterations (or had max_iters == 0).
@@ -956,13 +1013,10 @@ __global__ __launch_bounds__(kThreads) void k_rbgs(Dims g, int color, T omega,
 
 // Gradient subtraction and the closing wall masks (src/fluid.rs:509-528).
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_gradient_bc(Dims g, T inv_h,
+__device__ __forceinline__ void body_gradient_bc(int i, int j, int k, Dims g, T inv_h,
                                                           const T *__restrict__ p,
                                                           T *__restrict__ u, T *__restrict__ v,
                                                           T *__restrict__ w) {
-    const int i = blockIdx.x * kBX + threadIdx.x;
-    const int j = blockIdx.y * kBY + threadIdx.y;
-    const int k = blockIdx.z;
     if (i >= g.nx || j >= g.ny) return;
     const int id = (k * g.ny + j) * g.nx + i;
     T uu = u[id], vv = v[id], ww = w[id];
@@ -978,6 +1032,13 @@ __global__ __launch_bounds__(kThreads) void k_gradient_bc(Dims g, T inv_h,
     u[id] = uu;
     v[id] = vv;
     w[id] = ww;
+}
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_gradient_bc(Dims g, T inv_h,
+                                                          const T *__restrict__ p,
+                                                          T *__restrict__ u, T *__restrict__ v,
+                                                          T *__restrict__ w) {
+    body_gradient_bc<T>(blockIdx.x * kBX + threadIdx.x, blockIdx.y * kBY + threadIdx.y, blockIdx.z, g, inv_h, p, u, v, w);
 }
 
 // ---------------------------------------------------------------------------
@@ -1088,19 +1149,15 @@ __global__ __launch_bounds__(kThreads) void k_gradient64_bc(Dims g, double inv_h
 constexpr int kTelParts = 7;
 
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_telemetry_part(Dims g, int zchunk, double h,
-                                                             const T *__restrict__ u,
-                                                             const T *__restrict__ v,
-                                                             const T *__restrict__ w,
-                                                             const T *__restrict__ rho,
-                                                             double *__restrict__ part) {
+__device__ __forceinline__ void body_telemetry_part(int i, int j, int kb, int ke, int slot, Dims g,
+                                                    double h, const T *__restrict__ u,
+                                                    const T *__restrict__ v,
+                                                    const T *__restrict__ w,
+                                                    const T *__restrict__ rho,
+                                                    double *__restrict__ part) {
     __shared__ double lds[32];
     const double dv = h * h * h;
     const double inv_2h = 0.5 / h;
-    const int i = blockIdx.x * kBX + threadIdx.x;
-    const int j = blockIdx.y * kBY + threadIdx.y;
-    const int kb = blockIdx.z * zchunk;
-    const int ke = min(kb + zchunk, g.nz);
     double acc[6] = {0, 0, 0, 0, 0, 0};
     double peak = 0.0;
     if (i < g.nx && j < g.ny) {
@@ -1133,16 +1190,29 @@ __global__ __launch_bounds__(kThreads) void k_telemetry_part(Dims g, int zchunk,
     block_sum<6>(acc, lds);
     peak = block_max(peak, lds);
     if (tid_lin() == 0) {
-        const int bid = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-        double *o = part + (size_t)bid * kTelParts;
+        double *o = part + (size_t)slot * kTelParts;
         for (int a = 0; a < 6; ++a) o[a] = acc[a];
         o[6] = peak;
     }
 }
 
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_telemetry_part(Dims g, int zchunk, double h,
+                                                             const T *__restrict__ u,
+                                                             const T *__restrict__ v,
+                                                             const T *__restrict__ w,
+                                                             const T *__restrict__ rho,
+                                                             double *__restrict__ part) {
+    const int kb = blockIdx.z * zchunk;
+    body_telemetry_part<T>(blockIdx.x * kBX + threadIdx.x, blockIdx.y * kBY + threadIdx.y, kb,
+                           min(kb + zchunk, g.nz),
+                           (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x, g, h,
+                           u, v, w, rho, part);
+}
+
 // One workgroup: fixed-order combination of the partials into a
 // FluidTelemetryFrame {time, ke, enstrophy, peak, mass, com xyz}.
-__global__ __launch_bounds__(kThreads) void k_telemetry_final(const double *__restrict__ part,
+__device__ __forceinline__ void body_telemetry_final(const double *__restrict__ part,
                                                               int nb, double time_sec,
                                                               double *__restrict__ out) {
     __shared__ double lds[32];
@@ -1172,6 +1242,12 @@ __global__ __launch_bounds__(kThreads) void k_telemetry_final(const double *__re
         out[6] = fmin(fmax(cy, -1.0), 1.0);
         out[7] = fmin(fmax(cz, -1.0), 1.0);
     }
+}
+
+__global__ __launch_bounds__(kThreads) void k_telemetry_final(const double *__restrict__ part,
+                                                              int nb, double time_sec,
+                                                              double *__restrict__ out) {
+    body_telemetry_final(part, nb, time_sec, out);
 }
 
 // ---------------------------------------------------------------------------

@@ -172,6 +172,48 @@ def bench_mixed(steps: int, warmup: int) -> dict:
     return out
 
 
+def bench_batched(steps: int, warmup: int, dt: float = 0.01) -> dict:
+    """64 grids drawn from {32^3, 48^3, 64^3, 96^3, 128x96x128}: one launch
+    per stage for all of them, against the same grids stepped one at a time
+    with the single-grid solver.  FP32, CG rel tol 1e-5 (max 200), zero
+    initial state, sphere sources in the bottom quarter."""
+    import aether_fluid as af
+    insts = af.batched_instances(64)
+    nvox = sum(d["nx"] * d["ny"] * d["nz"] for d in insts)
+    times = [dt * i for i in range(warmup + steps)]
+    b = af.FluidBatch(insts, solver_iters=200, cg_rel_tol=1e-5)
+    if warmup:
+        b.run(dt, times[:warmup])
+    t0 = time.perf_counter()
+    b.run(dt, times[warmup:])
+    el_b = time.perf_counter() - t0
+    it_b = b.iterations()
+    b.close()
+    el_1 = 0.0
+    it_1 = []
+    for d in insts:
+        s = af.FluidSolver3D(d["nx"], d["ny"], d["nz"])
+        s.vorticity_eps, s.buoyancy_beta, s.gravity_alpha = (d["vorticity_eps"], 1.0,
+                                                             d["gravity_alpha"])
+        s.set_sphere_source(d["source_center"], d["source_radius"])
+        s.solver_iters, s.cg_rel_tol = 200, 1e-5
+        if warmup:
+            s.run(dt, times[:warmup])
+        t0 = time.perf_counter()
+        s.run(dt, times[warmup:])
+        el_1 += time.perf_counter() - t0
+        it_1.append(s.run_iters())
+        s.close()
+    return {"workload": "64 batched smoke grids, FP32, CG rel tol 1e-5", "instances": 64,
+            "total_voxels": nvox, "steps": steps, "warmup": warmup,
+            "batched": {"voxel_steps_per_s": nvox * steps / el_b, "ms_per_step": el_b * 1e3 / steps,
+                        "cg_iters_mean": float(np.mean(it_b))},
+            "one_at_a_time": {"voxel_steps_per_s": nvox * steps / el_1,
+                              "ms_per_step": el_1 * 1e3 / steps,
+                              "cg_iters_mean": float(np.mean(it_1))},
+            "speedup": el_1 / el_b}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -179,7 +221,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
-    ap.add_argument("--workload", default="plain_128", choices=["plain_128", "mixed_256"],
+    ap.add_argument("--workload", default="plain_128", choices=["plain_128", "mixed_256", "batched_64"],
                     help="plain_128: the headline line; mixed_256: mixed-precision "
                          "projection at 256^3 against the all-FP64 projection")
     ap.add_argument("--probe", action="store_true",
@@ -192,6 +234,9 @@ def main():
         sys.exit(2)
 
     import aether_fluid as af
+    if args.workload == "batched_64":
+        print(json.dumps(bench_batched(args.steps, args.warmup)), flush=True)
+        return
     if args.workload == "mixed_256":
         print(json.dumps(bench_mixed(args.steps, args.warmup)), flush=True)
         return

@@ -172,6 +172,34 @@ int aether_fluid_max_density(aether_fluid_t s, double *out);
 int aether_rng_seed(uint64_t seed, uint64_t state[4]);
 int aether_rng_fill_range(uint64_t state[4], double lo, double hi, double *out, size_t n);
 
+/* Batched instances (no counterpart in the reference, which steps one grid
+ * per FluidSolver3D).  Many independent grids of different sizes advance
+ * together, one launch per stage over all their tiles.  Shared: dt, solver
+ * knobs, dissipation (from `shared`; CG only).  Per instance: grid, h,
+ * forcing and a sphere source that sets density/temperature. */
+typedef struct aether_fluid_batch_s *aether_fluid_batch_t;
+typedef struct {
+    int32_t nx, ny, nz;
+    int32_t reserved;
+    double h;                 /* <= 0: 1/max(nx,ny,nz) as src/fluid.rs:56 */
+    double vorticity_eps, buoyancy_beta, gravity_alpha;
+    double source_center[3];  /* cell-index coordinates */
+    double source_radius;     /* <= 0: no source */
+    double source_density, source_temperature;
+} aether_fluid_instance_desc;
+int aether_fluid_batch_create(const aether_fluid_instance_desc *desc, int count, int precision,
+                              const aether_fluid_params *shared, aether_fluid_batch_t *out);
+int aether_fluid_batch_destroy(aether_fluid_batch_t b);
+int aether_fluid_batch_upload(aether_fluid_batch_t b, int inst, int field, const double *src,
+                              size_t n);
+int aether_fluid_batch_download(aether_fluid_batch_t b, int inst, int field, double *dst,
+                                size_t n);
+/* `out` (may be NULL) receives nsteps*count frames, step-major. */
+int aether_fluid_batch_run(aether_fluid_batch_t b, double dt, const double *times, int nsteps,
+                           aether_fluid_telemetry *out);
+/* CG iterations of the last run, nsteps*count values, step-major. */
+int aether_fluid_batch_iterations(aether_fluid_batch_t b, int32_t *out, int n, int *count);
+
 /* Measurement hooks used by the benchmark driver. */
 /* Average device time (ms) per launch of the CG iteration kernels during the
  * last aether_fluid_run with timing enabled (HIP events on the solver's
