@@ -8,10 +8,11 @@ from .solver import (FluidSolver3D, FluidTelemetryFrame, PoissonSolverKind, rng_
                      stream_copy_ms)
 from .scenarios import seeded_config_state, SMOKE_CONFIGS, batched_instances
 from .batch import FluidBatch
+from .adjoint import SmokeRollout
 
 __all__ = [
     "AetherFluidError", "FIELDS", "FP32", "FP32_IR64", "FP64", "SOLVER_CG", "SOLVER_RBGS",
     "SOURCE_JETS", "SOURCE_NONE", "SOURCE_SPHERE", "LIB_PATH", "SIGNATURES", "build", "lib",
     "FluidSolver3D", "FluidTelemetryFrame", "PoissonSolverKind", "rng_fill", "stream_copy_ms",
-    "seeded_config_state", "SMOKE_CONFIGS", "batched_instances", "FluidBatch",
+    "seeded_config_state", "SMOKE_CONFIGS", "batched_instances", "FluidBatch", "SmokeRollout",
 ]

@@ -88,6 +88,15 @@ class InstanceDesc(C.Structure):
     ]
 
 
+class RolloutDesc(C.Structure):
+    _fields_ = [
+        ("nsteps", C.c_int32), ("checkpoint_every", C.c_int32), ("dt", C.c_double),
+        ("patch_lo", C.c_int32 * 3), ("patch_hi", C.c_int32 * 3),
+        ("adjoint_cg_iters", C.c_int32), ("reserved", C.c_int32),
+        ("adjoint_cg_rel_tol", C.c_double),
+    ]
+
+
 # every symbol declared in include/aether_fluid.h: name -> (restype, argtypes)
 _vp = C.c_void_p
 _pd = C.POINTER(C.c_double)
@@ -132,6 +141,8 @@ SIGNATURES = {
     "aether_fluid_batch_run": (C.c_int, [_vp, C.c_double, _pd, C.c_int, C.POINTER(Telemetry)]),
     "aether_fluid_batch_iterations": (C.c_int, [_vp, C.POINTER(C.c_int32), C.c_int,
                                                 C.POINTER(C.c_int)]),
+    "aether_fluid_rollout": (C.c_int, [_vp, C.POINTER(RolloutDesc), _pd, C.c_double, _pd, _pd,
+                                       _pd, _pd]),
     "aether_rng_seed": (C.c_int, [C.c_uint64, _pu64]),
     "aether_rng_fill_range": (C.c_int, [_pu64, C.c_double, C.c_double, _pd, C.c_size_t]),
     "aether_fluid_set_kernel_timing": (C.c_int, [_vp, C.c_int]),

@@ -105,3 +105,19 @@ def batched_instances(count: int = 64, seed: int = 0xAE7E0004):
                         source_center=(cx, cy, cz), source_radius=r,
                         source_density=1.0, source_temperature=1.0))
     return out
+
+
+def rollout_problem(n: int = 64, seed: int = 0xAE7E0003):
+    """The differentiable-rollout workload scaled to an n^3 grid (64 in the
+    spec): inflow patch i, k in [n/4, 3n/4), j in [1, 1 + n/16) with seeded
+    U[0, 1] controls, Gaussian density target sigma = n/8 centred at
+    (n/2, 3n/4, n/2), h = 1/n."""
+    lo = (n // 4, 1, n // 4)
+    hi = (3 * n // 4, 1 + max(1, n // 16), 3 * n // 4)
+    cnt = (hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2])
+    controls, _ = rng_fill(seed, 0.0, 1.0, cnt)
+    k, j, i = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    sig = n / 8.0
+    c = (n / 2.0, 3.0 * n / 4.0, n / 2.0)
+    target = np.exp(-((i - c[0]) ** 2 + (j - c[1]) ** 2 + (k - c[2]) ** 2) / (2 * sig * sig))
+    return dict(patch_lo=lo, patch_hi=hi, controls=controls, target=target.ravel())

@@ -254,6 +254,33 @@ class FluidSolver3D:
                                                            C.byref(cnt)))
         return [buf[a] for a in range(cnt.value)]
 
+    def rollout(self, nsteps: int, dt: float, patch_lo, patch_hi, controls, beta: float,
+                target, checkpoint_every: int = 8, grad: bool = True,
+                adjoint_cg_iters: int | None = None, adjoint_cg_rel_tol: float | None = None):
+        """Differentiable rollout from the current state (left unchanged).
+
+        controls: inflow density/temperature of the patch [patch_lo, patch_hi)
+        (x fastest), beta: buoyancy coefficient, target: density target.
+        Returns (loss, dloss/dcontrols, dloss/dbeta); the gradients are None
+        when grad is False."""
+        self._sync_params()
+        d = L.RolloutDesc()
+        d.nsteps, d.checkpoint_every, d.dt = nsteps, checkpoint_every, dt
+        d.patch_lo = (C.c_int32 * 3)(*patch_lo)
+        d.patch_hi = (C.c_int32 * 3)(*patch_hi)
+        d.adjoint_cg_iters = self.solver_iters if adjoint_cg_iters is None else adjoint_cg_iters
+        d.adjoint_cg_rel_tol = (self.cg_rel_tol if adjoint_cg_rel_tol is None
+                                else adjoint_cg_rel_tol)
+        c = np.ascontiguousarray(np.asarray(controls, dtype=np.float64).reshape(-1))
+        t = np.ascontiguousarray(np.asarray(target, dtype=np.float64).reshape(-1))
+        loss = C.c_double()
+        gc = np.zeros_like(c) if grad else None
+        gb = C.c_double()
+        L.check(self._lib.aether_fluid_rollout(
+            self._h, C.byref(d), _dptr(c), float(beta), _dptr(t), C.byref(loss),
+            _dptr(gc) if grad else None, C.byref(gb) if grad else None))
+        return loss.value, gc, (gb.value if grad else None)
+
     def set_kernel_timing(self, enable: bool) -> None:
         L.check(self._lib.aether_fluid_set_kernel_timing(self._h, int(enable)))
 

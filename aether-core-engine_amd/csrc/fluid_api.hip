@@ -23,6 +23,7 @@
 #include "fluid_kernels.hpp"
 #include "cg_resident.hpp"
 #include "batch_kernels.hpp"
+#include "adjoint_kernels.hpp"
 
 namespace aether {
 namespace {
@@ -192,6 +193,8 @@ class SolverBase {
     virtual double max_density() = 0;
     virtual void stats(aether_fluid_solve_stats *out) = 0;
     virtual const std::vector<int> &last_run_iters() const = 0;
+    virtual double rollout(const aether_rollout_desc &d, const double *ctrl, double beta,
+                           const double *target, double *grad_c, double *grad_beta) = 0;
     aether_fluid_params params = default_params();
     KernelTimer timer;
 };
@@ -669,6 +672,13 @@ class Solver final : public SolverBase {
     }
 
     void enqueue_sources(double t, double dt) {
+        if (patch_on_) {  // rollout inflow: density and temperature set from the controls
+            const int cnt = patch_n_.x * patch_n_.y * patch_n_.z;
+            k_patch_set<T><<<(cnt + kThreads - 1) / kThreads, cell_blk_, 0, st_>>>(
+                g_, patch_lo_, patch_n_, ctrl_.p, cur_[3], cur_[4]);
+            LAUNCH_CHECK();
+            return;
+        }
         if (params.source_kind == AETHER_SOURCE_JETS) {
             // scalar set-up of src/fluid.rs:533-563 on the host (libm sin/cos)
             const double TAU = 6.28318530717958647692528676655900577;
@@ -922,6 +932,243 @@ class Solver final : public SolverBase {
         }
     }
 
+
+    // ---------------------------------------------------------------------
+    // Differentiable rollouts (checkpointed forward + hand-written adjoint)
+    // ---------------------------------------------------------------------
+    struct StepRec {
+        DevBuf<T> u0, v0, w0, u1, v1, w1, rs, ts, u3, v3, w3;
+    };
+    struct Ckpt {
+        DevBuf<T> f[5], p;
+    };
+
+    void copy_arr(const T *a, T *b) {
+        k_copy<T><<<blocks_for(n_), kThreads, 0, st_>>>(a, (long)n_, b);
+        LAUNCH_CHECK();
+    }
+
+    void save_ckpt(Ckpt &c) {
+        for (int a = 0; a < 5; ++a) {
+            if (!c.f[a].p) c.f[a].alloc(n_);
+            copy_arr(cur_[a], c.f[a].p);
+        }
+        if (!c.p.p) c.p.alloc(n_);
+        copy_arr(p_.p, c.p.p);
+    }
+
+    void load_ckpt(const Ckpt &c) {
+        for (int a = 0; a < 5; ++a) copy_arr(c.f[a].p, cur_[a]);
+        copy_arr(c.p.p, p_.p);
+    }
+
+    void step_plain(double dt) {
+        enqueue_advect1(0, dt, params.dissipation_velocity);
+        enqueue_advect1(1, dt, params.dissipation_velocity);
+        enqueue_advect1(2, dt, params.dissipation_velocity);
+        enqueue_sources(0.0, dt);
+        enqueue_buoyancy_confinement(dt);
+        enqueue_project(-1);
+        enqueue_advect_scalars(dt);
+    }
+
+    // The same step as step_plain, keeping what the adjoint step needs.
+    void step_recorded(double dt, StepRec &r) {
+        for (DevBuf<T> *b : {&r.u0, &r.v0, &r.w0, &r.u1, &r.v1, &r.w1, &r.rs, &r.ts, &r.u3,
+                             &r.v3, &r.w3})
+            if (!b->p) b->alloc(n_);
+        copy_arr(cur_[0], r.u0.p);
+        copy_arr(cur_[1], r.v0.p);
+        copy_arr(cur_[2], r.w0.p);
+        enqueue_advect1(0, dt, params.dissipation_velocity);
+        enqueue_advect1(1, dt, params.dissipation_velocity);
+        enqueue_advect1(2, dt, params.dissipation_velocity);
+        copy_arr(cur_[0], r.u1.p);
+        copy_arr(cur_[1], r.v1.p);
+        copy_arr(cur_[2], r.w1.p);
+        enqueue_sources(0.0, dt);
+        copy_arr(cur_[3], r.rs.p);
+        copy_arr(cur_[4], r.ts.p);
+        enqueue_buoyancy_confinement(dt);
+        enqueue_project(-1);
+        copy_arr(cur_[0], r.u3.p);
+        copy_arr(cur_[1], r.v3.p);
+        copy_arr(cur_[2], r.w3.p);
+        enqueue_advect_scalars(dt);
+    }
+
+    // Adjoint of one MacCormack advection: lam_out -> lam_src, lam_U/V/W.
+    void adv_adjoint(double dt, const T *src, const T *U, const T *V, const T *W, double diss,
+                     int clamp0, const T *lam_out, T *lam_src, T *lam_U, T *lam_V, T *lam_W) {
+        const T s = (T)(dt / h_);
+        k_sl_forward<T, 1><<<cell_grid_, cell_blk_, 0, st_>>>(g_, s, U, V, W, src, nullptr,
+                                                              fwd_[0].p, nullptr);
+        LAUNCH_CHECK();
+        zero(adj_lf_.p);
+        k_adj_mac_correct<T><<<cell_grid_, cell_blk_, 0, st_>>>(
+            g_, s, (T)diss, clamp0, src, fwd_[0].p, U, V, W, lam_out, lam_src, adj_lf_.p, lam_U,
+            lam_V, lam_W);
+        LAUNCH_CHECK();
+        k_adj_sl_forward<T><<<cell_grid_, cell_blk_, 0, st_>>>(g_, s, src, U, V, W, adj_lf_.p,
+                                                               lam_src, lam_U, lam_V, lam_W);
+        LAUNCH_CHECK();
+    }
+
+    // P^T = P: the forward projection applied to (lu, lv, lw) with its own
+    // pressure buffer (warm-started by the previous adjoint step).
+    void project_arrays(T *lu, T *lv, T *lw, int iters, double rel_tol) {
+        T *save[3] = {cur_[0], cur_[1], cur_[2]};
+        cur_[0] = lu;
+        cur_[1] = lv;
+        cur_[2] = lw;
+        std::swap(p_.p, adj_p_.p);
+        const int si = params.solver_iters;
+        const double st = params.cg_rel_tol;
+        const int sk = params.solver_kind;
+        params.solver_iters = iters;
+        params.cg_rel_tol = rel_tol;
+        params.solver_kind = AETHER_SOLVER_CG;
+        enqueue_project(-1);
+        params.solver_iters = si;
+        params.cg_rel_tol = st;
+        params.solver_kind = sk;
+        std::swap(p_.p, adj_p_.p);
+        for (int a = 0; a < 3; ++a) cur_[a] = save[a];
+    }
+
+    double adjoint_step(double dt, StepRec &r, const aether_rollout_desc &d) {
+        // a) density / temperature advection (velocity u3)
+        zero(adj_b_[0].p);
+        zero(adj_b_[1].p);
+        adv_adjoint(dt, r.rs.p, r.u3.p, r.v3.p, r.w3.p, params.dissipation_density, 1,
+                    adj_ar_.p, adj_b_[0].p, adj_a_[0].p, adj_a_[1].p, adj_a_[2].p);
+        adv_adjoint(dt, r.ts.p, r.u3.p, r.v3.p, r.w3.p, params.dissipation_temperature, 1,
+                    adj_at_.p, adj_b_[1].p, adj_a_[0].p, adj_a_[1].p, adj_a_[2].p);
+        // b) projection (self-adjoint)
+        project_arrays(adj_a_[0].p, adj_a_[1].p, adj_a_[2].p, d.adjoint_cg_iters,
+                       d.adjoint_cg_rel_tol);
+        // c) confinement + buoyancy + wall masks around velocity u1
+        const double inv_2h = 0.5 / h_;
+        k_curl_mag<T><<<cell_grid_, cell_blk_, 0, st_>>>(g_, (T)inv_2h, r.u1.p, r.v1.p, r.w1.p,
+                                                          curl_.p);
+        LAUNCH_CHECK();
+        for (int a = 0; a < 3; ++a) zero(adj_c_[a].p);
+        k_adj_confine1<T><<<cell_grid_, cell_blk_, 0, st_>>>(
+            g_, (T)dt, (T)inv_2h, (T)(params.vorticity_eps * h_), (T)params.buoyancy_beta,
+            (T)params.gravity_alpha, r.u1.p, r.v1.p, r.w1.p, curl_.p, r.ts.p, adj_a_[0].p,
+            adj_a_[1].p, adj_a_[2].p, adj_c_[0].p, adj_c_[1].p, adj_c_[2].p, adj_lw_[0].p,
+            adj_lw_[1].p, adj_lw_[2].p, adj_lg_[0].p, adj_lg_[1].p, adj_lg_[2].p, adj_b_[0].p,
+            adj_b_[1].p, adj_part_.p);
+        LAUNCH_CHECK();
+        const int ncell = (int)(cell_grid_.x * cell_grid_.y * cell_grid_.z);
+        k_partials_final<<<1, kThreads, 0, st_>>>(adj_part_.p, ncell, 0, scalar_.p);
+        LAUNCH_CHECK();
+        k_adj_confine2<T><<<cell_grid_, cell_blk_, 0, st_>>>(
+            g_, (T)inv_2h, r.u1.p, r.v1.p, r.w1.p, curl_.p, adj_lg_[0].p, adj_lg_[1].p,
+            adj_lg_[2].p, adj_lw_[0].p, adj_lw_[1].p, adj_lw_[2].p);
+        LAUNCH_CHECK();
+        k_adj_confine3<T><<<cell_grid_, cell_blk_, 0, st_>>>(g_, (T)inv_2h, adj_lw_[0].p,
+                                                            adj_lw_[1].p, adj_lw_[2].p,
+                                                            adj_c_[0].p, adj_c_[1].p, adj_c_[2].p);
+        LAUNCH_CHECK();
+        // d) inflow patch
+        const int cnt = patch_n_.x * patch_n_.y * patch_n_.z;
+        k_patch_adj<T><<<(cnt + kThreads - 1) / kThreads, cell_blk_, 0, st_>>>(
+            g_, patch_lo_, patch_n_, adj_b_[0].p, adj_b_[1].p, grad_c_.p);
+        LAUNCH_CHECK();
+        // e-g) velocity advections, last to first: w (vel u1, v1, w0),
+        // v (vel u1, v0, w0), u (vel u0, v0, w0)
+        for (int a = 0; a < 3; ++a) zero(adj_d_[a].p);
+        const double dv = params.dissipation_velocity;
+        adv_adjoint(dt, r.w0.p, r.u1.p, r.v1.p, r.w0.p, dv, 0, adj_c_[2].p, adj_d_[2].p,
+                    adj_c_[0].p, adj_c_[1].p, adj_d_[2].p);
+        adv_adjoint(dt, r.v0.p, r.u1.p, r.v0.p, r.w0.p, dv, 0, adj_c_[1].p, adj_d_[1].p,
+                    adj_c_[0].p, adj_d_[1].p, adj_d_[2].p);
+        adv_adjoint(dt, r.u0.p, r.u0.p, r.v0.p, r.w0.p, dv, 0, adj_c_[0].p, adj_d_[0].p,
+                    adj_d_[0].p, adj_d_[1].p, adj_d_[2].p);
+        for (int a = 0; a < 3; ++a) std::swap(adj_a_[a].p, adj_d_[a].p);
+        std::swap(adj_ar_.p, adj_b_[0].p);
+        std::swap(adj_at_.p, adj_b_[1].p);
+        return read_scalar();  // this step's d loss / d beta
+    }
+
+    double rollout(const aether_rollout_desc &d, const double *ctrl, double beta,
+                   const double *target, double *grad_c, double *grad_beta) override {
+        if (ir64_) raise(AETHER_EINVAL, "rollouts need AETHER_FP32 or AETHER_FP64 precision");
+        if (d.nsteps < 1 || d.checkpoint_every < 1) raise(AETHER_EINVAL, "bad rollout lengths");
+        const int nn[3] = {g_.nx, g_.ny, g_.nz};
+        for (int a = 0; a < 3; ++a)
+            if (d.patch_lo[a] < 1 || d.patch_hi[a] > nn[a] - 1 || d.patch_hi[a] <= d.patch_lo[a])
+                raise(AETHER_EINVAL, "patch must be a non-empty box of interior cells");
+        patch_lo_ = make_int3(d.patch_lo[0], d.patch_lo[1], d.patch_lo[2]);
+        patch_n_ = make_int3(d.patch_hi[0] - d.patch_lo[0], d.patch_hi[1] - d.patch_lo[1],
+                             d.patch_hi[2] - d.patch_lo[2]);
+        const size_t cnt = (size_t)patch_n_.x * patch_n_.y * patch_n_.z;
+        if (ctrl_.n < cnt) ctrl_.alloc(cnt);
+        if (grad_c_.n < cnt) grad_c_.alloc(cnt);
+        if (!target_.p) target_.alloc(n_);
+        HIPT(hipMemcpyAsync(ctrl_.p, ctrl, cnt * sizeof(double), hipMemcpyHostToDevice, st_));
+        HIPT(hipMemcpyAsync(target_.p, target, n_ * sizeof(double), hipMemcpyHostToDevice, st_));
+        const aether_fluid_params saved = params;
+        params.buoyancy_beta = beta;
+        patch_on_ = true;
+        const int K = d.checkpoint_every;
+        const int nseg = (d.nsteps + K - 1) / K;
+        while ((int)ckpts_.size() < nseg) ckpts_.push_back(std::make_unique<Ckpt>());
+        double loss = 0.0;
+        try {
+            for (int t = 0; t < d.nsteps; ++t) {
+                if (t % K == 0) save_ckpt(*ckpts_[t / K]);
+                step_plain(d.dt);
+            }
+            const bool want = grad_c || grad_beta;
+            if (want) {
+                for (DevBuf<T> *b : {&adj_a_[0], &adj_a_[1], &adj_a_[2], &adj_ar_, &adj_at_,
+                                     &adj_b_[0], &adj_b_[1], &adj_c_[0], &adj_c_[1], &adj_c_[2],
+                                     &adj_d_[0], &adj_d_[1], &adj_d_[2], &adj_lf_, &adj_lw_[0],
+                                     &adj_lw_[1], &adj_lw_[2], &adj_lg_[0], &adj_lg_[1],
+                                     &adj_lg_[2], &adj_p_})
+                    if (!b->p) b->alloc(n_);
+                if (!adj_part_.p) adj_part_.alloc(cell_grid_.x * cell_grid_.y * cell_grid_.z);
+            }
+            const unsigned lb = blocks_for(n_);
+            k_loss<T><<<lb, kThreads, 0, st_>>>(cur_[3], target_.p, (long)n_,
+                                                want ? adj_ar_.p : nullptr, partA_.p);
+            LAUNCH_CHECK();
+            k_partials_final<<<1, kThreads, 0, st_>>>(partA_.p, (int)lb, 0, scalar_.p);
+            LAUNCH_CHECK();
+            loss = read_scalar();
+            if (want) {
+                for (int a = 0; a < 3; ++a) zero(adj_a_[a].p);
+                zero(adj_at_.p);
+                zero(adj_p_.p);
+                k_fill<double><<<blocks_for(cnt), kThreads, 0, st_>>>(grad_c_.p, (long)cnt, 0.0);
+                LAUNCH_CHECK();
+                while ((int)recs_.size() < K) recs_.push_back(std::make_unique<StepRec>());
+                double gbeta = 0.0;
+                for (int seg = nseg - 1; seg >= 0; --seg) {
+                    load_ckpt(*ckpts_[seg]);
+                    const int t0 = seg * K, t1 = std::min(d.nsteps, t0 + K);
+                    for (int t = t0; t < t1; ++t) step_recorded(d.dt, *recs_[t - t0]);
+                    for (int t = t1 - 1; t >= t0; --t) gbeta += adjoint_step(d.dt, *recs_[t - t0], d);
+                }
+                if (grad_c)
+                    HIPT(hipMemcpyAsync(grad_c, grad_c_.p, cnt * sizeof(double),
+                                        hipMemcpyDeviceToHost, st_));
+                if (grad_beta) *grad_beta = gbeta;
+            }
+            load_ckpt(*ckpts_[0]);  // the solver's state is left as it was
+            sync();
+        } catch (...) {
+            params = saved;
+            patch_on_ = false;
+            throw;
+        }
+        params = saved;
+        patch_on_ = false;
+        return loss;
+    }
+
     const std::vector<int> &last_run_iters() const override { return last_run_iters_; }
 
    private:
@@ -934,6 +1181,14 @@ class Solver final : public SolverBase {
     Tiling tl_;
     int zu_ = 4;
     bool ir64_ = false;           // mixed-precision projection (AETHER_FP32_IR64)
+    // rollout / adjoint state
+    bool patch_on_ = false;
+    int3 patch_lo_ = {0, 0, 0}, patch_n_ = {0, 0, 0};
+    DevBuf<double> ctrl_, grad_c_, target_, adj_part_;
+    std::vector<std::unique_ptr<Ckpt>> ckpts_;
+    std::vector<std::unique_ptr<StepRec>> recs_;
+    DevBuf<T> adj_a_[3], adj_ar_, adj_at_, adj_b_[2], adj_c_[3], adj_d_[3], adj_lf_, adj_lw_[3],
+        adj_lg_[3], adj_p_;
     DevBuf<double> p64_, b64_, partD64_;
     double rel_override_ = -1.0;  // inner-solve tolerance while refining
     int ir_outer_ = 0, ir_inner_ = 0;
@@ -1676,6 +1931,16 @@ int aether_fluid_batch_iterations(aether_fluid_batch_t b, int32_t *out, int n, i
         const auto &v = b->impl->iters();
         if (count) *count = (int)v.size();
         for (int a = 0; a < n && a < (int)v.size(); ++a) out[a] = v[a];
+    });
+}
+
+int aether_fluid_rollout(aether_fluid_t s, const aether_rollout_desc *d, const double *controls,
+                         double beta, const double *target, double *loss, double *grad_controls,
+                         double *grad_beta) {
+    return guarded([&] {
+        checked(s);
+        if (!d || !controls || !target || !loss) raise(AETHER_EINVAL, "null argument");
+        *loss = s->impl->rollout(*d, controls, beta, target, grad_controls, grad_beta);
     });
 }
 

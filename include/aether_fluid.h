@@ -200,6 +200,30 @@ int aether_fluid_batch_run(aether_fluid_batch_t b, double dt, const double *time
 /* CG iterations of the last run, nsteps*count values, step-major. */
 int aether_fluid_batch_iterations(aether_fluid_batch_t b, int32_t *out, int n, int *count);
 
+/* Differentiable rollouts (no counterpart in the reference).  Starting from
+ * the solver's current state (which is left unchanged), run `nsteps` steps
+ * in which the inflow patch [patch_lo, patch_hi) has density and
+ * temperature SET to `controls` (patch cells, x fastest) in the source slot
+ * of src/fluid.rs:603 and buoyancy_beta = `beta`; return
+ * loss = sum over cells (density - target)^2.  If grad_controls / grad_beta
+ * are non-NULL, also run the hand-written adjoint (checkpoint every
+ * `checkpoint_every` steps, segments recomputed) and return dloss/dcontrols
+ * and dloss/dbeta.  The adjoint projection is solved by CG to
+ * adjoint_cg_rel_tol (max adjoint_cg_iters). */
+typedef struct {
+    int32_t nsteps;
+    int32_t checkpoint_every;
+    double dt;
+    int32_t patch_lo[3];
+    int32_t patch_hi[3];
+    int32_t adjoint_cg_iters;
+    int32_t reserved;
+    double adjoint_cg_rel_tol;
+} aether_rollout_desc;
+int aether_fluid_rollout(aether_fluid_t s, const aether_rollout_desc *d, const double *controls,
+                         double beta, const double *target, double *loss, double *grad_controls,
+                         double *grad_beta);
+
 /* Measurement hooks used by the benchmark driver. */
 /* Average device time (ms) per launch of the CG iteration kernels during the
  * last aether_fluid_run with timing enabled (HIP events on the solver's

@@ -432,3 +432,71 @@ def test_batched_equals_one_at_a_time_bitwise():
                                            rtol=1e-12, atol=1e-300)
     finally:
         os.environ.pop("AETHER_CG_ZU", None)
+
+
+def _rollout_setup(n, precision, tol=1e-12, iters=600):
+    import aether_fluid as af
+    from aether_fluid.scenarios import rollout_problem
+    prob = rollout_problem(n)
+    s = af.FluidSolver3D(n, n, n, precision=precision)
+    s.gravity_alpha, s.vorticity_eps = 0.05, 0.3
+    s.solver_kind, s.solver_iters, s.cg_rel_tol = 0, iters, tol
+    return s, prob
+
+
+def _directions(count, nctrl, seed=123):
+    rng = np.random.default_rng(seed)
+    return [(rng.uniform(-1, 1, nctrl), rng.uniform(-1, 1)) for _ in range(count)]
+
+
+def test_adjoint_fp64_matches_central_differences():
+    """FP64 forward + FP64 adjoint against FP64 central differences of the
+    loss along random directions (controls and beta)."""
+    _announce("adjoint fp64 vs FD 16^3")
+    n, steps = 16, 6
+    s, pb = _rollout_setup(n, "fp64")
+    beta = 1.0
+    args = dict(nsteps=steps, dt=0.02, patch_lo=pb["patch_lo"], patch_hi=pb["patch_hi"],
+                target=pb["target"], checkpoint_every=2)
+    loss, gc, gb = s.rollout(controls=pb["controls"], beta=beta, **args)
+    assert loss > 0 and np.all(np.isfinite(gc))
+    eps = 1e-6
+    for dc, db in _directions(4, len(pb["controls"])):
+        lp, _, _ = s.rollout(controls=pb["controls"] + eps * dc, beta=beta + eps * db,
+                             grad=False, **args)
+        lm, _, _ = s.rollout(controls=pb["controls"] - eps * dc, beta=beta - eps * db,
+                             grad=False, **args)
+        fd = (lp - lm) / (2 * eps)
+        ad = float(gc @ dc + gb * db)
+        print("fp64 dir", fd, ad)
+        assert abs(ad - fd) <= 1e-5 * abs(fd) + 1e-12, (ad, fd)
+
+
+def test_adjoint_fp32_matches_fp64_fd_and_checkpoints():
+    """FP32 adjoint (FP64 reductions) against FP64 central differences
+    (eps = 1e-3), and the gradient is independent of the checkpoint
+    interval; the solver state is left unchanged."""
+    _announce("adjoint fp32 vs fp64 FD 24^3")
+    n, steps = 24, 8
+    s32, pb = _rollout_setup(n, "fp32", tol=1e-7, iters=800)
+    s64, _ = _rollout_setup(n, "fp64")
+    u_before = s32.download("u")
+    args = dict(nsteps=steps, dt=0.02, patch_lo=pb["patch_lo"], patch_hi=pb["patch_hi"],
+                target=pb["target"])
+    loss, gc, gb = s32.rollout(controls=pb["controls"], beta=1.0, checkpoint_every=4, **args)
+    _, gc1, gb1 = s32.rollout(controls=pb["controls"], beta=1.0, checkpoint_every=1, **args)
+    np.testing.assert_allclose(gc1, gc, rtol=1e-4, atol=1e-6 * np.abs(gc).max())
+    assert abs(gb1 - gb) <= 1e-4 * abs(gb)
+    np.testing.assert_array_equal(s32.download("u"), u_before)
+    eps = 1e-3
+    worst = 0.0
+    for dc, db in _directions(4, len(pb["controls"]), seed=7):
+        lp, _, _ = s64.rollout(controls=pb["controls"] + eps * dc, beta=1.0 + eps * db,
+                               grad=False, checkpoint_every=4, **args)
+        lm, _, _ = s64.rollout(controls=pb["controls"] - eps * dc, beta=1.0 - eps * db,
+                               grad=False, checkpoint_every=4, **args)
+        fd = (lp - lm) / (2 * eps)
+        ad = float(gc @ dc + gb * db)
+        worst = max(worst, abs(ad - fd) / abs(fd))
+        print("fp32 dir", fd, ad)
+    assert worst <= 1e-3, worst
