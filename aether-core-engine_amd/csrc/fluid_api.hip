@@ -1117,11 +1117,19 @@ class Solver final : public SolverBase {
         while ((int)ckpts_.size() < nseg) ckpts_.push_back(std::make_unique<Ckpt>());
         double loss = 0.0;
         try {
+            const bool want = grad_c || grad_beta;
+            // the last segment is recorded during the forward sweep, so the
+            // reverse sweep recomputes every segment but that one
+            const int last0 = (nseg - 1) * K;
+            if (want)
+                while ((int)recs_.size() < K) recs_.push_back(std::make_unique<StepRec>());
             for (int t = 0; t < d.nsteps; ++t) {
                 if (t % K == 0) save_ckpt(*ckpts_[t / K]);
-                step_plain(d.dt);
+                if (want && t >= last0)
+                    step_recorded(d.dt, *recs_[t - last0]);
+                else
+                    step_plain(d.dt);
             }
-            const bool want = grad_c || grad_beta;
             if (want) {
                 for (DevBuf<T> *b : {&adj_a_[0], &adj_a_[1], &adj_a_[2], &adj_ar_, &adj_at_,
                                      &adj_b_[0], &adj_b_[1], &adj_c_[0], &adj_c_[1], &adj_c_[2],
@@ -1144,12 +1152,13 @@ class Solver final : public SolverBase {
                 zero(adj_p_.p);
                 k_fill<double><<<blocks_for(cnt), kThreads, 0, st_>>>(grad_c_.p, (long)cnt, 0.0);
                 LAUNCH_CHECK();
-                while ((int)recs_.size() < K) recs_.push_back(std::make_unique<StepRec>());
                 double gbeta = 0.0;
                 for (int seg = nseg - 1; seg >= 0; --seg) {
-                    load_ckpt(*ckpts_[seg]);
                     const int t0 = seg * K, t1 = std::min(d.nsteps, t0 + K);
-                    for (int t = t0; t < t1; ++t) step_recorded(d.dt, *recs_[t - t0]);
+                    if (seg != nseg - 1) {
+                        load_ckpt(*ckpts_[seg]);
+                        for (int t = t0; t < t1; ++t) step_recorded(d.dt, *recs_[t - t0]);
+                    }
                     for (int t = t1 - 1; t >= t0; --t) gbeta += adjoint_step(d.dt, *recs_[t - t0], d);
                 }
                 if (grad_c)

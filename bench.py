@@ -172,6 +172,76 @@ def bench_mixed(steps: int, warmup: int) -> dict:
     return out
 
 
+def bench_rollout(n: int, steps: int, ckpt: int, ndirs: int = 0, reps: int = 2,
+                  init_state=None) -> dict:
+    """Differentiable rollout at n^3: FP32 forward, FP32 adjoint with FP64
+    reductions, checkpoint every `ckpt` steps; inflow-patch controls and
+    buoyancy beta; loss = sum (rho_T - target)^2.  Times the forward-only
+    rollout against forward + adjoint, and (ndirs > 0) checks the gradient
+    against FP64 central differences (eps = 1e-3) along seeded directions."""
+    import aether_fluid as af
+    from aether_fluid.scenarios import rollout_problem
+    pb = rollout_problem(n)
+    s = af.FluidSolver3D(n, n, n, precision="fp32")
+    s.gravity_alpha, s.vorticity_eps = 0.05, 0.3
+    s.solver_kind, s.solver_iters, s.cg_rel_tol = 0, 200, 1e-5
+    if init_state is not None:
+        for k, a in init_state.items():
+            s.upload(k, a)
+    args = dict(nsteps=steps, dt=0.02, patch_lo=pb["patch_lo"], patch_hi=pb["patch_hi"],
+                target=pb["target"], checkpoint_every=ckpt)
+    s.rollout(controls=pb["controls"], beta=1.0, grad=True, **args)  # warm-up
+    t_f, t_fa = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        s.rollout(controls=pb["controls"], beta=1.0, grad=False, **args)
+        t_f.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        loss, gc, gb = s.rollout(controls=pb["controls"], beta=1.0, grad=True, **args)
+        t_fa.append(time.perf_counter() - t0)
+    tf, tfa = min(t_f), min(t_fa)
+    nvox = n ** 3
+    out = {"workload": f"differentiable rollout {n}^3, {steps} steps, checkpoint every {ckpt}",
+           "fwd_voxel_steps_per_s": nvox * steps / tf,
+           "fwd_adjoint_voxel_steps_per_s": nvox * steps / tfa,
+           "fwd_adjoint_over_fwd": tfa / tf, "loss": loss, "grad_beta": gb}
+    if ndirs:
+        # The loss is piecewise smooth (MacCormack clamp, floor, near-wall
+        # switches): central differences converge to the derivative only when
+        # eps is small enough not to straddle kinks.  Report the FP32 adjoint
+        # against (a) the FP64 adjoint, (b) FP64 central differences at eps =
+        # 1e-7 and (c) at the specified eps = 1e-3.
+        s64 = af.FluidSolver3D(n, n, n, precision="fp64")
+        s64.gravity_alpha, s64.vorticity_eps = 0.05, 0.3
+        s64.solver_kind, s64.solver_iters, s64.cg_rel_tol = 0, 3000, 1e-12
+        if init_state is not None:
+            for k, a in init_state.items():
+                s64.upload(k, a)
+        _, gc64, gb64 = s64.rollout(controls=pb["controls"], beta=1.0, grad=True, **args)
+        rng = np.random.default_rng(0xAE7E0003)
+        errs = {"fp64_adjoint": [], "fd_1e-7": [], "fd_1e-3": []}
+        fd64 = {"fd_1e-7": [], "fd_1e-3": []}
+        for _ in range(ndirs):
+            dc, db = rng.uniform(-1, 1, len(pb["controls"])), rng.uniform(-1, 1)
+            ad = float(gc @ dc + gb * db)
+            ad64 = float(gc64 @ dc + gb64 * db)
+            errs["fp64_adjoint"].append(abs(ad - ad64) / abs(ad64))
+            for key, eps in (("fd_1e-7", 1e-7), ("fd_1e-3", 1e-3)):
+                lp, _, _ = s64.rollout(controls=pb["controls"] + eps * dc, beta=1.0 + eps * db,
+                                       grad=False, **args)
+                lm, _, _ = s64.rollout(controls=pb["controls"] - eps * dc, beta=1.0 - eps * db,
+                                       grad=False, **args)
+                fd = (lp - lm) / (2 * eps)
+                errs[key].append(abs(ad - fd) / abs(fd))
+                fd64[key].append(abs(ad64 - fd) / abs(fd))
+        out["fd_directions"] = ndirs
+        out["fp32_adjoint_rel_err"] = {k: {"max": max(v), "median": float(np.median(v))}
+                                       for k, v in errs.items()}
+        out["fp64_adjoint_rel_err"] = {k: {"max": max(v), "median": float(np.median(v))}
+                                       for k, v in fd64.items()}
+    return out
+
+
 def bench_batched(steps: int, warmup: int, dt: float = 0.01) -> dict:
     """64 grids drawn from {32^3, 48^3, 64^3, 96^3, 128x96x128}: one launch
     per stage for all of them, against the same grids stepped one at a time
@@ -220,8 +290,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the fwd+adjoint rollout")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
-    ap.add_argument("--workload", default="plain_128", choices=["plain_128", "mixed_256", "batched_64"],
+    ap.add_argument("--workload", default="plain_128", choices=["plain_128", "mixed_256", "batched_64", "rollout_64"],
                     help="plain_128: the headline line; mixed_256: mixed-precision "
                          "projection at 256^3 against the all-FP64 projection")
     ap.add_argument("--probe", action="store_true",
@@ -234,6 +305,9 @@ def main():
         sys.exit(2)
 
     import aether_fluid as af
+    if args.workload == "rollout_64":
+        print(json.dumps(bench_rollout(64, 40, 8, ndirs=16)), flush=True)
+        return
     if args.workload == "batched_64":
         print(json.dumps(bench_batched(args.steps, args.warmup)), flush=True)
         return
@@ -281,8 +355,18 @@ def main():
         "rbgs": {"value": nvox * args.steps / el_rb, "ms_per_step": el_rb * 1e3 / args.steps,
                  "sweeps": cfg["rbgs_sweeps"], "omega": cfg["sor_omega"]},
         "roofline": roofline,
-        "fwd_adjoint": None,
+        "fwd_adjoint": None,  # filled below
     }
+    if not args.no_adjoint:
+        from aether_fluid.scenarios import seeded_config_state
+        ro = bench_rollout(128, 8, 8, ndirs=0, reps=1,
+                           init_state=seeded_config_state(WORKLOAD))
+        line["fwd_adjoint"] = {"value": ro["fwd_adjoint_voxel_steps_per_s"],
+                               "unit": "voxel-steps/s",
+                               "fwd_value": ro["fwd_voxel_steps_per_s"],
+                               "fwd_adjoint_over_fwd": ro["fwd_adjoint_over_fwd"],
+                               "workload": "128^3 rollout, 8 steps, checkpoint every 8, "
+                                           "inflow-patch controls + beta, FP32 adjoint"}
     if run_iters:
         line["cg_iters_mean"] = float(np.mean(run_iters))
         line["step_algorithmic_gbps"] = (sum(algorithmic_bytes_per_step(nvox, k)
