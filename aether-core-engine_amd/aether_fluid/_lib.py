@@ -24,6 +24,7 @@ AETHER_ENODEV = -4
 FP32, FP64, FP32_IR64 = 0, 1, 2
 SOLVER_CG, SOLVER_RBGS = 0, 1
 SOURCE_JETS, SOURCE_SPHERE, SOURCE_NONE = 0, 1, 2
+CG_AUTO, CG_SCALAR, CG_X4, CG_FUSED = 0, 1, 2, 3
 FIELDS = {"u": 0, "v": 1, "w": 2, "density": 3, "temperature": 4, "pressure": 5,
           "divergence": 6}
 
@@ -41,7 +42,7 @@ class Params(C.Structure):
         ("dissipation_density", C.c_double),
         ("dissipation_temperature", C.c_double),
         ("source_kind", C.c_int32),
-        ("reserved0", C.c_int32),
+        ("cg_form", C.c_int32),
         ("source_center", C.c_double * 3),
         ("source_radius", C.c_double),
         ("source_density", C.c_double),

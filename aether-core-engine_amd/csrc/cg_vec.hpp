@@ -1,0 +1,286 @@
+// cg_vec.hpp -- the two-launch CG iteration with four x-cells per thread.
+//
+// Same arithmetic as body_cg_dir / body_cg_update (src/fluid.rs:463-487),
+// laid out for the memory pipeline: every thread owns a 4 (x) x ZU (z)
+// block, moves it with 16-byte loads and stores, and takes its x-neighbours
+// from the adjacent lanes (cross-lane permute) instead of from memory; only
+// lanes at a row-segment or wavefront edge load their x-neighbour.  A
+// one-cell-per-thread stencil issues ~10 four-byte memory instructions per
+// cell and is limited by the address unit, not by HBM; here it is ~2
+// sixteen-byte ones per four cells.  Needs nx % 4 == 0 (rows start
+// 16-byte aligned).
+#pragma once
+#include "fluid_kernels.hpp"
+
+namespace aether {
+
+// Workgroups of 256 threads cover txv float4 lanes of ty rows (txv = nx/4
+// capped at 64, ty = 256 / txv); threads past txv * ty idle.
+struct VTiling {
+    int gx, gy, gz, nb, remap;
+    int txv, ty;
+};
+
+__device__ __forceinline__ void vtile_coords(const VTiling &t, int &bx, int &by, int &bz) {
+    int b = blockIdx.x;
+    if (t.remap) b = (b & 7) * (t.nb >> 3) + (b >> 3);
+    bx = b % t.gx;
+    const int rest = b / t.gx;
+    by = rest % t.gy;
+    bz = rest / t.gy;
+}
+
+struct VCol {
+    int i0, j, k0;       // first cell (x multiple of 4), row, first plane
+    int off;             // in-plane offset of the (clamped) first cell
+    int offym, offyp;    // the same for the rows below / above (clamped)
+    int lane;
+    bool valid;          // owns four in-grid cells
+    bool needL, needR;   // x-neighbour comes from memory, not from the adjacent lane
+};
+
+template <int ZU>
+__device__ __forceinline__ VCol vcol_at(const Dims &g, const VTiling &vt, int bx, int by,
+                                        int bz) {
+    const int t = threadIdx.x;
+    const int txv = t % vt.txv, ty = t / vt.txv;
+    VCol c;
+    c.i0 = 4 * (bx * vt.txv + txv);
+    c.j = by * vt.ty + ty;
+    c.k0 = bz * ZU;
+    c.valid = ty < vt.ty && c.i0 < g.nx && c.j < g.ny;
+    const int ic0 = min(c.i0, g.nx - 4), jc = min(c.j, g.ny - 1);
+    c.off = jc * g.nx + ic0;
+    c.offym = max(jc - 1, 0) * g.nx + ic0;
+    c.offyp = min(jc + 1, g.ny - 1) * g.nx + ic0;
+    c.lane = t & 63;
+    // lane - 1 holds x-4..x-1 when it is in the same row segment and wave;
+    // lane + 1 likewise (an idle or other-row lane only ever sits next to a
+    // segment edge, which loads instead)
+    c.needL = c.valid && c.i0 > 0 && (c.lane == 0 || txv == 0);
+    c.needR = c.valid && c.i0 + 4 < g.nx && (c.lane == 63 || txv == vt.txv - 1);
+    return c;
+}
+
+__device__ __forceinline__ float f4(const float4 &v, int e) {
+    return e == 0 ? v.x : e == 1 ? v.y : e == 2 ? v.z : v.w;
+}
+__device__ __forceinline__ float4 axpy4(const float4 &x, float a, const float4 &y) {
+    return make_float4(x.x + a * y.x, x.y + a * y.y, x.z + a * y.z, x.w + a * y.w);
+}
+// value of lane `src` (every lane of the wave must execute this)
+__device__ __forceinline__ float lane_value(float v, int src) {
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(v)));
+}
+
+// A x on the four cells of plane k of a column (lap7 per cell, interior only);
+// returns the products x.Ax summed when `dot` is set.
+__device__ __forceinline__ void lap4(const Dims &g, const VCol &c, int k, const float4 &x,
+                                     float xl, float xr, const float4 &ym, const float4 &yp,
+                                     const float4 &zm, const float4 &zp, float (&q)[4]) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int i = c.i0 + e;
+        q[e] = 0.0f;
+        if (is_interior(g, i, c.j, k))
+            q[e] = lap7<float>(f4(x, e), e == 0 ? xl : f4(x, e - 1), e == 3 ? xr : f4(x, e + 1),
+                               f4(ym, e), f4(yp, e), f4(zm, e), f4(zp, e), i > 1, i + 2 < g.nx,
+                               c.j > 1, c.j + 2 < g.ny, k > 1, k + 2 < g.nz);
+    }
+}
+
+// CG launch 1 of iteration `it` (as body_cg_dir): stop rules on r.r, then
+// d = r + beta d_old, q = A d, partial d.q; writes d.
+template <int ZU, bool FIRST>
+__global__ __launch_bounds__(kThreads) void k_cg_dir4(
+    Dims g, VTiling vt, int it, double rel_tol2, int nb_in, const double *__restrict__ part_rr,
+    const double *__restrict__ part_bb, CgState *st, const float *__restrict__ r,
+    const float *__restrict__ dold, float *__restrict__ dnew, double *__restrict__ part_dq,
+    int *done_mirror, int solve_id) {
+    __shared__ double lds[8];
+    __shared__ int flag_lds[1];
+    int bx, by, bz;
+    vtile_coords(vt, bx, by, bz);
+    const VCol c = vcol_at<ZU>(g, vt, bx, by, bz);
+    const int slot = blockIdx.x;
+    const int flag_reg = st->done;
+    EarlyPartialsT<4> prr, pbb;
+    prr.load(part_rr, nb_in);
+    if (FIRST) pbb.load(part_bb, nb_in);
+    const double delta_prev = FIRST ? 0.0 : st->delta[(it - 1) & 1];
+    const double b2_prev = FIRST ? 0.0 : st->b2;
+    const float4 *r4 = reinterpret_cast<const float4 *>(r);
+    const float4 *d4 = reinterpret_cast<const float4 *>(dold);
+    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 rz[ZU + 2], dz[ZU + 2], ry[ZU][2], dy[ZU][2];
+    float xr[ZU][2], xd[ZU][2];
+#pragma unroll
+    for (int a = 0; a < ZU + 2; ++a) {
+        const int base = min(max(c.k0 - 1 + a, 0), g.nz - 1) * g.sxy;
+        rz[a] = r4[(base + c.off) >> 2];
+        dz[a] = FIRST ? z4 : d4[(base + c.off) >> 2];
+    }
+#pragma unroll
+    for (int a = 0; a < ZU; ++a) {
+        const int base = min(c.k0 + a, g.nz - 1) * g.sxy;
+        ry[a][0] = r4[(base + c.offym) >> 2];
+        ry[a][1] = r4[(base + c.offyp) >> 2];
+        dy[a][0] = FIRST ? z4 : d4[(base + c.offym) >> 2];
+        dy[a][1] = FIRST ? z4 : d4[(base + c.offyp) >> 2];
+        xr[a][0] = xr[a][1] = xd[a][0] = xd[a][1] = 0.0f;
+        if (c.needL) {
+            xr[a][0] = r[base + c.off - 1];
+            if (!FIRST) xd[a][0] = dold[base + c.off - 1];
+        }
+        if (c.needR) {
+            xr[a][1] = r[base + c.off + 4];
+            if (!FIRST) xd[a][1] = dold[base + c.off + 4];
+        }
+    }
+    // stopping rules (every workgroup reaches the same decision)
+    const bool lead = slot == 0 && threadIdx.x == 0;
+    double beta = 0.0;
+    if (threadIdx.x == 0) flag_lds[0] = flag_reg;
+    const double rr = prr.total(part_rr, nb_in, lds);  // its barrier publishes flag_lds
+    if (flag_lds[0]) return;
+    if (FIRST) {
+        const double bb = pbb.total(part_bb, nb_in, lds);
+        const bool stop = !(rr > 1e-24) || (rel_tol2 > 0.0 && rr <= rel_tol2 * bb);
+        if (lead) {
+            st->b2 = bb;
+            st->r2_init = rr;
+            st->delta[0] = rr;
+            if (stop) {
+                st->done = 1;
+                st->iters = 0;
+                st->r2_final = rr;
+                st->converged = 1;
+                *done_mirror = solve_id;
+            }
+        }
+        if (stop) return;
+    } else {
+        const bool stop = rr < 1e-20 || (rel_tol2 > 0.0 && rr <= rel_tol2 * b2_prev);
+        if (lead) {
+            if (stop) {
+                st->done = 1;
+                st->iters = it;
+                st->r2_final = rr;
+                st->converged = 1;
+                *done_mirror = solve_id;
+            } else {
+                st->delta[it & 1] = rr;
+            }
+        }
+        if (stop) return;
+        beta = rr / delta_prev;
+    }
+    const float tb = (float)beta;
+    float4 dc[ZU + 2];
+#pragma unroll
+    for (int a = 0; a < ZU + 2; ++a) dc[a] = FIRST ? rz[a] : axpy4(rz[a], tb, dz[a]);
+    const int srcL = (c.lane + 63) & 63, srcR = (c.lane + 1) & 63;
+    double acc[1] = {0.0};
+    float4 *dn4 = reinterpret_cast<float4 *>(dnew);
+#pragma unroll
+    for (int a = 0; a < ZU; ++a) {
+        const int k = c.k0 + a;
+        const float4 ym = FIRST ? ry[a][0] : axpy4(ry[a][0], tb, dy[a][0]);
+        const float4 yp = FIRST ? ry[a][1] : axpy4(ry[a][1], tb, dy[a][1]);
+        const float sl = lane_value(dc[a + 1].w, srcL), sr = lane_value(dc[a + 1].x, srcR);
+        const float xl = c.needL ? (FIRST ? xr[a][0] : xr[a][0] + tb * xd[a][0]) : sl;
+        const float xrv = c.needR ? (FIRST ? xr[a][1] : xr[a][1] + tb * xd[a][1]) : sr;
+        if (c.valid && k < g.nz) {
+            float q[4];
+            lap4(g, c, k, dc[a + 1], xl, xrv, ym, yp, dc[a], dc[a + 2], q);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) acc[0] += (double)f4(dc[a + 1], e) * (double)q[e];
+            dn4[(k * g.sxy + c.off) >> 2] = dc[a + 1];
+        }
+    }
+    block_sum<1>(acc, lds);
+    if (threadIdx.x == 0) part_dq[slot] = acc[0];
+}
+
+// CG launch 2 of iteration `it` (as body_cg_update): alpha from d.q,
+// p += alpha d, r -= alpha A d (A d recomputed), partial r.r.
+template <int ZU>
+__global__ __launch_bounds__(kThreads) void k_cg_update4(
+    Dims g, VTiling vt, int it, const double *__restrict__ part_dq, CgState *st,
+    const float *__restrict__ d, float *__restrict__ p, float *__restrict__ r,
+    double *__restrict__ part_rr, int *done_mirror, int solve_id) {
+    __shared__ double lds[8];
+    __shared__ int flag_lds[1];
+    int bx, by, bz;
+    vtile_coords(vt, bx, by, bz);
+    const VCol c = vcol_at<ZU>(g, vt, bx, by, bz);
+    const int slot = blockIdx.x;
+    const int nb = vt.nb;
+    const int flag_reg = st->done;
+    EarlyPartialsT<4> pdq;
+    pdq.load(part_dq, nb);
+    const double delta = st->delta[it & 1];
+    const float4 *d4 = reinterpret_cast<const float4 *>(d);
+    float4 *p4 = reinterpret_cast<float4 *>(p);
+    float4 *r4 = reinterpret_cast<float4 *>(r);
+    float4 dz[ZU + 2], dy[ZU][2], pc[ZU], rc[ZU];
+    float xd[ZU][2];
+#pragma unroll
+    for (int a = 0; a < ZU + 2; ++a) {
+        const int base = min(max(c.k0 - 1 + a, 0), g.nz - 1) * g.sxy;
+        dz[a] = d4[(base + c.off) >> 2];
+    }
+#pragma unroll
+    for (int a = 0; a < ZU; ++a) {
+        const int base = min(c.k0 + a, g.nz - 1) * g.sxy;
+        dy[a][0] = d4[(base + c.offym) >> 2];
+        dy[a][1] = d4[(base + c.offyp) >> 2];
+        pc[a] = p4[(base + c.off) >> 2];
+        rc[a] = r4[(base + c.off) >> 2];
+        xd[a][0] = xd[a][1] = 0.0f;
+        if (c.needL) xd[a][0] = d[base + c.off - 1];
+        if (c.needR) xd[a][1] = d[base + c.off + 4];
+    }
+    const bool lead = slot == 0 && threadIdx.x == 0;
+    if (threadIdx.x == 0) flag_lds[0] = flag_reg;
+    const double dq = pdq.total(part_dq, nb, lds);
+    if (flag_lds[0]) return;
+    if (fabs(dq) < 1e-28) {
+        if (lead) {
+            st->done = 1;
+            st->iters = it;
+            st->converged = 1;
+            st->r2_final = delta;
+            *done_mirror = solve_id;
+        }
+        return;
+    }
+    const float ta = (float)(delta / dq);
+    const int srcL = (c.lane + 63) & 63, srcR = (c.lane + 1) & 63;
+    double acc[1] = {0.0};
+#pragma unroll
+    for (int a = 0; a < ZU; ++a) {
+        const int k = c.k0 + a;
+        const float sl = lane_value(dz[a + 1].w, srcL), sr = lane_value(dz[a + 1].x, srcR);
+        const float xl = c.needL ? xd[a][0] : sl;
+        const float xrv = c.needR ? xd[a][1] : sr;
+        if (c.valid && k < g.nz) {
+            float q[4];
+            lap4(g, c, k, dz[a + 1], xl, xrv, dy[a][0], dy[a][1], dz[a], dz[a + 2], q);
+            const float4 pn = axpy4(pc[a], ta, dz[a + 1]);
+            const float4 rn = make_float4(rc[a].x - ta * q[0], rc[a].y - ta * q[1],
+                                          rc[a].z - ta * q[2], rc[a].w - ta * q[3]);
+            acc[0] += (double)rn.x * (double)rn.x;
+            acc[0] += (double)rn.y * (double)rn.y;
+            acc[0] += (double)rn.z * (double)rn.z;
+            acc[0] += (double)rn.w * (double)rn.w;
+            const int id4 = (k * g.sxy + c.off) >> 2;
+            p4[id4] = pn;
+            r4[id4] = rn;
+        }
+    }
+    block_sum<1>(acc, lds);
+    if (threadIdx.x == 0) part_rr[slot] = acc[0];
+}
+
+}  // namespace aether

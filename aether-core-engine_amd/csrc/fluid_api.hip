@@ -22,6 +22,8 @@
 #include "aether_fluid.h"
 #include "fluid_kernels.hpp"
 #include "cg_resident.hpp"
+#include "cg_fused.hpp"
+#include "cg_vec.hpp"
 #include "batch_kernels.hpp"
 #include "adjoint_kernels.hpp"
 
@@ -47,6 +49,37 @@ struct Failure {
     } while (0)
 
 #define LAUNCH_CHECK() HIPT(hipGetLastError())
+
+// Instantiates a launch for the z cells per thread of the four-wide CG tiling.
+#define VZU_DISPATCH(zu, launch)                \
+    do {                                        \
+        if ((zu) == 2) {                        \
+            constexpr int VZU = 2;              \
+            launch;                             \
+        } else {                                \
+            constexpr int VZU = 4;              \
+            launch;                             \
+        }                                       \
+    } while (0)
+
+// Instantiates a launch for the z planes per workgroup of the one-launch CG.
+#define KZ_DISPATCH(kz, launch)                 \
+    do {                                        \
+        switch (kz) {                           \
+        case 8: {                               \
+            constexpr int KZ = 8;               \
+            launch;                             \
+        } break;                                \
+        case 32: {                              \
+            constexpr int KZ = 32;              \
+            launch;                             \
+        } break;                                \
+        default: {                              \
+            constexpr int KZ = 16;              \
+            launch;                             \
+        } break;                                \
+        }                                       \
+    } while (0)
 
 // Instantiates a launch for the z-cells-per-thread of the CG tiling.
 #define ZU_DISPATCH(zu, launch)                 \
@@ -129,6 +162,8 @@ void check_params(const aether_fluid_params &p) {
     if (p.solver_kind != AETHER_SOLVER_CG && p.solver_kind != AETHER_SOLVER_RBGS)
         raise(AETHER_EINVAL, "solver_kind must be AETHER_SOLVER_CG or AETHER_SOLVER_RBGS");
     if (p.solver_iters < 0) raise(AETHER_EINVAL, "solver_iters must be >= 0");
+    if (p.cg_form < AETHER_CG_AUTO || p.cg_form > AETHER_CG_FUSED)
+        raise(AETHER_EINVAL, "cg_form must be one of AETHER_CG_AUTO/SCALAR/X4/FUSED");
     if (p.source_kind < AETHER_SOURCE_JETS || p.source_kind > AETHER_SOURCE_NONE)
         raise(AETHER_EINVAL, "unknown source_kind");
     if (!(p.cg_rel_tol >= 0.0)) raise(AETHER_EINVAL, "cg_rel_tol must be >= 0");
@@ -150,8 +185,8 @@ struct KernelTimer {
     };
     std::vector<Rec> recs;
     size_t used = 0;
-    double sum_ms[3] = {0, 0, 0};  // cg_dir, cg_update, cg_resident
-    long count[3] = {0, 0, 0};
+    double sum_ms[4] = {0, 0, 0, 0};  // cg_dir, cg_update, cg_resident, cg_fused
+    long count[4] = {0, 0, 0, 0};
     hipEvent_t get() {
         if (used == pool.size()) {
             hipEvent_t e;
@@ -163,7 +198,7 @@ struct KernelTimer {
     void reset() {
         recs.clear();
         used = 0;
-        for (int a = 0; a < 3; ++a) {
+        for (int a = 0; a < 4; ++a) {
             sum_ms[a] = 0;
             count[a] = 0;
         }
@@ -240,7 +275,45 @@ class Solver final : public SolverBase {
         tl_.nb = gx * gy * tl_.gz;
         tl_.remap = (tl_.nb % 8 == 0) ? 1 : 0;
         if (const char *e = std::getenv("AETHER_XCD_REMAP")) tl_.remap &= std::atoi(e) != 0;
-        const int npart = std::max(nb_, tl_.nb);
+        // AETHER_CG_FORM (1/2/3) overrides the AETHER_CG_AUTO choice
+        if (const char *e = std::getenv("AETHER_CG_FORM")) {
+            const int f = std::atoi(e);
+            if (f >= AETHER_CG_SCALAR && f <= AETHER_CG_FUSED) env_form_ = f;
+        }
+        // four-wide two-launch CG: 256-thread groups of txv float4 lanes x ty
+        // rows x vzu planes (vzu = 4 measured faster than 2 at 128^3 and 256^3)
+        if (nx % 4 == 0) {
+            const int w4 = nx / 4;
+            tlv_.txv = std::min(w4, 64);
+            tlv_.ty = kThreads / tlv_.txv;
+            tlv_.gx = (w4 + tlv_.txv - 1) / tlv_.txv;
+            tlv_.gy = (ny + tlv_.ty - 1) / tlv_.ty;
+            vzu_ = 4;
+            if (const char *e = std::getenv("AETHER_CG_VZU")) {
+                const int z = std::atoi(e);
+                if (z == 2 || z == 4) vzu_ = z;
+            }
+            tlv_.gz = (nz + vzu_ - 1) / vzu_;
+            tlv_.nb = tlv_.gx * tlv_.gy * tlv_.gz;
+            tlv_.remap = (tlv_.nb % 8 == 0) ? 1 : 0;
+            if (const char *e = std::getenv("AETHER_XCD_REMAP")) tlv_.remap &= std::atoi(e) != 0;
+        }
+        // one-launch CG: 64 x 4 x KZ tiles, KZ = 16 unless overridden
+        if (const char *e = std::getenv("AETHER_CG_KZ")) {
+            const int z = std::atoi(e);
+            if (z == 8 || z == 16 || z == 32) kz_ = z;
+        }
+        if (const char *e = std::getenv("AETHER_CG_PF")) {
+            const int f = std::atoi(e);
+            if (f == 2 || f == 4 || f == 6) pf_ = f;
+        }
+        tlf_.gx = gx;
+        tlf_.gy = gy;
+        tlf_.gz = (nz + kz_ - 1) / kz_;
+        tlf_.nb = gx * gy * tlf_.gz;
+        tlf_.remap = (tlf_.nb % 8 == 0) ? 1 : 0;
+        if (const char *e = std::getenv("AETHER_XCD_REMAP")) tlf_.remap &= std::atoi(e) != 0;
+        const int npart = std::max(std::max(nb_, tlv_.nb), std::max(tl_.nb, tlf_.nb));
         if (const char *e = std::getenv("AETHER_CG_CHUNK")) cg_chunk_ = std::max(0, std::atoi(e));
         HIPT(hipHostMalloc((void **)&done_host_, 64, hipHostMallocMapped));
         *done_host_ = 0;
@@ -494,7 +567,129 @@ class Solver final : public SolverBase {
         HIPT(hipExtLaunchKernel(fn, dim3(tl_.nb), cell_blk_, ptrs, 0, st_, e0, e1, 0));
     }
 
+    // One-launch CG iteration `it` (cg_fused.hpp).  gamma partials alternate
+    // between partB_ (even it: written by the initialisation) and partA_,
+    // delta partials between partE_ and partF_, r between r_ and r2_.
+    void enqueue_cg_fused(int it, double rel2, int solve_id, int step_index) {
+        const int nb_in = it == 0 ? tl_.nb : tlf_.nb;
+        const double *pg_in = (it & 1) ? partA_.p : partB_.p;
+        const double *pd_in = (it & 1) ? partF_.p : partE_.p;
+        double *pg_out = (it & 1) ? partB_.p : partA_.p;
+        double *pd_out = (it & 1) ? partE_.p : partF_.p;
+        const double *pbb = partC_.p;
+        const T *r_in = (it & 1) ? r2_.p : r_.p;
+        T *r_out = (it & 1) ? r_.p : r2_.p;
+        const T *d_in = d_[(it + 1) & 1].p;
+        T *d_out = d_[it & 1].p;
+        T *pp = p_.p;
+        CgState *st = cg_.p;
+        int *dm = done_dev_;
+        if constexpr (sizeof(T) == 4) {
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            if (timer.enabled) {
+                e0 = timer.get();
+                e1 = timer.get();
+                timer.recs.push_back({3, step_index, it, e0, e1});
+            }
+            KZ_DISPATCH(kz_, ({
+                const void *fn;
+                switch (pf_) {
+                case 2:
+                    fn = it == 0 ? (const void *)k_cg_fused<T, KZ, 2, true>
+                                 : (const void *)k_cg_fused<T, KZ, 2, false>;
+                    break;
+                case 6:
+                    fn = it == 0 ? (const void *)k_cg_fused<T, KZ, 6, true>
+                                 : (const void *)k_cg_fused<T, KZ, 6, false>;
+                    break;
+                default:
+                    fn = it == 0 ? (const void *)k_cg_fused<T, KZ, 4, true>
+                                 : (const void *)k_cg_fused<T, KZ, 4, false>;
+                }
+                Dims g = g_;
+                Tiling tl = tlf_;
+                int itv = it, nbv = nb_in, sid = solve_id;
+                double rt = rel2;
+                void *args[] = {&g,   &tl,    &itv, &rt, &nbv, &pg_in, &pd_in, &pbb, &pg_out,
+                                &pd_out, &st, &r_in, &r_out, &d_in, &d_out, &pp, &dm, &sid};
+                HIPT(hipExtLaunchKernel(fn, dim3(tlf_.nb), cell_blk_, args, 0, st_, e0, e1, 0));
+            }));
+        } else {
+            raise(AETHER_EINVAL, "the one-launch CG is built for FP32 only");
+        }
+    }
+
+    // The CG layout of the next solve (AETHER_CG_SCALAR/X4/FUSED).
+    int effective_form() const {
+        if constexpr (sizeof(T) != 4) {
+            return AETHER_CG_SCALAR;
+        } else {
+            int f = params.cg_form;
+            if (f == AETHER_CG_AUTO)
+                f = env_form_ ? env_form_ : (g_.nx % 4 == 0 ? AETHER_CG_X4 : AETHER_CG_SCALAR);
+            if (f == AETHER_CG_X4 && g_.nx % 4 != 0)
+                raise(AETHER_EINVAL, "AETHER_CG_X4 needs nx % 4 == 0");
+            return f;
+        }
+    }
+
+    void ensure_fused_buffers() {
+        if (r2_.p) return;
+        const size_t npart = (size_t)std::max(std::max(nb_, tlv_.nb), std::max(tl_.nb, tlf_.nb));
+        r2_.alloc(n_);
+        zero(r2_.p);
+        partE_.alloc(npart);
+        partF_.alloc(npart);
+    }
+
+    // Four-wide two-launch CG iteration `it` (cg_vec.hpp); same buffers and
+    // partial-sum roles as the one-cell-per-thread kernels.
+    void enqueue_cg_x4(int it, double rel2, int solve_id, int step_index) {
+        if constexpr (sizeof(T) == 4) {
+            const int nb_in = it == 0 ? tl_.nb : tlv_.nb;
+            hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
+            if (timer.enabled) {
+                e0 = timer.get();
+                e1 = timer.get();
+                e2 = timer.get();
+                e3 = timer.get();
+                timer.recs.push_back({0, step_index, it, e0, e1});
+                timer.recs.push_back({1, step_index, it, e2, e3});
+            }
+            Dims g = g_;
+            VTiling vt = tlv_;
+            int itv = it, nbv = nb_in, sid = solve_id;
+            double rt = rel2;
+            const double *prr = partB_.p, *pbb = partC_.p, *pdq_in = partA_.p;
+            double *pdq = partA_.p, *prr_out = partB_.p;
+            CgState *st = cg_.p;
+            const T *r = r_.p, *dold = d_[(it + 1) & 1].p, *dcur = d_[it & 1].p;
+            T *dnew = d_[it & 1].p, *pp = p_.p, *rw = r_.p;
+            int *dm = done_dev_;
+            VZU_DISPATCH(vzu_, ({
+                const void *f1 = it == 0 ? (const void *)k_cg_dir4<VZU, true>
+                                         : (const void *)k_cg_dir4<VZU, false>;
+                void *a1[] = {&g, &vt, &itv, &rt, &nbv, &prr, &pbb, &st, &r, &dold, &dnew, &pdq,
+                              &dm, &sid};
+                HIPT(hipExtLaunchKernel(f1, dim3(vt.nb), dim3(kThreads), a1, 0, st_, e0, e1, 0));
+                void *a2[] = {&g, &vt, &itv, &pdq_in, &st, &dcur, &pp, &rw, &prr_out, &dm, &sid};
+                HIPT(hipExtLaunchKernel((const void *)k_cg_update4<VZU>, dim3(vt.nb),
+                                        dim3(kThreads), a2, 0, st_, e2, e3, 0));
+            }));
+        } else {
+            raise(AETHER_EINVAL, "the four-wide CG is built for FP32 only");
+        }
+    }
+
     void enqueue_cg_iteration(int it, double rel2, int solve_id, int step_index) {
+        if (form_ == AETHER_CG_FUSED) {
+            enqueue_cg_fused(it, rel2, solve_id, step_index);
+            return;
+        }
+        if (form_ == AETHER_CG_X4) {
+            enqueue_cg_x4(it, rel2, solve_id, step_index);
+            return;
+        }
         const double *pB = partB_.p, *pC = partC_.p, *pA = partA_.p;
         const T *dold = d_[(it + 1) & 1].p, *dcur = d_[it & 1].p, *rr = r_.p;
         if (timer.enabled) {
@@ -546,6 +741,13 @@ class Solver final : public SolverBase {
                 chunk_events_.push_back(e);
             }
         }
+        form_ = effective_form();
+        if (form_ == AETHER_CG_FUSED) ensure_fused_buffers();
+        if (form_ == AETHER_CG_FUSED && max_iters > 0) {  // delta_0 = r_0 . A r_0 partials
+            ZU_DISPATCH(zu_, (k_cg_rar<T, ZU><<<tl_.nb, cell_blk_, 0, st_>>>(g_, tl_, r_.p,
+                                                                             partE_.p)));
+            LAUNCH_CHECK();
+        }
         for (int c = 0; c < nchunks; ++c) {
             if (c >= 2 && cg_chunk_ > 0) {
                 HIPT(hipEventSynchronize(chunk_events_[c - 2]));
@@ -556,7 +758,15 @@ class Solver final : public SolverBase {
                 enqueue_cg_iteration(it, rel2, solve_id, step_index);
             HIPT(hipEventRecord(chunk_events_[c], st_));
         }
-        k_cg_finalize<<<1, cell_blk_, 0, st_>>>(max_iters, rel2, partB_.p, partC_.p, tl_.nb, cg_.p,
+        const double *part_rr = partB_.p;
+        int nb_rr = tl_.nb;
+        if (form_ == AETHER_CG_FUSED && max_iters > 0) {  // gamma of the last launch
+            part_rr = (max_iters & 1) ? partA_.p : partB_.p;
+            nb_rr = tlf_.nb;
+        } else if (form_ == AETHER_CG_X4 && max_iters > 0) {
+            nb_rr = tlv_.nb;
+        }
+        k_cg_finalize<<<1, cell_blk_, 0, st_>>>(max_iters, rel2, part_rr, partC_.p, nb_rr, cg_.p,
                                                  step_index >= 0 ? iters_dev_.p + step_index
                                                                  : iters_log_.p);
         LAUNCH_CHECK();
@@ -814,7 +1024,7 @@ class Solver final : public SolverBase {
         if (timer.enabled) {
             // count only launches that did work: iterations < iters of that step
             for (const auto &r : timer.recs) {
-                if (r.which < 2 && r.it >= iters[r.step]) continue;
+                if (r.which != 2 && r.it >= iters[r.step]) continue;
                 float ms = 0;
                 HIPT(hipEventElapsedTime(&ms, r.a, r.b));
                 timer.sum_ms[r.which] += ms;
@@ -1189,6 +1399,15 @@ class Solver final : public SolverBase {
     int zchunk_, nb_;
     Tiling tl_;
     int zu_ = 4;
+    int env_form_ = 0;    // AETHER_CG_FORM override of AETHER_CG_AUTO (0: none)
+    int form_ = AETHER_CG_SCALAR;  // layout of the solve being enqueued
+    VTiling tlv_ = {0, 0, 0, 0, 0, 1, 1};  // four-wide CG tiling (nx % 4 == 0)
+    int vzu_ = 4;
+    Tiling tlf_;          // one-launch CG: 64 x 4 x kz_ tiling
+    int kz_ = 16;
+    int pf_ = 4;          // planes of loads in flight ahead of use
+    DevBuf<T> r2_;                 // second residual buffer of the one-launch CG
+    DevBuf<double> partE_, partF_;  // its delta partials
     bool ir64_ = false;           // mixed-precision projection (AETHER_FP32_IR64)
     // rollout / adjoint state
     bool patch_on_ = false;
@@ -2022,8 +2241,9 @@ int aether_fluid_set_kernel_timing(aether_fluid_t s, int enable) {
 int aether_fluid_kernel_timing(aether_fluid_t s, int which, double *avg_ms, long *launches) {
     return guarded([&] {
         checked(s);
-        if (which < 0 || which > 2)
-            raise(AETHER_EINVAL, "which must be 0 (cg_dir), 1 (cg_update) or 2 (cg_resident)");
+        if (which < 0 || which > 3)
+            raise(AETHER_EINVAL,
+                  "which must be 0 (cg_dir), 1 (cg_update), 2 (cg_resident) or 3 (cg_fused)");
         const auto &t = s->impl->timer;
         if (avg_ms) *avg_ms = t.count[which] ? t.sum_ms[which] / t.count[which] : 0.0;
         if (launches) *launches = t.count[which];

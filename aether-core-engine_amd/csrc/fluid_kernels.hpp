@@ -41,6 +41,8 @@ struct CgState {
     int iters;
     int converged;
     int pad;
+    double cg_gamma[2];  // single-reduction CG: r.r and alpha of the current
+    double cg_alpha[2];  // iteration, double-buffered by iteration parity
 };
 
 __device__ __forceinline__ int tid_lin() { return threadIdx.y * kBX + threadIdx.x; }
@@ -104,13 +106,14 @@ __device__ __forceinline__ double sum_partials(const double *part, int nb, doubl
 }
 
 // Partial sums of the previous kernel, loaded early (before the data loads of
-// the caller) so that both memory round trips overlap.  Up to 8 per thread
+// the caller) so that both memory round trips overlap.  Up to NQ per thread
 // are held in registers; any excess is summed in a loop afterwards.
-struct EarlyPartials {
-    double v[8];
+template <int NQ>
+struct EarlyPartialsT {
+    double v[NQ];
     __device__ __forceinline__ void load(const double *__restrict__ part, int nb) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
+        for (int q = 0; q < NQ; ++q) {
             const int a = tid_lin() + q * kThreads;
             const double x = part[min(a, nb - 1)];  // always a valid address
             v[q] = a < nb ? x : 0.0;
@@ -119,12 +122,22 @@ struct EarlyPartials {
     // fixed-order total, identical in every workgroup
     __device__ __forceinline__ double total(const double *__restrict__ part, int nb,
                                             double *lds) const {
-        double acc[1] = {((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]))};
-        for (int a = tid_lin() + 8 * kThreads; a < nb; a += kThreads) acc[0] += part[a];
+        double acc[1];
+        if constexpr (NQ == 8)
+            acc[0] = ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
+        else if constexpr (NQ == 4)
+            acc[0] = (v[0] + v[1]) + (v[2] + v[3]);
+        else {
+            acc[0] = 0.0;
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) acc[0] += v[q];
+        }
+        for (int a = tid_lin() + NQ * kThreads; a < nb; a += kThreads) acc[0] += part[a];
         block_sum<1>(acc, lds);
         return acc[0];
     }
 };
+using EarlyPartials = EarlyPartialsT<8>;
 
 __device__ __forceinline__ bool is_interior(const Dims &g, int i, int j, int k) {
     return i >= 1 && i <= g.nx - 2 && j >= 1 && j <= g.ny - 2 && k >= 1 && k <= g.nz - 2;

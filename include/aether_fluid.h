@@ -44,6 +44,13 @@ extern "C" {
 #define AETHER_SOLVER_CG 0
 #define AETHER_SOLVER_RBGS 1
 
+/* CG iteration layout (all follow src/fluid.rs:463-487; the FP64
+ * instantiation always uses AETHER_CG_SCALAR, the reference's evaluation order) */
+#define AETHER_CG_AUTO 0     /* FP32: AETHER_CG_X4 when nx % 4 == 0, else AETHER_CG_SCALAR */
+#define AETHER_CG_SCALAR 1   /* two launches per iteration, one cell per thread */
+#define AETHER_CG_X4 2       /* two launches per iteration, four x-cells per thread (FP32, nx % 4 == 0) */
+#define AETHER_CG_FUSED 3    /* one launch per iteration, single-reduction form (FP32) */
+
 /* source kinds */
 #define AETHER_SOURCE_JETS 0   /* src/fluid.rs:532-595 inject_cyberpunk_sources */
 #define AETHER_SOURCE_SPHERE 1 /* sphere (cell-index coords, d^2<r^2, interior cells) that sets rho and T */
@@ -76,7 +83,7 @@ typedef struct {
     double dissipation_density;     /* 0.996 (src/fluid.rs:607) */
     double dissipation_temperature; /* 0.985 (src/fluid.rs:608) */
     int32_t source_kind;            /* AETHER_SOURCE_JETS */
-    int32_t reserved0;
+    int32_t cg_form;                /* AETHER_CG_AUTO */
     double source_center[3];        /* sphere source centre, cell-index coordinates */
     double source_radius;           /* sphere radius in cells */
     double source_density;          /* value set inside the sphere */

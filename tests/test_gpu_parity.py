@@ -391,7 +391,7 @@ def test_mixed_precision_matches_fp64_projection_steps():
 def test_batched_equals_one_at_a_time_bitwise():
     """Four grids of different sizes in one batch give, bit for bit, the
     fields and CG iteration counts of four single-grid solvers run one at a
-    time with the same tiling (8 z cells per thread)."""
+    time with the same CG layout (one cell per thread, 8 z cells per thread)."""
     _announce("batched vs one-at-a-time")
     import os
     import aether_fluid as af
@@ -421,6 +421,7 @@ def test_batched_equals_one_at_a_time_bitwise():
                                                                  d["gravity_alpha"])
             s.set_sphere_source(d["source_center"], d["source_radius"])
             s.solver_iters, s.cg_rel_tol = 120, 1e-5
+            s.cg_form = af.CG_SCALAR
             for n in ("u", "v", "w", "density", "temperature"):
                 s.upload(n, states[q][n])
             single = s.run(0.02, times)
