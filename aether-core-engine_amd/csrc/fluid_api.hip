@@ -24,6 +24,7 @@
 #include "cg_resident.hpp"
 #include "cg_fused.hpp"
 #include "cg_vec.hpp"
+#include "cg_fused4.hpp"
 #include "batch_kernels.hpp"
 #include "adjoint_kernels.hpp"
 
@@ -294,9 +295,23 @@ class Solver final : public SolverBase {
                 if (z == 2 || z == 4) vzu_ = z;
             }
             tlv_.gz = (nz + vzu_ - 1) / vzu_;
+            // non-temporal CG stores are opt-in (AETHER_CG_NT=1): ~4% faster
+            // in an isolated 128^3 loop, ~3% slower inside the full step
+            if (const char *e = std::getenv("AETHER_CG_NT")) nt_ = std::atoi(e) != 0;
             tlv_.nb = tlv_.gx * tlv_.gy * tlv_.gz;
             tlv_.remap = (tlv_.nb % 8 == 0) ? 1 : 0;
             if (const char *e = std::getenv("AETHER_XCD_REMAP")) tlv_.remap &= std::atoi(e) != 0;
+        }
+        // one-launch four-wide CG: whole rows (nx <= 256), 4 planes per group
+        if (nx % 4 == 0 && nx <= 256) {
+            tl4_.txv = nx / 4;
+            tl4_.ty = kThreads / tl4_.txv;
+            tl4_.gx = 1;
+            tl4_.gy = (ny + tl4_.ty - 1) / tl4_.ty;
+            tl4_.gz = (nz + 3) / 4;
+            tl4_.nb = tl4_.gy * tl4_.gz;
+            tl4_.remap = (tl4_.nb % 8 == 0) ? 1 : 0;
+            if (const char *e = std::getenv("AETHER_XCD_REMAP")) tl4_.remap &= std::atoi(e) != 0;
         }
         // one-launch CG: 64 x 4 x KZ tiles, KZ = 16 unless overridden
         if (const char *e = std::getenv("AETHER_CG_KZ")) {
@@ -313,7 +328,8 @@ class Solver final : public SolverBase {
         tlf_.nb = gx * gy * tlf_.gz;
         tlf_.remap = (tlf_.nb % 8 == 0) ? 1 : 0;
         if (const char *e = std::getenv("AETHER_XCD_REMAP")) tlf_.remap &= std::atoi(e) != 0;
-        const int npart = std::max(std::max(nb_, tlv_.nb), std::max(tl_.nb, tlf_.nb));
+        const int npart = std::max(std::max(std::max(nb_, tlv_.nb), std::max(tl_.nb, tlf_.nb)),
+                                   tl4_.nb);
         if (const char *e = std::getenv("AETHER_CG_CHUNK")) cg_chunk_ = std::max(0, std::atoi(e));
         HIPT(hipHostMalloc((void **)&done_host_, 64, hipHostMallocMapped));
         *done_host_ = 0;
@@ -570,8 +586,10 @@ class Solver final : public SolverBase {
     // One-launch CG iteration `it` (cg_fused.hpp).  gamma partials alternate
     // between partB_ (even it: written by the initialisation) and partA_,
     // delta partials between partE_ and partF_, r between r_ and r2_.
+    int fused_nb() const { return tl4_.nb > 0 ? tl4_.nb : tlf_.nb; }
+
     void enqueue_cg_fused(int it, double rel2, int solve_id, int step_index) {
-        const int nb_in = it == 0 ? tl_.nb : tlf_.nb;
+        const int nb_in = it == 0 ? tl_.nb : fused_nb();
         const double *pg_in = (it & 1) ? partA_.p : partB_.p;
         const double *pd_in = (it & 1) ? partF_.p : partE_.p;
         double *pg_out = (it & 1) ? partB_.p : partA_.p;
@@ -590,6 +608,24 @@ class Solver final : public SolverBase {
                 e0 = timer.get();
                 e1 = timer.get();
                 timer.recs.push_back({3, step_index, it, e0, e1});
+            }
+            if (tl4_.nb > 0) {  // whole-row four-wide form (cg_fused4.hpp)
+                Dims g = g_;
+                VTiling vt = tl4_;
+                int itv = it, nbv = nb_in, sid = solve_id;
+                double rt = rel2;
+                void *args[] = {&g,   &vt,    &itv, &rt, &nbv, &pg_in, &pd_in, &pbb, &pg_out,
+                                &pd_out, &st, &r_in, &r_out, &d_in, &d_out, &pp, &dm, &sid};
+                const bool wide = std::max(nb_in, vt.nb) > 4 * kThreads;
+                const void *fn;
+                if (wide)
+                    fn = it == 0 ? (const void *)k_cg_fused4<4, true, 16>
+                                 : (const void *)k_cg_fused4<4, false, 16>;
+                else
+                    fn = it == 0 ? (const void *)k_cg_fused4<4, true, 4>
+                                 : (const void *)k_cg_fused4<4, false, 4>;
+                HIPT(hipExtLaunchKernel(fn, dim3(vt.nb), dim3(kThreads), args, 0, st_, e0, e1, 0));
+                return;
             }
             KZ_DISPATCH(kz_, ({
                 const void *fn;
@@ -635,7 +671,8 @@ class Solver final : public SolverBase {
 
     void ensure_fused_buffers() {
         if (r2_.p) return;
-        const size_t npart = (size_t)std::max(std::max(nb_, tlv_.nb), std::max(tl_.nb, tlf_.nb));
+        const size_t npart = (size_t)std::max(
+            std::max(std::max(nb_, tlv_.nb), std::max(tl_.nb, tlf_.nb)), tl4_.nb);
         r2_.alloc(n_);
         zero(r2_.p);
         partE_.alloc(npart);
@@ -666,15 +703,27 @@ class Solver final : public SolverBase {
             const T *r = r_.p, *dold = d_[(it + 1) & 1].p, *dcur = d_[it & 1].p;
             T *dnew = d_[it & 1].p, *pp = p_.p, *rw = r_.p;
             int *dm = done_dev_;
+            const bool wide = std::max(nb_in, vt.nb) > 4 * kThreads;  // > 1024 partials
             VZU_DISPATCH(vzu_, ({
-                const void *f1 = it == 0 ? (const void *)k_cg_dir4<VZU, true>
-                                         : (const void *)k_cg_dir4<VZU, false>;
+                const void *f1, *f2;
+                if (wide) {
+                    f1 = it == 0 ? (const void *)k_cg_dir4<VZU, true, 16, false>
+                                 : (const void *)k_cg_dir4<VZU, false, 16, false>;
+                    f2 = (const void *)k_cg_update4<VZU, 16, false>;
+                } else if (nt_) {
+                    f1 = it == 0 ? (const void *)k_cg_dir4<VZU, true, 4, true>
+                                 : (const void *)k_cg_dir4<VZU, false, 4, true>;
+                    f2 = (const void *)k_cg_update4<VZU, 4, true>;
+                } else {
+                    f1 = it == 0 ? (const void *)k_cg_dir4<VZU, true, 4, false>
+                                 : (const void *)k_cg_dir4<VZU, false, 4, false>;
+                    f2 = (const void *)k_cg_update4<VZU, 4, false>;
+                }
                 void *a1[] = {&g, &vt, &itv, &rt, &nbv, &prr, &pbb, &st, &r, &dold, &dnew, &pdq,
                               &dm, &sid};
                 HIPT(hipExtLaunchKernel(f1, dim3(vt.nb), dim3(kThreads), a1, 0, st_, e0, e1, 0));
                 void *a2[] = {&g, &vt, &itv, &pdq_in, &st, &dcur, &pp, &rw, &prr_out, &dm, &sid};
-                HIPT(hipExtLaunchKernel((const void *)k_cg_update4<VZU>, dim3(vt.nb),
-                                        dim3(kThreads), a2, 0, st_, e2, e3, 0));
+                HIPT(hipExtLaunchKernel(f2, dim3(vt.nb), dim3(kThreads), a2, 0, st_, e2, e3, 0));
             }));
         } else {
             raise(AETHER_EINVAL, "the four-wide CG is built for FP32 only");
@@ -762,7 +811,7 @@ class Solver final : public SolverBase {
         int nb_rr = tl_.nb;
         if (form_ == AETHER_CG_FUSED && max_iters > 0) {  // gamma of the last launch
             part_rr = (max_iters & 1) ? partA_.p : partB_.p;
-            nb_rr = tlf_.nb;
+            nb_rr = fused_nb();
         } else if (form_ == AETHER_CG_X4 && max_iters > 0) {
             nb_rr = tlv_.nb;
         }
@@ -1403,7 +1452,9 @@ class Solver final : public SolverBase {
     int form_ = AETHER_CG_SCALAR;  // layout of the solve being enqueued
     VTiling tlv_ = {0, 0, 0, 0, 0, 1, 1};  // four-wide CG tiling (nx % 4 == 0)
     int vzu_ = 4;
+    bool nt_ = false;  // non-temporal stores in the four-wide CG
     Tiling tlf_;          // one-launch CG: 64 x 4 x kz_ tiling
+    VTiling tl4_ = {0, 0, 0, 0, 0, 1, 1};  // one-launch four-wide: whole rows (nb 0: unusable)
     int kz_ = 16;
     int pf_ = 4;          // planes of loads in flight ahead of use
     DevBuf<T> r2_;                 // second residual buffer of the one-launch CG
