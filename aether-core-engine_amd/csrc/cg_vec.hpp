@@ -1,4 +1,5 @@
-// cg_vec.hpp -- the two-launch CG iteration with four x-cells per thread.
+// cg_vec.hpp -- the two-launch CG iteration (and the red-black SOR pass) with
+// four x-cells per thread.
 //
 // Same arithmetic as body_cg_dir / body_cg_update (src/fluid.rs:463-487),
 // laid out for the memory pipeline: every thread owns a 4 (x) x ZU (z)
@@ -297,6 +298,67 @@ __global__ __launch_bounds__(kThreads) void k_cg_update4(
     }
     block_sum<1>(acc, lds);
     if (threadIdx.x == 0) part_rr[slot] = acc[0];
+}
+
+// Red-black SOR pass of one colour (src/fluid.rs:400-437) on the four-wide
+// layout: the same per-cell arithmetic as k_rbgs.  A cell's neighbours all
+// have the other colour, which this pass leaves unchanged, so every thread
+// reads consistent values and writes back its whole float4 (the other-colour
+// cells with their unchanged bits).
+template <int ZU>
+__global__ __launch_bounds__(kThreads) void k_rbgs4(Dims g, VTiling vt, int color, float omega,
+                                                    const float *__restrict__ b,
+                                                    float *__restrict__ p) {
+    int bx, by, bz;
+    vtile_coords(vt, bx, by, bz);
+    const VCol c = vcol_at<ZU>(g, vt, bx, by, bz);
+    float4 *p4 = reinterpret_cast<float4 *>(p);
+    const float4 *b4 = reinterpret_cast<const float4 *>(b);
+    float4 pz[ZU + 2], py[ZU][2], bc[ZU];
+    float xl[ZU], xr[ZU];
+#pragma unroll
+    for (int a = 0; a < ZU + 2; ++a) {
+        const int base = min(max(c.k0 - 1 + a, 0), g.nz - 1) * g.sxy;
+        pz[a] = p4[(base + c.off) >> 2];
+    }
+#pragma unroll
+    for (int a = 0; a < ZU; ++a) {
+        const int base = min(c.k0 + a, g.nz - 1) * g.sxy;
+        py[a][0] = p4[(base + c.offym) >> 2];
+        py[a][1] = p4[(base + c.offyp) >> 2];
+        bc[a] = b4[(base + c.off) >> 2];
+        xl[a] = xr[a] = 0.0f;
+        if (c.needL) xl[a] = p[base + c.off - 1];
+        if (c.needR) xr[a] = p[base + c.off + 4];
+    }
+    const int srcL = (c.lane + 63) & 63, srcR = (c.lane + 1) & 63;
+#pragma unroll
+    for (int a = 0; a < ZU; ++a) {
+        const int k = c.k0 + a;
+        const float sl = lane_value(pz[a + 1].w, srcL), sr = lane_value(pz[a + 1].x, srcR);
+        const float left = c.needL ? xl[a] : sl, right = c.needR ? xr[a] : sr;
+        if (c.valid && k < g.nz) {
+            const float4 ctr = pz[a + 1];
+            float o[4] = {ctr.x, ctr.y, ctr.z, ctr.w};
+            bool any = false;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int i = c.i0 + e;
+                if (!is_interior(g, i, c.j, k) || ((i + c.j + k) & 1) != color) continue;
+                any = true;
+                float diag = 0.0f, nb = 0.0f;
+                if (i > 1) { diag += 1.0f; nb += e == 0 ? left : f4(ctr, e - 1); }
+                if (i + 2 < g.nx) { diag += 1.0f; nb += e == 3 ? right : f4(ctr, e + 1); }
+                if (c.j > 1) { diag += 1.0f; nb += f4(py[a][0], e); }
+                if (c.j + 2 < g.ny) { diag += 1.0f; nb += f4(py[a][1], e); }
+                if (k > 1) { diag += 1.0f; nb += f4(pz[a], e); }
+                if (k + 2 < g.nz) { diag += 1.0f; nb += f4(pz[a + 2], e); }
+                const float gs = (f4(bc[a], e) + nb) / diag;
+                o[e] = (1.0f - omega) * o[e] + omega * gs;
+            }
+            if (any) p4[(k * g.sxy + c.off) >> 2] = make_float4(o[0], o[1], o[2], o[3]);
+        }
+    }
 }
 
 }  // namespace aether

@@ -823,6 +823,22 @@ class Solver final : public SolverBase {
     }
 
     void enqueue_rbgs(int sweeps) {
+        bool four = false;
+        if constexpr (sizeof(T) == 4) four = tlv_.nb > 0 && params.cg_form != AETHER_CG_SCALAR;
+        if (four) {
+            if constexpr (sizeof(T) == 4) {
+                for (int it = 0; it < sweeps; ++it)
+                    for (int color = 0; color < 2; ++color) {
+                        VZU_DISPATCH(vzu_, (k_rbgs4<VZU><<<tlv_.nb, kThreads, 0, st_>>>(
+                                               g_, tlv_, color, (float)params.sor_omega, b_.p,
+                                               p_.p)));
+                        LAUNCH_CHECK();
+                    }
+            }
+            last_kind_ = AETHER_SOLVER_RBGS;
+            last_sweeps_ = sweeps;
+            return;
+        }
         for (int it = 0; it < sweeps; ++it)
             for (int color = 0; color < 2; ++color) {
                 k_rbgs<T><<<cell_grid_, cell_blk_, 0, st_>>>(g_, color, (T)params.sor_omega,
@@ -2261,6 +2277,10 @@ int aether_fluid_stream_copy(size_t nbytes, int reps, double *avg_ms) {
         int cus = 256;
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0);
         const unsigned blocks = (unsigned)std::min<long>((n4 + kThreads - 1) / kThreads, 8L * cus);
+        k_fill_hash<<<blocks, kThreads, 0, st>>>(reinterpret_cast<float *>(a.p), 4 * n4);
+        LAUNCH_CHECK();
+        k_fill_hash<<<blocks, kThreads, 0, st>>>(reinterpret_cast<float *>(b.p), 4 * n4);
+        LAUNCH_CHECK();
         k_stream_copy<<<blocks, kThreads, 0, st>>>(a.p, n4, b.p);  // warm-up
         LAUNCH_CHECK();
         hipEvent_t e0, e1;

@@ -234,10 +234,13 @@ int aether_fluid_rollout(aether_fluid_t s, const aether_rollout_desc *d, const d
 /* Measurement hooks used by the benchmark driver. */
 /* Average device time (ms) per launch of the CG iteration kernels during the
  * last aether_fluid_run with timing enabled (HIP events on the solver's
- * stream); *launches counts the kernels timed. */
+ * stream); *launches counts the kernels timed.  which: 0 direction launch
+ * (cg_dir), 1 update launch (cg_update), 2 register-resident solve,
+ * 3 one-launch iteration (cg_fused). */
 int aether_fluid_set_kernel_timing(aether_fluid_t s, int enable);
-/* Plain device-to-device copy of nbytes (multiple of 16), `reps` launches
- * timed with HIP events: the measured-bandwidth reference point. */
+/* Plain device-to-device float4 copy of nbytes (multiple of 16) of
+ * full-entropy data, `reps` launches timed with HIP events: the
+ * measured-bandwidth reference point. */
 int aether_fluid_stream_copy(size_t nbytes, int reps, double *avg_ms);
 int aether_fluid_kernel_timing(aether_fluid_t s, int which, double *avg_ms, long *launches);
 
