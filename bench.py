@@ -175,18 +175,19 @@ def bench_mixed(steps: int, warmup: int) -> dict:
 
 
 def bench_rollout(n: int, steps: int, ckpt: int, ndirs: int = 0, reps: int = 2,
-                  init_state=None) -> dict:
+                  init_state=None, cg_tol: float = 1e-5, cg_iters: int = 200) -> dict:
     """Differentiable rollout at n^3: FP32 forward, FP32 adjoint with FP64
     reductions, checkpoint every `ckpt` steps; inflow-patch controls and
     buoyancy beta; loss = sum (rho_T - target)^2.  Times the forward-only
     rollout against forward + adjoint, and (ndirs > 0) checks the gradient
-    against FP64 central differences (eps = 1e-3) along seeded directions."""
+    against the FP64 adjoint and FP64 central differences along seeded
+    directions.  cg_tol / cg_iters: forward and adjoint projection solves."""
     import aether_fluid as af
     from aether_fluid.scenarios import rollout_problem
     pb = rollout_problem(n)
     s = af.FluidSolver3D(n, n, n, precision="fp32")
     s.gravity_alpha, s.vorticity_eps = 0.05, 0.3
-    s.solver_kind, s.solver_iters, s.cg_rel_tol = 0, 200, 1e-5
+    s.solver_kind, s.solver_iters, s.cg_rel_tol = 0, cg_iters, cg_tol
     if init_state is not None:
         for k, a in init_state.items():
             s.upload(k, a)
@@ -204,6 +205,7 @@ def bench_rollout(n: int, steps: int, ckpt: int, ndirs: int = 0, reps: int = 2,
     tf, tfa = min(t_f), min(t_fa)
     nvox = n ** 3
     out = {"workload": f"differentiable rollout {n}^3, {steps} steps, checkpoint every {ckpt}",
+           "cg_rel_tol": cg_tol, "cg_max_iters": cg_iters,
            "fwd_voxel_steps_per_s": nvox * steps / tf,
            "fwd_adjoint_voxel_steps_per_s": nvox * steps / tfa,
            "fwd_adjoint_over_fwd": tfa / tf, "loss": loss, "grad_beta": gb}
@@ -220,6 +222,9 @@ def bench_rollout(n: int, steps: int, ckpt: int, ndirs: int = 0, reps: int = 2,
             for k, a in init_state.items():
                 s64.upload(k, a)
         _, gc64, gb64 = s64.rollout(controls=pb["controls"], beta=1.0, grad=True, **args)
+        # whole-gradient error of the FP32 adjoint against the FP64 adjoint
+        out["fp32_grad_rel_l2_vs_fp64"] = float(np.linalg.norm(gc - gc64) / np.linalg.norm(gc64))
+        out["fp32_grad_beta_rel_vs_fp64"] = float(abs(gb - gb64) / abs(gb64))
         rng = np.random.default_rng(0xAE7E0003)
         errs = {"fp64_adjoint": [], "fd_1e-7": [], "fd_1e-3": []}
         fd64 = {"fd_1e-7": [], "fd_1e-3": []}
@@ -361,7 +366,7 @@ def main():
     }
     if not args.no_adjoint:
         from aether_fluid.scenarios import seeded_config_state
-        ro = bench_rollout(128, 8, 8, ndirs=0, reps=1,
+        ro = bench_rollout(128, 8, 8, ndirs=0, reps=2,
                            init_state=seeded_config_state(WORKLOAD))
         line["fwd_adjoint"] = {"value": ro["fwd_adjoint_voxel_steps_per_s"],
                                "unit": "voxel-steps/s",
