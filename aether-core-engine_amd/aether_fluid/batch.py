@@ -14,10 +14,13 @@ class FluidBatch:
     """instances: list of dicts with keys nx, ny, nz and optionally h,
     vorticity_eps, buoyancy_beta, gravity_alpha, source_center,
     source_radius, source_density, source_temperature.  Shared knobs
-    (solver_iters, cg_rel_tol, dissipation) come from keyword arguments."""
+    (solver_iters, cg_rel_tol, dissipation, cg_form) come from keyword
+    arguments; cg_form CG_AUTO runs the four-wide CG when every nx % 4 == 0
+    (FP32)."""
 
     def __init__(self, instances, precision: str = "fp32", solver_iters: int = 200,
-                 cg_rel_tol: float = 1e-5, dissipation=(0.996, 0.996, 0.985)):
+                 cg_rel_tol: float = 1e-5, dissipation=(0.996, 0.996, 0.985),
+                 cg_form: int = L.CG_AUTO):
         self._lib = L.lib()
         n = len(instances)
         descs = (L.InstanceDesc * n)()
@@ -35,6 +38,7 @@ class FluidBatch:
         p = L.Params()
         L.check(self._lib.aether_fluid_default_params(C.byref(p)))
         p.solver_kind, p.solver_iters, p.cg_rel_tol = 0, solver_iters, cg_rel_tol
+        p.cg_form = cg_form
         (p.dissipation_velocity, p.dissipation_density,
          p.dissipation_temperature) = dissipation
         prec = {"fp32": L.FP32, "fp64": L.FP64}[precision]

@@ -9,7 +9,7 @@
 // partial sums and stop flag: a converged instance's tiles return at once
 // while the others keep iterating.
 #pragma once
-#include "fluid_kernels.hpp"
+#include "cg_vec.hpp"
 
 namespace aether {
 
@@ -35,6 +35,9 @@ struct BInst {
     int slo[3], sn[3];                    // sphere bounding box, interior-clamped
     double *tel;                          // telemetry ring, 8 doubles per step
     int *iters;                           // CG iterations per step
+    VTiling vt;                           // four-wide CG tiling (FP32, nx % 4 == 0)
+    int nbv;                              // its tile count (= partial sums)
+    int pad2;
 };
 
 struct Slots {
@@ -176,18 +179,48 @@ __global__ __launch_bounds__(kThreads) void kb_cg_update(const BInst<T> *__restr
                           done_mirror + it.inst, solve_id);
 }
 
+// Four-wide CG launches over every instance's tiles (FP32, all nx % 4 == 0):
+// the items list each instance's four-wide tiles in the order of the
+// single-grid launch, so `local` is the same partial-sum slot and a batch
+// computes bit for bit what the instances compute one at a time.
+template <bool FIRST, int NQ>
+__global__ __launch_bounds__(kThreads) void kb_cg_dir4(const BInst<float> *__restrict__ I,
+                                                       const BItem *__restrict__ items, int iter,
+                                                       double rel_tol2, int *done_mirror,
+                                                       int solve_id) {
+    const BItem it = items[blockIdx.x];
+    const BInst<float> &in = I[it.inst];
+    body_cg_dir4<4, FIRST, NQ, false>(vcol_at<4>(in.g, in.vt, it.bx, it.by, it.bz), it.local,
+                                      in.g, iter, rel_tol2, FIRST ? in.nb : in.nbv, in.pb, in.pc,
+                                      in.st, in.r, in.d[(iter + 1) & 1], in.d[iter & 1], in.pa,
+                                      done_mirror + it.inst, solve_id);
+}
+
+template <int NQ>
+__global__ __launch_bounds__(kThreads) void kb_cg_update4(const BInst<float> *__restrict__ I,
+                                                          const BItem *__restrict__ items,
+                                                          int iter, int *done_mirror,
+                                                          int solve_id) {
+    const BItem it = items[blockIdx.x];
+    const BInst<float> &in = I[it.inst];
+    body_cg_update4<4, NQ, false>(vcol_at<4>(in.g, in.vt, it.bx, it.by, it.bz), it.local, in.nbv,
+                                  in.g, iter, in.pa, in.st, in.d[iter & 1], in.p, in.r, in.pb,
+                                  done_mirror + it.inst, solve_id);
+}
+
 // One workgroup per instance: closes solves that ran out of iterations and
-// logs the iteration count of step `step`.
+// logs the iteration count of step `step`.  x4: the solve ran the four-wide
+// launches (their r.r partials are nbv wide).
 template <typename T>
 __global__ __launch_bounds__(kThreads) void kb_cg_finalize(const BInst<T> *__restrict__ I,
                                                            int max_iters, double rel_tol2,
-                                                           int step) {
+                                                           int step, int x4) {
     const BInst<T> &in = I[blockIdx.x];
     __shared__ double lds[8];
     __shared__ int flag_lds[1];
     CgState *st = in.st;
     if (!block_flag(&st->done, flag_lds)) {
-        const double rr = sum_partials(in.pb, in.nb, lds);
+        const double rr = sum_partials(in.pb, (x4 && max_iters > 0) ? in.nbv : in.nb, lds);
         double bb = 0.0;
         if (max_iters == 0) bb = sum_partials(in.pc, in.nb, lds);
         if (tid_lin() == 0) {

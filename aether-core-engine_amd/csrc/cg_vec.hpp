@@ -108,18 +108,16 @@ __device__ __forceinline__ void lap4(const Dims &g, const VCol &c, int k, const 
 // CG launch 1 of iteration `it` (as body_cg_dir): stop rules on r.r, then
 // d = r + beta d_old, q = A d, partial d.q; writes d.  NQ: partial sums per
 // thread held in registers (256 NQ >= nb keeps their reload one round trip).
+// `slot` is the workgroup's partial-sum slot (0 leads), nb_in the number of
+// partials the previous launch wrote.
 template <int ZU, bool FIRST, int NQ, bool NT>
-__global__ __launch_bounds__(kThreads) void k_cg_dir4(
-    Dims g, VTiling vt, int it, double rel_tol2, int nb_in, const double *__restrict__ part_rr,
-    const double *__restrict__ part_bb, CgState *st, const float *__restrict__ r,
-    const float *__restrict__ dold, float *__restrict__ dnew, double *__restrict__ part_dq,
-    int *done_mirror, int solve_id) {
+__device__ __forceinline__ void body_cg_dir4(
+    const VCol &c, int slot, Dims g, int it, double rel_tol2, int nb_in,
+    const double *__restrict__ part_rr, const double *__restrict__ part_bb, CgState *st,
+    const float *__restrict__ r, const float *__restrict__ dold, float *__restrict__ dnew,
+    double *__restrict__ part_dq, int *done_mirror, int solve_id) {
     __shared__ double lds[8];
     __shared__ int flag_lds[1];
-    int bx, by, bz;
-    vtile_coords(vt, bx, by, bz);
-    const VCol c = vcol_at<ZU>(g, vt, bx, by, bz);
-    const int slot = blockIdx.x;
     const int flag_reg = st->done;
     EarlyPartialsT<NQ> prr, pbb;
     prr.load(part_rr, nb_in);
@@ -219,20 +217,28 @@ __global__ __launch_bounds__(kThreads) void k_cg_dir4(
     if (threadIdx.x == 0) part_dq[slot] = acc[0];
 }
 
+template <int ZU, bool FIRST, int NQ, bool NT>
+__global__ __launch_bounds__(kThreads) void k_cg_dir4(
+    Dims g, VTiling vt, int it, double rel_tol2, int nb_in, const double *__restrict__ part_rr,
+    const double *__restrict__ part_bb, CgState *st, const float *__restrict__ r,
+    const float *__restrict__ dold, float *__restrict__ dnew, double *__restrict__ part_dq,
+    int *done_mirror, int solve_id) {
+    int bx, by, bz;
+    vtile_coords(vt, bx, by, bz);
+    body_cg_dir4<ZU, FIRST, NQ, NT>(vcol_at<ZU>(g, vt, bx, by, bz), blockIdx.x, g, it, rel_tol2,
+                                    nb_in, part_rr, part_bb, st, r, dold, dnew, part_dq,
+                                    done_mirror, solve_id);
+}
+
 // CG launch 2 of iteration `it` (as body_cg_update): alpha from d.q,
 // p += alpha d, r -= alpha A d (A d recomputed), partial r.r.
 template <int ZU, int NQ, bool NT>
-__global__ __launch_bounds__(kThreads) void k_cg_update4(
-    Dims g, VTiling vt, int it, const double *__restrict__ part_dq, CgState *st,
-    const float *__restrict__ d, float *__restrict__ p, float *__restrict__ r,
+__device__ __forceinline__ void body_cg_update4(
+    const VCol &c, int slot, int nb, Dims g, int it, const double *__restrict__ part_dq,
+    CgState *st, const float *__restrict__ d, float *__restrict__ p, float *__restrict__ r,
     double *__restrict__ part_rr, int *done_mirror, int solve_id) {
     __shared__ double lds[8];
     __shared__ int flag_lds[1];
-    int bx, by, bz;
-    vtile_coords(vt, bx, by, bz);
-    const VCol c = vcol_at<ZU>(g, vt, bx, by, bz);
-    const int slot = blockIdx.x;
-    const int nb = vt.nb;
     const int flag_reg = st->done;
     EarlyPartialsT<NQ> pdq;
     pdq.load(part_dq, nb);
@@ -298,6 +304,17 @@ __global__ __launch_bounds__(kThreads) void k_cg_update4(
     }
     block_sum<1>(acc, lds);
     if (threadIdx.x == 0) part_rr[slot] = acc[0];
+}
+
+template <int ZU, int NQ, bool NT>
+__global__ __launch_bounds__(kThreads) void k_cg_update4(
+    Dims g, VTiling vt, int it, const double *__restrict__ part_dq, CgState *st,
+    const float *__restrict__ d, float *__restrict__ p, float *__restrict__ r,
+    double *__restrict__ part_rr, int *done_mirror, int solve_id) {
+    int bx, by, bz;
+    vtile_coords(vt, bx, by, bz);
+    body_cg_update4<ZU, NQ, NT>(vcol_at<ZU>(g, vt, bx, by, bz), blockIdx.x, vt.nb, g, it,
+                                part_dq, st, d, p, r, part_rr, done_mirror, solve_id);
 }
 
 // Red-black SOR pass of one colour (src/fluid.rs:400-437) on the four-wide

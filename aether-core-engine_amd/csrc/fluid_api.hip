@@ -1536,6 +1536,7 @@ class Batch final : public BatchBase {
         double h;
         size_t n;
         int gx, gy, gz, nb;
+        VTiling vt = {0, 0, 0, 0, 0, 1, 1};  // four-wide CG tiling (nx % 4 == 0)
         aether_fluid_instance_desc d;
         DevBuf<T> f[2][5], fwd0, fwd1, curl, p, b, r, dd[2];
         DevBuf<double> pa, pb, pc, ptel, tel, staging;
@@ -1547,8 +1548,11 @@ class Batch final : public BatchBase {
     Batch(const aether_fluid_instance_desc *desc, int count, const aether_fluid_params &shared,
           hipStream_t stream)
         : params_(shared), st_(stream) {
-        std::vector<BItem> cells, cg;
+        std::vector<BItem> cells, cg, cg4;
         std::vector<int2> src;
+        // four-wide CG: FP32 and every nx % 4 == 0
+        x4_ok_ = sizeof(T) == 4;
+        for (int q = 0; q < count; ++q) x4_ok_ = x4_ok_ && desc[q].nx % 4 == 0;
         for (int q = 0; q < count; ++q) {
             auto in = std::make_unique<Inst>();
             const aether_fluid_instance_desc &d = desc[q];
@@ -1563,14 +1567,26 @@ class Batch final : public BatchBase {
             in->gy = (d.ny + kBY - 1) / kBY;
             in->gz = (d.nz + kBatchZU - 1) / kBatchZU;
             in->nb = in->gx * in->gy * in->gz;
+            if (x4_ok_) {  // as the single-grid four-wide tiling (4 z cells per thread)
+                VTiling &v = in->vt;
+                v.txv = std::min(d.nx / 4, 64);
+                v.ty = kThreads / v.txv;
+                v.gx = (d.nx / 4 + v.txv - 1) / v.txv;
+                v.gy = (d.ny + v.ty - 1) / v.ty;
+                v.gz = (d.nz + 3) / 4;
+                v.nb = v.gx * v.gy * v.gz;
+                v.remap = (v.nb % 8 == 0) ? 1 : 0;
+                wide_ = wide_ || std::max(in->nb, v.nb) > 4 * kThreads;
+            }
+            const int npart = std::max(in->nb, in->vt.nb);
             for (int a = 0; a < 2; ++a)
                 for (int f = 0; f < 5; ++f) in->f[a][f].alloc(in->n);
             for (DevBuf<T> *b : {&in->fwd0, &in->fwd1, &in->curl, &in->p, &in->b, &in->r,
                                  &in->dd[0], &in->dd[1]})
                 b->alloc(in->n);
-            in->pa.alloc(in->nb);
-            in->pb.alloc(in->nb);
-            in->pc.alloc(in->nb);
+            in->pa.alloc(npart);
+            in->pb.alloc(npart);
+            in->pc.alloc(npart);
             in->ptel.alloc((size_t)in->nb * kTelParts);
             in->st.alloc(1);
             for (int a = 0; a < 2; ++a)
@@ -1590,6 +1606,14 @@ class Batch final : public BatchBase {
                 const int t = remap ? (b & 7) * (in->nb >> 3) + (b >> 3) : b;
                 const int bx = t % in->gx, rest = t / in->gx;
                 cg.push_back({q, bx, rest % in->gy, rest / in->gy, b, {0, 0, 0}});
+            }
+            if (x4_ok_) {
+                const VTiling &v = in->vt;
+                for (int b = 0; b < v.nb; ++b) {
+                    const int t = v.remap ? (b & 7) * (v.nb >> 3) + (b >> 3) : b;
+                    const int bx = t % v.gx, rest = t / v.gx;
+                    cg4.push_back({q, bx, rest % v.gy, rest / v.gy, b, {0, 0, 0}});
+                }
             }
             // sphere source box (interior cells)
             if (d.source_radius > 0.0) {
@@ -1627,6 +1651,12 @@ class Batch final : public BatchBase {
         nsrc_ = (int)src.size();
         cell_items_.alloc(ncell_);
         cg_items_.alloc(ncg_);
+        ncg4_ = (int)cg4.size();
+        if (ncg4_) {
+            cg4_items_.alloc(ncg4_);
+            HIPT(hipMemcpyAsync(cg4_items_.p, cg4.data(), cg4.size() * sizeof(BItem),
+                                hipMemcpyHostToDevice, st_));
+        }
         HIPT(hipMemcpyAsync(cell_items_.p, cells.data(), cells.size() * sizeof(BItem),
                             hipMemcpyHostToDevice, st_));
         HIPT(hipMemcpyAsync(cg_items_.p, cg.data(), cg.size() * sizeof(BItem),
@@ -1730,6 +1760,8 @@ class Batch final : public BatchBase {
             t.stemp = in.d.source_temperature;
             t.tel = in.tel.p;
             t.iters = in.iters.p;
+            t.vt = in.vt;
+            t.nbv = in.vt.nb;
         }
         HIPT(hipMemcpyAsync(table_.p, tab.data(), tab.size() * sizeof(BInst<T>),
                             hipMemcpyHostToDevice, st_));
@@ -1745,8 +1777,51 @@ class Batch final : public BatchBase {
         *slot[f] ^= 1;
     }
 
+    // The batch's CG layout for the current knobs: four-wide or one cell per thread.
+    bool use_x4() const {
+        switch (params_.cg_form) {
+        case AETHER_CG_SCALAR:
+            return false;
+        case AETHER_CG_X4:
+            if (!x4_ok_) raise(AETHER_EINVAL, "AETHER_CG_X4 needs FP32 and nx % 4 == 0 in every instance");
+            return true;
+        case AETHER_CG_FUSED:
+            raise(AETHER_EINVAL, "AETHER_CG_FUSED is not available for batches");
+        default:
+            return x4_ok_;
+        }
+    }
+
+    void launch_cg4(int it, double rel2, int solve_id) {
+        if constexpr (sizeof(T) == 4) {
+            if (wide_) {
+                if (it == 0)
+                    kb_cg_dir4<true, 16><<<ncg4_, kThreads, 0, st_>>>(table_.p, cg4_items_.p, it,
+                                                                      rel2, done_dev_, solve_id);
+                else
+                    kb_cg_dir4<false, 16><<<ncg4_, kThreads, 0, st_>>>(
+                        table_.p, cg4_items_.p, it, rel2, done_dev_, solve_id);
+                LAUNCH_CHECK();
+                kb_cg_update4<16><<<ncg4_, kThreads, 0, st_>>>(table_.p, cg4_items_.p, it,
+                                                               done_dev_, solve_id);
+            } else {
+                if (it == 0)
+                    kb_cg_dir4<true, 4><<<ncg4_, kThreads, 0, st_>>>(table_.p, cg4_items_.p, it,
+                                                                     rel2, done_dev_, solve_id);
+                else
+                    kb_cg_dir4<false, 4><<<ncg4_, kThreads, 0, st_>>>(table_.p, cg4_items_.p, it,
+                                                                      rel2, done_dev_, solve_id);
+                LAUNCH_CHECK();
+                kb_cg_update4<4><<<ncg4_, kThreads, 0, st_>>>(table_.p, cg4_items_.p, it,
+                                                              done_dev_, solve_id);
+            }
+            LAUNCH_CHECK();
+        }
+    }
+
     void cg_solve(int step) {
         const int max_iters = params_.solver_iters;
+        const bool x4 = use_x4();
         const double rel2 = params_.cg_rel_tol > 0 ? params_.cg_rel_tol * params_.cg_rel_tol : 0.0;
         const int solve_id = ++solve_seq_;
         const int chunk = 8;
@@ -1766,6 +1841,10 @@ class Batch final : public BatchBase {
                 if (all) break;
             }
             for (int it = c * chunk; it < std::min(max_iters, (c + 1) * chunk); ++it) {
+                if (x4) {
+                    launch_cg4(it, rel2, solve_id);
+                    continue;
+                }
                 if (it == 0)
                     kb_cg_dir<T, kBatchZU, true><<<ncg_, blk, 0, st_>>>(
                         table_.p, cg_items_.p, it, rel2, done_dev_, solve_id);
@@ -1779,7 +1858,7 @@ class Batch final : public BatchBase {
             }
             HIPT(hipEventRecord(chunk_events_[c], st_));
         }
-        kb_cg_finalize<T><<<count(), blk, 0, st_>>>(table_.p, max_iters, rel2, step);
+        kb_cg_finalize<T><<<count(), blk, 0, st_>>>(table_.p, max_iters, rel2, step, x4 ? 1 : 0);
         LAUNCH_CHECK();
     }
 
@@ -1860,9 +1939,11 @@ class Batch final : public BatchBase {
     std::vector<std::unique_ptr<Inst>> inst_;
     std::vector<int> box_lo_, box_n_;
     DevBuf<BInst<T>> table_;
-    DevBuf<BItem> cell_items_, cg_items_;
+    DevBuf<BItem> cell_items_, cg_items_, cg4_items_;
     DevBuf<int2> src_items_;
-    int ncell_ = 0, ncg_ = 0, nsrc_ = 0;
+    int ncell_ = 0, ncg_ = 0, nsrc_ = 0, ncg4_ = 0;
+    bool x4_ok_ = false;  // FP32 and every nx % 4 == 0: four-wide CG available
+    bool wide_ = false;   // some instance has more than 1024 CG partials
     Slots sl_ = {0, 0, 0, 0, 0};
     int *done_host_ = nullptr, *done_dev_ = nullptr;
     int solve_seq_ = 0;

@@ -388,21 +388,13 @@ def test_mixed_precision_matches_fp64_projection_steps():
         assert rel_l2(a.download(n), b.download(n)) <= 1e-5, n
 
 
-def test_batched_equals_one_at_a_time_bitwise():
-    """Four grids of different sizes in one batch give, bit for bit, the
-    fields and CG iteration counts of four single-grid solvers run one at a
-    time with the same CG layout (one cell per thread, 8 z cells per thread)."""
-    _announce("batched vs one-at-a-time")
+def _batched_vs_single(insts, single_form, case):
+    """A batch of grids of different sizes gives, bit for bit, the fields and
+    CG iteration counts of single-grid solvers run one at a time with the
+    same CG layout (and 8 z cells per thread in the initialisation tiling)."""
+    _announce(case)
     import os
     import aether_fluid as af
-    insts = [dict(nx=32, ny=32, nz=32, vorticity_eps=0.3, buoyancy_beta=1.0, gravity_alpha=0.05,
-                  source_center=(16.0, 4.0, 16.0), source_radius=3.5),
-             dict(nx=24, ny=40, nz=20, vorticity_eps=0.2, buoyancy_beta=1.0, gravity_alpha=0.08,
-                  source_center=(12.0, 5.0, 10.0), source_radius=3.0),
-             dict(nx=70, ny=20, nz=18, vorticity_eps=0.45, buoyancy_beta=1.0, gravity_alpha=0.03,
-                  source_center=(30.0, 3.0, 9.0), source_radius=2.5),
-             dict(nx=16, ny=16, nz=40, vorticity_eps=0.1, buoyancy_beta=1.0, gravity_alpha=0.1,
-                  source_center=(8.0, 3.0, 20.0), source_radius=2.0)]
     b = af.FluidBatch(insts, solver_iters=120, cg_rel_tol=1e-5)
     states = []
     for q, d in enumerate(insts):
@@ -421,7 +413,7 @@ def test_batched_equals_one_at_a_time_bitwise():
                                                                  d["gravity_alpha"])
             s.set_sphere_source(d["source_center"], d["source_radius"])
             s.solver_iters, s.cg_rel_tol = 120, 1e-5
-            s.cg_form = af.CG_SCALAR
+            s.cg_form = single_form
             for n in ("u", "v", "w", "density", "temperature"):
                 s.upload(n, states[q][n])
             single = s.run(0.02, times)
@@ -433,6 +425,36 @@ def test_batched_equals_one_at_a_time_bitwise():
                                            rtol=1e-12, atol=1e-300)
     finally:
         os.environ.pop("AETHER_CG_ZU", None)
+
+
+def test_batched_equals_one_at_a_time_bitwise():
+    """nx = 70 is not a multiple of 4, so the batch runs the one-cell-per-
+    thread CG; compared with single grids pinned to that layout."""
+    import aether_fluid as af
+    insts = [dict(nx=32, ny=32, nz=32, vorticity_eps=0.3, buoyancy_beta=1.0, gravity_alpha=0.05,
+                  source_center=(16.0, 4.0, 16.0), source_radius=3.5),
+             dict(nx=24, ny=40, nz=20, vorticity_eps=0.2, buoyancy_beta=1.0, gravity_alpha=0.08,
+                  source_center=(12.0, 5.0, 10.0), source_radius=3.0),
+             dict(nx=70, ny=20, nz=18, vorticity_eps=0.45, buoyancy_beta=1.0, gravity_alpha=0.03,
+                  source_center=(30.0, 3.0, 9.0), source_radius=2.5),
+             dict(nx=16, ny=16, nz=40, vorticity_eps=0.1, buoyancy_beta=1.0, gravity_alpha=0.1,
+                  source_center=(8.0, 3.0, 20.0), source_radius=2.0)]
+    _batched_vs_single(insts, af.CG_SCALAR, "batched vs one-at-a-time (scalar CG)")
+
+
+def test_batched_four_wide_equals_one_at_a_time_bitwise():
+    """Every nx % 4 == 0 (including row widths of 6 and 17 float4 lanes, whose
+    workgroups leave threads idle), so the batch runs the four-wide CG."""
+    import aether_fluid as af
+    insts = [dict(nx=32, ny=32, nz=32, vorticity_eps=0.3, buoyancy_beta=1.0, gravity_alpha=0.05,
+                  source_center=(16.0, 4.0, 16.0), source_radius=3.5),
+             dict(nx=24, ny=40, nz=20, vorticity_eps=0.2, buoyancy_beta=1.0, gravity_alpha=0.08,
+                  source_center=(12.0, 5.0, 10.0), source_radius=3.0),
+             dict(nx=68, ny=20, nz=18, vorticity_eps=0.45, buoyancy_beta=1.0, gravity_alpha=0.03,
+                  source_center=(30.0, 3.0, 9.0), source_radius=2.5),
+             dict(nx=16, ny=16, nz=40, vorticity_eps=0.1, buoyancy_beta=1.0, gravity_alpha=0.1,
+                  source_center=(8.0, 3.0, 20.0), source_radius=2.0)]
+    _batched_vs_single(insts, af.CG_X4, "batched vs one-at-a-time (four-wide CG)")
 
 
 def _rollout_setup(n, precision, tol=1e-12, iters=600):
