@@ -10,10 +10,13 @@ from .solver import (FluidSolver3D, FluidTelemetryFrame, PoissonSolverKind, rng_
 from .scenarios import seeded_config_state, SMOKE_CONFIGS, batched_instances
 from .batch import FluidBatch
 from .adjoint import SmokeRollout
+from .medium import FluidMedium
 
 __all__ = [
-    "AetherFluidError", "CG_AUTO", "CG_FUSED", "CG_SCALAR", "CG_X4", "FIELDS", "FP32", "FP32_IR64", "FP64", "SOLVER_CG", "SOLVER_RBGS",
-    "SOURCE_JETS", "SOURCE_NONE", "SOURCE_SPHERE", "LIB_PATH", "SIGNATURES", "build", "lib",
-    "FluidSolver3D", "FluidTelemetryFrame", "PoissonSolverKind", "rng_fill", "stream_copy_ms",
+    "AetherFluidError", "CG_AUTO", "CG_FUSED", "CG_SCALAR", "CG_X4", "FIELDS", "FP32",
+    "FP32_IR64", "FP64", "SOLVER_CG", "SOLVER_RBGS", "SOURCE_JETS", "SOURCE_NONE",
+    "SOURCE_SPHERE", "LIB_PATH", "SIGNATURES", "build", "lib", "FluidSolver3D",
+    "FluidTelemetryFrame", "PoissonSolverKind", "rng_fill", "stream_copy_ms",
     "seeded_config_state", "SMOKE_CONFIGS", "batched_instances", "FluidBatch", "SmokeRollout",
+    "FluidMedium",
 ]

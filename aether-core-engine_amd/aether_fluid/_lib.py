@@ -134,6 +134,11 @@ SIGNATURES = {
     "aether_fluid_upload": (C.c_int, [_vp, C.c_int, _pd, C.c_size_t]),
     "aether_fluid_download": (C.c_int, [_vp, C.c_int, _pd, C.c_size_t]),
     "aether_fluid_max_density": (C.c_int, [_vp, _pd]),
+    "aether_fluid_medium_create": (C.c_int, [_vp, _pd, _pd, C.c_double, C.c_double,
+                                             C.POINTER(_vp)]),
+    "aether_fluid_medium_destroy": (C.c_int, [_vp]),
+    "aether_fluid_medium_info": (C.c_int, [_vp, _pd, _pd, _pd]),
+    "aether_fluid_medium_sample": (C.c_int, [_vp, _pd, C.c_size_t, _pd]),
     "aether_fluid_batch_create": (C.c_int, [C.POINTER(InstanceDesc), C.c_int, C.c_int,
                                             C.POINTER(Params), C.POINTER(_vp)]),
     "aether_fluid_batch_destroy": (C.c_int, [_vp]),

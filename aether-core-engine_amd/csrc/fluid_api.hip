@@ -209,6 +209,56 @@ struct KernelTimer {
     }
 };
 
+// FluidMedium (src/fluid.rs:687-788): device snapshots of density and
+// temperature taken at from_solver, sampled by sample_density_and_temp.
+class MediumBase {
+   public:
+    virtual ~MediumBase() = default;
+    virtual void sample(const double *xyz, size_t npts, double *out) = 0;
+    double max_density = 0.0, majorant_sigma_t = 0.0, phase_g = 0.0;
+};
+
+template <typename T>
+class Medium final : public MediumBase {
+   public:
+    // Copies density and temperature (on `copy_stream`, which must be idle
+    // afterwards) into buffers the medium owns; samples on its own stream.
+    Medium(Dims g, const T *rho, const T *temp, const double *lo, const double *hi,
+           hipStream_t copy_stream)
+        : g_(g) {
+        const size_t n = (size_t)g.sxy * g.nz;
+        rho_.alloc(n);
+        temp_.alloc(n);
+        HIPT(hipMemcpyAsync(rho_.p, rho, n * sizeof(T), hipMemcpyDeviceToDevice, copy_stream));
+        HIPT(hipMemcpyAsync(temp_.p, temp, n * sizeof(T), hipMemcpyDeviceToDevice, copy_stream));
+        HIPT(hipStreamSynchronize(copy_stream));
+        for (int a = 0; a < 3; ++a) {
+            box_.lo[a] = lo[a];
+            box_.hi[a] = hi[a];
+        }
+        HIPT(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+    }
+    ~Medium() override {
+        if (st_) (void)hipStreamDestroy(st_);
+    }
+    void sample(const double *xyz, size_t npts, double *out) override {
+        if (npts == 0) return;
+        DevBuf<double> dxyz(npts * 3), dout(npts * 2);
+        HIPT(hipMemcpyAsync(dxyz.p, xyz, npts * 3 * sizeof(double), hipMemcpyHostToDevice, st_));
+        k_medium_sample<T><<<(unsigned)((npts + kThreads - 1) / kThreads), kThreads, 0, st_>>>(
+            g_, box_, rho_.p, temp_.p, dxyz.p, (long)npts, dout.p);
+        LAUNCH_CHECK();
+        HIPT(hipMemcpyAsync(out, dout.p, npts * 2 * sizeof(double), hipMemcpyDeviceToHost, st_));
+        HIPT(hipStreamSynchronize(st_));
+    }
+
+   private:
+    Dims g_;
+    Box3 box_;
+    DevBuf<T> rho_, temp_;
+    hipStream_t st_ = nullptr;
+};
+
 class SolverBase {
    public:
     virtual ~SolverBase() = default;
@@ -227,6 +277,7 @@ class SolverBase {
     virtual void download(int field, double *dst) = 0;
     virtual void sample(int field, const double *xyz, size_t npts, double *out) = 0;
     virtual double max_density() = 0;
+    virtual std::unique_ptr<MediumBase> make_medium(const double *lo, const double *hi) = 0;
     virtual void stats(aether_fluid_solve_stats *out) = 0;
     virtual const std::vector<int> &last_run_iters() const = 0;
     virtual double rollout(const aether_rollout_desc &d, const double *ctrl, double beta,
@@ -1172,6 +1223,12 @@ class Solver final : public SolverBase {
         LAUNCH_CHECK();
         HIPT(hipMemcpyAsync(out, dout.p, npts * sizeof(double), hipMemcpyDeviceToHost, st_));
         sync();
+    }
+
+    std::unique_ptr<MediumBase> make_medium(const double *lo, const double *hi) override {
+        auto m = std::make_unique<Medium<T>>(g_, cur_[3], cur_[4], lo, hi, st_);
+        m->max_density = max_density();
+        return m;
     }
 
     double max_density() override {
@@ -2220,6 +2277,57 @@ int aether_fluid_max_density(aether_fluid_t s, double *out) {
         checked(s);
         if (!out) raise(AETHER_EINVAL, "null output pointer");
         *out = s->impl->max_density();
+    });
+}
+
+struct aether_fluid_medium_s {
+    int device;
+    std::unique_ptr<MediumBase> impl;
+};
+
+int aether_fluid_medium_create(aether_fluid_t s, const double *bounds_min,
+                               const double *bounds_max, double sigma_t_scale, double phase_g,
+                               aether_fluid_medium_t *out) {
+    return guarded([&] {
+        if (!out) raise(AETHER_EINVAL, "null output handle pointer");
+        *out = nullptr;
+        checked(s);
+        if (!bounds_min || !bounds_max) raise(AETHER_EINVAL, "null bounds");
+        auto m = std::make_unique<aether_fluid_medium_s>();
+        m->device = s->device;
+        m->impl = s->impl->make_medium(bounds_min, bounds_max);
+        // src/fluid.rs:709, 720
+        m->impl->majorant_sigma_t = std::max(m->impl->max_density * sigma_t_scale, 1e-4);
+        m->impl->phase_g = std::min(std::max(phase_g, -0.9), 0.9);
+        *out = m.release();
+    });
+}
+
+int aether_fluid_medium_destroy(aether_fluid_medium_t m) {
+    return guarded([&] {
+        if (!m) return;
+        (void)hipSetDevice(m->device);
+        delete m;
+    });
+}
+
+int aether_fluid_medium_info(aether_fluid_medium_t m, double *max_density,
+                             double *majorant_sigma_t, double *phase_g) {
+    return guarded([&] {
+        if (!m) raise(AETHER_EINVAL, "null medium handle");
+        if (max_density) *max_density = m->impl->max_density;
+        if (majorant_sigma_t) *majorant_sigma_t = m->impl->majorant_sigma_t;
+        if (phase_g) *phase_g = m->impl->phase_g;
+    });
+}
+
+int aether_fluid_medium_sample(aether_fluid_medium_t m, const double *xyz, size_t npts,
+                               double *out) {
+    return guarded([&] {
+        if (!m) raise(AETHER_EINVAL, "null medium handle");
+        if (npts && (!xyz || !out)) raise(AETHER_EINVAL, "null point or output array");
+        HIPT(hipSetDevice(m->device));
+        m->impl->sample(xyz, npts, out);
     });
 }
 

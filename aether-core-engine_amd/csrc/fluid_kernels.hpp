@@ -1350,4 +1350,65 @@ __global__ __launch_bounds__(kThreads) void k_sample_points(Dims g, const T *__r
                            c[1], c[2]);
 }
 
+// FluidMedium::sample_density_and_temp (src/fluid.rs:739-788) at npts points,
+// evaluated in FP64 from a snapshot of density and temperature; out holds
+// (density, temperature) pairs.  Aabb::extent = max(max - min, 0) per axis
+// (src/math.rs:832-834); Rust's clamp / max as written there.
+struct Box3 {
+    double lo[3], hi[3];
+};
+
+__device__ __forceinline__ double rclamp_d(double x, double lo, double hi) {
+    return x < lo ? lo : (x > hi ? hi : x);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_medium_sample(Dims g, Box3 box,
+                                                            const T *__restrict__ rho,
+                                                            const T *__restrict__ temp,
+                                                            const double *__restrict__ xyz,
+                                                            long npts, double *__restrict__ out) {
+    const long a = (long)blockIdx.x * kThreads + tid_lin();
+    if (a >= npts) return;
+    const double p[3] = {xyz[3 * a], xyz[3 * a + 1], xyz[3 * a + 2]};
+    if (p[0] < box.lo[0] || p[0] > box.hi[0] || p[1] < box.lo[1] || p[1] > box.hi[1] ||
+        p[2] < box.lo[2] || p[2] > box.hi[2]) {
+        out[2 * a] = 0.0;
+        out[2 * a + 1] = 0.0;
+        return;
+    }
+    double u[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const double ext = fmax(box.hi[d] - box.lo[d], 0.0);
+        u[d] = (p[d] - box.lo[d]) / fmax(ext, 1e-9);
+    }
+    // edge feathering (src/fluid.rs:754-758)
+    const double fx = rclamp_d(4.0 * u[0] * (1.0 - u[0]), 0.0, 1.0);
+    const double fz = rclamp_d(4.0 * u[2] * (1.0 - u[2]), 0.0, 1.0);
+    const double fy = rclamp_d(1.0 - fmax(u[1] - 0.95, 0.0) / 0.05, 0.0, 1.0);
+    const double feather = sqrt(fx * fz * fy);
+    const int n[3] = {g.nx, g.ny, g.nz};
+    int i0[3];
+    double c[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const double x = rclamp_d(u[d] * (double)(n[d] - 1), 0.5, (double)n[d] - 1.500001);
+        const double fl = floor(x);
+        i0[d] = min(max((int)fl, 0), n[d] - 2);  // NaN-safe index
+        c[d] = x - fl;
+    }
+    const int b = (i0[2] * g.ny + i0[1]) * g.nx + i0[0];
+    const double dv =
+        lerp3<double>(rho[b], rho[b + 1], rho[b + g.nx], rho[b + g.nx + 1], rho[b + g.sxy],
+                      rho[b + g.sxy + 1], rho[b + g.sxy + g.nx], rho[b + g.sxy + g.nx + 1],
+                      c[0], c[1], c[2]);
+    const double tv =
+        lerp3<double>(temp[b], temp[b + 1], temp[b + g.nx], temp[b + g.nx + 1], temp[b + g.sxy],
+                      temp[b + g.sxy + 1], temp[b + g.sxy + g.nx], temp[b + g.sxy + g.nx + 1],
+                      c[0], c[1], c[2]);
+    out[2 * a] = dv * feather;
+    out[2 * a + 1] = tv * feather;
+}
+
 }  // namespace aether

@@ -173,6 +173,26 @@ int aether_fluid_download(aether_fluid_t s, int field, double *dst, size_t n);
  * (reduced on the device) and the f64 density/temperature snapshots. */
 int aether_fluid_max_density(aether_fluid_t s, double *out);
 
+/* src/fluid.rs:687-788 FluidMedium, the renderer-side view of the solver.
+ * aether_fluid_medium_create is FluidMedium::from_solver (fluid.rs:703-735):
+ * it snapshots density and temperature on the device (later solver steps do
+ * not change the medium), reduces max_density there, and sets
+ * majorant_sigma_t = max(max_density * sigma_t_scale, 1e-4) and
+ * phase_g = clamp(phase_g, -0.9, 0.9).  The spectral curves are renderer
+ * constants and stay with the renderer. */
+typedef struct aether_fluid_medium_s *aether_fluid_medium_t;
+int aether_fluid_medium_create(aether_fluid_t s, const double *bounds_min,
+                               const double *bounds_max, double sigma_t_scale, double phase_g,
+                               aether_fluid_medium_t *out);
+int aether_fluid_medium_destroy(aether_fluid_medium_t m);
+int aether_fluid_medium_info(aether_fluid_medium_t m, double *max_density,
+                             double *majorant_sigma_t, double *phase_g);
+/* fluid.rs:739-788 sample_density_and_temp at npts points (x,y,z
+ * interleaved), evaluated in FP64 on the device: out receives npts
+ * (density, temperature) pairs, (0, 0) outside the bounds. */
+int aether_fluid_medium_sample(aether_fluid_medium_t m, const double *xyz, size_t npts,
+                               double *out);
+
 /* Seeded inputs: the reference's SplitMix64-seeded Xoshiro256++ stream
  * (src/rng.rs:21-33, 48-64, 74-88).  Fills out[0..n) with range_f64(lo,hi)
  * draws continuing from *state (4 words), which is advanced. */

@@ -206,6 +206,44 @@ double of_sample_trilinear(const OFluid *s, int field, double x, double y, doubl
     return sample_trilinear(s, of_field((OFluid *)s, field), x, y, z);
 }
 
+/* FluidMedium (fluid.rs:687-788), the renderer-side consumer of the
+ * solver's density and temperature. */
+
+/* fluid.rs:704-708: max over the density array, starting from 0 */
+double of_max_density(const OFluid *s) {
+    double m = 0.0;
+    for (size_t a = 0; a < s->n; ++a) m = fmax(m, s->density[a]);
+    return m;
+}
+
+/* fluid.rs:739-788 sample_density_and_temp, bounds = [bmin, bmax]
+ * (Aabb::extent = max(max - min, 0) per axis, math.rs:832-834) */
+void of_medium_sample(const OFluid *s, const double *bmin, const double *bmax, double px,
+                      double py, double pz, double *d_out, double *t_out) {
+    if (px < bmin[0] || px > bmax[0] || py < bmin[1] || py > bmax[1] || pz < bmin[2] ||
+        pz > bmax[2]) {
+        *d_out = 0.0;
+        *t_out = 0.0;
+        return;
+    }
+    double ex = fmax(bmax[0] - bmin[0], 0.0), ey = fmax(bmax[1] - bmin[1], 0.0),
+           ez = fmax(bmax[2] - bmin[2], 0.0);
+    double ux = (px - bmin[0]) / fmax(ex, 1e-9);
+    double uy = (py - bmin[1]) / fmax(ey, 1e-9);
+    double uz = (pz - bmin[2]) / fmax(ez, 1e-9);
+    double fx = rclamp(4.0 * ux * (1.0 - ux), 0.0, 1.0);
+    double fz = rclamp(4.0 * uz * (1.0 - uz), 0.0, 1.0);
+    double fy = rclamp(1.0 - fmax(uy - 0.95, 0.0) / 0.05, 0.0, 1.0);
+    double feather = sqrt(fx * fz * fy);
+    double gx = rclamp(ux * (double)(s->nx - 1), 0.5, (double)s->nx - 1.500001);
+    double gy = rclamp(uy * (double)(s->ny - 1), 0.5, (double)s->ny - 1.500001);
+    double gz = rclamp(uz * (double)(s->nz - 1), 0.5, (double)s->nz - 1.500001);
+    /* the clamped position lies in the clamp box, where sample_trilinear's
+     * own clamp is the identity: the same interpolation, same association */
+    *d_out = sample_trilinear(s, s->density, gx, gy, gz) * feather;
+    *t_out = sample_trilinear(s, s->temperature, gx, gy, gz) * feather;
+}
+
 /* fluid.rs:131-150 */
 static void stencil_minmax(const OFluid *s, const double *f, double x, double y, double z,
                            double *lo_out, double *hi_out) {

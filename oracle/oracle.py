@@ -51,6 +51,8 @@ def _load(omp: bool = False) -> C.CDLL:
         "of_h": (d, [vp]),
         "of_stats": (None, [vp, C.POINTER(l), pd, pd, pd, C.POINTER(i)]),
         "of_sample_trilinear": (d, [vp, i, d, d, d]),
+        "of_max_density": (d, [vp]),
+        "of_medium_sample": (None, [vp, pd, pd, d, d, d, pd, pd]),
         "of_advect": (None, [vp, i, d, d]),
         "of_enforce_bc": (None, [vp]),
         "of_buoyancy_confinement": (None, [vp, d]),
@@ -203,6 +205,24 @@ class OracleFluid:
 
     def sample_trilinear(self, field: str, x: float, y: float, z: float) -> float:
         return self._lib.of_sample_trilinear(self._h, FIELDS[field], x, y, z)
+
+    def max_density(self) -> float:
+        """fluid.rs:704-708 (FluidMedium::from_solver's max over density)"""
+        return self._lib.of_max_density(self._h)
+
+    def medium_sample(self, bmin, bmax, points) -> np.ndarray:
+        """fluid.rs:739-788 FluidMedium::sample_density_and_temp at each
+        point (rows of xyz); returns an (n, 2) array of (density, temp)."""
+        lo = (C.c_double * 3)(*bmin)
+        hi = (C.c_double * 3)(*bmax)
+        pts = np.asarray(points, dtype=np.float64).reshape(-1, 3)
+        out = np.empty((len(pts), 2))
+        dv, tv = C.c_double(), C.c_double()
+        for a, (x, y, z) in enumerate(pts):
+            self._lib.of_medium_sample(self._h, lo, hi, float(x), float(y), float(z),
+                                       C.byref(dv), C.byref(tv))
+            out[a] = (dv.value, tv.value)
+        return out
 
     def apply_laplacian(self, x: np.ndarray) -> np.ndarray:
         x = np.ascontiguousarray(x, dtype=np.float64).reshape(self.nz, self.ny, self.nx)

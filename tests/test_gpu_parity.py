@@ -388,6 +388,38 @@ def test_mixed_precision_matches_fp64_projection_steps():
         assert rel_l2(a.download(n), b.download(n)) <= 1e-5, n
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_fluid_medium_vs_oracle(precision):
+    """FluidMedium::from_solver and sample_density_and_temp (fluid.rs:703-788)
+    on the device against the oracle: points inside, outside and on the
+    walls of the bounds; the medium is a snapshot (a later solver step leaves
+    it unchanged)."""
+    _announce(f"fluid medium {precision} 14x18x12")
+    import aether_fluid as af
+    nx, ny, nz = 14, 18, 12
+    st = random_state(nx, ny, nz, seed=77)
+    gpu, orc = make_pair(nx, ny, nz, precision=precision, state=st)
+    if precision == "fp32":  # the oracle sees the FP32-rounded fields the device holds
+        for n in ("density", "temperature"):
+            orc.field(n)[...] = gpu.download(n).reshape(nz, ny, nx)
+    lo, hi = (-2.0, 0.5, 1.0), (2.0, 3.0, 2.5)
+    rng = np.random.default_rng(8)
+    pts = np.column_stack([rng.uniform(lo[a] - 0.2, hi[a] + 0.2, 400) for a in range(3)])
+    pts[:8] = [lo, hi, (lo[0], 1.0, 2.0), (hi[0], 1.0, 2.0), (0.0, hi[1], 2.0), (0.0, 3.0, 1.75),
+               (0.0, 2.9, 1.75), (0.0, 2.9875, 1.75)]
+    med = af.FluidMedium.from_solver(gpu, (lo, hi), sigma_t_scale=3.5, phase_g=1.4)
+    got = med.sample_density_and_temp(pts)
+    exp = orc.medium_sample(lo, hi, pts)
+    assert np.all((got == 0.0) == (exp == 0.0))
+    np.testing.assert_allclose(got, exp, rtol=1e-13, atol=1e-15)
+    md = orc.max_density()
+    assert med.max_density == md
+    assert med.majorant_sigma_t == max(md * 3.5, 1e-4) and med.phase_g == 0.9
+    gpu.gravity_alpha, gpu.solver_iters = 0.05, 5
+    gpu.step(0.05, 0.0)
+    np.testing.assert_array_equal(med.sample_density_and_temp(pts), got)
+
+
 def _batched_vs_single(insts, single_form, case):
     """A batch of grids of different sizes gives, bit for bit, the fields and
     CG iteration counts of single-grid solvers run one at a time with the

@@ -116,6 +116,59 @@ def test_trilinear_clamps_outside():
     assert a == b
 
 
+# ---------------------------------------------------------------- FluidMedium
+def _feather(u):
+    """fluid.rs:755-758 restated independently (numpy clip)."""
+    fx = np.clip(4.0 * u[0] * (1.0 - u[0]), 0.0, 1.0)
+    fz = np.clip(4.0 * u[2] * (1.0 - u[2]), 0.0, 1.0)
+    fy = np.clip(1.0 - max(u[1] - 0.95, 0.0) / 0.05, 0.0, 1.0)
+    return math.sqrt(fx * fz * fy)
+
+
+def test_medium_sample_affine_fields_and_feather():
+    """Inside the clamp box, trilinear interpolation of affine density and
+    temperature fields is exact; the result is scaled by the edge feather
+    and sampled at u * (n - 1) of the bounds (fluid.rs:739-788)."""
+    nx, ny, nz = 10, 12, 9
+    s = OracleFluid(nx, ny, nz)
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    s.field("density")[...] = 0.5 + 0.25 * i + 0.125 * j - 0.0625 * k
+    s.field("temperature")[...] = 2.0 - 0.5 * i + 0.75 * j + 0.25 * k
+    lo, hi = (-1.0, 0.0, 2.0), (3.0, 2.5, 4.0)
+    rng = np.random.default_rng(5)
+    pts = np.column_stack([rng.uniform(lo[a], hi[a], 300) for a in range(3)])
+    got = s.medium_sample(lo, hi, pts)
+    for p, (d, t) in zip(pts, got):
+        u = [(p[a] - lo[a]) / (hi[a] - lo[a]) for a in range(3)]
+        g = [min(max(u[a] * (n - 1), 0.5), n - 1.500001) for a, n in enumerate((nx, ny, nz))]
+        f = _feather(u)
+        assert abs(d - (0.5 + 0.25 * g[0] + 0.125 * g[1] - 0.0625 * g[2]) * f) < 1e-12
+        assert abs(t - (2.0 - 0.5 * g[0] + 0.75 * g[1] + 0.25 * g[2]) * f) < 1e-12
+
+
+def test_medium_sample_outside_and_feather_values():
+    s = OracleFluid(8, 8, 8)
+    s.field("density")[...] = 1.0
+    s.field("temperature")[...] = 3.0
+    lo, hi = (0.0, 0.0, 0.0), (1.0, 2.0, 1.0)
+    out = s.medium_sample(lo, hi, [(-0.01, 1.0, 0.5), (0.5, 2.01, 0.5), (0.5, 1.0, 1.5)])
+    assert np.all(out == 0.0)
+    # centre: feather 1; uy = 0.975: fy = 0.5, feather sqrt(0.5); x wall: 0
+    out = s.medium_sample(lo, hi, [(0.5, 1.0, 0.5), (0.5, 1.95, 0.5), (0.0, 1.0, 0.5)])
+    assert out[0, 0] == 1.0 and out[0, 1] == 3.0
+    assert abs(out[1, 0] - math.sqrt(0.5)) < 1e-15
+    assert out[2, 0] == 0.0 and out[2, 1] == 0.0
+
+
+def test_medium_max_density_starts_from_zero():
+    """fluid.rs:704-707: max over density starting from 0."""
+    s = OracleFluid(6, 6, 6)
+    s.field("density")[...] = -1.0
+    assert s.max_density() == 0.0
+    s.field("density")[2, 3, 4] = 7.5
+    assert s.max_density() == 7.5
+
+
 def test_maccormack_integer_shift_is_exact():
     """Uniform velocity moving exactly one cell per step: interior cells away
     from the walls receive the upwind neighbour's value times dissipation."""
