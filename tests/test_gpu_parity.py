@@ -190,6 +190,30 @@ def test_fp64_main_rs_stage_20_steps():
     assert max(errs.values()) <= 1e-6, errs
 
 
+def test_cli_selfcheck_and_fluid_stage(capsys):
+    """The command line: the verify.rs:229-250 self-check reproduces the
+    reference's printed digits (FP64) and passes its 3% criterion (FP64 and
+    FP32); the main.rs:82-113 stage at its real grid (36x52x36) gives the
+    oracle's telemetry (12 steps) and reports the medium's max density."""
+    _announce("cli selfcheck + main.rs stage 36x52x36")
+    from aether_fluid import cli
+    from oracle.oracle import OracleFluid
+    r = cli.selfcheck("fp64")
+    assert r["passed"] and r["matches_reference_digits"], r
+    assert cli.selfcheck("fp32")["passed"]
+    assert cli.main(["selfcheck"]) == 0
+    assert "[PASS] 3D Navier-Stokes CG Poisson Projection" in capsys.readouterr().out
+    st = cli.fluid_stage(12, "fp64")
+    orc = OracleFluid(36, 52, 36)
+    orc.vorticity_eps, orc.solver_iters = 6.5, 28
+    to = np.array([orc.step(0.08, 15.0 * (i / 11.0)) for i in range(12)])
+    tg = np.array([f.as_array() for f in st["frames"]])
+    np.testing.assert_allclose(tg, to, rtol=1e-6, atol=1e-12)
+    assert st["peak_ke"] == tg[:, 1].max() and st["peak_vort"] == tg[:, 3].max()
+    assert abs(st["max_density"] - orc.max_density()) <= 1e-6 * orc.max_density()
+    assert "[1/4] 3D Navier-Stokes Solver (36x52x36, 12 steps)" in st["line"]
+
+
 def _plain_pair(name, dims, precision):
     from aether_fluid.scenarios import SMOKE_CONFIGS, seeded_config_state
     cfg = SMOKE_CONFIGS[name]
