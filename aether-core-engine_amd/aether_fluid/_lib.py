@@ -50,7 +50,7 @@ class Params(C.Structure):
         ("ir_inner_rel_tol", C.c_double),
         ("ir_div_tol", C.c_double),
         ("ir_max_outer", C.c_int32),
-        ("reserved1", C.c_int32),
+        ("batch_graph", C.c_int32),
     ]
 
 

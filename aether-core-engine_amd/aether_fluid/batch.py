@@ -16,11 +16,11 @@ class FluidBatch:
     source_radius, source_density, source_temperature.  Shared knobs
     (solver_iters, cg_rel_tol, dissipation, cg_form) come from keyword
     arguments; cg_form CG_AUTO runs the four-wide CG when every nx % 4 == 0
-    (FP32)."""
+    (FP32).  graph=True captures each step into a HIP graph and replays it."""
 
     def __init__(self, instances, precision: str = "fp32", solver_iters: int = 200,
                  cg_rel_tol: float = 1e-5, dissipation=(0.996, 0.996, 0.985),
-                 cg_form: int = L.CG_AUTO):
+                 cg_form: int = L.CG_AUTO, graph: bool = False):
         self._lib = L.lib()
         n = len(instances)
         descs = (L.InstanceDesc * n)()
@@ -39,6 +39,7 @@ class FluidBatch:
         L.check(self._lib.aether_fluid_default_params(C.byref(p)))
         p.solver_kind, p.solver_iters, p.cg_rel_tol = 0, solver_iters, cg_rel_tol
         p.cg_form = cg_form
+        p.batch_graph = 1 if graph else 0
         (p.dissipation_velocity, p.dissipation_density,
          p.dissipation_temperature) = dissipation
         prec = {"fp32": L.FP32, "fp64": L.FP64}[precision]

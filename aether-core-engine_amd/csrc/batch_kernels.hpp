@@ -209,12 +209,14 @@ __global__ __launch_bounds__(kThreads) void kb_cg_update4(const BInst<float> *__
 }
 
 // One workgroup per instance: closes solves that ran out of iterations and
-// logs the iteration count of step `step`.  x4: the solve ran the four-wide
-// launches (their r.r partials are nbv wide).
+// logs the iteration count of step `step` (of *step_dev when it is given: a
+// step captured in a graph reads its index from the device).  x4: the solve
+// ran the four-wide launches (their r.r partials are nbv wide).
 template <typename T>
 __global__ __launch_bounds__(kThreads) void kb_cg_finalize(const BInst<T> *__restrict__ I,
                                                            int max_iters, double rel_tol2,
-                                                           int step, int x4) {
+                                                           int step, int x4,
+                                                           const int *__restrict__ step_dev) {
     const BInst<T> &in = I[blockIdx.x];
     __shared__ double lds[8];
     __shared__ int flag_lds[1];
@@ -236,7 +238,7 @@ __global__ __launch_bounds__(kThreads) void kb_cg_finalize(const BInst<T> *__res
         }
     }
     __syncthreads();
-    if (tid_lin() == 0) in.iters[step] = st->iters;
+    if (tid_lin() == 0) in.iters[step_dev ? *step_dev : step] = st->iters;
 }
 
 template <typename T, int ZU>
@@ -251,11 +253,22 @@ __global__ __launch_bounds__(kThreads) void kb_telemetry_part(const BInst<T> *__
                            in.f[sl.v][1], in.f[sl.w][2], in.f[sl.rho][3], in.ptel);
 }
 
+// step_dev / times_dev (optional): the step index and the time axis on the
+// device, for steps captured in a graph.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void kb_telemetry_final(const BInst<T> *__restrict__ I,
-                                                               int step, double time_sec) {
+                                                               int step, double time_sec,
+                                                               const int *__restrict__ step_dev,
+                                                               const double *__restrict__ times_dev) {
     const BInst<T> &in = I[blockIdx.x];
-    body_telemetry_final(in.ptel, in.nb, time_sec, in.tel + (size_t)step * 8);
+    const int s = step_dev ? *step_dev : step;
+    body_telemetry_final(in.ptel, in.nb, step_dev ? times_dev[s] : time_sec,
+                         in.tel + (size_t)s * 8);
+}
+
+// Last node of a captured step: advances the device step index.
+__global__ void kb_step_advance(int *step_dev) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *step_dev += 1;
 }
 
 }  // namespace aether

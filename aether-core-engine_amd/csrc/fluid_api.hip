@@ -1876,7 +1876,9 @@ class Batch final : public BatchBase {
         }
     }
 
-    void cg_solve(int step) {
+    // graph: the step is being captured, so every iteration is queued (no
+    // host look-ahead) and the step index comes from the device.
+    void cg_solve(int step, bool graph = false) {
         const int max_iters = params_.solver_iters;
         const bool x4 = use_x4();
         const double rel2 = params_.cg_rel_tol > 0 ? params_.cg_rel_tol * params_.cg_rel_tol : 0.0;
@@ -1890,7 +1892,7 @@ class Batch final : public BatchBase {
         }
         const dim3 blk(kBX, kBY);
         for (int c = 0; c < nchunks; ++c) {
-            if (c >= 2) {
+            if (c >= 2 && !graph) {
                 HIPT(hipEventSynchronize(chunk_events_[c - 2]));
                 bool all = true;
                 for (int q = 0; q < count() && all; ++q)
@@ -1913,15 +1915,16 @@ class Batch final : public BatchBase {
                                                                  done_dev_, solve_id);
                 LAUNCH_CHECK();
             }
-            HIPT(hipEventRecord(chunk_events_[c], st_));
+            if (!graph) HIPT(hipEventRecord(chunk_events_[c], st_));
         }
-        kb_cg_finalize<T><<<count(), blk, 0, st_>>>(table_.p, max_iters, rel2, step, x4 ? 1 : 0);
+        kb_cg_finalize<T><<<count(), blk, 0, st_>>>(table_.p, max_iters, rel2, step, x4 ? 1 : 0,
+                                                    graph ? step_dev_.p : nullptr);
         LAUNCH_CHECK();
     }
 
-    // src/fluid.rs:598-611 for every instance at once
-    void step(double dt, double t, int s) {
-        (void)t;
+    // src/fluid.rs:598-611 for every instance at once.  graph: the launches
+    // are being captured; the step index and time are read on the device.
+    void step(double dt, double t, int s, bool graph = false) {
         const dim3 blk(kBX, kBY);
         advect1(0, dt, params_.dissipation_velocity);
         advect1(1, dt, params_.dissipation_velocity);
@@ -1939,7 +1942,7 @@ class Batch final : public BatchBase {
         sl_.w ^= 1;
         kb_div_cginit<T, kBatchZU><<<ncg_, blk, 0, st_>>>(table_.p, cg_items_.p, sl_);
         LAUNCH_CHECK();
-        cg_solve(s);
+        cg_solve(s, graph);
         kb_gradient_bc<T><<<ncell_, blk, 0, st_>>>(table_.p, cell_items_.p, sl_);
         LAUNCH_CHECK();
         kb_sl_forward<T, 2><<<ncell_, blk, 0, st_>>>(table_.p, cell_items_.p, dt, sl_, 3);
@@ -1952,8 +1955,63 @@ class Batch final : public BatchBase {
         sl_.t ^= 1;
         kb_telemetry_part<T, kBatchZU><<<ncg_, blk, 0, st_>>>(table_.p, cg_items_.p, sl_);
         LAUNCH_CHECK();
-        kb_telemetry_final<T><<<count(), blk, 0, st_>>>(table_.p, s, t);
+        kb_telemetry_final<T><<<count(), blk, 0, st_>>>(table_.p, s, t,
+                                                        graph ? step_dev_.p : nullptr,
+                                                        graph ? times_dev_.p : nullptr);
         LAUNCH_CHECK();
+        if (graph) {
+            kb_step_advance<<<1, 64, 0, st_>>>(step_dev_.p);
+            LAUNCH_CHECK();
+        }
+    }
+
+    // Captures one step into an executable graph (the host-side slot
+    // bookkeeping advances as if the step had run).
+    hipGraphExec_t capture_step(double dt) {
+        hipGraph_t graph = nullptr;
+        HIPT(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
+        try {
+            step(dt, 0.0, 0, true);
+        } catch (...) {
+            (void)hipStreamEndCapture(st_, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        HIPT(hipStreamEndCapture(st_, &graph));
+        hipGraphExec_t exec = nullptr;
+        const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        HIPT(e);
+        return exec;
+    }
+
+    // Steps as graph replays: the ping-pong slots of density/temperature
+    // alternate per step, so two graphs (one per parity) serve every step.
+    void run_graphs(double dt, const double *times, int nsteps) {
+        if (times_dev_.n < (size_t)nsteps) times_dev_.alloc(nsteps);
+        HIPT(hipMemcpyAsync(times_dev_.p, times, nsteps * sizeof(double), hipMemcpyHostToDevice,
+                            st_));
+        if (!step_dev_.p) step_dev_.alloc(1);
+        k_fill<int><<<1, kThreads, 0, st_>>>(step_dev_.p, 1L, 0);
+        LAUNCH_CHECK();
+        HIPT(hipStreamSynchronize(st_));
+        struct Execs {
+            hipGraphExec_t g[2] = {nullptr, nullptr};
+            ~Execs() {
+                for (auto e : g)
+                    if (e) (void)hipGraphExecDestroy(e);
+            }
+        } ex;
+        const Slots start = sl_;
+        ex.g[0] = capture_step(dt);
+        if (nsteps > 1) ex.g[1] = capture_step(dt);
+        sl_ = start;
+        for (int s = 0; s < nsteps; ++s) HIPT(hipGraphLaunch(ex.g[s & 1], st_));
+        if (nsteps & 1) {  // the slots the replays left current
+            sl_.rho ^= 1;
+            sl_.t ^= 1;
+        }
+        HIPT(hipStreamSynchronize(st_));
     }
 
     void run(double dt, const double *times, int nsteps, aether_fluid_telemetry *out) override {
@@ -1963,7 +2021,10 @@ class Batch final : public BatchBase {
             if (in->iters.n < (size_t)nsteps) in->iters.alloc(nsteps);
         }
         write_table();
-        for (int s = 0; s < nsteps; ++s) step(dt, times[s], s);
+        if (params_.batch_graph)
+            run_graphs(dt, times, nsteps);
+        else
+            for (int s = 0; s < nsteps; ++s) step(dt, times[s], s);
         const int cnt = count();
         std::vector<double> raw((size_t)nsteps * 8);
         iters_.assign((size_t)nsteps * cnt, 0);
@@ -2000,6 +2061,8 @@ class Batch final : public BatchBase {
     DevBuf<int2> src_items_;
     int ncell_ = 0, ncg_ = 0, nsrc_ = 0, ncg4_ = 0;
     bool x4_ok_ = false;  // FP32 and every nx % 4 == 0: four-wide CG available
+    DevBuf<int> step_dev_;          // device step index of captured steps
+    DevBuf<double> times_dev_;      // their time axis
     bool wide_ = false;   // some instance has more than 1024 CG partials
     Slots sl_ = {0, 0, 0, 0, 0};
     int *done_host_ = nullptr, *done_dev_ = nullptr;

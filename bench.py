@@ -266,7 +266,18 @@ def bench_batched(steps: int, warmup: int, dt: float = 0.01) -> dict:
     b.run(dt, times[warmup:])
     el_b = time.perf_counter() - t0
     it_b = b.iterations()
+    rho_b = b.download(len(insts) - 1, "density")
     b.close()
+    # the same batch with every step captured into a HIP graph and replayed
+    bg = af.FluidBatch(insts, solver_iters=200, cg_rel_tol=1e-5, graph=True)
+    if warmup:
+        bg.run(dt, times[:warmup])
+    t0 = time.perf_counter()
+    bg.run(dt, times[warmup:])
+    el_g = time.perf_counter() - t0
+    same = bool(np.array_equal(bg.download(len(insts) - 1, "density"), rho_b) and
+                np.array_equal(bg.iterations(), it_b))
+    bg.close()
     el_1 = 0.0
     it_1 = []
     for d in insts:
@@ -286,6 +297,10 @@ def bench_batched(steps: int, warmup: int, dt: float = 0.01) -> dict:
             "total_voxels": nvox, "steps": steps, "warmup": warmup,
             "batched": {"voxel_steps_per_s": nvox * steps / el_b, "ms_per_step": el_b * 1e3 / steps,
                         "cg_iters_mean": float(np.mean(it_b))},
+            "batched_graph": {"voxel_steps_per_s": nvox * steps / el_g,
+                              "ms_per_step": el_g * 1e3 / steps,
+                              "bitwise_equal_to_stream_launches": same,
+                              "note": "includes capturing two step graphs per run"},
             "one_at_a_time": {"voxel_steps_per_s": nvox * steps / el_1,
                               "ms_per_step": el_1 * 1e3 / steps,
                               "cg_iters_mean": float(np.mean(it_1))},

@@ -91,7 +91,9 @@ typedef struct {
     double ir_inner_rel_tol;        /* AETHER_FP32_IR64: inner FP32 CG tolerance (1e-3) */
     double ir_div_tol;              /* AETHER_FP32_IR64: stop when max|div u| <= this (1e-7) */
     int32_t ir_max_outer;           /* AETHER_FP32_IR64: max refinement sweeps (30) */
-    int32_t reserved1;
+    int32_t batch_graph;            /* batches only: 1 = capture each step into a HIP graph
+                                       and replay it (all solver_iters CG launches are in
+                                       the graph; converged instances return at once) */
 } aether_fluid_params;
 
 /* src/fluid.rs:6-15 FluidTelemetryFrame (center_of_mass flattened) */

@@ -444,6 +444,39 @@ def test_fluid_medium_vs_oracle(precision):
     np.testing.assert_array_equal(med.sample_density_and_temp(pts), got)
 
 
+def test_batched_graph_replay_equals_stream_launches():
+    """Steps captured into HIP graphs and replayed give, bit for bit, what the
+    same batch gives launched step by step: odd step counts (the density /
+    temperature slot parity) and a second run continuing the first."""
+    _announce("batched graph replay vs stream launches")
+    import aether_fluid as af
+    insts = [dict(nx=32, ny=24, nz=20, vorticity_eps=0.3, buoyancy_beta=1.0, gravity_alpha=0.05,
+                  source_center=(16.0, 4.0, 10.0), source_radius=3.0),
+             dict(nx=20, ny=28, nz=16, vorticity_eps=0.2, buoyancy_beta=1.0, gravity_alpha=0.08,
+                  source_center=(10.0, 4.0, 8.0), source_radius=2.5)]
+    runs = []
+    for graph in (False, True):
+        b = af.FluidBatch(insts, solver_iters=60, cg_rel_tol=1e-5, graph=graph)
+        for q, d in enumerate(insts):
+            st = random_state(d["nx"], d["ny"], d["nz"], seed=90 + q, vel=0.5)
+            for n in ("u", "v", "w", "density", "temperature"):
+                b.upload(q, n, st[n])
+        f1 = b.run(0.02, [0.0, 0.02, 0.04])
+        it1 = b.iterations()
+        f2 = b.run(0.02, [0.06, 0.08])
+        it2 = b.iterations()
+        fields = [b.download(q, n) for q in range(2)
+                  for n in ("u", "v", "w", "density", "temperature", "pressure")]
+        tel = np.array([[fr.as_array() for fr in row] for row in f1 + f2])
+        runs.append((fields, tel, it1, it2))
+    (fa, ta, ia1, ia2), (fb, tb, ib1, ib2) = runs
+    np.testing.assert_array_equal(ia1, ib1)
+    np.testing.assert_array_equal(ia2, ib2)
+    np.testing.assert_array_equal(ta, tb)
+    for x, y in zip(fa, fb):
+        np.testing.assert_array_equal(x, y)
+
+
 def _batched_vs_single(insts, single_form, case):
     """A batch of grids of different sizes gives, bit for bit, the fields and
     CG iteration counts of single-grid solvers run one at a time with the
