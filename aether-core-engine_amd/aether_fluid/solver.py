@@ -282,6 +282,23 @@ class FluidSolver3D:
             _dptr(gc) if grad else None, C.byref(gb) if grad else None))
         return loss.value, gc, (gb.value if grad else None)
 
+    _TUNE_CANDIDATES = (("x4", 2), ("x4", 4), ("scalar", 4), ("scalar", 8), ("scalar", 16),
+                        ("fused", 4))
+
+    def autotune_cg(self, iters: int = 30) -> dict:
+        """Times `iters` CG iterations of every layout candidate on a synthetic
+        right-hand side and keeps the fastest as this solver's CG_AUTO choice
+        (pressure and divergence are restored).  Returns the choice and the
+        time per iteration (us) of each candidate that applies."""
+        us = (C.c_double * 6)()
+        form, z = C.c_int(), C.c_int()
+        L.check(self._lib.aether_fluid_autotune_cg(self._h, int(iters), us, C.byref(form),
+                                                   C.byref(z)))
+        names = {L.CG_SCALAR: "scalar", L.CG_X4: "x4", L.CG_FUSED: "fused"}
+        return {"choice": {"form": names[form.value], "z_cells": z.value},
+                "us_per_iter": {f"{f}_z{zc}": round(us[a], 3)
+                                for a, (f, zc) in enumerate(self._TUNE_CANDIDATES) if us[a] >= 0}}
+
     def set_kernel_timing(self, enable: bool) -> None:
         L.check(self._lib.aether_fluid_set_kernel_timing(self._h, int(enable)))
 

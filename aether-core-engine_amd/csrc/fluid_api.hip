@@ -278,6 +278,7 @@ class SolverBase {
     virtual void sample(int field, const double *xyz, size_t npts, double *out) = 0;
     virtual double max_density() = 0;
     virtual std::unique_ptr<MediumBase> make_medium(const double *lo, const double *hi) = 0;
+    virtual void autotune(int iters, double *us, int *form, int *z) = 0;
     virtual void stats(aether_fluid_solve_stats *out) = 0;
     virtual const std::vector<int> &last_run_iters() const = 0;
     virtual double rollout(const aether_rollout_desc &d, const double *ctrl, double beta,
@@ -325,8 +326,8 @@ class Solver final : public SolverBase {
         tl_.gy = gy;
         tl_.gz = (nz + zu_ - 1) / zu_;
         tl_.nb = gx * gy * tl_.gz;
-        tl_.remap = (tl_.nb % 8 == 0) ? 1 : 0;
-        if (const char *e = std::getenv("AETHER_XCD_REMAP")) tl_.remap &= std::atoi(e) != 0;
+        if (const char *e = std::getenv("AETHER_XCD_REMAP")) remap_ok_ = std::atoi(e) != 0;
+        tl_.remap = (tl_.nb % 8 == 0 && remap_ok_) ? 1 : 0;
         // AETHER_CG_FORM (1/2/3) overrides the AETHER_CG_AUTO choice
         if (const char *e = std::getenv("AETHER_CG_FORM")) {
             const int f = std::atoi(e);
@@ -711,13 +712,122 @@ class Solver final : public SolverBase {
         if constexpr (sizeof(T) != 4) {
             return AETHER_CG_SCALAR;
         } else {
+            if (trial_form_) return trial_form_;  // the autotuner's candidate
             int f = params.cg_form;
             if (f == AETHER_CG_AUTO)
-                f = env_form_ ? env_form_ : (g_.nx % 4 == 0 ? AETHER_CG_X4 : AETHER_CG_SCALAR);
+                f = env_form_ ? env_form_
+                              : tuned_form_ ? tuned_form_
+                                            : (g_.nx % 4 == 0 ? AETHER_CG_X4 : AETHER_CG_SCALAR);
             if (f == AETHER_CG_X4 && g_.nx % 4 != 0)
                 raise(AETHER_EINVAL, "AETHER_CG_X4 needs nx % 4 == 0");
             return f;
         }
+    }
+
+    // Partial-sum buffers hold at least n slots (their contents are per solve).
+    void ensure_partials(int n) {
+        for (DevBuf<double> *b : {&partA_, &partB_, &partC_})
+            if ((int)b->n < n) b->alloc(n);
+        if (r2_.p)
+            for (DevBuf<double> *b : {&partE_, &partF_})
+                if ((int)b->n < n) b->alloc(n);
+    }
+
+    // z cells per thread of the one-cell-per-thread tiling (also the CG
+    // initialisation's tiling) and of the four-wide tiling.
+    void set_tilings(int zu, int vzu) {
+        zu_ = zu;
+        tl_.gz = (g_.nz + zu_ - 1) / zu_;
+        tl_.nb = tl_.gx * tl_.gy * tl_.gz;
+        tl_.remap = (tl_.nb % 8 == 0 && remap_ok_) ? 1 : 0;
+        ensure_partials(tl_.nb);
+        if (tlv_.nb > 0) {
+            vzu_ = vzu;
+            tlv_.gz = (g_.nz + vzu_ - 1) / vzu_;
+            tlv_.nb = tlv_.gx * tlv_.gy * tlv_.gz;
+            tlv_.remap = (tlv_.nb % 8 == 0 && remap_ok_) ? 1 : 0;
+            ensure_partials(tlv_.nb);
+        }
+    }
+
+    // Times `iters` CG iterations of every candidate layout on a synthetic
+    // right-hand side of this grid (p and b are saved and restored) and
+    // keeps the fastest as this handle's AETHER_CG_AUTO choice.  us[6]
+    // receives the time per iteration of: four-wide with 2 and 4 z cells per
+    // thread, one cell per thread with 4, 8 and 16, the one-launch form
+    // (-1 where a candidate does not apply).
+    void autotune(int iters, double *us, int *form_out, int *z_out) override {
+        if (iters < 1) raise(AETHER_EINVAL, "iters must be >= 1");
+        struct Cand {
+            int form, z;
+        };
+        const Cand cands[6] = {{AETHER_CG_X4, 2},     {AETHER_CG_X4, 4},
+                               {AETHER_CG_SCALAR, 4}, {AETHER_CG_SCALAR, 8},
+                               {AETHER_CG_SCALAR, 16}, {AETHER_CG_FUSED, 4}};
+        const bool fp32 = sizeof(T) == 4;
+        DevBuf<T> saved_p(n_), saved_b(n_);
+        HIPT(hipMemcpyAsync(saved_p.p, p_.p, n_ * sizeof(T), hipMemcpyDeviceToDevice, st_));
+        HIPT(hipMemcpyAsync(saved_b.p, b_.p, n_ * sizeof(T), hipMemcpyDeviceToDevice, st_));
+        k_fill_hash<T><<<blocks_for(n_), kThreads, 0, st_>>>(b_.p, (long)n_);
+        LAUNCH_CHECK();
+        const int zu0 = zu_, vzu0 = vzu_;
+        const double rel0 = rel_override_;
+        rel_override_ = 0.0;  // absolute guards only: every candidate runs `iters`
+        hipEvent_t e0, e1;
+        HIPT(hipEventCreate(&e0));
+        HIPT(hipEventCreate(&e1));
+        double best = 1e300;
+        Cand pick = {AETHER_CG_SCALAR, zu0};
+        try {
+            for (int a = 0; a < 6; ++a) {
+                const Cand c = cands[a];
+                const bool ok = c.form == AETHER_CG_SCALAR ||
+                                (fp32 && (c.form == AETHER_CG_FUSED || tlv_.nb > 0));
+                if (us) us[a] = -1.0;
+                if (!ok) continue;
+                set_tilings(c.form == AETHER_CG_SCALAR ? c.z : zu0,
+                            c.form == AETHER_CG_X4 ? c.z : vzu0);
+                trial_form_ = fp32 ? c.form : 0;
+                float ms = 0.0f;
+                for (int rep = 0; rep < 2; ++rep) {  // the first run warms up
+                    zero(p_.p);
+                    ZU_DISPATCH(zu_, (k_div_cginit<T, ZU><<<tl_.nb, cell_blk_, 0, st_>>>(
+                                          g_, tl_, (T)h_, 0, cur_[0], cur_[1], cur_[2], p_.p,
+                                          b_.p, r_.p, partB_.p, partC_.p, cg_.p)));
+                    LAUNCH_CHECK();
+                    HIPT(hipEventRecord(e0, st_));
+                    enqueue_cg_loop(iters, -1);
+                    HIPT(hipEventRecord(e1, st_));
+                    HIPT(hipEventSynchronize(e1));
+                    HIPT(hipEventElapsedTime(&ms, e0, e1));
+                }
+                const double t = 1e3 * (double)ms / iters;
+                if (us) us[a] = t;
+                if (t < best) {
+                    best = t;
+                    pick = c;
+                }
+            }
+        } catch (...) {
+            trial_form_ = 0;
+            rel_override_ = rel0;
+            set_tilings(zu0, vzu0);
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            throw;
+        }
+        trial_form_ = 0;
+        rel_override_ = rel0;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        set_tilings(pick.form == AETHER_CG_SCALAR ? pick.z : zu0,
+                    pick.form == AETHER_CG_X4 ? pick.z : vzu0);
+        tuned_form_ = fp32 ? pick.form : 0;
+        HIPT(hipMemcpyAsync(p_.p, saved_p.p, n_ * sizeof(T), hipMemcpyDeviceToDevice, st_));
+        HIPT(hipMemcpyAsync(b_.p, saved_b.p, n_ * sizeof(T), hipMemcpyDeviceToDevice, st_));
+        sync();
+        if (form_out) *form_out = pick.form;
+        if (z_out) *z_out = pick.z;
     }
 
     void ensure_fused_buffers() {
@@ -1522,6 +1632,9 @@ class Solver final : public SolverBase {
     Tiling tl_;
     int zu_ = 4;
     int env_form_ = 0;    // AETHER_CG_FORM override of AETHER_CG_AUTO (0: none)
+    int tuned_form_ = 0;  // the autotuner's AETHER_CG_AUTO choice (0: none)
+    int trial_form_ = 0;  // layout forced while the autotuner times a candidate
+    bool remap_ok_ = true;  // XCD-contiguous tile order allowed (AETHER_XCD_REMAP)
     int form_ = AETHER_CG_SCALAR;  // layout of the solve being enqueued
     VTiling tlv_ = {0, 0, 0, 0, 0, 1, 1};  // four-wide CG tiling (nx % 4 == 0)
     int vzu_ = 4;
@@ -2343,6 +2456,14 @@ int aether_fluid_max_density(aether_fluid_t s, double *out) {
     });
 }
 
+int aether_fluid_autotune_cg(aether_fluid_t s, int iters, double *us_per_iter, int *form,
+                             int *z_cells) {
+    return guarded([&] {
+        checked(s);
+        s->impl->autotune(iters, us_per_iter, form, z_cells);
+    });
+}
+
 struct aether_fluid_medium_s {
     int device;
     std::unique_ptr<MediumBase> impl;
@@ -2529,9 +2650,9 @@ int aether_fluid_stream_copy(size_t nbytes, int reps, double *avg_ms) {
         int cus = 256;
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0);
         const unsigned blocks = (unsigned)std::min<long>((n4 + kThreads - 1) / kThreads, 8L * cus);
-        k_fill_hash<<<blocks, kThreads, 0, st>>>(reinterpret_cast<float *>(a.p), 4 * n4);
+        k_fill_hash<float><<<blocks, kThreads, 0, st>>>(reinterpret_cast<float *>(a.p), 4 * n4);
         LAUNCH_CHECK();
-        k_fill_hash<<<blocks, kThreads, 0, st>>>(reinterpret_cast<float *>(b.p), 4 * n4);
+        k_fill_hash<float><<<blocks, kThreads, 0, st>>>(reinterpret_cast<float *>(b.p), 4 * n4);
         LAUNCH_CHECK();
         k_stream_copy<<<blocks, kThreads, 0, st>>>(a.p, n4, b.p);  // warm-up
         LAUNCH_CHECK();

@@ -1310,13 +1310,14 @@ __global__ __launch_bounds__(kThreads) void k_stream_copy(const float4 *__restri
 // Fills x with full-entropy values in [-1, 1) (a SplitMix64 hash of the
 // index): bandwidth measured on zero-filled buffers is optimistic on MI355X,
 // so the calibration copy streams data of the same character as the fields.
-__global__ __launch_bounds__(kThreads) void k_fill_hash(float *__restrict__ x, long n) {
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_fill_hash(T *__restrict__ x, long n) {
     for (long a = (long)blockIdx.x * kThreads + tid_lin(); a < n; a += (long)gridDim.x * kThreads) {
         unsigned long long z = (unsigned long long)a + 0x9E3779B97F4A7C15ull;
         z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
         z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
         z ^= z >> 31;
-        x[a] = (float)((double)(z >> 11) * (1.0 / 4503599627370496.0) - 1.0);
+        x[a] = (T)((double)(z >> 11) * (1.0 / 4503599627370496.0) - 1.0);
     }
 }
 

@@ -175,6 +175,17 @@ int aether_fluid_download(aether_fluid_t s, int field, double *dst, size_t n);
  * (reduced on the device) and the f64 density/temperature snapshots. */
 int aether_fluid_max_density(aether_fluid_t s, double *out);
 
+/* Extension (no counterpart in the reference): CG layout autotuner.  Times
+ * `iters` CG iterations of each candidate on a synthetic right-hand side of
+ * this grid (pressure and divergence are saved and restored; the last-solve
+ * statistics then describe the tuning solves) and makes the fastest the
+ * AETHER_CG_AUTO choice of this handle.  us_per_iter (6 entries, may be
+ * NULL): four-wide with 2 and 4 z cells per thread, one cell per thread with
+ * 4, 8 and 16, the one-launch form; -1 where a candidate does not apply (FP64
+ * tunes the one-cell-per-thread tiling only).  *form / *z_cells: the choice. */
+int aether_fluid_autotune_cg(aether_fluid_t s, int iters, double *us_per_iter, int *form,
+                             int *z_cells);
+
 /* src/fluid.rs:687-788 FluidMedium, the renderer-side view of the solver.
  * aether_fluid_medium_create is FluidMedium::from_solver (fluid.rs:703-735):
  * it snapshots density and temperature on the device (later solver steps do

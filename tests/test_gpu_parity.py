@@ -444,6 +444,40 @@ def test_fluid_medium_vs_oracle(precision):
     np.testing.assert_array_equal(med.sample_density_and_temp(pts), got)
 
 
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_autotune_keeps_state_and_results(precision):
+    """The CG autotuner restores pressure and divergence, and the step after
+    it (with whatever layout it chose) matches the untuned solver to the
+    one-step FP32 bar (FP32) or to reduction-order differences (FP64)."""
+    _announce(f"cg autotune {precision} 40x32x36")
+    import aether_fluid as af
+    nx, ny, nz = 40, 32, 36
+    st = random_state(nx, ny, nz, seed=31)
+    a = af.FluidSolver3D(nx, ny, nz, precision=precision)
+    b = af.FluidSolver3D(nx, ny, nz, precision=precision)
+    for s in (a, b):
+        for n, v in st.items():
+            s.upload(n, v)
+        s.gravity_alpha, s.solver_iters, s.cg_rel_tol = 0.05, 120, 1e-6
+        s.set_sphere_source((20.0, 4.0, 18.0), 3.0)
+    p0, d0 = a.download("pressure"), a.download("divergence")
+    r = a.autotune_cg(10)
+    print("autotune", r)
+    np.testing.assert_array_equal(a.download("pressure"), p0)
+    np.testing.assert_array_equal(a.download("divergence"), d0)
+    if precision == "fp64":
+        assert set(r["us_per_iter"]) == {"scalar_z4", "scalar_z8", "scalar_z16"}
+    else:
+        assert len(r["us_per_iter"]) == 6
+    fa, fb = a.step(0.02, 0.0).as_array(), b.step(0.02, 0.0).as_array()
+    assert abs(a.last_solve_stats()["iters"] - b.last_solve_stats()["iters"]) <= 1
+    bar = 1e-5 if precision == "fp32" else 1e-9  # another layout sums its dots in another order
+    for n in ("u", "v", "w", "density"):
+        assert rel_l2(a.download(n), b.download(n)) <= bar, n
+    assert rel_l2(a.download("pressure"), b.download("pressure")) <= 10 * bar
+    np.testing.assert_allclose(fa, fb, rtol=bar)
+
+
 def test_batched_graph_replay_equals_stream_launches():
     """Steps captured into HIP graphs and replayed give, bit for bit, what the
     same batch gives launched step by step: odd step counts (the density /
