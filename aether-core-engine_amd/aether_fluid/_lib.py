@@ -134,6 +134,7 @@ SIGNATURES = {
     "aether_fluid_upload": (C.c_int, [_vp, C.c_int, _pd, C.c_size_t]),
     "aether_fluid_download": (C.c_int, [_vp, C.c_int, _pd, C.c_size_t]),
     "aether_fluid_max_density": (C.c_int, [_vp, _pd]),
+    "aether_fluid_batch_autotune_cg": (C.c_int, [_vp, C.c_int, _pd, C.POINTER(C.c_int)]),
     "aether_fluid_autotune_cg": (C.c_int, [_vp, C.c_int, _pd, C.POINTER(C.c_int),
                                            C.POINTER(C.c_int)]),
     "aether_fluid_medium_create": (C.c_int, [_vp, _pd, _pd, C.c_double, C.c_double,

@@ -83,6 +83,18 @@ class FluidBatch:
         return [[FluidTelemetryFrame._from(out[s * self.count + q]) for q in range(self.count)]
                 for s in range(len(ts))]
 
+    def autotune_cg(self, iters: int = 20) -> dict:
+        """Times `iters` CG iterations of the batch in each layout on synthetic
+        right-hand sides and keeps the faster as the batch's CG_AUTO choice
+        (pressure and divergence are restored)."""
+        us = (C.c_double * 2)()
+        form = C.c_int()
+        L.check(self._lib.aether_fluid_batch_autotune_cg(self._h, int(iters), us,
+                                                         C.byref(form)))
+        return {"choice": {L.CG_X4: "x4", L.CG_SCALAR: "scalar"}[form.value],
+                "us_per_iter": {k: round(us[a], 3) for a, k in enumerate(("x4", "scalar"))
+                                if us[a] >= 0}}
+
     def iterations(self) -> np.ndarray:
         """CG iterations of the last run, shape (steps, instances)."""
         cnt = C.c_int()

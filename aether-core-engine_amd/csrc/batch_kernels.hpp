@@ -144,13 +144,15 @@ __global__ __launch_bounds__(kThreads) void kb_gradient_bc(const BInst<T> *__res
                         in.f[sl.w][2]);
 }
 
+// with_div 0: keep the stored b (the autotuner's synthetic right-hand side).
 template <typename T, int ZU>
 __global__ __launch_bounds__(kThreads) void kb_div_cginit(const BInst<T> *__restrict__ I,
                                                           const BItem *__restrict__ items,
-                                                          Slots sl) {
+                                                          Slots sl, int with_div) {
     const BItem it = items[blockIdx.x];
     const BInst<T> &in = I[it.inst];
-    body_div_cginit<T, ZU>(col_at<ZU>(in.g, it.bx, it.by, it.bz), it.local, in.g, (T)in.h, 1,
+    body_div_cginit<T, ZU>(col_at<ZU>(in.g, it.bx, it.by, it.bz), it.local, in.g, (T)in.h,
+                           with_div,
                            in.f[sl.u][0], in.f[sl.v][1], in.f[sl.w][2], in.p, in.b, in.r, in.pb,
                            in.pc, in.st);
 }

@@ -1692,6 +1692,7 @@ constexpr int kBatchZU = 8;  // z cells per thread of the batched CG tiling
 class BatchBase {
    public:
     virtual ~BatchBase() = default;
+    virtual void autotune(int iters, double *us, int *form) = 0;
     virtual void upload(int inst, int field, const double *src, size_t n) = 0;
     virtual void download(int inst, int field, double *dst, size_t n) = 0;
     virtual void run(double dt, const double *times, int nsteps, aether_fluid_telemetry *out) = 0;
@@ -1958,8 +1959,89 @@ class Batch final : public BatchBase {
         case AETHER_CG_FUSED:
             raise(AETHER_EINVAL, "AETHER_CG_FUSED is not available for batches");
         default:
+            if (trial_form_) return trial_form_ == AETHER_CG_X4;
+            if (tuned_form_) return tuned_form_ == AETHER_CG_X4;
             return x4_ok_;
         }
+    }
+
+    // Times `iters` CG iterations of the batch in each layout (four-wide when
+    // every nx % 4 == 0, one cell per thread) on synthetic right-hand sides
+    // (every instance's pressure and divergence are saved and restored) and
+    // keeps the faster as the AETHER_CG_AUTO choice.  us[2]: time per
+    // iteration of the whole batch, four-wide then one cell per thread (-1
+    // when not applicable).
+    void autotune(int iters, double *us, int *form_out) override {
+        if (iters < 1) raise(AETHER_EINVAL, "iters must be >= 1");
+        for (auto &in : inst_) {  // finalize logs iteration 0's count
+            if (in->iters.n < 1) in->iters.alloc(1);
+            if (in->tel.n < 8) in->tel.alloc(8);
+        }
+        write_table();
+        std::vector<std::unique_ptr<DevBuf<T>>> saved;
+        for (auto &in : inst_) {
+            for (DevBuf<T> *src : {&in->p, &in->b}) {
+                saved.push_back(std::make_unique<DevBuf<T>>(in->n));
+                HIPT(hipMemcpyAsync(saved.back()->p, src->p, in->n * sizeof(T),
+                                    hipMemcpyDeviceToDevice, st_));
+            }
+            k_fill_hash<T><<<4096, kThreads, 0, st_>>>(in->b.p, (long)in->n);
+            LAUNCH_CHECK();
+        }
+        const aether_fluid_params keep = params_;
+        params_.cg_rel_tol = 0.0;  // absolute guards only: each layout runs `iters`
+        params_.solver_iters = iters;
+        hipEvent_t e0, e1;
+        HIPT(hipEventCreate(&e0));
+        HIPT(hipEventCreate(&e1));
+        const int forms[2] = {AETHER_CG_X4, AETHER_CG_SCALAR};
+        double best = 1e300;
+        int pick = AETHER_CG_SCALAR;
+        auto cleanup = [&] {
+            trial_form_ = 0;
+            params_ = keep;
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+        };
+        try {
+            for (int a = 0; a < 2; ++a) {
+                if (us) us[a] = -1.0;
+                if (forms[a] == AETHER_CG_X4 && !x4_ok_) continue;
+                trial_form_ = forms[a];
+                float ms = 0.0f;
+                for (int rep = 0; rep < 2; ++rep) {  // the first run warms up
+                    for (auto &in : inst_) fill(in->p.p, in->n);
+                    kb_div_cginit<T, kBatchZU><<<ncg_, dim3(kBX, kBY), 0, st_>>>(
+                        table_.p, cg_items_.p, sl_, 0);
+                    LAUNCH_CHECK();
+                    HIPT(hipEventRecord(e0, st_));
+                    cg_solve(0);
+                    HIPT(hipEventRecord(e1, st_));
+                    HIPT(hipEventSynchronize(e1));
+                    HIPT(hipEventElapsedTime(&ms, e0, e1));
+                }
+                const double t = 1e3 * (double)ms / iters;
+                if (us) us[a] = t;
+                if (t < best) {
+                    best = t;
+                    pick = forms[a];
+                }
+            }
+        } catch (...) {
+            cleanup();
+            throw;
+        }
+        cleanup();
+        tuned_form_ = pick;
+        size_t k = 0;
+        for (auto &in : inst_)
+            for (DevBuf<T> *dst : {&in->p, &in->b}) {
+                HIPT(hipMemcpyAsync(dst->p, saved[k]->p, in->n * sizeof(T),
+                                    hipMemcpyDeviceToDevice, st_));
+                ++k;
+            }
+        HIPT(hipStreamSynchronize(st_));
+        if (form_out) *form_out = pick;
     }
 
     void launch_cg4(int it, double rel2, int solve_id) {
@@ -2053,7 +2135,7 @@ class Batch final : public BatchBase {
         sl_.u ^= 1;
         sl_.v ^= 1;
         sl_.w ^= 1;
-        kb_div_cginit<T, kBatchZU><<<ncg_, blk, 0, st_>>>(table_.p, cg_items_.p, sl_);
+        kb_div_cginit<T, kBatchZU><<<ncg_, blk, 0, st_>>>(table_.p, cg_items_.p, sl_, 1);
         LAUNCH_CHECK();
         cg_solve(s, graph);
         kb_gradient_bc<T><<<ncell_, blk, 0, st_>>>(table_.p, cell_items_.p, sl_);
@@ -2174,6 +2256,8 @@ class Batch final : public BatchBase {
     DevBuf<int2> src_items_;
     int ncell_ = 0, ncg_ = 0, nsrc_ = 0, ncg4_ = 0;
     bool x4_ok_ = false;  // FP32 and every nx % 4 == 0: four-wide CG available
+    int tuned_form_ = 0;  // the autotuner's AETHER_CG_AUTO choice (0: none)
+    int trial_form_ = 0;  // layout forced while the autotuner times it
     DevBuf<int> step_dev_;          // device step index of captured steps
     DevBuf<double> times_dev_;      // their time axis
     bool wide_ = false;   // some instance has more than 1024 CG partials
@@ -2453,6 +2537,15 @@ int aether_fluid_max_density(aether_fluid_t s, double *out) {
         checked(s);
         if (!out) raise(AETHER_EINVAL, "null output pointer");
         *out = s->impl->max_density();
+    });
+}
+
+int aether_fluid_batch_autotune_cg(aether_fluid_batch_t b, int iters, double *us_per_iter,
+                                   int *form) {
+    return guarded([&] {
+        if (!b) raise(AETHER_EINVAL, "null batch handle");
+        HIPT(hipSetDevice(b->device));
+        b->impl->autotune(iters, us_per_iter, form);
     });
 }
 

@@ -260,6 +260,8 @@ def bench_batched(steps: int, warmup: int, dt: float = 0.01) -> dict:
     nvox = sum(d["nx"] * d["ny"] * d["nz"] for d in insts)
     times = [dt * i for i in range(warmup + steps)]
     b = af.FluidBatch(insts, solver_iters=200, cg_rel_tol=1e-5)
+    tune_b = b.autotune_cg(20)  # CG layout of the batch, chosen by timing
+    form_b = af.CG_X4 if tune_b["choice"] == "x4" else af.CG_SCALAR
     if warmup:
         b.run(dt, times[:warmup])
     t0 = time.perf_counter()
@@ -269,7 +271,7 @@ def bench_batched(steps: int, warmup: int, dt: float = 0.01) -> dict:
     rho_b = b.download(len(insts) - 1, "density")
     b.close()
     # the same batch with every step captured into a HIP graph and replayed
-    bg = af.FluidBatch(insts, solver_iters=200, cg_rel_tol=1e-5, graph=True)
+    bg = af.FluidBatch(insts, solver_iters=200, cg_rel_tol=1e-5, graph=True, cg_form=form_b)
     if warmup:
         bg.run(dt, times[:warmup])
     t0 = time.perf_counter()
@@ -280,12 +282,16 @@ def bench_batched(steps: int, warmup: int, dt: float = 0.01) -> dict:
     bg.close()
     el_1 = 0.0
     it_1 = []
+    single_choices = {}
     for d in insts:
         s = af.FluidSolver3D(d["nx"], d["ny"], d["nz"])
         s.vorticity_eps, s.buoyancy_beta, s.gravity_alpha = (d["vorticity_eps"], 1.0,
                                                              d["gravity_alpha"])
         s.set_sphere_source(d["source_center"], d["source_radius"])
         s.solver_iters, s.cg_rel_tol = 200, 1e-5
+        ch = s.autotune_cg(20)["choice"]  # each single grid tuned the same way
+        key = f'{ch["form"]}_z{ch["z_cells"]}'
+        single_choices[key] = single_choices.get(key, 0) + 1
         if warmup:
             s.run(dt, times[:warmup])
         t0 = time.perf_counter()
@@ -296,14 +302,15 @@ def bench_batched(steps: int, warmup: int, dt: float = 0.01) -> dict:
     return {"workload": "64 batched smoke grids, FP32, CG rel tol 1e-5", "instances": 64,
             "total_voxels": nvox, "steps": steps, "warmup": warmup,
             "batched": {"voxel_steps_per_s": nvox * steps / el_b, "ms_per_step": el_b * 1e3 / steps,
-                        "cg_iters_mean": float(np.mean(it_b))},
+                        "cg_iters_mean": float(np.mean(it_b)), "autotune": tune_b},
             "batched_graph": {"voxel_steps_per_s": nvox * steps / el_g,
                               "ms_per_step": el_g * 1e3 / steps,
                               "bitwise_equal_to_stream_launches": same,
                               "note": "includes capturing two step graphs per run"},
             "one_at_a_time": {"voxel_steps_per_s": nvox * steps / el_1,
                               "ms_per_step": el_1 * 1e3 / steps,
-                              "cg_iters_mean": float(np.mean(it_1))},
+                              "cg_iters_mean": float(np.mean(it_1)),
+                              "autotune_choices": single_choices},
             "speedup": el_1 / el_b}
 
 

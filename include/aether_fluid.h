@@ -239,6 +239,14 @@ int aether_fluid_batch_run(aether_fluid_batch_t b, double dt, const double *time
                            aether_fluid_telemetry *out);
 /* CG iterations of the last run, nsteps*count values, step-major. */
 int aether_fluid_batch_iterations(aether_fluid_batch_t b, int32_t *out, int n, int *count);
+/* CG layout autotuner of a batch: times `iters` CG iterations of the whole
+ * batch in the four-wide layout (FP32, every nx % 4 == 0) and in the
+ * one-cell-per-thread layout, on synthetic right-hand sides (every
+ * instance's pressure and divergence are saved and restored), and keeps the
+ * faster as the batch's AETHER_CG_AUTO choice.  us_per_iter[2] (may be
+ * NULL): four-wide, one cell per thread (-1 when not applicable). */
+int aether_fluid_batch_autotune_cg(aether_fluid_batch_t b, int iters, double *us_per_iter,
+                                   int *form);
 
 /* Differentiable rollouts (no counterpart in the reference).  Starting from
  * the solver's current state (which is left unchanged), run `nsteps` steps
