@@ -383,6 +383,7 @@ class Solver final : public SolverBase {
         const int npart = std::max(std::max(std::max(nb_, tlv_.nb), std::max(tl_.nb, tlf_.nb)),
                                    tl4_.nb);
         if (const char *e = std::getenv("AETHER_CG_CHUNK")) cg_chunk_ = std::max(0, std::atoi(e));
+        if (const char *e = std::getenv("AETHER_ADJ_WINDOW")) adj_window_ = std::atoi(e) != 0;
         HIPT(hipHostMalloc((void **)&done_host_, 64, hipHostMallocMapped));
         *done_host_ = 0;
         HIPT(hipHostGetDevicePointer((void **)&done_dev_, done_host_, 0));
@@ -1447,6 +1448,23 @@ class Solver final : public SolverBase {
                                                               fwd_[0].p, nullptr);
         LAUNCH_CHECK();
         zero(adj_lf_.p);
+        const dim3 grid((g_.nx + WTX - 1) / WTX, (g_.ny + WTY - 1) / WTY,
+                        (g_.nz + WTZ - 1) / WTZ);
+        // LDS-windowed scatters (adjoint_kernels.hpp) once the grid has a tile
+        // per CU (256); smaller grids keep the one-plane launches, which
+        // spread over more workgroups (measured: 128^3 fwd+adj/fwd 3.23 ->
+        // 2.27 windowed; 64^3 2.74 direct, 2.83 windowed)
+        if (adj_window_ && (long)grid.x * grid.y * grid.z >= 256) {
+            const dim3 blk(WTX, WTY);
+            k_adj_mac_correct_w<T><<<grid, blk, 0, st_>>>(g_, s, (T)diss, clamp0, src,
+                                                          fwd_[0].p, U, V, W, lam_out, lam_src,
+                                                          adj_lf_.p, lam_U, lam_V, lam_W);
+            LAUNCH_CHECK();
+            k_adj_sl_forward_w<T><<<grid, blk, 0, st_>>>(g_, s, src, U, V, W, adj_lf_.p,
+                                                         lam_src, lam_U, lam_V, lam_W);
+            LAUNCH_CHECK();
+            return;
+        }
         k_adj_mac_correct<T><<<cell_grid_, cell_blk_, 0, st_>>>(
             g_, s, (T)diss, clamp0, src, fwd_[0].p, U, V, W, lam_out, lam_src, adj_lf_.p, lam_U,
             lam_V, lam_W);
@@ -1634,6 +1652,7 @@ class Solver final : public SolverBase {
     int env_form_ = 0;    // AETHER_CG_FORM override of AETHER_CG_AUTO (0: none)
     int tuned_form_ = 0;  // the autotuner's AETHER_CG_AUTO choice (0: none)
     int trial_form_ = 0;  // layout forced while the autotuner times a candidate
+    bool adj_window_ = true;  // LDS-windowed adjoint scatters (AETHER_ADJ_WINDOW=0: direct)
     bool remap_ok_ = true;  // XCD-contiguous tile order allowed (AETHER_XCD_REMAP)
     int form_ = AETHER_CG_SCALAR;  // layout of the solve being enqueued
     VTiling tlv_ = {0, 0, 0, 0, 0, 1, 1};  // four-wide CG tiling (nx % 4 == 0)
