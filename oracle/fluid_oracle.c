@@ -4,6 +4,9 @@
  * A plain-C, f64 restatement of the reference smoke solver `FluidSolver3D`
  * (src/fluid.rs:23-673) and of its seeded RNG (src/rng.rs:5-88).  It is the
  * parity checker for the HIP library and the CPU baseline timed by bench.py.
+ * of_soundtrack restates the telemetry-driven audio synthesis and 24-bit
+ * dithered encoder (src/audio.rs:1-639) so that telemetry can be checked
+ * against the reference's rendered soundtrack; nothing else uses it.
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
  * load it; the product library never links or calls it.
  *
@@ -745,3 +748,412 @@ double orng_range_f64(ORng *r, double lo, double hi) { /* rng.rs:86-88 */
 }
 
 uint64_t of_splitmix64_next(uint64_t *state) { return splitmix64(state); }
+
+/* ------------------------------------------------------------------ */
+/* Telemetry -> soundtrack (audio.rs:1-639), restated to pin the whole  */
+/* step() + compute_telemetry chain against the reference's rendered    */
+/* renders/simulation_soundtrack.wav (main.rs:82-94, 166-168).          */
+/* ------------------------------------------------------------------ */
+#define AU_SR 48000
+#define AU_FRAMES (AU_SR * 15)
+#define AU_TAU TAU_                                      /* std::f64::consts::TAU */
+#define AU_PI 3.14159265358979323846264338327950288       /* std::f64::consts::PI */
+#define AU_FRAC_1_SQRT_2 0.707106781186547524400844362104849039
+
+typedef struct { double ic1, ic2; } AuSvf2;
+typedef struct { AuSvf2 s1, s2; } AuSvf4;
+
+/* audio.rs:27-47 TptSvf2Pole::process */
+static void au_svf2(AuSvf2 *s, double in, double cutoff_hz, double res, double fs, double *lp,
+                    double *bp) {
+    double fc = rclamp(cutoff_hz, 20.0, 0.47 * fs);
+    double g = tan(AU_PI * fc / fs);
+    double k = 2.0 * (1.0 - rclamp(res, 0.0, 0.97));
+    double a1 = 1.0 / (1.0 + g * (g + k));
+    double a2 = g * a1;
+    double a3 = g * a2;
+    double v3 = in - s->ic2;
+    double v1 = a1 * s->ic1 + a2 * v3;
+    double v2 = s->ic2 + a2 * s->ic1 + a3 * v3;
+    s->ic1 = 2.0 * v1 - s->ic1;
+    s->ic2 = 2.0 * v2 - s->ic2;
+    *lp = v2;
+    *bp = v1;
+}
+
+/* audio.rs:59-65 */
+static double au_lowpass4(AuSvf4 *s, double in, double fc, double res, double fs) {
+    double lp1, bp1, lp2, bp2;
+    au_svf2(&s->s1, in, fc, res * 0.88, fs, &lp1, &bp1);
+    double driven = tanh(lp1 * 1.08) * (1.0 / 1.08);
+    au_svf2(&s->s2, driven, fc, res, fs, &lp2, &bp2);
+    return lp2;
+}
+
+/* audio.rs:67-72 */
+static double au_bandpass4(AuSvf4 *s, double in, double fc, double res, double fs) {
+    double lp1, bp1, lp2, bp2;
+    au_svf2(&s->s1, in, fc, res * 0.85, fs, &lp1, &bp1);
+    au_svf2(&s->s2, bp1, fc, res, fs, &lp2, &bp2);
+    return bp2;
+}
+
+/* audio.rs:82-111 */
+static void au_hadamard8(double *x) {
+    for (int i = 0; i < 8; i += 2) {
+        double a = x[i], b = x[i + 1];
+        x[i] = a + b;
+        x[i + 1] = a - b;
+    }
+    for (int base = 0; base < 8; base += 4)
+        for (int j = 0; j < 2; ++j) {
+            double a = x[base + j], b = x[base + j + 2];
+            x[base + j] = a + b;
+            x[base + j + 2] = a - b;
+        }
+    for (int j = 0; j < 4; ++j) {
+        double a = x[j], b = x[j + 4];
+        x[j] = a + b;
+        x[j + 4] = a - b;
+    }
+    const double inv_sqrt8 = 0.5 * AU_FRAC_1_SQRT_2;
+    for (int i = 0; i < 8; ++i) x[i] *= inv_sqrt8;
+}
+
+typedef struct {
+    double *buf[8];
+    size_t len[8], base[8], wp[8];
+    double damp_state[8], gain_dc[8], damp_coeff[8], lfo_phase[8], lfo_inc[8];
+} AuFdn;
+
+/* audio.rs:125-159 HadamardFdn8::new */
+static void au_fdn_init(AuFdn *f, double fs, double rt60_low, double rt60_high) {
+    static const size_t base[8] = {1499, 1877, 2267, 2683, 3119, 3583, 4079, 4637};
+    for (int i = 0; i < 8; ++i) {
+        double d_sec = (double)base[i] / fs;
+        double g_low = pow(10.0, -3.0 * d_sec / fmax(rt60_low, 0.2));
+        double g_high = pow(10.0, -3.0 * d_sec / fmax(rt60_high, 0.1));
+        f->gain_dc[i] = fmin(g_low, 0.992);
+        f->damp_coeff[i] = rclamp(1.0 - (g_high / fmax(g_low, 1e-6)), 0.08, 0.72);
+        f->lfo_phase[i] = (double)i * (AU_TAU / 8.0);
+        f->lfo_inc[i] = AU_TAU * (0.21 + 0.07 * (double)i) / fs;
+        f->base[i] = base[i];
+        f->len[i] = base[i] + 64;
+        f->buf[i] = (double *)calloc(f->len[i], sizeof(double));
+        f->wp[i] = 0;
+        f->damp_state[i] = 0.0;
+    }
+}
+
+static void au_fdn_free(AuFdn *f) {
+    for (int i = 0; i < 8; ++i) free(f->buf[i]);
+}
+
+/* audio.rs:161-215 HadamardFdn8::process_stereo */
+static void au_fdn_process(AuFdn *f, double in_l, double in_r, double *wet_l, double *wet_r) {
+    double delayed[8];
+    for (int i = 0; i < 8; ++i) {
+        const double *buf = f->buf[i];
+        size_t len = f->len[i];
+        double mod_samples = 6.5 * sin(f->lfo_phase[i]);
+        f->lfo_phase[i] += f->lfo_inc[i];
+        if (f->lfo_phase[i] >= AU_TAU) f->lfo_phase[i] -= AU_TAU;
+        double read_offset = (double)f->base[i] + mod_samples;
+        double read_f = (double)f->wp[i] + (double)len - read_offset;
+        size_t r0 = ((size_t)floor(read_f)) % len;
+        size_t r1 = (r0 + 1) % len;
+        double frac = read_f - floor(read_f);
+        double raw = buf[r0] * (1.0 - frac) + buf[r1] * frac;
+        double lp = raw * (1.0 - f->damp_coeff[i]) + f->damp_state[i] * f->damp_coeff[i];
+        f->damp_state[i] = lp;
+        delayed[i] = lp * f->gain_dc[i];
+    }
+    double mixed[8];
+    memcpy(mixed, delayed, sizeof(mixed));
+    au_hadamard8(mixed);
+    double inputs[8] = {in_l,          in_r,        0.7 * (in_l + in_r), 0.7 * (in_l - in_r),
+                        -in_l * 0.8,   in_r * 0.8,  -0.6 * (in_l - in_r), 0.6 * (in_l + in_r)};
+    for (int i = 0; i < 8; ++i) {
+        size_t wp = f->wp[i];
+        f->buf[i][wp] = mixed[i] + inputs[i] * 0.35;
+        f->wp[i] = (wp + 1) % f->len[i];
+    }
+    *wet_l = 0.38 * (delayed[0] - delayed[2] + delayed[4] - delayed[6]);
+    *wet_r = 0.38 * (delayed[1] - delayed[3] + delayed[5] - delayed[7]);
+}
+
+typedef struct {
+    int active;
+    double freq_hz, age_sec, duration_sec, velocity, pan;
+    double phase_c1, phase_m1, phase_c2, phase_m2, prev_m1;
+} AuVoice;
+
+/* audio.rs:241-257 (Default) */
+static void au_voice_default(AuVoice *v) {
+    memset(v, 0, sizeof(*v));
+    v->freq_hz = 440.0;
+    v->duration_sec = 3.5;
+}
+
+/* audio.rs:260-272 */
+static void au_voice_trigger(AuVoice *v, double freq, double vel, double pan, double dur) {
+    v->active = 1;
+    v->freq_hz = freq;
+    v->age_sec = 0.0;
+    v->duration_sec = dur;
+    v->velocity = vel;
+    v->pan = rclamp(pan, -0.95, 0.95);
+    v->phase_c1 = v->phase_m1 = v->phase_c2 = v->phase_m2 = v->prev_m1 = 0.0;
+}
+
+/* audio.rs:274-316 */
+static void au_voice_render(AuVoice *v, double dt, double ke, double *l, double *r) {
+    *l = *r = 0.0;
+    if (!v->active) return;
+    double t = v->age_sec;
+    if (t >= v->duration_sec) {
+        v->active = 0;
+        return;
+    }
+    double attack = fmin(t / 0.0025, 1.0);
+    double env_body = attack * exp(-t * 1.45);
+    double env_mod1 = exp(-t * 4.8);
+    double env_chime = attack * exp(-t * 3.1);
+    double env_mod2 = exp(-t * 11.5);
+    double tail_gate = rclamp((v->duration_sec - t) / 0.06, 0.0, 1.0);
+    double idx1 = (1.35 + 2.45 * ke) * env_mod1;
+    double idx2 = (0.85 + 2.10 * ke) * env_mod2;
+    double m1 = sin(v->phase_m1 + 0.22 * v->prev_m1);
+    v->prev_m1 = m1;
+    double c1 = sin(v->phase_c1 + idx1 * m1) * env_body;
+    double m2 = sin(v->phase_m2);
+    double c2 = sin(v->phase_c2 + idx2 * m2) * env_chime * 0.36;
+    v->phase_c1 = fmod(v->phase_c1 + AU_TAU * v->freq_hz * dt, AU_TAU);
+    v->phase_m1 = fmod(v->phase_m1 + AU_TAU * (v->freq_hz * 2.0007) * dt, AU_TAU);
+    v->phase_c2 = fmod(v->phase_c2 + AU_TAU * (v->freq_hz * 2.0015) * dt, AU_TAU);
+    v->phase_m2 = fmod(v->phase_m2 + AU_TAU * (v->freq_hz * 5.4142) * dt, AU_TAU);
+    v->age_sec += dt;
+    double mono = (c1 + c2) * v->velocity * tail_gate;
+    double angle = (v->pan + 1.0) * (AU_PI * 0.25);
+    *l = mono * cos(angle);
+    *r = mono * sin(angle);
+}
+
+typedef struct { double buf[128]; size_t wp; } AuMicroDelay;
+
+/* audio.rs:319-338 */
+static double au_micro_delay(AuMicroDelay *m, double x, double delay_samples) {
+    m->buf[m->wp] = x;
+    double d = rclamp(delay_samples, 0.0, 64.0);
+    double rf = (double)m->wp + 128.0 - d;
+    size_t r0 = ((size_t)floor(rf)) & 127;
+    size_t r1 = (r0 + 1) & 127;
+    double frac = rf - floor(rf);
+    m->wp = (m->wp + 1) & 127;
+    return m->buf[r0] * (1.0 - frac) + m->buf[r1] * frac;
+}
+
+static double au_midi_to_hz(double midi) { return 440.0 * pow(2.0, (midi - 69.0) / 12.0); }
+
+/* audio.rs:344-355 */
+static double au_poly_blep(double t, double dt) {
+    if (t < dt) {
+        double x = t / dt;
+        return x + x - x * x - 1.0;
+    } else if (t > 1.0 - dt) {
+        double x = (t - 1.0) / dt;
+        return x * x + x + x + 1.0;
+    }
+    return 0.0;
+}
+
+static double au_fract(double x) { return x - trunc(x); }
+
+/* audio.rs:358-394; frames hold 8 values per step (of_telemetry order) */
+static void au_curves(const double *fr, long n, double t_sec, double max_ke, double max_vort,
+                      double *ke_n, double *vort_n, double *cx_o, double *cy_o) {
+    if (n == 1) {
+        *ke_n = rclamp(fr[1] / fmax(max_ke, 1e-9), 0.0, 1.0);
+        *vort_n = rclamp(fr[3] / fmax(max_vort, 1e-9), 0.0, 1.0);
+        *cx_o = rclamp(fr[5], -1.0, 1.0);
+        *cy_o = rclamp(0.5 * (fr[6] + 1.0), 0.0, 1.0);
+        return;
+    }
+    double u = rclamp(t_sec / 15.0, 0.0, 1.0) * (double)(n - 1);
+    long i0 = (long)floor(u);
+    if (i0 > n - 1) i0 = n - 1;
+    long i1 = i0 + 1 < n - 1 ? i0 + 1 : n - 1;
+    double frac = u - (double)i0;
+    const double *a = fr + 8 * i0, *b = fr + 8 * i1;
+    double ke = a[1] * (1.0 - frac) + b[1] * frac;
+    double vort = a[3] * (1.0 - frac) + b[3] * frac;
+    double cx = a[5] * (1.0 - frac) + b[5] * frac;
+    double cy = a[6] * (1.0 - frac) + b[6] * frac;
+    *ke_n = rclamp(ke / fmax(max_ke, 1e-9), 0.0, 1.0);
+    *vort_n = rclamp(vort / fmax(max_vort, 1e-9), 0.0, 1.0);
+    *cx_o = rclamp(cx, -1.0, 1.0);
+    *cy_o = rclamp(0.5 * (cy + 1.0), 0.0, 1.0);
+}
+
+/* audio.rs:397-560 synthesize_fluid_soundtrack, then the TPDF-dithered
+ * 24-bit quantisation of audio.rs:593-604: pcm receives 2 * nsamples values
+ * (left, right interleaved; nsamples <= 720000).  max_ke / max_vort <= 0:
+ * the normalisers of audio.rs:401-406 (the maxima over the frames); > 0:
+ * used as given. */
+void of_soundtrack_ex(const double *frames, long n, double max_ke_in, double max_vort_in,
+                      long nsamples, int32_t *pcm) {
+    const double fs = (double)AU_SR, dt = 1.0 / fs;
+    double max_ke = 1e-6, max_vort = 1e-6;
+    for (long a = 0; a < n; ++a) {
+        max_ke = fmax(max_ke, frames[8 * a + 1]);
+        max_vort = fmax(max_vort, frames[8 * a + 3]);
+    }
+    if (max_ke_in > 0.0) max_ke = max_ke_in;
+    if (max_vort_in > 0.0) max_vort = max_vort_in;
+    if (nsamples > AU_FRAMES) nsamples = AU_FRAMES;
+    AuVoice voices[16];
+    for (int v = 0; v < 16; ++v) au_voice_default(&voices[v]);
+    AuSvf4 pad_l_f = {{0, 0}, {0, 0}}, pad_r_f = {{0, 0}, {0, 0}};
+    AuSvf4 br_l_f = {{0, 0}, {0, 0}}, br_r_f = {{0, 0}, {0, 0}};
+    AuFdn fdn;
+    au_fdn_init(&fdn, fs, 3.85, 1.35);
+    AuMicroDelay itd_l, itd_r;
+    memset(&itd_l, 0, sizeof(itd_l));
+    memset(&itd_r, 0, sizeof(itd_r));
+    ORng rng;
+    orng_seed(&rng, 0xAE74E9C02026ull);
+    static const double chord_pads[4][4] = {
+        {38.0, 57.0, 60.0, 64.0}, {34.0, 53.0, 57.0, 64.0}, {31.0, 50.0, 58.0, 62.0},
+        {33.0, 52.0, 57.0, 64.0}};
+    static const double bell_scales[4][8] = {
+        {62.0, 64.0, 65.0, 69.0, 72.0, 76.0, 79.0, 81.0},
+        {58.0, 62.0, 64.0, 65.0, 69.0, 72.0, 76.0, 77.0},
+        {55.0, 58.0, 62.0, 65.0, 69.0, 70.0, 74.0, 79.0},
+        {57.0, 62.0, 64.0, 67.0, 69.0, 74.0, 76.0, 81.0}};
+    static const size_t pattern[8] = {0, 3, 4, 2, 6, 4, 5, 7};
+    double pad_phases[4][3] = {{0}};
+    const double detunes[3] = {0.9955, 1.0, 1.0048};
+    double sub_phase = 0.0;
+    double smooth_ke = 0.25, smooth_vort = 0.25, prev_ke = 0.25;
+    size_t next_grid_step = 0, step_counter = 0;
+    double flux_cooldown = 0.0;
+    const size_t step_interval = (size_t)AU_SR * 3 / 16;
+    ORng dither;
+    orng_seed(&dither, 0x24B17DA7A202ull);
+    for (size_t fi = 0; fi < (size_t)nsamples; ++fi) {
+        double t_sec = (double)fi * dt;
+        size_t chord_idx = (size_t)floor(t_sec / 3.75);
+        if (chord_idx > 3) chord_idx = 3;
+        double raw_ke, raw_vort, com_x, com_y;
+        au_curves(frames, n, t_sec, max_ke, max_vort, &raw_ke, &raw_vort, &com_x, &com_y);
+        smooth_ke += 0.0015 * (raw_ke - smooth_ke);
+        smooth_vort += 0.0015 * (raw_vort - smooth_vort);
+        double ke_flux = (smooth_ke - prev_ke) * fs;
+        prev_ke = smooth_ke;
+        if (flux_cooldown > 0.0) flux_cooldown -= dt;
+        int grid_hit = fi >= next_grid_step && t_sec < 14.15;
+        int flux_hit = ke_flux > 0.22 && flux_cooldown <= 0.0 && t_sec > 0.4 && t_sec < 14.0;
+        if (grid_hit || flux_hit) {
+            if (grid_hit) next_grid_step += step_interval;
+            if (flux_hit) flux_cooldown = 0.14;
+            const double *scale = bell_scales[chord_idx];
+            size_t pattern_offset = pattern[step_counter & 7];
+            size_t fluid_offset = (size_t)round(com_y * 3.0);
+            if (fluid_offset > 3) fluid_offset = 3;
+            size_t degree = (pattern_offset + fluid_offset) & 7;
+            double midi = scale[degree];
+            double vel = fmin(0.16 + 0.18 * smooth_ke + (flux_hit ? 0.06 : 0.0), 0.36);
+            double spread = (step_counter & 1) == 0 ? -0.45 : 0.45;
+            double pan = rclamp(0.65 * com_x * 3.0 + spread, -0.92, 0.92);
+            size_t slot = 0;
+            double best_age = -1.0;
+            for (size_t vi = 0; vi < 16; ++vi) {
+                if (!voices[vi].active) {
+                    slot = vi;
+                    break;
+                }
+                if (voices[vi].age_sec > best_age) {
+                    best_age = voices[vi].age_sec;
+                    slot = vi;
+                }
+            }
+            au_voice_trigger(&voices[slot], au_midi_to_hz(midi), vel, pan, 3.1 + 1.2 * smooth_ke);
+            step_counter += 1;
+        }
+        double bells_l = 0.0, bells_r = 0.0;
+        for (int v = 0; v < 16; ++v) {
+            double vl, vr;
+            au_voice_render(&voices[v], dt, smooth_ke, &vl, &vr);
+            bells_l += vl;
+            bells_r += vr;
+        }
+        const double *chord = chord_pads[chord_idx];
+        double pad_raw_l = 0.0, pad_raw_r = 0.0;
+        for (int ni = 0; ni < 4; ++ni) {
+            double base_f = au_midi_to_hz(chord[ni]);
+            for (int di = 0; di < 3; ++di) {
+                double f = base_f * detunes[di];
+                double p_inc = f * dt;
+                double ph = pad_phases[ni][di];
+                double saw = 2.0 * ph - 1.0 - au_poly_blep(ph, p_inc);
+                pad_phases[ni][di] = au_fract(ph + p_inc);
+                if (di == 0) {
+                    pad_raw_l += saw * 0.095;
+                    pad_raw_r += saw * 0.035;
+                } else if (di == 2) {
+                    pad_raw_l += saw * 0.035;
+                    pad_raw_r += saw * 0.095;
+                } else {
+                    pad_raw_l += saw * 0.065;
+                    pad_raw_r += saw * 0.065;
+                }
+            }
+        }
+        double sub_f = au_midi_to_hz(chord[0]);
+        double sub_sig = sin(AU_TAU * sub_phase) * (0.19 + 0.08 * smooth_ke);
+        sub_phase = au_fract(sub_phase + sub_f * dt);
+        double lfo_breath = 0.5 + 0.5 * sin(AU_TAU * 0.18 * t_sec);
+        double cutoff_hz = 185.0 * pow(2.0, 3.9 * smooth_ke + 0.55 * lfo_breath);
+        double resonance = rclamp(0.24 + 0.48 * smooth_vort, 0.15, 0.78);
+        double pad_l = au_lowpass4(&pad_l_f, pad_raw_l, cutoff_hz, resonance, fs) + sub_sig;
+        double pad_r = au_lowpass4(&pad_r_f, pad_raw_r, cutoff_hz * 1.03, resonance, fs) + sub_sig;
+        double white_l = 2.0 * orng_next_f64(&rng) - 1.0;
+        double white_r = 2.0 * orng_next_f64(&rng) - 1.0;
+        double air_fc = 620.0 + 2400.0 * com_y + 1200.0 * smooth_vort;
+        double air_amp = 0.022 * smooth_vort;
+        double air_l = au_bandpass4(&br_l_f, white_l, air_fc, 0.68, fs) * air_amp;
+        double air_r = au_bandpass4(&br_r_f, white_r, air_fc * 1.06, 0.68, fs) * air_amp;
+        double azimuth = rclamp(com_x * 2.5, -1.0, 1.0);
+        double delay_l = azimuth > 0.0 ? azimuth * 24.0 : 0.0;
+        double delay_r = azimuth < 0.0 ? -azimuth * 24.0 : 0.0;
+        double dry_l = au_micro_delay(&itd_l, bells_l + air_l, delay_l) + pad_l * 0.78;
+        double dry_r = au_micro_delay(&itd_r, bells_r + air_r, delay_r) + pad_r * 0.78;
+        double rev_l = bells_l * 0.72 + pad_l * 0.42 + air_l * 0.85;
+        double rev_r = bells_r * 0.72 + pad_r * 0.42 + air_r * 0.85;
+        double wet_l, wet_r;
+        au_fdn_process(&fdn, rev_l, rev_r, &wet_l, &wet_r);
+        double fade_in = rclamp(t_sec / 0.08, 0.0, 1.0);
+        double fade_out = 1.0;
+        if (t_sec > 14.35) {
+            double u = rclamp((15.0 - t_sec) / 0.65, 0.0, 1.0);
+            fade_out = 0.5 - 0.5 * cos(AU_PI * u);
+        }
+        double master = fade_in * fade_out;
+        double mix[2];
+        mix[0] = tanh((dry_l * 0.74 + wet_l * 0.52) * master * 0.92) * 0.91;
+        mix[1] = tanh((dry_r * 0.74 + wet_r * 0.52) * master * 0.92) * 0.91;
+        for (int c = 0; c < 2; ++c) { /* audio.rs:593-604 */
+            double a = orng_next_f64(&dither);
+            double b = orng_next_f64(&dither);
+            double tpdf = (a - b) * 0.5;
+            double v = rclamp(round(rclamp(mix[c], -1.0, 1.0) * 8388607.0 + tpdf), -8388608.0,
+                              8388607.0);
+            pcm[2 * fi + c] = (int32_t)v;
+        }
+    }
+    au_fdn_free(&fdn);
+}
+
+void of_soundtrack(const double *frames, long n, int32_t *pcm) {
+    of_soundtrack_ex(frames, n, 0.0, 0.0, AU_FRAMES, pcm);
+}

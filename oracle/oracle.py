@@ -52,6 +52,8 @@ def _load(omp: bool = False) -> C.CDLL:
         "of_stats": (None, [vp, C.POINTER(l), pd, pd, pd, C.POINTER(i)]),
         "of_sample_trilinear": (d, [vp, i, d, d, d]),
         "of_max_density": (d, [vp]),
+        "of_soundtrack": (None, [pd, l, C.POINTER(C.c_int32)]),
+        "of_soundtrack_ex": (None, [pd, l, d, d, l, C.POINTER(C.c_int32)]),
         "of_medium_sample": (None, [vp, pd, pd, d, d, d, pd, pd]),
         "of_advect": (None, [vp, i, d, d]),
         "of_enforce_bc": (None, [vp]),
@@ -84,6 +86,21 @@ def _load(omp: bool = False) -> C.CDLL:
 def _ptr(a: np.ndarray):
     assert a.dtype == np.float64 and a.flags.c_contiguous
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def soundtrack(frames, max_ke: float = 0.0, max_vort: float = 0.0,
+               nsamples: int = 720000) -> np.ndarray:
+    """audio.rs:397-604: the 15 s, 48 kHz stereo soundtrack of a telemetry
+    sequence (rows of 8: time, KE, enstrophy, peak |w|, mass, COM xyz),
+    quantised to 24-bit PCM with the TPDF dither; shape (nsamples, 2).
+    max_ke / max_vort > 0 replace the normalisers (the maxima over the
+    frames)."""
+    lib = _load()
+    fr = np.ascontiguousarray(np.asarray(frames, dtype=np.float64).reshape(-1, 8))
+    out = np.empty(2 * nsamples, dtype=np.int32)
+    lib.of_soundtrack_ex(fr.ctypes.data_as(C.POINTER(C.c_double)), len(fr), float(max_ke),
+                         float(max_vort), nsamples, out.ctypes.data_as(C.POINTER(C.c_int32)))
+    return out.reshape(-1, 2)
 
 
 class OracleRng:

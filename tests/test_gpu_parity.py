@@ -214,6 +214,44 @@ def test_cli_selfcheck_and_fluid_stage(capsys):
     assert "[1/4] 3D Navier-Stokes Solver (36x52x36, 12 steps)" in st["line"]
 
 
+def _soundtrack_head():
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                        "soundtrack_head.npz")
+    with np.load(path) as z:
+        return {k: z[k] for k in z.files}
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_steps_reproduce_reference_soundtrack_head(precision):
+    """renders/simulation_soundtrack.wav, first second, driven by the GPU's
+    own telemetry: the first 9 steps of the main.rs:82-94 run (36x52x36,
+    dt 0.08, eps 6.5, 28 CG iterations) through the audio.rs:397-639
+    restatement with the two fitted normalisers. FP64: every stored sample
+    within 1 LSB of 24-bit. FP32: telemetry stays within ~1e-6 relative
+    through frame 4, so the first 0.5 s is within 16 LSB; from frame 4 on the
+    flow amplifies any difference ~20x per step (FP64's ulp-level
+    differences grow at the same rate, 1e-15 -> 1e-11 by frame 8), so the
+    rest of the second only gets a loose 2^14 LSB bound."""
+    _announce(f"{precision} main.rs run 36x52x36 x9 -> soundtrack head")
+    import aether_fluid as af
+    from oracle.oracle import soundtrack
+    fx = _soundtrack_head()
+    s = af.FluidSolver3D(36, 52, 36, precision=precision)
+    s.vorticity_eps, s.solver_iters = 6.5, 28
+    fr = np.array([f.as_array() for f in s.run(0.08, [15.0 * (i / 119) for i in range(9)])])
+    fr = np.concatenate([fr, np.repeat(fr[-1:], 111, 0)])
+    n, st = int(fx["nsamples"]), int(fx["stride"])
+    pcm = soundtrack(fr, fx["norm"][0], fx["norm"][1], n)[::st].astype(np.int64)
+    d = np.abs(pcm - fx["pcm_head"].astype(np.int64))
+    print(f"[{precision}] max |diff| {d.max()} LSB, exact {np.mean(d == 0):.5f}", flush=True)
+    if precision == "fp64":
+        assert d.max() <= 1
+    else:
+        assert d[: 24000 // st].max() <= 16
+        assert d.max() <= 1 << 14
+
+
 def _plain_pair(name, dims, precision):
     from aether_fluid.scenarios import SMOKE_CONFIGS, seeded_config_state
     cfg = SMOKE_CONFIGS[name]

@@ -4,14 +4,16 @@ Each test checks the oracle against something it does not restate: the
 reference's own known answer and ratio tests, published RNG vectors, or a
 property fixed by the mathematics (exactness on affine fields, integer
 shifts, an independently assembled graph Laplacian, a dense least-squares
-solve, the post-projection residual identity, closed-form telemetry).
+solve, the post-projection residual identity, closed-form telemetry), or
+the first second of the reference's rendered soundtrack, which the first
+nine steps of its default run drive.
 """
 import math
 
 import numpy as np
 import pytest
 
-from oracle.oracle import OracleFluid, OracleRng, splitmix64_stream
+from oracle.oracle import OracleFluid, OracleRng, soundtrack, splitmix64_stream
 
 
 def _interior(nx, ny, nz):
@@ -424,3 +426,63 @@ def test_telemetry_solid_body_rotation_enstrophy():
     interior_n = (nx - 2) * (ny - 2) * (nz - 2)
     assert math.isclose(t[3], 2 * c, rel_tol=1e-12)
     assert math.isclose(t[2], 0.5 * (2 * c) ** 2 * h ** 3 * interior_n, rel_tol=1e-12)
+
+
+# ------------------------------------------------------- soundtrack head pin
+def _soundtrack_fixture():
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                        "soundtrack_head.npz")
+    with np.load(path) as z:
+        return {k: z[k] for k in z.files}
+
+
+def _main_rs_frames(eps=6.5, iters=28, beta=None, nframes=11):
+    """Telemetry of the first steps of the main.rs:82-94 run (36x52x36,
+    dt 0.08, timeline 15 * i / 119), padded to 120 rows with the last one."""
+    s = OracleFluid(36, 52, 36)
+    s.vorticity_eps, s.solver_iters = eps, iters
+    if beta is not None:
+        s.buoyancy_beta = beta
+    fr = np.array([s.step(0.08, 15.0 * (i / 119)) for i in range(nframes)])
+    return np.concatenate([fr, np.repeat(fr[-1:], 120 - nframes, 0)])
+
+
+def _head_diff(frames, fx):
+    n, st = int(fx["nsamples"]), int(fx["stride"])
+    pcm = soundtrack(frames, fx["norm"][0], fx["norm"][1], n)[::st].astype(np.int64)
+    return pcm - fx["pcm_head"].astype(np.int64)
+
+
+def test_soundtrack_head_reproduces_reference_wav():
+    """renders/simulation_soundtrack.wav, first second: the oracle's first
+    steps of the default run (frames 0-8 are read), through the audio.rs:397-639 restatement
+    and the two fitted normalisers, give every stored sample within 1 LSB of
+    24-bit (platform libm rounding at the dither quantiser)."""
+    fx = _soundtrack_fixture()
+    d = _head_diff(_main_rs_frames(), fx)
+    assert np.abs(d).max() <= 1
+    assert np.mean(d == 0) > 0.999
+
+
+@pytest.mark.parametrize("knob", [dict(iters=27), dict(eps=6.45), dict(beta=6.79)])
+def test_soundtrack_head_catches_planted_mistakes(knob):
+    """One CG iteration fewer, or a 1% / 0.15% change of the confinement or
+    buoyancy coefficient, moves the first second by far more than 1 LSB."""
+    fx = _soundtrack_fixture()
+    d = _head_diff(_main_rs_frames(**knob), fx)
+    assert np.abs(d).max() > 10000
+
+
+def test_soundtrack_first_second_reads_frames_0_to_8_only():
+    """audio.rs:375-379: sample t reads frames floor(t / 15 * 119) and the
+    next, so for t < 1 s (u < 7.94) frames past 8 cannot reach the first
+    second; the synthesis is deterministic (fixed dither seed, audio.rs:623)."""
+    fr = _main_rs_frames()
+    alt = fr.copy()
+    alt[9:, 1:] *= 3.0
+    a = soundtrack(fr, 0.06, 190.0, 48000)
+    np.testing.assert_array_equal(a, soundtrack(alt, 0.06, 190.0, 48000))
+    np.testing.assert_array_equal(a, soundtrack(fr, 0.06, 190.0, 48000))
+    alt[8, 1] *= 1.01
+    assert not np.array_equal(a, soundtrack(alt, 0.06, 190.0, 48000))
