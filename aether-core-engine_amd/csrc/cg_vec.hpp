@@ -306,14 +306,16 @@ __device__ __forceinline__ void body_cg_update4(
     if (threadIdx.x == 0) part_rr[slot] = acc[0];
 }
 
+// nb_in: the number of d.q partials to sum (vt.nb, or 1 when a separate
+// kernel has already reduced them to one total).
 template <int ZU, int NQ, bool NT>
 __global__ __launch_bounds__(kThreads) void k_cg_update4(
-    Dims g, VTiling vt, int it, const double *__restrict__ part_dq, CgState *st,
+    Dims g, VTiling vt, int it, int nb_in, const double *__restrict__ part_dq, CgState *st,
     const float *__restrict__ d, float *__restrict__ p, float *__restrict__ r,
     double *__restrict__ part_rr, int *done_mirror, int solve_id) {
     int bx, by, bz;
     vtile_coords(vt, bx, by, bz);
-    body_cg_update4<ZU, NQ, NT>(vcol_at<ZU>(g, vt, bx, by, bz), blockIdx.x, vt.nb, g, it,
+    body_cg_update4<ZU, NQ, NT>(vcol_at<ZU>(g, vt, bx, by, bz), blockIdx.x, nb_in, g, it,
                                 part_dq, st, d, p, r, part_rr, done_mirror, solve_id);
 }
 

@@ -405,6 +405,7 @@ class Solver final : public SolverBase {
         partB_.alloc(npart);
         partC_.alloc(npart);
         partD64_.alloc(npart);
+        tot_.alloc(4);
         setup_resident(gx, gy, nz);
         part_tel_.alloc((size_t)nb_ * kTelParts);
         scalar_.alloc(8);
@@ -865,12 +866,27 @@ class Solver final : public SolverBase {
             const T *r = r_.p, *dold = d_[(it + 1) & 1].p, *dcur = d_[it & 1].p;
             T *dnew = d_[it & 1].p, *pp = p_.p, *rw = r_.p;
             int *dm = done_dev_;
+            // Past 1024 partials, the direction kernel's workgroups re-reading
+            // all of them costs more than one extra launch (at 256^3, 4096 x
+            // 32 KB of L2 reads: ~12 us against ~3 us), so one workgroup sums
+            // them first, in the order the kernel would use, and the
+            // direction kernel reads one total.  The update kernel hides its
+            // partial loads behind its five passes and keeps summing them.
             const bool wide = std::max(nb_in, vt.nb) > 4 * kThreads;  // > 1024 partials
+            int nbu = vt.nb;
+            if (wide) {
+                k_partials_total<<<1, kThreads, 0, st_>>>(prr, nb_in, tot_.p);
+                if (it == 0) k_partials_total<<<1, kThreads, 0, st_>>>(pbb, nb_in, tot_.p + 1);
+                LAUNCH_CHECK();
+                prr = tot_.p;
+                pbb = tot_.p + 1;
+                nbv = 1;
+            }
             VZU_DISPATCH(vzu_, ({
                 const void *f1, *f2;
                 if (wide) {
-                    f1 = it == 0 ? (const void *)k_cg_dir4<VZU, true, 16, false>
-                                 : (const void *)k_cg_dir4<VZU, false, 16, false>;
+                    f1 = it == 0 ? (const void *)k_cg_dir4<VZU, true, 1, false>
+                                 : (const void *)k_cg_dir4<VZU, false, 1, false>;
                     f2 = (const void *)k_cg_update4<VZU, 16, false>;
                 } else if (nt_) {
                     f1 = it == 0 ? (const void *)k_cg_dir4<VZU, true, 4, true>
@@ -884,7 +900,8 @@ class Solver final : public SolverBase {
                 void *a1[] = {&g, &vt, &itv, &rt, &nbv, &prr, &pbb, &st, &r, &dold, &dnew, &pdq,
                               &dm, &sid};
                 HIPT(hipExtLaunchKernel(f1, dim3(vt.nb), dim3(kThreads), a1, 0, st_, e0, e1, 0));
-                void *a2[] = {&g, &vt, &itv, &pdq_in, &st, &dcur, &pp, &rw, &prr_out, &dm, &sid};
+                void *a2[] = {&g, &vt, &itv, &nbu, &pdq_in, &st, &dcur, &pp, &rw, &prr_out, &dm,
+                              &sid};
                 HIPT(hipExtLaunchKernel(f2, dim3(vt.nb), dim3(kThreads), a2, 0, st_, e2, e3, 0));
             }));
         } else {
@@ -1693,6 +1710,7 @@ class Solver final : public SolverBase {
     T *cur_[5], *nxt_[5];  // u, v, w, density, temperature (current / scratch)
     DevBuf<T> fwd_[2], curl_, p_, b_, r_, d_[2];
     DevBuf<double> partA_, partB_, partC_, part_tel_, scalar_, tel_, tel_ring_, staging_;
+    DevBuf<double> tot_;  // pre-summed r.r and b.b of the wide four-wide path
     DevBuf<CgState> cg_;
     DevBuf<int> iters_log_, iters_dev_;
     int last_kind_ = AETHER_SOLVER_CG;

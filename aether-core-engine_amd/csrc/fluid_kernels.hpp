@@ -47,17 +47,45 @@ struct CgState {
 
 __device__ __forceinline__ int tid_lin() { return threadIdx.y * kBX + threadIdx.x; }
 
+// Lane exchange inside 16-lane rows through DPP (no LDS round trip):
+// kXor1 / kXor2 are quad_perm [1,0,3,2] / [2,3,0,1]; kMirror8 / kMirror16
+// reverse each half-row / row.  Once the lanes of every 4-group (8-group)
+// hold equal values, the 8- (16-) lane mirror pairs the same groups as an
+// xor by 4 (8) would.
+constexpr int kXor1 = 0xB1, kXor2 = 0x4E, kMirror8 = 0x141, kMirror16 = 0x140;
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+__device__ __forceinline__ double lane_f64(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Butterfly over the 64 lanes: pairs by xor 1, 2, 4, 8 inside each row
+// (DPP), then rows (0 + 1) + (2 + 3).  Every lane ends with the
+// bit-identical total.
 __device__ __forceinline__ double wave_sum(double v) {
-    // xor butterfly: every lane ends with the bit-identical total
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+    v += dpp_f64<kXor1>(v);
+    v += dpp_f64<kXor2>(v);
+    v += dpp_f64<kMirror8>(v);
+    v += dpp_f64<kMirror16>(v);
+    return (lane_f64(v, 0) + lane_f64(v, 16)) + (lane_f64(v, 32) + lane_f64(v, 48));
 }
 
 __device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
-    return v;
+    v = fmax(v, dpp_f64<kXor1>(v));
+    v = fmax(v, dpp_f64<kXor2>(v));
+    v = fmax(v, dpp_f64<kMirror8>(v));
+    v = fmax(v, dpp_f64<kMirror16>(v));
+    return fmax(fmax(lane_f64(v, 0), lane_f64(v, 16)), fmax(lane_f64(v, 32), lane_f64(v, 48)));
 }
 
 // Sum NV values over the workgroup; every thread receives the totals.
@@ -1266,6 +1294,18 @@ __global__ __launch_bounds__(kThreads) void k_telemetry_final(const double *__re
 // ---------------------------------------------------------------------------
 // Small utilities
 // ---------------------------------------------------------------------------
+// One workgroup: the total of nb partials in the order of
+// EarlyPartialsT<16>::total (so a kernel that reads this total gets the
+// value it would have summed itself), with every load in flight at once.
+__global__ __launch_bounds__(kThreads) void k_partials_total(const double *__restrict__ part,
+                                                             int nb, double *__restrict__ out) {
+    __shared__ double lds[8];
+    EarlyPartialsT<16> v;
+    v.load(part, nb);
+    const double s = v.total(part, nb, lds);
+    if (tid_lin() == 0) *out = s;
+}
+
 __global__ __launch_bounds__(kThreads) void k_partials_final(const double *__restrict__ part,
                                                              int nb, int op,
                                                              double *__restrict__ out) {

@@ -369,6 +369,34 @@ def test_resident_cg_matches_streaming_cg_128():
     assert abs(sa["r2"] - sb["r2"]) <= 1e-3 * sb["r2"]
 
 
+def test_wide_four_wide_cg_matches_scalar_cg():
+    """Past 1024 partial sums (168x160x160: 1080 four-wide tiles) the
+    four-wide CG has one workgroup sum the r.r partials before each
+    direction launch.
+    It runs the same iterations as the one-cell-per-thread CG, reaches the
+    same fields, and repeats bit for bit."""
+    _announce("wide four-wide CG vs scalar CG 168x160x160")
+    import aether_fluid as af
+    dims = (168, 160, 160)
+    state = random_state(*dims, seed=3)
+    runs = []
+    for form in (af.CG_X4, af.CG_X4, af.CG_SCALAR):
+        s = af.FluidSolver3D(*dims, precision="fp32")
+        s.cg_form, s.solver_iters, s.cg_rel_tol = form, 200, 1e-4
+        for name, arr in state.items():
+            s.upload(name, arr)
+        s.project()
+        runs.append((s.last_solve_stats(), {n: s.download(n) for n in ("u", "v", "w", "pressure")}))
+    (sa, fa), (sb, fb), (sc, fc) = runs
+    assert sa["iters"] == sb["iters"]
+    for n in fa:
+        np.testing.assert_array_equal(fa[n], fb[n])
+    assert 0 < sa["iters"] < 200 and abs(sa["iters"] - sc["iters"]) <= 1, (sa, sc)
+    for n in fa:
+        err = rel_l2(fa[n], fc[n])
+        assert err <= 1e-5, (n, err)
+
+
 def test_full_size_128_projection_properties():
     """Size-independent checks at 128^3: the recursive residual reached the
     reported value, the projected field's divergence equals -r/h^2 per cell
