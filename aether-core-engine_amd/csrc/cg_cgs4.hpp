@@ -1,0 +1,187 @@
+// cg_cgs4.hpp -- one launch per CG iteration in the single-reduction form
+// (Chronopoulos-Gear) with the products w = A r and s = A d carried in
+// memory, four x-cells per thread.
+//
+// Launch `it` holds gamma_it = r.r and delta_it = r.w as partials of the
+// previous launch; with beta = gamma_it / gamma_(it-1) and
+// mu = delta_it - beta gamma_it / alpha_(it-1) (= d.A d), alpha = gamma_it / mu:
+//   d <- r + beta d        s <- w + beta s        p <- p + alpha d
+//   r' = r - alpha s       w' = A r'              partials of r'.r', r'.w'
+// The reference's loop (src/fluid.rs:463-487) produces the same iterates in
+// exact arithmetic.  Because s and w are stored, r' is needed only on a
+// radius-1 neighbourhood, which each thread forms from the r, w and s it
+// loads there (y and z halos, x edges at segment ends) and from its
+// neighbour lanes (x): no LDS exchange, one stencil.  r, w and s are read
+// on halos by other workgroups, so they ping-pong between two buffers; d and
+// p are updated in place.  10 array passes per iteration in one launch.
+#pragma once
+#include "cg_vec.hpp"
+
+namespace aether {
+
+template <int ZU, bool FIRST, int NQ, bool NT>
+__global__ __launch_bounds__(kThreads) void k_cg_cgs4(
+    Dims g, VTiling vt, int it, double rel_tol2, int nb_in, const double *__restrict__ pg_in,
+    const double *__restrict__ pd_in, const double *__restrict__ pbb, double *__restrict__ pg_out,
+    double *__restrict__ pd_out, CgState *st, const float *__restrict__ r_in,
+    float *__restrict__ r_out, const float *__restrict__ w_in, float *__restrict__ w_out,
+    const float *__restrict__ s_in, float *__restrict__ s_out, float *__restrict__ d,
+    float *__restrict__ p, int *done_mirror, int solve_id) {
+    int bx, by, bz;
+    vtile_coords(vt, bx, by, bz);
+    const VCol c = vcol_at<ZU>(g, vt, bx, by, bz);
+    __shared__ double lds[8];
+    __shared__ int flag_lds[1];
+    const int flag_reg = st->done;
+    EarlyPartialsT<NQ> eg, ed, eb;
+    eg.load(pg_in, nb_in);
+    ed.load(pd_in, nb_in);
+    if (FIRST) eb.load(pbb, nb_in);
+    double gprev = 1.0, aprev = 1.0, b2 = 0.0;
+    if (!FIRST) {
+        gprev = st->cg_gamma[(it - 1) & 1];
+        aprev = st->cg_alpha[(it - 1) & 1];
+        b2 = st->b2;
+    }
+    const float4 *r4 = reinterpret_cast<const float4 *>(r_in);
+    const float4 *w4 = reinterpret_cast<const float4 *>(w_in);
+    const float4 *s4 = reinterpret_cast<const float4 *>(s_in);
+    float4 *d4 = reinterpret_cast<float4 *>(d);
+    float4 *p4 = reinterpret_cast<float4 *>(p);
+    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 rz[ZU + 2], wz[ZU + 2], sz[ZU + 2], ry[ZU][2], wy[ZU][2], sy[ZU][2], dc[ZU], pc[ZU];
+    float er[ZU][2], ew[ZU][2], es[ZU][2];
+#pragma unroll
+    for (int a = 0; a < ZU + 2; ++a) {
+        const int o = (min(max(c.k0 - 1 + a, 0), g.nz - 1) * g.sxy + c.off) >> 2;
+        rz[a] = r4[o];
+        wz[a] = w4[o];
+        sz[a] = FIRST ? z4 : s4[o];
+    }
+#pragma unroll
+    for (int a = 0; a < ZU; ++a) {
+        const int base = min(c.k0 + a, g.nz - 1) * g.sxy;
+        const int om = (base + c.offym) >> 2, op = (base + c.offyp) >> 2, oc = (base + c.off) >> 2;
+        ry[a][0] = r4[om];
+        ry[a][1] = r4[op];
+        wy[a][0] = w4[om];
+        wy[a][1] = w4[op];
+        sy[a][0] = FIRST ? z4 : s4[om];
+        sy[a][1] = FIRST ? z4 : s4[op];
+        dc[a] = FIRST ? z4 : d4[oc];
+        pc[a] = p4[oc];
+#pragma unroll
+        for (int e = 0; e < 2; ++e) er[a][e] = ew[a][e] = es[a][e] = 0.0f;
+        if (c.needL) {
+            er[a][0] = r_in[base + c.off - 1];
+            ew[a][0] = w_in[base + c.off - 1];
+            if (!FIRST) es[a][0] = s_in[base + c.off - 1];
+        }
+        if (c.needR) {
+            er[a][1] = r_in[base + c.off + 4];
+            ew[a][1] = w_in[base + c.off + 4];
+            if (!FIRST) es[a][1] = s_in[base + c.off + 4];
+        }
+    }
+
+    // ---- scalars (every workgroup reaches the same decision) -------------
+    if (threadIdx.x == 0) flag_lds[0] = flag_reg;
+    double tot[2] = {eg.local(pg_in, nb_in), ed.local(pd_in, nb_in)};
+    block_sum<2>(tot, lds);  // its barrier publishes flag_lds
+    if (flag_lds[0]) return;
+    const double gam = tot[0], del = tot[1];
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    double beta = 0.0, mu = del;
+    bool stop;
+    if (FIRST) {
+        b2 = eb.total(pbb, nb_in, lds);
+        stop = !(gam > 1e-24) || (rel_tol2 > 0.0 && gam <= rel_tol2 * b2);
+        if (lead) {
+            st->b2 = b2;
+            st->r2_init = gam;
+        }
+    } else {
+        stop = gam < 1e-20 || (rel_tol2 > 0.0 && gam <= rel_tol2 * b2);
+        beta = gam / gprev;
+        mu = del - beta * gam / aprev;
+    }
+    if (!stop && fabs(mu) < 1e-28) stop = true;
+    if (stop) {
+        if (lead) {
+            st->done = 1;
+            st->iters = it;
+            st->r2_final = gam;
+            st->converged = 1;
+            *done_mirror = solve_id;
+        }
+        return;
+    }
+    const double alpha = gam / mu;
+    if (lead) {
+        st->cg_gamma[it & 1] = gam;
+        st->cg_alpha[it & 1] = alpha;
+    }
+    const float tb = (float)beta, ta = (float)alpha;
+
+    // ---- s = w + beta s and r' = r - alpha s on the radius-1 neighbourhood
+    float4 rn[ZU + 2], rny[ZU][2], sn[ZU];
+    float rne[ZU][2];
+#pragma unroll
+    for (int a = 0; a < ZU + 2; ++a) {
+        const float4 sv = FIRST ? wz[a] : axpy4(wz[a], tb, sz[a]);
+        if (a >= 1 && a <= ZU) sn[a - 1] = sv;
+        rn[a] = make_float4(rz[a].x - ta * sv.x, rz[a].y - ta * sv.y, rz[a].z - ta * sv.z,
+                            rz[a].w - ta * sv.w);
+    }
+#pragma unroll
+    for (int a = 0; a < ZU; ++a) {
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const float4 sv = FIRST ? wy[a][e] : axpy4(wy[a][e], tb, sy[a][e]);
+            rny[a][e] = make_float4(ry[a][e].x - ta * sv.x, ry[a][e].y - ta * sv.y,
+                                    ry[a][e].z - ta * sv.z, ry[a][e].w - ta * sv.w);
+            const float se = FIRST ? ew[a][e] : ew[a][e] + tb * es[a][e];
+            rne[a][e] = er[a][e] - ta * se;
+        }
+    }
+
+    // ---- w' = A r', d, p and the partials of r'.r', r'.w' ----------------
+    const int srcL = (c.lane + 63) & 63, srcR = (c.lane + 1) & 63;
+    double acc[2] = {0.0, 0.0};
+    float4 *ro4 = reinterpret_cast<float4 *>(r_out);
+    float4 *wo4 = reinterpret_cast<float4 *>(w_out);
+    float4 *so4 = reinterpret_cast<float4 *>(s_out);
+#pragma unroll
+    for (int a = 0; a < ZU; ++a) {
+        const int k = c.k0 + a;
+        const float sl = lane_value(rn[a + 1].w, srcL), sr = lane_value(rn[a + 1].x, srcR);
+        const float xl = c.needL ? rne[a][0] : sl;
+        const float xrv = c.needR ? rne[a][1] : sr;
+        if (c.valid && k < g.nz) {
+            float q[4];
+            lap4(g, c, k, rn[a + 1], xl, xrv, rny[a][0], rny[a][1], rn[a], rn[a + 2], q);
+            const float4 wn = make_float4(q[0], q[1], q[2], q[3]);
+            const float4 dn = FIRST ? rz[a + 1] : axpy4(rz[a + 1], tb, dc[a]);
+            const float4 pn = axpy4(pc[a], ta, dn);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const double rv = (double)f4(rn[a + 1], e);
+                acc[0] += rv * rv;
+                acc[1] += rv * (double)q[e];
+            }
+            const int id4 = (k * g.sxy + c.off) >> 2;
+            st4<NT>(ro4 + id4, rn[a + 1]);
+            st4<NT>(wo4 + id4, wn);
+            st4<NT>(so4 + id4, sn[a]);
+            st4<NT>(d4 + id4, dn);
+            st4<NT>(p4 + id4, pn);
+        }
+    }
+    block_sum<2>(acc, lds);
+    if (threadIdx.x == 0) {
+        pg_out[blockIdx.x] = acc[0];
+        pd_out[blockIdx.x] = acc[1];
+    }
+}
+
+}  // namespace aether

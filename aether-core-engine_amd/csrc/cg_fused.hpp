@@ -39,10 +39,12 @@ constexpr int kR1Per = (kR1 + kThreads - 1) / kThreads;           // 2
 static_assert(kFX == kBX && kFY == kBY && kFX * kFY == kThreads, "tile = workgroup shape");
 
 // delta_0 = r_0 . A r_0 partials over the initialisation tiling (the other
-// half of the first iteration's reduction; gamma_0 comes with r_0).
+// half of the first iteration's reduction; gamma_0 comes with r_0).  When
+// w_out is set, A r_0 is also stored there (0 off the interior).
 template <typename T, int ZU>
 __global__ __launch_bounds__(kThreads) void k_cg_rar(Dims g, Tiling tl, const T *__restrict__ r,
-                                                     double *__restrict__ part) {
+                                                     double *__restrict__ part,
+                                                     T *__restrict__ w_out) {
     __shared__ double lds[8];
     const Col c = col_of<ZU>(g, tl);
     T rz[ZU + 2], rn[ZU][4];
@@ -60,12 +62,14 @@ __global__ __launch_bounds__(kThreads) void k_cg_rar(Dims g, Tiling tl, const T 
 #pragma unroll
     for (int a = 0; a < ZU; ++a) {
         const int k = c.k0 + a;
-        if (c.valid && k < g.nz && is_interior(g, c.i, c.j, k)) {
-            const T q = lap7<T>(rz[a + 1], rn[a][0], rn[a][1], rn[a][2], rn[a][3], rz[a],
-                                rz[a + 2], c.i > 1, c.i + 2 < g.nx, c.j > 1, c.j + 2 < g.ny,
-                                k > 1, k + 2 < g.nz);
+        if (!c.valid || k >= g.nz) continue;
+        T q = T(0);
+        if (is_interior(g, c.i, c.j, k)) {
+            q = lap7<T>(rz[a + 1], rn[a][0], rn[a][1], rn[a][2], rn[a][3], rz[a], rz[a + 2],
+                        c.i > 1, c.i + 2 < g.nx, c.j > 1, c.j + 2 < g.ny, k > 1, k + 2 < g.nz);
             acc[0] += (double)rz[a + 1] * (double)q;
         }
+        if (w_out) w_out[col_id(g, c, k)] = q;
     }
     block_sum<1>(acc, lds);
     if (tid_lin() == 0) part[blockIdx.x] = acc[0];

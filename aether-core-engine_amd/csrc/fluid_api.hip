@@ -24,7 +24,7 @@
 #include "cg_resident.hpp"
 #include "cg_fused.hpp"
 #include "cg_vec.hpp"
-#include "cg_fused4.hpp"
+#include "cg_cgs4.hpp"
 #include "batch_kernels.hpp"
 #include "adjoint_kernels.hpp"
 
@@ -354,14 +354,16 @@ class Solver final : public SolverBase {
             tlv_.remap = (tlv_.nb % 8 == 0) ? 1 : 0;
             if (const char *e = std::getenv("AETHER_XCD_REMAP")) tlv_.remap &= std::atoi(e) != 0;
         }
-        // one-launch four-wide CG: whole rows (nx <= 256), 4 planes per group
-        if (nx % 4 == 0 && nx <= 256) {
-            tl4_.txv = nx / 4;
+        // one-launch four-wide CG (cg_cgs4.hpp): the four-wide tiling with 4
+        // planes per thread
+        if (nx % 4 == 0) {
+            const int w4 = nx / 4;
+            tl4_.txv = std::min(w4, 64);
             tl4_.ty = kThreads / tl4_.txv;
-            tl4_.gx = 1;
+            tl4_.gx = (w4 + tl4_.txv - 1) / tl4_.txv;
             tl4_.gy = (ny + tl4_.ty - 1) / tl4_.ty;
             tl4_.gz = (nz + 3) / 4;
-            tl4_.nb = tl4_.gy * tl4_.gz;
+            tl4_.nb = tl4_.gx * tl4_.gy * tl4_.gz;
             tl4_.remap = (tl4_.nb % 8 == 0) ? 1 : 0;
             if (const char *e = std::getenv("AETHER_XCD_REMAP")) tl4_.remap &= std::atoi(e) != 0;
         }
@@ -663,21 +665,27 @@ class Solver final : public SolverBase {
                 e1 = timer.get();
                 timer.recs.push_back({3, step_index, it, e0, e1});
             }
-            if (tl4_.nb > 0) {  // whole-row four-wide form (cg_fused4.hpp)
+            if (tl4_.nb > 0) {  // four-wide, A r and A d carried (cg_cgs4.hpp)
                 Dims g = g_;
                 VTiling vt = tl4_;
                 int itv = it, nbv = nb_in, sid = solve_id;
                 double rt = rel2;
-                void *args[] = {&g,   &vt,    &itv, &rt, &nbv, &pg_in, &pd_in, &pbb, &pg_out,
-                                &pd_out, &st, &r_in, &r_out, &d_in, &d_out, &pp, &dm, &sid};
+                const T *w_in = w_[it & 1].p, *s_in = s_[it & 1].p;
+                T *w_out = w_[(it + 1) & 1].p, *s_out = s_[(it + 1) & 1].p, *dd = d_[0].p;
+                void *args[] = {&g,     &vt,   &itv,  &rt,    &nbv,  &pg_in, &pd_in,
+                                &pbb,   &pg_out, &pd_out, &st, &r_in, &r_out, &w_in,
+                                &w_out, &s_in, &s_out, &dd, &pp, &dm, &sid};
                 const bool wide = std::max(nb_in, vt.nb) > 4 * kThreads;
                 const void *fn;
                 if (wide)
-                    fn = it == 0 ? (const void *)k_cg_fused4<4, true, 16>
-                                 : (const void *)k_cg_fused4<4, false, 16>;
+                    fn = it == 0 ? (const void *)k_cg_cgs4<4, true, 16, false>
+                                 : (const void *)k_cg_cgs4<4, false, 16, false>;
+                else if (nt_)
+                    fn = it == 0 ? (const void *)k_cg_cgs4<4, true, 4, true>
+                                 : (const void *)k_cg_cgs4<4, false, 4, true>;
                 else
-                    fn = it == 0 ? (const void *)k_cg_fused4<4, true, 4>
-                                 : (const void *)k_cg_fused4<4, false, 4>;
+                    fn = it == 0 ? (const void *)k_cg_cgs4<4, true, 4, false>
+                                 : (const void *)k_cg_cgs4<4, false, 4, false>;
                 HIPT(hipExtLaunchKernel(fn, dim3(vt.nb), dim3(kThreads), args, 0, st_, e0, e1, 0));
                 return;
             }
@@ -710,6 +718,17 @@ class Solver final : public SolverBase {
     }
 
     // The CG layout of the next solve (AETHER_CG_SCALAR/X4/FUSED).
+    // Untuned choice for FP32.  The four-wide one-launch CG moves 10 array
+    // passes per iteration in one launch, the two-launch form 8 in two: the
+    // one launch wins while its eight FP32 arrays (32 N bytes) stay resident
+    // in the 256 MB Infinity Cache next to the step's other fields (measured:
+    // 20% faster at 96^3, 16% at 128^3, slower from 192^3 on).
+    int default_form() const {
+        if (g_.nx % 4 != 0) return AETHER_CG_SCALAR;
+        return n_ <= kOneLaunchMaxCells ? AETHER_CG_FUSED : AETHER_CG_X4;
+    }
+    static constexpr size_t kOneLaunchMaxCells = 3200000;  // 32 N bytes <= ~100 MB
+
     int effective_form() const {
         if constexpr (sizeof(T) != 4) {
             return AETHER_CG_SCALAR;
@@ -717,9 +736,7 @@ class Solver final : public SolverBase {
             if (trial_form_) return trial_form_;  // the autotuner's candidate
             int f = params.cg_form;
             if (f == AETHER_CG_AUTO)
-                f = env_form_ ? env_form_
-                              : tuned_form_ ? tuned_form_
-                                            : (g_.nx % 4 == 0 ? AETHER_CG_X4 : AETHER_CG_SCALAR);
+                f = env_form_ ? env_form_ : tuned_form_ ? tuned_form_ : default_form();
             if (f == AETHER_CG_X4 && g_.nx % 4 != 0)
                 raise(AETHER_EINVAL, "AETHER_CG_X4 needs nx % 4 == 0");
             return f;
@@ -833,13 +850,19 @@ class Solver final : public SolverBase {
     }
 
     void ensure_fused_buffers() {
-        if (r2_.p) return;
-        const size_t npart = (size_t)std::max(
-            std::max(std::max(nb_, tlv_.nb), std::max(tl_.nb, tlf_.nb)), tl4_.nb);
-        r2_.alloc(n_);
-        zero(r2_.p);
-        partE_.alloc(npart);
-        partF_.alloc(npart);
+        if (!r2_.p) {
+            const size_t npart = (size_t)std::max(
+                std::max(std::max(nb_, tlv_.nb), std::max(tl_.nb, tlf_.nb)), tl4_.nb);
+            r2_.alloc(n_);
+            zero(r2_.p);
+            partE_.alloc(npart);
+            partF_.alloc(npart);
+        }
+        if (sizeof(T) == 4 && tl4_.nb > 0 && !w_[0].p)
+            for (DevBuf<T> *b : {&w_[0], &w_[1], &s_[0], &s_[1]}) {
+                b->alloc(n_);
+                zero(b->p);
+            }
     }
 
     // Four-wide two-launch CG iteration `it` (cg_vec.hpp); same buffers and
@@ -965,15 +988,19 @@ class Solver final : public SolverBase {
         if ((int)chunk_events_.size() < nchunks) {
             for (int a = (int)chunk_events_.size(); a < nchunks; ++a) {
                 hipEvent_t e;
-                HIPT(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+                // pacing only: the done flag they gate is written straight to
+                // coherent host memory, so no system-scope cache fence is
+                // needed (with one, every record idled the GPU ~6 us)
+                HIPT(hipEventCreateWithFlags(&e, hipEventDisableTiming | hipEventDisableSystemFence));
                 chunk_events_.push_back(e);
             }
         }
         form_ = effective_form();
         if (form_ == AETHER_CG_FUSED) ensure_fused_buffers();
         if (form_ == AETHER_CG_FUSED && max_iters > 0) {  // delta_0 = r_0 . A r_0 partials
+            T *w0 = (sizeof(T) == 4 && tl4_.nb > 0) ? w_[0].p : nullptr;  // w_0 = A r_0
             ZU_DISPATCH(zu_, (k_cg_rar<T, ZU><<<tl_.nb, cell_blk_, 0, st_>>>(g_, tl_, r_.p,
-                                                                             partE_.p)));
+                                                                             partE_.p, w0)));
             LAUNCH_CHECK();
         }
         for (int c = 0; c < nchunks; ++c) {
@@ -1680,6 +1707,7 @@ class Solver final : public SolverBase {
     int kz_ = 16;
     int pf_ = 4;          // planes of loads in flight ahead of use
     DevBuf<T> r2_;                 // second residual buffer of the one-launch CG
+    DevBuf<T> w_[2], s_[2];        // A r and A d of the four-wide one-launch CG
     DevBuf<double> partE_, partF_;  // its delta partials
     bool ir64_ = false;           // mixed-precision projection (AETHER_FP32_IR64)
     // rollout / adjoint state
@@ -2119,7 +2147,10 @@ class Batch final : public BatchBase {
         const int nchunks = (max_iters + chunk - 1) / chunk;
         while ((int)chunk_events_.size() < nchunks) {
             hipEvent_t e;
-            HIPT(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            // pacing only: the done flag they gate is written straight to
+                // coherent host memory, so no system-scope cache fence is
+                // needed (with one, every record idled the GPU ~6 us)
+                HIPT(hipEventCreateWithFlags(&e, hipEventDisableTiming | hipEventDisableSystemFence));
             chunk_events_.push_back(e);
         }
         const dim3 blk(kBX, kBY);

@@ -147,20 +147,25 @@ struct EarlyPartialsT {
             v[q] = a < nb ? x : 0.0;
         }
     }
+    // this thread's share of the total, in a fixed order
+    __device__ __forceinline__ double local(const double *__restrict__ part, int nb) const {
+        double acc;
+        if constexpr (NQ == 8)
+            acc = ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
+        else if constexpr (NQ == 4)
+            acc = (v[0] + v[1]) + (v[2] + v[3]);
+        else {
+            acc = 0.0;
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) acc += v[q];
+        }
+        for (int a = tid_lin() + NQ * kThreads; a < nb; a += kThreads) acc += part[a];
+        return acc;
+    }
     // fixed-order total, identical in every workgroup
     __device__ __forceinline__ double total(const double *__restrict__ part, int nb,
                                             double *lds) const {
-        double acc[1];
-        if constexpr (NQ == 8)
-            acc[0] = ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
-        else if constexpr (NQ == 4)
-            acc[0] = (v[0] + v[1]) + (v[2] + v[3]);
-        else {
-            acc[0] = 0.0;
-#pragma unroll
-            for (int q = 0; q < NQ; ++q) acc[0] += v[q];
-        }
-        for (int a = tid_lin() + NQ * kThreads; a < nb; a += kThreads) acc[0] += part[a];
+        double acc[1] = {local(part, nb)};
         block_sum<1>(acc, lds);
         return acc[0];
     }

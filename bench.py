@@ -65,16 +65,17 @@ def kernel_roofline(solver_kind: str, nvox: int, steps: int = 2):
 
     Algorithmic bytes (FP32 array passes of N cells): cg_dir reads r, d_old
     and writes d (3 passes); cg_update reads p, d, r and writes p, r (5
-    passes); the register-resident solve does the same work in one launch,
-    8 passes per CG iteration it runs; the one-launch single-reduction
-    iteration (cg_fused) reads p, r, d and writes p, r, d (6 passes)."""
+    passes); a launch that runs whole CG iterations counts the same 8
+    passes per iteration: the register-resident solve (per iteration it
+    runs) and the one-launch iteration (cg_fused; the four-wide one moves
+    10 passes, carrying A r and A d, which the measured traffic shows)."""
     cfg, s = make_solver(solver_kind)
     s.set_kernel_timing(True)
     s.run(cfg["dt"], [cfg["dt"] * i for i in range(steps)])
     iters = s.run_iters()
     kern = {}
     for which, name, passes in ((0, "cg_dir", 3), (1, "cg_update", 5), (2, "cg_resident", None),
-                                (3, "cg_fused", 6)):
+                                (3, "cg_fused", 8)):
         avg, n = s.kernel_timing(which)
         if n == 0:
             continue

@@ -46,7 +46,9 @@ extern "C" {
 
 /* CG iteration layout (all follow src/fluid.rs:463-487; the FP64
  * instantiation always uses AETHER_CG_SCALAR, the reference's evaluation order) */
-#define AETHER_CG_AUTO 0     /* FP32: AETHER_CG_X4 when nx % 4 == 0, else AETHER_CG_SCALAR */
+#define AETHER_CG_AUTO 0     /* FP32: nx % 4 != 0: AETHER_CG_SCALAR; else AETHER_CG_FUSED up to
+                                3.2 M cells (cache-resident), AETHER_CG_X4 above; or the
+                                aether_fluid_autotune_cg choice */
 #define AETHER_CG_SCALAR 1   /* two launches per iteration, one cell per thread */
 #define AETHER_CG_X4 2       /* two launches per iteration, four x-cells per thread (FP32, nx % 4 == 0) */
 #define AETHER_CG_FUSED 3    /* one launch per iteration, single-reduction form (FP32) */
