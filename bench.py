@@ -349,9 +349,12 @@ def main():
     if args.workload == "mixed_256":
         print(json.dumps(bench_mixed(args.steps, args.warmup)), flush=True)
         return
-    if args.probe:
+    if args.probe:  # short run for counter collection: both CG layouts, then the copy
         _, s = make_solver("cg")
         s.run(SMOKE_DT, [0.0, SMOKE_DT])
+        _, s2 = make_solver("cg")
+        s2.cg_form = af.CG_X4
+        s2.run(SMOKE_DT, [0.0, SMOKE_DT])
         af.stream_copy_ms(COPY_BYTES, 5)
         print("probe done", flush=True)
         return
