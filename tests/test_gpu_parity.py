@@ -371,8 +371,9 @@ def test_resident_cg_matches_streaming_cg_128():
 
 def test_wide_four_wide_cg_matches_scalar_cg():
     """Past 1024 partial sums (168x160x160: 1080 four-wide tiles) the
-    four-wide CG has one workgroup sum the r.r partials before each
-    direction launch.
+    two-launch four-wide CG has one workgroup sum the r.r partials before
+    each direction launch, and the one-launch form holds 16 partials per
+    thread.
     It runs the same iterations as the one-cell-per-thread CG, reaches the
     same fields, and repeats bit for bit."""
     _announce("wide four-wide CG vs scalar CG 168x160x160")
@@ -380,21 +381,52 @@ def test_wide_four_wide_cg_matches_scalar_cg():
     dims = (168, 160, 160)
     state = random_state(*dims, seed=3)
     runs = []
-    for form in (af.CG_X4, af.CG_X4, af.CG_SCALAR):
+    for form in (af.CG_X4, af.CG_X4, af.CG_SCALAR, af.CG_FUSED):
         s = af.FluidSolver3D(*dims, precision="fp32")
         s.cg_form, s.solver_iters, s.cg_rel_tol = form, 200, 1e-4
         for name, arr in state.items():
             s.upload(name, arr)
         s.project()
         runs.append((s.last_solve_stats(), {n: s.download(n) for n in ("u", "v", "w", "pressure")}))
-    (sa, fa), (sb, fb), (sc, fc) = runs
+    (sa, fa), (sb, fb), (sc, fc), (sd, fd) = runs
     assert sa["iters"] == sb["iters"]
     for n in fa:
         np.testing.assert_array_equal(fa[n], fb[n])
     assert 0 < sa["iters"] < 200 and abs(sa["iters"] - sc["iters"]) <= 1, (sa, sc)
+    # the one-launch form at this size: 1080 tiles, its wide-partials variant
+    assert abs(sd["iters"] - sc["iters"]) <= 1, (sd, sc)
     for n in fa:
-        err = rel_l2(fa[n], fc[n])
-        assert err <= 1e-5, (n, err)
+        assert rel_l2(fa[n], fc[n]) <= 1e-5, (n, rel_l2(fa[n], fc[n]))
+        assert rel_l2(fd[n], fc[n]) <= 1e-5, (n, rel_l2(fd[n], fc[n]))
+
+
+def test_one_launch_cg_segment_edges_vs_oracle():
+    """The one-launch four-wide CG where rows span two 64-lane segments
+    (320 cells: 64 + 16 float4 lanes), so x-neighbours across every wave
+    boundary come from the edge loads, not from the adjacent lane: one
+    projection against the f64 oracle (FP32 bar), and against the two-launch
+    four-wide CG."""
+    _announce("one-launch CG segment edges 320x24x20")
+    import aether_fluid as af
+    dims = (320, 24, 20)
+    state = random_state(*dims, seed=11)
+    gpu, orc = make_pair(*dims, precision="fp32", state=state)
+    set_knobs(gpu, orc, iters=200, rel_tol=3e-4)  # the oracle converges in 112 iterations
+    gpu.cg_form = af.CG_FUSED
+    gpu.project()
+    orc.project()
+    so, sg = orc.stats(), gpu.last_solve_stats()
+    assert 0 < so["iters"] < 200 and abs(sg["iters"] - so["iters"]) <= 1, (sg, so)
+    errs = compare_fields(gpu, orc, ("u", "v", "w", "pressure"))
+    assert max(errs.values()) <= 1e-5, errs
+    x4 = af.FluidSolver3D(*dims, precision="fp32")
+    x4.cg_form, x4.solver_iters, x4.cg_rel_tol = af.CG_X4, 200, 3e-4
+    for name, arr in state.items():
+        x4.upload(name, arr)
+    x4.project()
+    assert abs(x4.last_solve_stats()["iters"] - sg["iters"]) <= 1
+    for n in ("u", "v", "w", "pressure"):
+        assert rel_l2(gpu.download(n), x4.download(n)) <= 1e-5, n
 
 
 def test_full_size_128_projection_properties():
