@@ -676,16 +676,20 @@ class Solver final : public SolverBase {
                                 &pbb,   &pg_out, &pd_out, &st, &r_in, &r_out, &w_in,
                                 &w_out, &s_in, &s_out, &dd, &pp, &dm, &sid};
                 const bool wide = std::max(nb_in, vt.nb) > 4 * kThreads;
+                const bool odd = it & 1;  // p is updated on odd launches
                 const void *fn;
                 if (wide)
-                    fn = it == 0 ? (const void *)k_cg_cgs4<4, true, 16, false>
-                                 : (const void *)k_cg_cgs4<4, false, 16, false>;
+                    fn = it == 0 ? (const void *)k_cg_cgs4<4, true, 16, false, false>
+                         : odd   ? (const void *)k_cg_cgs4<4, false, 16, false, true>
+                                 : (const void *)k_cg_cgs4<4, false, 16, false, false>;
                 else if (nt_)
-                    fn = it == 0 ? (const void *)k_cg_cgs4<4, true, 4, true>
-                                 : (const void *)k_cg_cgs4<4, false, 4, true>;
+                    fn = it == 0 ? (const void *)k_cg_cgs4<4, true, 4, true, false>
+                         : odd   ? (const void *)k_cg_cgs4<4, false, 4, true, true>
+                                 : (const void *)k_cg_cgs4<4, false, 4, true, false>;
                 else
-                    fn = it == 0 ? (const void *)k_cg_cgs4<4, true, 4, false>
-                                 : (const void *)k_cg_cgs4<4, false, 4, false>;
+                    fn = it == 0 ? (const void *)k_cg_cgs4<4, true, 4, false, false>
+                         : odd   ? (const void *)k_cg_cgs4<4, false, 4, false, true>
+                                 : (const void *)k_cg_cgs4<4, false, 4, false, false>;
                 HIPT(hipExtLaunchKernel(fn, dim3(vt.nb), dim3(kThreads), args, 0, st_, e0, e1, 0));
                 return;
             }
@@ -1012,6 +1016,18 @@ class Solver final : public SolverBase {
             for (int it = c * chunk; it < it_end; ++it)
                 enqueue_cg_iteration(it, rel2, solve_id, step_index);
             HIPT(hipEventRecord(chunk_events_[c], st_));
+        }
+        if constexpr (sizeof(T) == 4) {
+            if (form_ == AETHER_CG_FUSED && tl4_.nb > 0 && (max_iters & 1)) {
+                // the cap ended the solve after an even launch of the four-wide
+                // one-launch form: its alpha d term is still pending
+                const long n4 = (long)(n_ / 4);
+                k_cg_cgs4_tail<<<(unsigned)std::min<long>((n4 + kThreads - 1) / kThreads, 4096),
+                                 kThreads, 0, st_>>>(cg_.p, max_iters - 1,
+                                                     reinterpret_cast<const float4 *>(d_[0].p),
+                                                     reinterpret_cast<float4 *>(p_.p), n4);
+                LAUNCH_CHECK();
+            }
         }
         const double *part_rr = partB_.p;
         int nb_rr = tl_.nb;
@@ -2147,10 +2163,8 @@ class Batch final : public BatchBase {
         const int nchunks = (max_iters + chunk - 1) / chunk;
         while ((int)chunk_events_.size() < nchunks) {
             hipEvent_t e;
-            // pacing only: the done flag they gate is written straight to
-                // coherent host memory, so no system-scope cache fence is
-                // needed (with one, every record idled the GPU ~6 us)
-                HIPT(hipEventCreateWithFlags(&e, hipEventDisableTiming | hipEventDisableSystemFence));
+            // pacing only (as the single-grid look-ahead events)
+            HIPT(hipEventCreateWithFlags(&e, hipEventDisableTiming | hipEventDisableSystemFence));
             chunk_events_.push_back(e);
         }
         const dim3 blk(kBX, kBY);

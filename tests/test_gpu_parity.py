@@ -400,6 +400,31 @@ def test_wide_four_wide_cg_matches_scalar_cg():
         assert rel_l2(fd[n], fc[n]) <= 1e-5, (n, rel_l2(fd[n], fc[n]))
 
 
+@pytest.mark.parametrize("iters,tol", [(25, 0.0), (200, 3e-2), (200, 1e-2), (200, 3e-3),
+                                       (200, 1e-3)])
+def test_one_launch_cg_pending_p_update_vs_oracle(iters, tol):
+    """The one-launch four-wide CG brings p up to date on odd launches only:
+    an odd iteration cap (25, no tolerance) leaves the last term to the tail
+    kernel, and the tolerances stop the oracle after 11, 16, 27 and 43
+    iterations (mostly odd, so the stopping launch applies the pending
+    term). FP32 bar against the f64 oracle."""
+    _announce(f"one-launch CG pending p update 64x48x40 iters={iters} tol={tol}")
+    import aether_fluid as af
+    dims = (64, 48, 40)
+    gpu, orc = make_pair(*dims, precision="fp32", state=random_state(*dims, seed=5))
+    set_knobs(gpu, orc, iters=iters, rel_tol=tol)
+    gpu.cg_form = af.CG_FUSED
+    gpu.project()
+    orc.project()
+    so, sg = orc.stats(), gpu.last_solve_stats()
+    if tol == 0.0:
+        assert sg["iters"] == so["iters"] == iters
+    else:
+        assert abs(sg["iters"] - so["iters"]) <= 1, (sg, so)
+    errs = compare_fields(gpu, orc, ("u", "v", "w", "pressure"))
+    assert max(errs.values()) <= 1e-5, errs
+
+
 def test_one_launch_cg_segment_edges_vs_oracle():
     """The one-launch four-wide CG where rows span two 64-lane segments
     (320 cells: 64 + 16 float4 lanes), so x-neighbours across every wave
