@@ -38,7 +38,8 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
     int bx, by, bz;
     vtile_coords(vt, bx, by, bz);
     const VCol c = vcol_at<ZU>(g, vt, bx, by, bz);
-    __shared__ double lds[8];
+    // separate slots for each reduction, so none needs a trailing barrier
+    __shared__ double lds[8], lds_b[8], lds_out[8];
     __shared__ int flag_lds[1];
     const int flag_reg = st->done;
     EarlyPartialsT<NQ> eg, ed, eb;
@@ -95,14 +96,14 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
     // ---- scalars (every workgroup reaches the same decision) -------------
     if (threadIdx.x == 0) flag_lds[0] = flag_reg;
     double tot[2] = {eg.local(pg_in, nb_in), ed.local(pd_in, nb_in)};
-    block_sum<2>(tot, lds);  // its barrier publishes flag_lds
+    block_sum<2, false>(tot, lds);  // its barrier publishes flag_lds
     if (flag_lds[0]) return;
     const double gam = tot[0], del = tot[1];
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     double beta = 0.0, mu = del;
     bool stop;
     if (FIRST) {
-        b2 = eb.total(pbb, nb_in, lds);
+        b2 = eb.total(pbb, nb_in, lds_b);
         stop = !(gam > 1e-24) || (rel_tol2 > 0.0 && gam <= rel_tol2 * b2);
         if (lead) {
             st->b2 = b2;
@@ -192,7 +193,7 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
             if (PUPD) st4<NT>(p4 + id4, axpy4(axpy4(pc[a], tap, dc[a]), ta, dn));
         }
     }
-    block_sum<2>(acc, lds);
+    block_sum<2, false>(acc, lds_out);
     if (threadIdx.x == 0) {
         pg_out[blockIdx.x] = acc[0];
         pd_out[blockIdx.x] = acc[1];

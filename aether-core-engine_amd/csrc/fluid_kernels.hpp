@@ -90,7 +90,10 @@ __device__ __forceinline__ double wave_max(double v) {
 
 // Sum NV values over the workgroup; every thread receives the totals.
 // lds must hold 4*NV doubles.
-template <int NV>
+// REUSE: lds may be written again later in the kernel, so a second barrier
+// keeps a fast wave from overwriting it while a slow one still reads it.
+// Callers that give every reduction its own lds slots pass false.
+template <int NV, bool REUSE = true>
 __device__ __forceinline__ void block_sum(double (&v)[NV], double *lds) {
     const int t = tid_lin(), lane = t & 63, wv = t >> 6;
 #pragma unroll
@@ -103,7 +106,7 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double *lds) {
 #pragma unroll
     for (int a = 0; a < NV; ++a)
         v[a] = ((lds[a * 4 + 0] + lds[a * 4 + 1]) + lds[a * 4 + 2]) + lds[a * 4 + 3];
-    __syncthreads();
+    if (REUSE) __syncthreads();
 }
 
 __device__ __forceinline__ double block_max(double v, double *lds) {
