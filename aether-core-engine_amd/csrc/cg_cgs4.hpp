@@ -27,7 +27,7 @@
 
 namespace aether {
 
-template <int ZU, bool FIRST, int NQ, bool NT, bool PUPD>
+template <int ZU, bool FIRST, int NQ, int ST, bool PUPD>
 __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
     Dims g, VTiling vt, int it, double rel_tol2, int nb_in, const double *__restrict__ pg_in,
     const double *__restrict__ pd_in, const double *__restrict__ pbb, double *__restrict__ pg_out,
@@ -57,6 +57,9 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
     const float4 *s4 = reinterpret_cast<const float4 *>(s_in);
     float4 *d4 = reinterpret_cast<float4 *>(d);
     float4 *p4 = reinterpret_cast<float4 *>(p);
+    const long nbytes = (long)g.sxy * g.nz * 4;
+    const Out4<ST> o_r(r_out, nbytes), o_w(w_out, nbytes), o_s(s_out, nbytes), o_d(d, nbytes),
+        o_p(p, nbytes);
     const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
     float4 rz[ZU + 2], wz[ZU + 2], sz[ZU + 2], ry[ZU][2], wy[ZU][2], sy[ZU][2], dc[ZU], pc[ZU];
     float er[ZU][2], ew[ZU][2], es[ZU][2];
@@ -128,7 +131,7 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
 #pragma unroll
             for (int a = 0; a < ZU; ++a) {
                 const int k = c.k0 + a;
-                if (c.valid && k < g.nz) p4[(k * g.sxy + c.off) >> 2] = axpy4(pc[a], tap, dc[a]);
+                if (c.valid && k < g.nz) o_p.put((k * g.sxy + c.off) >> 2, axpy4(pc[a], tap, dc[a]));
             }
         }
         return;
@@ -165,9 +168,6 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
     // ---- w' = A r', d, p and the partials of r'.r', r'.w' ----------------
     const int srcL = (c.lane + 63) & 63, srcR = (c.lane + 1) & 63;
     double acc[2] = {0.0, 0.0};
-    float4 *ro4 = reinterpret_cast<float4 *>(r_out);
-    float4 *wo4 = reinterpret_cast<float4 *>(w_out);
-    float4 *so4 = reinterpret_cast<float4 *>(s_out);
 #pragma unroll
     for (int a = 0; a < ZU; ++a) {
         const int k = c.k0 + a;
@@ -186,11 +186,11 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
                 acc[1] += rv * (double)q[e];
             }
             const int id4 = (k * g.sxy + c.off) >> 2;
-            st4<NT>(ro4 + id4, rn[a + 1]);
-            st4<NT>(wo4 + id4, wn);
-            st4<NT>(so4 + id4, sn[a]);
-            st4<NT>(d4 + id4, dn);
-            if (PUPD) st4<NT>(p4 + id4, axpy4(axpy4(pc[a], tap, dc[a]), ta, dn));
+            o_r.put(id4, rn[a + 1]);
+            o_w.put(id4, wn);
+            o_s.put(id4, sn[a]);
+            o_d.put(id4, dn);
+            if (PUPD) o_p.put(id4, axpy4(axpy4(pc[a], tap, dc[a]), ta, dn));
         }
     }
     block_sum<2, false>(acc, lds_out);

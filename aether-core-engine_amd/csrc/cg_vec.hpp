@@ -78,6 +78,38 @@ __device__ __forceinline__ void st4(float4 *p, const float4 &v) {
     }
 }
 
+// Output array of a four-wide kernel under one store policy.  ST 0: plain
+// stores; 1: non-temporal; 2: write-through, through a buffer resource with
+// the sc1 cache-policy bit, so that the stores leave no dirty lines in L2 for
+// the end-of-kernel write-back (measured on MI355X: the one-launch CG
+// iteration 28.2 -> 22.1 us per launch pair at 128^3).  ST 2 needs the array
+// under 2 GiB (32-bit byte offsets); the buffer's range check drops any store
+// past nbytes.
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+constexpr int kStPlain = 0, kStNT = 1, kStWT = 2;
+template <int ST>
+struct Out4 {
+    float4 *p;
+    __amdgpu_buffer_rsrc_t rs;
+    __device__ __forceinline__ Out4(float *ptr, long nbytes) : p(reinterpret_cast<float4 *>(ptr)) {
+        if constexpr (ST == kStWT)
+            rs = __builtin_amdgcn_make_buffer_rsrc(ptr, (short)0, (int)nbytes, 0x00020000);
+    }
+    __device__ __forceinline__ void put(int id4, const float4 &v) const {
+        if constexpr (ST == kStWT) {
+            __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4_t, v), rs, id4 * 16, 0,
+                                                   16 /* sc1 */);
+        } else if constexpr (ST == kStNT) {
+            __builtin_nontemporal_store(v.x, &p[id4].x);
+            __builtin_nontemporal_store(v.y, &p[id4].y);
+            __builtin_nontemporal_store(v.z, &p[id4].z);
+            __builtin_nontemporal_store(v.w, &p[id4].w);
+        } else {
+            p[id4] = v;
+        }
+    }
+};
+
 __device__ __forceinline__ float f4(const float4 &v, int e) {
     return e == 0 ? v.x : e == 1 ? v.y : e == 2 ? v.z : v.w;
 }

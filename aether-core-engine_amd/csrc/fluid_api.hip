@@ -643,6 +643,8 @@ class Solver final : public SolverBase {
     // between partB_ (even it: written by the initialisation) and partA_,
     // delta partials between partE_ and partF_, r between r_ and r2_.
     int fused_nb() const { return tl4_.nb > 0 ? tl4_.nb : tlf_.nb; }
+    // write-through stores address an FP32 array through 32-bit byte offsets
+    bool wt_ok() const { return n_ * 4 < (size_t(1) << 31); }
 
     void enqueue_cg_fused(int it, double rel2, int solve_id, int step_index) {
         const int nb_in = it == 0 ? tl_.nb : fused_nb();
@@ -677,19 +679,23 @@ class Solver final : public SolverBase {
                                 &w_out, &s_in, &s_out, &dd, &pp, &dm, &sid};
                 const bool wide = std::max(nb_in, vt.nb) > 4 * kThreads;
                 const bool odd = it & 1;  // p is updated on odd launches
+                // write-through stores (arrays under 2 GiB), unless
+                // AETHER_CG_NT asks for non-temporal ones
+                const int pol = nt_ ? kStNT : wt_ok() ? kStWT : kStPlain;
+#define CGS4_PICK(NQ_, ST_)                                                            \
+    (it == 0 ? (const void *)k_cg_cgs4<4, true, NQ_, ST_, false>                       \
+     : odd   ? (const void *)k_cg_cgs4<4, false, NQ_, ST_, true>                       \
+             : (const void *)k_cg_cgs4<4, false, NQ_, ST_, false>)
                 const void *fn;
                 if (wide)
-                    fn = it == 0 ? (const void *)k_cg_cgs4<4, true, 16, false, false>
-                         : odd   ? (const void *)k_cg_cgs4<4, false, 16, false, true>
-                                 : (const void *)k_cg_cgs4<4, false, 16, false, false>;
-                else if (nt_)
-                    fn = it == 0 ? (const void *)k_cg_cgs4<4, true, 4, true, false>
-                         : odd   ? (const void *)k_cg_cgs4<4, false, 4, true, true>
-                                 : (const void *)k_cg_cgs4<4, false, 4, true, false>;
+                    fn = pol == kStWT ? CGS4_PICK(16, kStWT) : CGS4_PICK(16, kStPlain);
+                else if (pol == kStNT)
+                    fn = CGS4_PICK(4, kStNT);
+                else if (pol == kStWT)
+                    fn = CGS4_PICK(4, kStWT);
                 else
-                    fn = it == 0 ? (const void *)k_cg_cgs4<4, true, 4, false, false>
-                         : odd   ? (const void *)k_cg_cgs4<4, false, 4, false, true>
-                                 : (const void *)k_cg_cgs4<4, false, 4, false, false>;
+                    fn = CGS4_PICK(4, kStPlain);
+#undef CGS4_PICK
                 HIPT(hipExtLaunchKernel(fn, dim3(vt.nb), dim3(kThreads), args, 0, st_, e0, e1, 0));
                 return;
             }
