@@ -192,7 +192,7 @@ __global__ __launch_bounds__(kThreads) void kb_cg_dir4(const BInst<float> *__res
                                                        int solve_id) {
     const BItem it = items[blockIdx.x];
     const BInst<float> &in = I[it.inst];
-    body_cg_dir4<4, FIRST, NQ, false>(vcol_at<4>(in.g, in.vt, it.bx, it.by, it.bz), it.local,
+    body_cg_dir4<4, FIRST, NQ, kStWT>(vcol_at<4>(in.g, in.vt, it.bx, it.by, it.bz), it.local,
                                       in.g, iter, rel_tol2, FIRST ? in.nb : in.nbv, in.pb, in.pc,
                                       in.st, in.r, in.d[(iter + 1) & 1], in.d[iter & 1], in.pa,
                                       done_mirror + it.inst, solve_id);
@@ -205,7 +205,7 @@ __global__ __launch_bounds__(kThreads) void kb_cg_update4(const BInst<float> *__
                                                           int solve_id) {
     const BItem it = items[blockIdx.x];
     const BInst<float> &in = I[it.inst];
-    body_cg_update4<4, NQ, false>(vcol_at<4>(in.g, in.vt, it.bx, it.by, it.bz), it.local, in.nbv,
+    body_cg_update4<4, NQ, kStWT>(vcol_at<4>(in.g, in.vt, it.bx, it.by, it.bz), it.local, in.nbv,
                                   in.g, iter, in.pa, in.st, in.d[iter & 1], in.p, in.r, in.pb,
                                   done_mirror + it.inst, solve_id);
 }

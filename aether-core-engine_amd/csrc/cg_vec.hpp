@@ -63,28 +63,15 @@ __device__ __forceinline__ VCol vcol_at(const Dims &g, const VTiling &vt, int bx
     return c;
 }
 
-// Store of a CG vector, non-temporal when NT.  Measured on MI355X: NT is
-// ~4% faster in an isolated 128^3 CG loop but ~3% slower inside the full
-// step, and ~7% slower at 256^3; plain stores are the default.
-template <bool NT>
-__device__ __forceinline__ void st4(float4 *p, const float4 &v) {
-    if constexpr (NT) {
-        __builtin_nontemporal_store(v.x, &p->x);
-        __builtin_nontemporal_store(v.y, &p->y);
-        __builtin_nontemporal_store(v.z, &p->z);
-        __builtin_nontemporal_store(v.w, &p->w);
-    } else {
-        *p = v;
-    }
-}
-
 // Output array of a four-wide kernel under one store policy.  ST 0: plain
 // stores; 1: non-temporal; 2: write-through, through a buffer resource with
 // the sc1 cache-policy bit, so that the stores leave no dirty lines in L2 for
-// the end-of-kernel write-back (measured on MI355X: the one-launch CG
-// iteration 28.2 -> 22.1 us per launch pair at 128^3).  ST 2 needs the array
-// under 2 GiB (32-bit byte offsets); the buffer's range check drops any store
-// past nbytes.
+// the end-of-kernel write-back.  Measured on MI355X at 128^3: write-through
+// takes the one-launch CG iteration from 28.2 to 22.1 us per launch pair;
+// non-temporal stores are slower inside the step (they also bypass the
+// Infinity Cache that the next launch reads from).  ST 2 needs the array
+// under 2 GiB (32-bit byte offsets); the buffer's range check drops any
+// store past nbytes.
 typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
 constexpr int kStPlain = 0, kStNT = 1, kStWT = 2;
 template <int ST>
@@ -142,7 +129,7 @@ __device__ __forceinline__ void lap4(const Dims &g, const VCol &c, int k, const 
 // thread held in registers (256 NQ >= nb keeps their reload one round trip).
 // `slot` is the workgroup's partial-sum slot (0 leads), nb_in the number of
 // partials the previous launch wrote.
-template <int ZU, bool FIRST, int NQ, bool NT>
+template <int ZU, bool FIRST, int NQ, int ST>
 __device__ __forceinline__ void body_cg_dir4(
     const VCol &c, int slot, Dims g, int it, double rel_tol2, int nb_in,
     const double *__restrict__ part_rr, const double *__restrict__ part_bb, CgState *st,
@@ -228,7 +215,7 @@ __device__ __forceinline__ void body_cg_dir4(
     for (int a = 0; a < ZU + 2; ++a) dc[a] = FIRST ? rz[a] : axpy4(rz[a], tb, dz[a]);
     const int srcL = (c.lane + 63) & 63, srcR = (c.lane + 1) & 63;
     double acc[1] = {0.0};
-    float4 *dn4 = reinterpret_cast<float4 *>(dnew);
+    const Out4<ST> o_d(dnew, (long)g.sxy * g.nz * 4);
 #pragma unroll
     for (int a = 0; a < ZU; ++a) {
         const int k = c.k0 + a;
@@ -242,14 +229,14 @@ __device__ __forceinline__ void body_cg_dir4(
             lap4(g, c, k, dc[a + 1], xl, xrv, ym, yp, dc[a], dc[a + 2], q);
 #pragma unroll
             for (int e = 0; e < 4; ++e) acc[0] += (double)f4(dc[a + 1], e) * (double)q[e];
-            st4<NT>(dn4 + ((k * g.sxy + c.off) >> 2), dc[a + 1]);
+            o_d.put((k * g.sxy + c.off) >> 2, dc[a + 1]);
         }
     }
     block_sum<1>(acc, lds);
     if (threadIdx.x == 0) part_dq[slot] = acc[0];
 }
 
-template <int ZU, bool FIRST, int NQ, bool NT>
+template <int ZU, bool FIRST, int NQ, int ST>
 __global__ __launch_bounds__(kThreads) void k_cg_dir4(
     Dims g, VTiling vt, int it, double rel_tol2, int nb_in, const double *__restrict__ part_rr,
     const double *__restrict__ part_bb, CgState *st, const float *__restrict__ r,
@@ -257,14 +244,14 @@ __global__ __launch_bounds__(kThreads) void k_cg_dir4(
     int *done_mirror, int solve_id) {
     int bx, by, bz;
     vtile_coords(vt, bx, by, bz);
-    body_cg_dir4<ZU, FIRST, NQ, NT>(vcol_at<ZU>(g, vt, bx, by, bz), blockIdx.x, g, it, rel_tol2,
+    body_cg_dir4<ZU, FIRST, NQ, ST>(vcol_at<ZU>(g, vt, bx, by, bz), blockIdx.x, g, it, rel_tol2,
                                     nb_in, part_rr, part_bb, st, r, dold, dnew, part_dq,
                                     done_mirror, solve_id);
 }
 
 // CG launch 2 of iteration `it` (as body_cg_update): alpha from d.q,
 // p += alpha d, r -= alpha A d (A d recomputed), partial r.r.
-template <int ZU, int NQ, bool NT>
+template <int ZU, int NQ, int ST>
 __device__ __forceinline__ void body_cg_update4(
     const VCol &c, int slot, int nb, Dims g, int it, const double *__restrict__ part_dq,
     CgState *st, const float *__restrict__ d, float *__restrict__ p, float *__restrict__ r,
@@ -278,6 +265,7 @@ __device__ __forceinline__ void body_cg_update4(
     const float4 *d4 = reinterpret_cast<const float4 *>(d);
     float4 *p4 = reinterpret_cast<float4 *>(p);
     float4 *r4 = reinterpret_cast<float4 *>(r);
+    const Out4<ST> o_p(p, (long)g.sxy * g.nz * 4), o_r(r, (long)g.sxy * g.nz * 4);
     float4 dz[ZU + 2], dy[ZU][2], pc[ZU], rc[ZU];
     float xd[ZU][2];
 #pragma unroll
@@ -330,8 +318,8 @@ __device__ __forceinline__ void body_cg_update4(
             acc[0] += (double)rn.z * (double)rn.z;
             acc[0] += (double)rn.w * (double)rn.w;
             const int id4 = (k * g.sxy + c.off) >> 2;
-            st4<NT>(p4 + id4, pn);
-            st4<NT>(r4 + id4, rn);
+            o_p.put(id4, pn);
+            o_r.put(id4, rn);
         }
     }
     block_sum<1>(acc, lds);
@@ -340,14 +328,14 @@ __device__ __forceinline__ void body_cg_update4(
 
 // nb_in: the number of d.q partials to sum (vt.nb, or 1 when a separate
 // kernel has already reduced them to one total).
-template <int ZU, int NQ, bool NT>
+template <int ZU, int NQ, int ST>
 __global__ __launch_bounds__(kThreads) void k_cg_update4(
     Dims g, VTiling vt, int it, int nb_in, const double *__restrict__ part_dq, CgState *st,
     const float *__restrict__ d, float *__restrict__ p, float *__restrict__ r,
     double *__restrict__ part_rr, int *done_mirror, int solve_id) {
     int bx, by, bz;
     vtile_coords(vt, bx, by, bz);
-    body_cg_update4<ZU, NQ, NT>(vcol_at<ZU>(g, vt, bx, by, bz), blockIdx.x, nb_in, g, it,
+    body_cg_update4<ZU, NQ, ST>(vcol_at<ZU>(g, vt, bx, by, bz), blockIdx.x, nb_in, g, it,
                                 part_dq, st, d, p, r, part_rr, done_mirror, solve_id);
 }
 
@@ -356,14 +344,15 @@ __global__ __launch_bounds__(kThreads) void k_cg_update4(
 // have the other colour, which this pass leaves unchanged, so every thread
 // reads consistent values and writes back its whole float4 (the other-colour
 // cells with their unchanged bits).
-template <int ZU>
+template <int ZU, int ST>
 __global__ __launch_bounds__(kThreads) void k_rbgs4(Dims g, VTiling vt, int color, float omega,
                                                     const float *__restrict__ b,
                                                     float *__restrict__ p) {
     int bx, by, bz;
     vtile_coords(vt, bx, by, bz);
     const VCol c = vcol_at<ZU>(g, vt, bx, by, bz);
-    float4 *p4 = reinterpret_cast<float4 *>(p);
+    const float4 *p4 = reinterpret_cast<const float4 *>(p);
+    const Out4<ST> o_p(p, (long)g.sxy * g.nz * 4);
     const float4 *b4 = reinterpret_cast<const float4 *>(b);
     float4 pz[ZU + 2], py[ZU][2], bc[ZU];
     float xl[ZU], xr[ZU];
@@ -407,7 +396,7 @@ __global__ __launch_bounds__(kThreads) void k_rbgs4(Dims g, VTiling vt, int colo
                 const float gs = (f4(bc[a], e) + nb) / diag;
                 o[e] = (1.0f - omega) * o[e] + omega * gs;
             }
-            if (any) p4[(k * g.sxy + c.off) >> 2] = make_float4(o[0], o[1], o[2], o[3]);
+            if (any) o_p.put((k * g.sxy + c.off) >> 2, make_float4(o[0], o[1], o[2], o[3]));
         }
     }
 }

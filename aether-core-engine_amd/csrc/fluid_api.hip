@@ -915,20 +915,29 @@ class Solver final : public SolverBase {
                 pbb = tot_.p + 1;
                 nbv = 1;
             }
+            const bool wt = !nt_ && wt_ok();  // write-through stores (see Out4)
             VZU_DISPATCH(vzu_, ({
                 const void *f1, *f2;
-                if (wide) {
-                    f1 = it == 0 ? (const void *)k_cg_dir4<VZU, true, 1, false>
-                                 : (const void *)k_cg_dir4<VZU, false, 1, false>;
-                    f2 = (const void *)k_cg_update4<VZU, 16, false>;
+                if (wide && wt) {
+                    f1 = it == 0 ? (const void *)k_cg_dir4<VZU, true, 1, kStWT>
+                                 : (const void *)k_cg_dir4<VZU, false, 1, kStWT>;
+                    f2 = (const void *)k_cg_update4<VZU, 16, kStWT>;
+                } else if (wide) {
+                    f1 = it == 0 ? (const void *)k_cg_dir4<VZU, true, 1, kStPlain>
+                                 : (const void *)k_cg_dir4<VZU, false, 1, kStPlain>;
+                    f2 = (const void *)k_cg_update4<VZU, 16, kStPlain>;
                 } else if (nt_) {
-                    f1 = it == 0 ? (const void *)k_cg_dir4<VZU, true, 4, true>
-                                 : (const void *)k_cg_dir4<VZU, false, 4, true>;
-                    f2 = (const void *)k_cg_update4<VZU, 4, true>;
+                    f1 = it == 0 ? (const void *)k_cg_dir4<VZU, true, 4, kStNT>
+                                 : (const void *)k_cg_dir4<VZU, false, 4, kStNT>;
+                    f2 = (const void *)k_cg_update4<VZU, 4, kStNT>;
+                } else if (wt) {
+                    f1 = it == 0 ? (const void *)k_cg_dir4<VZU, true, 4, kStWT>
+                                 : (const void *)k_cg_dir4<VZU, false, 4, kStWT>;
+                    f2 = (const void *)k_cg_update4<VZU, 4, kStWT>;
                 } else {
-                    f1 = it == 0 ? (const void *)k_cg_dir4<VZU, true, 4, false>
-                                 : (const void *)k_cg_dir4<VZU, false, 4, false>;
-                    f2 = (const void *)k_cg_update4<VZU, 4, false>;
+                    f1 = it == 0 ? (const void *)k_cg_dir4<VZU, true, 4, kStPlain>
+                                 : (const void *)k_cg_dir4<VZU, false, 4, kStPlain>;
+                    f2 = (const void *)k_cg_update4<VZU, 4, kStPlain>;
                 }
                 void *a1[] = {&g, &vt, &itv, &rt, &nbv, &prr, &pbb, &st, &r, &dold, &dnew, &pdq,
                               &dm, &sid};
@@ -1057,9 +1066,14 @@ class Solver final : public SolverBase {
             if constexpr (sizeof(T) == 4) {
                 for (int it = 0; it < sweeps; ++it)
                     for (int color = 0; color < 2; ++color) {
-                        VZU_DISPATCH(vzu_, (k_rbgs4<VZU><<<tlv_.nb, kThreads, 0, st_>>>(
-                                               g_, tlv_, color, (float)params.sor_omega, b_.p,
-                                               p_.p)));
+                        if (wt_ok())
+                            VZU_DISPATCH(vzu_, (k_rbgs4<VZU, kStWT><<<tlv_.nb, kThreads, 0, st_>>>(
+                                                   g_, tlv_, color, (float)params.sor_omega, b_.p,
+                                                   p_.p)));
+                        else
+                            VZU_DISPATCH(vzu_, (k_rbgs4<VZU, kStPlain><<<tlv_.nb, kThreads, 0, st_>>>(
+                                                   g_, tlv_, color, (float)params.sor_omega, b_.p,
+                                                   p_.p)));
                         LAUNCH_CHECK();
                     }
             }
@@ -1810,7 +1824,10 @@ class Batch final : public BatchBase {
         std::vector<int2> src;
         // four-wide CG: FP32 and every nx % 4 == 0
         x4_ok_ = sizeof(T) == 4;
-        for (int q = 0; q < count; ++q) x4_ok_ = x4_ok_ && desc[q].nx % 4 == 0;
+        // (its kernels store write-through, which addresses arrays under 2 GiB)
+        for (int q = 0; q < count; ++q)
+            x4_ok_ = x4_ok_ && desc[q].nx % 4 == 0 &&
+                     (size_t)desc[q].nx * desc[q].ny * desc[q].nz * 4 < (size_t(1) << 31);
         for (int q = 0; q < count; ++q) {
             auto in = std::make_unique<Inst>();
             const aether_fluid_instance_desc &d = desc[q];
