@@ -97,7 +97,12 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
     }
 
     // ---- scalars (every workgroup reaches the same decision) -------------
-    if (threadIdx.x == 0) flag_lds[0] = flag_reg;
+    // The solve's stop word holds (stopping launch + 1).  Only launches after
+    // the stopping one take the early exit: a workgroup of the stopping
+    // launch may already see that launch's word (written mid-launch by
+    // workgroup 0), and it must still reach the stop below, which applies
+    // the pending p term on odd launches.
+    if (threadIdx.x == 0) flag_lds[0] = flag_reg != 0 && flag_reg <= it;
     double tot[2] = {eg.local(pg_in, nb_in), ed.local(pd_in, nb_in)};
     block_sum<2, false>(tot, lds);  // its barrier publishes flag_lds
     if (flag_lds[0]) return;
@@ -120,7 +125,7 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
     if (!stop && fabs(mu) < 1e-28) stop = true;
     if (stop) {
         if (lead) {
-            st->done = 1;
+            st->done = it + 1;
             st->iters = it;
             st->r2_final = gam;
             st->converged = 1;

@@ -37,7 +37,8 @@ struct CgState {
     double b2;        // ||b||^2
     double r2_init;   // ||b - A p0||^2
     double r2_final;  // recursive residual at exit
-    int done;
+    int done;  // nonzero once the solve has stopped (the one-launch four-wide
+               // form stores its stopping launch + 1)
     int iters;
     int converged;
     int pad;

@@ -425,6 +425,24 @@ def test_one_launch_cg_pending_p_update_vs_oracle(iters, tol):
     assert max(errs.values()) <= 1e-5, errs
 
 
+@pytest.mark.parametrize("tol", [1e-2, 5e-3, 3e-3])
+def test_one_launch_cg_stop_with_two_rounds_of_tiles(tol):
+    """128x128x160 (its default CG form): 640 tiles, more than are resident
+    at once, so some workgroups of the stopping launch start after
+    workgroup 0 has written the stop. They must still apply the pending p
+    term. The oracle stops after 21, 27 and 33 iterations (odd launches)."""
+    _announce(f"one-launch CG stop, two rounds of tiles, tol={tol}")
+    dims = (128, 128, 160)
+    gpu, orc = make_pair(*dims, precision="fp32", state=random_state(*dims, seed=21))
+    set_knobs(gpu, orc, iters=200, rel_tol=tol)
+    gpu.project()
+    orc.project()
+    so, sg = orc.stats(), gpu.last_solve_stats()
+    assert abs(sg["iters"] - so["iters"]) <= 1, (sg, so)
+    errs = compare_fields(gpu, orc, ("u", "v", "w", "pressure"))
+    assert max(errs.values()) <= 1e-5, errs
+
+
 def test_one_launch_cg_segment_edges_vs_oracle():
     """The one-launch four-wide CG where rows span two 64-lane segments
     (320 cells: 64 + 16 float4 lanes), so x-neighbours across every wave
