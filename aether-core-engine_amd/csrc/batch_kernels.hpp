@@ -185,29 +185,56 @@ __global__ __launch_bounds__(kThreads) void kb_cg_update(const BInst<T> *__restr
 // the items list each instance's four-wide tiles in the order of the
 // single-grid launch, so `local` is the same partial-sum slot and a batch
 // computes bit for bit what the instances compute one at a time.
-template <bool FIRST, int NQ>
+// PEND / PM: p is brought up to date on odd iterations only, as in the
+// single-grid four-wide CG (same arithmetic, so batches stay bit-identical
+// to the instances run one at a time).
+template <bool FIRST, int NQ, bool PEND>
 __global__ __launch_bounds__(kThreads) void kb_cg_dir4(const BInst<float> *__restrict__ I,
                                                        const BItem *__restrict__ items, int iter,
                                                        double rel_tol2, int *done_mirror,
                                                        int solve_id) {
     const BItem it = items[blockIdx.x];
     const BInst<float> &in = I[it.inst];
-    body_cg_dir4<4, FIRST, NQ, kStWT>(vcol_at<4>(in.g, in.vt, it.bx, it.by, it.bz), it.local,
-                                      in.g, iter, rel_tol2, FIRST ? in.nb : in.nbv, in.pb, in.pc,
-                                      in.st, in.r, in.d[(iter + 1) & 1], in.d[iter & 1], in.pa,
-                                      done_mirror + it.inst, solve_id);
+    body_cg_dir4<4, FIRST, NQ, kStWT, PEND>(vcol_at<4>(in.g, in.vt, it.bx, it.by, it.bz),
+                                            it.local, in.g, iter, rel_tol2,
+                                            FIRST ? in.nb : in.nbv, in.pb, in.pc, in.st, in.r,
+                                            in.d[(iter + 1) & 1], in.d[iter & 1], in.pa,
+                                            done_mirror + it.inst, solve_id, in.p);
 }
 
-template <int NQ>
+template <int NQ, int PM>
 __global__ __launch_bounds__(kThreads) void kb_cg_update4(const BInst<float> *__restrict__ I,
                                                           const BItem *__restrict__ items,
                                                           int iter, int *done_mirror,
                                                           int solve_id) {
     const BItem it = items[blockIdx.x];
     const BInst<float> &in = I[it.inst];
-    body_cg_update4<4, NQ, kStWT>(vcol_at<4>(in.g, in.vt, it.bx, it.by, it.bz), it.local, in.nbv,
-                                  in.g, iter, in.pa, in.st, in.d[iter & 1], in.p, in.r, in.pb,
-                                  done_mirror + it.inst, solve_id);
+    body_cg_update4<4, NQ, kStWT, PM>(vcol_at<4>(in.g, in.vt, it.bx, it.by, it.bz), it.local,
+                                      in.nbv, in.g, iter, in.pa, in.st, in.d[iter & 1], in.p,
+                                      in.r, in.pb, done_mirror + it.inst, solve_id,
+                                      in.d[(iter + 1) & 1]);
+}
+
+// Closes, per instance, a four-wide solve that an odd iteration cap ended
+// after an even launch: p += alpha_last d_last over the instance's tiles.
+__global__ __launch_bounds__(kThreads) void kb_cg_p_tail(const BInst<float> *__restrict__ I,
+                                                         const BItem *__restrict__ items,
+                                                         int last_it) {
+    const BItem it = items[blockIdx.x];
+    const BInst<float> &in = I[it.inst];
+    if (in.st->done) return;
+    const VCol c = vcol_at<4>(in.g, in.vt, it.bx, it.by, it.bz);
+    const float ta = (float)in.st->cg_alpha[last_it & 1];
+    float4 *p4 = reinterpret_cast<float4 *>(in.p);
+    const float4 *d4 = reinterpret_cast<const float4 *>(in.d[last_it & 1]);
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        const int k = c.k0 + a;
+        if (c.valid && k < in.g.nz) {
+            const int id4 = (k * in.g.sxy + c.off) >> 2;
+            p4[id4] = axpy4(p4[id4], ta, d4[id4]);
+        }
+    }
 }
 
 // One workgroup per instance: closes solves that ran out of iterations and

@@ -20,7 +20,7 @@
 // with d_(it-1) the value this launch reads anyway, which is the same two
 // FP32 additions in the same order as updating it every launch.  A launch
 // that stops an odd iteration applies the pending term; when the iteration
-// cap ends the solve after an even launch, k_cg_cgs4_tail does.  9 array
+// cap ends the solve after an even launch, k_cg_p_tail does.  9 array
 // passes per iteration on average (10 on odd launches, 8 on even ones).
 #pragma once
 #include "cg_vec.hpp"
@@ -203,18 +203,6 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
         pg_out[blockIdx.x] = acc[0];
         pd_out[blockIdx.x] = acc[1];
     }
-}
-
-// Closes a solve whose iteration cap ended it after an even launch (the
-// last launch left alpha_(it) d_(it) pending): p += alpha d.  Skipped when
-// a launch stopped the solve (that launch applied the pending term).
-__global__ __launch_bounds__(kThreads) void k_cg_cgs4_tail(const CgState *st, int last_it,
-                                                           const float4 *__restrict__ d,
-                                                           float4 *__restrict__ p, long n4) {
-    if (st->done) return;
-    const float ta = (float)st->cg_alpha[last_it & 1];
-    for (long a = (long)blockIdx.x * kThreads + threadIdx.x; a < n4; a += (long)gridDim.x * kThreads)
-        p[a] = axpy4(p[a], ta, d[a]);
 }
 
 }  // namespace aether

@@ -129,12 +129,12 @@ __device__ __forceinline__ void lap4(const Dims &g, const VCol &c, int k, const 
 // thread held in registers (256 NQ >= nb keeps their reload one round trip).
 // `slot` is the workgroup's partial-sum slot (0 leads), nb_in the number of
 // partials the previous launch wrote.
-template <int ZU, bool FIRST, int NQ, int ST>
+template <int ZU, bool FIRST, int NQ, int ST, bool PEND>
 __device__ __forceinline__ void body_cg_dir4(
     const VCol &c, int slot, Dims g, int it, double rel_tol2, int nb_in,
     const double *__restrict__ part_rr, const double *__restrict__ part_bb, CgState *st,
     const float *__restrict__ r, const float *__restrict__ dold, float *__restrict__ dnew,
-    double *__restrict__ part_dq, int *done_mirror, int solve_id) {
+    double *__restrict__ part_dq, int *done_mirror, int solve_id, float *__restrict__ p) {
     __shared__ double lds[8];
     __shared__ int flag_lds[1];
     const int flag_reg = st->done;
@@ -171,10 +171,12 @@ __device__ __forceinline__ void body_cg_dir4(
             if (!FIRST) xd[a][1] = dold[base + c.off + 4];
         }
     }
-    // stopping rules (every workgroup reaches the same decision)
+    // stopping rules (every workgroup reaches the same decision); the stop
+    // word holds the stopping launch + 1 (launch 2 it here), so only later
+    // launches take the early exit
     const bool lead = slot == 0 && threadIdx.x == 0;
     double beta = 0.0;
-    if (threadIdx.x == 0) flag_lds[0] = flag_reg;
+    if (threadIdx.x == 0) flag_lds[0] = flag_reg != 0 && flag_reg <= 2 * it;
     const double rr = prr.total(part_rr, nb_in, lds);  // its barrier publishes flag_lds
     if (flag_lds[0]) return;
     if (FIRST) {
@@ -185,7 +187,7 @@ __device__ __forceinline__ void body_cg_dir4(
             st->r2_init = rr;
             st->delta[0] = rr;
             if (stop) {
-                st->done = 1;
+                st->done = 2 * it + 1;
                 st->iters = 0;
                 st->r2_final = rr;
                 st->converged = 1;
@@ -197,7 +199,7 @@ __device__ __forceinline__ void body_cg_dir4(
         const bool stop = rr < 1e-20 || (rel_tol2 > 0.0 && rr <= rel_tol2 * b2_prev);
         if (lead) {
             if (stop) {
-                st->done = 1;
+                st->done = 2 * it + 1;
                 st->iters = it;
                 st->r2_final = rr;
                 st->converged = 1;
@@ -206,7 +208,21 @@ __device__ __forceinline__ void body_cg_dir4(
                 st->delta[it & 1] = rr;
             }
         }
-        if (stop) return;
+        if (stop) {
+            if (PEND) {  // p += alpha_(it-1) d_(it-1), left pending by update launch it - 1
+                const float tap = (float)st->cg_alpha[(it - 1) & 1];
+                float4 *p4 = reinterpret_cast<float4 *>(p);
+#pragma unroll
+                for (int a = 0; a < ZU; ++a) {
+                    const int k = c.k0 + a;
+                    if (c.valid && k < g.nz) {
+                        const int id4 = (k * g.sxy + c.off) >> 2;
+                        p4[id4] = axpy4(p4[id4], tap, dz[a + 1]);
+                    }
+                }
+            }
+            return;
+        }
         beta = rr / delta_prev;
     }
     const float tb = (float)beta;
@@ -236,26 +252,34 @@ __device__ __forceinline__ void body_cg_dir4(
     if (threadIdx.x == 0) part_dq[slot] = acc[0];
 }
 
-template <int ZU, bool FIRST, int NQ, int ST>
+template <int ZU, bool FIRST, int NQ, int ST, bool PEND>
 __global__ __launch_bounds__(kThreads) void k_cg_dir4(
     Dims g, VTiling vt, int it, double rel_tol2, int nb_in, const double *__restrict__ part_rr,
     const double *__restrict__ part_bb, CgState *st, const float *__restrict__ r,
     const float *__restrict__ dold, float *__restrict__ dnew, double *__restrict__ part_dq,
-    int *done_mirror, int solve_id) {
+    int *done_mirror, int solve_id, float *__restrict__ p) {
     int bx, by, bz;
     vtile_coords(vt, bx, by, bz);
-    body_cg_dir4<ZU, FIRST, NQ, ST>(vcol_at<ZU>(g, vt, bx, by, bz), blockIdx.x, g, it, rel_tol2,
-                                    nb_in, part_rr, part_bb, st, r, dold, dnew, part_dq,
-                                    done_mirror, solve_id);
+    body_cg_dir4<ZU, FIRST, NQ, ST, PEND>(vcol_at<ZU>(g, vt, bx, by, bz), blockIdx.x, g, it,
+                                          rel_tol2, nb_in, part_rr, part_bb, st, r, dold, dnew,
+                                          part_dq, done_mirror, solve_id, p);
 }
 
 // CG launch 2 of iteration `it` (as body_cg_update): alpha from d.q,
 // p += alpha d, r -= alpha A d (A d recomputed), partial r.r.
-template <int ZU, int NQ, int ST>
+// PM: how p is brought up to date.  kPEvery: p += alpha d every launch.
+// kPSkip (even it): not at all (alpha_it d_it stays pending).  kPBoth (odd
+// it): p <- (p + alpha_(it-1) d_(it-1)) + alpha_it d_it, with d_(it-1) in
+// dprev (the other d buffer): the same two FP32 additions in the same order
+// as kPEvery, so every mode gives bit-identical p.  The lead stores alpha
+// for the launches and the tail that apply a pending term.
+constexpr int kPEvery = 0, kPSkip = 1, kPBoth = 2;
+template <int ZU, int NQ, int ST, int PM>
 __device__ __forceinline__ void body_cg_update4(
     const VCol &c, int slot, int nb, Dims g, int it, const double *__restrict__ part_dq,
     CgState *st, const float *__restrict__ d, float *__restrict__ p, float *__restrict__ r,
-    double *__restrict__ part_rr, int *done_mirror, int solve_id) {
+    double *__restrict__ part_rr, int *done_mirror, int solve_id,
+    const float *__restrict__ dprev) {
     __shared__ double lds[8];
     __shared__ int flag_lds[1];
     const int flag_reg = st->done;
@@ -266,7 +290,8 @@ __device__ __forceinline__ void body_cg_update4(
     float4 *p4 = reinterpret_cast<float4 *>(p);
     float4 *r4 = reinterpret_cast<float4 *>(r);
     const Out4<ST> o_p(p, (long)g.sxy * g.nz * 4), o_r(r, (long)g.sxy * g.nz * 4);
-    float4 dz[ZU + 2], dy[ZU][2], pc[ZU], rc[ZU];
+    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 dz[ZU + 2], dy[ZU][2], pc[ZU], rc[ZU], dp[ZU];
     float xd[ZU][2];
 #pragma unroll
     for (int a = 0; a < ZU + 2; ++a) {
@@ -278,27 +303,39 @@ __device__ __forceinline__ void body_cg_update4(
         const int base = min(c.k0 + a, g.nz - 1) * g.sxy;
         dy[a][0] = d4[(base + c.offym) >> 2];
         dy[a][1] = d4[(base + c.offyp) >> 2];
-        pc[a] = p4[(base + c.off) >> 2];
+        pc[a] = PM == kPSkip ? z4 : p4[(base + c.off) >> 2];
+        dp[a] = PM == kPBoth ? reinterpret_cast<const float4 *>(dprev)[(base + c.off) >> 2] : z4;
         rc[a] = r4[(base + c.off) >> 2];
         xd[a][0] = xd[a][1] = 0.0f;
         if (c.needL) xd[a][0] = d[base + c.off - 1];
         if (c.needR) xd[a][1] = d[base + c.off + 4];
     }
     const bool lead = slot == 0 && threadIdx.x == 0;
-    if (threadIdx.x == 0) flag_lds[0] = flag_reg;
+    // the stop word holds the stopping launch + 1 (launch 2 it + 1 here)
+    if (threadIdx.x == 0) flag_lds[0] = flag_reg != 0 && flag_reg <= 2 * it + 1;
     const double dq = pdq.total(part_dq, nb, lds);
     if (flag_lds[0]) return;
+    const float tap = PM == kPBoth ? (float)st->cg_alpha[(it - 1) & 1] : 0.0f;
     if (fabs(dq) < 1e-28) {
         if (lead) {
-            st->done = 1;
+            st->done = 2 * it + 2;
             st->iters = it;
             st->converged = 1;
             st->r2_final = delta;
             *done_mirror = solve_id;
         }
+        if (PM == kPBoth) {  // the pending term of launch it - 1
+#pragma unroll
+            for (int a = 0; a < ZU; ++a) {
+                const int k = c.k0 + a;
+                if (c.valid && k < g.nz) o_p.put((k * g.sxy + c.off) >> 2, axpy4(pc[a], tap, dp[a]));
+            }
+        }
         return;
     }
-    const float ta = (float)(delta / dq);
+    const double alpha = delta / dq;
+    if (lead) st->cg_alpha[it & 1] = alpha;
+    const float ta = (float)alpha;
     const int srcL = (c.lane + 63) & 63, srcR = (c.lane + 1) & 63;
     double acc[1] = {0.0};
 #pragma unroll
@@ -310,7 +347,8 @@ __device__ __forceinline__ void body_cg_update4(
         if (c.valid && k < g.nz) {
             float q[4];
             lap4(g, c, k, dz[a + 1], xl, xrv, dy[a][0], dy[a][1], dz[a], dz[a + 2], q);
-            const float4 pn = axpy4(pc[a], ta, dz[a + 1]);
+            const float4 pn = PM == kPBoth ? axpy4(axpy4(pc[a], tap, dp[a]), ta, dz[a + 1])
+                                           : axpy4(pc[a], ta, dz[a + 1]);
             const float4 rn = make_float4(rc[a].x - ta * q[0], rc[a].y - ta * q[1],
                                           rc[a].z - ta * q[2], rc[a].w - ta * q[3]);
             acc[0] += (double)rn.x * (double)rn.x;
@@ -318,7 +356,7 @@ __device__ __forceinline__ void body_cg_update4(
             acc[0] += (double)rn.z * (double)rn.z;
             acc[0] += (double)rn.w * (double)rn.w;
             const int id4 = (k * g.sxy + c.off) >> 2;
-            o_p.put(id4, pn);
+            if (PM != kPSkip) o_p.put(id4, pn);
             o_r.put(id4, rn);
         }
     }
@@ -328,15 +366,28 @@ __device__ __forceinline__ void body_cg_update4(
 
 // nb_in: the number of d.q partials to sum (vt.nb, or 1 when a separate
 // kernel has already reduced them to one total).
-template <int ZU, int NQ, int ST>
+template <int ZU, int NQ, int ST, int PM>
 __global__ __launch_bounds__(kThreads) void k_cg_update4(
     Dims g, VTiling vt, int it, int nb_in, const double *__restrict__ part_dq, CgState *st,
     const float *__restrict__ d, float *__restrict__ p, float *__restrict__ r,
-    double *__restrict__ part_rr, int *done_mirror, int solve_id) {
+    double *__restrict__ part_rr, int *done_mirror, int solve_id,
+    const float *__restrict__ dprev) {
     int bx, by, bz;
     vtile_coords(vt, bx, by, bz);
-    body_cg_update4<ZU, NQ, ST>(vcol_at<ZU>(g, vt, bx, by, bz), blockIdx.x, nb_in, g, it,
-                                part_dq, st, d, p, r, part_rr, done_mirror, solve_id);
+    body_cg_update4<ZU, NQ, ST, PM>(vcol_at<ZU>(g, vt, bx, by, bz), blockIdx.x, nb_in, g, it,
+                                    part_dq, st, d, p, r, part_rr, done_mirror, solve_id, dprev);
+}
+
+// Closes a four-wide solve whose iteration cap ended it after an even launch
+// (that launch left alpha_(it) d_(it) pending): p += alpha d.  Skipped when a
+// launch stopped the solve (that launch applied the pending term).
+__global__ __launch_bounds__(kThreads) void k_cg_p_tail(const CgState *st, int last_it,
+                                                        const float4 *__restrict__ d,
+                                                        float4 *__restrict__ p, long n4) {
+    if (st->done) return;
+    const float ta = (float)st->cg_alpha[last_it & 1];
+    for (long a = (long)blockIdx.x * kThreads + threadIdx.x; a < n4; a += (long)gridDim.x * kThreads)
+        p[a] = axpy4(p[a], ta, d[a]);
 }
 
 // Red-black SOR pass of one colour (src/fluid.rs:400-437) on the four-wide

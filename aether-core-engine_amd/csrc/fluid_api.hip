@@ -916,36 +916,37 @@ class Solver final : public SolverBase {
                 nbv = 1;
             }
             const bool wt = !nt_ && wt_ok();  // write-through stores (see Out4)
+            // p is brought up to date on odd iterations (body_cg_update4)
+            const bool odd = it & 1;
+            const T *dprev = dold;  // d_(it-1), for the odd update
+#define X4_PICK(NQD, NQU, ST_)                                                             \
+    do {                                                                                   \
+        f1 = it == 0 ? (const void *)k_cg_dir4<VZU, true, NQD, ST_, false>                 \
+             : odd   ? (const void *)k_cg_dir4<VZU, false, NQD, ST_, true>                 \
+                     : (const void *)k_cg_dir4<VZU, false, NQD, ST_, false>;               \
+        f2 = odd ? (const void *)k_cg_update4<VZU, NQU, ST_, kPBoth>                       \
+                 : (const void *)k_cg_update4<VZU, NQU, ST_, kPSkip>;                      \
+    } while (0)
             VZU_DISPATCH(vzu_, ({
                 const void *f1, *f2;
-                if (wide && wt) {
-                    f1 = it == 0 ? (const void *)k_cg_dir4<VZU, true, 1, kStWT>
-                                 : (const void *)k_cg_dir4<VZU, false, 1, kStWT>;
-                    f2 = (const void *)k_cg_update4<VZU, 16, kStWT>;
-                } else if (wide) {
-                    f1 = it == 0 ? (const void *)k_cg_dir4<VZU, true, 1, kStPlain>
-                                 : (const void *)k_cg_dir4<VZU, false, 1, kStPlain>;
-                    f2 = (const void *)k_cg_update4<VZU, 16, kStPlain>;
-                } else if (nt_) {
-                    f1 = it == 0 ? (const void *)k_cg_dir4<VZU, true, 4, kStNT>
-                                 : (const void *)k_cg_dir4<VZU, false, 4, kStNT>;
-                    f2 = (const void *)k_cg_update4<VZU, 4, kStNT>;
-                } else if (wt) {
-                    f1 = it == 0 ? (const void *)k_cg_dir4<VZU, true, 4, kStWT>
-                                 : (const void *)k_cg_dir4<VZU, false, 4, kStWT>;
-                    f2 = (const void *)k_cg_update4<VZU, 4, kStWT>;
-                } else {
-                    f1 = it == 0 ? (const void *)k_cg_dir4<VZU, true, 4, kStPlain>
-                                 : (const void *)k_cg_dir4<VZU, false, 4, kStPlain>;
-                    f2 = (const void *)k_cg_update4<VZU, 4, kStPlain>;
-                }
+                if (wide && wt)
+                    X4_PICK(1, 16, kStWT);
+                else if (wide)
+                    X4_PICK(1, 16, kStPlain);
+                else if (nt_)
+                    X4_PICK(4, 4, kStNT);
+                else if (wt)
+                    X4_PICK(4, 4, kStWT);
+                else
+                    X4_PICK(4, 4, kStPlain);
                 void *a1[] = {&g, &vt, &itv, &rt, &nbv, &prr, &pbb, &st, &r, &dold, &dnew, &pdq,
-                              &dm, &sid};
+                              &dm, &sid, &pp};
                 HIPT(hipExtLaunchKernel(f1, dim3(vt.nb), dim3(kThreads), a1, 0, st_, e0, e1, 0));
                 void *a2[] = {&g, &vt, &itv, &nbu, &pdq_in, &st, &dcur, &pp, &rw, &prr_out, &dm,
-                              &sid};
+                              &sid, &dprev};
                 HIPT(hipExtLaunchKernel(f2, dim3(vt.nb), dim3(kThreads), a2, 0, st_, e2, e3, 0));
             }));
+#undef X4_PICK
         } else {
             raise(AETHER_EINVAL, "the four-wide CG is built for FP32 only");
         }
@@ -1033,14 +1034,16 @@ class Solver final : public SolverBase {
             HIPT(hipEventRecord(chunk_events_[c], st_));
         }
         if constexpr (sizeof(T) == 4) {
-            if (form_ == AETHER_CG_FUSED && tl4_.nb > 0 && (max_iters & 1)) {
-                // the cap ended the solve after an even launch of the four-wide
-                // one-launch form: its alpha d term is still pending
+            // the four-wide forms bring p up to date on odd iterations: a cap
+            // that ends the solve after an even one leaves its alpha d pending
+            const bool fused4 = form_ == AETHER_CG_FUSED && tl4_.nb > 0;
+            if ((fused4 || form_ == AETHER_CG_X4) && (max_iters & 1)) {
                 const long n4 = (long)(n_ / 4);
-                k_cg_cgs4_tail<<<(unsigned)std::min<long>((n4 + kThreads - 1) / kThreads, 4096),
-                                 kThreads, 0, st_>>>(cg_.p, max_iters - 1,
-                                                     reinterpret_cast<const float4 *>(d_[0].p),
-                                                     reinterpret_cast<float4 *>(p_.p), n4);
+                const T *dl = fused4 ? d_[0].p : d_[(max_iters - 1) & 1].p;
+                k_cg_p_tail<<<(unsigned)std::min<long>((n4 + kThreads - 1) / kThreads, 4096),
+                              kThreads, 0, st_>>>(cg_.p, max_iters - 1,
+                                                  reinterpret_cast<const float4 *>(dl),
+                                                  reinterpret_cast<float4 *>(p_.p), n4);
                 LAUNCH_CHECK();
             }
         }
@@ -2150,28 +2153,34 @@ class Batch final : public BatchBase {
 
     void launch_cg4(int it, double rel2, int solve_id) {
         if constexpr (sizeof(T) == 4) {
-            if (wide_) {
-                if (it == 0)
-                    kb_cg_dir4<true, 16><<<ncg4_, kThreads, 0, st_>>>(table_.p, cg4_items_.p, it,
-                                                                      rel2, done_dev_, solve_id);
-                else
-                    kb_cg_dir4<false, 16><<<ncg4_, kThreads, 0, st_>>>(
-                        table_.p, cg4_items_.p, it, rel2, done_dev_, solve_id);
-                LAUNCH_CHECK();
-                kb_cg_update4<16><<<ncg4_, kThreads, 0, st_>>>(table_.p, cg4_items_.p, it,
-                                                               done_dev_, solve_id);
-            } else {
-                if (it == 0)
-                    kb_cg_dir4<true, 4><<<ncg4_, kThreads, 0, st_>>>(table_.p, cg4_items_.p, it,
-                                                                     rel2, done_dev_, solve_id);
-                else
-                    kb_cg_dir4<false, 4><<<ncg4_, kThreads, 0, st_>>>(table_.p, cg4_items_.p, it,
-                                                                      rel2, done_dev_, solve_id);
-                LAUNCH_CHECK();
-                kb_cg_update4<4><<<ncg4_, kThreads, 0, st_>>>(table_.p, cg4_items_.p, it,
-                                                              done_dev_, solve_id);
-            }
-            LAUNCH_CHECK();
+            // p is brought up to date on odd iterations (kb_cg_update4)
+            const bool odd = it & 1;
+            const dim3 g(ncg4_), b(kThreads);
+#define KB_DIR4(NQ_)                                                                       \
+    do {                                                                                   \
+        if (it == 0)                                                                       \
+            kb_cg_dir4<true, NQ_, false><<<g, b, 0, st_>>>(table_.p, cg4_items_.p, it,     \
+                                                           rel2, done_dev_, solve_id);     \
+        else if (odd)                                                                      \
+            kb_cg_dir4<false, NQ_, true><<<g, b, 0, st_>>>(table_.p, cg4_items_.p, it,     \
+                                                           rel2, done_dev_, solve_id);     \
+        else                                                                               \
+            kb_cg_dir4<false, NQ_, false><<<g, b, 0, st_>>>(table_.p, cg4_items_.p, it,    \
+                                                            rel2, done_dev_, solve_id);    \
+        LAUNCH_CHECK();                                                                    \
+        if (odd)                                                                           \
+            kb_cg_update4<NQ_, kPBoth><<<g, b, 0, st_>>>(table_.p, cg4_items_.p, it,       \
+                                                         done_dev_, solve_id);             \
+        else                                                                               \
+            kb_cg_update4<NQ_, kPSkip><<<g, b, 0, st_>>>(table_.p, cg4_items_.p, it,       \
+                                                         done_dev_, solve_id);             \
+        LAUNCH_CHECK();                                                                    \
+    } while (0)
+            if (wide_)
+                KB_DIR4(16);
+            else
+                KB_DIR4(4);
+#undef KB_DIR4
         }
     }
 
@@ -2216,6 +2225,12 @@ class Batch final : public BatchBase {
                 LAUNCH_CHECK();
             }
             if (!graph) HIPT(hipEventRecord(chunk_events_[c], st_));
+        }
+        if constexpr (sizeof(T) == 4) {
+            if (x4 && (max_iters & 1)) {  // an odd cap leaves the last alpha d pending
+                kb_cg_p_tail<<<ncg4_, kThreads, 0, st_>>>(table_.p, cg4_items_.p, max_iters - 1);
+                LAUNCH_CHECK();
+            }
         }
         kb_cg_finalize<T><<<count(), blk, 0, st_>>>(table_.p, max_iters, rel2, step, x4 ? 1 : 0,
                                                     graph ? step_dev_.p : nullptr);

@@ -400,20 +400,21 @@ def test_wide_four_wide_cg_matches_scalar_cg():
         assert rel_l2(fd[n], fc[n]) <= 1e-5, (n, rel_l2(fd[n], fc[n]))
 
 
+@pytest.mark.parametrize("form", ["fused", "x4"])
 @pytest.mark.parametrize("iters,tol", [(25, 0.0), (200, 3e-2), (200, 1e-2), (200, 3e-3),
                                        (200, 1e-3)])
-def test_one_launch_cg_pending_p_update_vs_oracle(iters, tol):
-    """The one-launch four-wide CG brings p up to date on odd launches only:
-    an odd iteration cap (25, no tolerance) leaves the last term to the tail
+def test_four_wide_cg_pending_p_update_vs_oracle(form, iters, tol):
+    """Both four-wide CG forms bring p up to date on odd iterations only: an
+    odd iteration cap (25, no tolerance) leaves the last term to the tail
     kernel, and the tolerances stop the oracle after 11, 16, 27 and 43
     iterations (mostly odd, so the stopping launch applies the pending
     term). FP32 bar against the f64 oracle."""
-    _announce(f"one-launch CG pending p update 64x48x40 iters={iters} tol={tol}")
+    _announce(f"four-wide CG ({form}) pending p update 64x48x40 iters={iters} tol={tol}")
     import aether_fluid as af
     dims = (64, 48, 40)
     gpu, orc = make_pair(*dims, precision="fp32", state=random_state(*dims, seed=5))
     set_knobs(gpu, orc, iters=iters, rel_tol=tol)
-    gpu.cg_form = af.CG_FUSED
+    gpu.cg_form = af.CG_FUSED if form == "fused" else af.CG_X4
     gpu.project()
     orc.project()
     so, sg = orc.stats(), gpu.last_solve_stats()
@@ -425,16 +426,19 @@ def test_one_launch_cg_pending_p_update_vs_oracle(iters, tol):
     assert max(errs.values()) <= 1e-5, errs
 
 
+@pytest.mark.parametrize("form", ["fused", "x4"])
 @pytest.mark.parametrize("tol", [1e-2, 5e-3, 3e-3])
-def test_one_launch_cg_stop_with_two_rounds_of_tiles(tol):
-    """128x128x160 (its default CG form): 640 tiles, more than are resident
-    at once, so some workgroups of the stopping launch start after
-    workgroup 0 has written the stop. They must still apply the pending p
-    term. The oracle stops after 21, 27 and 33 iterations (odd launches)."""
-    _announce(f"one-launch CG stop, two rounds of tiles, tol={tol}")
+def test_four_wide_cg_stop_with_two_rounds_of_tiles(form, tol):
+    """128x128x160: 640 four-wide tiles, more than are resident at once for
+    the one-launch form, so some workgroups of the stopping launch start
+    after workgroup 0 has written the stop. They must still apply the
+    pending p term. The oracle stops after 21, 27 and 33 iterations."""
+    _announce(f"four-wide CG ({form}) stop, two rounds of tiles, tol={tol}")
+    import aether_fluid as af
     dims = (128, 128, 160)
     gpu, orc = make_pair(*dims, precision="fp32", state=random_state(*dims, seed=21))
     set_knobs(gpu, orc, iters=200, rel_tol=tol)
+    gpu.cg_form = af.CG_FUSED if form == "fused" else af.CG_X4
     gpu.project()
     orc.project()
     so, sg = orc.stats(), gpu.last_solve_stats()
@@ -686,14 +690,14 @@ def test_batched_graph_replay_equals_stream_launches():
         np.testing.assert_array_equal(x, y)
 
 
-def _batched_vs_single(insts, single_form, case):
+def _batched_vs_single(insts, single_form, case, cap=120, tol=1e-5):
     """A batch of grids of different sizes gives, bit for bit, the fields and
     CG iteration counts of single-grid solvers run one at a time with the
     same CG layout (and 8 z cells per thread in the initialisation tiling)."""
     _announce(case)
     import os
     import aether_fluid as af
-    b = af.FluidBatch(insts, solver_iters=120, cg_rel_tol=1e-5)
+    b = af.FluidBatch(insts, solver_iters=cap, cg_rel_tol=tol)
     states = []
     for q, d in enumerate(insts):
         st = random_state(d["nx"], d["ny"], d["nz"], seed=40 + q, vel=0.5)
@@ -710,7 +714,7 @@ def _batched_vs_single(insts, single_form, case):
             s.vorticity_eps, s.buoyancy_beta, s.gravity_alpha = (d["vorticity_eps"], 1.0,
                                                                  d["gravity_alpha"])
             s.set_sphere_source(d["source_center"], d["source_radius"])
-            s.solver_iters, s.cg_rel_tol = 120, 1e-5
+            s.solver_iters, s.cg_rel_tol = cap, tol
             s.cg_form = single_form
             for n in ("u", "v", "w", "density", "temperature"):
                 s.upload(n, states[q][n])
@@ -753,6 +757,10 @@ def test_batched_four_wide_equals_one_at_a_time_bitwise():
              dict(nx=16, ny=16, nz=40, vorticity_eps=0.1, buoyancy_beta=1.0, gravity_alpha=0.1,
                   source_center=(8.0, 3.0, 20.0), source_radius=2.0)]
     _batched_vs_single(insts, af.CG_X4, "batched vs one-at-a-time (four-wide CG)")
+    # an odd cap with no tolerance ends every solve after an even iteration,
+    # whose alpha d term the batch's tail kernel applies
+    _batched_vs_single(insts, af.CG_X4, "batched vs one-at-a-time (four-wide CG, cap 7)",
+                       cap=7, tol=0.0)
 
 
 def _rollout_setup(n, precision, tol=1e-12, iters=600):
