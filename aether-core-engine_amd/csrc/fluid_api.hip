@@ -347,8 +347,9 @@ class Solver final : public SolverBase {
                 if (z == 2 || z == 4) vzu_ = z;
             }
             tlv_.gz = (nz + vzu_ - 1) / vzu_;
-            // non-temporal CG stores are opt-in (AETHER_CG_NT=1): ~4% faster
-            // in an isolated 128^3 loop, ~3% slower inside the full step
+            // the four-wide CG kernels store write-through by default (Out4);
+            // AETHER_CG_NT=1 asks for non-temporal stores instead (slower in
+            // the step: they also skip the Infinity Cache)
             if (const char *e = std::getenv("AETHER_CG_NT")) nt_ = std::atoi(e) != 0;
             tlv_.nb = tlv_.gx * tlv_.gy * tlv_.gz;
             tlv_.remap = (tlv_.nb % 8 == 0) ? 1 : 0;
