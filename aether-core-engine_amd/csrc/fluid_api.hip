@@ -1743,7 +1743,7 @@ class Solver final : public SolverBase {
     int vzu_ = 4;
     bool nt_ = false;  // non-temporal stores in the four-wide CG
     Tiling tlf_;          // one-launch CG: 64 x 4 x kz_ tiling
-    VTiling tl4_ = {0, 0, 0, 0, 0, 1, 1};  // one-launch four-wide: whole rows (nb 0: unusable)
+    VTiling tl4_ = {0, 0, 0, 0, 0, 1, 1};  // one-launch four-wide CG tiling (nb 0: unusable)
     int kz_ = 16;
     int pf_ = 4;          // planes of loads in flight ahead of use
     DevBuf<T> r2_;                 // second residual buffer of the one-launch CG
