@@ -20,6 +20,7 @@ AETHER_EINVAL = -1
 AETHER_EHIP = -2
 AETHER_ENOMEM = -3
 AETHER_ENODEV = -4
+AETHER_EINTERNAL = -5
 
 FP32, FP64, FP32_IR64 = 0, 1, 2
 SOLVER_CG, SOLVER_RBGS = 0, 1

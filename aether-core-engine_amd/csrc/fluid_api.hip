@@ -112,6 +112,9 @@ int guarded(F &&f) {
     } catch (const std::bad_alloc &) {
         g_last_error = "host allocation failed";
         return AETHER_ENOMEM;
+    } catch (const std::exception &e) {  // nothing may unwind through the C ABI
+        g_last_error = std::string("internal error: ") + e.what();
+        return AETHER_EINTERNAL;
     }
 }
 
@@ -2625,6 +2628,7 @@ int aether_fluid_last_solve_stats(aether_fluid_t s, aether_fluid_solve_stats *ou
 int aether_fluid_last_run_iterations(aether_fluid_t s, int32_t *out, int n, int *count) {
     return guarded([&] {
         checked(s);
+        if (n > 0 && !out) raise(AETHER_EINVAL, "null output array");
         const auto &v = s->impl->last_run_iters();
         if (count) *count = (int)v.size();
         for (int a = 0; a < n && a < (int)v.size(); ++a) out[a] = v[a];
@@ -2810,6 +2814,7 @@ int aether_fluid_batch_run(aether_fluid_batch_t b, double dt, const double *time
 int aether_fluid_batch_iterations(aether_fluid_batch_t b, int32_t *out, int n, int *count) {
     return guarded([&] {
         checked_batch(b);
+        if (n > 0 && !out) raise(AETHER_EINVAL, "null output array");
         const auto &v = b->impl->iters();
         if (count) *count = (int)v.size();
         for (int a = 0; a < n && a < (int)v.size(); ++a) out[a] = v[a];
@@ -2857,12 +2862,18 @@ int aether_fluid_stream_copy(size_t nbytes, int reps, double *avg_ms) {
         HIPT(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         struct Guard {
             hipStream_t s;
-            ~Guard() { (void)hipStreamDestroy(s); }
+            hipEvent_t e[2] = {nullptr, nullptr};
+            ~Guard() {
+                for (hipEvent_t x : e)
+                    if (x) (void)hipEventDestroy(x);
+                (void)hipStreamDestroy(s);
+            }
         } guard{st};
         DevBuf<float4> a(nbytes / 16), b(nbytes / 16);
         const long n4 = (long)(nbytes / 16);
-        int cus = 256;
-        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0);
+        int dev = 0, cus = 256;
+        HIPT(hipGetDevice(&dev));
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
         const unsigned blocks = (unsigned)std::min<long>((n4 + kThreads - 1) / kThreads, 8L * cus);
         k_fill_hash<float><<<blocks, kThreads, 0, st>>>(reinterpret_cast<float *>(a.p), 4 * n4);
         LAUNCH_CHECK();
@@ -2870,9 +2881,9 @@ int aether_fluid_stream_copy(size_t nbytes, int reps, double *avg_ms) {
         LAUNCH_CHECK();
         k_stream_copy<<<blocks, kThreads, 0, st>>>(a.p, n4, b.p);  // warm-up
         LAUNCH_CHECK();
-        hipEvent_t e0, e1;
-        HIPT(hipEventCreate(&e0));
-        HIPT(hipEventCreate(&e1));
+        HIPT(hipEventCreate(&guard.e[0]));
+        HIPT(hipEventCreate(&guard.e[1]));
+        hipEvent_t e0 = guard.e[0], e1 = guard.e[1];
         HIPT(hipEventRecord(e0, st));
         for (int r = 0; r < reps; ++r) {
             k_stream_copy<<<blocks, kThreads, 0, st>>>(a.p, n4, b.p);
@@ -2882,8 +2893,6 @@ int aether_fluid_stream_copy(size_t nbytes, int reps, double *avg_ms) {
         HIPT(hipEventSynchronize(e1));
         float ms = 0;
         HIPT(hipEventElapsedTime(&ms, e0, e1));
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
         *avg_ms = ms / reps;
     });
 }

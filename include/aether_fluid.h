@@ -34,6 +34,7 @@ extern "C" {
 #define AETHER_EHIP (-2)     /* HIP runtime error */
 #define AETHER_ENOMEM (-3)   /* device allocation failed */
 #define AETHER_ENODEV (-4)   /* no usable gfx950 device */
+#define AETHER_EINTERNAL (-5) /* unexpected internal failure (message in aether_fluid_last_error) */
 
 /* precision of the device state */
 #define AETHER_FP32 0        /* FP32 fields and stencils, FP64 CG dot products */

@@ -375,6 +375,13 @@ def test_invalid_arguments_raise():
         s.upload("u", np.zeros(7))
     with pytest.raises(af.AetherFluidError):
         s.advect_maccormack_in_place(7, 0.1, 1.0)
+    # a count query may pass no array; an array length without one is an error
+    import ctypes as C
+    from aether_fluid import _lib as L
+    cnt = C.c_int(-1)
+    assert s._lib.aether_fluid_last_run_iterations(s._h, None, 0, C.byref(cnt)) == L.AETHER_OK
+    assert cnt.value == 0
+    assert s._lib.aether_fluid_last_run_iterations(s._h, None, 4, C.byref(cnt)) == L.AETHER_EINVAL
 
 
 def _step_128(resident: bool, solver="cg"):
