@@ -732,9 +732,9 @@ class Solver final : public SolverBase {
     }
 
     // The CG layout of the next solve (AETHER_CG_SCALAR/X4/FUSED).
-    // Untuned choice for FP32.  The four-wide one-launch CG moves 10 array
-    // passes per iteration in one launch, the two-launch form 8 in two: the
-    // one launch wins while its eight FP32 arrays (32 N bytes) stay resident
+    // Untuned choice for FP32.  The four-wide one-launch CG moves 9 array
+    // passes per iteration on average in one launch, the two-launch form 7.5
+    // in two: the one launch wins while its eight FP32 arrays (32 N bytes) stay resident
     // in the 256 MB Infinity Cache next to the step's other fields (measured:
     // 20% faster at 96^3, 16% at 128^3, slower from 192^3 on).
     int default_form() const {
