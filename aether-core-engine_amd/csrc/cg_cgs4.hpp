@@ -15,13 +15,18 @@
 // on halos by other workgroups, so they ping-pong between two buffers; d and
 // p are updated in place.
 //
-// p is brought up to date every other launch (PUPD: odd `it`):
+// d and p are brought up to date every other launch (PUPD: odd `it`).  An
+// even launch neither loads nor stores them; the odd launch after it
+// rebuilds the skipped direction from the residual that launch read, which
+// is still in this launch's output buffer at the thread's own cells:
+//   d_(it-1) = r_(it-1) + beta_(it-1) d_(it-2)      (d_0 = r_0)
+//   d_it     = r_it + beta_it d_(it-1)
 //   p <- (p + alpha_(it-1) d_(it-1)) + alpha_it d_it
-// with d_(it-1) the value this launch reads anyway, which is the same two
-// FP32 additions in the same order as updating it every launch.  A launch
-// that stops an odd iteration applies the pending term; when the iteration
-// cap ends the solve after an even launch, k_cg_p_tail does.  9 array
-// passes per iteration on average (10 on odd launches, 8 on even ones).
+// the same FP32 operations in the same order as updating both every launch,
+// so the iterates are bit-identical.  A launch that stops an odd iteration
+// applies the pending p term; when the iteration cap ends the solve after
+// an even launch, k_cg_cgs4_tail does.  8.5 array passes per iteration on
+// average (11 on odd launches, 6 on even ones).
 #pragma once
 #include "cg_vec.hpp"
 
@@ -46,13 +51,16 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
     eg.load(pg_in, nb_in);
     ed.load(pd_in, nb_in);
     if (FIRST) eb.load(pbb, nb_in);
-    double gprev = 1.0, aprev = 1.0, b2 = 0.0;
+    double gprev = 1.0, aprev = 1.0, bprev = 0.0, b2 = 0.0;
     if (!FIRST) {
         gprev = st->cg_gamma[(it - 1) & 1];
         aprev = st->cg_alpha[(it - 1) & 1];
         b2 = st->b2;
     }
+    if (PUPD) bprev = st->cg_beta[(it - 1) & 1];
     const float4 *r4 = reinterpret_cast<const float4 *>(r_in);
+    // r_(it-1), read at this thread's own cells before they are overwritten
+    const float4 *ro4 = reinterpret_cast<const float4 *>(r_out);
     const float4 *w4 = reinterpret_cast<const float4 *>(w_in);
     const float4 *s4 = reinterpret_cast<const float4 *>(s_in);
     float4 *d4 = reinterpret_cast<float4 *>(d);
@@ -61,7 +69,7 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
     const Out4<ST> o_r(r_out, nbytes), o_w(w_out, nbytes), o_s(s_out, nbytes), o_d(d, nbytes),
         o_p(p, nbytes);
     const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-    float4 rz[ZU + 2], wz[ZU + 2], sz[ZU + 2], ry[ZU][2], wy[ZU][2], sy[ZU][2], dc[ZU], pc[ZU];
+    float4 rz[ZU + 2], wz[ZU + 2], sz[ZU + 2], ry[ZU][2], wy[ZU][2], sy[ZU][2];
     float er[ZU][2], ew[ZU][2], es[ZU][2];
 #pragma unroll
     for (int a = 0; a < ZU + 2; ++a) {
@@ -73,15 +81,13 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
 #pragma unroll
     for (int a = 0; a < ZU; ++a) {
         const int base = min(c.k0 + a, g.nz - 1) * g.sxy;
-        const int om = (base + c.offym) >> 2, op = (base + c.offyp) >> 2, oc = (base + c.off) >> 2;
+        const int om = (base + c.offym) >> 2, op = (base + c.offyp) >> 2;
         ry[a][0] = r4[om];
         ry[a][1] = r4[op];
         wy[a][0] = w4[om];
         wy[a][1] = w4[op];
         sy[a][0] = FIRST ? z4 : s4[om];
         sy[a][1] = FIRST ? z4 : s4[op];
-        dc[a] = FIRST ? z4 : d4[oc];
-        pc[a] = PUPD ? p4[oc] : z4;
 #pragma unroll
         for (int e = 0; e < 2; ++e) er[a][e] = ew[a][e] = es[a][e] = 0.0f;
         if (c.needL) {
@@ -132,11 +138,16 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
             *done_mirror = solve_id;
         }
         if (PUPD) {  // the pending term of launch it - 1
-            const float tap = (float)aprev;
+            const float tap = (float)aprev, tbp = (float)bprev;
 #pragma unroll
             for (int a = 0; a < ZU; ++a) {
                 const int k = c.k0 + a;
-                if (c.valid && k < g.nz) o_p.put((k * g.sxy + c.off) >> 2, axpy4(pc[a], tap, dc[a]));
+                if (c.valid && k < g.nz) {
+                    const int oc = (k * g.sxy + c.off) >> 2;
+                    const float4 rv = ro4[oc];
+                    const float4 dp = it == 1 ? rv : axpy4(rv, tbp, d4[oc]);
+                    o_p.put(oc, axpy4(p4[oc], tap, dp));
+                }
             }
         }
         return;
@@ -145,8 +156,9 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
     if (lead) {
         st->cg_gamma[it & 1] = gam;
         st->cg_alpha[it & 1] = alpha;
+        st->cg_beta[it & 1] = beta;
     }
-    const float tb = (float)beta, ta = (float)alpha, tap = (float)aprev;
+    const float tb = (float)beta, ta = (float)alpha, tap = (float)aprev, tbp = (float)bprev;
 
     // ---- s = w + beta s and r' = r - alpha s on the radius-1 neighbourhood
     float4 rn[ZU + 2], rny[ZU][2], sn[ZU];
@@ -170,6 +182,21 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
         }
     }
 
+    // ---- odd launches: d_(it-1), p.  Loaded only now, after the r, w, s
+    // registers above are consumed, so that the launch keeps two waves per
+    // SIMD; the stencil below overlaps their latency.
+    float4 dp[ZU], pc[ZU];
+    if (PUPD) {
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int a = 0; a < ZU; ++a) {
+            const int oc = (min(c.k0 + a, g.nz - 1) * g.sxy + c.off) >> 2;
+            const float4 rv = ro4[oc], dv = d4[oc];
+            dp[a] = it == 1 ? rv : axpy4(rv, tbp, dv);  // d_(it-1); d_0 = r_0
+            pc[a] = p4[oc];
+        }
+    }
+
     // ---- w' = A r', d, p and the partials of r'.r', r'.w' ----------------
     const int srcL = (c.lane + 63) & 63, srcR = (c.lane + 1) & 63;
     double acc[2] = {0.0, 0.0};
@@ -183,7 +210,6 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
             float q[4];
             lap4(g, c, k, rn[a + 1], xl, xrv, rny[a][0], rny[a][1], rn[a], rn[a + 2], q);
             const float4 wn = make_float4(q[0], q[1], q[2], q[3]);
-            const float4 dn = FIRST ? rz[a + 1] : axpy4(rz[a + 1], tb, dc[a]);
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const double rv = (double)f4(rn[a + 1], e);
@@ -194,14 +220,35 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
             o_r.put(id4, rn[a + 1]);
             o_w.put(id4, wn);
             o_s.put(id4, sn[a]);
-            o_d.put(id4, dn);
-            if (PUPD) o_p.put(id4, axpy4(axpy4(pc[a], tap, dc[a]), ta, dn));
+            if (PUPD) {
+                const float4 dn = axpy4(rz[a + 1], tb, dp[a]);
+                o_d.put(id4, dn);
+                o_p.put(id4, axpy4(axpy4(pc[a], tap, dp[a]), ta, dn));
+            }
         }
     }
     block_sum<2, false>(acc, lds_out);
     if (threadIdx.x == 0) {
         pg_out[blockIdx.x] = acc[0];
         pd_out[blockIdx.x] = acc[1];
+    }
+}
+
+// Closes a solve whose iteration cap ended it after an even launch `last_it`
+// of k_cg_cgs4, which left d_last unbuilt and alpha_last d_last pending:
+// d_last = r_last + beta_last d_(last-1) (d_0 = r_0), p += alpha_last d_last,
+// with r_last the residual that launch read.  Skipped when a launch stopped
+// the solve (that launch applied what was pending).
+__global__ __launch_bounds__(kThreads) void k_cg_cgs4_tail(const CgState *st, int last_it,
+                                                           const float4 *__restrict__ r_last,
+                                                           const float4 *__restrict__ d,
+                                                           float4 *__restrict__ p, long n4) {
+    if (st->done) return;
+    const float ta = (float)st->cg_alpha[last_it & 1], tb = (float)st->cg_beta[last_it & 1];
+    for (long a = (long)blockIdx.x * kThreads + threadIdx.x; a < n4;
+         a += (long)gridDim.x * kThreads) {
+        const float4 dl = last_it == 0 ? r_last[a] : axpy4(r_last[a], tb, d[a]);
+        p[a] = axpy4(p[a], ta, dl);
     }
 }
 

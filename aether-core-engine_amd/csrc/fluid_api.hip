@@ -732,7 +732,7 @@ class Solver final : public SolverBase {
     }
 
     // The CG layout of the next solve (AETHER_CG_SCALAR/X4/FUSED).
-    // Untuned choice for FP32.  The four-wide one-launch CG moves 9 array
+    // Untuned choice for FP32.  The four-wide one-launch CG moves 8.5 array
     // passes per iteration on average in one launch, the two-launch form 7.5
     // in two: the one launch wins while its eight FP32 arrays (32 N bytes) stay resident
     // in the 256 MB Infinity Cache next to the step's other fields (measured:
@@ -1043,11 +1043,20 @@ class Solver final : public SolverBase {
             const bool fused4 = form_ == AETHER_CG_FUSED && tl4_.nb > 0;
             if ((fused4 || form_ == AETHER_CG_X4) && (max_iters & 1)) {
                 const long n4 = (long)(n_ / 4);
-                const T *dl = fused4 ? d_[0].p : d_[(max_iters - 1) & 1].p;
-                k_cg_p_tail<<<(unsigned)std::min<long>((n4 + kThreads - 1) / kThreads, 4096),
-                              kThreads, 0, st_>>>(cg_.p, max_iters - 1,
-                                                  reinterpret_cast<const float4 *>(dl),
-                                                  reinterpret_cast<float4 *>(p_.p), n4);
+                const unsigned nbt =
+                    (unsigned)std::min<long>((n4 + kThreads - 1) / kThreads, 4096);
+                const int last = max_iters - 1;
+                if (fused4) {  // the one-launch form also left d_last unbuilt
+                    const T *rl = (last & 1) ? r2_.p : r_.p;  // the residual launch `last` read
+                    k_cg_cgs4_tail<<<nbt, kThreads, 0, st_>>>(
+                        cg_.p, last, reinterpret_cast<const float4 *>(rl),
+                        reinterpret_cast<const float4 *>(d_[0].p),
+                        reinterpret_cast<float4 *>(p_.p), n4);
+                } else {
+                    k_cg_p_tail<<<nbt, kThreads, 0, st_>>>(
+                        cg_.p, last, reinterpret_cast<const float4 *>(d_[last & 1].p),
+                        reinterpret_cast<float4 *>(p_.p), n4);
+                }
                 LAUNCH_CHECK();
             }
         }

@@ -42,8 +42,9 @@ struct CgState {
     int iters;
     int converged;
     int pad;
-    double cg_gamma[2];  // single-reduction CG: r.r and alpha of the current
-    double cg_alpha[2];  // iteration, double-buffered by iteration parity
+    double cg_gamma[2];  // single-reduction CG: r.r, alpha and beta of the
+    double cg_alpha[2];  // current iteration, double-buffered by iteration
+    double cg_beta[2];   // parity
 };
 
 __device__ __forceinline__ int tid_lin() { return threadIdx.y * kBX + threadIdx.x; }
