@@ -114,8 +114,9 @@ __global__ __launch_bounds__(kThreads) void kb_curl_mag(const BInst<T> *__restri
     const BInst<T> &in = I[it.inst];
     int i, j, k;
     cell_of<T>(it, i, j, k);
+    const SphereSrc none = {0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     body_curl_mag<T>(i, j, k, in.g, (T)(0.5 / in.h), in.f[sl.u][0], in.f[sl.v][1],
-                     in.f[sl.w][2], in.curl);
+                     in.f[sl.w][2], in.curl, none, nullptr, nullptr);
 }
 
 template <typename T>

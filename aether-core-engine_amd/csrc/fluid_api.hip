@@ -560,11 +560,18 @@ class Solver final : public SolverBase {
         LAUNCH_CHECK();
         std::swap(cur_[kind], nxt_[kind]);
     }
-    // density and temperature share one velocity: one fused pass pair
-    void enqueue_advect_scalars(double dt) {
+    // density and temperature share one velocity: one fused pass pair.
+    // with_gradient: the projection left its gradient subtraction to this
+    // pass, which forms the projected velocity and traces along it at once.
+    void enqueue_advect_scalars(double dt, bool with_gradient = false) {
         const T s = (T)(dt / h_);
-        k_sl_forward<T, 2><<<cell_grid_, cell_blk_, 0, st_>>>(
-            g_, s, cur_[0], cur_[1], cur_[2], cur_[3], cur_[4], fwd_[0].p, fwd_[1].p);
+        if (with_gradient)
+            k_gradient_sl_forward<T><<<cell_grid_, cell_blk_, 0, st_>>>(
+                g_, (T)(1.0 / h_), p_.p, cur_[0], cur_[1], cur_[2], s, cur_[3], cur_[4],
+                fwd_[0].p, fwd_[1].p);
+        else
+            k_sl_forward<T, 2><<<cell_grid_, cell_blk_, 0, st_>>>(
+                g_, s, cur_[0], cur_[1], cur_[2], cur_[3], cur_[4], fwd_[0].p, fwd_[1].p);
         LAUNCH_CHECK();
         k_mac_correct<T, 2><<<cell_grid_, cell_blk_, 0, st_>>>(
             g_, s, cur_[0], cur_[1], cur_[2], cur_[3], fwd_[0].p, nxt_[3],
@@ -585,11 +592,19 @@ class Solver final : public SolverBase {
         enqueue_buoyancy_confinement(dt);
         sync();
     }
-    void enqueue_buoyancy_confinement(double dt) {
+    // with_sphere: also apply the sphere source, which the step runs just
+    // before (the curl pass reads no density or temperature)
+    void enqueue_buoyancy_confinement(double dt, bool with_sphere = false) {
         const double inv_2h = 0.5 / h_;                    // src/fluid.rs:257
         const double eps_h = params.vorticity_eps * h_;   // src/fluid.rs:284
+        SphereSrc sph = {0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        if (with_sphere && params.source_radius > 0.0) {
+            const double r = params.source_radius;
+            sph = {1,     params.source_center[0], params.source_center[1], params.source_center[2],
+                   r * r, params.source_density,   params.source_temperature};
+        }
         k_curl_mag<T><<<cell_grid_, cell_blk_, 0, st_>>>(g_, (T)inv_2h, cur_[0], cur_[1],
-                                                          cur_[2], curl_.p);
+                                                          cur_[2], curl_.p, sph, cur_[3], cur_[4]);
         LAUNCH_CHECK();
         k_confine_buoy_bc<T><<<cell_grid_, cell_blk_, 0, st_>>>(
             g_, (T)dt, (T)inv_2h, (T)eps_h, (T)params.buoyancy_beta, (T)params.gravity_alpha,
@@ -1150,10 +1165,13 @@ class Solver final : public SolverBase {
         last_kind_ = AETHER_SOLVER_CG;
     }
 
-    void enqueue_project(int step_index) {
+    // defer_gradient: leave the closing gradient subtraction to the scalar
+    // advection pass that follows (enqueue_advect_scalars with_gradient);
+    // returns whether it was left (never for the mixed-precision projection)
+    bool enqueue_project(int step_index, bool defer_gradient = false) {
         if (ir64_) {
             project_ir(step_index);
-            return;
+            return false;
         }
         // src/fluid.rs:496-529; the opening wall BC is applied through masked
         // reads in the divergence, the closing one in the gradient kernel
@@ -1173,9 +1191,11 @@ class Solver final : public SolverBase {
             LAUNCH_CHECK();
             enqueue_rbgs(params.solver_iters);
         }
+        if (defer_gradient) return true;
         k_gradient_bc<T><<<cell_grid_, cell_blk_, 0, st_>>>(g_, (T)(1.0 / h_), p_.p, cur_[0],
                                                              cur_[1], cur_[2]);
         LAUNCH_CHECK();
+        return false;
     }
 
     void project() override {
@@ -1309,15 +1329,17 @@ class Solver final : public SolverBase {
         if (out) unpack(raw, out);
     }
 
-    // src/fluid.rs:598-611
+    // src/fluid.rs:598-611.  A sphere source rides in the curl pass, and the
+    // projection's gradient in the scalar advection's first pass.
     void enqueue_step(double dt, double t, double *tel_dst, int step_index) {
         enqueue_advect1(0, dt, params.dissipation_velocity);
         enqueue_advect1(1, dt, params.dissipation_velocity);
         enqueue_advect1(2, dt, params.dissipation_velocity);
-        enqueue_sources(t, dt);
-        enqueue_buoyancy_confinement(dt);
-        enqueue_project(step_index);
-        enqueue_advect_scalars(dt);
+        const bool sphere = !patch_on_ && params.source_kind == AETHER_SOURCE_SPHERE;
+        if (!sphere) enqueue_sources(t, dt);
+        enqueue_buoyancy_confinement(dt, sphere);
+        const bool deferred = enqueue_project(step_index, true);
+        enqueue_advect_scalars(dt, deferred);
         enqueue_telemetry(t, tel_dst);
     }
 
@@ -1605,8 +1627,9 @@ class Solver final : public SolverBase {
                        d.adjoint_cg_rel_tol);
         // c) confinement + buoyancy + wall masks around velocity u1
         const double inv_2h = 0.5 / h_;
+        const SphereSrc none = {0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         k_curl_mag<T><<<cell_grid_, cell_blk_, 0, st_>>>(g_, (T)inv_2h, r.u1.p, r.v1.p, r.w1.p,
-                                                          curl_.p);
+                                                          curl_.p, none, nullptr, nullptr);
         LAUNCH_CHECK();
         for (int a = 0; a < 3; ++a) zero(adj_c_[a].p);
         k_adj_confine1<T><<<cell_grid_, cell_blk_, 0, st_>>>(

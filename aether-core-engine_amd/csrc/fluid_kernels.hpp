@@ -408,27 +408,46 @@ __device__ __forceinline__ void curl_at(const Dims &g, const T *__restrict__ u,
     oz = dv_dx - du_dy;
 }
 
+// A sphere source applied inside another grid pass: the interior cells with
+// d^2 < r^2 in cell-index coordinates get density and temperature set (as
+// k_sphere, whose box covers the same interior cells).
+struct SphereSrc {
+    int on;
+    double cx, cy, cz, r2, rho, temp;
+};
+
 template <typename T>
 __device__ __forceinline__ void body_curl_mag(int i, int j, int k, Dims g, T inv_2h, const T *__restrict__ u,
                                                        const T *__restrict__ v,
                                                        const T *__restrict__ w,
-                                                       T *__restrict__ cmag) {
+                                                       T *__restrict__ cmag, const SphereSrc &sph,
+                                                       T *__restrict__ rho, T *__restrict__ temp) {
     if (i >= g.nx || j >= g.ny) return;
     const int id = (k * g.ny + j) * g.nx + i;
     if (!is_interior(g, i, j, k)) {
         cmag[id] = T(0);
         return;
     }
+    if (sph.on) {
+        const double dx = (double)i - sph.cx, dy = (double)j - sph.cy, dz = (double)k - sph.cz;
+        if (dx * dx + dy * dy + dz * dz < sph.r2) {
+            rho[id] = (T)sph.rho;
+            temp[id] = (T)sph.temp;
+        }
+    }
     T ox, oy, oz;
     curl_at<T>(g, u, v, w, id, inv_2h, ox, oy, oz);
     cmag[id] = sqrt(ox * ox + oy * oy + oz * oz);
 }
+// |curl u| (src/fluid.rs:259-281); with sph.on, also the sphere source that
+// the step applies just before (the curl reads no density or temperature).
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_curl_mag(Dims g, T inv_2h, const T *__restrict__ u,
                                                        const T *__restrict__ v,
                                                        const T *__restrict__ w,
-                                                       T *__restrict__ cmag) {
-    body_curl_mag<T>(blockIdx.x * kBX + threadIdx.x, blockIdx.y * kBY + threadIdx.y, blockIdx.z, g, inv_2h, u, v, w, cmag);
+                                                       T *__restrict__ cmag, SphereSrc sph,
+                                                       T *__restrict__ rho, T *__restrict__ temp) {
+    body_curl_mag<T>(blockIdx.x * kBX + threadIdx.x, blockIdx.y * kBY + threadIdx.y, blockIdx.z, g, inv_2h, u, v, w, cmag, sph, rho, temp);
 }
 
 // Confinement force eps*h*(N x w) on cells [2, n-3], then Boussinesq lift on
@@ -1062,15 +1081,16 @@ __global__ __launch_bounds__(kThreads) void k_rbgs(Dims g, int color, T omega,
     p[id] = (T(1) - omega) * p[id] + omega * gs;
 }
 
-// Gradient subtraction and the closing wall masks (src/fluid.rs:509-528).
+// Gradient subtraction and the closing wall masks (src/fluid.rs:509-528):
+// the projected velocity of cell id.
 template <typename T>
-__device__ __forceinline__ void body_gradient_bc(int i, int j, int k, Dims g, T inv_h,
-                                                          const T *__restrict__ p,
-                                                          T *__restrict__ u, T *__restrict__ v,
-                                                          T *__restrict__ w) {
-    if (i >= g.nx || j >= g.ny) return;
-    const int id = (k * g.ny + j) * g.nx + i;
-    T uu = u[id], vv = v[id], ww = w[id];
+__device__ __forceinline__ void gradient_bc_at(const Dims &g, int i, int j, int k, int id,
+                                               T inv_h, const T *__restrict__ p,
+                                               const T *__restrict__ u, const T *__restrict__ v,
+                                               const T *__restrict__ w, T &uu, T &vv, T &ww) {
+    uu = u[id];
+    vv = v[id];
+    ww = w[id];
     if (is_interior(g, i, j, k)) {
         const T pc = p[id];
         if (i > 1) uu -= (pc - p[id - 1]) * inv_h;
@@ -1080,6 +1100,16 @@ __device__ __forceinline__ void body_gradient_bc(int i, int j, int k, Dims g, T 
     if (bc_u(g, i, j, k)) uu = T(0);
     if (bc_v(g, i, j, k)) vv = T(0);
     if (bc_w(g, i, j, k)) ww = T(0);
+}
+template <typename T>
+__device__ __forceinline__ void body_gradient_bc(int i, int j, int k, Dims g, T inv_h,
+                                                          const T *__restrict__ p,
+                                                          T *__restrict__ u, T *__restrict__ v,
+                                                          T *__restrict__ w) {
+    if (i >= g.nx || j >= g.ny) return;
+    const int id = (k * g.ny + j) * g.nx + i;
+    T uu, vv, ww;
+    gradient_bc_at<T>(g, i, j, k, id, inv_h, p, u, v, w, uu, vv, ww);
     u[id] = uu;
     v[id] = vv;
     w[id] = ww;
@@ -1090,6 +1120,40 @@ __global__ __launch_bounds__(kThreads) void k_gradient_bc(Dims g, T inv_h,
                                                           T *__restrict__ u, T *__restrict__ v,
                                                           T *__restrict__ w) {
     body_gradient_bc<T>(blockIdx.x * kBX + threadIdx.x, blockIdx.y * kBY + threadIdx.y, blockIdx.z, g, inv_h, p, u, v, w);
+}
+
+// The end of a step's projection fused with the start of its scalar
+// advection: the projected velocity of the cell (as k_gradient_bc), then the
+// forward semi-Lagrangian pass of density and temperature along it (as
+// k_sl_forward<T, 2>).  The back-trace reads only the cell's own velocity
+// (src/fluid.rs:160), so one pass serves both, with the same arithmetic.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_gradient_sl_forward(
+    Dims g, T inv_h, const T *__restrict__ p, T *__restrict__ u, T *__restrict__ v,
+    T *__restrict__ w, T s, const T *__restrict__ src0, const T *__restrict__ src1,
+    T *__restrict__ fwd0, T *__restrict__ fwd1) {
+    const int i = blockIdx.x * kBX + threadIdx.x, j = blockIdx.y * kBY + threadIdx.y,
+              k = blockIdx.z;
+    if (i >= g.nx || j >= g.ny) return;
+    const int id = (k * g.ny + j) * g.nx + i;
+    T uu, vv, ww;
+    gradient_bc_at<T>(g, i, j, k, id, inv_h, p, u, v, w, uu, vv, ww);
+    u[id] = uu;
+    v[id] = vv;
+    w[id] = ww;
+    if (!is_interior(g, i, j, k)) {
+        fwd0[id] = T(0);
+        fwd1[id] = T(0);
+        return;
+    }
+    int i0, j0, k0;
+    T tx, ty, tz;
+    bool nw;
+    axis_loc(i, s * uu, g.nx, i0, tx, nw);
+    axis_loc(j, s * vv, g.ny, j0, ty, nw);
+    axis_loc(k, s * ww, g.nz, k0, tz, nw);
+    fwd0[id] = gather_lerp<T>(g, src0, i0, j0, k0, tx, ty, tz);
+    fwd1[id] = gather_lerp<T>(g, src1, i0, j0, k0, tx, ty, tz);
 }
 
 // ---------------------------------------------------------------------------
