@@ -173,6 +173,36 @@ def test_fp64_full_step_reference_defaults():
     assert np.all(np.isfinite(d)) and np.all(d >= 0)
 
 
+@pytest.mark.parametrize("dims", [(6, 6, 6), (8, 6, 7), (12, 6, 9), (7, 9, 6)])
+def test_smallest_grids_vs_oracle(dims):
+    """Grids at the reference's lower limit (n >= 6, src/fluid.rs:54; a 4^3
+    interior at 6^3) and rows of two or three float4 lanes: two FP64 steps
+    with the reference defaults (jets, CG 32), then one FP32 projection per
+    CG layout the grid allows, each against the oracle."""
+    _announce(f"smallest grids {dims}")
+    import aether_fluid as af
+    st = random_state(*dims, seed=3)
+    gpu, orc = make_pair(*dims, state=st)
+    set_knobs(gpu, orc)
+    for t in (0.2, 0.4):
+        tg = gpu.step(0.08, t).as_array()
+        to = orc.step(0.08, t)
+        np.testing.assert_allclose(tg, to, rtol=1e-10, atol=1e-300)
+    errs = compare_fields(gpu, orc)
+    assert max(errs.values()) <= 1e-10, errs
+    forms = [af.CG_SCALAR, af.CG_FUSED] + ([af.CG_X4] if dims[0] % 4 == 0 else [])
+    for form in forms:
+        g32, o = make_pair(*dims, precision="fp32", state=st)
+        set_knobs(g32, o, iters=60, rel_tol=1e-4)
+        g32.cg_form = form
+        g32.project()
+        o.project()
+        sg, so = g32.last_solve_stats(), o.stats()
+        assert abs(sg["iters"] - so["iters"]) <= 1, (form, sg, so)
+        errs = compare_fields(g32, o, ("u", "v", "w", "pressure"))
+        assert max(errs.values()) <= 1e-5, (form, errs)
+
+
 def test_fp64_main_rs_stage_20_steps():
     """The src/main.rs:82-94 loop shape (eps 6.5, 28 iters, dt 0.08) at
     20x28x20 for 20 steps (tests/engine_integration.rs:18-22 grid)."""
