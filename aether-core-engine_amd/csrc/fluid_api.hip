@@ -560,6 +560,30 @@ class Solver final : public SolverBase {
         LAUNCH_CHECK();
         std::swap(cur_[kind], nxt_[kind]);
     }
+    // The three in-place velocity advections of a step (src/fluid.rs:599-601)
+    // in four passes: each component's correction also traces the next
+    // component along the velocity it leaves (k_mac_correct_forward).  The
+    // forward fields alternate between the two buffers, since a correction
+    // gathers its own forward field around each cell while it writes the next.
+    void enqueue_advect_velocity(double dt, double diss) {
+        const T s = (T)(dt / h_);  // src/fluid.rs:154
+        k_sl_forward<T, 1><<<cell_grid_, cell_blk_, 0, st_>>>(g_, s, cur_[0], cur_[1], cur_[2],
+                                                              cur_[0], nullptr, fwd_[0].p,
+                                                              nullptr);
+        LAUNCH_CHECK();
+        for (int kind = 0; kind < 2; ++kind) {
+            k_mac_correct_forward<T><<<cell_grid_, cell_blk_, 0, st_>>>(
+                g_, s, cur_[0], cur_[1], cur_[2], cur_[kind], fwd_[kind].p, nxt_[kind], (T)diss,
+                kind, cur_[kind + 1], fwd_[kind ^ 1].p);
+            LAUNCH_CHECK();
+            std::swap(cur_[kind], nxt_[kind]);
+        }
+        k_mac_correct<T, 1><<<cell_grid_, cell_blk_, 0, st_>>>(
+            g_, s, cur_[0], cur_[1], cur_[2], cur_[2], fwd_[0].p, nxt_[2], (T)diss, nullptr,
+            nullptr, nullptr, T(0), 0);
+        LAUNCH_CHECK();
+        std::swap(cur_[2], nxt_[2]);
+    }
     // density and temperature share one velocity: one fused pass pair.
     // with_gradient: the projection left its gradient subtraction to this
     // pass, which forms the projected velocity and traces along it at once.
@@ -1329,12 +1353,11 @@ class Solver final : public SolverBase {
         if (out) unpack(raw, out);
     }
 
-    // src/fluid.rs:598-611.  A sphere source rides in the curl pass, and the
-    // projection's gradient in the scalar advection's first pass.
+    // src/fluid.rs:598-611.  The velocity advections share passes, a sphere
+    // source rides in the curl pass, and the projection's gradient in the
+    // scalar advection's first pass.
     void enqueue_step(double dt, double t, double *tel_dst, int step_index) {
-        enqueue_advect1(0, dt, params.dissipation_velocity);
-        enqueue_advect1(1, dt, params.dissipation_velocity);
-        enqueue_advect1(2, dt, params.dissipation_velocity);
+        enqueue_advect_velocity(dt, params.dissipation_velocity);
         const bool sphere = !patch_on_ && params.source_kind == AETHER_SOURCE_SPHERE;
         if (!sphere) enqueue_sources(t, dt);
         enqueue_buoyancy_confinement(dt, sphere);

@@ -374,6 +374,52 @@ __global__ __launch_bounds__(kThreads) void k_mac_correct(
     body_mac_correct<T, NF>(blockIdx.x * kBX + threadIdx.x, blockIdx.y * kBY + threadIdx.y, blockIdx.z, g, s, u, v, w, src0, fwd0, out0, diss0, src1, fwd1, out1, diss1, clamp0);
 }
 
+// MacCormack correction of velocity component `kind` (0: u, 1: v) fused with
+// the forward semi-Lagrangian pass of the next component.  The reference
+// advects u, v, w in place one after the other (src/fluid.rs:599-601), so the
+// next component's trace runs along the velocity with this component already
+// corrected; at its own cell that value is the one this thread just formed,
+// and the trace reads no other cell's velocity (src/fluid.rs:160).  u, v, w
+// are the velocity of this correction (for kind 1, u is already corrected).
+// Same arithmetic as k_mac_correct<T, 1> followed by k_sl_forward<T, 1>.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_mac_correct_forward(
+    Dims g, T s, const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ w,
+    const T *__restrict__ src, const T *__restrict__ fwd, T *__restrict__ out, T diss, int kind,
+    const T *__restrict__ src_next, T *__restrict__ fwd_next) {
+    const int i = blockIdx.x * kBX + threadIdx.x, j = blockIdx.y * kBY + threadIdx.y,
+              k = blockIdx.z;
+    if (i >= g.nx || j >= g.ny) return;
+    const int id = (k * g.ny + j) * g.nx + i;
+    if (!is_interior(g, i, j, k)) {
+        out[id] = T(0);
+        fwd_next[id] = T(0);
+        return;
+    }
+    const T su = s * u[id], sv = s * v[id], sw = s * w[id];
+    int i0, j0, k0, fi0, fj0, fk0;
+    T tx, ty, tz, ftx, fty, ftz;
+    bool nwx, nwy, nwz, dummy;
+    axis_loc(i, su, g.nx, i0, tx, nwx);
+    axis_loc(j, sv, g.ny, j0, ty, nwy);
+    axis_loc(k, sw, g.nz, k0, tz, nwz);
+    axis_loc(i, -su, g.nx, fi0, ftx, dummy);
+    axis_loc(j, -sv, g.ny, fj0, fty, dummy);
+    axis_loc(k, -sw, g.nz, fk0, ftz, dummy);
+    const T val = mac_correct_one<T>(g, src, fwd, id, nwx || nwy || nwz, i0, j0, k0, fi0, fj0,
+                                     fk0, ftx, fty, ftz, diss, false);
+    out[id] = val;
+    // the next component's trace: this component replaced by its new value
+    const T su2 = kind == 0 ? s * val : su, sv2 = kind == 1 ? s * val : sv;
+    int n0, m0, l0;
+    T nx_, ny_, nz_;
+    bool nw;
+    axis_loc(i, su2, g.nx, n0, nx_, nw);
+    axis_loc(j, sv2, g.ny, m0, ny_, nw);
+    axis_loc(k, sw, g.nz, l0, nz_, nw);
+    fwd_next[id] = gather_lerp<T>(g, src_next, n0, m0, l0, nx_, ny_, nz_);
+}
+
 // ---------------------------------------------------------------------------
 // Walls, vorticity confinement, buoyancy (src/fluid.rs:236-330)
 // ---------------------------------------------------------------------------
