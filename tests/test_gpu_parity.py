@@ -203,6 +203,40 @@ def test_smallest_grids_vs_oracle(dims):
         assert max(errs.values()) <= 1e-5, (form, errs)
 
 
+@pytest.mark.parametrize("dims", [(15, 32, 9), (44, 9, 11), (36, 37, 41), (13, 41, 20),
+                                  (28, 18, 18), (68, 12, 10)])
+def test_assorted_grid_shapes_vs_oracle(dims):
+    """Shapes drawn to miss the tile sizes (64 x 4 cells, 64-lane rows, 4 z
+    planes per thread): two FP64 steps with a sphere source and the
+    relative CG tolerance against the oracle, then one FP32 projection per
+    CG layout the grid allows."""
+    _announce(f"assorted grid {dims}")
+    import aether_fluid as af
+    nx, ny, nz = dims
+    st = random_state(*dims, seed=sum(dims))
+    src = dict(center=(nx / 2.0, 2.5, nz / 2.0), radius=max(2.0, min(dims) / 4.0))
+    gpu, orc = make_pair(*dims, state=st)
+    set_knobs(gpu, orc, iters=200, rel_tol=1e-5, source=src)
+    for t in (0.0, 0.05):
+        tg = gpu.step(0.05, t).as_array()
+        to = orc.step(0.05, t)
+        np.testing.assert_allclose(tg, to, rtol=1e-10, atol=1e-300)
+    assert gpu.last_solve_stats()["iters"] == orc.stats()["iters"]
+    errs = compare_fields(gpu, orc)
+    assert max(errs.values()) <= 1e-10, errs
+    forms = [af.CG_SCALAR, af.CG_FUSED] + ([af.CG_X4] if nx % 4 == 0 else [])
+    for form in forms:
+        g32, o = make_pair(*dims, precision="fp32", state=st)
+        set_knobs(g32, o, iters=150, rel_tol=1e-4)
+        g32.cg_form = form
+        g32.project()
+        o.project()
+        sg, so = g32.last_solve_stats(), o.stats()
+        assert abs(sg["iters"] - so["iters"]) <= 1, (form, sg, so)
+        errs = compare_fields(g32, o, ("u", "v", "w", "pressure"))
+        assert max(errs.values()) <= 1e-5, (form, errs)
+
+
 def test_fp64_main_rs_stage_20_steps():
     """The src/main.rs:82-94 loop shape (eps 6.5, 28 iters, dt 0.08) at
     20x28x20 for 20 steps (tests/engine_integration.rs:18-22 grid)."""
