@@ -390,6 +390,36 @@ __global__ __launch_bounds__(kThreads) void k_cg_p_tail(const CgState *st, int l
         p[a] = axpy4(p[a], ta, d[a]);
 }
 
+// Red-black SOR update of the colour-`colour` interior cells among four
+// x-cells (src/fluid.rs:400-437; per-cell arithmetic as k_rbgs): ctr holds
+// the cells, left/right their x-neighbours outside the float4, ym/yp/zm/zp
+// the rows and planes beside them, bb the right-hand side.  Cells of the
+// other colour or off the interior keep their bits.
+__device__ __forceinline__ float4 sor_colour4(const Dims &g, int i0, int j, int k, int colour,
+                                              float omega, const float4 &ctr, float left,
+                                              float right, const float4 &ym, const float4 &yp,
+                                              const float4 &zm, const float4 &zp,
+                                              const float4 &bb, bool &any) {
+    float o[4] = {ctr.x, ctr.y, ctr.z, ctr.w};
+    any = false;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int i = i0 + e;
+        if (!is_interior(g, i, j, k) || ((i + j + k) & 1) != colour) continue;
+        any = true;
+        float diag = 0.0f, nb = 0.0f;
+        if (i > 1) { diag += 1.0f; nb += e == 0 ? left : f4(ctr, e - 1); }
+        if (i + 2 < g.nx) { diag += 1.0f; nb += e == 3 ? right : f4(ctr, e + 1); }
+        if (j > 1) { diag += 1.0f; nb += f4(ym, e); }
+        if (j + 2 < g.ny) { diag += 1.0f; nb += f4(yp, e); }
+        if (k > 1) { diag += 1.0f; nb += f4(zm, e); }
+        if (k + 2 < g.nz) { diag += 1.0f; nb += f4(zp, e); }
+        const float gs = (f4(bb, e) + nb) / diag;
+        o[e] = (1.0f - omega) * o[e] + omega * gs;
+    }
+    return make_float4(o[0], o[1], o[2], o[3]);
+}
+
 // Red-black SOR pass of one colour (src/fluid.rs:400-437) on the four-wide
 // layout: the same per-cell arithmetic as k_rbgs.  A cell's neighbours all
 // have the other colour, which this pass leaves unchanged, so every thread
@@ -429,25 +459,96 @@ __global__ __launch_bounds__(kThreads) void k_rbgs4(Dims g, VTiling vt, int colo
         const float sl = lane_value(pz[a + 1].w, srcL), sr = lane_value(pz[a + 1].x, srcR);
         const float left = c.needL ? xl[a] : sl, right = c.needR ? xr[a] : sr;
         if (c.valid && k < g.nz) {
-            const float4 ctr = pz[a + 1];
-            float o[4] = {ctr.x, ctr.y, ctr.z, ctr.w};
-            bool any = false;
+            bool any;
+            const float4 o = sor_colour4(g, c.i0, c.j, k, color, omega, pz[a + 1], left, right,
+                                         py[a][0], py[a][1], pz[a], pz[a + 2], bc[a], any);
+            if (any) o_p.put((k * g.sxy + c.off) >> 2, o);
+        }
+    }
+}
+
+// One whole red-black SOR sweep (src/fluid.rs:400-437) in one launch, from
+// p_in to p_out.  Each thread forms, from p_in, the colour-0 values that
+// its colour-1 cells read: its own cells, one plane past each end of its
+// column, and the rows on either side; it then updates its colour-1 cells
+// and writes its whole column.  Those colour-0 values are what the
+// colour-0 pass of the two-launch sweep would store there (same inputs,
+// same arithmetic), so the sweep is bit-identical to two k_rbgs4 passes.
+// Neighbouring workgroups form the shared colour-0 rows twice, which is why
+// the sweep reads p_in and writes a second array.  Needs every grid row to
+// lie in one wave with no lane past its end (nx = 4 w4, w4 dividing 64):
+// then x-neighbours always come from adjacent lanes and no edge loads are
+// needed.
+template <int ZU, int ST>
+__global__ __launch_bounds__(kThreads) void k_rbsor4(Dims g, VTiling vt, float omega,
+                                                     const float *__restrict__ b,
+                                                     const float *__restrict__ p_in,
+                                                     float *__restrict__ p_out) {
+    int bx, by, bz;
+    vtile_coords(vt, bx, by, bz);
+    const VCol c = vcol_at<ZU>(g, vt, bx, by, bz);
+    const float4 *p4 = reinterpret_cast<const float4 *>(p_in);
+    const float4 *b4 = reinterpret_cast<const float4 *>(b);
+    const Out4<ST> o_p(p_out, (long)g.sxy * g.nz * 4);
+    const int jc = min(c.j, g.ny - 1), ic0 = c.off - jc * g.nx;
+    auto row = [&](int dj) { return min(max(c.j + dj, 0), g.ny - 1) * g.nx + ic0; };
+    auto plane = [&](int k) { return min(max(k, 0), g.nz - 1) * g.sxy; };
+    const int rm2 = row(-2), rm1 = c.offym, rp1 = c.offyp, rp2 = row(2);
+    float4 C[ZU + 4];         // row j, planes k0-2 .. k0+ZU+1
+    float4 Y1[2][ZU + 2];     // rows j-1, j+1, planes k0-1 .. k0+ZU
+    float4 Y2[2][ZU];         // rows j-2, j+2, planes k0 .. k0+ZU-1
+    float4 BC[ZU + 2];        // b on row j, planes k0-1 .. k0+ZU
+    float4 BY[2][ZU];         // b on rows j-1, j+1, planes k0 .. k0+ZU-1
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int i = c.i0 + e;
-                if (!is_interior(g, i, c.j, k) || ((i + c.j + k) & 1) != color) continue;
-                any = true;
-                float diag = 0.0f, nb = 0.0f;
-                if (i > 1) { diag += 1.0f; nb += e == 0 ? left : f4(ctr, e - 1); }
-                if (i + 2 < g.nx) { diag += 1.0f; nb += e == 3 ? right : f4(ctr, e + 1); }
-                if (c.j > 1) { diag += 1.0f; nb += f4(py[a][0], e); }
-                if (c.j + 2 < g.ny) { diag += 1.0f; nb += f4(py[a][1], e); }
-                if (k > 1) { diag += 1.0f; nb += f4(pz[a], e); }
-                if (k + 2 < g.nz) { diag += 1.0f; nb += f4(pz[a + 2], e); }
-                const float gs = (f4(bc[a], e) + nb) / diag;
-                o[e] = (1.0f - omega) * o[e] + omega * gs;
-            }
-            if (any) o_p.put((k * g.sxy + c.off) >> 2, make_float4(o[0], o[1], o[2], o[3]));
+    for (int a = 0; a < ZU + 4; ++a) C[a] = p4[(plane(c.k0 - 2 + a) + c.off) >> 2];
+#pragma unroll
+    for (int a = 0; a < ZU + 2; ++a) {
+        const int base = plane(c.k0 - 1 + a);
+        Y1[0][a] = p4[(base + rm1) >> 2];
+        Y1[1][a] = p4[(base + rp1) >> 2];
+        BC[a] = b4[(base + c.off) >> 2];
+    }
+#pragma unroll
+    for (int a = 0; a < ZU; ++a) {
+        const int base = plane(c.k0 + a);
+        Y2[0][a] = p4[(base + rm2) >> 2];
+        Y2[1][a] = p4[(base + rp2) >> 2];
+        BY[0][a] = b4[(base + rm1) >> 2];
+        BY[1][a] = b4[(base + rp1) >> 2];
+    }
+    const int srcL = (c.lane + 63) & 63, srcR = (c.lane + 1) & 63;
+    bool any;
+    // colour 0 on row j, planes k0-1 .. k0+ZU
+    float4 RC[ZU + 2];
+#pragma unroll
+    for (int a = 0; a < ZU + 2; ++a) {
+        const float sl = lane_value(C[a + 1].w, srcL), sr = lane_value(C[a + 1].x, srcR);
+        RC[a] = sor_colour4(g, c.i0, c.j, c.k0 - 1 + a, 0, omega, C[a + 1], sl, sr, Y1[0][a],
+                            Y1[1][a], C[a], C[a + 2], BC[a], any);
+    }
+    // colour 0 on rows j-1 and j+1, planes k0 .. k0+ZU-1
+    float4 RY[2][ZU];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+#pragma unroll
+        for (int a = 0; a < ZU; ++a) {
+            const float4 ctr = Y1[s][a + 1];
+            const float sl = lane_value(ctr.w, srcL), sr = lane_value(ctr.x, srcR);
+            const float4 ym = s == 0 ? Y2[0][a] : C[a + 2];
+            const float4 yp = s == 0 ? C[a + 2] : Y2[1][a];
+            RY[s][a] = sor_colour4(g, c.i0, c.j + (s == 0 ? -1 : 1), c.k0 + a, 0, omega, ctr, sl,
+                                   sr, ym, yp, Y1[s][a], Y1[s][a + 2], BY[s][a], any);
+        }
+    }
+    // colour 1 on the thread's own cells, from the colour-0 values above
+#pragma unroll
+    for (int a = 0; a < ZU; ++a) {
+        const int k = c.k0 + a;
+        const float sl = lane_value(RC[a + 1].w, srcL), sr = lane_value(RC[a + 1].x, srcR);
+        if (c.valid && k < g.nz) {
+            const float4 o = sor_colour4(g, c.i0, c.j, k, 1, omega, RC[a + 1], sl, sr, RY[0][a],
+                                         RY[1][a], RC[a], RC[a + 2], BC[a + 1], any);
+            o_p.put((k * g.sxy + c.off) >> 2, o);  // every cell: p_out holds the whole sweep
         }
     }
 }

@@ -1119,6 +1119,30 @@ class Solver final : public SolverBase {
         if constexpr (sizeof(T) == 4) four = tlv_.nb > 0 && params.cg_form != AETHER_CG_SCALAR;
         if (four) {
             if constexpr (sizeof(T) == 4) {
+                // one launch per sweep when every row fills whole wave
+                // segments (k_rbsor4), from p to a second array and back
+                if (tlv_.gx == 1 && 64 % tlv_.txv == 0 && sweeps > 0) {
+                    if (!p2_.p) p2_.alloc(n_);
+                    for (int it = 0; it < sweeps; ++it) {
+                        const T *pin = (it & 1) ? p2_.p : p_.p;
+                        T *pout = (it & 1) ? p_.p : p2_.p;
+                        if (wt_ok())
+                            VZU_DISPATCH(vzu_, (k_rbsor4<VZU, kStWT><<<tlv_.nb, kThreads, 0, st_>>>(
+                                                   g_, tlv_, (float)params.sor_omega, b_.p, pin,
+                                                   pout)));
+                        else
+                            VZU_DISPATCH(vzu_, (k_rbsor4<VZU, kStPlain><<<tlv_.nb, kThreads, 0, st_>>>(
+                                                   g_, tlv_, (float)params.sor_omega, b_.p, pin,
+                                                   pout)));
+                        LAUNCH_CHECK();
+                    }
+                    if (sweeps & 1)
+                        HIPT(hipMemcpyAsync(p_.p, p2_.p, n_ * sizeof(T), hipMemcpyDeviceToDevice,
+                                            st_));
+                    last_kind_ = AETHER_SOLVER_RBGS;
+                    last_sweeps_ = sweeps;
+                    return;
+                }
                 for (int it = 0; it < sweeps; ++it)
                     for (int color = 0; color < 2; ++color) {
                         if (wt_ok())
@@ -1805,6 +1829,7 @@ class Solver final : public SolverBase {
     int kz_ = 16;
     int pf_ = 4;          // planes of loads in flight ahead of use
     DevBuf<T> r2_;                 // second residual buffer of the one-launch CG
+    DevBuf<T> p2_;                 // second pressure buffer of the one-launch SOR sweep
     DevBuf<T> w_[2], s_[2];        // A r and A d of the four-wide one-launch CG
     DevBuf<double> partE_, partF_;  // its delta partials
     bool ir64_ = false;           // mixed-precision projection (AETHER_FP32_IR64)

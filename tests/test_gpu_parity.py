@@ -158,6 +158,33 @@ def test_fp64_rbgs_bitexact():
         np.testing.assert_array_equal(gpu.download(n), orc.field(n).ravel())
 
 
+@pytest.mark.parametrize("dims,sweeps", [((32, 24, 20), 7), ((32, 24, 20), 8),
+                                         ((256, 10, 9), 5)])
+def test_one_launch_sor_sweep_equals_colour_passes(dims, sweeps):
+    """FP32 RB-SOR on rows of 8 and 64 float4 lanes, where a sweep runs as
+    one launch (colour-0 values formed where the colour-1 cells read them),
+    with odd and even sweep counts: the projection is bitwise equal to the
+    one-cell-per-thread colour passes, and within the FP32 bar of the
+    oracle."""
+    _announce(f"one-launch SOR sweep {dims} x{sweeps}")
+    import aether_fluid as af
+    st = random_state(*dims, seed=41)
+    a, orc = make_pair(*dims, precision="fp32", state=st)
+    set_knobs(a, orc, kind=1, iters=sweeps, omega=1.8)
+    b = af.FluidSolver3D(*dims, precision="fp32")
+    for n, v in st.items():
+        b.upload(n, v)
+    b.solver_kind, b.solver_iters, b.sor_omega = 1, sweeps, 1.8
+    b.cg_form = af.CG_SCALAR  # one cell per thread, two passes per sweep
+    a.project()
+    b.project()
+    orc.project()
+    for n in ("u", "v", "w", "pressure"):
+        np.testing.assert_array_equal(a.download(n), b.download(n))
+    errs = compare_fields(a, orc, ("u", "v", "w", "pressure"))
+    assert max(errs.values()) <= 1e-5, errs
+
+
 def test_fp64_full_step_reference_defaults():
     """Two reference-default steps (jets, CG 32) as in src/fluid.rs:959-968."""
     _announce("fp64 full step 16x20x16 x2")
