@@ -468,57 +468,50 @@ __global__ __launch_bounds__(kThreads) void k_rbgs4(Dims g, VTiling vt, int colo
 }
 
 // One whole red-black SOR sweep (src/fluid.rs:400-437) in one launch, from
-// p_in to p_out.  Each thread forms, from p_in, the colour-0 values that
-// its colour-1 cells read: its own cells, one plane past each end of its
-// column, and the rows on either side; it then updates its colour-1 cells
-// and writes its whole column.  Those colour-0 values are what the
-// colour-0 pass of the two-launch sweep would store there (same inputs,
-// same arithmetic), so the sweep is bit-identical to two k_rbgs4 passes.
-// Neighbouring workgroups form the shared colour-0 rows twice, which is why
-// the sweep reads p_in and writes a second array.  Needs every grid row to
-// lie in one wave with no lane past its end (nx = 4 w4, w4 dividing 64):
-// then x-neighbours always come from adjacent lanes and no edge loads are
-// needed.
+// p_in to p_out.  Each thread forms, from p_in, the colour-0 values of its
+// own column and one plane past each end of it, and shares its own planes
+// through LDS; the threads on the first and last row of the workgroup's
+// tile also form the colour-0 values of the row beyond the tile.  Each
+// thread then updates its colour-1 cells and writes its whole column.
+// Those colour-0 values are what the colour-0 pass of the two-launch sweep
+// would store there (same inputs, same arithmetic), so the sweep is
+// bit-identical to two k_rbgs4 passes.  Neighbouring workgroups form the
+// rows and planes between them twice, which is why the sweep reads p_in and
+// writes a second array.  Needs every grid row to lie in one wave with no
+// lane past its end (nx = 4 w4, w4 dividing 64): then x-neighbours always
+// come from adjacent lanes and no edge loads are needed.
 template <int ZU, int ST>
 __global__ __launch_bounds__(kThreads) void k_rbsor4(Dims g, VTiling vt, float omega,
                                                      const float *__restrict__ b,
                                                      const float *__restrict__ p_in,
                                                      float *__restrict__ p_out) {
+    __shared__ float4 lds_rc[ZU][kThreads];
     int bx, by, bz;
     vtile_coords(vt, bx, by, bz);
     const VCol c = vcol_at<ZU>(g, vt, bx, by, bz);
     const float4 *p4 = reinterpret_cast<const float4 *>(p_in);
     const float4 *b4 = reinterpret_cast<const float4 *>(b);
     const Out4<ST> o_p(p_out, (long)g.sxy * g.nz * 4);
+    const int t = threadIdx.x, ty = t / vt.txv;
+    const bool first_row = ty == 0, last_row = ty == vt.ty - 1;
     const int jc = min(c.j, g.ny - 1), ic0 = c.off - jc * g.nx;
     auto row = [&](int dj) { return min(max(c.j + dj, 0), g.ny - 1) * g.nx + ic0; };
     auto plane = [&](int k) { return min(max(k, 0), g.nz - 1) * g.sxy; };
-    const int rm2 = row(-2), rm1 = c.offym, rp1 = c.offyp, rp2 = row(2);
-    float4 C[ZU + 4];         // row j, planes k0-2 .. k0+ZU+1
-    float4 Y1[2][ZU + 2];     // rows j-1, j+1, planes k0-1 .. k0+ZU
-    float4 Y2[2][ZU];         // rows j-2, j+2, planes k0 .. k0+ZU-1
-    float4 BC[ZU + 2];        // b on row j, planes k0-1 .. k0+ZU
-    float4 BY[2][ZU];         // b on rows j-1, j+1, planes k0 .. k0+ZU-1
+    float4 C[ZU + 4];      // row j, planes k0-2 .. k0+ZU+1
+    float4 Y1[2][ZU + 2];  // rows j-1, j+1, planes k0-1 .. k0+ZU
+    float4 BC[ZU + 2];     // b on row j, planes k0-1 .. k0+ZU
 #pragma unroll
     for (int a = 0; a < ZU + 4; ++a) C[a] = p4[(plane(c.k0 - 2 + a) + c.off) >> 2];
 #pragma unroll
     for (int a = 0; a < ZU + 2; ++a) {
         const int base = plane(c.k0 - 1 + a);
-        Y1[0][a] = p4[(base + rm1) >> 2];
-        Y1[1][a] = p4[(base + rp1) >> 2];
+        Y1[0][a] = p4[(base + c.offym) >> 2];
+        Y1[1][a] = p4[(base + c.offyp) >> 2];
         BC[a] = b4[(base + c.off) >> 2];
-    }
-#pragma unroll
-    for (int a = 0; a < ZU; ++a) {
-        const int base = plane(c.k0 + a);
-        Y2[0][a] = p4[(base + rm2) >> 2];
-        Y2[1][a] = p4[(base + rp2) >> 2];
-        BY[0][a] = b4[(base + rm1) >> 2];
-        BY[1][a] = b4[(base + rp1) >> 2];
     }
     const int srcL = (c.lane + 63) & 63, srcR = (c.lane + 1) & 63;
     bool any;
-    // colour 0 on row j, planes k0-1 .. k0+ZU
+    // colour 0 on row j, planes k0-1 .. k0+ZU; the thread's own planes to LDS
     float4 RC[ZU + 2];
 #pragma unroll
     for (int a = 0; a < ZU + 2; ++a) {
@@ -526,28 +519,49 @@ __global__ __launch_bounds__(kThreads) void k_rbsor4(Dims g, VTiling vt, float o
         RC[a] = sor_colour4(g, c.i0, c.j, c.k0 - 1 + a, 0, omega, C[a + 1], sl, sr, Y1[0][a],
                             Y1[1][a], C[a], C[a + 2], BC[a], any);
     }
-    // colour 0 on rows j-1 and j+1, planes k0 .. k0+ZU-1
-    float4 RY[2][ZU];
 #pragma unroll
-    for (int s = 0; s < 2; ++s) {
+    for (int a = 0; a < ZU; ++a) lds_rc[a][t] = RC[a + 1];
+    // x-neighbours on rows j-1 and j+1 (every lane takes part in the exchange)
+    float yl[2][ZU], yr[2][ZU];
+#pragma unroll
+    for (int s2 = 0; s2 < 2; ++s2)
 #pragma unroll
         for (int a = 0; a < ZU; ++a) {
-            const float4 ctr = Y1[s][a + 1];
-            const float sl = lane_value(ctr.w, srcL), sr = lane_value(ctr.x, srcR);
-            const float4 ym = s == 0 ? Y2[0][a] : C[a + 2];
-            const float4 yp = s == 0 ? C[a + 2] : Y2[1][a];
-            RY[s][a] = sor_colour4(g, c.i0, c.j + (s == 0 ? -1 : 1), c.k0 + a, 0, omega, ctr, sl,
-                                   sr, ym, yp, Y1[s][a], Y1[s][a + 2], BY[s][a], any);
+            yl[s2][a] = lane_value(Y1[s2][a + 1].w, srcL);
+            yr[s2][a] = lane_value(Y1[s2][a + 1].x, srcR);
         }
-    }
+    // colour 0 on the row beyond the tile (first and last tile rows only)
+    float4 RE[ZU];
+#pragma unroll
+    for (int a = 0; a < ZU; ++a) RE[a] = make_float4(0.f, 0.f, 0.f, 0.f);
+    // (one branch per side, each with constant array indices, so that the
+    // arrays stay in registers)
+    auto edge_row = [&](const float4(&y)[ZU + 2], const float(&l)[ZU], const float(&r)[ZU],
+                        int dj) {
+        const int rfar = row(2 * dj), rnear = dj < 0 ? c.offym : c.offyp;
+#pragma unroll
+        for (int a = 0; a < ZU; ++a) {
+            const int base = plane(c.k0 + a);
+            const float4 far = p4[(base + rfar) >> 2], bb = b4[(base + rnear) >> 2];
+            RE[a] = sor_colour4(g, c.i0, c.j + dj, c.k0 + a, 0, omega, y[a + 1], l[a], r[a],
+                                dj < 0 ? far : C[a + 2], dj < 0 ? C[a + 2] : far, y[a],
+                                y[a + 2], bb, any);
+        }
+    };
+    if (first_row) edge_row(Y1[0], yl[0], yr[0], -1);
+    if (last_row) edge_row(Y1[1], yl[1], yr[1], 1);
+    __syncthreads();
     // colour 1 on the thread's own cells, from the colour-0 values above
 #pragma unroll
     for (int a = 0; a < ZU; ++a) {
         const int k = c.k0 + a;
         const float sl = lane_value(RC[a + 1].w, srcL), sr = lane_value(RC[a + 1].x, srcR);
+        // (indices kept in range on the rows that take RE instead)
+        const float4 ym = first_row ? RE[a] : lds_rc[a][max(t - vt.txv, 0)];
+        const float4 yp = last_row ? RE[a] : lds_rc[a][min(t + vt.txv, kThreads - 1)];
         if (c.valid && k < g.nz) {
-            const float4 o = sor_colour4(g, c.i0, c.j, k, 1, omega, RC[a + 1], sl, sr, RY[0][a],
-                                         RY[1][a], RC[a], RC[a + 2], BC[a + 1], any);
+            const float4 o = sor_colour4(g, c.i0, c.j, k, 1, omega, RC[a + 1], sl, sr, ym, yp,
+                                         RC[a], RC[a + 2], BC[a + 1], any);
             o_p.put((k * g.sxy + c.off) >> 2, o);  // every cell: p_out holds the whole sweep
         }
     }
