@@ -1,0 +1,615 @@
+// batch.hpp -- many smoke grids advanced together, one launch per stage.
+// Part of the library's single translation unit: included once, by
+// fluid_api.hip, after the headers before it there.
+#pragma once
+#include "host_common.hpp"
+#include "cg_vec.hpp"
+#include "batch_kernels.hpp"
+
+namespace aether {
+namespace {
+
+// ---------------------------------------------------------------------------
+// Batched smoke grids: every stage is one launch over the tiles of all
+// instances (batch_kernels.hpp).  Instances share dt, the solver knobs and
+// the dissipation; each has its own grid, h, forcing and sphere source.
+// ---------------------------------------------------------------------------
+constexpr int kBatchZU = 8;  // z cells per thread of the batched CG tiling
+
+class BatchBase {
+   public:
+    virtual ~BatchBase() = default;
+    virtual void autotune(int iters, double *us, int *form) = 0;
+    virtual void upload(int inst, int field, const double *src, size_t n) = 0;
+    virtual void download(int inst, int field, double *dst, size_t n) = 0;
+    virtual void run(double dt, const double *times, int nsteps, aether_fluid_telemetry *out) = 0;
+    virtual const std::vector<int> &iters() const = 0;
+    virtual int count() const = 0;
+};
+
+template <typename T>
+class Batch final : public BatchBase {
+    struct Inst {
+        Dims g;
+        double h;
+        size_t n;
+        int gx, gy, gz, nb;
+        VTiling vt = {0, 0, 0, 0, 0, 1, 1};  // four-wide CG tiling (nx % 4 == 0)
+        aether_fluid_instance_desc d;
+        DevBuf<T> f[2][5], fwd0, fwd1, curl, p, b, r, dd[2];
+        DevBuf<double> pa, pb, pc, ptel, tel, staging;
+        DevBuf<CgState> st;
+        DevBuf<int> iters;
+    };
+
+   public:
+    Batch(const aether_fluid_instance_desc *desc, int count, const aether_fluid_params &shared,
+          hipStream_t stream)
+        : params_(shared), st_(stream) {
+        std::vector<BItem> cells, cg, cg4;
+        std::vector<int2> src;
+        // four-wide CG: FP32 and every nx % 4 == 0
+        x4_ok_ = sizeof(T) == 4;
+        // (its kernels store write-through, which addresses arrays under 2 GiB)
+        for (int q = 0; q < count; ++q)
+            x4_ok_ = x4_ok_ && desc[q].nx % 4 == 0 &&
+                     (size_t)desc[q].nx * desc[q].ny * desc[q].nz * 4 < (size_t(1) << 31);
+        for (int q = 0; q < count; ++q) {
+            auto in = std::make_unique<Inst>();
+            const aether_fluid_instance_desc &d = desc[q];
+            in->d = d;
+            in->g.nx = d.nx;
+            in->g.ny = d.ny;
+            in->g.nz = d.nz;
+            in->g.sxy = d.nx * d.ny;
+            in->h = d.h > 0.0 ? d.h : 1.0 / (double)std::max(d.nx, std::max(d.ny, d.nz));
+            in->n = (size_t)d.nx * d.ny * d.nz;
+            in->gx = (d.nx + kBX - 1) / kBX;
+            in->gy = (d.ny + kBY - 1) / kBY;
+            in->gz = (d.nz + kBatchZU - 1) / kBatchZU;
+            in->nb = in->gx * in->gy * in->gz;
+            if (x4_ok_) {  // as the single-grid four-wide tiling (4 z cells per thread)
+                VTiling &v = in->vt;
+                v.txv = std::min(d.nx / 4, 64);
+                v.ty = kThreads / v.txv;
+                v.gx = (d.nx / 4 + v.txv - 1) / v.txv;
+                v.gy = (d.ny + v.ty - 1) / v.ty;
+                v.gz = (d.nz + 3) / 4;
+                v.nb = v.gx * v.gy * v.gz;
+                v.remap = (v.nb % 8 == 0) ? 1 : 0;
+                wide_ = wide_ || std::max(in->nb, v.nb) > 4 * kThreads;
+            }
+            const int npart = std::max(in->nb, in->vt.nb);
+            for (int a = 0; a < 2; ++a)
+                for (int f = 0; f < 5; ++f) in->f[a][f].alloc(in->n);
+            for (DevBuf<T> *b : {&in->fwd0, &in->fwd1, &in->curl, &in->p, &in->b, &in->r,
+                                 &in->dd[0], &in->dd[1]})
+                b->alloc(in->n);
+            in->pa.alloc(npart);
+            in->pb.alloc(npart);
+            in->pc.alloc(npart);
+            in->ptel.alloc((size_t)in->nb * kTelParts);
+            in->st.alloc(1);
+            for (int a = 0; a < 2; ++a)
+                for (int f = 0; f < 5; ++f) fill(in->f[a][f].p, in->n);
+            for (DevBuf<T> *b : {&in->fwd0, &in->fwd1, &in->curl, &in->p, &in->b, &in->r,
+                                 &in->dd[0], &in->dd[1]})
+                fill(b->p, in->n);
+            HIPT(hipMemsetAsync(in->st.p, 0, sizeof(CgState), st_));
+            // cell work items: one per (64 x 4 columns, z plane)
+            for (int k = 0; k < d.nz; ++k)
+                for (int by = 0; by < in->gy; ++by)
+                    for (int bx = 0; bx < in->gx; ++bx) cells.push_back({q, bx, by, k, 0, {0, 0, 0}});
+            // CG tiles, numbered like the single-grid launch (same XCD-slab
+            // order), so each tile's partial sum lands in the same slot
+            const bool remap = in->nb % 8 == 0;
+            for (int b = 0; b < in->nb; ++b) {
+                const int t = remap ? (b & 7) * (in->nb >> 3) + (b >> 3) : b;
+                const int bx = t % in->gx, rest = t / in->gx;
+                cg.push_back({q, bx, rest % in->gy, rest / in->gy, b, {0, 0, 0}});
+            }
+            if (x4_ok_) {
+                const VTiling &v = in->vt;
+                for (int b = 0; b < v.nb; ++b) {
+                    const int t = v.remap ? (b & 7) * (v.nb >> 3) + (b >> 3) : b;
+                    const int bx = t % v.gx, rest = t / v.gx;
+                    cg4.push_back({q, bx, rest % v.gy, rest / v.gy, b, {0, 0, 0}});
+                }
+            }
+            // sphere source box (interior cells)
+            if (d.source_radius > 0.0) {
+                const int nn[3] = {d.nx, d.ny, d.nz};
+                int lo[3], hi[3];
+                bool empty = false;
+                for (int a = 0; a < 3; ++a) {
+                    lo[a] = std::max(1, (int)std::floor(d.source_center[a] - d.source_radius));
+                    hi[a] = std::min(nn[a] - 2, (int)std::ceil(d.source_center[a] + d.source_radius));
+                    if (hi[a] < lo[a]) empty = true;
+                }
+                if (!empty) {
+                    const int cnt = (hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1);
+                    for (int a = 0; a < cnt; a += kThreads) src.push_back(make_int2(q, a));
+                    for (int a = 0; a < 3; ++a) {
+                        box_lo_.push_back(lo[a]);
+                        box_n_.push_back(hi[a] - lo[a] + 1);
+                    }
+                } else {
+                    for (int a = 0; a < 3; ++a) {
+                        box_lo_.push_back(1);
+                        box_n_.push_back(0);
+                    }
+                }
+            } else {
+                for (int a = 0; a < 3; ++a) {
+                    box_lo_.push_back(1);
+                    box_n_.push_back(0);
+                }
+            }
+            inst_.push_back(std::move(in));
+        }
+        ncell_ = (int)cells.size();
+        ncg_ = (int)cg.size();
+        nsrc_ = (int)src.size();
+        cell_items_.alloc(ncell_);
+        cg_items_.alloc(ncg_);
+        ncg4_ = (int)cg4.size();
+        if (ncg4_) {
+            cg4_items_.alloc(ncg4_);
+            HIPT(hipMemcpyAsync(cg4_items_.p, cg4.data(), cg4.size() * sizeof(BItem),
+                                hipMemcpyHostToDevice, st_));
+        }
+        HIPT(hipMemcpyAsync(cell_items_.p, cells.data(), cells.size() * sizeof(BItem),
+                            hipMemcpyHostToDevice, st_));
+        HIPT(hipMemcpyAsync(cg_items_.p, cg.data(), cg.size() * sizeof(BItem),
+                            hipMemcpyHostToDevice, st_));
+        if (nsrc_) {
+            src_items_.alloc(nsrc_);
+            HIPT(hipMemcpyAsync(src_items_.p, src.data(), src.size() * sizeof(int2),
+                                hipMemcpyHostToDevice, st_));
+        }
+        table_.alloc(count);
+        HIPT(hipHostMalloc((void **)&done_host_, sizeof(int) * count, hipHostMallocMapped));
+        for (int q = 0; q < count; ++q) done_host_[q] = 0;
+        HIPT(hipHostGetDevicePointer((void **)&done_dev_, done_host_, 0));
+        HIPT(hipStreamSynchronize(st_));
+    }
+
+    ~Batch() override {
+        (void)hipStreamSynchronize(st_);
+        for (auto e : chunk_events_) (void)hipEventDestroy(e);
+        if (done_host_) (void)hipHostFree(done_host_);
+    }
+
+    int count() const override { return (int)inst_.size(); }
+    const std::vector<int> &iters() const override { return iters_; }
+
+    void fill(T *x, size_t n) {
+        const unsigned b = (unsigned)std::min<size_t>((n + kThreads - 1) / kThreads, 4096);
+        k_fill<T><<<std::max(b, 1u), kThreads, 0, st_>>>(x, (long)n, T(0));
+        LAUNCH_CHECK();
+    }
+
+    T *field(Inst &in, int field) {
+        const int slots[5] = {sl_.u, sl_.v, sl_.w, sl_.rho, sl_.t};
+        if (field >= 0 && field < 5) return in.f[slots[field]][field].p;
+        if (field == AETHER_FIELD_PRESSURE) return in.p.p;
+        if (field == AETHER_FIELD_DIVERGENCE) return in.b.p;
+        raise(AETHER_EINVAL, "unknown field id " + std::to_string(field));
+    }
+
+    Inst &inst(int q) {
+        if (q < 0 || q >= count()) raise(AETHER_EINVAL, "instance index out of range");
+        return *inst_[q];
+    }
+
+    void upload(int q, int f, const double *src, size_t n) override {
+        Inst &in = inst(q);
+        if (n != in.n) raise(AETHER_EINVAL, "upload size does not match the instance grid");
+        T *dst = field(in, f);
+        if (!in.staging.p) in.staging.alloc(in.n);
+        HIPT(hipMemcpyAsync(in.staging.p, src, n * sizeof(double), hipMemcpyHostToDevice, st_));
+        k_convert<double, T><<<4096, kThreads, 0, st_>>>(in.staging.p, (long)n, dst);
+        LAUNCH_CHECK();
+        HIPT(hipStreamSynchronize(st_));
+    }
+
+    void download(int q, int f, double *dst, size_t n) override {
+        Inst &in = inst(q);
+        if (n != in.n) raise(AETHER_EINVAL, "download size does not match the instance grid");
+        T *src = field(in, f);
+        if (!in.staging.p) in.staging.alloc(in.n);
+        k_convert<T, double><<<4096, kThreads, 0, st_>>>(src, (long)n, in.staging.p);
+        LAUNCH_CHECK();
+        HIPT(hipMemcpyAsync(dst, in.staging.p, n * sizeof(double), hipMemcpyDeviceToHost, st_));
+        HIPT(hipStreamSynchronize(st_));
+    }
+
+    void write_table() {
+        std::vector<BInst<T>> tab(count());
+        for (int q = 0; q < count(); ++q) {
+            Inst &in = *inst_[q];
+            BInst<T> &t = tab[q];
+            std::memset(&t, 0, sizeof(t));
+            t.g = in.g;
+            t.h = in.h;
+            for (int a = 0; a < 2; ++a)
+                for (int f = 0; f < 5; ++f) t.f[a][f] = in.f[a][f].p;
+            t.fwd0 = in.fwd0.p;
+            t.fwd1 = in.fwd1.p;
+            t.curl = in.curl.p;
+            t.p = in.p.p;
+            t.b = in.b.p;
+            t.r = in.r.p;
+            t.d[0] = in.dd[0].p;
+            t.d[1] = in.dd[1].p;
+            t.pa = in.pa.p;
+            t.pb = in.pb.p;
+            t.pc = in.pc.p;
+            t.ptel = in.ptel.p;
+            t.st = in.st.p;
+            t.nb = in.nb;
+            t.eps = in.d.vorticity_eps;
+            t.beta = in.d.buoyancy_beta;
+            t.alpha = in.d.gravity_alpha;
+            for (int a = 0; a < 3; ++a) {
+                t.sc[a] = in.d.source_center[a];
+                t.slo[a] = box_lo_[3 * q + a];
+                t.sn[a] = box_n_[3 * q + a];
+            }
+            t.sr2 = in.d.source_radius * in.d.source_radius;
+            t.srho = in.d.source_density;
+            t.stemp = in.d.source_temperature;
+            t.tel = in.tel.p;
+            t.iters = in.iters.p;
+            t.vt = in.vt;
+            t.nbv = in.vt.nb;
+        }
+        HIPT(hipMemcpyAsync(table_.p, tab.data(), tab.size() * sizeof(BInst<T>),
+                            hipMemcpyHostToDevice, st_));
+    }
+
+    void advect1(int f, double dt, double diss) {
+        kb_sl_forward<T, 1><<<ncell_, dim3(kBX, kBY), 0, st_>>>(table_.p, cell_items_.p, dt, sl_, f);
+        LAUNCH_CHECK();
+        kb_mac_correct<T, 1><<<ncell_, dim3(kBX, kBY), 0, st_>>>(table_.p, cell_items_.p, dt, sl_,
+                                                                f, diss, 0.0, 0);
+        LAUNCH_CHECK();
+        int *slot[5] = {&sl_.u, &sl_.v, &sl_.w, &sl_.rho, &sl_.t};
+        *slot[f] ^= 1;
+    }
+
+    // The batch's CG layout for the current knobs: four-wide or one cell per thread.
+    bool use_x4() const {
+        switch (params_.cg_form) {
+        case AETHER_CG_SCALAR:
+            return false;
+        case AETHER_CG_X4:
+            if (!x4_ok_) raise(AETHER_EINVAL, "AETHER_CG_X4 needs FP32 and nx % 4 == 0 in every instance");
+            return true;
+        case AETHER_CG_FUSED:
+            raise(AETHER_EINVAL, "AETHER_CG_FUSED is not available for batches");
+        default:
+            if (trial_form_) return trial_form_ == AETHER_CG_X4;
+            if (tuned_form_) return tuned_form_ == AETHER_CG_X4;
+            return x4_ok_;
+        }
+    }
+
+    // Times `iters` CG iterations of the batch in each layout (four-wide when
+    // every nx % 4 == 0, one cell per thread) on synthetic right-hand sides
+    // (every instance's pressure and divergence are saved and restored) and
+    // keeps the faster as the AETHER_CG_AUTO choice.  us[2]: time per
+    // iteration of the whole batch, four-wide then one cell per thread (-1
+    // when not applicable).
+    void autotune(int iters, double *us, int *form_out) override {
+        if (iters < 1) raise(AETHER_EINVAL, "iters must be >= 1");
+        for (auto &in : inst_) {  // finalize logs iteration 0's count
+            if (in->iters.n < 1) in->iters.alloc(1);
+            if (in->tel.n < 8) in->tel.alloc(8);
+        }
+        write_table();
+        std::vector<std::unique_ptr<DevBuf<T>>> saved;
+        for (auto &in : inst_) {
+            for (DevBuf<T> *src : {&in->p, &in->b}) {
+                saved.push_back(std::make_unique<DevBuf<T>>(in->n));
+                HIPT(hipMemcpyAsync(saved.back()->p, src->p, in->n * sizeof(T),
+                                    hipMemcpyDeviceToDevice, st_));
+            }
+            k_fill_hash<T><<<4096, kThreads, 0, st_>>>(in->b.p, (long)in->n);
+            LAUNCH_CHECK();
+        }
+        const aether_fluid_params keep = params_;
+        params_.cg_rel_tol = 0.0;  // absolute guards only: each layout runs `iters`
+        params_.solver_iters = iters;
+        hipEvent_t e0, e1;
+        HIPT(hipEventCreate(&e0));
+        HIPT(hipEventCreate(&e1));
+        const int forms[2] = {AETHER_CG_X4, AETHER_CG_SCALAR};
+        double best = 1e300;
+        int pick = AETHER_CG_SCALAR;
+        auto cleanup = [&] {
+            trial_form_ = 0;
+            params_ = keep;
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+        };
+        try {
+            for (int a = 0; a < 2; ++a) {
+                if (us) us[a] = -1.0;
+                if (forms[a] == AETHER_CG_X4 && !x4_ok_) continue;
+                trial_form_ = forms[a];
+                float ms = 0.0f;
+                for (int rep = 0; rep < 2; ++rep) {  // the first run warms up
+                    for (auto &in : inst_) fill(in->p.p, in->n);
+                    kb_div_cginit<T, kBatchZU><<<ncg_, dim3(kBX, kBY), 0, st_>>>(
+                        table_.p, cg_items_.p, sl_, 0);
+                    LAUNCH_CHECK();
+                    HIPT(hipEventRecord(e0, st_));
+                    cg_solve(0);
+                    HIPT(hipEventRecord(e1, st_));
+                    HIPT(hipEventSynchronize(e1));
+                    HIPT(hipEventElapsedTime(&ms, e0, e1));
+                }
+                const double t = 1e3 * (double)ms / iters;
+                if (us) us[a] = t;
+                if (t < best) {
+                    best = t;
+                    pick = forms[a];
+                }
+            }
+        } catch (...) {
+            cleanup();
+            throw;
+        }
+        cleanup();
+        tuned_form_ = pick;
+        size_t k = 0;
+        for (auto &in : inst_)
+            for (DevBuf<T> *dst : {&in->p, &in->b}) {
+                HIPT(hipMemcpyAsync(dst->p, saved[k]->p, in->n * sizeof(T),
+                                    hipMemcpyDeviceToDevice, st_));
+                ++k;
+            }
+        HIPT(hipStreamSynchronize(st_));
+        if (form_out) *form_out = pick;
+    }
+
+    void launch_cg4(int it, double rel2, int solve_id) {
+        if constexpr (sizeof(T) == 4) {
+            // p is brought up to date on odd iterations (kb_cg_update4)
+            const bool odd = it & 1;
+            const dim3 g(ncg4_), b(kThreads);
+#define KB_DIR4(NQ_)                                                                       \
+    do {                                                                                   \
+        if (it == 0)                                                                       \
+            kb_cg_dir4<true, NQ_, false><<<g, b, 0, st_>>>(table_.p, cg4_items_.p, it,     \
+                                                           rel2, done_dev_, solve_id);     \
+        else if (odd)                                                                      \
+            kb_cg_dir4<false, NQ_, true><<<g, b, 0, st_>>>(table_.p, cg4_items_.p, it,     \
+                                                           rel2, done_dev_, solve_id);     \
+        else                                                                               \
+            kb_cg_dir4<false, NQ_, false><<<g, b, 0, st_>>>(table_.p, cg4_items_.p, it,    \
+                                                            rel2, done_dev_, solve_id);    \
+        LAUNCH_CHECK();                                                                    \
+        if (odd)                                                                           \
+            kb_cg_update4<NQ_, kPBoth><<<g, b, 0, st_>>>(table_.p, cg4_items_.p, it,       \
+                                                         done_dev_, solve_id);             \
+        else                                                                               \
+            kb_cg_update4<NQ_, kPSkip><<<g, b, 0, st_>>>(table_.p, cg4_items_.p, it,       \
+                                                         done_dev_, solve_id);             \
+        LAUNCH_CHECK();                                                                    \
+    } while (0)
+            if (wide_)
+                KB_DIR4(16);
+            else
+                KB_DIR4(4);
+#undef KB_DIR4
+        }
+    }
+
+    // graph: the step is being captured, so every iteration is queued (no
+    // host look-ahead) and the step index comes from the device.
+    void cg_solve(int step, bool graph = false) {
+        const int max_iters = params_.solver_iters;
+        const bool x4 = use_x4();
+        const double rel2 = params_.cg_rel_tol > 0 ? params_.cg_rel_tol * params_.cg_rel_tol : 0.0;
+        const int solve_id = ++solve_seq_;
+        const int chunk = 8;
+        const int nchunks = (max_iters + chunk - 1) / chunk;
+        while ((int)chunk_events_.size() < nchunks) {
+            hipEvent_t e;
+            // pacing only (as the single-grid look-ahead events)
+            HIPT(hipEventCreateWithFlags(&e, hipEventDisableTiming | hipEventDisableSystemFence));
+            chunk_events_.push_back(e);
+        }
+        const dim3 blk(kBX, kBY);
+        for (int c = 0; c < nchunks; ++c) {
+            if (c >= 2 && !graph) {
+                HIPT(hipEventSynchronize(chunk_events_[c - 2]));
+                bool all = true;
+                for (int q = 0; q < count() && all; ++q)
+                    all = __atomic_load_n(done_host_ + q, __ATOMIC_ACQUIRE) == solve_id;
+                if (all) break;
+            }
+            for (int it = c * chunk; it < std::min(max_iters, (c + 1) * chunk); ++it) {
+                if (x4) {
+                    launch_cg4(it, rel2, solve_id);
+                    continue;
+                }
+                if (it == 0)
+                    kb_cg_dir<T, kBatchZU, true><<<ncg_, blk, 0, st_>>>(
+                        table_.p, cg_items_.p, it, rel2, done_dev_, solve_id);
+                else
+                    kb_cg_dir<T, kBatchZU, false><<<ncg_, blk, 0, st_>>>(
+                        table_.p, cg_items_.p, it, rel2, done_dev_, solve_id);
+                LAUNCH_CHECK();
+                kb_cg_update<T, kBatchZU><<<ncg_, blk, 0, st_>>>(table_.p, cg_items_.p, it,
+                                                                 done_dev_, solve_id);
+                LAUNCH_CHECK();
+            }
+            if (!graph) HIPT(hipEventRecord(chunk_events_[c], st_));
+        }
+        if constexpr (sizeof(T) == 4) {
+            if (x4 && (max_iters & 1)) {  // an odd cap leaves the last alpha d pending
+                kb_cg_p_tail<<<ncg4_, kThreads, 0, st_>>>(table_.p, cg4_items_.p, max_iters - 1);
+                LAUNCH_CHECK();
+            }
+        }
+        kb_cg_finalize<T><<<count(), blk, 0, st_>>>(table_.p, max_iters, rel2, step, x4 ? 1 : 0,
+                                                    graph ? step_dev_.p : nullptr);
+        LAUNCH_CHECK();
+    }
+
+    // src/fluid.rs:598-611 for every instance at once.  graph: the launches
+    // are being captured; the step index and time are read on the device.
+    void step(double dt, double t, int s, bool graph = false) {
+        const dim3 blk(kBX, kBY);
+        advect1(0, dt, params_.dissipation_velocity);
+        advect1(1, dt, params_.dissipation_velocity);
+        advect1(2, dt, params_.dissipation_velocity);
+        if (nsrc_) {
+            kb_sphere<T><<<nsrc_, blk, 0, st_>>>(table_.p, src_items_.p, sl_);
+            LAUNCH_CHECK();
+        }
+        kb_curl_mag<T><<<ncell_, blk, 0, st_>>>(table_.p, cell_items_.p, sl_);
+        LAUNCH_CHECK();
+        kb_confine_buoy_bc<T><<<ncell_, blk, 0, st_>>>(table_.p, cell_items_.p, dt, sl_);
+        LAUNCH_CHECK();
+        sl_.u ^= 1;
+        sl_.v ^= 1;
+        sl_.w ^= 1;
+        kb_div_cginit<T, kBatchZU><<<ncg_, blk, 0, st_>>>(table_.p, cg_items_.p, sl_, 1);
+        LAUNCH_CHECK();
+        cg_solve(s, graph);
+        kb_gradient_bc<T><<<ncell_, blk, 0, st_>>>(table_.p, cell_items_.p, sl_);
+        LAUNCH_CHECK();
+        kb_sl_forward<T, 2><<<ncell_, blk, 0, st_>>>(table_.p, cell_items_.p, dt, sl_, 3);
+        LAUNCH_CHECK();
+        kb_mac_correct<T, 2><<<ncell_, blk, 0, st_>>>(table_.p, cell_items_.p, dt, sl_, 3,
+                                                      params_.dissipation_density,
+                                                      params_.dissipation_temperature, 1);
+        LAUNCH_CHECK();
+        sl_.rho ^= 1;
+        sl_.t ^= 1;
+        kb_telemetry_part<T, kBatchZU><<<ncg_, blk, 0, st_>>>(table_.p, cg_items_.p, sl_);
+        LAUNCH_CHECK();
+        kb_telemetry_final<T><<<count(), blk, 0, st_>>>(table_.p, s, t,
+                                                        graph ? step_dev_.p : nullptr,
+                                                        graph ? times_dev_.p : nullptr);
+        LAUNCH_CHECK();
+        if (graph) {
+            kb_step_advance<<<1, 64, 0, st_>>>(step_dev_.p);
+            LAUNCH_CHECK();
+        }
+    }
+
+    // Captures one step into an executable graph (the host-side slot
+    // bookkeeping advances as if the step had run).
+    hipGraphExec_t capture_step(double dt) {
+        hipGraph_t graph = nullptr;
+        HIPT(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
+        try {
+            step(dt, 0.0, 0, true);
+        } catch (...) {
+            (void)hipStreamEndCapture(st_, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        HIPT(hipStreamEndCapture(st_, &graph));
+        hipGraphExec_t exec = nullptr;
+        const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        HIPT(e);
+        return exec;
+    }
+
+    // Steps as graph replays: the ping-pong slots of density/temperature
+    // alternate per step, so two graphs (one per parity) serve every step.
+    void run_graphs(double dt, const double *times, int nsteps) {
+        if (times_dev_.n < (size_t)nsteps) times_dev_.alloc(nsteps);
+        HIPT(hipMemcpyAsync(times_dev_.p, times, nsteps * sizeof(double), hipMemcpyHostToDevice,
+                            st_));
+        if (!step_dev_.p) step_dev_.alloc(1);
+        k_fill<int><<<1, kThreads, 0, st_>>>(step_dev_.p, 1L, 0);
+        LAUNCH_CHECK();
+        HIPT(hipStreamSynchronize(st_));
+        struct Execs {
+            hipGraphExec_t g[2] = {nullptr, nullptr};
+            ~Execs() {
+                for (auto e : g)
+                    if (e) (void)hipGraphExecDestroy(e);
+            }
+        } ex;
+        const Slots start = sl_;
+        ex.g[0] = capture_step(dt);
+        if (nsteps > 1) ex.g[1] = capture_step(dt);
+        sl_ = start;
+        for (int s = 0; s < nsteps; ++s) HIPT(hipGraphLaunch(ex.g[s & 1], st_));
+        if (nsteps & 1) {  // the slots the replays left current
+            sl_.rho ^= 1;
+            sl_.t ^= 1;
+        }
+        HIPT(hipStreamSynchronize(st_));
+    }
+
+    void run(double dt, const double *times, int nsteps, aether_fluid_telemetry *out) override {
+        if (nsteps <= 0) return;
+        for (auto &in : inst_) {
+            if (in->tel.n < (size_t)nsteps * 8) in->tel.alloc((size_t)nsteps * 8);
+            if (in->iters.n < (size_t)nsteps) in->iters.alloc(nsteps);
+        }
+        write_table();
+        if (params_.batch_graph)
+            run_graphs(dt, times, nsteps);
+        else
+            for (int s = 0; s < nsteps; ++s) step(dt, times[s], s);
+        const int cnt = count();
+        std::vector<double> raw((size_t)nsteps * 8);
+        iters_.assign((size_t)nsteps * cnt, 0);
+        std::vector<int> it(nsteps);
+        for (int q = 0; q < cnt; ++q) {
+            HIPT(hipMemcpyAsync(raw.data(), inst_[q]->tel.p, raw.size() * sizeof(double),
+                                hipMemcpyDeviceToHost, st_));
+            HIPT(hipMemcpyAsync(it.data(), inst_[q]->iters.p, nsteps * sizeof(int),
+                                hipMemcpyDeviceToHost, st_));
+            HIPT(hipStreamSynchronize(st_));
+            for (int s = 0; s < nsteps; ++s) {
+                iters_[(size_t)s * cnt + q] = it[s];
+                if (out) {
+                    aether_fluid_telemetry &o = out[(size_t)s * cnt + q];
+                    const double *r = raw.data() + (size_t)s * 8;
+                    o.time_sec = r[0];
+                    o.kinetic_energy = r[1];
+                    o.enstrophy = r[2];
+                    o.peak_vorticity = r[3];
+                    o.total_mass = r[4];
+                    for (int a = 0; a < 3; ++a) o.center_of_mass[a] = r[5 + a];
+                }
+            }
+        }
+    }
+
+   private:
+    aether_fluid_params params_;
+    hipStream_t st_;
+    std::vector<std::unique_ptr<Inst>> inst_;
+    std::vector<int> box_lo_, box_n_;
+    DevBuf<BInst<T>> table_;
+    DevBuf<BItem> cell_items_, cg_items_, cg4_items_;
+    DevBuf<int2> src_items_;
+    int ncell_ = 0, ncg_ = 0, nsrc_ = 0, ncg4_ = 0;
+    bool x4_ok_ = false;  // FP32 and every nx % 4 == 0: four-wide CG available
+    int tuned_form_ = 0;  // the autotuner's AETHER_CG_AUTO choice (0: none)
+    int trial_form_ = 0;  // layout forced while the autotuner times it
+    DevBuf<int> step_dev_;          // device step index of captured steps
+    DevBuf<double> times_dev_;      // their time axis
+    bool wide_ = false;   // some instance has more than 1024 CG partials
+    Slots sl_ = {0, 0, 0, 0, 0};
+    int *done_host_ = nullptr, *done_dev_ = nullptr;
+    int solve_seq_ = 0;
+    std::vector<hipEvent_t> chunk_events_;
+    std::vector<int> iters_;
+};
+
+}  // namespace
+}  // namespace aether
