@@ -15,29 +15,32 @@
 // on halos by other workgroups, so they ping-pong between two buffers; d and
 // p are updated in place.
 //
-// d and p are brought up to date every other launch (PUPD: odd `it`).  An
-// even launch neither loads nor stores them; the odd launch after it
-// rebuilds the skipped direction from the residual that launch read, which
-// is still in this launch's output buffer at the thread's own cells:
-//   d_(it-1) = r_(it-1) + beta_(it-1) d_(it-2)      (d_0 = r_0)
+// d and p are brought up to date every third launch (PH = it mod 3 = 2).
+// The other launches neither load nor store them; the third rebuilds the
+// two directions they skipped from the residuals those launches read, which
+// the three-array residual rotation still holds at the thread's own cells
+// (r_(it-2) in this launch's output array, r_(it-1) in r_prev):
+//   d_(it-2) = r_(it-2) + beta_(it-2) d_(it-3)      (d_0 = r_0)
+//   d_(it-1) = r_(it-1) + beta_(it-1) d_(it-2)
 //   d_it     = r_it + beta_it d_(it-1)
-//   p <- (p + alpha_(it-1) d_(it-1)) + alpha_it d_it
+//   p <- ((p + alpha_(it-2) d_(it-2)) + alpha_(it-1) d_(it-1)) + alpha_it d_it
 // the same FP32 operations in the same order as updating both every launch,
-// so the iterates are bit-identical.  A launch that stops an odd iteration
-// applies the pending p term; when the iteration cap ends the solve after
-// an even launch, k_cg_cgs4_tail does.  8.5 array passes per iteration on
-// average (11 on odd launches, 6 on even ones).
+// so the iterates are bit-identical.  A launch that stops the solve applies
+// the p terms left pending before it; when the iteration cap ends it after
+// another launch, k_cg_cgs4_tail does.  8 array passes per iteration on
+// average (12 on every third launch, 6 on the others).
 #pragma once
 #include "cg_vec.hpp"
 
 namespace aether {
 
-template <int ZU, bool FIRST, int NQ, int ST, bool PUPD>
+template <int ZU, bool FIRST, int NQ, int ST, int PH>
 __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
     Dims g, VTiling vt, int it, double rel_tol2, int nb_in, const double *__restrict__ pg_in,
     const double *__restrict__ pd_in, const double *__restrict__ pbb, double *__restrict__ pg_out,
     double *__restrict__ pd_out, CgState *st, const float *__restrict__ r_in,
-    float *__restrict__ r_out, const float *__restrict__ w_in, float *__restrict__ w_out,
+    float *__restrict__ r_out, const float *__restrict__ r_prev,
+    const float *__restrict__ w_in, float *__restrict__ w_out,
     const float *__restrict__ s_in, float *__restrict__ s_out, float *__restrict__ d,
     float *__restrict__ p, int *done_mirror, int solve_id) {
     int bx, by, bz;
@@ -51,16 +54,22 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
     eg.load(pg_in, nb_in);
     ed.load(pd_in, nb_in);
     if (FIRST) eb.load(pbb, nb_in);
-    double gprev = 1.0, aprev = 1.0, bprev = 0.0, b2 = 0.0;
+    double gprev = 1.0, aprev = 1.0, aprev2 = 0.0, bprev = 0.0, bprev2 = 0.0, b2 = 0.0;
     if (!FIRST) {
         gprev = st->cg_gamma[(it - 1) & 1];
-        aprev = st->cg_alpha[(it - 1) & 1];
+        aprev = st->cg_alpha[(it - 1) & 3];
         b2 = st->b2;
     }
-    if (PUPD) bprev = st->cg_beta[(it - 1) & 1];
+    if (PH >= 1) bprev = st->cg_beta[(it - 1) & 3];
+    if (PH == 2) {
+        aprev2 = st->cg_alpha[(it - 2) & 3];
+        bprev2 = st->cg_beta[(it - 2) & 3];
+    }
     const float4 *r4 = reinterpret_cast<const float4 *>(r_in);
-    // r_(it-1), read at this thread's own cells before they are overwritten
-    const float4 *ro4 = reinterpret_cast<const float4 *>(r_out);
+    // r_(it-2) (read at this thread's own cells before they are overwritten)
+    // and r_(it-1)
+    const float4 *ro2 = reinterpret_cast<const float4 *>(r_out);
+    const float4 *ro1 = reinterpret_cast<const float4 *>(r_prev);
     const float4 *w4 = reinterpret_cast<const float4 *>(w_in);
     const float4 *s4 = reinterpret_cast<const float4 *>(s_in);
     float4 *d4 = reinterpret_cast<float4 *>(d);
@@ -107,7 +116,7 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
     // the stopping one take the early exit: a workgroup of the stopping
     // launch may already see that launch's word (written mid-launch by
     // workgroup 0), and it must still reach the stop below, which applies
-    // the pending p term on odd launches.
+    // the pending p terms.
     if (threadIdx.x == 0) flag_lds[0] = flag_reg != 0 && flag_reg <= it;
     double tot[2] = {eg.local(pg_in, nb_in), ed.local(pd_in, nb_in)};
     block_sum<2, false>(tot, lds);  // its barrier publishes flag_lds
@@ -137,16 +146,24 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
             st->converged = 1;
             *done_mirror = solve_id;
         }
-        if (PUPD) {  // the pending term of launch it - 1
-            const float tap = (float)aprev, tbp = (float)bprev;
+        if (PH >= 1) {  // the terms of the launches since the last third one
+            const float tap = (float)aprev, tbp = (float)bprev, tap2 = (float)aprev2,
+                        tbp2 = (float)bprev2;
 #pragma unroll
             for (int a = 0; a < ZU; ++a) {
                 const int k = c.k0 + a;
                 if (c.valid && k < g.nz) {
                     const int oc = (k * g.sxy + c.off) >> 2;
-                    const float4 rv = ro4[oc];
-                    const float4 dp = it == 1 ? rv : axpy4(rv, tbp, d4[oc]);
-                    o_p.put(oc, axpy4(p4[oc], tap, dp));
+                    const float4 dv = d4[oc], r1 = ro1[oc];
+                    if (PH == 1) {  // alpha_(it-1) d_(it-1), d_(it-2) stored
+                        const float4 d1 = it == 1 ? r1 : axpy4(r1, tbp, dv);
+                        o_p.put(oc, axpy4(p4[oc], tap, d1));
+                    } else {  // and alpha_(it-2) d_(it-2), d_(it-3) stored
+                        const float4 r2 = ro2[oc];
+                        const float4 d2 = it == 2 ? r2 : axpy4(r2, tbp2, dv);
+                        const float4 d1 = axpy4(r1, tbp, d2);
+                        o_p.put(oc, axpy4(axpy4(p4[oc], tap2, d2), tap, d1));
+                    }
                 }
             }
         }
@@ -155,10 +172,11 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
     const double alpha = gam / mu;
     if (lead) {
         st->cg_gamma[it & 1] = gam;
-        st->cg_alpha[it & 1] = alpha;
-        st->cg_beta[it & 1] = beta;
+        st->cg_alpha[it & 3] = alpha;
+        st->cg_beta[it & 3] = beta;
     }
-    const float tb = (float)beta, ta = (float)alpha, tap = (float)aprev, tbp = (float)bprev;
+    const float tb = (float)beta, ta = (float)alpha, tap = (float)aprev, tbp = (float)bprev,
+                tap2 = (float)aprev2, tbp2 = (float)bprev2;
 
     // ---- s = w + beta s and r' = r - alpha s on the radius-1 neighbourhood
     float4 rn[ZU + 2], rny[ZU][2], sn[ZU];
@@ -182,18 +200,24 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
         }
     }
 
-    // ---- odd launches: d_(it-1), p.  Loaded only now, after the r, w, s
-    // registers above are consumed, so that the launch keeps two waves per
-    // SIMD; the stencil below overlaps their latency.
-    float4 dp[ZU], pc[ZU];
-    if (PUPD) {
+    // ---- every third launch: d_(it-2), d_(it-1), d_it and p, finished and
+    // stored at once.  Loaded only now, after the r, w, s registers above
+    // are consumed, and held by nothing below, so that the launch keeps two
+    // waves per SIMD.
+    if (PH == 2) {
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int a = 0; a < ZU; ++a) {
-            const int oc = (min(c.k0 + a, g.nz - 1) * g.sxy + c.off) >> 2;
-            const float4 rv = ro4[oc], dv = d4[oc];
-            dp[a] = it == 1 ? rv : axpy4(rv, tbp, dv);  // d_(it-1); d_0 = r_0
-            pc[a] = p4[oc];
+            const int k = c.k0 + a;
+            const int oc = (min(k, g.nz - 1) * g.sxy + c.off) >> 2;
+            const float4 r2 = ro2[oc], r1 = ro1[oc], dv = d4[oc], pv = p4[oc];
+            const float4 d2 = it == 2 ? r2 : axpy4(r2, tbp2, dv);  // d_(it-2); d_0 = r_0
+            const float4 d1 = axpy4(r1, tbp, d2);                 // d_(it-1)
+            const float4 dn = axpy4(rz[a + 1], tb, d1);            // d_it
+            if (c.valid && k < g.nz) {
+                o_d.put(oc, dn);
+                o_p.put(oc, axpy4(axpy4(axpy4(pv, tap2, d2), tap, d1), ta, dn));
+            }
         }
     }
 
@@ -220,11 +244,6 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
             o_r.put(id4, rn[a + 1]);
             o_w.put(id4, wn);
             o_s.put(id4, sn[a]);
-            if (PUPD) {
-                const float4 dn = axpy4(rz[a + 1], tb, dp[a]);
-                o_d.put(id4, dn);
-                o_p.put(id4, axpy4(axpy4(pc[a], tap, dp[a]), ta, dn));
-            }
         }
     }
     block_sum<2, false>(acc, lds_out);
@@ -234,21 +253,32 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
     }
 }
 
-// Closes a solve whose iteration cap ended it after an even launch `last_it`
-// of k_cg_cgs4, which left d_last unbuilt and alpha_last d_last pending:
-// d_last = r_last + beta_last d_(last-1) (d_0 = r_0), p += alpha_last d_last,
-// with r_last the residual that launch read.  Skipped when a launch stopped
-// the solve (that launch applied what was pending).
+// Closes a solve whose iteration cap ended it after launch `last_it` of
+// k_cg_cgs4 when that was not a third launch (last_it mod 3 = 0 or 1): it
+// rebuilds the directions since the last third launch and applies their
+// pending p terms, in the order the third launch would have.  r_lm1 and
+// r_l: the residuals launches last_it - 1 and last_it read.  Skipped when a
+// launch stopped the solve (that launch applied what was pending).
 __global__ __launch_bounds__(kThreads) void k_cg_cgs4_tail(const CgState *st, int last_it,
-                                                           const float4 *__restrict__ r_last,
+                                                           const float4 *__restrict__ r_lm1,
+                                                           const float4 *__restrict__ r_l,
                                                            const float4 *__restrict__ d,
                                                            float4 *__restrict__ p, long n4) {
     if (st->done) return;
-    const float ta = (float)st->cg_alpha[last_it & 1], tb = (float)st->cg_beta[last_it & 1];
+    const int L = last_it;
+    const float ta = (float)st->cg_alpha[L & 3], tb = (float)st->cg_beta[L & 3];
+    const float ta1 = L >= 1 ? (float)st->cg_alpha[(L - 1) & 3] : 0.0f;
+    const float tb1 = L >= 1 ? (float)st->cg_beta[(L - 1) & 3] : 0.0f;
     for (long a = (long)blockIdx.x * kThreads + threadIdx.x; a < n4;
          a += (long)gridDim.x * kThreads) {
-        const float4 dl = last_it == 0 ? r_last[a] : axpy4(r_last[a], tb, d[a]);
-        p[a] = axpy4(p[a], ta, dl);
+        if (L % 3 == 0) {  // alpha_L d_L; d_(L-1) stored
+            const float4 dl = L == 0 ? r_l[a] : axpy4(r_l[a], tb, d[a]);
+            p[a] = axpy4(p[a], ta, dl);
+        } else {  // alpha_(L-1) d_(L-1) + alpha_L d_L; d_(L-2) stored
+            const float4 d1 = L == 1 ? r_lm1[a] : axpy4(r_lm1[a], tb1, d[a]);
+            const float4 dl = axpy4(r_l[a], tb, d1);
+            p[a] = axpy4(axpy4(p[a], ta1, d1), ta, dl);
+        }
     }
 }
 

@@ -42,9 +42,9 @@ struct CgState {
     int iters;
     int converged;
     int pad;
-    double cg_gamma[2];  // single-reduction CG: r.r, alpha and beta of the
-    double cg_alpha[2];  // current iteration, double-buffered by iteration
-    double cg_beta[2];   // parity
+    double cg_gamma[2];  // CG scalars by iteration: r.r (two slots, by parity),
+    double cg_alpha[4];  // alpha and beta (four slots, by it mod 4: the
+    double cg_beta[4];   // one-launch CG reads up to two iterations back)
 };
 
 __device__ __forceinline__ int tid_lin() { return threadIdx.y * kBX + threadIdx.x; }

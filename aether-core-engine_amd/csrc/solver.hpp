@@ -478,18 +478,25 @@ class Solver final : public SolverBase {
                 double rt = rel2;
                 const T *w_in = w_[it & 1].p, *s_in = s_[it & 1].p;
                 T *w_out = w_[(it + 1) & 1].p, *s_out = s_[(it + 1) & 1].p, *dd = d_[0].p;
+                // r rotates through three arrays: launch `it` reads R[it % 3],
+                // writes R[(it + 1) % 3] (which holds r_(it-2)) and finds
+                // r_(it-1) in R[(it + 2) % 3]
+                T *R[3] = {r_.p, r2_.p, r3_.p};
+                const T *r4_in = R[it % 3], *r4_prev = R[(it + 2) % 3];
+                T *r4_out = R[(it + 1) % 3];
                 void *args[] = {&g,     &vt,   &itv,  &rt,    &nbv,  &pg_in, &pd_in,
-                                &pbb,   &pg_out, &pd_out, &st, &r_in, &r_out, &w_in,
-                                &w_out, &s_in, &s_out, &dd, &pp, &dm, &sid};
+                                &pbb,   &pg_out, &pd_out, &st, &r4_in, &r4_out, &r4_prev,
+                                &w_in,  &w_out, &s_in, &s_out, &dd, &pp, &dm, &sid};
                 const bool wide = std::max(nb_in, vt.nb) > 4 * kThreads;
-                const bool odd = it & 1;  // p is updated on odd launches
+                const int ph = it % 3;  // d and p are updated on every third launch
                 // write-through stores (arrays under 2 GiB), unless
                 // AETHER_CG_NT asks for non-temporal ones
                 const int pol = nt_ ? kStNT : wt_ok() ? kStWT : kStPlain;
 #define CGS4_PICK(NQ_, ST_)                                                            \
-    (it == 0 ? (const void *)k_cg_cgs4<4, true, NQ_, ST_, false>                       \
-     : odd   ? (const void *)k_cg_cgs4<4, false, NQ_, ST_, true>                       \
-             : (const void *)k_cg_cgs4<4, false, NQ_, ST_, false>)
+    (it == 0   ? (const void *)k_cg_cgs4<4, true, NQ_, ST_, 0>                         \
+     : ph == 1 ? (const void *)k_cg_cgs4<4, false, NQ_, ST_, 1>                        \
+     : ph == 2 ? (const void *)k_cg_cgs4<4, false, NQ_, ST_, 2>                        \
+               : (const void *)k_cg_cgs4<4, false, NQ_, ST_, 0>)
                 const void *fn;
                 if (wide)
                     fn = pol == kStWT ? CGS4_PICK(16, kStWT) : CGS4_PICK(16, kStPlain);
@@ -532,7 +539,7 @@ class Solver final : public SolverBase {
     }
 
     // The CG layout of the next solve (AETHER_CG_SCALAR/X4/FUSED).
-    // Untuned choice for FP32.  The four-wide one-launch CG moves 8.5 array
+    // Untuned choice for FP32.  The four-wide one-launch CG moves 8 array
     // passes per iteration on average in one launch, the two-launch form 7.5
     // in two: the one launch wins while its eight FP32 arrays (32 N bytes) stay resident
     // in the 256 MB Infinity Cache next to the step's other fields (measured:
@@ -673,7 +680,7 @@ class Solver final : public SolverBase {
             partF_.alloc(npart);
         }
         if (sizeof(T) == 4 && tl4_.nb > 0 && !w_[0].p)
-            for (DevBuf<T> *b : {&w_[0], &w_[1], &s_[0], &s_[1]}) {
+            for (DevBuf<T> *b : {&w_[0], &w_[1], &s_[0], &s_[1], &r3_}) {
                 b->alloc(n_);
                 zero(b->p);
             }
@@ -838,18 +845,21 @@ class Solver final : public SolverBase {
             HIPT(hipEventRecord(chunk_events_[c], st_));
         }
         if constexpr (sizeof(T) == 4) {
-            // the four-wide forms bring p up to date on odd iterations: a cap
-            // that ends the solve after an even one leaves its alpha d pending
+            // the four-wide two-launch form brings p up to date on odd
+            // iterations, the one-launch form on every third: a cap that ends
+            // the solve on another iteration leaves terms pending
             const bool fused4 = form_ == AETHER_CG_FUSED && tl4_.nb > 0;
-            if ((fused4 || form_ == AETHER_CG_X4) && (max_iters & 1)) {
+            if (max_iters > 0 && ((fused4 && max_iters % 3 != 0) ||
+                                  (form_ == AETHER_CG_X4 && (max_iters & 1)))) {
                 const long n4 = (long)(n_ / 4);
                 const unsigned nbt =
                     (unsigned)std::min<long>((n4 + kThreads - 1) / kThreads, 4096);
                 const int last = max_iters - 1;
-                if (fused4) {  // the one-launch form also left d_last unbuilt
-                    const T *rl = (last & 1) ? r2_.p : r_.p;  // the residual launch `last` read
+                if (fused4) {  // the one-launch form also left the directions unbuilt
+                    const T *R[3] = {r_.p, r2_.p, r3_.p};  // residual launch `it` read: R[it % 3]
                     k_cg_cgs4_tail<<<nbt, kThreads, 0, st_>>>(
-                        cg_.p, last, reinterpret_cast<const float4 *>(rl),
+                        cg_.p, last, reinterpret_cast<const float4 *>(R[(last + 2) % 3]),
+                        reinterpret_cast<const float4 *>(R[last % 3]),
                         reinterpret_cast<const float4 *>(d_[0].p),
                         reinterpret_cast<float4 *>(p_.p), n4);
                 } else {
@@ -1591,6 +1601,7 @@ class Solver final : public SolverBase {
     DevBuf<T> p2_;                 // second pressure buffer of the one-launch SOR sweep
     VTiling tls_ = {0, 0, 0, 0, 0, 1, 1};  // its tiling (2 planes per thread; nb 0: unusable)
     DevBuf<T> w_[2], s_[2];        // A r and A d of the four-wide one-launch CG
+    DevBuf<T> r3_;                 // its third residual array
     DevBuf<double> partE_, partF_;  // its delta partials
     bool ir64_ = false;           // mixed-precision projection (AETHER_FP32_IR64)
     // rollout / adjoint state
