@@ -1,34 +1,41 @@
 // cg_cgs4.hpp -- one launch per CG iteration in the single-reduction form
-// (Chronopoulos-Gear) with the products w = A r and s = A d carried in
-// memory, four x-cells per thread.
+// (Chronopoulos-Gear) with the product w = A r carried in memory, four
+// x-cells per thread.
 //
 // Launch `it` holds gamma_it = r.r and delta_it = r.w as partials of the
 // previous launch; with beta = gamma_it / gamma_(it-1) and
 // mu = delta_it - beta gamma_it / alpha_(it-1) (= d.A d), alpha = gamma_it / mu:
-//   d <- r + beta d        s <- w + beta s        p <- p + alpha d
-//   r' = r - alpha s       w' = A r'              partials of r'.r', r'.w'
+//   s = A d = w + beta A d_(it-1)          d <- r + beta d     p <- p + alpha d
+//   r' = r - alpha s       w' = A r'       partials of r'.r', r'.w'
 // The reference's loop (src/fluid.rs:463-487) produces the same iterates in
-// exact arithmetic.  Because s and w are stored, r' is needed only on a
-// radius-1 neighbourhood, which each thread forms from the r, w and s it
-// loads there (y and z halos, x edges at segment ends) and from its
-// neighbour lanes (x): no LDS exchange, one stencil.  r, w and s are read
-// on halos by other workgroups, so they ping-pong between two buffers; d and
-// p are updated in place.
+// exact arithmetic.  A d_(it-1) is not stored: the previous launch formed
+// r = r_(it-1) - alpha_(it-1) A d_(it-1), so
+//   s = w + (beta / alpha_(it-1)) (r_(it-1) - r)
+// (s = w on the first launch).  In exact CG r_(it-1) and r are orthogonal,
+// so |r_(it-1) - r| >= |r_(it-1)| and the difference loses no significant
+// digits.  r' is needed only on a radius-1 neighbourhood, which each thread
+// forms from the r, r_(it-1) and w it loads there (y and z halos, x edges
+// at segment ends) and from its neighbour lanes (x): no LDS exchange, one
+// stencil.  r and w are read on halos by other workgroups: w ping-pongs
+// between two buffers and r rotates through three (launch `it` reads
+// R[it % 3], finds r_(it-1) in R[(it + 2) % 3] and writes R[(it + 1) % 3],
+// which held r_(it-2)); d and p are updated in place.
 //
 // d and p are brought up to date every third launch (PH = it mod 3 = 2).
 // The other launches neither load nor store them; the third rebuilds the
 // two directions they skipped from the residuals those launches read, which
-// the three-array residual rotation still holds at the thread's own cells
-// (r_(it-2) in this launch's output array, r_(it-1) in r_prev):
+// the rotation still holds at the thread's own cells (r_(it-2) in this
+// launch's output array, read before it is overwritten; r_(it-1)):
 //   d_(it-2) = r_(it-2) + beta_(it-2) d_(it-3)      (d_0 = r_0)
 //   d_(it-1) = r_(it-1) + beta_(it-1) d_(it-2)
 //   d_it     = r_it + beta_it d_(it-1)
 //   p <- ((p + alpha_(it-2) d_(it-2)) + alpha_(it-1) d_(it-1)) + alpha_it d_it
-// the same FP32 operations in the same order as updating both every launch,
-// so the iterates are bit-identical.  A launch that stops the solve applies
-// the p terms left pending before it; when the iteration cap ends it after
-// another launch, k_cg_cgs4_tail does.  8 array passes per iteration on
-// average (12 on every third launch, 6 on the others).
+// the same FP32 operations in the same order as updating both every launch.
+// A launch that stops the solve applies the p terms left pending before it;
+// when the iteration cap ends it after another launch, k_cg_cgs4_tail does.
+// Array passes per iteration: r, r_(it-1), w read and r', w' written (5),
+// plus r_(it-2), d, p read and d, p written every third launch: 6.67 on
+// average (10 on every third launch, 5 on the others).
 #pragma once
 #include "cg_vec.hpp"
 
@@ -40,8 +47,7 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
     const double *__restrict__ pd_in, const double *__restrict__ pbb, double *__restrict__ pg_out,
     double *__restrict__ pd_out, CgState *st, const float *__restrict__ r_in,
     float *__restrict__ r_out, const float *__restrict__ r_prev,
-    const float *__restrict__ w_in, float *__restrict__ w_out,
-    const float *__restrict__ s_in, float *__restrict__ s_out, float *__restrict__ d,
+    const float *__restrict__ w_in, float *__restrict__ w_out, float *__restrict__ d,
     float *__restrict__ p, int *done_mirror, int solve_id) {
     int bx, by, bz;
     vtile_coords(vt, bx, by, bz);
@@ -71,21 +77,20 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
     const float4 *ro2 = reinterpret_cast<const float4 *>(r_out);
     const float4 *ro1 = reinterpret_cast<const float4 *>(r_prev);
     const float4 *w4 = reinterpret_cast<const float4 *>(w_in);
-    const float4 *s4 = reinterpret_cast<const float4 *>(s_in);
     float4 *d4 = reinterpret_cast<float4 *>(d);
     float4 *p4 = reinterpret_cast<float4 *>(p);
     const long nbytes = (long)g.sxy * g.nz * 4;
-    const Out4<ST> o_r(r_out, nbytes), o_w(w_out, nbytes), o_s(s_out, nbytes), o_d(d, nbytes),
-        o_p(p, nbytes);
+    const Out4<ST> o_r(r_out, nbytes), o_w(w_out, nbytes), o_d(d, nbytes), o_p(p, nbytes);
     const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-    float4 rz[ZU + 2], wz[ZU + 2], sz[ZU + 2], ry[ZU][2], wy[ZU][2], sy[ZU][2];
-    float er[ZU][2], ew[ZU][2], es[ZU][2];
+    // r, w and r_(it-1) on the radius-1 neighbourhood (z column, y rows, x edges)
+    float4 rz[ZU + 2], wz[ZU + 2], qz[ZU + 2], ry[ZU][2], wy[ZU][2], qy[ZU][2];
+    float er[ZU][2], ew[ZU][2], eq[ZU][2];
 #pragma unroll
     for (int a = 0; a < ZU + 2; ++a) {
         const int o = (min(max(c.k0 - 1 + a, 0), g.nz - 1) * g.sxy + c.off) >> 2;
         rz[a] = r4[o];
         wz[a] = w4[o];
-        sz[a] = FIRST ? z4 : s4[o];
+        qz[a] = FIRST ? z4 : ro1[o];
     }
 #pragma unroll
     for (int a = 0; a < ZU; ++a) {
@@ -95,19 +100,19 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
         ry[a][1] = r4[op];
         wy[a][0] = w4[om];
         wy[a][1] = w4[op];
-        sy[a][0] = FIRST ? z4 : s4[om];
-        sy[a][1] = FIRST ? z4 : s4[op];
+        qy[a][0] = FIRST ? z4 : ro1[om];
+        qy[a][1] = FIRST ? z4 : ro1[op];
 #pragma unroll
-        for (int e = 0; e < 2; ++e) er[a][e] = ew[a][e] = es[a][e] = 0.0f;
+        for (int e = 0; e < 2; ++e) er[a][e] = ew[a][e] = eq[a][e] = 0.0f;
         if (c.needL) {
             er[a][0] = r_in[base + c.off - 1];
             ew[a][0] = w_in[base + c.off - 1];
-            if (!FIRST) es[a][0] = s_in[base + c.off - 1];
+            if (!FIRST) eq[a][0] = r_prev[base + c.off - 1];
         }
         if (c.needR) {
             er[a][1] = r_in[base + c.off + 4];
             ew[a][1] = w_in[base + c.off + 4];
-            if (!FIRST) es[a][1] = s_in[base + c.off + 4];
+            if (!FIRST) eq[a][1] = r_prev[base + c.off + 4];
         }
     }
 
@@ -177,14 +182,16 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
     }
     const float tb = (float)beta, ta = (float)alpha, tap = (float)aprev, tbp = (float)bprev,
                 tap2 = (float)aprev2, tbp2 = (float)bprev2;
+    // beta / alpha_(it-1): scales r_(it-1) - r into beta A d_(it-1)
+    const float tc = FIRST ? 0.0f : (float)(beta / aprev);
 
-    // ---- s = w + beta s and r' = r - alpha s on the radius-1 neighbourhood
-    float4 rn[ZU + 2], rny[ZU][2], sn[ZU];
+    // ---- s = w + (beta / alpha_(it-1)) (r_(it-1) - r) and r' = r - alpha s
+    // on the radius-1 neighbourhood
+    float4 rn[ZU + 2], rny[ZU][2];
     float rne[ZU][2];
 #pragma unroll
     for (int a = 0; a < ZU + 2; ++a) {
-        const float4 sv = FIRST ? wz[a] : axpy4(wz[a], tb, sz[a]);
-        if (a >= 1 && a <= ZU) sn[a - 1] = sv;
+        const float4 sv = FIRST ? wz[a] : axpy4(wz[a], tc, sub4(qz[a], rz[a]));
         rn[a] = make_float4(rz[a].x - ta * sv.x, rz[a].y - ta * sv.y, rz[a].z - ta * sv.z,
                             rz[a].w - ta * sv.w);
     }
@@ -192,10 +199,10 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
     for (int a = 0; a < ZU; ++a) {
 #pragma unroll
         for (int e = 0; e < 2; ++e) {
-            const float4 sv = FIRST ? wy[a][e] : axpy4(wy[a][e], tb, sy[a][e]);
+            const float4 sv = FIRST ? wy[a][e] : axpy4(wy[a][e], tc, sub4(qy[a][e], ry[a][e]));
             rny[a][e] = make_float4(ry[a][e].x - ta * sv.x, ry[a][e].y - ta * sv.y,
                                     ry[a][e].z - ta * sv.z, ry[a][e].w - ta * sv.w);
-            const float se = FIRST ? ew[a][e] : ew[a][e] + tb * es[a][e];
+            const float se = FIRST ? ew[a][e] : ew[a][e] + tc * (eq[a][e] - er[a][e]);
             rne[a][e] = er[a][e] - ta * se;
         }
     }
@@ -243,7 +250,6 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
             const int id4 = (k * g.sxy + c.off) >> 2;
             o_r.put(id4, rn[a + 1]);
             o_w.put(id4, wn);
-            o_s.put(id4, sn[a]);
         }
     }
     block_sum<2, false>(acc, lds_out);

@@ -103,6 +103,9 @@ __device__ __forceinline__ float f4(const float4 &v, int e) {
 __device__ __forceinline__ float4 axpy4(const float4 &x, float a, const float4 &y) {
     return make_float4(x.x + a * y.x, x.y + a * y.y, x.z + a * y.z, x.w + a * y.w);
 }
+__device__ __forceinline__ float4 sub4(const float4 &x, const float4 &y) {
+    return make_float4(x.x - y.x, x.y - y.y, x.z - y.z, x.w - y.w);
+}
 // value of lane `src` (every lane of the wave must execute this)
 __device__ __forceinline__ float lane_value(float v, int src) {
     return __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(v)));

@@ -471,13 +471,13 @@ class Solver final : public SolverBase {
                 e1 = timer.get();
                 timer.recs.push_back({3, step_index, it, e0, e1});
             }
-            if (tl4_.nb > 0) {  // four-wide, A r and A d carried (cg_cgs4.hpp)
+            if (tl4_.nb > 0) {  // four-wide, A r carried (cg_cgs4.hpp)
                 Dims g = g_;
                 VTiling vt = tl4_;
                 int itv = it, nbv = nb_in, sid = solve_id;
                 double rt = rel2;
-                const T *w_in = w_[it & 1].p, *s_in = s_[it & 1].p;
-                T *w_out = w_[(it + 1) & 1].p, *s_out = s_[(it + 1) & 1].p, *dd = d_[0].p;
+                const T *w_in = w_[it & 1].p;
+                T *w_out = w_[(it + 1) & 1].p, *dd = d_[0].p;
                 // r rotates through three arrays: launch `it` reads R[it % 3],
                 // writes R[(it + 1) % 3] (which holds r_(it-2)) and finds
                 // r_(it-1) in R[(it + 2) % 3]
@@ -486,7 +486,7 @@ class Solver final : public SolverBase {
                 T *r4_out = R[(it + 1) % 3];
                 void *args[] = {&g,     &vt,   &itv,  &rt,    &nbv,  &pg_in, &pd_in,
                                 &pbb,   &pg_out, &pd_out, &st, &r4_in, &r4_out, &r4_prev,
-                                &w_in,  &w_out, &s_in, &s_out, &dd, &pp, &dm, &sid};
+                                &w_in,  &w_out, &dd, &pp, &dm, &sid};
                 const bool wide = std::max(nb_in, vt.nb) > 4 * kThreads;
                 const int ph = it % 3;  // d and p are updated on every third launch
                 // write-through stores (arrays under 2 GiB), unless
@@ -680,7 +680,7 @@ class Solver final : public SolverBase {
             partF_.alloc(npart);
         }
         if (sizeof(T) == 4 && tl4_.nb > 0 && !w_[0].p)
-            for (DevBuf<T> *b : {&w_[0], &w_[1], &s_[0], &s_[1], &r3_}) {
+            for (DevBuf<T> *b : {&w_[0], &w_[1], &r3_}) {
                 b->alloc(n_);
                 zero(b->p);
             }
@@ -1600,7 +1600,7 @@ class Solver final : public SolverBase {
     DevBuf<T> r2_;                 // second residual buffer of the one-launch CG
     DevBuf<T> p2_;                 // second pressure buffer of the one-launch SOR sweep
     VTiling tls_ = {0, 0, 0, 0, 0, 1, 1};  // its tiling (2 planes per thread; nb 0: unusable)
-    DevBuf<T> w_[2], s_[2];        // A r and A d of the four-wide one-launch CG
+    DevBuf<T> w_[2];               // A r of the four-wide one-launch CG (ping-pong)
     DevBuf<T> r3_;                 // its third residual array
     DevBuf<double> partE_, partF_;  // its delta partials
     bool ir64_ = false;           // mixed-precision projection (AETHER_FP32_IR64)

@@ -67,9 +67,10 @@ def kernel_roofline(solver_kind: str, nvox: int, steps: int = 2):
     and writes d (3 passes); cg_update reads p, d, r and writes p, r (5
     passes); a launch that runs whole CG iterations counts the same 8
     passes per iteration: the register-resident solve (per iteration it
-    runs) and the one-launch iteration (cg_fused; the four-wide one moves 8
-    passes on average, carrying A r and A d and touching d and p on every
-    third launch only, which the measured traffic shows)."""
+    runs) and the one-launch iteration (cg_fused; the four-wide one moves
+    6.67 passes on average, carrying A r, rebuilding A d from the previous
+    residual and touching d and p on every third launch only, which the
+    measured traffic shows)."""
     cfg, s = make_solver(solver_kind)
     s.set_kernel_timing(True)
     s.run(cfg["dt"], [cfg["dt"] * i for i in range(steps)])
