@@ -217,8 +217,8 @@ def bench_rollout(n: int, steps: int, ckpt: int, ndirs: int = 0, reps: int = 2,
         # The loss is piecewise smooth (MacCormack clamp, floor, near-wall
         # switches): central differences converge to the derivative only when
         # eps is small enough not to straddle kinks.  Report the FP32 adjoint
-        # against (a) the FP64 adjoint, (b) FP64 central differences at eps =
-        # 1e-7 and (c) at the specified eps = 1e-3.
+        # against (a) the FP64 adjoint and (b) FP64 central differences at the
+        # specified eps = 1e-3 and at 1e-4 ... 1e-7.
         s64 = af.FluidSolver3D(n, n, n, precision="fp64")
         s64.gravity_alpha, s64.vorticity_eps = 0.05, 0.3
         s64.solver_kind, s64.solver_iters, s64.cg_rel_tol = 0, 3000, 1e-12
@@ -230,14 +230,17 @@ def bench_rollout(n: int, steps: int, ckpt: int, ndirs: int = 0, reps: int = 2,
         out["fp32_grad_rel_l2_vs_fp64"] = float(np.linalg.norm(gc - gc64) / np.linalg.norm(gc64))
         out["fp32_grad_beta_rel_vs_fp64"] = float(abs(gb - gb64) / abs(gb64))
         rng = np.random.default_rng(0xAE7E0003)
-        errs = {"fp64_adjoint": [], "fd_1e-7": [], "fd_1e-3": []}
-        fd64 = {"fd_1e-7": [], "fd_1e-3": []}
+        # the specified eps = 1e-3, then smaller steps down to 1e-7
+        fd_eps = (("fd_1e-3", 1e-3), ("fd_1e-4", 1e-4), ("fd_1e-5", 1e-5), ("fd_1e-6", 1e-6),
+                  ("fd_1e-7", 1e-7))
+        errs = {"fp64_adjoint": [], **{k: [] for k, _ in fd_eps}}
+        fd64 = {k: [] for k, _ in fd_eps}
         for _ in range(ndirs):
             dc, db = rng.uniform(-1, 1, len(pb["controls"])), rng.uniform(-1, 1)
             ad = float(gc @ dc + gb * db)
             ad64 = float(gc64 @ dc + gb64 * db)
             errs["fp64_adjoint"].append(abs(ad - ad64) / abs(ad64))
-            for key, eps in (("fd_1e-7", 1e-7), ("fd_1e-3", 1e-3)):
+            for key, eps in fd_eps:
                 lp, _, _ = s64.rollout(controls=pb["controls"] + eps * dc, beta=1.0 + eps * db,
                                        grad=False, **args)
                 lm, _, _ = s64.rollout(controls=pb["controls"] - eps * dc, beta=1.0 - eps * db,
