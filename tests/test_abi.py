@@ -6,6 +6,7 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -74,3 +75,13 @@ def test_no_device_here_reports_enodev():
     with pytest.raises(af.AetherFluidError) as e:
         af.FluidSolver3D(8, 8, 8)
     assert e.value.code == -4
+
+
+def test_tape_backward_before_forward_raises():
+    """The rollout tape (torch.autograd.Function shape) refuses backward()
+    before a forward(..., need_grad=True); no device call is made."""
+    import aether_fluid as af
+    tape = af.SmokeRollout(None, nsteps=2, dt=0.02, patch_lo=(1, 1, 1), patch_hi=(3, 2, 3),
+                           target=np.zeros(8))
+    with pytest.raises(RuntimeError):
+        tape.backward(1.0)

@@ -967,3 +967,33 @@ def test_adjoint_fp32_matches_fp64_fd_and_checkpoints():
         worst = max(worst, abs(ad - fd) / abs(fd))
         print("fp32 dir", fd, ad)
     assert worst <= 1e-3, worst
+
+
+def test_tape_api_and_torch_function():
+    """The tape API returns rollout()'s loss and gradient scaled by the
+    incoming adjoint, and the torch.autograd.Function form delivers the same
+    gradients through loss.backward()."""
+    import torch
+    import aether_fluid as af
+    from aether_fluid.adjoint import torch_function
+    _announce("tape api fp64 16^3")
+    n, steps = 16, 4
+    s, pb = _rollout_setup(n, "fp64")
+    kw = dict(nsteps=steps, dt=0.02, patch_lo=pb["patch_lo"], patch_hi=pb["patch_hi"],
+              target=pb["target"], checkpoint_every=2)
+    loss, gc, gb = s.rollout(controls=pb["controls"], beta=1.0, **kw)
+    tape = af.SmokeRollout(s, **kw)
+    l2 = tape.forward(pb["controls"], 1.0)
+    assert abs(l2 - loss) <= 1e-12 * abs(loss)
+    g2c, g2b = tape.backward(2.0)
+    np.testing.assert_allclose(g2c, 2.0 * gc, rtol=1e-10, atol=1e-14 * np.abs(gc).max())
+    assert abs(g2b - 2.0 * gb) <= 1e-10 * abs(gb)
+    fn = torch_function()
+    c = torch.tensor(pb["controls"], dtype=torch.float64, requires_grad=True)
+    b = torch.tensor(1.0, dtype=torch.float64, requires_grad=True)
+    out = fn.apply(c, b, tape)
+    (3.0 * out).backward()
+    assert abs(float(out.detach()) - loss) <= 1e-12 * abs(loss)
+    np.testing.assert_allclose(c.grad.numpy(), 3.0 * gc, rtol=1e-10,
+                               atol=1e-14 * np.abs(gc).max())
+    assert abs(float(b.grad) - 3.0 * gb) <= 1e-10 * abs(gb)
