@@ -17,38 +17,65 @@
 // forms from the r, r_(it-1) and w it loads there (y and z halos, x edges
 // at segment ends) and from its neighbour lanes (x): no LDS exchange, one
 // stencil.  r and w are read on halos by other workgroups: w ping-pongs
-// between two buffers and r rotates through three (launch `it` reads
-// R[it % 3], finds r_(it-1) in R[(it + 2) % 3] and writes R[(it + 1) % 3],
-// which held r_(it-2)); d and p are updated in place.
+// between two buffers and r rotates through a ring of K arrays (launch `it`
+// reads r_it from ring[it % K], r_(it-1) from ring[(it - 1) % K] and writes
+// ring[(it + 1) % K], which held r_(it+1-K)); d and p are updated in place.
 //
-// d and p are brought up to date every third launch (PH = it mod 3 = 2).
-// The other launches neither load nor store them; the third rebuilds the
-// two directions they skipped from the residuals those launches read, which
-// the rotation still holds at the thread's own cells (r_(it-2) in this
-// launch's output array, read before it is overwritten; r_(it-1)):
-//   d_(it-2) = r_(it-2) + beta_(it-2) d_(it-3)      (d_0 = r_0)
-//   d_(it-1) = r_(it-1) + beta_(it-1) d_(it-2)
-//   d_it     = r_it + beta_it d_(it-1)
-//   p <- ((p + alpha_(it-2) d_(it-2)) + alpha_(it-1) d_(it-1)) + alpha_it d_it
+// d and p are brought up to date once every K launches, by the launches
+// with it mod K = K - 1.  The other launches neither load nor store them;
+// the K-th rebuilds the K - 1 directions they skipped from the residuals
+// those launches read, which the ring still holds at the thread's own cells
+// (r_(it+1-K) in this launch's output array, read before it is
+// overwritten):
+//   d_j = r_j + beta_j d_(j-1)      j = it+1-K .. it   (d_0 = r_0)
+//   p <- (...((p + alpha_(it+1-K) d_(it+1-K)) + ...) + alpha_it d_it
 // the same FP32 operations in the same order as updating both every launch.
 // A launch that stops the solve applies the p terms left pending before it;
 // when the iteration cap ends it after another launch, k_cg_cgs4_tail does.
 // Array passes per iteration: r, r_(it-1), w read and r', w' written (5),
-// plus r_(it-2), d, p read and d, p written every third launch: 6.67 on
-// average (10 on every third launch, 5 on the others).
+// plus r_(it+1-K) .. r_(it-2), d, p read and d, p written by the K-th
+// launch: 5 + (K + 2) / K on average.
 #pragma once
+#include <utility>
+
 #include "cg_vec.hpp"
 
 namespace aether {
 
-template <int ZU, bool FIRST, int NQ, int ST, int PH>
+// The residual ring of the one-launch CG (entries past K unused), or, for
+// k_cg_cgs4_tail, the pending residuals in launch order.
+struct CgRing {
+    float *r[kCgRingMax];
+};
+
+// Applies at cell `oc` the p terms of the M launches before launch `it`
+// (it mod K = PH), in launch order: rebuilds d_(it-M) .. d_(it-1) from
+// dd = d_(it-M-1) (unused when it == M: d_0 = r_0) and the ring's residuals,
+// and returns d_(it-1) in dd and the updated p in pv.  fa/fb[m]: alpha and
+// beta of launch it - m.
+template <int K, int PH, int M>
+__device__ __forceinline__ void cgs4_pending(const CgRing &ring, int it, int oc,
+                                             const float (&fa)[K], const float (&fb)[K],
+                                             float4 &dd, float4 &pv) {
+#pragma unroll
+    for (int m = M; m >= 1; --m) {
+        const float4 rj = reinterpret_cast<const float4 *>(ring.r[(PH - m + K) % K])[oc];
+        dd = it == m ? rj : axpy4(rj, fb[m], dd);
+        pv = axpy4(pv, fa[m], dd);
+    }
+}
+
+template <int ZU, bool FIRST, int NQ, int ST, int K, int PH>
 __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
     Dims g, VTiling vt, int it, double rel_tol2, int nb_in, const double *__restrict__ pg_in,
     const double *__restrict__ pd_in, const double *__restrict__ pbb, double *__restrict__ pg_out,
-    double *__restrict__ pd_out, CgState *st, const float *__restrict__ r_in,
-    float *__restrict__ r_out, const float *__restrict__ r_prev,
-    const float *__restrict__ w_in, float *__restrict__ w_out, float *__restrict__ d,
-    float *__restrict__ p, int *done_mirror, int solve_id) {
+    double *__restrict__ pd_out, CgState *st, CgRing ring, const float *__restrict__ w_in,
+    float *__restrict__ w_out, float *__restrict__ d, float *__restrict__ p, int *done_mirror,
+    int solve_id) {
+    // K >= 3: the array a launch overwrites is neither its r_it nor its r_(it-1)
+    static_assert(K >= 3 && K <= kCgRingMax && PH >= 0 && PH < K && (!FIRST || PH == 0),
+                  "one-launch CG cycle");
+    constexpr int kMask = kCgRingMax - 1;
     int bx, by, bz;
     vtile_coords(vt, bx, by, bz);
     const VCol c = vcol_at<ZU>(g, vt, bx, by, bz);
@@ -60,21 +87,26 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
     eg.load(pg_in, nb_in);
     ed.load(pd_in, nb_in);
     if (FIRST) eb.load(pbb, nb_in);
-    double gprev = 1.0, aprev = 1.0, aprev2 = 0.0, bprev = 0.0, bprev2 = 0.0, b2 = 0.0;
+    double gprev = 1.0, aprev = 1.0, b2 = 0.0;
     if (!FIRST) {
         gprev = st->cg_gamma[(it - 1) & 1];
-        aprev = st->cg_alpha[(it - 1) & 3];
+        aprev = st->cg_alpha[(it - 1) & kMask];
         b2 = st->b2;
     }
-    if (PH >= 1) bprev = st->cg_beta[(it - 1) & 3];
-    if (PH == 2) {
-        aprev2 = st->cg_alpha[(it - 2) & 3];
-        bprev2 = st->cg_beta[(it - 2) & 3];
+    // alpha and beta of the launches whose p terms are pending: fa/fb[m] of
+    // launch it - m, m = 1 .. PH
+    float fa[K], fb[K];
+#pragma unroll
+    for (int m = 0; m < K; ++m) fa[m] = fb[m] = 0.0f;
+#pragma unroll
+    for (int m = 1; m <= PH; ++m) {
+        fa[m] = (float)st->cg_alpha[(it - m) & kMask];
+        fb[m] = (float)st->cg_beta[(it - m) & kMask];
     }
+    const float *__restrict__ r_in = ring.r[PH];
+    float *__restrict__ r_out = ring.r[(PH + 1) % K];
+    const float *__restrict__ r_prev = ring.r[(PH + K - 1) % K];
     const float4 *r4 = reinterpret_cast<const float4 *>(r_in);
-    // r_(it-2) (read at this thread's own cells before they are overwritten)
-    // and r_(it-1)
-    const float4 *ro2 = reinterpret_cast<const float4 *>(r_out);
     const float4 *ro1 = reinterpret_cast<const float4 *>(r_prev);
     const float4 *w4 = reinterpret_cast<const float4 *>(w_in);
     float4 *d4 = reinterpret_cast<float4 *>(d);
@@ -151,24 +183,15 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
             st->converged = 1;
             *done_mirror = solve_id;
         }
-        if (PH >= 1) {  // the terms of the launches since the last third one
-            const float tap = (float)aprev, tbp = (float)bprev, tap2 = (float)aprev2,
-                        tbp2 = (float)bprev2;
+        if (PH >= 1) {  // the terms of the launches since the last K-th one
 #pragma unroll
             for (int a = 0; a < ZU; ++a) {
                 const int k = c.k0 + a;
                 if (c.valid && k < g.nz) {
                     const int oc = (k * g.sxy + c.off) >> 2;
-                    const float4 dv = d4[oc], r1 = ro1[oc];
-                    if (PH == 1) {  // alpha_(it-1) d_(it-1), d_(it-2) stored
-                        const float4 d1 = it == 1 ? r1 : axpy4(r1, tbp, dv);
-                        o_p.put(oc, axpy4(p4[oc], tap, d1));
-                    } else {  // and alpha_(it-2) d_(it-2), d_(it-3) stored
-                        const float4 r2 = ro2[oc];
-                        const float4 d2 = it == 2 ? r2 : axpy4(r2, tbp2, dv);
-                        const float4 d1 = axpy4(r1, tbp, d2);
-                        o_p.put(oc, axpy4(axpy4(p4[oc], tap2, d2), tap, d1));
-                    }
+                    float4 dd = d4[oc], pv = p4[oc];
+                    cgs4_pending<K, PH, PH>(ring, it, oc, fa, fb, dd, pv);
+                    o_p.put(oc, pv);
                 }
             }
         }
@@ -177,11 +200,10 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
     const double alpha = gam / mu;
     if (lead) {
         st->cg_gamma[it & 1] = gam;
-        st->cg_alpha[it & 3] = alpha;
-        st->cg_beta[it & 3] = beta;
+        st->cg_alpha[it & kMask] = alpha;
+        st->cg_beta[it & kMask] = beta;
     }
-    const float tb = (float)beta, ta = (float)alpha, tap = (float)aprev, tbp = (float)bprev,
-                tap2 = (float)aprev2, tbp2 = (float)bprev2;
+    const float tb = (float)beta, ta = (float)alpha;
     // beta / alpha_(it-1): scales r_(it-1) - r into beta A d_(it-1)
     const float tc = FIRST ? 0.0f : (float)(beta / aprev);
 
@@ -207,28 +229,28 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
         }
     }
 
-    // ---- every third launch: d_(it-2), d_(it-1), d_it and p, finished and
-    // stored at once.  Loaded only now, after the r, w, s registers above
-    // are consumed, and held by nothing below, so that the launch keeps two
-    // waves per SIMD.
-    if (PH == 2) {
+    // ---- every K-th launch: d_(it+1-K) .. d_it and p, finished and stored
+    // at once.  The ring's residuals, d and p are loaded only now, after the
+    // halo registers above are consumed, and held by nothing below, so that
+    // the launch keeps two waves per SIMD.
+    if (PH == K - 1) {
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int a = 0; a < ZU; ++a) {
             const int k = c.k0 + a;
             const int oc = (min(k, g.nz - 1) * g.sxy + c.off) >> 2;
-            const float4 r2 = ro2[oc], r1 = ro1[oc], dv = d4[oc], pv = p4[oc];
-            const float4 d2 = it == 2 ? r2 : axpy4(r2, tbp2, dv);  // d_(it-2); d_0 = r_0
-            const float4 d1 = axpy4(r1, tbp, d2);                 // d_(it-1)
-            const float4 dn = axpy4(rz[a + 1], tb, d1);            // d_it
+            float4 dd = d4[oc], pv = p4[oc];
+            cgs4_pending<K, PH, K - 1>(ring, it, oc, fa, fb, dd, pv);
+            dd = axpy4(rz[a + 1], tb, dd);  // d_it
+            pv = axpy4(pv, ta, dd);
             if (c.valid && k < g.nz) {
-                o_d.put(oc, dn);
-                o_p.put(oc, axpy4(axpy4(axpy4(pv, tap2, d2), tap, d1), ta, dn));
+                o_d.put(oc, dd);
+                o_p.put(oc, pv);
             }
         }
     }
 
-    // ---- w' = A r', d, p and the partials of r'.r', r'.w' ----------------
+    // ---- w' = A r' and the partials of r'.r', r'.w' ----------------------
     const int srcL = (c.lane + 63) & 63, srcR = (c.lane + 1) & 63;
     double acc[2] = {0.0, 0.0};
 #pragma unroll
@@ -260,31 +282,90 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
 }
 
 // Closes a solve whose iteration cap ended it after launch `last_it` of
-// k_cg_cgs4 when that was not a third launch (last_it mod 3 = 0 or 1): it
-// rebuilds the directions since the last third launch and applies their
-// pending p terms, in the order the third launch would have.  r_lm1 and
-// r_l: the residuals launches last_it - 1 and last_it read.  Skipped when a
-// launch stopped the solve (that launch applied what was pending).
+// k_cg_cgs4 when that was not a K-th launch (last_it mod K < K - 1): it
+// rebuilds the directions since the last K-th launch and applies their
+// pending p terms, in the order the K-th launch would have.  pend.r[q]: the
+// residual that launch last_it - last_it mod K + q read, q = 0 .. last_it
+// mod K.  Skipped when a launch stopped the solve (that launch applied what
+// was pending).
+template <int K>
 __global__ __launch_bounds__(kThreads) void k_cg_cgs4_tail(const CgState *st, int last_it,
-                                                           const float4 *__restrict__ r_lm1,
-                                                           const float4 *__restrict__ r_l,
+                                                           CgRing pend,
                                                            const float4 *__restrict__ d,
                                                            float4 *__restrict__ p, long n4) {
+    constexpr int kMask = kCgRingMax - 1;
     if (st->done) return;
-    const int L = last_it;
-    const float ta = (float)st->cg_alpha[L & 3], tb = (float)st->cg_beta[L & 3];
-    const float ta1 = L >= 1 ? (float)st->cg_alpha[(L - 1) & 3] : 0.0f;
-    const float tb1 = L >= 1 ? (float)st->cg_beta[(L - 1) & 3] : 0.0f;
+    const int L = last_it, npend = L % K + 1, j0 = L - L % K;
+    float fa[K], fb[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+        fa[q] = q < npend ? (float)st->cg_alpha[(j0 + q) & kMask] : 0.0f;
+        fb[q] = q < npend ? (float)st->cg_beta[(j0 + q) & kMask] : 0.0f;
+    }
     for (long a = (long)blockIdx.x * kThreads + threadIdx.x; a < n4;
          a += (long)gridDim.x * kThreads) {
-        if (L % 3 == 0) {  // alpha_L d_L; d_(L-1) stored
-            const float4 dl = L == 0 ? r_l[a] : axpy4(r_l[a], tb, d[a]);
-            p[a] = axpy4(p[a], ta, dl);
-        } else {  // alpha_(L-1) d_(L-1) + alpha_L d_L; d_(L-2) stored
-            const float4 d1 = L == 1 ? r_lm1[a] : axpy4(r_lm1[a], tb1, d[a]);
-            const float4 dl = axpy4(r_l[a], tb, d1);
-            p[a] = axpy4(axpy4(p[a], ta1, d1), ta, dl);
+        float4 dd = d[a], pv = p[a];
+#pragma unroll
+        for (int q = 0; q < K - 1; ++q) {
+            if (q < npend) {
+                const float4 rj = reinterpret_cast<const float4 *>(pend.r[q])[a];
+                dd = j0 + q == 0 ? rj : axpy4(rj, fb[q], dd);
+                pv = axpy4(pv, fa[q], dd);
+            }
         }
+        p[a] = pv;
+    }
+}
+
+// ---- host-side selection of the instantiations --------------------------
+
+// Default launches per d/p update of the one-launch four-wide CG: 6.25
+// passes per iteration, and a 200-iteration cap needs no tail launch (3: 6.67
+// passes and a tail; the 128^3 step measured ~1% slower).
+constexpr int kCgsCycle = 8;
+
+// Cycle lengths the library is built with.
+inline bool cgs4_cycle_ok(int k) { return k == 3 || k == 4 || k == 8; }
+
+template <int K, int NQ, int ST, int... PH>
+inline const void *cgs4_phase(int ph, std::integer_sequence<int, PH...>) {
+    const void *fns[] = {(const void *)k_cg_cgs4<4, false, NQ, ST, K, PH>...};
+    return fns[ph];
+}
+
+template <int K, int NQ, int ST>
+inline const void *cgs4_pick(bool first, int ph) {
+    if (first) return (const void *)k_cg_cgs4<4, true, NQ, ST, K, 0>;
+    return cgs4_phase<K, NQ, ST>(ph, std::make_integer_sequence<int, K>{});
+}
+
+// wide: more than 1024 partial sums per launch (16 per thread); pol: store
+// policy (the wide form offers write-through and plain stores)
+template <int K>
+inline const void *cgs4_pick_policy(bool first, int ph, bool wide, int pol) {
+    if (wide)
+        return pol == kStWT ? cgs4_pick<K, 16, kStWT>(first, ph) : cgs4_pick<K, 16, kStPlain>(first, ph);
+    if (pol == kStNT) return cgs4_pick<K, 4, kStNT>(first, ph);
+    if (pol == kStWT) return cgs4_pick<K, 4, kStWT>(first, ph);
+    return cgs4_pick<K, 4, kStPlain>(first, ph);
+}
+
+// The k_cg_cgs4 instantiation for launch phase ph (= it mod cycle) of a
+// solve with `cycle` launches per d/p update (cgs4_cycle_ok(cycle)).
+inline const void *cgs4_kernel(int cycle, bool first, int ph, bool wide, int pol) {
+    switch (cycle) {
+    case 4: return cgs4_pick_policy<4>(first, ph, wide, pol);
+    case 8: return cgs4_pick_policy<8>(first, ph, wide, pol);
+    default: return cgs4_pick_policy<3>(first, ph, wide, pol);
+    }
+}
+
+inline void cgs4_tail_launch(int cycle, unsigned nb, hipStream_t st, const CgState *cg, int last,
+                             const CgRing &pend, const float4 *d, float4 *p, long n4) {
+    switch (cycle) {
+    case 4: k_cg_cgs4_tail<4><<<nb, kThreads, 0, st>>>(cg, last, pend, d, p, n4); break;
+    case 8: k_cg_cgs4_tail<8><<<nb, kThreads, 0, st>>>(cg, last, pend, d, p, n4); break;
+    default: k_cg_cgs4_tail<3><<<nb, kThreads, 0, st>>>(cg, last, pend, d, p, n4);
     }
 }
 

@@ -32,6 +32,11 @@ struct Dims {
 
 // Device state of one pressure solve (written only by workgroup 0 of a
 // kernel, read by later kernels on the same stream).
+// Longest update cycle of the one-launch four-wide CG: its residual ring
+// holds up to this many arrays, and CgState keeps alpha and beta of this
+// many iterations.
+constexpr int kCgRingMax = 8;
+
 struct CgState {
     double delta[2];  // r.r of the current iteration, double-buffered by iteration parity
     double b2;        // ||b||^2
@@ -42,9 +47,9 @@ struct CgState {
     int iters;
     int converged;
     int pad;
-    double cg_gamma[2];  // CG scalars by iteration: r.r (two slots, by parity),
-    double cg_alpha[4];  // alpha and beta (four slots, by it mod 4: the
-    double cg_beta[4];   // one-launch CG reads up to two iterations back)
+    double cg_gamma[2];           // CG scalars by iteration: r.r (two slots, by parity),
+    double cg_alpha[kCgRingMax];  // alpha and beta (by it mod kCgRingMax: the
+    double cg_beta[kCgRingMax];   // one-launch CG reads up to K - 1 iterations back)
 };
 
 __device__ __forceinline__ int tid_lin() { return threadIdx.y * kBX + threadIdx.x; }

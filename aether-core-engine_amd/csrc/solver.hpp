@@ -88,6 +88,12 @@ class Solver final : public SolverBase {
             const int f = std::atoi(e);
             if (f >= AETHER_CG_SCALAR && f <= AETHER_CG_FUSED) env_form_ = f;
         }
+        // AETHER_CG_CYCLE: launches per d/p update of the one-launch
+        // four-wide CG (cg_cgs4.hpp)
+        if (const char *e = std::getenv("AETHER_CG_CYCLE")) {
+            const int k = std::atoi(e);
+            if (cgs4_cycle_ok(k)) cycle_ = k;
+        }
         // four-wide two-launch CG: 256-thread groups of txv float4 lanes x ty
         // rows x vzu planes (vzu = 4 measured faster than 2 at 128^3 and 256^3)
         if (nx % 4 == 0) {
@@ -478,35 +484,18 @@ class Solver final : public SolverBase {
                 double rt = rel2;
                 const T *w_in = w_[it & 1].p;
                 T *w_out = w_[(it + 1) & 1].p, *dd = d_[0].p;
-                // r rotates through three arrays: launch `it` reads R[it % 3],
-                // writes R[(it + 1) % 3] (which holds r_(it-2)) and finds
-                // r_(it-1) in R[(it + 2) % 3]
-                T *R[3] = {r_.p, r2_.p, r3_.p};
-                const T *r4_in = R[it % 3], *r4_prev = R[(it + 2) % 3];
-                T *r4_out = R[(it + 1) % 3];
-                void *args[] = {&g,     &vt,   &itv,  &rt,    &nbv,  &pg_in, &pd_in,
-                                &pbb,   &pg_out, &pd_out, &st, &r4_in, &r4_out, &r4_prev,
-                                &w_in,  &w_out, &dd, &pp, &dm, &sid};
+                // r rotates through a ring of cycle_ arrays: launch `it` reads
+                // ring[it % K], writes ring[(it + 1) % K] (which holds
+                // r_(it+1-K)) and finds r_(it-1) in ring[(it - 1) % K]
+                CgRing ring{};
+                for (int q = 0; q < cycle_; ++q) ring.r[q] = ring_at(q);
+                void *args[] = {&g,      &vt,     &itv, &rt,   &nbv,  &pg_in, &pd_in, &pbb, &pg_out,
+                                &pd_out, &st,     &ring, &w_in, &w_out, &dd,   &pp,    &dm,  &sid};
                 const bool wide = std::max(nb_in, vt.nb) > 4 * kThreads;
-                const int ph = it % 3;  // d and p are updated on every third launch
                 // write-through stores (arrays under 2 GiB), unless
                 // AETHER_CG_NT asks for non-temporal ones
                 const int pol = nt_ ? kStNT : wt_ok() ? kStWT : kStPlain;
-#define CGS4_PICK(NQ_, ST_)                                                            \
-    (it == 0   ? (const void *)k_cg_cgs4<4, true, NQ_, ST_, 0>                         \
-     : ph == 1 ? (const void *)k_cg_cgs4<4, false, NQ_, ST_, 1>                        \
-     : ph == 2 ? (const void *)k_cg_cgs4<4, false, NQ_, ST_, 2>                        \
-               : (const void *)k_cg_cgs4<4, false, NQ_, ST_, 0>)
-                const void *fn;
-                if (wide)
-                    fn = pol == kStWT ? CGS4_PICK(16, kStWT) : CGS4_PICK(16, kStPlain);
-                else if (pol == kStNT)
-                    fn = CGS4_PICK(4, kStNT);
-                else if (pol == kStWT)
-                    fn = CGS4_PICK(4, kStWT);
-                else
-                    fn = CGS4_PICK(4, kStPlain);
-#undef CGS4_PICK
+                const void *fn = cgs4_kernel(cycle_, it == 0, it % cycle_, wide, pol);
                 HIPT(hipExtLaunchKernel(fn, dim3(vt.nb), dim3(kThreads), args, 0, st_, e0, e1, 0));
                 return;
             }
@@ -679,12 +668,22 @@ class Solver final : public SolverBase {
             partE_.alloc(npart);
             partF_.alloc(npart);
         }
-        if (sizeof(T) == 4 && tl4_.nb > 0 && !w_[0].p)
-            for (DevBuf<T> *b : {&w_[0], &w_[1], &r3_}) {
-                b->alloc(n_);
-                zero(b->p);
-            }
+        if (sizeof(T) == 4 && tl4_.nb > 0) {
+            if (!w_[0].p)
+                for (DevBuf<T> *b : {&w_[0], &w_[1]}) {
+                    b->alloc(n_);
+                    zero(b->p);
+                }
+            for (int q = 0; q + 2 < cycle_; ++q)  // the residual ring past r_ and r2_
+                if (!rring_[q].p) {
+                    rring_[q].alloc(n_);
+                    zero(rring_[q].p);
+                }
+        }
     }
+
+    // Residual array q of the one-launch four-wide CG's ring.
+    T *ring_at(int q) { return q == 0 ? r_.p : q == 1 ? r2_.p : rring_[q - 2].p; }
 
     // Four-wide two-launch CG iteration `it` (cg_vec.hpp); same buffers and
     // partial-sum roles as the one-cell-per-thread kernels.
@@ -846,22 +845,24 @@ class Solver final : public SolverBase {
         }
         if constexpr (sizeof(T) == 4) {
             // the four-wide two-launch form brings p up to date on odd
-            // iterations, the one-launch form on every third: a cap that ends
-            // the solve on another iteration leaves terms pending
+            // iterations, the one-launch form once every cycle_ launches: a
+            // cap that ends the solve on another iteration leaves terms pending
             const bool fused4 = form_ == AETHER_CG_FUSED && tl4_.nb > 0;
-            if (max_iters > 0 && ((fused4 && max_iters % 3 != 0) ||
+            if (max_iters > 0 && ((fused4 && max_iters % cycle_ != 0) ||
                                   (form_ == AETHER_CG_X4 && (max_iters & 1)))) {
                 const long n4 = (long)(n_ / 4);
                 const unsigned nbt =
                     (unsigned)std::min<long>((n4 + kThreads - 1) / kThreads, 4096);
                 const int last = max_iters - 1;
                 if (fused4) {  // the one-launch form also left the directions unbuilt
-                    const T *R[3] = {r_.p, r2_.p, r3_.p};  // residual launch `it` read: R[it % 3]
-                    k_cg_cgs4_tail<<<nbt, kThreads, 0, st_>>>(
-                        cg_.p, last, reinterpret_cast<const float4 *>(R[(last + 2) % 3]),
-                        reinterpret_cast<const float4 *>(R[last % 3]),
-                        reinterpret_cast<const float4 *>(d_[0].p),
-                        reinterpret_cast<float4 *>(p_.p), n4);
+                    // the residuals the pending launches read, in launch order
+                    // (launch `it` read ring[it % K])
+                    CgRing pend{};
+                    const int j0 = last - last % cycle_;
+                    for (int q = 0; q <= last % cycle_; ++q) pend.r[q] = ring_at((j0 + q) % cycle_);
+                    cgs4_tail_launch(cycle_, nbt, st_, cg_.p, last, pend,
+                                     reinterpret_cast<const float4 *>(d_[0].p),
+                                     reinterpret_cast<float4 *>(p_.p), n4);
                 } else {
                     k_cg_p_tail<<<nbt, kThreads, 0, st_>>>(
                         cg_.p, last, reinterpret_cast<const float4 *>(d_[last & 1].p),
@@ -1601,7 +1602,8 @@ class Solver final : public SolverBase {
     DevBuf<T> p2_;                 // second pressure buffer of the one-launch SOR sweep
     VTiling tls_ = {0, 0, 0, 0, 0, 1, 1};  // its tiling (2 planes per thread; nb 0: unusable)
     DevBuf<T> w_[2];               // A r of the four-wide one-launch CG (ping-pong)
-    DevBuf<T> r3_;                 // its third residual array
+    DevBuf<T> rring_[kCgRingMax - 2];  // its residual ring past r_ and r2_
+    int cycle_ = kCgsCycle;        // its launches per d/p update
     DevBuf<double> partE_, partF_;  // its delta partials
     bool ir64_ = false;           // mixed-precision projection (AETHER_FP32_IR64)
     // rollout / adjoint state

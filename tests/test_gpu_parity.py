@@ -585,6 +585,44 @@ def test_four_wide_cg_stop_with_two_rounds_of_tiles(form, tol):
     assert max(errs.values()) <= 1e-5, errs
 
 
+@pytest.mark.parametrize("cycle", [3, 4, 8])
+def test_one_launch_cg_cycle_positions_vs_oracle(cycle, monkeypatch):
+    """The one-launch four-wide CG brings d and p up to date once every
+    `cycle` launches.  Iteration caps 16..23 end the solve at every position
+    of the cycle (the tail kernel closes caps that are not multiples of it),
+    and tolerances taken from the oracle's residual after 16..23 iterations
+    stop it at every position (the stopping launch applies the pending
+    terms).  FP32 bar against the f64 oracle."""
+    import aether_fluid as af
+    from pairs import load_state
+    monkeypatch.setenv("AETHER_CG_CYCLE", str(cycle))
+    dims = (64, 48, 40)
+    state = random_state(*dims, seed=5)
+    gpu, orc = make_pair(*dims, precision="fp32", state=state)
+    gpu.cg_form = af.CG_FUSED
+    for n in range(16, 24):
+        for mode in ("cap", "tol"):
+            iters, tol = n, 0.0
+            if mode == "tol":
+                load_state(gpu, orc, state)
+                set_knobs(gpu, orc, iters=n, rel_tol=0.0)
+                orc.project()
+                so = orc.stats()
+                iters, tol = 200, float(np.sqrt(so["r2"] / so["b2"])) * (1 + 1e-6)
+            _announce(f"one-launch CG cycle {cycle} 64x48x40 {mode} {n}")
+            load_state(gpu, orc, state)
+            set_knobs(gpu, orc, iters=iters, rel_tol=tol)
+            gpu.project()
+            orc.project()
+            so, sg = orc.stats(), gpu.last_solve_stats()
+            if mode == "cap":
+                assert sg["iters"] == so["iters"] == n, (sg, so)
+            else:
+                assert abs(so["iters"] - n) <= 1 and abs(sg["iters"] - so["iters"]) <= 1, (n, sg, so)
+            errs = compare_fields(gpu, orc, ("u", "v", "w", "pressure"))
+            assert max(errs.values()) <= 1e-5, (mode, n, errs)
+
+
 def test_one_launch_cg_segment_edges_vs_oracle():
     """The one-launch four-wide CG where rows span two 64-lane segments
     (320 cells: 64 + 16 float4 lanes), so x-neighbours across every wave
