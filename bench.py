@@ -36,6 +36,14 @@ def algorithmic_bytes_per_step(nvox: int, cg_iters: int) -> float:
     return 4.0 * nvox * (59 + 8 * cg_iters)
 
 
+def algorithmic_bytes_per_rbgs_step(nvox: int, sweeps: int) -> float:
+    """FP32 array passes of one red-black SOR step with s sweeps: the stages
+    with the divergence writing b only (58), and per sweep two colour passes
+    that each read p and their half of b and write their half of p (4):
+    4 B * N * (58 + 4s)."""
+    return 4.0 * nvox * (58 + 4 * sweeps)
+
+
 def make_solver(solver_kind: str):
     import aether_fluid as af
     from aether_fluid.scenarios import SMOKE_CONFIGS, configure, seeded_config_state
@@ -396,7 +404,9 @@ def main():
                    "solver": "cg", "parallelism": "single GPU"},
         "cg_iters_last_step": stats["iters"],
         "rbgs": {"value": nvox * args.steps / el_rb, "ms_per_step": el_rb * 1e3 / args.steps,
-                 "sweeps": cfg["rbgs_sweeps"], "omega": cfg["sor_omega"]},
+                 "sweeps": cfg["rbgs_sweeps"], "omega": cfg["sor_omega"],
+                 "step_algorithmic_gbps": algorithmic_bytes_per_rbgs_step(
+                     nvox, cfg["rbgs_sweeps"]) * args.steps / el_rb / 1e9},
         "roofline": roofline,
         "fwd_adjoint": None,  # filled below
     }
