@@ -939,6 +939,18 @@ def test_batched_four_wide_equals_one_at_a_time_bitwise():
                        cap=7, tol=0.0)
 
 
+def test_batched_64_workload_equals_one_at_a_time_bitwise():
+    """The batched workload's own 64 seeded grids (sizes from 32^3, 48^3,
+    64^3, 96^3 and 128x96x128; 31.3 M voxels; seeded sources and forcing),
+    three steps with CG to relative tolerance 1e-5: the batch (four-wide CG)
+    equals the grids run one at a time with the same layout, bit for bit."""
+    import aether_fluid as af
+    insts = af.batched_instances(64)
+    assert sum(d["nx"] * d["ny"] * d["nz"] for d in insts) > 30_000_000
+    _batched_vs_single(insts, af.CG_X4, "batched vs one-at-a-time, the 64-grid workload",
+                       cap=200, tol=1e-5)
+
+
 def _rollout_setup(n, precision, tol=1e-12, iters=600):
     import aether_fluid as af
     from aether_fluid.scenarios import rollout_problem
