@@ -733,6 +733,42 @@ def test_mixed_precision_matches_fp64_projection_steps():
         assert rel_l2(a.download(n), b.download(n)) <= 1e-5, n
 
 
+def test_mixed_precision_256_real_size_step():
+    """The mixed-precision workload at its real size (256^3, dt 0.005, the
+    seeded divergent state, sphere source r = 24): one step refines the FP32
+    projection to max|div u| <= 1e-7 in the FP64 residual, and the velocity
+    and density match the all-FP64 GPU step (FP64 CG to relative tolerance
+    1e-14) within 1e-4 relative L2."""
+    _announce("mixed vs fp64, one 256^3 step")
+    import aether_fluid as af
+    from aether_fluid.scenarios import SMOKE_CONFIGS, seeded_config_state
+    cfg = SMOKE_CONFIGS["mixed_256"]
+    assert tuple(cfg["dims"]) == (256, 256, 256)
+    state = seeded_config_state("mixed_256")
+    out = {}
+    for prec in ("fp32_ir64", "fp64"):
+        s = af.FluidSolver3D(*cfg["dims"], h=cfg["h"], precision=prec)
+        for n, a in state.items():
+            s.upload(n, a)
+        s.gravity_alpha, s.buoyancy_beta, s.vorticity_eps = cfg["alpha"], cfg["beta"], cfg["eps"]
+        s.set_sphere_source(cfg["source"]["center"], cfg["source"]["radius"])
+        s.solver_kind, s.solver_iters = 0, 4000
+        if prec == "fp64":
+            s.cg_rel_tol = 1e-14
+        else:
+            s.solver_iters = 1000
+            s.ir_inner_rel_tol, s.ir_div_tol, s.ir_max_outer = 1e-3, 1e-7, 30
+        s.run(cfg["dt"], [0.0])
+        out[prec] = (s.last_solve_stats(), {n: s.download(n) for n in ("u", "v", "w", "density")})
+        s.close()
+    st32, f32 = out["fp32_ir64"]
+    print("ir stats", st32)
+    assert st32["converged"] and st32["div_max"] <= 1e-7
+    errs = {n: rel_l2(f32[n], out["fp64"][1][n]) for n in f32}
+    print("mixed vs fp64", errs)
+    assert max(errs.values()) <= 1e-4, errs
+
+
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_fluid_medium_vs_oracle(precision):
     """FluidMedium::from_solver and sample_density_and_temp (fluid.rs:703-788)
