@@ -413,6 +413,43 @@ def test_fp32_plain_128_one_step_vs_oracle(solver):
         assert abs(gpu.last_solve_stats()["iters"] - orc.stats()["iters"]) <= 1
 
 
+@pytest.mark.parametrize("solver", ["rbgs", "cg"])
+def test_fp64_plain_128_one_step_vs_oracle(solver):
+    """The headline workload's step at its real size in the FP64
+    instantiation, against one f64 oracle step (OpenMP build, whose RBGS
+    equals the sequential sweep bit for bit) from the same seeded state.
+    RB-SOR (60 sweeps): every field bit-identical.  CG (cap 200, reached):
+    the same iteration count, fields within 1e-12 relative L2 (fixed-order
+    device dot products against the oracle's)."""
+    _announce(f"fp64 plain 128^3 one step vs oracle ({solver})")
+    import aether_fluid as af
+    from aether_fluid.scenarios import SMOKE_CONFIGS, configure, seeded_config_state
+    from oracle.oracle import OracleFluid
+    cfg = SMOKE_CONFIGS["plain_128"]
+    state = seeded_config_state("plain_128")
+    gpu = af.FluidSolver3D(*cfg["dims"], h=cfg["h"], precision="fp64")
+    configure(gpu, "plain_128", solver)
+    orc = OracleFluid(*cfg["dims"], h=cfg["h"], omp=True)
+    orc.vorticity_eps, orc.buoyancy_beta, orc.gravity_alpha = cfg["eps"], cfg["beta"], cfg["alpha"]
+    if solver == "cg":
+        orc.solver_kind, orc.solver_iters, orc.cg_rel_tol = 0, cfg["cg_max"], cfg["cg_tol"]
+    else:
+        orc.solver_kind, orc.solver_iters, orc.sor_omega = 1, cfg["rbgs_sweeps"], cfg["sor_omega"]
+    orc.set_source(1, cfg["source"]["center"], cfg["source"]["radius"], 1.0, 1.0)
+    load_state(gpu, orc, state)
+    gpu.step(cfg["dt"], 0.0)
+    orc.step(cfg["dt"], 0.0)
+    names = ("u", "v", "w", "density", "temperature", "pressure")
+    if solver == "rbgs":
+        for n in names:
+            np.testing.assert_array_equal(gpu.download(n), orc.field(n).ravel(), err_msg=n)
+    else:
+        assert gpu.last_solve_stats()["iters"] == orc.stats()["iters"]
+        errs = compare_fields(gpu, orc, names)
+        print("rel L2", errs, flush=True)
+        assert max(errs.values()) <= 1e-12, errs
+
+
 def test_fp32_cfg0_few_steps():
     """32x48x32, h = 1/32, zero start, sphere source: 5 FP32 steps."""
     _announce("fp32 32x48x32 5 steps")
