@@ -1087,7 +1087,8 @@ __global__ __launch_bounds__(kThreads) void k_cg_finalize(int max_iters, double 
                                                           const double *__restrict__ part_rr,
                                                           const double *__restrict__ part_bb,
                                                           int nb, CgState *st,
-                                                          int *__restrict__ iters_log) {
+                                                          int *__restrict__ iters_log,
+                                                          int *iters_mirror) {
     __shared__ double lds[8];
     __shared__ int flag_lds[1];
     if (!block_flag(&st->done, flag_lds)) {
@@ -1108,6 +1109,8 @@ __global__ __launch_bounds__(kThreads) void k_cg_finalize(int max_iters, double 
     }
     __syncthreads();
     if (iters_log && tid_lin() == 0) *iters_log = st->iters;
+    // the host sizes the next solve's launch look-ahead from this count
+    if (iters_mirror && tid_lin() == 0) *iters_mirror = st->iters;
 }
 
 // Red-black SOR sweep of one colour (src/fluid.rs:400-437).  Same-colour

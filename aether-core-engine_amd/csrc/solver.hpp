@@ -158,7 +158,8 @@ class Solver final : public SolverBase {
         if (const char *e = std::getenv("AETHER_CG_CHUNK")) cg_chunk_ = std::max(0, std::atoi(e));
         if (const char *e = std::getenv("AETHER_ADJ_WINDOW")) adj_window_ = std::atoi(e) != 0;
         HIPT(hipHostMalloc((void **)&done_host_, 64, hipHostMallocMapped));
-        *done_host_ = 0;
+        done_host_[0] = 0;  // id of the last solve a CG launch stopped
+        done_host_[1] = 0;  // iterations of the last finished solve
         HIPT(hipHostGetDevicePointer((void **)&done_dev_, done_host_, 0));
         for (int a = 0; a < 5; ++a) {
             f_[a].alloc(n_);
@@ -804,17 +805,29 @@ class Solver final : public SolverBase {
         LAUNCH_CHECK();
     }
 
-    // CG iterations (src/fluid.rs:463-487), two launches each.  Every scalar
-    // stays on the device; the host only watches a mapped "solve finished"
-    // word, two chunks behind, to stop queueing launches once the device has
-    // stopped (launches queued after the stop return without touching memory
-    // other than their loads).
+    // CG iterations (src/fluid.rs:463-487).  Every scalar stays on the
+    // device; the host only watches a mapped "solve finished" word, two
+    // chunks behind, to stop queueing launches once the device has stopped
+    // (launches queued after the stop return without touching memory other
+    // than their loads).  The first chunk is as long as the last finished
+    // solve, less two chunks (all of max_iters when that solve ran to the
+    // cap), so a run of similar solves waits on few events.
     void enqueue_cg_loop(int max_iters, int step_index) {
         const double rel2 = rel_tol2();
         const int solve_id = ++solve_seq_;
         int chunk = cg_chunk_;
         if (chunk <= 0) chunk = max_iters > 0 ? max_iters : 1;
-        const int nchunks = (max_iters + chunk - 1) / chunk;
+        int first = chunk;
+        if (cg_chunk_ > 0) {
+            const int prev = __atomic_load_n(done_host_ + 1, __ATOMIC_ACQUIRE);
+            if (prev >= max_iters)
+                first = std::max(chunk, max_iters);
+            else
+                first = std::max(chunk, prev - 2 * chunk);
+        }
+        // chunk c: launches [c_begin(c), c_begin(c + 1)) (capped at max_iters)
+        auto c_begin = [&](int c) { return c == 0 ? 0 : first + (c - 1) * chunk; };
+        const int nchunks = max_iters > first ? 1 + (max_iters - first + chunk - 1) / chunk : 1;
         if ((int)chunk_events_.size() < nchunks) {
             for (int a = (int)chunk_events_.size(); a < nchunks; ++a) {
                 hipEvent_t e;
@@ -838,8 +851,8 @@ class Solver final : public SolverBase {
                 HIPT(hipEventSynchronize(chunk_events_[c - 2]));
                 if (__atomic_load_n(done_host_, __ATOMIC_ACQUIRE) == solve_id) break;
             }
-            const int it_end = std::min(max_iters, (c + 1) * chunk);
-            for (int it = c * chunk; it < it_end; ++it)
+            const int it_end = std::min(max_iters, c_begin(c + 1));
+            for (int it = c_begin(c); it < it_end; ++it)
                 enqueue_cg_iteration(it, rel2, solve_id, step_index);
             HIPT(hipEventRecord(chunk_events_[c], st_));
         }
@@ -881,7 +894,8 @@ class Solver final : public SolverBase {
         }
         k_cg_finalize<<<1, cell_blk_, 0, st_>>>(max_iters, rel2, part_rr, partC_.p, nb_rr, cg_.p,
                                                  step_index >= 0 ? iters_dev_.p + step_index
-                                                                 : iters_log_.p);
+                                                                 : iters_log_.p,
+                                                 done_dev_ + 1);
         LAUNCH_CHECK();
         last_kind_ = AETHER_SOLVER_CG;
     }
@@ -1621,7 +1635,9 @@ class Solver final : public SolverBase {
     std::vector<int> ir_log_;
     int cg_chunk_ = 8;         // CG iterations per host look-ahead chunk (0: no look-ahead)
     int solve_seq_ = 0;        // id of the current pressure solve
-    int *done_host_ = nullptr; // mapped pinned word the device sets to the finished solve id
+    int *done_host_ = nullptr; // mapped pinned words: [0] the device sets to the id of a
+                               // solve a CG launch stopped, [1] to the iterations of the
+                               // last finished solve
     int *done_dev_ = nullptr;
     std::vector<hipEvent_t> chunk_events_;
     // resident (single cooperative launch) CG: tiling, eligibility
