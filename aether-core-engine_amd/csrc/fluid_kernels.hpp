@@ -1109,7 +1109,7 @@ __global__ __launch_bounds__(kThreads) void k_cg_finalize(int max_iters, double 
     }
     __syncthreads();
     if (iters_log && tid_lin() == 0) *iters_log = st->iters;
-    // the host sizes the next solve's launch look-ahead from this count
+    // the host queues the next solve at once when this one ran to the cap
     if (iters_mirror && tid_lin() == 0) *iters_mirror = st->iters;
 }
 

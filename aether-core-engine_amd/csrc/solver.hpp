@@ -809,22 +809,18 @@ class Solver final : public SolverBase {
     // device; the host only watches a mapped "solve finished" word, two
     // chunks behind, to stop queueing launches once the device has stopped
     // (launches queued after the stop return without touching memory other
-    // than their loads).  The first chunk is as long as the last finished
-    // solve, less two chunks (all of max_iters when that solve ran to the
-    // cap), so a run of similar solves waits on few events.
+    // than their loads).  When the last finished solve ran to the cap, the
+    // whole solve is queued at once: runs of capped solves (the 128^3 smoke
+    // step) then wait on no event.  Solves that stop early keep the paced
+    // chunks, since a long first chunk would queue early-exit launches.
     void enqueue_cg_loop(int max_iters, int step_index) {
         const double rel2 = rel_tol2();
         const int solve_id = ++solve_seq_;
         int chunk = cg_chunk_;
         if (chunk <= 0) chunk = max_iters > 0 ? max_iters : 1;
         int first = chunk;
-        if (cg_chunk_ > 0) {
-            const int prev = __atomic_load_n(done_host_ + 1, __ATOMIC_ACQUIRE);
-            if (prev >= max_iters)
-                first = std::max(chunk, max_iters);
-            else
-                first = std::max(chunk, prev - 2 * chunk);
-        }
+        if (cg_chunk_ > 0 && __atomic_load_n(done_host_ + 1, __ATOMIC_ACQUIRE) >= max_iters)
+            first = std::max(chunk, max_iters);
         // chunk c: launches [c_begin(c), c_begin(c + 1)) (capped at max_iters)
         auto c_begin = [&](int c) { return c == 0 ? 0 : first + (c - 1) * chunk; };
         const int nchunks = max_iters > first ? 1 + (max_iters - first + chunk - 1) / chunk : 1;
