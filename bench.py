@@ -342,8 +342,8 @@ def main():
                     help="plain_128: the headline line; mixed_256: mixed-precision "
                          "projection at 256^3 against the all-FP64 projection")
     ap.add_argument("--probe", action="store_true",
-                    help="counter-collection mode: 2 steps of the workload and a 256 MiB "
-                         "stream copy, no timing output")
+                    help="counter-collection mode: 2 steps of the workload with each CG "
+                         "layout and with RB-SOR, and a 256 MiB stream copy; no timing output")
     args = ap.parse_args()
     if args.steps is None:
         args.steps = {"plain_128": 100, "mixed_256": 20, "batched_64": 50}.get(args.workload, 20)
@@ -362,12 +362,14 @@ def main():
     if args.workload == "mixed_256":
         print(json.dumps(bench_mixed(args.steps, args.warmup)), flush=True)
         return
-    if args.probe:  # short run for counter collection: both CG layouts, then the copy
+    if args.probe:  # short run for counter collection: both CG layouts, RB-SOR, the copy
         _, s = make_solver("cg")
         s.run(SMOKE_DT, [0.0, SMOKE_DT])
         _, s2 = make_solver("cg")
         s2.cg_form = af.CG_X4
         s2.run(SMOKE_DT, [0.0, SMOKE_DT])
+        _, s3 = make_solver("rbgs")
+        s3.run(SMOKE_DT, [0.0, SMOKE_DT])
         af.stream_copy_ms(COPY_BYTES, 5)
         print("probe done", flush=True)
         return
