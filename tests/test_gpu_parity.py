@@ -697,6 +697,42 @@ def test_one_launch_cg_cycle_positions_vs_oracle(cycle, monkeypatch):
             assert max(errs.values()) <= 1e-5, (mode, n, errs)
 
 
+@pytest.mark.parametrize("form", ["fused", "x4"])
+def test_cg_queued_whole_after_capped_solve_equals_paced(form):
+    """After a solve that ran to its cap the host queues the next solve's
+    launches all at once instead of in paced chunks.  When that solve stops
+    early (tolerance reached after ~11 iterations of a cap of 60), the
+    launches queued after the stop must leave the result unchanged: fields
+    and iteration count equal, bit for bit, those of a fresh handle whose
+    solve runs paced."""
+    _announce(f"CG queued whole after a capped solve ({form})")
+    import aether_fluid as af
+    dims = (64, 48, 40)
+    state = random_state(*dims, seed=5)
+    runs = []
+    for history in (True, False):
+        s = af.FluidSolver3D(*dims, precision="fp32")
+        s.cg_form = af.CG_FUSED if form == "fused" else af.CG_X4
+        s.solver_kind = 0
+        if history:  # a capped solve first
+            for n, a in state.items():
+                s.upload(n, a)
+            s.solver_iters, s.cg_rel_tol = 60, 0.0
+            s.project()
+            assert s.last_solve_stats()["iters"] == 60
+        for n, a in state.items():
+            s.upload(n, a)
+        s.solver_iters, s.cg_rel_tol = 60, 3e-2
+        s.project()
+        runs.append((s.last_solve_stats()["iters"],
+                     {n: s.download(n) for n in ("u", "v", "w", "pressure")}))
+        s.close()
+    (ia, fa), (ib, fb) = runs
+    assert ia == ib and 0 < ia < 60, (ia, ib)
+    for n in fa:
+        np.testing.assert_array_equal(fa[n], fb[n], err_msg=n)
+
+
 def test_one_launch_cg_segment_edges_vs_oracle():
     """The one-launch four-wide CG where rows span two 64-lane segments
     (320 cells: 64 + 16 float4 lanes), so x-neighbours across every wave
