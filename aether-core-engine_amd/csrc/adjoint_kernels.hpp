@@ -21,6 +21,7 @@
 #include "fluid_kernels.hpp"
 
 namespace aether {
+namespace {
 
 template <typename T>
 __device__ __forceinline__ void atomic_acc(T *p, T v) {
@@ -530,4 +531,5 @@ __global__ __launch_bounds__(kThreads) void k_copy(const T *__restrict__ a, long
         b[q] = a[q];
 }
 
+}  // namespace
 }  // namespace aether

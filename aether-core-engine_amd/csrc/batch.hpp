@@ -1,6 +1,5 @@
 // batch.hpp -- many smoke grids advanced together, one launch per stage.
-// Part of the library's single translation unit: included once, by
-// fluid_api.hip, after the headers before it there.
+// Included by batch.hip only.
 #pragma once
 #include "host_common.hpp"
 #include "cg_vec.hpp"
@@ -15,17 +14,6 @@ namespace {
 // the dissipation; each has its own grid, h, forcing and sphere source.
 // ---------------------------------------------------------------------------
 constexpr int kBatchZU = 8;  // z cells per thread of the batched CG tiling
-
-class BatchBase {
-   public:
-    virtual ~BatchBase() = default;
-    virtual void autotune(int iters, double *us, int *form) = 0;
-    virtual void upload(int inst, int field, const double *src, size_t n) = 0;
-    virtual void download(int inst, int field, double *dst, size_t n) = 0;
-    virtual void run(double dt, const double *times, int nsteps, aether_fluid_telemetry *out) = 0;
-    virtual const std::vector<int> &iters() const = 0;
-    virtual int count() const = 0;
-};
 
 template <typename T>
 class Batch final : public BatchBase {

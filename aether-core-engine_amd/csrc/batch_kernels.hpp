@@ -12,6 +12,7 @@
 #include "cg_vec.hpp"
 
 namespace aether {
+namespace {
 
 struct BItem {
     int inst;       // instance index
@@ -301,4 +302,5 @@ __global__ void kb_step_advance(int *step_dev) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *step_dev += 1;
 }
 
+}  // namespace
 }  // namespace aether

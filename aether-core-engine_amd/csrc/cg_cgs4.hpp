@@ -41,6 +41,7 @@
 #include "cg_vec.hpp"
 
 namespace aether {
+namespace {
 
 // The residual ring of the one-launch CG (entries past K unused), or, for
 // k_cg_cgs4_tail, the pending residuals in launch order.
@@ -319,54 +320,35 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4_tail(const CgState *st, in
 
 // ---- host-side selection of the instantiations --------------------------
 
-// Default launches per d/p update of the one-launch four-wide CG: 6.25
-// passes per iteration, and a 200-iteration cap needs no tail launch (3: 6.67
-// passes and a tail; the 128^3 step measured ~1% slower).
+// Launches per d/p update of the one-launch four-wide CG: 6.25 passes per
+// iteration, and a 200-iteration cap needs no tail launch (a cycle of 3: 6.67
+// passes and a tail; the 128^3 step measured ~1% slower with 3 or 4).
 constexpr int kCgsCycle = 8;
 
-// Cycle lengths the library is built with.
-inline bool cgs4_cycle_ok(int k) { return k == 3 || k == 4 || k == 8; }
-
-template <int K, int NQ, int ST, int... PH>
+template <int NQ, int ST, int... PH>
 inline const void *cgs4_phase(int ph, std::integer_sequence<int, PH...>) {
-    const void *fns[] = {(const void *)k_cg_cgs4<4, false, NQ, ST, K, PH>...};
+    const void *fns[] = {(const void *)k_cg_cgs4<4, false, NQ, ST, kCgsCycle, PH>...};
     return fns[ph];
 }
 
-template <int K, int NQ, int ST>
+template <int NQ, int ST>
 inline const void *cgs4_pick(bool first, int ph) {
-    if (first) return (const void *)k_cg_cgs4<4, true, NQ, ST, K, 0>;
-    return cgs4_phase<K, NQ, ST>(ph, std::make_integer_sequence<int, K>{});
+    if (first) return (const void *)k_cg_cgs4<4, true, NQ, ST, kCgsCycle, 0>;
+    return cgs4_phase<NQ, ST>(ph, std::make_integer_sequence<int, kCgsCycle>{});
 }
 
-// wide: more than 1024 partial sums per launch (16 per thread); pol: store
-// policy (the wide form offers write-through and plain stores)
-template <int K>
-inline const void *cgs4_pick_policy(bool first, int ph, bool wide, int pol) {
-    if (wide)
-        return pol == kStWT ? cgs4_pick<K, 16, kStWT>(first, ph) : cgs4_pick<K, 16, kStPlain>(first, ph);
-    if (pol == kStNT) return cgs4_pick<K, 4, kStNT>(first, ph);
-    if (pol == kStWT) return cgs4_pick<K, 4, kStWT>(first, ph);
-    return cgs4_pick<K, 4, kStPlain>(first, ph);
+// The k_cg_cgs4 instantiation for launch phase ph (= it mod kCgsCycle).
+// wide: more than 1024 partial sums per launch (16 per thread); wt:
+// write-through stores (arrays under 2 GiB), else plain stores.
+inline const void *cgs4_kernel(bool first, int ph, bool wide, bool wt) {
+    if (wide) return wt ? cgs4_pick<16, kStWT>(first, ph) : cgs4_pick<16, kStPlain>(first, ph);
+    return wt ? cgs4_pick<4, kStWT>(first, ph) : cgs4_pick<4, kStPlain>(first, ph);
 }
 
-// The k_cg_cgs4 instantiation for launch phase ph (= it mod cycle) of a
-// solve with `cycle` launches per d/p update (cgs4_cycle_ok(cycle)).
-inline const void *cgs4_kernel(int cycle, bool first, int ph, bool wide, int pol) {
-    switch (cycle) {
-    case 4: return cgs4_pick_policy<4>(first, ph, wide, pol);
-    case 8: return cgs4_pick_policy<8>(first, ph, wide, pol);
-    default: return cgs4_pick_policy<3>(first, ph, wide, pol);
-    }
-}
-
-inline void cgs4_tail_launch(int cycle, unsigned nb, hipStream_t st, const CgState *cg, int last,
+inline void cgs4_tail_launch(unsigned nb, hipStream_t st, const CgState *cg, int last,
                              const CgRing &pend, const float4 *d, float4 *p, long n4) {
-    switch (cycle) {
-    case 4: k_cg_cgs4_tail<4><<<nb, kThreads, 0, st>>>(cg, last, pend, d, p, n4); break;
-    case 8: k_cg_cgs4_tail<8><<<nb, kThreads, 0, st>>>(cg, last, pend, d, p, n4); break;
-    default: k_cg_cgs4_tail<3><<<nb, kThreads, 0, st>>>(cg, last, pend, d, p, n4);
-    }
+    k_cg_cgs4_tail<kCgsCycle><<<nb, kThreads, 0, st>>>(cg, last, pend, d, p, n4);
 }
 
+}  // namespace
 }  // namespace aether

@@ -30,6 +30,7 @@
 #include "fluid_kernels.hpp"
 
 namespace aether {
+namespace {
 
 constexpr int kFX = 64, kFY = 4;                                  // output tile
 constexpr int kR2X = kFX + 4, kR2Y = kFY + 4, kR2 = kR2X * kR2Y;  // 68 x 8 = 544
@@ -37,6 +38,8 @@ constexpr int kR1X = kFX + 2, kR1Y = kFY + 2, kR1 = kR1X * kR1Y;  // 66 x 6 = 39
 constexpr int kR2Per = (kR2 + kThreads - 1) / kThreads;           // 3
 constexpr int kR1Per = (kR1 + kThreads - 1) / kThreads;           // 2
 static_assert(kFX == kBX && kFY == kBY && kFX * kFY == kThreads, "tile = workgroup shape");
+constexpr int kFusedKZ = 16;  // z planes per workgroup
+constexpr int kFusedPF = 4;   // planes of loads in flight ahead of their use
 
 // delta_0 = r_0 . A r_0 partials over the initialisation tiling (the other
 // half of the first iteration's reduction; gamma_0 comes with r_0).  When
@@ -263,4 +266,5 @@ __global__ __launch_bounds__(kThreads) void k_cg_fused(
     }
 }
 
+}  // namespace
 }  // namespace aether

@@ -14,6 +14,7 @@
 #include "fluid_kernels.hpp"
 
 namespace aether {
+namespace {
 
 // Workgroups of 256 threads cover txv float4 lanes of ty rows (txv = nx/4
 // capped at 64, ty = 256 / txv); threads past txv * ty idle.
@@ -570,4 +571,5 @@ __global__ __launch_bounds__(kThreads) void k_rbsor4(Dims g, VTiling vt, float o
     }
 }
 
+}  // namespace
 }  // namespace aether

@@ -21,9 +21,7 @@
 
 #include "aether_fluid.h"
 #include "host_common.hpp"
-#include "medium.hpp"
-#include "solver.hpp"
-#include "batch.hpp"
+#include "fluid_kernels.hpp"
 
 
 using namespace aether;
@@ -118,10 +116,9 @@ int aether_fluid_create(int nx, int ny, int nz, double h, int precision, aether_
         HIPT(hipGetDevice(&s->device));
         HIPT(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
         if (precision == AETHER_FP64)
-            s->impl = std::make_unique<Solver<double>>(nx, ny, nz, s->h, s->stream);
+            s->impl = make_solver_f64(nx, ny, nz, s->h, s->stream);
         else
-            s->impl = std::make_unique<Solver<float>>(nx, ny, nz, s->h, s->stream,
-                                                      precision == AETHER_FP32_IR64);
+            s->impl = make_solver_f32(nx, ny, nz, s->h, s->stream, precision == AETHER_FP32_IR64);
         *out = s.release();
     });
 }
@@ -392,10 +389,7 @@ int aether_fluid_batch_create(const aether_fluid_instance_desc *desc, int count,
         auto b = std::make_unique<aether_fluid_batch_s>();
         HIPT(hipGetDevice(&b->device));
         HIPT(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
-        if (precision == AETHER_FP64)
-            b->impl = std::make_unique<Batch<double>>(desc, count, *shared, b->stream);
-        else
-            b->impl = std::make_unique<Batch<float>>(desc, count, *shared, b->stream);
+        b->impl = make_batch(desc, count, precision, *shared, b->stream);
         *out = b.release();
     });
 }
@@ -539,9 +533,8 @@ int aether_fluid_set_kernel_timing(aether_fluid_t s, int enable) {
 int aether_fluid_kernel_timing(aether_fluid_t s, int which, double *avg_ms, long *launches) {
     return guarded([&] {
         checked(s);
-        if (which < 0 || which > 3)
-            raise(AETHER_EINVAL,
-                  "which must be 0 (cg_dir), 1 (cg_update), 2 (cg_resident) or 3 (cg_fused)");
+        if (which < 0 || which > 3 || which == 2)
+            raise(AETHER_EINVAL, "which must be 0 (cg_dir), 1 (cg_update) or 3 (cg_fused)");
         const auto &t = s->impl->timer;
         if (avg_ms) *avg_ms = t.count[which] ? t.sum_ms[which] / t.count[which] : 0.0;
         if (launches) *launches = t.count[which];

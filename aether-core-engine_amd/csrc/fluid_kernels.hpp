@@ -20,6 +20,7 @@
 #include <stdint.h>
 
 namespace aether {
+namespace {  // kernels and device helpers: private to each translation unit
 
 constexpr int kBX = 64;  // one wavefront spans 64 consecutive x cells
 constexpr int kBY = 4;   // four wavefronts per workgroup
@@ -1088,7 +1089,8 @@ __global__ __launch_bounds__(kThreads) void k_cg_finalize(int max_iters, double 
                                                           const double *__restrict__ part_bb,
                                                           int nb, CgState *st,
                                                           int *__restrict__ iters_log,
-                                                          int *iters_mirror) {
+                                                          unsigned long long *iters_mirror,
+                                                          int solve_id) {
     __shared__ double lds[8];
     __shared__ int flag_lds[1];
     if (!block_flag(&st->done, flag_lds)) {
@@ -1109,8 +1111,10 @@ __global__ __launch_bounds__(kThreads) void k_cg_finalize(int max_iters, double 
     }
     __syncthreads();
     if (iters_log && tid_lin() == 0) *iters_log = st->iters;
-    // the host queues the next solve at once when this one ran to the cap
-    if (iters_mirror && tid_lin() == 0) *iters_mirror = st->iters;
+    // the host queues the next solve at once when a recent one ran to the
+    // cap; id and count go out in one 64-bit store, so they never tear
+    if (iters_mirror && tid_lin() == 0)
+        *iters_mirror = ((unsigned long long)(unsigned)solve_id << 32) | (unsigned)st->iters;
 }
 
 // Red-black SOR sweep of one colour (src/fluid.rs:400-437).  Same-colour
@@ -1580,4 +1584,5 @@ __global__ __launch_bounds__(kThreads) void k_medium_sample(Dims g, Box3 box,
     out[2 * a + 1] = tv * feather;
 }
 
+}  // namespace
 }  // namespace aether

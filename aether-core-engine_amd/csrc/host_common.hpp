@@ -1,7 +1,8 @@
 // host_common.hpp -- error handling, device buffers, launch-dispatch macros, parameter
 // defaults and checks, and the kernel timer shared by the solver and the batch.
-// Part of the library's single translation unit: included once, by
-// fluid_api.hip, after the headers before it there.
+// Shared by every translation unit of the library: everything here has
+// external linkage (inline), so an error raised in one unit is caught in
+// another.
 #pragma once
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
@@ -16,19 +17,17 @@
 #include <vector>
 
 #include "aether_fluid.h"
-#include "fluid_kernels.hpp"
 
 namespace aether {
-namespace {
 
-thread_local std::string g_last_error;
+inline thread_local std::string g_last_error;
 
 struct Failure {
     int code;
     std::string msg;
 };
 
-[[noreturn]] void raise(int code, const std::string &msg) { throw Failure{code, msg}; }
+[[noreturn]] inline void raise(int code, const std::string &msg) { throw Failure{code, msg}; }
 
 #define HIPT(expr)                                                                        \
     do {                                                                                  \
@@ -50,25 +49,6 @@ struct Failure {
         } else {                                \
             constexpr int VZU = 4;              \
             launch;                             \
-        }                                       \
-    } while (0)
-
-// Instantiates a launch for the z planes per workgroup of the one-launch CG.
-#define KZ_DISPATCH(kz, launch)                 \
-    do {                                        \
-        switch (kz) {                           \
-        case 8: {                               \
-            constexpr int KZ = 8;               \
-            launch;                             \
-        } break;                                \
-        case 32: {                              \
-            constexpr int KZ = 32;              \
-            launch;                             \
-        } break;                                \
-        default: {                              \
-            constexpr int KZ = 16;              \
-            launch;                             \
-        } break;                                \
         }                                       \
     } while (0)
 
@@ -129,7 +109,7 @@ struct DevBuf {
     DevBuf &operator=(const DevBuf &) = delete;
 };
 
-aether_fluid_params default_params() {
+inline aether_fluid_params default_params() {
     aether_fluid_params p;
     std::memset(&p, 0, sizeof(p));
     p.vorticity_eps = 5.5;  // src/fluid.rs:79-83
@@ -152,7 +132,7 @@ aether_fluid_params default_params() {
     return p;
 }
 
-void check_params(const aether_fluid_params &p) {
+inline void check_params(const aether_fluid_params &p) {
     if (p.solver_kind != AETHER_SOLVER_CG && p.solver_kind != AETHER_SOLVER_RBGS)
         raise(AETHER_EINVAL, "solver_kind must be AETHER_SOLVER_CG or AETHER_SOLVER_RBGS");
     if (p.solver_iters < 0) raise(AETHER_EINVAL, "solver_iters must be >= 0");
@@ -179,7 +159,7 @@ struct KernelTimer {
     };
     std::vector<Rec> recs;
     size_t used = 0;
-    double sum_ms[4] = {0, 0, 0, 0};  // cg_dir, cg_update, cg_resident, cg_fused
+    double sum_ms[4] = {0, 0, 0, 0};  // cg_dir, cg_update, (unused), cg_fused
     long count[4] = {0, 0, 0, 0};
     hipEvent_t get() {
         if (used == pool.size()) {
@@ -202,5 +182,64 @@ struct KernelTimer {
     }
 };
 
-}  // namespace
+// Interfaces shared across translation units; the implementations are
+// class templates private to solver_f32.hip, solver_f64.hip and batch.hip.
+
+// FluidMedium (src/fluid.rs:687-788): device snapshots of density and
+// temperature taken at from_solver, sampled by sample_density_and_temp.
+class MediumBase {
+   public:
+    virtual ~MediumBase() = default;
+    virtual void sample(const double *xyz, size_t npts, double *out) = 0;
+    double max_density = 0.0, majorant_sigma_t = 0.0, phase_g = 0.0;
+};
+
+// One grid (FluidSolver3D, src/fluid.rs:23-673).
+class SolverBase {
+   public:
+    virtual ~SolverBase() = default;
+    virtual void advect(int kind, double dt, double diss) = 0;
+    virtual void enforce_bc() = 0;
+    virtual void buoyancy_confinement(double dt) = 0;
+    virtual double divergence_norm() = 0;
+    virtual double divergence_max() = 0;
+    virtual void project() = 0;
+    virtual void solve_cg(int iters) = 0;
+    virtual void solve_rbgs(int sweeps) = 0;
+    virtual void inject_sources(double t, double dt) = 0;
+    virtual void telemetry(double t, aether_fluid_telemetry *out) = 0;
+    virtual void run(double dt, const double *times, int nsteps, aether_fluid_telemetry *out) = 0;
+    virtual void upload(int field, const double *src) = 0;
+    virtual void download(int field, double *dst) = 0;
+    virtual void sample(int field, const double *xyz, size_t npts, double *out) = 0;
+    virtual double max_density() = 0;
+    virtual std::unique_ptr<MediumBase> make_medium(const double *lo, const double *hi) = 0;
+    virtual void autotune(int iters, double *us, int *form, int *z) = 0;
+    virtual void stats(aether_fluid_solve_stats *out) = 0;
+    virtual const std::vector<int> &last_run_iters() const = 0;
+    virtual double rollout(const aether_rollout_desc &d, const double *ctrl, double beta,
+                           const double *target, double *grad_c, double *grad_beta) = 0;
+    aether_fluid_params params = default_params();
+    KernelTimer timer;
+};
+
+// Many grids advanced together, one launch per stage.
+class BatchBase {
+   public:
+    virtual ~BatchBase() = default;
+    virtual void autotune(int iters, double *us, int *form) = 0;
+    virtual void upload(int inst, int field, const double *src, size_t n) = 0;
+    virtual void download(int inst, int field, double *dst, size_t n) = 0;
+    virtual void run(double dt, const double *times, int nsteps, aether_fluid_telemetry *out) = 0;
+    virtual const std::vector<int> &iters() const = 0;
+    virtual int count() const = 0;
+};
+
+std::unique_ptr<SolverBase> make_solver_f32(int nx, int ny, int nz, double h, hipStream_t st,
+                                            bool ir64);
+std::unique_ptr<SolverBase> make_solver_f64(int nx, int ny, int nz, double h, hipStream_t st);
+std::unique_ptr<BatchBase> make_batch(const aether_fluid_instance_desc *desc, int count,
+                                      int precision, const aether_fluid_params &shared,
+                                      hipStream_t st);
+
 }  // namespace aether

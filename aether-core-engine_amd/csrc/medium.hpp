@@ -1,20 +1,12 @@
 // medium.hpp -- FluidMedium (src/fluid.rs:687-788) on the device.
-// Part of the library's single translation unit: included once, by
-// fluid_api.hip, after the headers before it there.
+// Included by the solver translation units (the medium samples the solver's
+// storage type).
 #pragma once
 #include "host_common.hpp"
+#include "fluid_kernels.hpp"
 
 namespace aether {
 namespace {
-
-// FluidMedium (src/fluid.rs:687-788): device snapshots of density and
-// temperature taken at from_solver, sampled by sample_density_and_temp.
-class MediumBase {
-   public:
-    virtual ~MediumBase() = default;
-    virtual void sample(const double *xyz, size_t npts, double *out) = 0;
-    double max_density = 0.0, majorant_sigma_t = 0.0, phase_g = 0.0;
-};
 
 template <typename T>
 class Medium final : public MediumBase {

@@ -1,11 +1,9 @@
 // solver.hpp -- one grid: the smoke step, its stages, the pressure solvers, the
 // autotuner and the differentiable rollout.
-// Part of the library's single translation unit: included once, by
-// fluid_api.hip, after the headers before it there.
+// Included by solver_f32.hip and solver_f64.hip, which instantiate it.
 #pragma once
 #include "host_common.hpp"
 #include "medium.hpp"
-#include "cg_resident.hpp"
 #include "cg_fused.hpp"
 #include "cg_vec.hpp"
 #include "cg_cgs4.hpp"
@@ -13,34 +11,6 @@
 
 namespace aether {
 namespace {
-
-class SolverBase {
-   public:
-    virtual ~SolverBase() = default;
-    virtual void advect(int kind, double dt, double diss) = 0;
-    virtual void enforce_bc() = 0;
-    virtual void buoyancy_confinement(double dt) = 0;
-    virtual double divergence_norm() = 0;
-    virtual double divergence_max() = 0;
-    virtual void project() = 0;
-    virtual void solve_cg(int iters) = 0;
-    virtual void solve_rbgs(int sweeps) = 0;
-    virtual void inject_sources(double t, double dt) = 0;
-    virtual void telemetry(double t, aether_fluid_telemetry *out) = 0;
-    virtual void run(double dt, const double *times, int nsteps, aether_fluid_telemetry *out) = 0;
-    virtual void upload(int field, const double *src) = 0;
-    virtual void download(int field, double *dst) = 0;
-    virtual void sample(int field, const double *xyz, size_t npts, double *out) = 0;
-    virtual double max_density() = 0;
-    virtual std::unique_ptr<MediumBase> make_medium(const double *lo, const double *hi) = 0;
-    virtual void autotune(int iters, double *us, int *form, int *z) = 0;
-    virtual void stats(aether_fluid_solve_stats *out) = 0;
-    virtual const std::vector<int> &last_run_iters() const = 0;
-    virtual double rollout(const aether_rollout_desc &d, const double *ctrl, double beta,
-                           const double *target, double *grad_c, double *grad_beta) = 0;
-    aether_fluid_params params = default_params();
-    KernelTimer timer;
-};
 
 template <typename T>
 class Solver final : public SolverBase {
@@ -81,19 +51,7 @@ class Solver final : public SolverBase {
         tl_.gy = gy;
         tl_.gz = (nz + zu_ - 1) / zu_;
         tl_.nb = gx * gy * tl_.gz;
-        if (const char *e = std::getenv("AETHER_XCD_REMAP")) remap_ok_ = std::atoi(e) != 0;
-        tl_.remap = (tl_.nb % 8 == 0 && remap_ok_) ? 1 : 0;
-        // AETHER_CG_FORM (1/2/3) overrides the AETHER_CG_AUTO choice
-        if (const char *e = std::getenv("AETHER_CG_FORM")) {
-            const int f = std::atoi(e);
-            if (f >= AETHER_CG_SCALAR && f <= AETHER_CG_FUSED) env_form_ = f;
-        }
-        // AETHER_CG_CYCLE: launches per d/p update of the one-launch
-        // four-wide CG (cg_cgs4.hpp)
-        if (const char *e = std::getenv("AETHER_CG_CYCLE")) {
-            const int k = std::atoi(e);
-            if (cgs4_cycle_ok(k)) cycle_ = k;
-        }
+        tl_.remap = (tl_.nb % 8 == 0) ? 1 : 0;
         // four-wide two-launch CG: 256-thread groups of txv float4 lanes x ty
         // rows x vzu planes (vzu = 4 measured faster than 2 at 128^3 and 256^3)
         if (nx % 4 == 0) {
@@ -103,18 +61,9 @@ class Solver final : public SolverBase {
             tlv_.gx = (w4 + tlv_.txv - 1) / tlv_.txv;
             tlv_.gy = (ny + tlv_.ty - 1) / tlv_.ty;
             vzu_ = 4;
-            if (const char *e = std::getenv("AETHER_CG_VZU")) {
-                const int z = std::atoi(e);
-                if (z == 2 || z == 4) vzu_ = z;
-            }
             tlv_.gz = (nz + vzu_ - 1) / vzu_;
-            // the four-wide CG kernels store write-through by default (Out4);
-            // AETHER_CG_NT=1 asks for non-temporal stores instead (slower in
-            // the step: they also skip the Infinity Cache)
-            if (const char *e = std::getenv("AETHER_CG_NT")) nt_ = std::atoi(e) != 0;
             tlv_.nb = tlv_.gx * tlv_.gy * tlv_.gz;
             tlv_.remap = (tlv_.nb % 8 == 0) ? 1 : 0;
-            if (const char *e = std::getenv("AETHER_XCD_REMAP")) tlv_.remap &= std::atoi(e) != 0;
         }
         // one-launch SOR sweep (k_rbsor4): the four-wide tiling with 2 planes
         // per thread (0.79 ms per 128^3 RB-SOR step against 0.87 with 4)
@@ -123,7 +72,6 @@ class Solver final : public SolverBase {
             tls_.gz = (nz + 1) / 2;
             tls_.nb = tls_.gx * tls_.gy * tls_.gz;
             tls_.remap = (tls_.nb % 8 == 0) ? 1 : 0;
-            if (const char *e = std::getenv("AETHER_XCD_REMAP")) tls_.remap &= std::atoi(e) != 0;
         }
         // one-launch four-wide CG (cg_cgs4.hpp): the four-wide tiling with 4
         // planes per thread
@@ -136,30 +84,23 @@ class Solver final : public SolverBase {
             tl4_.gz = (nz + 3) / 4;
             tl4_.nb = tl4_.gx * tl4_.gy * tl4_.gz;
             tl4_.remap = (tl4_.nb % 8 == 0) ? 1 : 0;
-            if (const char *e = std::getenv("AETHER_XCD_REMAP")) tl4_.remap &= std::atoi(e) != 0;
         }
-        // one-launch CG: 64 x 4 x KZ tiles, KZ = 16 unless overridden
-        if (const char *e = std::getenv("AETHER_CG_KZ")) {
-            const int z = std::atoi(e);
-            if (z == 8 || z == 16 || z == 32) kz_ = z;
-        }
-        if (const char *e = std::getenv("AETHER_CG_PF")) {
-            const int f = std::atoi(e);
-            if (f == 2 || f == 4 || f == 6) pf_ = f;
-        }
+        // one-launch CG with one cell per thread (nx % 4 != 0): 64 x 4 x 16 tiles
         tlf_.gx = gx;
         tlf_.gy = gy;
-        tlf_.gz = (nz + kz_ - 1) / kz_;
+        tlf_.gz = (nz + kFusedKZ - 1) / kFusedKZ;
         tlf_.nb = gx * gy * tlf_.gz;
         tlf_.remap = (tlf_.nb % 8 == 0) ? 1 : 0;
-        if (const char *e = std::getenv("AETHER_XCD_REMAP")) tlf_.remap &= std::atoi(e) != 0;
         const int npart = std::max(std::max(std::max(nb_, tlv_.nb), std::max(tl_.nb, tlf_.nb)),
                                    tl4_.nb);
-        if (const char *e = std::getenv("AETHER_CG_CHUNK")) cg_chunk_ = std::max(0, std::atoi(e));
-        if (const char *e = std::getenv("AETHER_ADJ_WINDOW")) adj_window_ = std::atoi(e) != 0;
-        HIPT(hipHostMalloc((void **)&done_host_, 64, hipHostMallocMapped));
-        done_host_[0] = 0;  // id of the last solve a CG launch stopped
-        done_host_[1] = 0;  // iterations of the last finished solve
+        // [0]: id of the last solve a CG launch stopped; [1..2]: one 64-bit
+        // word, (id << 32) | iterations of the last finished solve.
+        // Coherent, so device writes are seen without a cache flush.
+        HIPT(hipHostMalloc((void **)&done_host_, 64,
+                           hipHostMallocMapped | hipHostMallocCoherent));
+        done_host_[0] = 0;
+        done_host_[1] = 0;
+        done_host_[2] = 0;
         HIPT(hipHostGetDevicePointer((void **)&done_dev_, done_host_, 0));
         for (int a = 0; a < 5; ++a) {
             f_[a].alloc(n_);
@@ -180,7 +121,6 @@ class Solver final : public SolverBase {
         partC_.alloc(npart);
         partD64_.alloc(npart);
         tot_.alloc(4);
-        setup_resident(gx, gy, nz);
         part_tel_.alloc((size_t)nb_ * kTelParts);
         scalar_.alloc(8);
         tel_.alloc(8);
@@ -202,93 +142,6 @@ class Solver final : public SolverBase {
         sync();
     }
 
-    template <int ZT>
-    bool resident_fits(int nb) {
-        int per_cu = 0, cus = 0, dev = 0;
-        HIPT(hipGetDevice(&dev));
-        HIPT(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        HIPT(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cg_resident<T, ZT, 0>,
-                                                          kThreads, 0));
-        return per_cu > 0 && (long)per_cu * cus >= nb;
-    }
-
-    // The resident CG keeps every cell in registers: usable when all tiles
-    // can be co-resident on the device (about 2-4 M cells on MI355X).  It is
-    // opt-in (AETHER_CG_RESIDENT=1): on MI355X a grid-wide barrier over 1024
-    // workgroups measured ~46 us (per-workgroup agent-scope fences), against
-    // ~10 us for a whole streaming CG kernel, so the two-kernel streaming
-    // iteration is the faster default.
-    void setup_resident(int gx, int gy, int nz) {
-        const char *e = std::getenv("AETHER_CG_RESIDENT");
-        if (!e || std::atoi(e) == 0) return;
-        if constexpr (sizeof(T) != 4) return;  // FP64 keeps the streaming kernels
-        for (int zt : {8, 16}) {
-            Tiling t;
-            t.gx = gx;
-            t.gy = gy;
-            t.gz = (nz + zt - 1) / zt;
-            t.nb = gx * gy * t.gz;
-            t.remap = (t.nb % 8 == 0) ? 1 : 0;
-            const bool fits = zt == 8 ? resident_fits<8>(t.nb) : resident_fits<16>(t.nb);
-            if (fits) {
-                tl_res_ = t;
-                res_zt_ = zt;
-                break;
-            }
-        }
-        if (!res_zt_) return;
-        if (const char *e = std::getenv("AETHER_GRID_BARRIER")) barrier_mode_ = std::atoi(e) ? 1 : 0;
-        if ((int)partA_.n < tl_res_.nb) partA_.alloc(tl_res_.nb);
-        partD_.alloc(tl_res_.nb);
-        gsync_.alloc(1);
-        HIPT(hipMemsetAsync(gsync_.p, 0, sizeof(GridSync), st_));
-    }
-
-    void enqueue_cg_resident(int max_iters, int step_index) {
-        Dims g = g_;
-        Tiling tl = tl_res_;
-        int mi = max_iters;
-        double rel2 = rel_tol2();
-        const double *p0 = partB_.p, *b0 = partC_.p;
-        int nb0 = tl_.nb;
-        double *pa = partA_.p, *pd = partD_.p;
-        T *pg = p_.p, *rg = r_.p, *dg = d_[0].p;
-        CgState *st = cg_.p;
-        int *log = step_index >= 0 ? iters_dev_.p + step_index : iters_log_.p;
-        GridSync *gs = gsync_.p;
-        void *args[] = {&g, &tl, &mi, &rel2, &p0, &b0, &nb0, &pa, &pd, &pg, &rg, &dg, &st, &log,
-                        &gs};
-        k_grid_sync_reset<<<1, 64, 0, st_>>>(gsync_.p);
-        LAUNCH_CHECK();
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        if (timer.enabled) {
-            e0 = timer.get();
-            HIPT(hipEventRecord(e0, st_));
-        }
-        const void *fn;
-        if (barrier_mode_ == 1)
-            fn = res_zt_ == 8 ? (const void *)k_cg_resident<T, 8, 1>
-                              : (const void *)k_cg_resident<T, 16, 1>;
-        else
-            fn = res_zt_ == 8 ? (const void *)k_cg_resident<T, 8, 0>
-                              : (const void *)k_cg_resident<T, 16, 0>;
-        HIPT(hipLaunchCooperativeKernel(fn, dim3(tl.nb), cell_blk_, args, 0, st_));
-        if (timer.enabled) {
-            e1 = timer.get();
-            HIPT(hipEventRecord(e1, st_));
-            timer.recs.push_back({2, step_index, 0, e0, e1});
-        }
-        last_kind_ = AETHER_SOLVER_CG;
-    }
-
-    void check_grid_sync() {
-        if (!res_zt_) return;
-        GridSync gs;
-        HIPT(hipMemcpyAsync(&gs, gsync_.p, sizeof(gs), hipMemcpyDeviceToHost, st_));
-        sync();
-        if (gs.error) raise(AETHER_EHIP, "resident CG: grid barrier timed out");
-    }
-
     ~Solver() override {
         (void)hipStreamSynchronize(st_);
         for (auto e : chunk_events_) (void)hipEventDestroy(e);
@@ -303,6 +156,12 @@ class Solver final : public SolverBase {
     void zero(T *x) {
         k_fill<T><<<blocks_for(n_), kThreads, 0, st_>>>(x, (long)n_, T(0));
         LAUNCH_CHECK();
+    }
+    // Workgroups of a grid-stride reduction writing part[blockIdx.x]: never
+    // more than the partial-sum buffer holds.
+    static unsigned partial_blocks(size_t n, const DevBuf<double> &part) {
+        if (part.n == 0) raise(AETHER_EINTERNAL, "reduction into an unallocated buffer");
+        return std::min<unsigned>(blocks_for(n), (unsigned)std::min<size_t>(part.n, 4096));
     }
     static unsigned blocks_for(size_t n) {
         size_t b = (n + kThreads - 1) / kThreads;
@@ -485,44 +344,29 @@ class Solver final : public SolverBase {
                 double rt = rel2;
                 const T *w_in = w_[it & 1].p;
                 T *w_out = w_[(it + 1) & 1].p, *dd = d_[0].p;
-                // r rotates through a ring of cycle_ arrays: launch `it` reads
+                // r rotates through a ring of kCgsCycle arrays: launch `it` reads
                 // ring[it % K], writes ring[(it + 1) % K] (which holds
                 // r_(it+1-K)) and finds r_(it-1) in ring[(it - 1) % K]
                 CgRing ring{};
-                for (int q = 0; q < cycle_; ++q) ring.r[q] = ring_at(q);
+                for (int q = 0; q < kCgsCycle; ++q) ring.r[q] = ring_at(q);
                 void *args[] = {&g,      &vt,     &itv, &rt,   &nbv,  &pg_in, &pd_in, &pbb, &pg_out,
                                 &pd_out, &st,     &ring, &w_in, &w_out, &dd,   &pp,    &dm,  &sid};
                 const bool wide = std::max(nb_in, vt.nb) > 4 * kThreads;
-                // write-through stores (arrays under 2 GiB), unless
-                // AETHER_CG_NT asks for non-temporal ones
-                const int pol = nt_ ? kStNT : wt_ok() ? kStWT : kStPlain;
-                const void *fn = cgs4_kernel(cycle_, it == 0, it % cycle_, wide, pol);
+                // write-through stores for arrays under 2 GiB
+                const void *fn = cgs4_kernel(it == 0, it % kCgsCycle, wide, wt_ok());
                 HIPT(hipExtLaunchKernel(fn, dim3(vt.nb), dim3(kThreads), args, 0, st_, e0, e1, 0));
                 return;
             }
-            KZ_DISPATCH(kz_, ({
-                const void *fn;
-                switch (pf_) {
-                case 2:
-                    fn = it == 0 ? (const void *)k_cg_fused<T, KZ, 2, true>
-                                 : (const void *)k_cg_fused<T, KZ, 2, false>;
-                    break;
-                case 6:
-                    fn = it == 0 ? (const void *)k_cg_fused<T, KZ, 6, true>
-                                 : (const void *)k_cg_fused<T, KZ, 6, false>;
-                    break;
-                default:
-                    fn = it == 0 ? (const void *)k_cg_fused<T, KZ, 4, true>
-                                 : (const void *)k_cg_fused<T, KZ, 4, false>;
-                }
-                Dims g = g_;
-                Tiling tl = tlf_;
-                int itv = it, nbv = nb_in, sid = solve_id;
-                double rt = rel2;
-                void *args[] = {&g,   &tl,    &itv, &rt, &nbv, &pg_in, &pd_in, &pbb, &pg_out,
-                                &pd_out, &st, &r_in, &r_out, &d_in, &d_out, &pp, &dm, &sid};
-                HIPT(hipExtLaunchKernel(fn, dim3(tlf_.nb), cell_blk_, args, 0, st_, e0, e1, 0));
-            }));
+            // one cell per thread (nx % 4 != 0): radius-2 recompute (cg_fused.hpp)
+            const void *fn = it == 0 ? (const void *)k_cg_fused<T, kFusedKZ, kFusedPF, true>
+                                     : (const void *)k_cg_fused<T, kFusedKZ, kFusedPF, false>;
+            Dims g = g_;
+            Tiling tl = tlf_;
+            int itv = it, nbv = nb_in, sid = solve_id;
+            double rt = rel2;
+            void *args[] = {&g,      &tl, &itv,  &rt,   &nbv,  &pg_in, &pd_in, &pbb, &pg_out,
+                            &pd_out, &st, &r_in, &r_out, &d_in, &d_out, &pp,   &dm,  &sid};
+            HIPT(hipExtLaunchKernel(fn, dim3(tlf_.nb), cell_blk_, args, 0, st_, e0, e1, 0));
         } else {
             raise(AETHER_EINVAL, "the one-launch CG is built for FP32 only");
         }
@@ -547,7 +391,7 @@ class Solver final : public SolverBase {
             if (trial_form_) return trial_form_;  // the autotuner's candidate
             int f = params.cg_form;
             if (f == AETHER_CG_AUTO)
-                f = env_form_ ? env_form_ : tuned_form_ ? tuned_form_ : default_form();
+                f = tuned_form_ ? tuned_form_ : default_form();
             if (f == AETHER_CG_X4 && g_.nx % 4 != 0)
                 raise(AETHER_EINVAL, "AETHER_CG_X4 needs nx % 4 == 0");
             return f;
@@ -569,13 +413,13 @@ class Solver final : public SolverBase {
         zu_ = zu;
         tl_.gz = (g_.nz + zu_ - 1) / zu_;
         tl_.nb = tl_.gx * tl_.gy * tl_.gz;
-        tl_.remap = (tl_.nb % 8 == 0 && remap_ok_) ? 1 : 0;
+        tl_.remap = (tl_.nb % 8 == 0) ? 1 : 0;
         ensure_partials(tl_.nb);
         if (tlv_.nb > 0) {
             vzu_ = vzu;
             tlv_.gz = (g_.nz + vzu_ - 1) / vzu_;
             tlv_.nb = tlv_.gx * tlv_.gy * tlv_.gz;
-            tlv_.remap = (tlv_.nb % 8 == 0 && remap_ok_) ? 1 : 0;
+            tlv_.remap = (tlv_.nb % 8 == 0) ? 1 : 0;
             ensure_partials(tlv_.nb);
         }
     }
@@ -675,7 +519,7 @@ class Solver final : public SolverBase {
                     b->alloc(n_);
                     zero(b->p);
                 }
-            for (int q = 0; q + 2 < cycle_; ++q)  // the residual ring past r_ and r2_
+            for (int q = 0; q + 2 < kCgsCycle; ++q)  // the residual ring past r_ and r2_
                 if (!rring_[q].p) {
                     rring_[q].alloc(n_);
                     zero(rring_[q].p);
@@ -726,7 +570,7 @@ class Solver final : public SolverBase {
                 pbb = tot_.p + 1;
                 nbv = 1;
             }
-            const bool wt = !nt_ && wt_ok();  // write-through stores (see Out4)
+            const bool wt = wt_ok();  // write-through stores (see Out4)
             // p is brought up to date on odd iterations (body_cg_update4)
             const bool odd = it & 1;
             const T *dprev = dold;  // d_(it-1), for the odd update
@@ -744,8 +588,6 @@ class Solver final : public SolverBase {
                     X4_PICK(1, 16, kStWT);
                 else if (wide)
                     X4_PICK(1, 16, kStPlain);
-                else if (nt_)
-                    X4_PICK(4, 4, kStNT);
                 else if (wt)
                     X4_PICK(4, 4, kStWT);
                 else
@@ -809,18 +651,26 @@ class Solver final : public SolverBase {
     // device; the host only watches a mapped "solve finished" word, two
     // chunks behind, to stop queueing launches once the device has stopped
     // (launches queued after the stop return without touching memory other
-    // than their loads).  When the last finished solve ran to the cap, the
-    // whole solve is queued at once: runs of capped solves (the 128^3 smoke
-    // step) then wait on no event.  Solves that stop early keep the paced
-    // chunks, since a long first chunk would queue early-exit launches.
+    // than their loads).  When one of the last two solves finished at the
+    // cap, the whole solve is queued at once: runs of capped solves (the
+    // 128^3 smoke step) then wait on no event.  Solves that stop early keep
+    // the paced chunks, since a long first chunk would queue early-exit
+    // launches.  The host runs up to two solves ahead of the device, so the
+    // latest finished solve may be the one before the previous; an older
+    // count is not used.  After a capped solve, at most two whole solves of
+    // early-exit launches are queued before a converging one is seen.
     void enqueue_cg_loop(int max_iters, int step_index) {
         const double rel2 = rel_tol2();
         const int solve_id = ++solve_seq_;
-        int chunk = cg_chunk_;
-        if (chunk <= 0) chunk = max_iters > 0 ? max_iters : 1;
+        const int chunk = kCgChunk;
         int first = chunk;
-        if (cg_chunk_ > 0 && __atomic_load_n(done_host_ + 1, __ATOMIC_ACQUIRE) >= max_iters)
-            first = std::max(chunk, max_iters);
+        {
+            const unsigned long long last = __atomic_load_n(
+                reinterpret_cast<unsigned long long *>(done_host_ + 2), __ATOMIC_ACQUIRE);
+            const int last_id = (int)(last >> 32), last_iters = (int)(last & 0xffffffffu);
+            if (last_id > 0 && last_id >= solve_id - 2 && last_iters >= max_iters)
+                first = std::max(chunk, max_iters);
+        }
         // chunk c: launches [c_begin(c), c_begin(c + 1)) (capped at max_iters)
         auto c_begin = [&](int c) { return c == 0 ? 0 : first + (c - 1) * chunk; };
         const int nchunks = max_iters > first ? 1 + (max_iters - first + chunk - 1) / chunk : 1;
@@ -843,7 +693,7 @@ class Solver final : public SolverBase {
             LAUNCH_CHECK();
         }
         for (int c = 0; c < nchunks; ++c) {
-            if (c >= 2 && cg_chunk_ > 0) {
+            if (c >= 2) {
                 HIPT(hipEventSynchronize(chunk_events_[c - 2]));
                 if (__atomic_load_n(done_host_, __ATOMIC_ACQUIRE) == solve_id) break;
             }
@@ -854,10 +704,10 @@ class Solver final : public SolverBase {
         }
         if constexpr (sizeof(T) == 4) {
             // the four-wide two-launch form brings p up to date on odd
-            // iterations, the one-launch form once every cycle_ launches: a
+            // iterations, the one-launch form once every kCgsCycle launches: a
             // cap that ends the solve on another iteration leaves terms pending
             const bool fused4 = form_ == AETHER_CG_FUSED && tl4_.nb > 0;
-            if (max_iters > 0 && ((fused4 && max_iters % cycle_ != 0) ||
+            if (max_iters > 0 && ((fused4 && max_iters % kCgsCycle != 0) ||
                                   (form_ == AETHER_CG_X4 && (max_iters & 1)))) {
                 const long n4 = (long)(n_ / 4);
                 const unsigned nbt =
@@ -867,9 +717,10 @@ class Solver final : public SolverBase {
                     // the residuals the pending launches read, in launch order
                     // (launch `it` read ring[it % K])
                     CgRing pend{};
-                    const int j0 = last - last % cycle_;
-                    for (int q = 0; q <= last % cycle_; ++q) pend.r[q] = ring_at((j0 + q) % cycle_);
-                    cgs4_tail_launch(cycle_, nbt, st_, cg_.p, last, pend,
+                    const int j0 = last - last % kCgsCycle;
+                    for (int q = 0; q <= last % kCgsCycle; ++q)
+                        pend.r[q] = ring_at((j0 + q) % kCgsCycle);
+                    cgs4_tail_launch(nbt, st_, cg_.p, last, pend,
                                      reinterpret_cast<const float4 *>(d_[0].p),
                                      reinterpret_cast<float4 *>(p_.p), n4);
                 } else {
@@ -891,7 +742,9 @@ class Solver final : public SolverBase {
         k_cg_finalize<<<1, cell_blk_, 0, st_>>>(max_iters, rel2, part_rr, partC_.p, nb_rr, cg_.p,
                                                  step_index >= 0 ? iters_dev_.p + step_index
                                                                  : iters_log_.p,
-                                                 done_dev_ + 1);
+                                                 reinterpret_cast<unsigned long long *>(
+                                                     done_dev_ + 2),
+                                                 solve_id);
         LAUNCH_CHECK();
         last_kind_ = AETHER_SOLVER_CG;
     }
@@ -1008,10 +861,7 @@ class Solver final : public SolverBase {
                                   g_, tl_, (T)h_, 1, cur_[0], cur_[1], cur_[2], p_.p, b_.p,
                                   r_.p, partB_.p, partC_.p, cg_.p)));
             LAUNCH_CHECK();
-            if (res_zt_)
-                enqueue_cg_resident(params.solver_iters, step_index);
-            else
-                enqueue_cg_loop(params.solver_iters, step_index);
+            enqueue_cg_loop(params.solver_iters, step_index);
         } else {
             k_divergence<T><<<red_grid_, cell_blk_, 0, st_>>>(g_, zchunk_, (T)h_, (T)(1.0 / h_),
                                                                1, cur_[0], cur_[1], cur_[2],
@@ -1029,7 +879,6 @@ class Solver final : public SolverBase {
     void project() override {
         enqueue_project(-1);
         sync();
-        check_grid_sync();
     }
 
     void solve_cg(int iters) override {
@@ -1038,12 +887,8 @@ class Solver final : public SolverBase {
                               g_, tl_, (T)h_, 0, cur_[0], cur_[1], cur_[2], p_.p, b_.p, r_.p,
                               partB_.p, partC_.p, cg_.p)));
         LAUNCH_CHECK();
-        if (res_zt_)
-            enqueue_cg_resident(iters, -1);
-        else
-            enqueue_cg_loop(iters, -1);
+        enqueue_cg_loop(iters, -1);
         sync();
-        check_grid_sync();
     }
 
     void solve_rbgs(int sweeps) override {
@@ -1188,7 +1033,6 @@ class Solver final : public SolverBase {
         sync();
         if (out)
             for (int s = 0; s < nsteps; ++s) unpack(raw.data() + (size_t)s * 8, out + s);
-        check_grid_sync();
         if (ir64_) iters.assign(ir_log_.begin(), ir_log_.end());
         if (params.solver_kind == AETHER_SOLVER_RBGS)
             std::fill(iters.begin(), iters.end(), params.solver_iters);
@@ -1289,7 +1133,7 @@ class Solver final : public SolverBase {
 
     double max_density() override {
         // src/fluid.rs:704-707: max(0, all density entries)
-        const unsigned nbk = std::min<unsigned>(blocks_for(n_), (unsigned)nb_);
+        const unsigned nbk = partial_blocks(n_, partA_);
         k_max_part<T><<<nbk, cell_blk_, 0, st_>>>(cur_[3], (long)n_, partA_.p);
         LAUNCH_CHECK();
         k_partials_final<<<1, kThreads, 0, st_>>>(partA_.p, (int)nbk, 1, scalar_.p);
@@ -1399,7 +1243,7 @@ class Solver final : public SolverBase {
         // per CU (256); smaller grids keep the one-plane launches, which
         // spread over more workgroups (measured: 128^3 fwd+adj/fwd 3.23 ->
         // 2.27 windowed; 64^3 2.74 direct, 2.83 windowed)
-        if (adj_window_ && (long)grid.x * grid.y * grid.z >= 256) {
+        if ((long)grid.x * grid.y * grid.z >= 256) {
             const dim3 blk(WTX, WTY);
             k_adj_mac_correct_w<T><<<grid, blk, 0, st_>>>(g_, s, (T)diss, clamp0, src,
                                                           fwd_[0].p, U, V, W, lam_out, lam_src,
@@ -1545,7 +1389,7 @@ class Solver final : public SolverBase {
                     if (!b->p) b->alloc(n_);
                 if (!adj_part_.p) adj_part_.alloc(cell_grid_.x * cell_grid_.y * cell_grid_.z);
             }
-            const unsigned lb = blocks_for(n_);
+            const unsigned lb = partial_blocks(n_, partA_);
             k_loss<T><<<lb, kThreads, 0, st_>>>(cur_[3], target_.p, (long)n_,
                                                 want ? adj_ar_.p : nullptr, partA_.p);
             LAUNCH_CHECK();
@@ -1595,25 +1439,18 @@ class Solver final : public SolverBase {
     int zchunk_, nb_;
     Tiling tl_;
     int zu_ = 4;
-    int env_form_ = 0;    // AETHER_CG_FORM override of AETHER_CG_AUTO (0: none)
     int tuned_form_ = 0;  // the autotuner's AETHER_CG_AUTO choice (0: none)
     int trial_form_ = 0;  // layout forced while the autotuner times a candidate
-    bool adj_window_ = true;  // LDS-windowed adjoint scatters (AETHER_ADJ_WINDOW=0: direct)
-    bool remap_ok_ = true;  // XCD-contiguous tile order allowed (AETHER_XCD_REMAP)
     int form_ = AETHER_CG_SCALAR;  // layout of the solve being enqueued
     VTiling tlv_ = {0, 0, 0, 0, 0, 1, 1};  // four-wide CG tiling (nx % 4 == 0)
     int vzu_ = 4;
-    bool nt_ = false;  // non-temporal stores in the four-wide CG
-    Tiling tlf_;          // one-launch CG: 64 x 4 x kz_ tiling
+    Tiling tlf_;          // one-launch CG, one cell per thread: 64 x 4 x kFusedKZ tiling
     VTiling tl4_ = {0, 0, 0, 0, 0, 1, 1};  // one-launch four-wide CG tiling (nb 0: unusable)
-    int kz_ = 16;
-    int pf_ = 4;          // planes of loads in flight ahead of use
     DevBuf<T> r2_;                 // second residual buffer of the one-launch CG
     DevBuf<T> p2_;                 // second pressure buffer of the one-launch SOR sweep
     VTiling tls_ = {0, 0, 0, 0, 0, 1, 1};  // its tiling (2 planes per thread; nb 0: unusable)
     DevBuf<T> w_[2];               // A r of the four-wide one-launch CG (ping-pong)
     DevBuf<T> rring_[kCgRingMax - 2];  // its residual ring past r_ and r2_
-    int cycle_ = kCgsCycle;        // its launches per d/p update
     DevBuf<double> partE_, partF_;  // its delta partials
     bool ir64_ = false;           // mixed-precision projection (AETHER_FP32_IR64)
     // rollout / adjoint state
@@ -1629,19 +1466,13 @@ class Solver final : public SolverBase {
     int ir_outer_ = 0, ir_inner_ = 0;
     double ir_div_ = 0.0;
     std::vector<int> ir_log_;
-    int cg_chunk_ = 8;         // CG iterations per host look-ahead chunk (0: no look-ahead)
+    static constexpr int kCgChunk = 8;  // CG iterations per host look-ahead chunk
     int solve_seq_ = 0;        // id of the current pressure solve
-    int *done_host_ = nullptr; // mapped pinned words: [0] the device sets to the id of a
-                               // solve a CG launch stopped, [1] to the iterations of the
-                               // last finished solve
+    int *done_host_ = nullptr; // mapped coherent host words: [0] the device sets to the id
+                               // of a solve a CG launch stopped; [1..2] (one 64-bit word)
+                               // to (id << 32) | iterations of the last finished solve
     int *done_dev_ = nullptr;
     std::vector<hipEvent_t> chunk_events_;
-    // resident (single cooperative launch) CG: tiling, eligibility
-    Tiling tl_res_;
-    int res_zt_ = 0;  // 0: not usable on this grid/device, else cells per thread
-    int barrier_mode_ = 0;  // grid barrier of the resident CG (see cg_resident.hpp)
-    DevBuf<double> partD_;
-    DevBuf<GridSync> gsync_;
     DevBuf<T> f_[5], alt_[5];
     T *cur_[5], *nxt_[5];  // u, v, w, density, temperature (current / scratch)
     DevBuf<T> fwd_[2], curl_, p_, b_, r_, d_[2];

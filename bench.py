@@ -8,8 +8,9 @@ relative residual 1e-5 (max 200 iterations).  A "step" is one full
 FluidSolver3D::step (src/fluid.rs:598-611) of that grid.
 
 Prints one JSON line: voxel-steps/s (value), the HBM roofline of the
-dominant kernel (CG update) measured with HIP events on the solver's stream,
-and the CPU oracle timed on this host's cores (cpu_baseline).
+dominant kernel (k_cg_cgs4, the one-launch CG iteration) measured with HIP
+events on the solver's stream, and the CPU oracle timed on this host's cores
+(cpu_baseline).
 """
 import argparse
 import json
@@ -73,26 +74,20 @@ def kernel_roofline(solver_kind: str, nvox: int, steps: int = 2):
 
     Algorithmic bytes (FP32 array passes of N cells): cg_dir reads r, d_old
     and writes d (3 passes); cg_update reads p, d, r and writes p, r (5
-    passes); a launch that runs whole CG iterations counts the same 8
-    passes per iteration: the register-resident solve (per iteration it
-    runs) and the one-launch iteration (cg_fused; the four-wide one moves
-    6.67 passes on average, carrying A r, rebuilding A d from the previous
-    residual and touching d and p on every third launch only, which the
-    measured traffic shows)."""
+    passes); the one-launch iteration (k_cg_cgs4) counts the 8 passes of a
+    whole CG iteration.  It physically moves 6.25 on average: it carries
+    A r, rebuilds A d from the previous residual and touches d and p on
+    every eighth launch only, which the measured traffic shows."""
     cfg, s = make_solver(solver_kind)
     s.set_kernel_timing(True)
     s.run(cfg["dt"], [cfg["dt"] * i for i in range(steps)])
     iters = s.run_iters()
     kern = {}
-    for which, name, passes in ((0, "cg_dir", 3), (1, "cg_update", 5), (2, "cg_resident", None),
-                                (3, "cg_fused", 8)):
+    for which, name, passes in ((0, "cg_dir", 3), (1, "cg_update", 5), (3, "k_cg_cgs4", 8)):
         avg, n = s.kernel_timing(which)
         if n == 0:
             continue
-        if passes is None:
-            bytes_ = 8 * 4.0 * nvox * (sum(iters) / n)  # mean iterations per launch
-        else:
-            bytes_ = passes * 4.0 * nvox
+        bytes_ = passes * 4.0 * nvox
         kern[name] = {"avg_ms": avg, "launches": n, "bytes": bytes_, "total_ms": avg * n,
                       "gbps": bytes_ / (avg * 1e-3) / 1e9 if avg > 0 else 0.0}
     s.set_kernel_timing(False)

@@ -279,7 +279,7 @@ int aether_fluid_rollout(aether_fluid_t s, const aether_rollout_desc *d, const d
 /* Average device time (ms) per launch of the CG iteration kernels during the
  * last aether_fluid_run with timing enabled (HIP events on the solver's
  * stream); *launches counts the kernels timed.  which: 0 direction launch
- * (cg_dir), 1 update launch (cg_update), 2 register-resident solve,
+ * (cg_dir), 1 update launch (cg_update), 2 (no longer used: an error),
  * 3 one-launch iteration (cg_fused). */
 int aether_fluid_set_kernel_timing(aether_fluid_t s, int enable);
 /* Plain device-to-device float4 copy of nbytes (multiple of 16) of

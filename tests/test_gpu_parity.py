@@ -549,16 +549,14 @@ def test_invalid_arguments_raise():
     assert s._lib.aether_fluid_last_run_iterations(s._h, None, 4, C.byref(cnt)) == L.AETHER_EINVAL
 
 
-def _step_128(resident: bool, solver="cg"):
-    import os
+def _step_128(solver="cg"):
+    """Two steps of the seeded 128^3 workload with the default CG layout (the
+    one-launch four-wide form, whose queue-whole schedule the second solve
+    takes after a capped first one)."""
     import aether_fluid as af
     from aether_fluid.scenarios import SMOKE_CONFIGS, configure, seeded_config_state
-    os.environ["AETHER_CG_RESIDENT"] = "1" if resident else "0"
-    try:
-        cfg = SMOKE_CONFIGS["plain_128"]
-        s = af.FluidSolver3D(*cfg["dims"], h=cfg["h"], precision="fp32")
-    finally:
-        os.environ.pop("AETHER_CG_RESIDENT", None)
+    cfg = SMOKE_CONFIGS["plain_128"]
+    s = af.FluidSolver3D(*cfg["dims"], h=cfg["h"], precision="fp32")
     configure(s, "plain_128", solver)
     for n, a in seeded_config_state("plain_128").items():
         s.upload(n, a)
@@ -566,110 +564,16 @@ def _step_128(resident: bool, solver="cg"):
     return s
 
 
-def test_resident_cg_matches_streaming_cg_128():
-    """The single-launch register-resident CG and the two-kernel streaming CG
-    run the same iterations on the full 128^3 workload (2 steps)."""
-    _announce("resident vs streaming CG 128^3")
-    a = _step_128(True)
-    b = _step_128(False)
-    assert a.run_iters() == b.run_iters() or all(
-        abs(x - y) <= 1 for x, y in zip(a.run_iters(), b.run_iters()))
-    for n in ("u", "v", "w", "pressure", "density"):
-        err = rel_l2(a.download(n), b.download(n))
-        assert err <= 1e-5, (n, err)
-    sa, sb = a.last_solve_stats(), b.last_solve_stats()
-    assert abs(sa["r2"] - sb["r2"]) <= 1e-3 * sb["r2"]
-
-
-def test_wide_four_wide_cg_matches_scalar_cg():
-    """Past 1024 partial sums (168x160x160: 1080 four-wide tiles) the
-    two-launch four-wide CG has one workgroup sum the r.r partials before
-    each direction launch, and the one-launch form holds 16 partials per
-    thread.
-    It runs the same iterations as the one-cell-per-thread CG, reaches the
-    same fields, and repeats bit for bit."""
-    _announce("wide four-wide CG vs scalar CG 168x160x160")
-    import aether_fluid as af
-    dims = (168, 160, 160)
-    state = random_state(*dims, seed=3)
-    runs = []
-    for form in (af.CG_X4, af.CG_X4, af.CG_SCALAR, af.CG_FUSED):
-        s = af.FluidSolver3D(*dims, precision="fp32")
-        s.cg_form, s.solver_iters, s.cg_rel_tol = form, 200, 1e-4
-        for name, arr in state.items():
-            s.upload(name, arr)
-        s.project()
-        runs.append((s.last_solve_stats(), {n: s.download(n) for n in ("u", "v", "w", "pressure")}))
-    (sa, fa), (sb, fb), (sc, fc), (sd, fd) = runs
-    assert sa["iters"] == sb["iters"]
-    for n in fa:
-        np.testing.assert_array_equal(fa[n], fb[n])
-    assert 0 < sa["iters"] < 200 and abs(sa["iters"] - sc["iters"]) <= 1, (sa, sc)
-    # the one-launch form at this size: 1080 tiles, its wide-partials variant
-    assert abs(sd["iters"] - sc["iters"]) <= 1, (sd, sc)
-    for n in fa:
-        assert rel_l2(fa[n], fc[n]) <= 1e-5, (n, rel_l2(fa[n], fc[n]))
-        assert rel_l2(fd[n], fc[n]) <= 1e-5, (n, rel_l2(fd[n], fc[n]))
-
-
-@pytest.mark.parametrize("form", ["fused", "x4"])
-@pytest.mark.parametrize("iters,tol", [(25, 0.0), (200, 3e-2), (200, 1e-2), (200, 3e-3),
-                                       (200, 1e-3)])
-def test_four_wide_cg_pending_p_update_vs_oracle(form, iters, tol):
-    """Both four-wide CG forms bring p up to date on odd iterations only: an
-    odd iteration cap (25, no tolerance) leaves the last term to the tail
-    kernel, and the tolerances stop the oracle after 11, 16, 27 and 43
-    iterations (mostly odd, so the stopping launch applies the pending
-    term). FP32 bar against the f64 oracle."""
-    _announce(f"four-wide CG ({form}) pending p update 64x48x40 iters={iters} tol={tol}")
-    import aether_fluid as af
-    dims = (64, 48, 40)
-    gpu, orc = make_pair(*dims, precision="fp32", state=random_state(*dims, seed=5))
-    set_knobs(gpu, orc, iters=iters, rel_tol=tol)
-    gpu.cg_form = af.CG_FUSED if form == "fused" else af.CG_X4
-    gpu.project()
-    orc.project()
-    so, sg = orc.stats(), gpu.last_solve_stats()
-    if tol == 0.0:
-        assert sg["iters"] == so["iters"] == iters
-    else:
-        assert abs(sg["iters"] - so["iters"]) <= 1, (sg, so)
-    errs = compare_fields(gpu, orc, ("u", "v", "w", "pressure"))
-    assert max(errs.values()) <= 1e-5, errs
-
-
-@pytest.mark.parametrize("form", ["fused", "x4"])
-@pytest.mark.parametrize("tol", [1e-2, 5e-3, 3e-3])
-def test_four_wide_cg_stop_with_two_rounds_of_tiles(form, tol):
-    """128x128x160: 640 four-wide tiles, more than are resident at once for
-    the one-launch form, so some workgroups of the stopping launch start
-    after workgroup 0 has written the stop. They must still apply the
-    pending p term. The oracle stops after 21, 27 and 33 iterations."""
-    _announce(f"four-wide CG ({form}) stop, two rounds of tiles, tol={tol}")
-    import aether_fluid as af
-    dims = (128, 128, 160)
-    gpu, orc = make_pair(*dims, precision="fp32", state=random_state(*dims, seed=21))
-    set_knobs(gpu, orc, iters=200, rel_tol=tol)
-    gpu.cg_form = af.CG_FUSED if form == "fused" else af.CG_X4
-    gpu.project()
-    orc.project()
-    so, sg = orc.stats(), gpu.last_solve_stats()
-    assert abs(sg["iters"] - so["iters"]) <= 1, (sg, so)
-    errs = compare_fields(gpu, orc, ("u", "v", "w", "pressure"))
-    assert max(errs.values()) <= 1e-5, errs
-
-
-@pytest.mark.parametrize("cycle", [3, 4, 8])
-def test_one_launch_cg_cycle_positions_vs_oracle(cycle, monkeypatch):
-    """The one-launch four-wide CG brings d and p up to date once every
-    `cycle` launches.  Iteration caps 16..23 end the solve at every position
-    of the cycle (the tail kernel closes caps that are not multiples of it),
-    and tolerances taken from the oracle's residual after 16..23 iterations
-    stop it at every position (the stopping launch applies the pending
-    terms).  FP32 bar against the f64 oracle."""
+def test_one_launch_cg_cycle_positions_vs_oracle():
+    """The one-launch four-wide CG brings d and p up to date once every 8
+    launches.  Iteration caps 16..23 end the solve at every position of the
+    cycle (the tail kernel closes caps that are not multiples of it), and
+    tolerances taken from the oracle's residual after 16..23 iterations stop
+    it at every position (the stopping launch applies the pending terms).
+    FP32 bar against the f64 oracle."""
     import aether_fluid as af
     from pairs import load_state
-    monkeypatch.setenv("AETHER_CG_CYCLE", str(cycle))
+    cycle = 8
     dims = (64, 48, 40)
     state = random_state(*dims, seed=5)
     gpu, orc = make_pair(*dims, precision="fp32", state=state)
@@ -767,8 +671,8 @@ def test_full_size_128_projection_properties():
     reported value, the projected field's divergence equals -r/h^2 per cell
     (so max|div u| = max|r|/h^2), density stays in [0, 1], and a repeat run
     is bitwise identical."""
-    _announce("128^3 properties")
-    s = _step_128(True)
+    _announce("128^3 properties (default one-launch CG)")
+    s = _step_128()
     st = s.last_solve_stats()
     h = s.h
     div_max = s.divergence_max()
@@ -777,8 +681,10 @@ def test_full_size_128_projection_properties():
     assert div_max <= (st["r2"] ** 0.5) / (h * h) * 1.0001 + 1e-6
     d = s.download("density")
     assert d.min() >= 0.0 and d.max() <= 1.0 + 1e-6
-    t = _step_128(True)
-    np.testing.assert_array_equal(s.download("pressure"), t.download("pressure"))
+    t = _step_128()
+    assert s.run_iters() == t.run_iters()
+    for n in ("u", "v", "w", "density", "temperature", "pressure"):
+        np.testing.assert_array_equal(s.download(n), t.download(n), err_msg=n)
 
 
 def _mixed_state(dims, seed=0xAE7E0002):
