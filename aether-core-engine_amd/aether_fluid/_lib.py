@@ -127,6 +127,9 @@ SIGNATURES = {
     "aether_fluid_solve_poisson_gauss_seidel": (C.c_int, [_vp, C.c_int]),
     "aether_fluid_inject_sources": (C.c_int, [_vp, C.c_double, C.c_double]),
     "aether_fluid_compute_telemetry": (C.c_int, [_vp, C.c_double, C.POINTER(Telemetry)]),
+    "aether_fluid_compute_telemetry_noalloc": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int,
+                                                         C.c_double, _vp, _vp, _vp, _vp,
+                                                         C.c_double, _vp, _vp]),
     "aether_fluid_sample_trilinear": (C.c_int, [_vp, C.c_int, _pd, C.c_size_t, _pd]),
     "aether_fluid_divergence_max": (C.c_int, [_vp, _pd]),
     "aether_fluid_last_solve_stats": (C.c_int, [_vp, C.POINTER(SolveStats)]),

@@ -17,6 +17,8 @@
 // divergence and gradient.
 #pragma once
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 #include <stdint.h>
 
 namespace aether {
@@ -25,6 +27,25 @@ namespace {  // kernels and device helpers: private to each translation unit
 constexpr int kBX = 64;  // one wavefront spans 64 consecutive x cells
 constexpr int kBY = 4;   // four wavefronts per workgroup
 constexpr int kThreads = kBX * kBY;
+
+// Tiling of the z-marching reduction launches (divergence, telemetry, max
+// residual): a workgroup covers a kBX x kBY column and marches zchunk planes,
+// with the chunk chosen so that a grid gives ~1024 partial sums (more when the
+// columns alone exceed that).  Partial slot = (z chunk * gy + y) * gx + x.
+struct RedTiling {
+    int gx, gy, nch, zchunk, nb;
+};
+inline RedTiling red_tiling(int nx, int ny, int nz) {
+    RedTiling t;
+    t.gx = (nx + kBX - 1) / kBX;
+    t.gy = (ny + kBY - 1) / kBY;
+    const int cols = t.gx * t.gy;
+    int nch = std::max(1, std::min(nz, (1024 + cols - 1) / cols));
+    t.zchunk = (nz + nch - 1) / nch;
+    t.nch = (nz + t.zchunk - 1) / t.zchunk;
+    t.nb = cols * t.nch;
+    return t;
+}
 
 struct Dims {
     int nx, ny, nz;

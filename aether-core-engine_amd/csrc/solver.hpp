@@ -28,11 +28,10 @@ class Solver final : public SolverBase {
         // reduction launches: each workgroup marches a chunk of z planes so
         // that a solve sees ~1k partial sums at most
         const int cols = gx * gy;
-        int nch = std::max(1, std::min(nz, (1024 + cols - 1) / cols));
-        zchunk_ = (nz + nch - 1) / nch;
-        nch = (nz + zchunk_ - 1) / zchunk_;
-        red_grid_ = dim3(gx, gy, nch);
-        nb_ = cols * nch;
+        const RedTiling rt = red_tiling(nx, ny, nz);
+        zchunk_ = rt.zchunk;
+        red_grid_ = dim3(rt.gx, rt.gy, rt.nch);
+        nb_ = rt.nb;
         // CG tiling: ZU z cells per thread; the smallest ZU giving <= 1024
         // workgroups (each CG launch re-reads that many partial sums; at
         // 128^3 ZU = 8 measured 7% faster than ZU = 4)

@@ -159,6 +159,25 @@ int aether_fluid_solve_poisson_conjugate_gradient(aether_fluid_t s, int max_iter
 int aether_fluid_solve_poisson_gauss_seidel(aether_fluid_t s, int sweeps);                              /* fluid.rs:397 */
 int aether_fluid_inject_sources(aether_fluid_t s, double time_sec, double dt);          /* fluid.rs:532 */
 int aether_fluid_compute_telemetry(aether_fluid_t s, double time_sec, aether_fluid_telemetry *out); /* fluid.rs:613 */
+/* No counterpart in the reference: a variant of aether_fluid_compute_telemetry
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  It reduces
+ * the telemetry (fluid.rs:613-672) of fields the caller holds on the device:
+ * u, v, w, density are nx*ny*nz arrays of float (AETHER_FP32) or double
+ * (AETHER_FP64) in the reference layout; h <= 0 selects 1/max(nx,ny,nz).
+ * out_dev: 8 doubles on the device {time, ke, enstrophy, peak |w|, mass,
+ * com x, y, z}; stream: a hipStream_t (NULL: the null stream).  It enqueues
+ * two kernels and returns without synchronising; it calls no allocation
+ * routine.  The partial sums go to a library-owned device array of 224 KiB
+ * (4096 slots) that each GPU holds until the process ends: grids needing
+ * more slots (more than 4096 columns of ceil(nx/64) x ceil(ny/4)) are
+ * rejected with AETHER_EINVAL.  Calls that overlap on one GPU (different
+ * streams or threads) share that array and race.  The frame equals
+ * aether_fluid_compute_telemetry's bit for bit. */
+int aether_fluid_compute_telemetry_noalloc(int precision, int nx, int ny, int nz, double h,
+                                           const void *u, const void *v, const void *w,
+                                           const void *density, double time_sec, double *out_dev,
+                                           void *stream);
 /* fluid.rs:94 sample_trilinear, for npts points (x,y,z interleaved). */
 int aether_fluid_sample_trilinear(aether_fluid_t s, int field, const double *xyz, size_t npts,
                                   double *out);

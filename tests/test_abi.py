@@ -85,3 +85,20 @@ def test_tape_backward_before_forward_raises():
                            target=np.zeros(8))
     with pytest.raises(RuntimeError):
         tape.backward(1.0)
+
+
+def test_noalloc_telemetry_rejects_bad_arguments_before_device_use():
+    """The no-allocation telemetry entry point checks precision, dimensions,
+    pointers and its scratch limit (4096 reduction partials) on the host."""
+    import aether_fluid as af
+    lib = af.lib()
+    p = C.c_void_p(16)  # never dereferenced: every call below is rejected first
+    args = lambda prec, nx, ny, nz, out=p: (prec, nx, ny, nz, 0.0, p, p, p, p, 0.0, out, None)
+    einval = af._lib.AETHER_EINVAL
+    assert lib.aether_fluid_compute_telemetry_noalloc(*args(2, 16, 16, 16)) == einval
+    assert b"precision" in lib.aether_fluid_last_error()
+    assert lib.aether_fluid_compute_telemetry_noalloc(*args(0, 5, 16, 16)) == einval
+    assert lib.aether_fluid_compute_telemetry_noalloc(*args(0, 16, 16, 16, None)) == einval
+    # ceil(4160/64) * ceil(256/4) = 65 * 64 = 4160 columns, one partial each
+    assert lib.aether_fluid_compute_telemetry_noalloc(*args(0, 4160, 256, 6)) == einval
+    assert b"4096" in lib.aether_fluid_last_error()
