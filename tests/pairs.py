@@ -62,3 +62,55 @@ def rel_l2(a, b):
 
 def compare_fields(gpu, orc, names=FIELD_NAMES):
     return {n: rel_l2(gpu.download(n), orc.field(n)) for n in names}
+
+
+def advection_switch_cells(u, v, w, dims, scale, delta=1e-6):
+    """Cells whose MacCormack result (src/fluid.rs:170-234) can jump when a
+    trace coordinate moves by `delta`, found from the (shared) velocity:
+
+    * a back-trace i - scale*u (and j, k), u != 0, whose clamped value lies
+      within delta of an integer: floor() picks the other corner cell for
+      the forward sample and the 8-corner extrema (fluid.rs:99-101,
+      137-139);
+    * a back-trace within delta of the near-wall thresholds 1.5 and
+      n - 2.5, which switch between the corrected and the forward value
+      (fluid.rs:196-201);
+    * a forward trace i + scale*u, u != 0 (the backward semi-Lagrangian
+      pass, fluid.rs:184), within delta of an integer;
+    * a cell whose forward trace has, among its 8 corners, a cell with a
+      switching back-trace: the backward pass reads that cell's forward
+      value.
+
+    Returns (mask of all of these, shaped (nz, ny, nx), and the number of
+    primary switch cells, the first three kinds).  Boundary cells are never
+    traced."""
+    nx, ny, nz = dims
+    shape = (nz, ny, nx)
+    u, v, w = (np.asarray(a, dtype=np.float64).reshape(shape) for a in (u, v, w))
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    interior = ((i >= 1) & (i <= nx - 2) & (j >= 1) & (j <= ny - 2) & (k >= 1) & (k <= nz - 2))
+    back = np.zeros(shape, dtype=bool)
+    fwd = np.zeros(shape, dtype=bool)
+    corner0 = []
+    for idx, vel, n in ((i, u, nx), (j, v, ny), (k, w, nz)):
+        for sign, out in ((-1.0, back), (1.0, fwd)):
+            x = idx + sign * scale * vel
+            xc = np.clip(x, 0.5, n - 1.500001)
+            # a zero velocity component traces to i itself in any precision:
+            # no switch there (the wall masks leave many exact zeros)
+            out |= (np.abs(xc - np.round(xc)) < delta) & (vel != 0.0)
+            if sign < 0:
+                out |= (np.abs(x - 1.5) < delta) | (np.abs(x - (n - 2.5)) < delta)
+            else:
+                corner0.append(np.floor(xc).astype(np.int64))
+    back &= interior
+    fwd &= interior
+    i0, j0, k0 = corner0
+    reads = np.zeros(shape, dtype=bool)
+    if back.any():
+        for dk in (0, 1):
+            for dj in (0, 1):
+                for di in (0, 1):
+                    reads |= back[k0 + dk, j0 + dj, i0 + di]
+        reads &= interior
+    return back | fwd | reads, int((back | fwd).sum())

@@ -5,8 +5,11 @@ Bars:
   confinement/buoyancy, gradient, RBGS): bit-exact against the oracle.
 * FP64 stages with reductions or device libm (CG, jets' exp, telemetry):
   relative error <= 1e-11 (fixed-order tree sums vs sequential sums).
-* FP32 instantiation: per-field relative L2 <= 1e-5 after one step of the
-  plain-smoke workloads, CG iteration count within +-1 of the oracle's.
+* FP32 instantiation, same FP32 inputs on both sides: per-field relative L2
+  <= 1e-5 after every stage of the 128^3 step, off the MacCormack floor and
+  near-wall switch cells that the test finds from the velocity; CG
+  iteration count within +-1 of the oracle's.  A whole unsynchronised step:
+  1e-5 at 48^3, 1e-4 at 128^3 (one switch cell spread by the projection).
 """
 import numpy as np
 import pytest
@@ -373,26 +376,14 @@ def test_fp32_plain_smoke_one_step(solver):
         assert abs(gpu.last_solve_stats()["iters"] - orc.stats()["iters"]) <= 1
 
 
-@pytest.mark.parametrize("solver", ["cg", "rbgs"])
-def test_fp32_plain_128_one_step_vs_oracle(solver):
-    """The headline workload at its real size: one FP32 step of the seeded
-    128^3 recipe (CG rel tol 1e-5, cap 200, reached; or RB-SOR 60 sweeps)
-    against one f64 oracle step (OpenMP build) from the same state.
-
-    Bound 1e-4 per field, not the 1e-5 of the 48^3 test, for one cell. Of
-    the 2,097,152 cells, (42, 37, 19) back-traces its w to z = 17.999999921
-    in f64, which is 18.0 in FP32. floor() then picks the next cell, whose
-    8 corners give the MacCormack clamp (src/fluid.rs:131-150, 196-208)
-    other extrema, and that cell's w moves by 0.032. The projection spreads
-    it: w 4.8e-5, u and v 2.0e-5. Every other cell agrees to ~1e-6. It is
-    the same with CG or RB-SOR and in every CG layout, and the FP64
-    instantiation matches the oracle to 1e-15 at this size."""
-    _announce(f"fp32 plain 128^3 one step vs oracle ({solver})")
+def _plain_128_fp32_pair(solver):
+    """The headline workload's FP32 solver and an f64 oracle (OpenMP build)
+    with the same knobs, both holding the seeded state as the GPU stores it
+    (FP32-rounded), so the two start from identical inputs."""
     import aether_fluid as af
     from aether_fluid.scenarios import SMOKE_CONFIGS, configure, seeded_config_state
     from oracle.oracle import OracleFluid
     cfg = SMOKE_CONFIGS["plain_128"]
-    state = seeded_config_state("plain_128")
     gpu = af.FluidSolver3D(*cfg["dims"], h=cfg["h"], precision="fp32")
     configure(gpu, "plain_128", solver)
     orc = OracleFluid(*cfg["dims"], h=cfg["h"], omp=True)
@@ -402,7 +393,103 @@ def test_fp32_plain_128_one_step_vs_oracle(solver):
     else:
         orc.solver_kind, orc.solver_iters, orc.sor_omega = 1, cfg["rbgs_sweeps"], cfg["sor_omega"]
     orc.set_source(1, cfg["source"]["center"], cfg["source"]["radius"], 1.0, 1.0)
-    load_state(gpu, orc, state)
+    for n, a in seeded_config_state("plain_128").items():
+        gpu.upload(n, a)
+    _sync_oracle(gpu, orc)
+    return cfg, gpu, orc
+
+
+_STATE = ("u", "v", "w", "density", "temperature", "pressure")
+
+
+def _sync_oracle(gpu, orc):
+    """Give the oracle the GPU's current state (FP32 values, exact in f64)."""
+    for n in _STATE:
+        orc.field(n)[...] = gpu.download(n).reshape(orc.nz, orc.ny, orc.nx)
+
+
+@pytest.mark.parametrize("solver", ["cg", "rbgs"])
+def test_fp32_plain_128_stages_vs_oracle_same_inputs(solver):
+    """The 128^3 FP32 step, stage by stage, against the f64 oracle fed the
+    GPU's own FP32 state before every stage (src/fluid.rs:598-611: advect
+    u, v, w; sphere source; confinement and buoyancy; projection; advect
+    density, temperature).  Bar: per-field relative L2 <= 1e-5 after every
+    stage.  The MacCormack stages are compared off the cells whose trace
+    lies within 1e-6 of a floor or near-wall switch (found from the shared
+    velocity by pairs.advection_switch_cells): there a last-bit difference
+    of the trace picks another corner cell, a discontinuity of the
+    reference's scheme, not an error.  Every cell that differs by more
+    than 1e-5 of the field's maximum must be one of those, and they must be
+    few (primary switches < 1e-4 of the cells, with the cells that read
+    them < 1e-3)."""
+    from pairs import advection_switch_cells
+    _announce(f"fp32 128^3 stages vs oracle on FP32 inputs ({solver})")
+    cfg, gpu, orc = _plain_128_fp32_pair(solver)
+    dt, dims, t = cfg["dt"], cfg["dims"], 0.0
+    scale = dt / cfg["h"]
+    ncell = int(np.prod(dims))
+    report = {}
+
+    def check(stage, names, mask=None):
+        for n in names:
+            a, b = gpu.download(n), orc.field(n).ravel()
+            diff = np.abs(a - b)
+            keep = np.ones(ncell, dtype=bool) if mask is None else ~mask.ravel()
+            err = np.linalg.norm((a - b)[keep]) / max(np.linalg.norm(b[keep]), 1e-300)
+            big = diff > 1e-5 * max(np.abs(b).max(), 1e-300)
+            report[f"{stage}:{n}"] = (err, int(big.sum()), int((~keep).sum()))
+            print(f"{stage:12s} {n:11s} rel L2 {err:.3e}  big {int(big.sum())}  "
+                  f"excluded {int((~keep).sum())}", flush=True)
+            assert err <= 1e-5, (stage, n, err)
+            assert not (big & keep).any(), (stage, n, np.flatnonzero(big & keep)[:8])
+
+    def switches():
+        vel = [orc.field(n).ravel().copy() for n in ("u", "v", "w")]
+        m, primary = advection_switch_cells(*vel, dims, scale)
+        print(f"switch cells: {primary} primary, {int(m.sum())} excluded", flush=True)
+        assert primary < 1e-4 * ncell and m.sum() < 1e-3 * ncell, (primary, m.sum())
+        return m
+
+    for kind, name, diss in ((0, "u", 0.996), (1, "v", 0.996), (2, "w", 0.996)):
+        m = switches()
+        gpu.advect_maccormack_in_place(kind, dt, diss)
+        orc.advect_maccormack_in_place(kind, dt, diss)
+        check(f"advect {name}", (name,), m)
+        _sync_oracle(gpu, orc)
+    gpu.inject_sources(t, dt)
+    orc.inject_sources(t, dt)
+    check("source", ("density", "temperature"))
+    _sync_oracle(gpu, orc)
+    gpu.apply_buoyancy_and_vorticity_confinement(dt)
+    orc.apply_buoyancy_and_vorticity_confinement(dt)
+    check("confine", ("u", "v", "w"))
+    _sync_oracle(gpu, orc)
+    gpu.project()
+    orc.project()
+    check("project", ("u", "v", "w"))
+    if solver == "cg":
+        assert abs(gpu.last_solve_stats()["iters"] - orc.stats()["iters"]) <= 1
+    _sync_oracle(gpu, orc)
+    for kind, name, diss in ((3, "density", 0.996), (4, "temperature", 0.985)):
+        m = switches()
+        gpu.advect_maccormack_in_place(kind, dt, diss)
+        orc.advect_maccormack_in_place(kind, dt, diss)
+        check(f"advect {name}", (name,), m)
+        _sync_oracle(gpu, orc)
+
+
+@pytest.mark.parametrize("solver", ["cg", "rbgs"])
+def test_fp32_plain_128_one_step_vs_oracle(solver):
+    """One whole FP32 step of the 128^3 workload against one f64 oracle step
+    from the same FP32-rounded state, unsynchronised between stages.
+
+    The same-input stage test above holds every stage to 1e-5 off the
+    floor and near-wall switch cells.  Over a whole step a switch cell's
+    difference (up to a few 1e-2 in that cell) passes through the
+    projection, which spreads it over the grid, so the whole-step fields
+    are bounded at 1e-4 per field; the telemetry at rtol 1e-4."""
+    _announce(f"fp32 plain 128^3 one step vs oracle on FP32 inputs ({solver})")
+    cfg, gpu, orc = _plain_128_fp32_pair(solver)
     tg = gpu.step(cfg["dt"], 0.0).as_array()
     to = orc.step(cfg["dt"], 0.0)
     errs = compare_fields(gpu, orc, ("u", "v", "w", "density", "temperature"))
