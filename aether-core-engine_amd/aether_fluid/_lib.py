@@ -156,6 +156,9 @@ SIGNATURES = {
                                                 C.POINTER(C.c_int)]),
     "aether_fluid_rollout": (C.c_int, [_vp, C.POINTER(RolloutDesc), _pd, C.c_double, _pd, _pd,
                                        _pd, _pd]),
+    "aether_fluid_rollout_forward": (C.c_int, [_vp, C.POINTER(RolloutDesc), _pd, C.c_double,
+                                               _pd]),
+    "aether_fluid_rollout_backward": (C.c_int, [_vp, _pd, _pd, _pd, _pd]),
     "aether_rng_seed": (C.c_int, [C.c_uint64, _pu64]),
     "aether_rng_fill_range": (C.c_int, [_pu64, C.c_double, C.c_double, _pd, C.c_size_t]),
     "aether_fluid_set_kernel_timing": (C.c_int, [_vp, C.c_int]),

@@ -265,13 +265,8 @@ class FluidSolver3D:
         Returns (loss, dloss/dcontrols, dloss/dbeta); the gradients are None
         when grad is False."""
         self._sync_params()
-        d = L.RolloutDesc()
-        d.nsteps, d.checkpoint_every, d.dt = nsteps, checkpoint_every, dt
-        d.patch_lo = (C.c_int32 * 3)(*patch_lo)
-        d.patch_hi = (C.c_int32 * 3)(*patch_hi)
-        d.adjoint_cg_iters = self.solver_iters if adjoint_cg_iters is None else adjoint_cg_iters
-        d.adjoint_cg_rel_tol = (self.cg_rel_tol if adjoint_cg_rel_tol is None
-                                else adjoint_cg_rel_tol)
+        d = self._rollout_desc(nsteps, dt, patch_lo, patch_hi, checkpoint_every,
+                               adjoint_cg_iters, adjoint_cg_rel_tol)
         c = np.ascontiguousarray(np.asarray(controls, dtype=np.float64).reshape(-1))
         t = np.ascontiguousarray(np.asarray(target, dtype=np.float64).reshape(-1))
         loss = C.c_double()
@@ -281,6 +276,54 @@ class FluidSolver3D:
             self._h, C.byref(d), _dptr(c), float(beta), _dptr(t), C.byref(loss),
             _dptr(gc) if grad else None, C.byref(gb) if grad else None))
         return loss.value, gc, (gb.value if grad else None)
+
+    STATE_FIELDS = ("u", "v", "w", "density", "temperature")
+
+    def _rollout_desc(self, nsteps, dt, patch_lo, patch_hi, checkpoint_every,
+                      adjoint_cg_iters=None, adjoint_cg_rel_tol=None):
+        d = L.RolloutDesc()
+        d.nsteps, d.checkpoint_every, d.dt = nsteps, checkpoint_every, dt
+        d.patch_lo = (C.c_int32 * 3)(*patch_lo)
+        d.patch_hi = (C.c_int32 * 3)(*patch_hi)
+        d.adjoint_cg_iters = self.solver_iters if adjoint_cg_iters is None else adjoint_cg_iters
+        d.adjoint_cg_rel_tol = (self.cg_rel_tol if adjoint_cg_rel_tol is None
+                                else adjoint_cg_rel_tol)
+        return d
+
+    def rollout_forward(self, nsteps: int, dt: float, patch_lo, patch_hi, controls, beta: float,
+                        checkpoint_every: int = 8, adjoint_cg_iters: int | None = None,
+                        adjoint_cg_rel_tol: float | None = None) -> dict:
+        """Forward sweep of a differentiable rollout from the current state
+        (left unchanged), recording the tape for rollout_backward.  Returns
+        the final state {u, v, w, density, temperature} (flat f64)."""
+        self._sync_params()
+        d = self._rollout_desc(nsteps, dt, patch_lo, patch_hi, checkpoint_every,
+                               adjoint_cg_iters, adjoint_cg_rel_tol)
+        c = np.ascontiguousarray(np.asarray(controls, dtype=np.float64).reshape(-1))
+        out = np.empty(5 * self.n, dtype=np.float64)
+        L.check(self._lib.aether_fluid_rollout_forward(self._h, C.byref(d), _dptr(c),
+                                                       float(beta), _dptr(out)))
+        self._tape_nctrl = c.size
+        return {name: out[a * self.n:(a + 1) * self.n] for a, name in enumerate(self.STATE_FIELDS)}
+
+    def rollout_backward(self, lam_final: dict | None):
+        """Vector-Jacobian product of the recorded rollout: lam_final holds
+        the adjoints of the final fields (missing names are zero).  Returns
+        (d/dcontrols, d/dbeta, {field: d/d initial field})."""
+        lam = None
+        if lam_final is not None:
+            lam = np.zeros(5 * self.n, dtype=np.float64)
+            for a, name in enumerate(self.STATE_FIELDS):
+                if name in lam_final and lam_final[name] is not None:
+                    lam[a * self.n:(a + 1) * self.n] = np.asarray(lam_final[name],
+                                                                  dtype=np.float64).reshape(-1)
+        gc = np.zeros(getattr(self, "_tape_nctrl", 0), dtype=np.float64)
+        gb = C.c_double()
+        gi = np.empty(5 * self.n, dtype=np.float64)
+        L.check(self._lib.aether_fluid_rollout_backward(
+            self._h, _dptr(lam) if lam is not None else None, _dptr(gc), C.byref(gb), _dptr(gi)))
+        return gc, gb.value, {name: gi[a * self.n:(a + 1) * self.n]
+                              for a, name in enumerate(self.STATE_FIELDS)}
 
     _TUNE_CANDIDATES = (("x4", 2), ("x4", 4), ("scalar", 4), ("scalar", 8), ("scalar", 16),
                         ("fused", 4))

@@ -9,7 +9,9 @@
 //     arg-min / arg-max corner), the near-wall branch and floor() cell
 //     choice are piecewise constant, and the trilinear weights are
 //     differentiated with respect to the back-traced position, which gives
-//     the velocity adjoint.  Gathers become scatters (atomic adds).
+//     the velocity adjoint.  Gathers become scatters, accumulated in 64-bit
+//     fixed point (below), so the result does not depend on the order in
+//     which threads add.
 //   confinement + buoyancy (src/fluid.rs:256-330): three gather passes
 //     (N x w and the normalisation of grad|w|, then the transpose of the
 //     central differences of |w| and of the curl); deterministic.
@@ -23,9 +25,61 @@
 namespace aether {
 namespace {
 
+// Fixed-point scatter accumulation.  A contribution v is rounded to the
+// nearest multiple of 2^-F and added as a 64-bit integer.  Integer addition
+// is associative (wrap-around included), so a cell's sum is the same
+// whatever order the threads add in: the adjoint is reproducible bit for
+// bit.  F is set per scatter from a bound B on the sum of the absolute
+// values of all contributions of that scatter (so on every cell's sum and
+// every partial sum): F = 61 - ceil(log2 B), which keeps |sum| * 2^F below
+// 2^61.  Each contribution is then exact to B * 2^-62 in absolute terms.
+struct FixScale {
+    double up;    // 2^F
+    double down;  // 2^-F
+};
+typedef unsigned long long fixacc_t;
+
+__device__ __forceinline__ long long fix_q(double v, double up) { return __double2ll_rn(v * up); }
+
+__device__ __forceinline__ void fix_add(fixacc_t *p, double v, double up) {
+    const long long q = fix_q(v, up);
+    if (q) atomicAdd(p, (fixacc_t)q);
+}
+
+__device__ __forceinline__ double fix_value(fixacc_t a, double down) {
+    return (double)(long long)a * down;
+}
+
+// Sum of |x| over n values (FP64 partials, one per workgroup).
 template <typename T>
-__device__ __forceinline__ void atomic_acc(T *p, T v) {
-    unsafeAtomicAdd(p, v);
+__global__ __launch_bounds__(kThreads) void k_abs_sum(const T *__restrict__ x, long n,
+                                                      double *__restrict__ part) {
+    __shared__ double lds[8];
+    double acc[1] = {0.0};
+    for (long a = (long)blockIdx.x * kThreads + tid_lin(); a < n; a += (long)gridDim.x * kThreads)
+        acc[0] += fabs((double)x[a]);
+    block_sum<1>(acc, lds);
+    if (tid_lin() == 0) part[blockIdx.x] = acc[0];
+}
+
+// The scale of a scatter whose contributions sum in absolute value to at
+// most factor * (sum of the nb partials).
+__global__ __launch_bounds__(kThreads) void k_fix_scale(const double *__restrict__ part, int nb,
+                                                        double factor, FixScale *out) {
+    __shared__ double lds[8];
+    const double s = sum_partials(part, nb, lds);
+    if (tid_lin() == 0) {
+        const double b = factor * s * (1.0 + 1e-6);
+        int f = 0;
+        if (b > 0.0 && b < 1e300) {
+            int e = 0;
+            frexp(b, &e);  // b < 2^e
+            f = 61 - e;
+        }
+        f = max(-1000, min(1000, f));
+        out->up = ldexp(1.0, f);
+        out->down = ldexp(1.0, -f);
+    }
 }
 
 // axis_loc plus the flag "position clamped" (zero derivative along the axis).
@@ -104,28 +158,37 @@ __device__ __forceinline__ void grad_pos(const Loc<T> &L, const T (&c)[8], T &gx
     if (L.cz) gz = T(0);
 }
 
-// Reverse of the MacCormack correction (src/fluid.rs:186-228) for one
-// field: lam_out is the adjoint of the advected field.  Accumulates into
-// lam_src (the advected field before the step), lam_fwd (the forward SL
-// result) and lam_U/V/W (the velocity used for the trace).
-template <typename T>
-__global__ __launch_bounds__(kThreads) void k_adj_mac_correct(
-    Dims g, T s, T diss, int clamp0, const T *__restrict__ src, const T *__restrict__ fwd,
-    const T *__restrict__ U, const T *__restrict__ V, const T *__restrict__ W,
-    const T *__restrict__ lam_out, T *lam_src, T *lam_fwd, T *lam_U, T *lam_V, T *lam_W) {
-    const int i = blockIdx.x * kBX + threadIdx.x;
-    const int j = blockIdx.y * kBY + threadIdx.y;
-    const int k = blockIdx.z;
-    if (i >= g.nx || j >= g.ny || !is_interior(g, i, j, k)) return;
-    const int id = (k * g.ny + j) * g.nx + i;
-    const T lo_adj = lam_out[id];
+// Scatter targets of the reverse advection kernels.  GlobalAcc adds straight
+// to a fixed-point array in global memory; ScatterWindow (below) first
+// accumulates in LDS.  Both take the cell's index and coordinates.
+struct GlobalAcc {
+    fixacc_t *p;
+    double up;
+    __device__ __forceinline__ void add(int gid, int, int, int, double v) const {
+        fix_add(p + gid, v, up);
+    }
+};
+
+// Reverse of the MacCormack correction (src/fluid.rs:186-228) at one cell:
+// lo_adj is the adjoint of the advected field there.  Scatters to the
+// adjoints of the field before the step (asrc) and of the forward SL result
+// (afwd), and returns the own-cell adjoint of the trace velocity in
+// vu, vv, vw.
+template <typename T, typename AS, typename AF>
+__device__ __forceinline__ void adj_mac_cell(const Dims &g, int i, int j, int k, int id, T s,
+                                             T diss, int clamp0, const T *__restrict__ src,
+                                             const T *__restrict__ fwd, const T *__restrict__ U,
+                                             const T *__restrict__ V, const T *__restrict__ W,
+                                             T lo_adj, const AS &asrc, AF &afwd, T &vu, T &vv,
+                                             T &vw) {
+    vu = vv = vw = T(0);
     if (lo_adj == T(0)) return;
     const T su = s * U[id], sv = s * V[id], sw = s * W[id];
     const Loc<T> Lb = locate<T>(g, i, j, k, su, sv, sw);
     const T f = fwd[id];
     if (Lb.nw) {
         if (clamp0 && f * diss < T(0)) return;  // max(0) selected 0
-        atomic_acc(lam_fwd + id, diss * lo_adj);
+        afwd.add(id, i, j, k, (double)(diss * lo_adj));
         return;
     }
     const Loc<T> Lf = locate<T>(g, i, j, k, -su, -sv, -sw);
@@ -149,40 +212,39 @@ __global__ __launch_bounds__(kThreads) void k_adj_mac_correct(
     if (clamp0 && val * diss < T(0)) return;
     const T gadj = diss * lo_adj;
     if (mc < lo) {
-        atomic_acc(lam_src + ids[alo], gadj);
+        asrc.add(ids[alo], Lb.i0 + (alo & 1), Lb.j0 + ((alo >> 1) & 1), Lb.k0 + (alo >> 2),
+                 (double)gadj);
     } else if (mc > hi) {
-        atomic_acc(lam_src + ids[ahi], gadj);
+        asrc.add(ids[ahi], Lb.i0 + (ahi & 1), Lb.j0 + ((ahi >> 1) & 1), Lb.k0 + (ahi >> 2),
+                 (double)gadj);
     } else {
-        atomic_acc(lam_fwd + id, gadj);
-        atomic_acc(lam_src + id, T(0.5) * gadj);
+        afwd.add(id, i, j, k, (double)gadj);
+        asrc.add(id, i, j, k, (double)(T(0.5) * gadj));
         T w[8];
         weights<T>(Lf, w);
         const T hb = T(-0.5) * gadj;
 #pragma unroll
-        for (int a = 0; a < 8; ++a) atomic_acc(lam_fwd + idf[a], hb * w[a]);
+        for (int a = 0; a < 8; ++a)
+            afwd.add(idf[a], Lf.i0 + (a & 1), Lf.j0 + ((a >> 1) & 1), Lf.k0 + (a >> 2),
+                     (double)(hb * w[a]));
         // bwd position = cell + s * velocity
         T gx, gy, gz;
         grad_pos<T>(Lf, cf, gx, gy, gz);
-        atomic_acc(lam_U + id, hb * gx * s);
-        atomic_acc(lam_V + id, hb * gy * s);
-        atomic_acc(lam_W + id, hb * gz * s);
+        vu = hb * gx * s;
+        vv = hb * gy * s;
+        vw = hb * gz * s;
     }
 }
 
 // Reverse of the forward semi-Lagrangian pass fwd = SL(src, +dt)
-// (src/fluid.rs:153-167): scatter lam_fwd to the 8 back-trace corners and
-// to the cell's velocity (position = cell - s * velocity).
-template <typename T>
-__global__ __launch_bounds__(kThreads) void k_adj_sl_forward(
-    Dims g, T s, const T *__restrict__ src, const T *__restrict__ U, const T *__restrict__ V,
-    const T *__restrict__ W, const T *__restrict__ lam_fwd, T *lam_src, T *lam_U, T *lam_V,
-    T *lam_W) {
-    const int i = blockIdx.x * kBX + threadIdx.x;
-    const int j = blockIdx.y * kBY + threadIdx.y;
-    const int k = blockIdx.z;
-    if (i >= g.nx || j >= g.ny || !is_interior(g, i, j, k)) return;
-    const int id = (k * g.ny + j) * g.nx + i;
-    const T lf = lam_fwd[id];
+// (src/fluid.rs:153-167) at one cell: lf is the adjoint of fwd there.
+// Scatters to the 8 back-trace corners of src and adds the own-cell
+// velocity adjoint (position = cell - s * velocity) to vu, vv, vw.
+template <typename T, typename AS>
+__device__ __forceinline__ void adj_sl_cell(const Dims &g, int i, int j, int k, int id, T s,
+                                            const T *__restrict__ src, const T *__restrict__ U,
+                                            const T *__restrict__ V, const T *__restrict__ W,
+                                            T lf, AS &asrc, T &vu, T &vv, T &vw) {
     if (lf == T(0)) return;
     const Loc<T> L = locate<T>(g, i, j, k, s * U[id], s * V[id], s * W[id]);
     T c[8], w[8];
@@ -190,12 +252,62 @@ __global__ __launch_bounds__(kThreads) void k_adj_sl_forward(
     corners<T>(g, src, L, c, ids);
     weights<T>(L, w);
 #pragma unroll
-    for (int a = 0; a < 8; ++a) atomic_acc(lam_src + ids[a], lf * w[a]);
+    for (int a = 0; a < 8; ++a)
+        asrc.add(ids[a], L.i0 + (a & 1), L.j0 + ((a >> 1) & 1), L.k0 + (a >> 2),
+                 (double)(lf * w[a]));
     T gx, gy, gz;
     grad_pos<T>(L, c, gx, gy, gz);
-    atomic_acc(lam_U + id, -lf * gx * s);
-    atomic_acc(lam_V + id, -lf * gy * s);
-    atomic_acc(lam_W + id, -lf * gz * s);
+    vu += -lf * gx * s;
+    vv += -lf * gy * s;
+    vw += -lf * gz * s;
+}
+
+// One thread per cell.  acc_src / acc_fwd: fixed-point adjoints of the
+// field before the step and of the forward SL result (zeroed by the
+// caller); tU/tV/tW: the own-cell velocity adjoint, written at every cell.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_adj_mac_correct(
+    Dims g, T s, T diss, int clamp0, const T *__restrict__ src, const T *__restrict__ fwd,
+    const T *__restrict__ U, const T *__restrict__ V, const T *__restrict__ W,
+    const T *__restrict__ lam_out, const FixScale *sc, fixacc_t *acc_src, fixacc_t *acc_fwd,
+    T *__restrict__ tU, T *__restrict__ tV, T *__restrict__ tW) {
+    const int i = blockIdx.x * kBX + threadIdx.x;
+    const int j = blockIdx.y * kBY + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= g.nx || j >= g.ny) return;
+    const int id = (k * g.ny + j) * g.nx + i;
+    T vu = T(0), vv = T(0), vw = T(0);
+    if (is_interior(g, i, j, k)) {
+        const GlobalAcc as{acc_src, sc->up};
+        GlobalAcc af{acc_fwd, sc->up};
+        adj_mac_cell<T>(g, i, j, k, id, s, diss, clamp0, src, fwd, U, V, W, lam_out[id], as, af,
+                        vu, vv, vw);
+    }
+    tU[id] = vu;
+    tV[id] = vv;
+    tW[id] = vw;
+}
+
+// One thread per cell: reads the forward adjoint from acc_fwd, scatters to
+// acc_src and adds to the own-cell velocity adjoint tU/tV/tW.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_adj_sl_forward(
+    Dims g, T s, const T *__restrict__ src, const T *__restrict__ U, const T *__restrict__ V,
+    const T *__restrict__ W, const FixScale *sc, const fixacc_t *__restrict__ acc_fwd,
+    fixacc_t *acc_src, T *__restrict__ tU, T *__restrict__ tV, T *__restrict__ tW) {
+    const int i = blockIdx.x * kBX + threadIdx.x;
+    const int j = blockIdx.y * kBY + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= g.nx || j >= g.ny || !is_interior(g, i, j, k)) return;
+    const int id = (k * g.ny + j) * g.nx + i;
+    const T lf = (T)fix_value(acc_fwd[id], sc->down);
+    if (lf == T(0)) return;
+    GlobalAcc as{acc_src, sc->up};
+    T vu = tU[id], vv = tV[id], vw = tW[id];
+    adj_sl_cell<T>(g, i, j, k, id, s, src, U, V, W, lf, as, vu, vv, vw);
+    tU[id] = vu;
+    tV[id] = vv;
+    tW[id] = vw;
 }
 
 // ---------------------------------------------------------------------------
@@ -206,151 +318,130 @@ __global__ __launch_bounds__(kThreads) void k_adj_sl_forward(
 // window cell, consecutive in x.  A corner outside the window (a trace longer
 // than H cells) goes straight to global memory.  With traces of up to H
 // cells this replaces 8 scattered global atomics per cell by ~4 row-coalesced
-// ones (window / tile = 39*15*15 / 2048 for H = 3).
+// ones (window / tile = 39*15*15 / 2048 for H = 3).  Window and global
+// memory hold the same fixed-point integers, so where a contribution is
+// summed first does not change the result.
 // ---------------------------------------------------------------------------
 constexpr int WTX = 32, WTY = 8, WTZ = 8, WH = 3;
 constexpr int WWX = WTX + 2 * WH + 1, WWY = WTY + 2 * WH + 1, WWZ = WTZ + 2 * WH + 1;
 constexpr int WN = WWX * WWY * WWZ;
 static_assert(WTX * WTY == kThreads, "one thread per column of the tile");
 
-template <typename T>
 struct ScatterWindow {
-    T *lds;
+    fixacc_t *lds;
+    fixacc_t *global;
+    double up;
     int x0, y0, z0;  // global cell of window entry 0
-    __device__ __forceinline__ void init(T *mem, int tx0, int ty0, int tz0) {
+    __device__ __forceinline__ void init(fixacc_t *mem, fixacc_t *glob, double scale, int tx0,
+                                         int ty0, int tz0) {
         lds = mem;
+        global = glob;
+        up = scale;
         x0 = tx0 - WH;
         y0 = ty0 - WH;
         z0 = tz0 - WH;
-        for (int e = threadIdx.y * WTX + threadIdx.x; e < WN; e += kThreads) lds[e] = T(0);
+        for (int e = threadIdx.y * WTX + threadIdx.x; e < WN; e += kThreads) lds[e] = 0;
     }
     // cell (i, j, k) = global index gid gets v
-    __device__ __forceinline__ void add(T *global, int gid, int i, int j, int k, T v) {
+    __device__ __forceinline__ void add(int gid, int i, int j, int k, double v) {
+        const long long q = fix_q(v, up);
+        if (!q) return;
         const unsigned a = (unsigned)(i - x0), b = (unsigned)(j - y0), c = (unsigned)(k - z0);
         if (a < (unsigned)WWX && b < (unsigned)WWY && c < (unsigned)WWZ)
-            atomicAdd(lds + ((int)c * WWY + (int)b) * WWX + (int)a, v);
+            atomicAdd(lds + ((int)c * WWY + (int)b) * WWX + (int)a, (fixacc_t)q);
         else
-            atomic_acc(global + gid, v);
+            atomicAdd(global + gid, (fixacc_t)q);
     }
-    __device__ __forceinline__ void flush(const Dims &g, T *global) const {
+    __device__ __forceinline__ void flush(const Dims &g) const {
         for (int e = threadIdx.y * WTX + threadIdx.x; e < WN; e += kThreads) {
-            const T v = lds[e];
-            if (v == T(0)) continue;
+            const fixacc_t v = lds[e];
+            if (v == 0) continue;
             const int a = e % WWX, rest = e / WWX;
             const int i = x0 + a, j = y0 + rest % WWY, k = z0 + rest / WWY;
             if (i < 0 || j < 0 || k < 0 || i >= g.nx || j >= g.ny || k >= g.nz) continue;
-            atomic_acc(global + (k * g.ny + j) * g.nx + i, v);
+            atomicAdd(global + (k * g.ny + j) * g.nx + i, v);
         }
     }
 };
 
-// k_adj_mac_correct with lam_fwd accumulated in a window (grid: tiles of
+// k_adj_mac_correct with acc_fwd accumulated in a window (grid: tiles of
 // WTX x WTY x WTZ cells, block WTX x WTY, each thread a z column).
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_adj_mac_correct_w(
     Dims g, T s, T diss, int clamp0, const T *__restrict__ src, const T *__restrict__ fwd,
     const T *__restrict__ U, const T *__restrict__ V, const T *__restrict__ W,
-    const T *__restrict__ lam_out, T *lam_src, T *lam_fwd, T *lam_U, T *lam_V, T *lam_W) {
-    __shared__ T win_mem[WN];
-    ScatterWindow<T> win;
+    const T *__restrict__ lam_out, const FixScale *sc, fixacc_t *acc_src, fixacc_t *acc_fwd,
+    T *__restrict__ tU, T *__restrict__ tV, T *__restrict__ tW) {
+    __shared__ fixacc_t win_mem[WN];
+    ScatterWindow win;
+    const double up = sc->up;
     const int tx0 = blockIdx.x * WTX, ty0 = blockIdx.y * WTY, tz0 = blockIdx.z * WTZ;
-    win.init(win_mem, tx0, ty0, tz0);
+    win.init(win_mem, acc_fwd, up, tx0, ty0, tz0);
     __syncthreads();
+    const GlobalAcc as{acc_src, up};
     const int i = tx0 + threadIdx.x, j = ty0 + threadIdx.y;
     for (int kk = 0; kk < WTZ; ++kk) {
         const int k = tz0 + kk;
-        if (i >= g.nx || j >= g.ny || k >= g.nz || !is_interior(g, i, j, k)) continue;
+        if (i >= g.nx || j >= g.ny || k >= g.nz) continue;
         const int id = (k * g.ny + j) * g.nx + i;
-        const T lo_adj = lam_out[id];
-        if (lo_adj == T(0)) continue;
-        const T su = s * U[id], sv = s * V[id], sw = s * W[id];
-        const Loc<T> Lb = locate<T>(g, i, j, k, su, sv, sw);
-        const T f = fwd[id];
-        if (Lb.nw) {
-            if (clamp0 && f * diss < T(0)) continue;  // max(0) selected 0
-            win.add(lam_fwd, id, i, j, k, diss * lo_adj);
-            continue;
-        }
-        const Loc<T> Lf = locate<T>(g, i, j, k, -su, -sv, -sw);
-        T cf[8], cs[8];
-        int idf[8], ids[8];
-        corners<T>(g, fwd, Lf, cf, idf);
-        corners<T>(g, src, Lb, cs, ids);
-        const T bwd = lerp3<T>(cf[0], cf[1], cf[2], cf[3], cf[4], cf[5], cf[6], cf[7], Lf.tx,
-                               Lf.ty, Lf.tz);
-        const T mc = f + T(0.5) * (src[id] - bwd);
-        T lo = cs[0], hi = cs[0];
-        int alo = 0, ahi = 0;
-#pragma unroll
-        for (int a = 1; a < 8; ++a) {
-            if (cs[a] < lo) { lo = cs[a]; alo = a; }
-            if (cs[a] > hi) { hi = cs[a]; ahi = a; }
-        }
-        T val = mc;
-        if (val < lo) val = lo;
-        if (val > hi) val = hi;
-        if (clamp0 && val * diss < T(0)) continue;
-        const T gadj = diss * lo_adj;
-        if (mc < lo) {
-            atomic_acc(lam_src + ids[alo], gadj);
-        } else if (mc > hi) {
-            atomic_acc(lam_src + ids[ahi], gadj);
-        } else {
-            win.add(lam_fwd, id, i, j, k, gadj);
-            atomic_acc(lam_src + id, T(0.5) * gadj);
-            T w[8];
-            weights<T>(Lf, w);
-            const T hb = T(-0.5) * gadj;
-#pragma unroll
-            for (int a = 0; a < 8; ++a)
-                win.add(lam_fwd, idf[a], Lf.i0 + (a & 1), Lf.j0 + ((a >> 1) & 1), Lf.k0 + (a >> 2),
-                        hb * w[a]);
-            T gx, gy, gz;
-            grad_pos<T>(Lf, cf, gx, gy, gz);
-            atomic_acc(lam_U + id, hb * gx * s);
-            atomic_acc(lam_V + id, hb * gy * s);
-            atomic_acc(lam_W + id, hb * gz * s);
-        }
+        T vu = T(0), vv = T(0), vw = T(0);
+        if (is_interior(g, i, j, k))
+            adj_mac_cell<T>(g, i, j, k, id, s, diss, clamp0, src, fwd, U, V, W, lam_out[id], as,
+                            win, vu, vv, vw);
+        tU[id] = vu;
+        tV[id] = vv;
+        tW[id] = vw;
     }
     __syncthreads();
-    win.flush(g, lam_fwd);
+    win.flush(g);
 }
 
-// k_adj_sl_forward with lam_src accumulated in a window (same tiling).
+// k_adj_sl_forward with acc_src accumulated in a window (same tiling).
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_adj_sl_forward_w(
     Dims g, T s, const T *__restrict__ src, const T *__restrict__ U, const T *__restrict__ V,
-    const T *__restrict__ W, const T *__restrict__ lam_fwd, T *lam_src, T *lam_U, T *lam_V,
-    T *lam_W) {
-    __shared__ T win_mem[WN];
-    ScatterWindow<T> win;
+    const T *__restrict__ W, const FixScale *sc, const fixacc_t *__restrict__ acc_fwd,
+    fixacc_t *acc_src, T *__restrict__ tU, T *__restrict__ tV, T *__restrict__ tW) {
+    __shared__ fixacc_t win_mem[WN];
+    ScatterWindow win;
+    const double down = sc->down;
     const int tx0 = blockIdx.x * WTX, ty0 = blockIdx.y * WTY, tz0 = blockIdx.z * WTZ;
-    win.init(win_mem, tx0, ty0, tz0);
+    win.init(win_mem, acc_src, sc->up, tx0, ty0, tz0);
     __syncthreads();
     const int i = tx0 + threadIdx.x, j = ty0 + threadIdx.y;
     for (int kk = 0; kk < WTZ; ++kk) {
         const int k = tz0 + kk;
         if (i >= g.nx || j >= g.ny || k >= g.nz || !is_interior(g, i, j, k)) continue;
         const int id = (k * g.ny + j) * g.nx + i;
-        const T lf = lam_fwd[id];
+        const T lf = (T)fix_value(acc_fwd[id], down);
         if (lf == T(0)) continue;
-        const Loc<T> L = locate<T>(g, i, j, k, s * U[id], s * V[id], s * W[id]);
-        T c[8], w[8];
-        int ids[8];
-        corners<T>(g, src, L, c, ids);
-        weights<T>(L, w);
-#pragma unroll
-        for (int a = 0; a < 8; ++a)
-            win.add(lam_src, ids[a], L.i0 + (a & 1), L.j0 + ((a >> 1) & 1), L.k0 + (a >> 2),
-                    lf * w[a]);
-        T gx, gy, gz;
-        grad_pos<T>(L, c, gx, gy, gz);
-        atomic_acc(lam_U + id, -lf * gx * s);
-        atomic_acc(lam_V + id, -lf * gy * s);
-        atomic_acc(lam_W + id, -lf * gz * s);
+        T vu = tU[id], vv = tV[id], vw = tW[id];
+        adj_sl_cell<T>(g, i, j, k, id, s, src, U, V, W, lf, win, vu, vv, vw);
+        tU[id] = vu;
+        tV[id] = vv;
+        tW[id] = vw;
     }
     __syncthreads();
-    win.flush(g, lam_src);
+    win.flush(g);
+}
+
+// The reverse advection's results into the adjoint arrays, in a fixed
+// order per cell: lam_src += acc_src, then lam_U += tU, lam_V += tV,
+// lam_W += tW.  The arrays may alias (the w advection's source and trace
+// velocity are the same field), so nothing here is __restrict__.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_adj_flush(long n, const FixScale *sc,
+                                                        const fixacc_t *acc_src, const T *tU,
+                                                        const T *tV, const T *tW, T *lam_src,
+                                                        T *lam_U, T *lam_V, T *lam_W) {
+    const double down = sc->down;
+    for (long q = (long)blockIdx.x * kThreads + tid_lin(); q < n; q += (long)gridDim.x * kThreads) {
+        const T a = (T)fix_value(acc_src[q], down), bu = tU[q], bv = tV[q], bw = tW[q];
+        lam_src[q] = lam_src[q] + a;
+        lam_U[q] = lam_U[q] + bu;
+        lam_V[q] = lam_V[q] + bv;
+        lam_W[q] = lam_W[q] + bw;
+    }
 }
 
 // Confinement/buoyancy reverse, pass 1 (src/fluid.rs:284-329): wall masks,

@@ -456,6 +456,24 @@ int aether_fluid_rollout(aether_fluid_t s, const aether_rollout_desc *d, const d
     });
 }
 
+int aether_fluid_rollout_forward(aether_fluid_t s, const aether_rollout_desc *d,
+                                 const double *controls, double beta, double *final_state) {
+    return guarded([&] {
+        checked(s);
+        if (!d || !controls) raise(AETHER_EINVAL, "null argument");
+        s->impl->rollout_forward(*d, controls, beta, final_state);
+    });
+}
+
+int aether_fluid_rollout_backward(aether_fluid_t s, const double *lam_final,
+                                  double *grad_controls, double *grad_beta,
+                                  double *grad_initial) {
+    return guarded([&] {
+        checked(s);
+        s->impl->rollout_backward(lam_final, grad_controls, grad_beta, grad_initial);
+    });
+}
+
 int aether_rng_seed(uint64_t seed, uint64_t state[4]) {
     return guarded([&] {
         if (!state) raise(AETHER_EINVAL, "null state");

@@ -219,6 +219,10 @@ class SolverBase {
     virtual const std::vector<int> &last_run_iters() const = 0;
     virtual double rollout(const aether_rollout_desc &d, const double *ctrl, double beta,
                            const double *target, double *grad_c, double *grad_beta) = 0;
+    virtual void rollout_forward(const aether_rollout_desc &d, const double *ctrl, double beta,
+                                 double *final_state) = 0;
+    virtual void rollout_backward(const double *lam_final, double *grad_c, double *grad_beta,
+                                  double *grad_init) = 0;
     aether_fluid_params params = default_params();
     KernelTimer timer;
 };

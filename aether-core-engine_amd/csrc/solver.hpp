@@ -1078,7 +1078,13 @@ class Solver final : public SolverBase {
             sync();
             return;
         }
-        T *dst = field_ptr(field);
+        upload_to(src, field_ptr(field));
+        sync();
+    }
+
+    // n_ f64 host values into a device array of T (converted on the device);
+    // enqueued, with a later synchronisation.
+    void upload_to(const double *src, T *dst) {
         if constexpr (sizeof(T) == 8) {
             HIPT(hipMemcpyAsync(dst, src, n_ * sizeof(double), hipMemcpyHostToDevice, st_));
         } else {
@@ -1089,16 +1095,12 @@ class Solver final : public SolverBase {
                                                                          dst);
             LAUNCH_CHECK();
         }
-        sync();
+        // the staging buffer is reused by the next call: wait for this copy
+        if constexpr (sizeof(T) != 8) sync();
     }
 
-    void download(int field, double *dst) override {
-        if (double *d64 = field64(field)) {
-            HIPT(hipMemcpyAsync(dst, d64, n_ * sizeof(double), hipMemcpyDeviceToHost, st_));
-            sync();
-            return;
-        }
-        T *src = field_ptr(field);
+    // A device array of T into n_ f64 host values (converted on the device).
+    void download_to(const T *src, double *dst) {
         if constexpr (sizeof(T) == 8) {
             HIPT(hipMemcpyAsync(dst, src, n_ * sizeof(double), hipMemcpyDeviceToHost, st_));
         } else {
@@ -1110,6 +1112,15 @@ class Solver final : public SolverBase {
                                 st_));
         }
         sync();
+    }
+
+    void download(int field, double *dst) override {
+        if (double *d64 = field64(field)) {
+            HIPT(hipMemcpyAsync(dst, d64, n_ * sizeof(double), hipMemcpyDeviceToHost, st_));
+            sync();
+            return;
+        }
+        download_to(field_ptr(field), dst);
     }
 
     void sample(int field, const double *xyz, size_t npts, double *out) override {
@@ -1228,37 +1239,63 @@ class Solver final : public SolverBase {
         enqueue_advect_scalars(dt);
     }
 
-    // Adjoint of one MacCormack advection: lam_out -> lam_src, lam_U/V/W.
+    // Adjoint of one MacCormack advection: lam_out -> lam_src, lam_U/V/W
+    // (added to).  The scatters accumulate in 64-bit fixed point (order-
+    // independent, so the gradient is reproducible bit for bit), scaled from
+    // |lam_out|_1: every contribution of the correction and the forward pass
+    // together sums in absolute value to at most 2.5 |diss| |lam_out|_1
+    // (the correction passes |diss lam| to the forward adjoint plus
+    // 0.5 |diss lam| through the trilinear weights, which sum to 1, and at
+    // most |diss lam| to the source; the forward pass passes its adjoint on
+    // through weights summing to 1).  lam_src may be the same array as one
+    // of lam_U/V/W: the flush adds in a fixed order per cell.
     void adv_adjoint(double dt, const T *src, const T *U, const T *V, const T *W, double diss,
                      int clamp0, const T *lam_out, T *lam_src, T *lam_U, T *lam_V, T *lam_W) {
         const T s = (T)(dt / h_);
         k_sl_forward<T, 1><<<cell_grid_, cell_blk_, 0, st_>>>(g_, s, U, V, W, src, nullptr,
                                                               fwd_[0].p, nullptr);
         LAUNCH_CHECK();
-        zero(adj_lf_.p);
+        const unsigned nbk = partial_blocks(n_, adj_part_);
+        k_abs_sum<T><<<nbk, kThreads, 0, st_>>>(lam_out, (long)n_, adj_part_.p);
+        LAUNCH_CHECK();
+        k_fix_scale<<<1, kThreads, 0, st_>>>(adj_part_.p, (int)nbk, 2.5 * std::fabs(diss),
+                                             fix_.p);
+        LAUNCH_CHECK();
+        k_fill<fixacc_t><<<blocks_for(n_), kThreads, 0, st_>>>(acc_src_.p, (long)n_, 0);
+        LAUNCH_CHECK();
+        k_fill<fixacc_t><<<blocks_for(n_), kThreads, 0, st_>>>(acc_fwd_.p, (long)n_, 0);
+        LAUNCH_CHECK();
         const dim3 grid((g_.nx + WTX - 1) / WTX, (g_.ny + WTY - 1) / WTY,
                         (g_.nz + WTZ - 1) / WTZ);
         // LDS-windowed scatters (adjoint_kernels.hpp) once the grid has a tile
-        // per CU (256); smaller grids keep the one-plane launches, which
-        // spread over more workgroups (measured: 128^3 fwd+adj/fwd 3.23 ->
-        // 2.27 windowed; 64^3 2.74 direct, 2.83 windowed)
+        // per CU (256); smaller grids keep the one-cell-per-thread launches,
+        // which spread over more workgroups (measured with float atomics:
+        // 128^3 fwd+adj/fwd 3.23 -> 2.27 windowed; 64^3 2.74 direct, 2.83
+        // windowed)
         if ((long)grid.x * grid.y * grid.z >= 256) {
             const dim3 blk(WTX, WTY);
-            k_adj_mac_correct_w<T><<<grid, blk, 0, st_>>>(g_, s, (T)diss, clamp0, src,
-                                                          fwd_[0].p, U, V, W, lam_out, lam_src,
-                                                          adj_lf_.p, lam_U, lam_V, lam_W);
+            k_adj_mac_correct_w<T><<<grid, blk, 0, st_>>>(
+                g_, s, (T)diss, clamp0, src, fwd_[0].p, U, V, W, lam_out, fix_.p, acc_src_.p,
+                acc_fwd_.p, adj_tu_[0].p, adj_tu_[1].p, adj_tu_[2].p);
             LAUNCH_CHECK();
-            k_adj_sl_forward_w<T><<<grid, blk, 0, st_>>>(g_, s, src, U, V, W, adj_lf_.p,
-                                                         lam_src, lam_U, lam_V, lam_W);
+            k_adj_sl_forward_w<T><<<grid, blk, 0, st_>>>(g_, s, src, U, V, W, fix_.p, acc_fwd_.p,
+                                                         acc_src_.p, adj_tu_[0].p, adj_tu_[1].p,
+                                                         adj_tu_[2].p);
             LAUNCH_CHECK();
-            return;
+        } else {
+            k_adj_mac_correct<T><<<cell_grid_, cell_blk_, 0, st_>>>(
+                g_, s, (T)diss, clamp0, src, fwd_[0].p, U, V, W, lam_out, fix_.p, acc_src_.p,
+                acc_fwd_.p, adj_tu_[0].p, adj_tu_[1].p, adj_tu_[2].p);
+            LAUNCH_CHECK();
+            k_adj_sl_forward<T><<<cell_grid_, cell_blk_, 0, st_>>>(
+                g_, s, src, U, V, W, fix_.p, acc_fwd_.p, acc_src_.p, adj_tu_[0].p, adj_tu_[1].p,
+                adj_tu_[2].p);
+            LAUNCH_CHECK();
         }
-        k_adj_mac_correct<T><<<cell_grid_, cell_blk_, 0, st_>>>(
-            g_, s, (T)diss, clamp0, src, fwd_[0].p, U, V, W, lam_out, lam_src, adj_lf_.p, lam_U,
-            lam_V, lam_W);
-        LAUNCH_CHECK();
-        k_adj_sl_forward<T><<<cell_grid_, cell_blk_, 0, st_>>>(g_, s, src, U, V, W, adj_lf_.p,
-                                                               lam_src, lam_U, lam_V, lam_W);
+        k_adj_flush<T><<<blocks_for(n_), kThreads, 0, st_>>>((long)n_, fix_.p, acc_src_.p,
+                                                             adj_tu_[0].p, adj_tu_[1].p,
+                                                             adj_tu_[2].p, lam_src, lam_U, lam_V,
+                                                             lam_W);
         LAUNCH_CHECK();
     }
 
@@ -1341,8 +1378,9 @@ class Solver final : public SolverBase {
         return read_scalar();  // this step's d loss / d beta
     }
 
-    double rollout(const aether_rollout_desc &d, const double *ctrl, double beta,
-                   const double *target, double *grad_c, double *grad_beta) override {
+    // Checks a rollout description and loads its controls; sets the patch
+    // source and beta.  Returns the number of control cells.
+    size_t rollout_setup(const aether_rollout_desc &d, const double *ctrl, double beta) {
         if (ir64_) raise(AETHER_EINVAL, "rollouts need AETHER_FP32 or AETHER_FP64 precision");
         if (d.nsteps < 1 || d.checkpoint_every < 1) raise(AETHER_EINVAL, "bad rollout lengths");
         const int nn[3] = {g_.nx, g_.ny, g_.nz};
@@ -1355,39 +1393,81 @@ class Solver final : public SolverBase {
         const size_t cnt = (size_t)patch_n_.x * patch_n_.y * patch_n_.z;
         if (ctrl_.n < cnt) ctrl_.alloc(cnt);
         if (grad_c_.n < cnt) grad_c_.alloc(cnt);
-        if (!target_.p) target_.alloc(n_);
         HIPT(hipMemcpyAsync(ctrl_.p, ctrl, cnt * sizeof(double), hipMemcpyHostToDevice, st_));
-        HIPT(hipMemcpyAsync(target_.p, target, n_ * sizeof(double), hipMemcpyHostToDevice, st_));
-        const aether_fluid_params saved = params;
         params.buoyancy_beta = beta;
         patch_on_ = true;
+        return cnt;
+    }
+
+    void ensure_adjoint_buffers() {
+        for (DevBuf<T> *b : {&adj_a_[0], &adj_a_[1], &adj_a_[2], &adj_ar_, &adj_at_, &adj_b_[0],
+                             &adj_b_[1], &adj_c_[0], &adj_c_[1], &adj_c_[2], &adj_d_[0],
+                             &adj_d_[1], &adj_d_[2], &adj_tu_[0], &adj_tu_[1], &adj_tu_[2],
+                             &adj_lw_[0], &adj_lw_[1], &adj_lw_[2], &adj_lg_[0], &adj_lg_[1],
+                             &adj_lg_[2], &adj_p_})
+            if (!b->p) b->alloc(n_);
+        for (DevBuf<fixacc_t> *b : {&acc_src_, &acc_fwd_})
+            if (!b->p) b->alloc(n_);
+        if (!fix_.p) fix_.alloc(1);
+        if (!adj_part_.p) adj_part_.alloc(cell_grid_.x * cell_grid_.y * cell_grid_.z);
+    }
+
+    // Forward sweep of d.nsteps steps from the current state, saving a
+    // checkpoint every d.checkpoint_every steps; with `record`, the last
+    // segment's steps are recorded for the reverse sweep.  Leaves the final
+    // state in the solver.
+    void rollout_forward_sweep(const aether_rollout_desc &d, bool record) {
         const int K = d.checkpoint_every;
         const int nseg = (d.nsteps + K - 1) / K;
         while ((int)ckpts_.size() < nseg) ckpts_.push_back(std::make_unique<Ckpt>());
+        const int last0 = (nseg - 1) * K;
+        if (record)
+            while ((int)recs_.size() < K) recs_.push_back(std::make_unique<StepRec>());
+        for (int t = 0; t < d.nsteps; ++t) {
+            if (t % K == 0) save_ckpt(*ckpts_[t / K]);
+            if (record && t >= last0)
+                step_recorded(d.dt, *recs_[t - last0]);
+            else
+                step_plain(d.dt);
+        }
+    }
+
+    // Reverse sweep: from the adjoints of the final state in adj_a_ (u, v,
+    // w), adj_ar_ (density) and adj_at_ (temperature) to those of the
+    // initial state (same arrays), the controls (grad_c_) and beta.  Needs
+    // the checkpoints and the recorded last segment of the forward sweep.
+    double rollout_reverse_sweep(const aether_rollout_desc &d, size_t cnt) {
+        const int K = d.checkpoint_every;
+        const int nseg = (d.nsteps + K - 1) / K;
+        zero(adj_p_.p);
+        k_fill<double><<<blocks_for(cnt), kThreads, 0, st_>>>(grad_c_.p, (long)cnt, 0.0);
+        LAUNCH_CHECK();
+        double gbeta = 0.0;
+        for (int seg = nseg - 1; seg >= 0; --seg) {
+            const int t0 = seg * K, t1 = std::min(d.nsteps, t0 + K);
+            if (seg != nseg - 1) {
+                load_ckpt(*ckpts_[seg]);
+                for (int t = t0; t < t1; ++t) step_recorded(d.dt, *recs_[t - t0]);
+            }
+            for (int t = t1 - 1; t >= t0; --t) gbeta += adjoint_step(d.dt, *recs_[t - t0], d);
+        }
+        return gbeta;
+    }
+
+    double rollout(const aether_rollout_desc &d, const double *ctrl, double beta,
+                   const double *target, double *grad_c, double *grad_beta) override {
+        const aether_fluid_params saved = params;
+        tape_valid_ = false;
         double loss = 0.0;
         try {
+            const size_t cnt = rollout_setup(d, ctrl, beta);
+            if (!target_.p) target_.alloc(n_);
+            HIPT(hipMemcpyAsync(target_.p, target, n_ * sizeof(double), hipMemcpyHostToDevice,
+                                st_));
             const bool want = grad_c || grad_beta;
-            // the last segment is recorded during the forward sweep, so the
-            // reverse sweep recomputes every segment but that one
-            const int last0 = (nseg - 1) * K;
-            if (want)
-                while ((int)recs_.size() < K) recs_.push_back(std::make_unique<StepRec>());
-            for (int t = 0; t < d.nsteps; ++t) {
-                if (t % K == 0) save_ckpt(*ckpts_[t / K]);
-                if (want && t >= last0)
-                    step_recorded(d.dt, *recs_[t - last0]);
-                else
-                    step_plain(d.dt);
-            }
-            if (want) {
-                for (DevBuf<T> *b : {&adj_a_[0], &adj_a_[1], &adj_a_[2], &adj_ar_, &adj_at_,
-                                     &adj_b_[0], &adj_b_[1], &adj_c_[0], &adj_c_[1], &adj_c_[2],
-                                     &adj_d_[0], &adj_d_[1], &adj_d_[2], &adj_lf_, &adj_lw_[0],
-                                     &adj_lw_[1], &adj_lw_[2], &adj_lg_[0], &adj_lg_[1],
-                                     &adj_lg_[2], &adj_p_})
-                    if (!b->p) b->alloc(n_);
-                if (!adj_part_.p) adj_part_.alloc(cell_grid_.x * cell_grid_.y * cell_grid_.z);
-            }
+            if (want) ensure_adjoint_buffers();
+            rollout_forward_sweep(d, want);
+            // loss and its adjoint 2 (rho_T - target), the density seed
             const unsigned lb = partial_blocks(n_, partA_);
             k_loss<T><<<lb, kThreads, 0, st_>>>(cur_[3], target_.p, (long)n_,
                                                 want ? adj_ar_.p : nullptr, partA_.p);
@@ -1398,18 +1478,7 @@ class Solver final : public SolverBase {
             if (want) {
                 for (int a = 0; a < 3; ++a) zero(adj_a_[a].p);
                 zero(adj_at_.p);
-                zero(adj_p_.p);
-                k_fill<double><<<blocks_for(cnt), kThreads, 0, st_>>>(grad_c_.p, (long)cnt, 0.0);
-                LAUNCH_CHECK();
-                double gbeta = 0.0;
-                for (int seg = nseg - 1; seg >= 0; --seg) {
-                    const int t0 = seg * K, t1 = std::min(d.nsteps, t0 + K);
-                    if (seg != nseg - 1) {
-                        load_ckpt(*ckpts_[seg]);
-                        for (int t = t0; t < t1; ++t) step_recorded(d.dt, *recs_[t - t0]);
-                    }
-                    for (int t = t1 - 1; t >= t0; --t) gbeta += adjoint_step(d.dt, *recs_[t - t0], d);
-                }
+                const double gbeta = rollout_reverse_sweep(d, cnt);
                 if (grad_c)
                     HIPT(hipMemcpyAsync(grad_c, grad_c_.p, cnt * sizeof(double),
                                         hipMemcpyDeviceToHost, st_));
@@ -1425,6 +1494,70 @@ class Solver final : public SolverBase {
         params = saved;
         patch_on_ = false;
         return loss;
+    }
+
+    // Tape form: the forward sweep keeps what the reverse sweep needs (the
+    // checkpoints, the recorded last segment, the controls and beta) until
+    // rollout_backward consumes it.
+    void rollout_forward(const aether_rollout_desc &d, const double *ctrl, double beta,
+                         double *final_state) override {
+        const aether_fluid_params saved = params;
+        tape_valid_ = false;
+        try {
+            tape_cnt_ = rollout_setup(d, ctrl, beta);
+            ensure_adjoint_buffers();
+            rollout_forward_sweep(d, true);
+            if (final_state)
+                for (int a = 0; a < 5; ++a) download_to(cur_[a], final_state + (size_t)a * n_);
+            load_ckpt(*ckpts_[0]);
+            sync();
+        } catch (...) {
+            params = saved;
+            patch_on_ = false;
+            throw;
+        }
+        tape_desc_ = d;
+        tape_params_ = params;
+        tape_valid_ = true;
+        params = saved;
+        patch_on_ = false;
+    }
+
+    void rollout_backward(const double *lam_final, double *grad_c, double *grad_beta,
+                          double *grad_init) override {
+        if (!tape_valid_)
+            raise(AETHER_EINVAL, "rollout_backward needs a rollout_forward before it (each "
+                                 "forward is consumed by one backward)");
+        const aether_fluid_params saved = params;
+        tape_valid_ = false;
+        try {
+            params = tape_params_;
+            patch_on_ = true;
+            T *seed[5] = {adj_a_[0].p, adj_a_[1].p, adj_a_[2].p, adj_ar_.p, adj_at_.p};
+            for (int a = 0; a < 5; ++a) {
+                if (lam_final)
+                    upload_to(lam_final + (size_t)a * n_, seed[a]);
+                else
+                    zero(seed[a]);
+            }
+            const double gbeta = rollout_reverse_sweep(tape_desc_, tape_cnt_);
+            if (grad_c)
+                HIPT(hipMemcpyAsync(grad_c, grad_c_.p, tape_cnt_ * sizeof(double),
+                                    hipMemcpyDeviceToHost, st_));
+            if (grad_beta) *grad_beta = gbeta;
+            if (grad_init) {
+                const T *adj[5] = {adj_a_[0].p, adj_a_[1].p, adj_a_[2].p, adj_ar_.p, adj_at_.p};
+                for (int a = 0; a < 5; ++a) download_to(adj[a], grad_init + (size_t)a * n_);
+            }
+            load_ckpt(*ckpts_[0]);
+            sync();
+        } catch (...) {
+            params = saved;
+            patch_on_ = false;
+            throw;
+        }
+        params = saved;
+        patch_on_ = false;
     }
 
     const std::vector<int> &last_run_iters() const override { return last_run_iters_; }
@@ -1458,8 +1591,15 @@ class Solver final : public SolverBase {
     DevBuf<double> ctrl_, grad_c_, target_, adj_part_;
     std::vector<std::unique_ptr<Ckpt>> ckpts_;
     std::vector<std::unique_ptr<StepRec>> recs_;
-    DevBuf<T> adj_a_[3], adj_ar_, adj_at_, adj_b_[2], adj_c_[3], adj_d_[3], adj_lf_, adj_lw_[3],
-        adj_lg_[3], adj_p_;
+    DevBuf<T> adj_a_[3], adj_ar_, adj_at_, adj_b_[2], adj_c_[3], adj_d_[3], adj_tu_[3],
+        adj_lw_[3], adj_lg_[3], adj_p_;
+    DevBuf<fixacc_t> acc_src_, acc_fwd_;  // fixed-point scatter accumulators of the adjoint
+    DevBuf<FixScale> fix_;
+    // the tape between rollout_forward and rollout_backward
+    bool tape_valid_ = false;
+    aether_rollout_desc tape_desc_{};
+    aether_fluid_params tape_params_{};
+    size_t tape_cnt_ = 0;
     DevBuf<double> p64_, b64_, partD64_;
     double rel_override_ = -1.0;  // inner-solve tolerance while refining
     int ir_outer_ = 0, ir_inner_ = 0;

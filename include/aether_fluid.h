@@ -293,6 +293,26 @@ typedef struct {
 int aether_fluid_rollout(aether_fluid_t s, const aether_rollout_desc *d, const double *controls,
                          double beta, const double *target, double *loss, double *grad_controls,
                          double *grad_beta);
+/* The same rollout as a reverse-mode node (no counterpart in the reference),
+ * for a loss the caller defines on the final state.  rollout_forward runs
+ * the forward sweep from the current state (left unchanged) and records the
+ * tape (checkpoints every checkpoint_every steps, the last segment's
+ * intermediates, controls, beta and the knobs); final_state (may be NULL)
+ * receives the final u, v, w, density, temperature, each nx*ny*nz f64 in
+ * that order.  rollout_backward consumes the tape: lam_final (5*nx*ny*nz,
+ * same order; NULL = zero) is the adjoint of the final state, and it
+ * returns the vector-Jacobian products for the controls (grad_controls,
+ * patch cells), beta (grad_beta) and the initial u, v, w, density,
+ * temperature (grad_initial, 5*nx*ny*nz; the patch cells, which the
+ * controls overwrite, get 0).  Any argument may be NULL.  The pressure
+ * warm start is held fixed (the projection is treated as an exact solve).
+ * The solver's state is the rollout's initial state again on return; each
+ * forward is consumed by one backward. */
+int aether_fluid_rollout_forward(aether_fluid_t s, const aether_rollout_desc *d,
+                                 const double *controls, double beta, double *final_state);
+int aether_fluid_rollout_backward(aether_fluid_t s, const double *lam_final,
+                                  double *grad_controls, double *grad_beta,
+                                  double *grad_initial);
 
 /* Measurement hooks used by the benchmark driver. */
 /* Average device time (ms) per launch of the CG iteration kernels during the

@@ -78,13 +78,14 @@ def test_no_device_here_reports_enodev():
 
 
 def test_tape_backward_before_forward_raises():
-    """The rollout tape (torch.autograd.Function shape) refuses backward()
-    before a forward(..., need_grad=True); no device call is made."""
+    """The rollout tape refuses backward() before a forward(); no device
+    call is made.  The C entry points reject a null handle."""
     import aether_fluid as af
-    tape = af.SmokeRollout(None, nsteps=2, dt=0.02, patch_lo=(1, 1, 1), patch_hi=(3, 2, 3),
-                           target=np.zeros(8))
+    tape = af.SmokeRollout(None, nsteps=2, dt=0.02, patch_lo=(1, 1, 1), patch_hi=(3, 2, 3))
     with pytest.raises(RuntimeError):
-        tape.backward(1.0)
+        tape.backward({})
+    assert af.lib().aether_fluid_rollout_backward(None, None, None, None, None) == -1
+    assert af.lib().aether_fluid_rollout_forward(None, None, None, 1.0, None) == -1
 
 
 def test_noalloc_telemetry_rejects_bad_arguments_before_device_use():

@@ -1130,8 +1130,8 @@ def test_adjoint_fp64_matches_central_differences():
 
 def test_adjoint_fp32_matches_fp64_fd_and_checkpoints():
     """FP32 adjoint (FP64 reductions) against FP64 central differences
-    (eps = 1e-3), and the gradient is independent of the checkpoint
-    interval; the solver state is left unchanged."""
+    (eps = 1e-3), and the gradient is bit for bit independent of the
+    checkpoint interval; the solver state is left unchanged."""
     _announce("adjoint fp32 vs fp64 FD 24^3")
     n, steps = 24, 8
     s32, pb = _rollout_setup(n, "fp32", tol=1e-7, iters=800)
@@ -1141,8 +1141,8 @@ def test_adjoint_fp32_matches_fp64_fd_and_checkpoints():
                 target=pb["target"])
     loss, gc, gb = s32.rollout(controls=pb["controls"], beta=1.0, checkpoint_every=4, **args)
     _, gc1, gb1 = s32.rollout(controls=pb["controls"], beta=1.0, checkpoint_every=1, **args)
-    np.testing.assert_allclose(gc1, gc, rtol=1e-4, atol=1e-6 * np.abs(gc).max())
-    assert abs(gb1 - gb) <= 1e-4 * abs(gb)
+    np.testing.assert_array_equal(gc1, gc)  # fixed-point scatters: no order dependence
+    assert gb1 == gb
     np.testing.assert_array_equal(s32.download("u"), u_before)
     eps = 1e-3
     worst = 0.0
@@ -1158,34 +1158,125 @@ def test_adjoint_fp32_matches_fp64_fd_and_checkpoints():
     assert worst <= 1e-3, worst
 
 
-def test_tape_api_and_torch_function():
-    """The tape API returns rollout()'s loss and gradient scaled by the
-    incoming adjoint, and the torch.autograd.Function form delivers the same
-    gradients through loss.backward()."""
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+@pytest.mark.parametrize("n", [24, 96])
+def test_adjoint_bitwise_reproducible(precision, n):
+    """The adjoint's scatters accumulate in 64-bit fixed point, so the
+    gradient does not depend on thread order: two identical rollouts, and
+    checkpoints every step against every 4 steps, give the same bits.  96^3
+    runs the LDS-windowed scatter kernels (at least 256 tiles), 24^3 the
+    one-cell-per-thread ones."""
+    _announce(f"adjoint bitwise reproducible {precision} {n}^3")
+    s, pb = _rollout_setup(n, precision, tol=1e-6, iters=400)
+    args = dict(nsteps=4, dt=0.02, patch_lo=pb["patch_lo"], patch_hi=pb["patch_hi"],
+                target=pb["target"])
+    runs = [s.rollout(controls=pb["controls"], beta=1.0, checkpoint_every=k, **args)
+            for k in (4, 4, 1)]
+    for loss, gc, gb in runs[1:]:
+        assert loss == runs[0][0]
+        np.testing.assert_array_equal(gc, runs[0][1])
+        assert gb == runs[0][2]
+    assert np.abs(runs[0][1]).max() > 0
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_rollout_vjp_reproduces_fused_loss_gradient(precision):
+    """The reverse-mode node: rollout_forward returns the final state, and
+    rollout_backward with the adjoint 2 (density_T - target) computed on the
+    host gives rollout()'s gradient bit for bit.  At 96^3 (3456 reduction
+    workgroups, more than the 1024 slots most partial buffers hold) the
+    fused loss equals sum (density_T - target)^2 of the downloaded final
+    state."""
+    _announce(f"rollout vjp vs fused loss gradient {precision} 96^3")
+    s, pb = _rollout_setup(96, precision, tol=1e-6, iters=400)
+    kw = dict(nsteps=3, dt=0.02, patch_lo=pb["patch_lo"], patch_hi=pb["patch_hi"],
+              checkpoint_every=2)
+    u0 = s.download("u")
+    loss, gc, gb = s.rollout(controls=pb["controls"], beta=1.0, target=pb["target"], **kw)
+    final = s.rollout_forward(controls=pb["controls"], beta=1.0, **kw)
+    d = final["density"] - pb["target"]
+    host_loss = float(np.sum(d * d))
+    print("loss", loss, host_loss, flush=True)
+    assert abs(loss - host_loss) <= 1e-12 * abs(host_loss)
+    g2c, g2b, g2i = s.rollout_backward({"density": 2.0 * d})
+    np.testing.assert_array_equal(g2c, gc)
+    assert g2b == gb
+    np.testing.assert_array_equal(s.download("u"), u0)  # state left as it was
+    # the inflow patch is overwritten by the controls: no initial adjoint there
+    lo, hi = pb["patch_lo"], pb["patch_hi"]
+    rho0 = g2i["density"].reshape(96, 96, 96)
+    assert np.all(rho0[lo[2]:hi[2], lo[1]:hi[1], lo[0]:hi[0]] == 0.0)
+    import aether_fluid as af
+    with pytest.raises(af.AetherFluidError):
+        s.rollout_backward({"density": 2.0 * d})  # the tape was consumed
+
+
+def test_rollout_vjp_other_loss_matches_central_differences():
+    """A loss the fused entry point does not offer, differentiated through
+    the node: L = sum T_T^2 + 0.5 sum u_T^2 of the final state, FP64.
+    The VJP for controls, beta and the initial u and density against FP64
+    central differences (eps 1e-7) along 4 seeded directions, rel 1e-5."""
+    import aether_fluid as af
+    _announce("rollout vjp, loss sum T^2 + 0.5 sum u^2, fp64 16^3")
+    n = 16
+    s, pb = _rollout_setup(n, "fp64", tol=1e-13, iters=1000)
+    rng = np.random.default_rng(11)
+    init = {"u": rng.uniform(-0.5, 0.5, n ** 3), "density": rng.uniform(0, 1, n ** 3)}
+    for k, a in init.items():
+        s.upload(k, a)
+    tape = af.SmokeRollout(s, nsteps=6, dt=0.02, patch_lo=pb["patch_lo"],
+                           patch_hi=pb["patch_hi"], checkpoint_every=2)
+
+    def loss_of(final):
+        return float(np.sum(final["temperature"] ** 2) + 0.5 * np.sum(final["u"] ** 2))
+
+    final = tape.forward(pb["controls"], 1.0)
+    gc, gb, gi = tape.backward({"temperature": 2.0 * final["temperature"], "u": final["u"]})
+    eps = 1e-7
+    for q, (dc, db) in enumerate(_directions(4, len(pb["controls"]), seed=21)):
+        du = rng.uniform(-1, 1, n ** 3)
+        dr = rng.uniform(-1, 1, n ** 3)
+        vals = []
+        for sign in (1.0, -1.0):
+            st = {"u": init["u"] + sign * eps * du, "density": init["density"] + sign * eps * dr}
+            vals.append(loss_of(tape.forward(pb["controls"] + sign * eps * dc,
+                                             1.0 + sign * eps * db, st)))
+        fd = (vals[0] - vals[1]) / (2 * eps)
+        ad = float(gc @ dc + gb * db + gi["u"] @ du + gi["density"] @ dr)
+        print("vjp dir", q, fd, ad, abs(ad - fd) / abs(fd), flush=True)
+        assert abs(ad - fd) <= 1e-5 * abs(fd), (q, ad, fd)
+
+
+def test_torch_function_over_the_node():
+    """torch.autograd.Function form: the final state is a (5, N) tensor, a
+    torch loss on it (here sum (rho_T - target)^2 + sum T_T^2) backpropagates
+    through the library's adjoint into the controls, beta and the initial
+    state, and equals the node's VJP with the same adjoints."""
     import torch
     import aether_fluid as af
     from aether_fluid.adjoint import torch_function
-    _announce("tape api fp64 16^3")
-    n, steps = 16, 4
+    _announce("torch function over the rollout node fp64 16^3")
+    n = 16
     s, pb = _rollout_setup(n, "fp64")
-    kw = dict(nsteps=steps, dt=0.02, patch_lo=pb["patch_lo"], patch_hi=pb["patch_hi"],
-              target=pb["target"], checkpoint_every=2)
-    loss, gc, gb = s.rollout(controls=pb["controls"], beta=1.0, **kw)
+    kw = dict(nsteps=4, dt=0.02, patch_lo=pb["patch_lo"], patch_hi=pb["patch_hi"],
+              checkpoint_every=2)
     tape = af.SmokeRollout(s, **kw)
-    l2 = tape.forward(pb["controls"], 1.0)
-    assert abs(l2 - loss) <= 1e-12 * abs(loss)
-    g2c, g2b = tape.backward(2.0)
-    np.testing.assert_allclose(g2c, 2.0 * gc, rtol=1e-10, atol=1e-14 * np.abs(gc).max())
-    assert abs(g2b - 2.0 * gb) <= 1e-10 * abs(gb)
+    final = tape.forward(pb["controls"], 1.0)
+    lam = {"density": 2.0 * (final["density"] - pb["target"]),
+           "temperature": 2.0 * final["temperature"]}
+    gc, gb, gi = tape.backward(lam)
     fn = torch_function()
     c = torch.tensor(pb["controls"], dtype=torch.float64, requires_grad=True)
     b = torch.tensor(1.0, dtype=torch.float64, requires_grad=True)
-    out = fn.apply(c, b, tape)
-    (3.0 * out).backward()
-    assert abs(float(out.detach()) - loss) <= 1e-12 * abs(loss)
-    np.testing.assert_allclose(c.grad.numpy(), 3.0 * gc, rtol=1e-10,
-                               atol=1e-14 * np.abs(gc).max())
-    assert abs(float(b.grad) - 3.0 * gb) <= 1e-10 * abs(gb)
+    init = torch.zeros((5, n ** 3), dtype=torch.float64, requires_grad=True)
+    out = fn.apply(c, b, init, tape)
+    tgt = torch.from_numpy(pb["target"])
+    loss = ((out[3] - tgt) ** 2).sum() + (out[4] ** 2).sum()
+    loss.backward()
+    np.testing.assert_array_equal(out[3].detach().numpy(), final["density"])
+    np.testing.assert_array_equal(c.grad.numpy(), gc)
+    assert float(b.grad) == gb
+    np.testing.assert_array_equal(init.grad[3].numpy(), gi["density"])
 
 
 def test_adjoint_fp64_rollout_64_real_size_matches_central_differences():
@@ -1193,13 +1284,25 @@ def test_adjoint_fp64_rollout_64_real_size_matches_central_differences():
     of dt 0.02, checkpoint every 8, inflow-patch controls (32x4x32) and the
     buoyancy coefficient.  The FP64 adjoint against FP64 central differences
     at eps = 1e-7 (small enough not to straddle the clamp and floor switches
-    of the discrete forward) along two seeded directions, relative error
-    <= 1e-5; and the FP32 adjoint (FP64 reductions) against the FP64 adjoint,
-    whole-gradient relative L2 <= 1e-3."""
-    _announce("adjoint fp64 vs FD, rollout 64^3 x 40 steps")
+    of the discrete forward) along all 16 seeded directions, relative error
+    <= 1e-5; the FP32 adjoint (FP64 reductions, fixed-point scatters)
+    against the same differences, relative error <= 1e-4 in every
+    direction, and against the FP64 adjoint, whole-gradient relative L2
+    <= 1e-3.
+
+    The FP32 rollout projects to relative residual 1e-6 (the workload's
+    tight tolerance; FP32 CG attains a true residual of ~6e-7 here).  The
+    gradient is the exact derivative of a piecewise-smooth forward (clamp
+    arg-min/max, floor, near-wall selections): a trajectory that differs by
+    ~1e-8 can sit on the other side of a selection in some cell, which moves
+    the gradient by ~1e-4 relative.  Measured: FP32 at tolerance 1e-6 is
+    within 9.8e-6 of the differences in every direction; an FP64 forward
+    projected only to 1e-5 is 8.6e-5 (relative L2) off the tight FP64
+    gradient, as is FP32 at 1e-5 or at 1e-7."""
+    _announce("adjoint fp64/fp32 vs FD, rollout 64^3 x 40 steps, 16 directions")
     n = 64
     s64, pb = _rollout_setup(n, "fp64", tol=1e-12, iters=3000)
-    s32, _ = _rollout_setup(n, "fp32", tol=1e-5, iters=200)
+    s32, _ = _rollout_setup(n, "fp32", tol=1e-6, iters=1000)
     args = dict(nsteps=40, dt=0.02, patch_lo=pb["patch_lo"], patch_hi=pb["patch_hi"],
                 target=pb["target"], checkpoint_every=8)
     assert len(pb["controls"]) == 32 * 4 * 32
@@ -1209,12 +1312,16 @@ def test_adjoint_fp64_rollout_64_real_size_matches_central_differences():
     assert np.linalg.norm(gc32 - gc) <= 1e-3 * np.linalg.norm(gc), np.linalg.norm(gc32 - gc)
     assert abs(gb32 - gb) <= 1e-3 * abs(gb)
     eps = 1e-7
-    for dc, db in _directions(2, len(pb["controls"]), seed=0xAE7E0003):
+    e64, e32 = [], []
+    for dc, db in _directions(16, len(pb["controls"]), seed=0xAE7E0003):
         lp, _, _ = s64.rollout(controls=pb["controls"] + eps * dc, beta=1.0 + eps * db,
                                grad=False, **args)
         lm, _, _ = s64.rollout(controls=pb["controls"] - eps * dc, beta=1.0 - eps * db,
                                grad=False, **args)
         fd = (lp - lm) / (2 * eps)
-        ad = float(gc @ dc + gb * db)
-        print("fp64 64^3 dir", fd, ad, abs(ad - fd) / abs(fd))
-        assert abs(ad - fd) <= 1e-5 * abs(fd), (ad, fd)
+        e64.append(abs(float(gc @ dc + gb * db) - fd) / abs(fd))
+        e32.append(abs(float(gc32 @ dc + gb32 * db) - fd) / abs(fd))
+        print(f"64^3 dir fd {fd:.9e} fp64 {e64[-1]:.2e} fp32 {e32[-1]:.2e}", flush=True)
+    print("max fp64", max(e64), "max fp32", max(e32), "median fp32", np.median(e32), flush=True)
+    assert max(e64) <= 1e-5, e64
+    assert max(e32) <= 1e-4, e32
