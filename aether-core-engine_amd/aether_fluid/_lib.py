@@ -131,11 +131,14 @@ SIGNATURES = {
                                                          C.c_double, _vp, _vp, _vp, _vp,
                                                          C.c_double, _vp, _vp]),
     "aether_fluid_sample_trilinear": (C.c_int, [_vp, C.c_int, _pd, C.c_size_t, _pd]),
+    "aether_fluid_sample_trilinear_field": (C.c_int, [_vp, _pd, C.c_size_t, _pd, C.c_size_t,
+                                                      _pd]),
     "aether_fluid_divergence_max": (C.c_int, [_vp, _pd]),
     "aether_fluid_last_solve_stats": (C.c_int, [_vp, C.POINTER(SolveStats)]),
     "aether_fluid_last_run_iterations": (C.c_int, [_vp, C.POINTER(C.c_int32), C.c_int,
                                                    C.POINTER(C.c_int)]),
     "aether_fluid_upload": (C.c_int, [_vp, C.c_int, _pd, C.c_size_t]),
+    "aether_fluid_nonfinite": (C.c_int, [_vp, C.POINTER(C.c_int)]),
     "aether_fluid_download": (C.c_int, [_vp, C.c_int, _pd, C.c_size_t]),
     "aether_fluid_max_density": (C.c_int, [_vp, _pd]),
     "aether_fluid_batch_autotune_cg": (C.c_int, [_vp, C.c_int, _pd, C.POINTER(C.c_int)]),

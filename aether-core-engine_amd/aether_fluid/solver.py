@@ -225,14 +225,29 @@ class FluidSolver3D:
         L.check(self._lib.aether_fluid_compute_telemetry(self._h, timeline_sec, C.byref(t)))
         return FluidTelemetryFrame._from(t)
 
-    def sample_trilinear(self, field: str, x, y=None, z=None):
-        """src/fluid.rs:94-128; scalar (x, y, z) or an (n, 3) array of points."""
+    def sample_trilinear(self, field, x, y=None, z=None):
+        """src/fluid.rs:94-128; scalar (x, y, z) or an (n, 3) array of points.
+        `field` is the name of one of the solver's fields, or (as the
+        reference's signature takes) any f64 array of nx*ny*nz values."""
         pts = np.asarray([[x, y, z]] if y is not None else x, dtype=np.float64).reshape(-1, 3)
         pts = np.ascontiguousarray(pts)
         out = np.empty(len(pts), dtype=np.float64)
-        L.check(self._lib.aether_fluid_sample_trilinear(self._h, L.FIELDS[field], _dptr(pts),
-                                                        len(pts), _dptr(out)))
+        if isinstance(field, str):
+            L.check(self._lib.aether_fluid_sample_trilinear(self._h, L.FIELDS[field], _dptr(pts),
+                                                            len(pts), _dptr(out)))
+        else:
+            f = np.ascontiguousarray(np.asarray(field, dtype=np.float64).reshape(-1))
+            L.check(self._lib.aether_fluid_sample_trilinear_field(self._h, _dptr(f), f.size,
+                                                                  _dptr(pts), len(pts),
+                                                                  _dptr(out)))
         return float(out[0]) if y is not None else out
+
+    def nonfinite(self) -> bool:
+        """True if a telemetry reduction since the last call met a non-finite
+        velocity, vorticity or density (the flag is then cleared)."""
+        f = C.c_int()
+        L.check(self._lib.aether_fluid_nonfinite(self._h, C.byref(f)))
+        return bool(f.value)
 
     def max_density(self) -> float:
         out = C.c_double()

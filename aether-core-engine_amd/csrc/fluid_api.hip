@@ -245,6 +245,19 @@ int aether_fluid_sample_trilinear(aether_fluid_t s, int field, const double *xyz
     });
 }
 
+int aether_fluid_sample_trilinear_field(aether_fluid_t s, const double *field, size_t n,
+                                        const double *xyz, size_t npts, double *out) {
+    return guarded([&] {
+        checked(s);
+        if (!field) raise(AETHER_EINVAL, "null field array");
+        if (n != cells(s))
+            raise(AETHER_EINVAL, "field size " + std::to_string(n) + " != nx*ny*nz = " +
+                                     std::to_string(cells(s)));
+        if (npts && (!xyz || !out)) raise(AETHER_EINVAL, "null point or output array");
+        s->impl->sample_field(field, xyz, npts, out);
+    });
+}
+
 int aether_fluid_last_solve_stats(aether_fluid_t s, aether_fluid_solve_stats *out) {
     return guarded([&] {
         checked(s);
@@ -260,6 +273,14 @@ int aether_fluid_last_run_iterations(aether_fluid_t s, int32_t *out, int n, int 
         const auto &v = s->impl->last_run_iters();
         if (count) *count = (int)v.size();
         for (int a = 0; a < n && a < (int)v.size(); ++a) out[a] = v[a];
+    });
+}
+
+int aether_fluid_nonfinite(aether_fluid_t s, int *flag) {
+    return guarded([&] {
+        checked(s);
+        if (!flag) raise(AETHER_EINVAL, "null flag pointer");
+        *flag = s->impl->take_nonfinite();
     });
 }
 

@@ -1348,9 +1348,11 @@ __device__ __forceinline__ void body_telemetry_part(int i, int j, int kb, int ke
                                                     const T *__restrict__ v,
                                                     const T *__restrict__ w,
                                                     const T *__restrict__ rho,
-                                                    double *__restrict__ part) {
+                                                    double *__restrict__ part,
+                                                    int *nonfinite = nullptr) {
     __shared__ double lds[32];
     const double dv = h * h * h;
+    bool bad = false;
     const double inv_2h = 0.5 / h;
     double acc[6] = {0, 0, 0, 0, 0, 0};
     double peak = 0.0;
@@ -1372,6 +1374,9 @@ __device__ __forceinline__ void body_telemetry_part(int i, int j, int kb, int ke
             acc[1] += 0.5 * w2 * dv;
             peak = fmax(peak, sqrt(w2));
             const double r = rho[id];
+            // a non-finite velocity or density reads as a non-finite |u|^2,
+            // |w|^2 or rho (inf - inf and NaN alike)
+            bad |= !isfinite(u2) || !isfinite(w2) || !isfinite(r);
             if (r > 1e-8) {
                 const double m = r * dv;
                 acc[2] += m;
@@ -1381,6 +1386,9 @@ __device__ __forceinline__ void body_telemetry_part(int i, int j, int kb, int ke
             }
         }
     }
+    // the device flag of src/fluid.rs's silent NaN propagation: an integer
+    // OR, the same whatever the order
+    if (bad && nonfinite) atomicOr(nonfinite, 1);
     block_sum<6>(acc, lds);
     peak = block_max(peak, lds);
     if (tid_lin() == 0) {
@@ -1396,12 +1404,13 @@ __global__ __launch_bounds__(kThreads) void k_telemetry_part(Dims g, int zchunk,
                                                              const T *__restrict__ v,
                                                              const T *__restrict__ w,
                                                              const T *__restrict__ rho,
-                                                             double *__restrict__ part) {
+                                                             double *__restrict__ part,
+                                                             int *nonfinite) {
     const int kb = blockIdx.z * zchunk;
     body_telemetry_part<T>(blockIdx.x * kBX + threadIdx.x, blockIdx.y * kBY + threadIdx.y, kb,
                            min(kb + zchunk, g.nz),
                            (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x, g, h,
-                           u, v, w, rho, part);
+                           u, v, w, rho, part, nonfinite);
 }
 
 // One workgroup: fixed-order combination of the partials into a

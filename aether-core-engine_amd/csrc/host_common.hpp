@@ -212,11 +212,16 @@ class SolverBase {
     virtual void upload(int field, const double *src) = 0;
     virtual void download(int field, double *dst) = 0;
     virtual void sample(int field, const double *xyz, size_t npts, double *out) = 0;
+    virtual void sample_field(const double *field, const double *xyz, size_t npts,
+                              double *out) = 0;
     virtual double max_density() = 0;
     virtual std::unique_ptr<MediumBase> make_medium(const double *lo, const double *hi) = 0;
     virtual void autotune(int iters, double *us, int *form, int *z) = 0;
     virtual void stats(aether_fluid_solve_stats *out) = 0;
     virtual const std::vector<int> &last_run_iters() const = 0;
+    // 1 if a telemetry reduction since the last call met a non-finite u, v, w,
+    // density or |curl|^2; clears the flag
+    virtual int take_nonfinite() = 0;
     virtual double rollout(const aether_rollout_desc &d, const double *ctrl, double beta,
                            const double *target, double *grad_c, double *grad_beta) = 0;
     virtual void rollout_forward(const aether_rollout_desc &d, const double *ctrl, double beta,

@@ -57,7 +57,7 @@ void launch_telemetry(const RedTiling &rt, Dims g, double h, const void *u, cons
                       hipStream_t st) {
     k_telemetry_part<T><<<dim3(rt.gx, rt.gy, rt.nch), dim3(kBX, kBY, 1), 0, st>>>(
         g, rt.zchunk, h, static_cast<const T *>(u), static_cast<const T *>(v),
-        static_cast<const T *>(w), static_cast<const T *>(rho), part);
+        static_cast<const T *>(w), static_cast<const T *>(rho), part, nullptr);
     LAUNCH_CHECK();
     k_telemetry_final<<<1, dim3(kBX, kBY, 1), 0, st>>>(part, rt.nb, t, out);
     LAUNCH_CHECK();

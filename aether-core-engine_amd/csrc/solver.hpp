@@ -125,6 +125,9 @@ class Solver final : public SolverBase {
         tel_.alloc(8);
         cg_.alloc(1);
         iters_log_.alloc(1);
+        nonfinite_.alloc(1);
+        k_fill<int><<<1, kThreads, 0, st_>>>(nonfinite_.p, 1, 0);
+        LAUNCH_CHECK();
         if (ir64_) {
             p64_.alloc(n_);
             b64_.alloc(n_);
@@ -976,7 +979,8 @@ class Solver final : public SolverBase {
 
     void enqueue_telemetry(double t, double *dst_dev) {
         k_telemetry_part<T><<<red_grid_, cell_blk_, 0, st_>>>(g_, zchunk_, h_, cur_[0], cur_[1],
-                                                              cur_[2], cur_[3], part_tel_.p);
+                                                              cur_[2], cur_[3], part_tel_.p,
+                                                              nonfinite_.p);
         LAUNCH_CHECK();
         k_telemetry_final<<<1, cell_blk_, 0, st_>>>(part_tel_.p, nb_, t, dst_dev);
         LAUNCH_CHECK();
@@ -1121,6 +1125,21 @@ class Solver final : public SolverBase {
             return;
         }
         download_to(field_ptr(field), dst);
+    }
+
+    // src/fluid.rs:94-128 on a caller's f64 field in this grid's layout,
+    // sampled in FP64 whatever the solver's precision.
+    void sample_field(const double *field, const double *xyz, size_t npts,
+                      double *out) override {
+        if (npts == 0) return;
+        DevBuf<double> f(n_), dxyz(npts * 3), dout(npts);
+        HIPT(hipMemcpyAsync(f.p, field, n_ * sizeof(double), hipMemcpyHostToDevice, st_));
+        HIPT(hipMemcpyAsync(dxyz.p, xyz, npts * 3 * sizeof(double), hipMemcpyHostToDevice, st_));
+        k_sample_points<double><<<(unsigned)((npts + kThreads - 1) / kThreads), cell_blk_, 0,
+                                  st_>>>(g_, f.p, dxyz.p, (long)npts, dout.p);
+        LAUNCH_CHECK();
+        HIPT(hipMemcpyAsync(out, dout.p, npts * sizeof(double), hipMemcpyDeviceToHost, st_));
+        sync();
     }
 
     void sample(int field, const double *xyz, size_t npts, double *out) override {
@@ -1562,6 +1581,15 @@ class Solver final : public SolverBase {
 
     const std::vector<int> &last_run_iters() const override { return last_run_iters_; }
 
+    int take_nonfinite() override {
+        int v = 0;
+        HIPT(hipMemcpyAsync(&v, nonfinite_.p, sizeof(int), hipMemcpyDeviceToHost, st_));
+        k_fill<int><<<1, kThreads, 0, st_>>>(nonfinite_.p, 1, 0);
+        LAUNCH_CHECK();
+        sync();
+        return v;
+    }
+
    private:
     Dims g_;
     double h_;
@@ -1619,6 +1647,7 @@ class Solver final : public SolverBase {
     DevBuf<double> tot_;  // pre-summed r.r and b.b of the wide four-wide path
     DevBuf<CgState> cg_;
     DevBuf<int> iters_log_, iters_dev_;
+    DevBuf<int> nonfinite_;  // set when a telemetry reduction met a non-finite value
     int last_kind_ = AETHER_SOLVER_CG;
     int last_sweeps_ = 0;
     std::vector<int> last_run_iters_;

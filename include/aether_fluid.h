@@ -178,15 +178,30 @@ int aether_fluid_compute_telemetry_noalloc(int precision, int nx, int ny, int nz
                                            const void *u, const void *v, const void *w,
                                            const void *density, double time_sec, double *out_dev,
                                            void *stream);
-/* fluid.rs:94 sample_trilinear, for npts points (x,y,z interleaved). */
+/* fluid.rs:94 sample_trilinear, for npts points (x,y,z interleaved), of one
+ * of the solver's own fields (an AETHER_FIELD_* id). */
 int aether_fluid_sample_trilinear(aether_fluid_t s, int field, const double *xyz, size_t npts,
                                   double *out);
+/* fluid.rs:94 sample_trilinear(&self, field: &[f64], x, y, z) as the
+ * reference declares it: any caller-supplied f64 field of n = nx*ny*nz
+ * values in this grid's layout, evaluated in FP64 with the grid's clamp box,
+ * at npts points (x,y,z interleaved). */
+int aether_fluid_sample_trilinear_field(aether_fluid_t s, const double *field, size_t n,
+                                        const double *xyz, size_t npts, double *out);
 /* Extension: max |div u| over interior cells (same stencil as fluid.rs:341-344). */
 int aether_fluid_divergence_max(aether_fluid_t s, double *out);
 int aether_fluid_last_solve_stats(aether_fluid_t s, aether_fluid_solve_stats *out);
 /* CG iterations of each step of the last aether_fluid_run / aether_fluid_step
  * (RBGS: sweeps); copies min(n, steps) values, *count = steps. */
 int aether_fluid_last_run_iterations(aether_fluid_t s, int32_t *out, int n, int *count);
+
+/* No counterpart in the reference, whose solver lets a NaN or infinity
+ * propagate silently (src/fluid.rs:94-128 clamps and floors a NaN trace to
+ * index 0).  Every telemetry reduction (step, run, compute_telemetry) raises
+ * a device flag when a cell's velocity, |curl u|^2 or density is not finite;
+ * *flag receives it (1 or 0) and it is cleared.  The fields themselves are
+ * left as the reference would leave them. */
+int aether_fluid_nonfinite(aether_fluid_t s, int *flag);
 
 /* Field access (the reference's pub Vec<f64> arrays, src/fluid.rs:28-34).
  * n must equal nx*ny*nz. */

@@ -618,6 +618,55 @@ def test_sample_trilinear_and_max_density():
     assert gpu.max_density() == max(0.0, st["density"].max())
 
 
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_nonfinite_flag(precision):
+    """A NaN uploaded into one velocity cell is reported by the telemetry
+    reduction's device flag (which a query clears); clean steps raise
+    nothing."""
+    _announce(f"non-finite flag {precision}")
+    import aether_fluid as af
+    dims = (20, 24, 16)
+    s = af.FluidSolver3D(*dims, precision=precision)
+    st = random_state(*dims, seed=4, vel=0.3)
+    for n, a in st.items():
+        s.upload(n, a)
+    s.solver_iters = 40
+    s.step(0.02, 0.0)
+    assert not s.nonfinite()
+    u = s.download("u")
+    u[(8 * dims[1] + 10) * dims[0] + 7] = np.nan
+    s.upload("u", u)
+    s.compute_telemetry(0.0)
+    assert s.nonfinite()
+    assert not s.nonfinite()  # cleared by the query
+    s.step(0.02, 0.02)  # the reference lets the NaN spread; the flag reports it
+    assert s.nonfinite()
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_sample_trilinear_caller_field(precision):
+    """sample_trilinear on a caller's own f64 field (src/fluid.rs:94 takes
+    any &[f64]): evaluated in FP64 whatever the solver's precision, so equal
+    to the oracle bit for bit, with points inside, on and beyond the clamp
+    box (src/fluid.rs:95-97)."""
+    _announce(f"sample_trilinear on a caller field ({precision})")
+    import aether_fluid as af
+    from oracle.oracle import OracleFluid
+    dims = (12, 14, 10)
+    field = np.random.default_rng(5).uniform(-3, 3, int(np.prod(dims)))
+    gpu = af.FluidSolver3D(*dims, precision=precision)
+    orc = OracleFluid(*dims)
+    orc.field("pressure")[...] = field.reshape(dims[2], dims[1], dims[0])
+    pts = np.random.default_rng(6).uniform(-2, 15, size=(200, 3))
+    pts[:4] = [[0.5, 0.5, 0.5], [10.499999, 12.499999, 8.499999], [3.0, 4.0, 5.0],
+               [11.0, 13.0, 9.0]]
+    got = gpu.sample_trilinear(field, pts)
+    exp = np.array([orc.sample_trilinear("pressure", *p) for p in pts])
+    np.testing.assert_array_equal(got, exp)
+    with pytest.raises(af.AetherFluidError):
+        gpu.sample_trilinear(field[:-1], pts)
+
+
 def test_invalid_arguments_raise():
     import aether_fluid as af
     with pytest.raises(af.AetherFluidError):

@@ -486,3 +486,18 @@ def test_soundtrack_first_second_reads_frames_0_to_8_only():
     np.testing.assert_array_equal(a, soundtrack(fr, 0.06, 190.0, 48000))
     alt[8, 1] *= 1.01
     assert not np.array_equal(a, soundtrack(alt, 0.06, 190.0, 48000))
+
+
+def test_oracle_under_address_and_undefined_behaviour_sanitizers():
+    """The checker itself under ASan + UBSan (oracle/sanitize_check.c): every
+    public oracle function on small scenarios, including the README known
+    answer; any out-of-bounds access, leak or undefined behaviour aborts."""
+    import os
+    import subprocess
+    here = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle")
+    subprocess.run(["make", "-s", "-C", here, "sanitize"], check=True)
+    res = subprocess.run([os.path.join(here, "oracle_sanitize_check")], capture_output=True,
+                         text=True, timeout=300)
+    assert res.returncode == 0, res.stderr[-4000:]
+    assert "runtime error" not in res.stderr and "AddressSanitizer" not in res.stderr
+    assert "known answer 2.5751e+00 6.7289e-03" in res.stdout
