@@ -10,6 +10,8 @@ import ctypes as C
 import os
 import subprocess
 
+import numpy as np
+
 PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.environ.get("AETHER_FLUID_LIB") or os.path.join(PKG_ROOT, "lib",
                                                                 "libaether_fluid_hip.so")
@@ -99,6 +101,14 @@ class RolloutDesc(C.Structure):
     ]
 
 
+# aether_medium_ray and aether_medium_interaction as numpy record layouts
+RAY_DTYPE = np.dtype([("origin", "<f8", (3,)), ("dir", "<f8", (3,)), ("t_min", "<f8"),
+                      ("t_far", "<f8")])
+INTERACTION_DTYPE = np.dtype([("hit", "<i4"), ("reserved", "<i4"), ("t", "<f8"),
+                              ("point", "<f8", (3,)), ("albedo", "<f8", (4,)),
+                              ("emission", "<f8", (4,))])
+assert RAY_DTYPE.itemsize == 64 and INTERACTION_DTYPE.itemsize == 104
+
 # every symbol declared in include/aether_fluid.h: name -> (restype, argtypes)
 _vp = C.c_void_p
 _pd = C.POINTER(C.c_double)
@@ -149,6 +159,11 @@ SIGNATURES = {
     "aether_fluid_medium_destroy": (C.c_int, [_vp]),
     "aether_fluid_medium_info": (C.c_int, [_vp, _pd, _pd, _pd]),
     "aether_fluid_medium_sample": (C.c_int, [_vp, _pd, C.c_size_t, _pd]),
+    "aether_fluid_medium_sample_interaction": (C.c_int, [_vp, _vp, C.c_size_t, _pd, _pu64,
+                                                         _vp]),
+    "aether_fluid_medium_transmittance": (C.c_int, [_vp, _vp, C.c_size_t, _pu64, _pd]),
+    "aether_fluid_medium_phase_hg_pdf": (C.c_int, [_vp, _pd, _pd, C.c_size_t, _pd]),
+    "aether_fluid_medium_sample_phase_hg": (C.c_int, [_vp, _pd, C.c_size_t, _pu64, _pd, _pd]),
     "aether_fluid_batch_create": (C.c_int, [C.POINTER(InstanceDesc), C.c_int, C.c_int,
                                             C.POINTER(Params), C.POINTER(_vp)]),
     "aether_fluid_batch_destroy": (C.c_int, [_vp]),

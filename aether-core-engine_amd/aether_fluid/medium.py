@@ -44,6 +44,69 @@ class FluidMedium:
         L.check(self._lib.aether_fluid_medium_sample(self._h, _dptr(pts), len(pts), _dptr(out)))
         return out
 
+    # --- volumetric sampling for the path tracer (src/fluid.rs:790-910) ----
+    @staticmethod
+    def make_rays(origins, dirs, t_min, t_far) -> np.ndarray:
+        """Rays as Ray::new gives them: dirs must already be normalised
+        (Ray::dir); t_far is the call's surface_t or max_dist."""
+        o = np.asarray(origins, dtype=np.float64).reshape(-1, 3)
+        r = np.zeros(len(o), dtype=L.RAY_DTYPE)
+        r["origin"], r["dir"] = o, np.asarray(dirs, dtype=np.float64).reshape(-1, 3)
+        r["t_min"], r["t_far"] = t_min, t_far
+        return r
+
+    @staticmethod
+    def _states(states) -> np.ndarray:
+        s = np.asarray(states)
+        if s.dtype != np.uint64 or not s.flags.c_contiguous or s.ndim != 2 or s.shape[1] != 4:
+            raise ValueError("rng states must be a C-contiguous (n, 4) uint64 array")
+        return s
+
+    def sample_interaction(self, rays, lambdas, states) -> np.ndarray:
+        """src/fluid.rs:793-838 for every ray (a RAY_DTYPE array); lambdas
+        (n, 4) nm; states (n, 4) uint64 Xoshiro256++ states, advanced in
+        place.  Returns an INTERACTION_DTYPE array (hit = 0: None)."""
+        rays = np.ascontiguousarray(rays, dtype=L.RAY_DTYPE)
+        lam = np.ascontiguousarray(np.asarray(lambdas, dtype=np.float64).reshape(-1, 4))
+        st = self._states(states)
+        assert len(lam) == len(rays) == len(st)
+        out = np.zeros(len(rays), dtype=L.INTERACTION_DTYPE)
+        L.check(self._lib.aether_fluid_medium_sample_interaction(
+            self._h, rays.ctypes.data, len(rays), _dptr(lam),
+            st.ctypes.data_as(C.POINTER(C.c_uint64)), out.ctypes.data))
+        return out
+
+    def transmittance(self, rays, states) -> np.ndarray:
+        """src/fluid.rs:841-874 for every ray; states advanced in place."""
+        rays = np.ascontiguousarray(rays, dtype=L.RAY_DTYPE)
+        st = self._states(states)
+        assert len(rays) == len(st)
+        out = np.empty(len(rays), dtype=np.float64)
+        L.check(self._lib.aether_fluid_medium_transmittance(
+            self._h, rays.ctypes.data, len(rays), st.ctypes.data_as(C.POINTER(C.c_uint64)),
+            _dptr(out)))
+        return out
+
+    def phase_hg_pdf(self, wo, wi) -> np.ndarray:
+        """src/fluid.rs:878-888 for every row pair."""
+        wo = np.ascontiguousarray(np.asarray(wo, dtype=np.float64).reshape(-1, 3))
+        wi = np.ascontiguousarray(np.asarray(wi, dtype=np.float64).reshape(-1, 3))
+        out = np.empty(len(wo), dtype=np.float64)
+        L.check(self._lib.aether_fluid_medium_phase_hg_pdf(self._h, _dptr(wo), _dptr(wi),
+                                                           len(wo), _dptr(out)))
+        return out
+
+    def sample_phase_hg(self, wo, states):
+        """src/fluid.rs:892-910 for every row of wo; returns (wi, pdf)."""
+        wo = np.ascontiguousarray(np.asarray(wo, dtype=np.float64).reshape(-1, 3))
+        st = self._states(states)
+        wi = np.empty_like(wo)
+        pdf = np.empty(len(wo), dtype=np.float64)
+        L.check(self._lib.aether_fluid_medium_sample_phase_hg(
+            self._h, _dptr(wo), len(wo), st.ctypes.data_as(C.POINTER(C.c_uint64)), _dptr(wi),
+            _dptr(pdf)))
+        return wi, pdf
+
     def close(self) -> None:
         if getattr(self, "_h", None):
             self._lib.aether_fluid_medium_destroy(self._h)

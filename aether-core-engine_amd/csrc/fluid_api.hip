@@ -348,6 +348,7 @@ int aether_fluid_medium_create(aether_fluid_t s, const double *bounds_min,
         m->device = s->device;
         m->impl = s->impl->make_medium(bounds_min, bounds_max);
         // src/fluid.rs:709, 720
+        m->impl->sigma_t_scale = sigma_t_scale;
         m->impl->majorant_sigma_t = std::max(m->impl->max_density * sigma_t_scale, 1e-4);
         m->impl->phase_g = std::min(std::max(phase_g, -0.9), 0.9);
         *out = m.release();
@@ -379,6 +380,54 @@ int aether_fluid_medium_sample(aether_fluid_medium_t m, const double *xyz, size_
         if (npts && (!xyz || !out)) raise(AETHER_EINVAL, "null point or output array");
         HIPT(hipSetDevice(m->device));
         m->impl->sample(xyz, npts, out);
+    });
+}
+
+namespace {
+aether_fluid_medium_s *checked_medium(aether_fluid_medium_t m) {
+    if (!m) raise(AETHER_EINVAL, "null medium handle");
+    HIPT(hipSetDevice(m->device));
+    return m;
+}
+}  // namespace
+
+int aether_fluid_medium_sample_interaction(aether_fluid_medium_t m, const aether_medium_ray *rays,
+                                           size_t nrays, const double *lambda,
+                                           uint64_t *rng_state, aether_medium_interaction *out) {
+    return guarded([&] {
+        checked_medium(m);
+        if (nrays && (!rays || !lambda || !rng_state || !out))
+            raise(AETHER_EINVAL, "null ray, wavelength, rng or output array");
+        m->impl->interaction(rays, nrays, lambda, rng_state, out);
+    });
+}
+
+int aether_fluid_medium_transmittance(aether_fluid_medium_t m, const aether_medium_ray *rays,
+                                      size_t nrays, uint64_t *rng_state, double *out) {
+    return guarded([&] {
+        checked_medium(m);
+        if (nrays && (!rays || !rng_state || !out))
+            raise(AETHER_EINVAL, "null ray, rng or output array");
+        m->impl->transmittance(rays, nrays, rng_state, out);
+    });
+}
+
+int aether_fluid_medium_phase_hg_pdf(aether_fluid_medium_t m, const double *wo, const double *wi,
+                                     size_t n, double *pdf) {
+    return guarded([&] {
+        checked_medium(m);
+        if (n && (!wo || !wi || !pdf)) raise(AETHER_EINVAL, "null direction or output array");
+        m->impl->phase_pdf(wo, wi, n, pdf);
+    });
+}
+
+int aether_fluid_medium_sample_phase_hg(aether_fluid_medium_t m, const double *wo, size_t n,
+                                        uint64_t *rng_state, double *wi, double *pdf) {
+    return guarded([&] {
+        checked_medium(m);
+        if (n && (!wo || !rng_state || !wi || !pdf))
+            raise(AETHER_EINVAL, "null direction, rng or output array");
+        m->impl->phase_sample(wo, n, rng_state, wi, pdf);
     });
 }
 

@@ -191,7 +191,14 @@ class MediumBase {
    public:
     virtual ~MediumBase() = default;
     virtual void sample(const double *xyz, size_t npts, double *out) = 0;
-    double max_density = 0.0, majorant_sigma_t = 0.0, phase_g = 0.0;
+    virtual void interaction(const aether_medium_ray *rays, size_t n, const double *lambda,
+                             uint64_t *rng, aether_medium_interaction *out) = 0;
+    virtual void transmittance(const aether_medium_ray *rays, size_t n, uint64_t *rng,
+                               double *out) = 0;
+    virtual void phase_pdf(const double *wo, const double *wi, size_t n, double *pdf) = 0;
+    virtual void phase_sample(const double *wo, size_t n, uint64_t *rng, double *wi,
+                              double *pdf) = 0;
+    double max_density = 0.0, majorant_sigma_t = 0.0, phase_g = 0.0, sigma_t_scale = 0.0;
 };
 
 // One grid (FluidSolver3D, src/fluid.rs:23-673).

@@ -4,6 +4,7 @@
 #pragma once
 #include "host_common.hpp"
 #include "fluid_kernels.hpp"
+#include "volume_kernels.hpp"
 
 namespace aether {
 namespace {
@@ -39,6 +40,84 @@ class Medium final : public MediumBase {
             g_, box_, rho_.p, temp_.p, dxyz.p, (long)npts, dout.p);
         LAUNCH_CHECK();
         HIPT(hipMemcpyAsync(out, dout.p, npts * 2 * sizeof(double), hipMemcpyDeviceToHost, st_));
+        HIPT(hipStreamSynchronize(st_));
+    }
+
+    MediumDev dev() const {
+        MediumDev m;
+        m.g = g_;
+        m.box = box_;
+        m.sigma_t_scale = sigma_t_scale;
+        m.max_density = max_density;
+        m.majorant = majorant_sigma_t;
+        m.phase_g = phase_g;
+        return m;
+    }
+    static unsigned blocks(size_t n) { return (unsigned)((n + kThreads - 1) / kThreads); }
+
+    // fluid.rs:793-838 for a batch of rays (inputs copied in, outputs and
+    // the advanced RNG states copied back)
+    void interaction(const aether_medium_ray *rays, size_t n, const double *lambda,
+                     uint64_t *rng, aether_medium_interaction *out) override {
+        if (n == 0) return;
+        DevBuf<aether_medium_ray> dr(n);
+        DevBuf<double> dl(4 * n);
+        DevBuf<uint64_t> ds(4 * n);
+        DevBuf<aether_medium_interaction> dout(n);
+        HIPT(hipMemcpyAsync(dr.p, rays, n * sizeof(*rays), hipMemcpyHostToDevice, st_));
+        HIPT(hipMemcpyAsync(dl.p, lambda, 4 * n * sizeof(double), hipMemcpyHostToDevice, st_));
+        HIPT(hipMemcpyAsync(ds.p, rng, 4 * n * sizeof(uint64_t), hipMemcpyHostToDevice, st_));
+        k_medium_interaction<T><<<blocks(n), kThreads, 0, st_>>>(dev(), rho_.p, temp_.p, dr.p,
+                                                                 dl.p, (long)n, ds.p, dout.p);
+        LAUNCH_CHECK();
+        HIPT(hipMemcpyAsync(out, dout.p, n * sizeof(*out), hipMemcpyDeviceToHost, st_));
+        HIPT(hipMemcpyAsync(rng, ds.p, 4 * n * sizeof(uint64_t), hipMemcpyDeviceToHost, st_));
+        HIPT(hipStreamSynchronize(st_));
+    }
+
+    // fluid.rs:841-874
+    void transmittance(const aether_medium_ray *rays, size_t n, uint64_t *rng,
+                       double *out) override {
+        if (n == 0) return;
+        DevBuf<aether_medium_ray> dr(n);
+        DevBuf<uint64_t> ds(4 * n);
+        DevBuf<double> dout(n);
+        HIPT(hipMemcpyAsync(dr.p, rays, n * sizeof(*rays), hipMemcpyHostToDevice, st_));
+        HIPT(hipMemcpyAsync(ds.p, rng, 4 * n * sizeof(uint64_t), hipMemcpyHostToDevice, st_));
+        k_medium_transmittance<T><<<blocks(n), kThreads, 0, st_>>>(dev(), rho_.p, temp_.p, dr.p,
+                                                                   (long)n, ds.p, dout.p);
+        LAUNCH_CHECK();
+        HIPT(hipMemcpyAsync(out, dout.p, n * sizeof(double), hipMemcpyDeviceToHost, st_));
+        HIPT(hipMemcpyAsync(rng, ds.p, 4 * n * sizeof(uint64_t), hipMemcpyDeviceToHost, st_));
+        HIPT(hipStreamSynchronize(st_));
+    }
+
+    // fluid.rs:878-888
+    void phase_pdf(const double *wo, const double *wi, size_t n, double *pdf) override {
+        if (n == 0) return;
+        DevBuf<double> dwo(3 * n), dwi(3 * n), dp(n);
+        HIPT(hipMemcpyAsync(dwo.p, wo, 3 * n * sizeof(double), hipMemcpyHostToDevice, st_));
+        HIPT(hipMemcpyAsync(dwi.p, wi, 3 * n * sizeof(double), hipMemcpyHostToDevice, st_));
+        k_phase_pdf<<<blocks(n), kThreads, 0, st_>>>(phase_g, dwo.p, dwi.p, (long)n, dp.p);
+        LAUNCH_CHECK();
+        HIPT(hipMemcpyAsync(pdf, dp.p, n * sizeof(double), hipMemcpyDeviceToHost, st_));
+        HIPT(hipStreamSynchronize(st_));
+    }
+
+    // fluid.rs:892-910
+    void phase_sample(const double *wo, size_t n, uint64_t *rng, double *wi,
+                      double *pdf) override {
+        if (n == 0) return;
+        DevBuf<double> dwo(3 * n), dwi(3 * n), dp(n);
+        DevBuf<uint64_t> ds(4 * n);
+        HIPT(hipMemcpyAsync(dwo.p, wo, 3 * n * sizeof(double), hipMemcpyHostToDevice, st_));
+        HIPT(hipMemcpyAsync(ds.p, rng, 4 * n * sizeof(uint64_t), hipMemcpyHostToDevice, st_));
+        k_phase_sample<<<blocks(n), kThreads, 0, st_>>>(phase_g, dwo.p, (long)n, ds.p, dwi.p,
+                                                        dp.p);
+        LAUNCH_CHECK();
+        HIPT(hipMemcpyAsync(wi, dwi.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost, st_));
+        HIPT(hipMemcpyAsync(pdf, dp.p, n * sizeof(double), hipMemcpyDeviceToHost, st_));
+        HIPT(hipMemcpyAsync(rng, ds.p, 4 * n * sizeof(uint64_t), hipMemcpyDeviceToHost, st_));
         HIPT(hipStreamSynchronize(st_));
     }
 

@@ -243,6 +243,44 @@ int aether_fluid_medium_info(aether_fluid_medium_t m, double *max_density,
 int aether_fluid_medium_sample(aether_fluid_medium_t m, const double *xyz, size_t npts,
                                double *out);
 
+/* FluidMedium's volumetric sampling for the path tracer (src/fluid.rs:790-910;
+ * called per ray by src/renderer.rs:231-306), on the device in FP64 for a
+ * batch of rays.  Every ray has its own Xoshiro256++ state (4 words, the
+ * reference's Rng, src/rng.rs:14-77) in rng_state[4*i .. 4*i+3]: it is read,
+ * advanced by exactly the draws the reference's call makes, and written
+ * back.  A ray is Ray::new's (src/math.rs:763-773): dir already normalised
+ * (inv_dir = 1/dir is formed here), t_min, and t_far = the call's
+ * surface_t / max_dist. */
+typedef struct {
+    double origin[3];
+    double dir[3];
+    double t_min;
+    double t_far;
+} aether_medium_ray;
+/* MediumInteraction (src/fluid.rs:680-685); hit = 0 for None. */
+typedef struct {
+    int32_t hit;
+    int32_t reserved;
+    double t;
+    double point[3];
+    double albedo[4];   /* scatter_albedo.eval4 at the ray's wavelengths */
+    double emission[4]; /* hot_core_emission.eval4 * glow */
+} aether_medium_interaction;
+/* fluid.rs:793-838 sample_interaction (Woodcock delta tracking).  lambda:
+ * 4 wavelengths (nm) per ray, the ray's Wavelengths4. */
+int aether_fluid_medium_sample_interaction(aether_fluid_medium_t m, const aether_medium_ray *rays,
+                                           size_t nrays, const double *lambda,
+                                           uint64_t *rng_state, aether_medium_interaction *out);
+/* fluid.rs:841-874 transmittance (ratio tracking, Russian roulette). */
+int aether_fluid_medium_transmittance(aether_fluid_medium_t m, const aether_medium_ray *rays,
+                                      size_t nrays, uint64_t *rng_state, double *out);
+/* fluid.rs:878-888 phase_hg_pdf(wo, wi) for n direction pairs (x,y,z each). */
+int aether_fluid_medium_phase_hg_pdf(aether_fluid_medium_t m, const double *wo, const double *wi,
+                                     size_t n, double *pdf);
+/* fluid.rs:892-910 sample_phase_hg(wo): wi (3 per direction) and its pdf. */
+int aether_fluid_medium_sample_phase_hg(aether_fluid_medium_t m, const double *wo, size_t n,
+                                        uint64_t *rng_state, double *wi, double *pdf);
+
 /* Seeded inputs: the reference's SplitMix64-seeded Xoshiro256++ stream
  * (src/rng.rs:21-33, 48-64, 74-88).  Fills out[0..n) with range_f64(lo,hi)
  * draws continuing from *state (4 words), which is advanced. */

@@ -4,6 +4,8 @@
  * A plain-C, f64 restatement of the reference smoke solver `FluidSolver3D`
  * (src/fluid.rs:23-673) and of its seeded RNG (src/rng.rs:5-88).  It is the
  * parity checker for the HIP library and the CPU baseline timed by bench.py.
+ * of_medium_* restate FluidMedium's volumetric sampling (src/fluid.rs:675-910)
+ * as the checker of the device sampler.
  * of_soundtrack restates the telemetry-driven audio synthesis and 24-bit
  * dithered encoder (src/audio.rs:1-639) so that telemetry can be checked
  * against the reference's rendered soundtrack; nothing else uses it.
@@ -750,6 +752,211 @@ double orng_range_f64(ORng *r, double lo, double hi) { /* rng.rs:86-88 */
 uint64_t of_splitmix64_next(uint64_t *state) { return splitmix64(state); }
 
 /* ------------------------------------------------------------------ */
+/* FluidMedium volumetric sampling (fluid.rs:793-910), with the math.rs  */
+/* pieces it calls: Vec3::dot/normalize (math.rs:34-65, dot is fused     */
+/* multiply-adds), Ray::at (math.rs:776-778), Aabb::intersect            */
+/* (math.rs:859-893), Onb::from_normal/to_world (math.rs:727-740),       */
+/* SpectralCurve::eval_at (math.rs:1148-1170), Rng::uniform_sphere       */
+/* (rng.rs:133-138).  The medium is the oracle's current density and     */
+/* temperature (FluidMedium::from_solver clones them, fluid.rs:703-735). */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+    double lo[3], hi[3];   /* bounds */
+    double sigma_t_scale;
+    double max_density;
+    double majorant;       /* max(max_density * sigma_t_scale, 1e-4) */
+    double phase_g;        /* clamped to [-0.9, 0.9] */
+} OMedium;
+
+/* one ray of Ray::new: dir normalised, inv_dir = 1 / dir per axis */
+typedef struct {
+    double o[3], d[3], inv[3], t_min;
+} ORay;
+
+static const double O_PI = 3.14159265358979323846264338327950288;
+static const double O_TAU = 6.28318530717958647692528676655900577;
+
+static double vdot(const double *a, const double *b) { /* math.rs:34-36 */
+    return fma(a[0], b[0], fma(a[1], b[1], a[2] * b[2]));
+}
+
+static void vnormalize(const double *v, double *out) { /* math.rs:58-65 */
+    const double len = sqrt(vdot(v, v));
+    if (len > 1e-18) {
+        const double inv = 1.0 / len;
+        out[0] = v[0] * inv;
+        out[1] = v[1] * inv;
+        out[2] = v[2] * inv;
+    } else {
+        out[0] = 0.0;
+        out[1] = 0.0;
+        out[2] = 1.0;
+    }
+}
+
+void of_ray_make(const double *o, const double *d, double t_min, ORay *r) { /* math.rs:763-773 */
+    double n[3];
+    vnormalize(d, n);
+    for (int a = 0; a < 3; ++a) {
+        r->o[a] = o[a];
+        r->d[a] = n[a];
+        r->inv[a] = 1.0 / n[a];
+    }
+    r->t_min = t_min;
+}
+
+/* math.rs:859-893; f64::min/max ignore a NaN operand like fmin/fmax */
+static int aabb_intersect(const OMedium *m, const ORay *r, double t_min, double t_max,
+                          double *t0, double *t1) {
+    for (int a = 0; a < 3; ++a) {
+        const double ta = (m->lo[a] - r->o[a]) * r->inv[a];
+        const double tb = (m->hi[a] - r->o[a]) * r->inv[a];
+        const double lo = fmin(ta, tb), hi = fmax(ta, tb);
+        t_min = fmax(t_min, lo);
+        t_max = fmin(t_max, hi);
+        if (a < 2 && t_max < t_min) return 0;
+    }
+    if (t_max >= t_min) {
+        *t0 = t_min;
+        *t1 = t_max;
+        return 1;
+    }
+    return 0;
+}
+
+/* SpectralCurve::eval_at (math.rs:1148-1170): kind 0 = SmoothRgb(c),
+ * 1 = NeonDischarge with FluidMedium's constants (fluid.rs:723-731) */
+static double spectral_eval(int kind, double lambda) {
+    const double w_b = 1.0 / (1.0 + exp((lambda - 492.0) / 14.0));
+    const double w_r = 1.0 / (1.0 + exp(-(lambda - 586.0) / 14.0));
+    const double w_g = fmax(1.0 - w_b - w_r, 0.0);
+    if (kind == 0) return 0.84 * w_r + 0.90 * w_g + 0.96 * w_b; /* fluid.rs:721 */
+    const double base = 0.18 * w_r + 0.72 * w_g + 1.15 * w_b;
+    const double e1 = (lambda - 486.0) / 14.0, e2 = (lambda - 618.0) / 16.0;
+    const double g1 = 1.6 * exp(-0.5 * (e1 * e1));
+    const double g2 = 1.1 * exp(-0.5 * (e2 * e2));
+    return base + g1 + g2;
+}
+
+/* fluid.rs:739-788 at p, for the medium's own bounds */
+static void medium_at(const OFluid *s, const OMedium *m, const double *p, double *rho,
+                      double *temp) {
+    of_medium_sample(s, m->lo, m->hi, p[0], p[1], p[2], rho, temp);
+}
+
+/* fluid.rs:793-838 sample_interaction.  out: [hit, t, px, py, pz, albedo x4,
+ * emission x4] (14 doubles); the RNG is advanced as the reference's. */
+void of_medium_sample_interaction(const OFluid *s, const OMedium *m, const ORay *r,
+                                  double surface_t, const double *lambda, ORng *rng,
+                                  double *out) {
+    for (int a = 0; a < 14; ++a) out[a] = 0.0;
+    if (m->max_density <= 1e-6) return;
+    double t_enter, t_exit;
+    if (!aabb_intersect(m, r, r->t_min, surface_t, &t_enter, &t_exit)) return;
+    double t = fmax(t_enter, r->t_min);
+    const double limit = fmin(t_exit, surface_t);
+    if (t >= limit) return;
+    const double inv_maj = 1.0 / m->majorant;
+    for (int it = 0; it < 256; ++it) {
+        const double u = fmax(1.0 - orng_next_f64(rng), 1e-15);
+        t -= log(u) * inv_maj;
+        if (t >= limit) return;
+        const double p[3] = {r->o[0] + r->d[0] * t, r->o[1] + r->d[1] * t,
+                             r->o[2] + r->d[2] * t};
+        double rho, temp;
+        medium_at(s, m, p, &rho, &temp);
+        const double sigma_t = rho * m->sigma_t_scale;
+        if (orng_next_f64(rng) * m->majorant < sigma_t) {
+            out[0] = 1.0;
+            out[1] = t;
+            out[2] = p[0];
+            out[3] = p[1];
+            out[4] = p[2];
+            const double gb = fmax(temp - 0.35, 0.0);
+            const double glow = gb * gb * 0.95;
+            for (int w = 0; w < 4; ++w) {
+                out[5 + w] = spectral_eval(0, lambda[w]);
+                out[9 + w] = glow > 1e-5 ? spectral_eval(1, lambda[w]) * glow : 0.0;
+            }
+            return;
+        }
+    }
+}
+
+/* fluid.rs:841-874 transmittance (ratio tracking with Russian roulette) */
+double of_medium_transmittance(const OFluid *s, const OMedium *m, const ORay *r, double max_dist,
+                               ORng *rng) {
+    if (m->max_density <= 1e-6) return 1.0;
+    double t_enter, t_exit;
+    if (!aabb_intersect(m, r, r->t_min, max_dist, &t_enter, &t_exit)) return 1.0;
+    double t = fmax(t_enter, r->t_min);
+    const double limit = fmin(t_exit, max_dist);
+    if (t >= limit) return 1.0;
+    double tr = 1.0;
+    const double inv_maj = 1.0 / m->majorant;
+    for (int it = 0; it < 256; ++it) {
+        const double u = fmax(1.0 - orng_next_f64(rng), 1e-15);
+        t -= log(u) * inv_maj;
+        if (t >= limit) break;
+        const double p[3] = {r->o[0] + r->d[0] * t, r->o[1] + r->d[1] * t,
+                             r->o[2] + r->d[2] * t};
+        double rho, temp;
+        medium_at(s, m, p, &rho, &temp);
+        const double sigma_t = fmin(rho * m->sigma_t_scale, m->majorant);
+        tr *= 1.0 - sigma_t * inv_maj;
+        if (tr < 0.05) {
+            const double q = fmin(1.0 - tr, 0.95);
+            if (orng_next_f64(rng) < q) return 0.0;
+            tr /= 1.0 - q;
+        }
+    }
+    return rclamp(tr, 0.0, 1.0);
+}
+
+/* fluid.rs:878-888 phase_hg_pdf */
+double of_phase_hg_pdf(double g, const double *wo, const double *wi) {
+    if (fabs(g) < 1e-4) return 1.0 / (4.0 * O_PI);
+    const double nwo[3] = {-wo[0], -wo[1], -wo[2]};
+    const double cos_theta = rclamp(vdot(nwo, wi), -1.0, 1.0);
+    const double denom = fmax(1.0 + g * g - 2.0 * g * cos_theta, 1e-12);
+    return (1.0 - g * g) / (4.0 * O_PI * denom * sqrt(denom));
+}
+
+/* fluid.rs:892-910 sample_phase_hg; wi_pdf: wi x3, pdf */
+void of_sample_phase_hg(double g, const double *wo, ORng *rng, double *wi_pdf) {
+    if (fabs(g) < 1e-4) { /* rng.rs:133-138 */
+        const double z = 1.0 - 2.0 * orng_next_f64(rng);
+        const double rr = sqrt(fmax(1.0 - z * z, 0.0));
+        const double phi = O_TAU * orng_next_f64(rng);
+        wi_pdf[0] = rr * cos(phi);
+        wi_pdf[1] = rr * sin(phi);
+        wi_pdf[2] = z;
+        wi_pdf[3] = 1.0 / (4.0 * O_PI);
+        return;
+    }
+    const double sv = 2.0 * orng_next_f64(rng) - 1.0;
+    const double sqr = (1.0 - g * g) / (1.0 + g * sv);
+    const double cos_theta = rclamp((1.0 + g * g - sqr * sqr) / (2.0 * g), -1.0, 1.0);
+    const double sin_theta = sqrt(fmax(1.0 - cos_theta * cos_theta, 0.0));
+    const double phi = O_TAU * orng_next_f64(rng);
+    const double nwo[3] = {-wo[0], -wo[1], -wo[2]};
+    double fwd[3], w[3];
+    vnormalize(nwo, fwd);
+    vnormalize(fwd, w); /* Onb::from_normal normalises again (math.rs:728) */
+    const double sign = copysign(1.0, w[2]);
+    const double a = -1.0 / (sign + w[2]);
+    const double b = w[0] * w[1] * a;
+    const double u[3] = {1.0 + sign * w[0] * w[0] * a, sign * b, -sign * w[0]};
+    const double v[3] = {b, sign + w[1] * w[1] * a, -w[1]};
+    const double lx = sin_theta * cos(phi), ly = sin_theta * sin(phi), lz = cos_theta;
+    const double world[3] = {u[0] * lx + v[0] * ly + w[0] * lz, u[1] * lx + v[1] * ly + w[1] * lz,
+                             u[2] * lx + v[2] * ly + w[2] * lz};
+    vnormalize(world, wi_pdf);
+    wi_pdf[3] = of_phase_hg_pdf(g, wo, wi_pdf);
+}
+
+/* ------------------------------------------------------------------ */
 /* Telemetry -> soundtrack (audio.rs:1-639), restated to pin the whole  */
 /* step() + compute_telemetry chain against the reference's rendered    */
 /* renders/simulation_soundtrack.wav (main.rs:82-94, 166-168).          */
@@ -1156,4 +1363,67 @@ void of_soundtrack_ex(const double *frames, long n, double max_ke_in, double max
 
 void of_soundtrack(const double *frames, long n, int32_t *pcm) {
     of_soundtrack_ex(frames, n, 0.0, 0.0, AU_FRAMES, pcm);
+}
+
+/* Batched forms of the volumetric samplers above, for the parity tests: ray
+ * a has origin o[3a..], direction d[3a..] (Ray::dir, already normalised;
+ * inv_dir = 1/dir as Ray::new forms it), t_min[a], far limit t_far[a], and
+ * its own RNG state states[4a..4a+3], advanced in place. */
+static void batch_ray(const double *o, const double *d, const double *t_min, long a, ORay *r) {
+    for (int q = 0; q < 3; ++q) {
+        r->o[q] = o[3 * a + q];
+        r->d[q] = d[3 * a + q];
+        r->inv[q] = 1.0 / d[3 * a + q];
+    }
+    r->t_min = t_min[a];
+}
+
+void of_medium_interaction_batch(const OFluid *s, const OMedium *m, long n, const double *o,
+                                 const double *d, const double *t_min, const double *t_far,
+                                 const double *lambda, uint64_t *states, double *out) {
+    for (long a = 0; a < n; ++a) {
+        ORay r;
+        ORng g;
+        batch_ray(o, d, t_min, a, &r);
+        memcpy(g.s, states + 4 * a, sizeof(g.s));
+        of_medium_sample_interaction(s, m, &r, t_far[a], lambda + 4 * a, &g, out + 14 * a);
+        memcpy(states + 4 * a, g.s, sizeof(g.s));
+    }
+}
+
+void of_medium_transmittance_batch(const OFluid *s, const OMedium *m, long n, const double *o,
+                                   const double *d, const double *t_min, const double *t_far,
+                                   uint64_t *states, double *out) {
+    for (long a = 0; a < n; ++a) {
+        ORay r;
+        ORng g;
+        batch_ray(o, d, t_min, a, &r);
+        memcpy(g.s, states + 4 * a, sizeof(g.s));
+        out[a] = of_medium_transmittance(s, m, &r, t_far[a], &g);
+        memcpy(states + 4 * a, g.s, sizeof(g.s));
+    }
+}
+
+void of_phase_sample_batch(double g, long n, const double *wo, uint64_t *states, double *wi,
+                           double *pdf) {
+    for (long a = 0; a < n; ++a) {
+        ORng r;
+        double out[4];
+        memcpy(r.s, states + 4 * a, sizeof(r.s));
+        of_sample_phase_hg(g, wo + 3 * a, &r, out);
+        memcpy(states + 4 * a, r.s, sizeof(r.s));
+        wi[3 * a] = out[0];
+        wi[3 * a + 1] = out[1];
+        wi[3 * a + 2] = out[2];
+        pdf[a] = out[3];
+    }
+}
+
+void of_phase_pdf_batch(double g, long n, const double *wo, const double *wi, double *pdf) {
+    for (long a = 0; a < n; ++a) pdf[a] = of_phase_hg_pdf(g, wo + 3 * a, wi + 3 * a);
+}
+
+/* Vec3::normalize (math.rs:58-65) of n vectors, as Ray::new applies to dir */
+void of_normalize_batch(long n, const double *v, double *out) {
+    for (long a = 0; a < n; ++a) vnormalize(v + 3 * a, out + 3 * a);
 }

@@ -74,6 +74,11 @@ def _load(omp: bool = False) -> C.CDLL:
         "orng_next_f64": (d, [vp]),
         "orng_range_f64": (d, [vp, d, d]),
         "of_splitmix64_next": (C.c_uint64, [C.POINTER(C.c_uint64)]),
+        "of_medium_interaction_batch": (None, [vp, vp, l, pd, pd, pd, pd, pd, vp, pd]),
+        "of_medium_transmittance_batch": (None, [vp, vp, l, pd, pd, pd, pd, vp, pd]),
+        "of_phase_sample_batch": (None, [d, l, pd, vp, pd, pd]),
+        "of_phase_pdf_batch": (None, [d, l, pd, pd, pd]),
+        "of_normalize_batch": (None, [l, pd, pd]),
     }
     for fn, (res, args) in sig.items():
         f = getattr(lib, fn)
@@ -241,6 +246,41 @@ class OracleFluid:
             out[a] = (dv.value, tv.value)
         return out
 
+    # --- FluidMedium volumetric sampling (fluid.rs:793-910) ---------------
+    def _omedium(self, bmin, bmax, sigma_t_scale, phase_g):
+        """FluidMedium::from_solver's parameters (fluid.rs:703-735) over the
+        oracle's current density and temperature."""
+        md = self.max_density()
+        return (C.c_double * 10)(*bmin, *bmax, sigma_t_scale, md,
+                                 max(md * sigma_t_scale, 1e-4), min(max(phase_g, -0.9), 0.9))
+
+    def medium_interaction(self, bmin, bmax, sigma_t_scale, phase_g, origins, dirs, t_min,
+                           t_far, lambdas, states):
+        """sample_interaction for each ray (dirs already normalised, as
+        Ray::dir); states (n, 4) uint64 are advanced in place.  Returns an
+        (n, 14) array: hit, t, point xyz, albedo x4, emission x4."""
+        m = self._omedium(bmin, bmax, sigma_t_scale, phase_g)
+        o, dd, t0, t1, lam = (np.ascontiguousarray(a, dtype=np.float64)
+                              for a in (origins, dirs, t_min, t_far, lambdas))
+        assert states.dtype == np.uint64 and states.flags.c_contiguous
+        out = np.empty((len(t0), 14))
+        self._lib.of_medium_interaction_batch(self._h, C.cast(m, C.c_void_p), len(t0), _ptr(o),
+                                              _ptr(dd), _ptr(t0), _ptr(t1), _ptr(lam),
+                                              states.ctypes.data, _ptr(out))
+        return out
+
+    def medium_transmittance(self, bmin, bmax, sigma_t_scale, phase_g, origins, dirs, t_min,
+                             t_far, states):
+        m = self._omedium(bmin, bmax, sigma_t_scale, phase_g)
+        o, dd, t0, t1 = (np.ascontiguousarray(a, dtype=np.float64)
+                         for a in (origins, dirs, t_min, t_far))
+        assert states.dtype == np.uint64 and states.flags.c_contiguous
+        out = np.empty(len(t0))
+        self._lib.of_medium_transmittance_batch(self._h, C.cast(m, C.c_void_p), len(t0), _ptr(o),
+                                                _ptr(dd), _ptr(t0), _ptr(t1), states.ctypes.data,
+                                                _ptr(out))
+        return out
+
     def apply_laplacian(self, x: np.ndarray) -> np.ndarray:
         x = np.ascontiguousarray(x, dtype=np.float64).reshape(self.nz, self.ny, self.nx)
         out = np.zeros_like(x)
@@ -255,6 +295,36 @@ class OracleFluid:
                            C.byref(conv))
         return {"iters": it.value, "b2": b2.value, "r2_init": r0.value, "r2": r2.value,
                 "converged": bool(conv.value)}
+
+    @staticmethod
+    def phase_sample(g, wo, states):
+        """sample_phase_hg (fluid.rs:892-910) for each row of wo with phase_g
+        g (already clamped); returns (wi (n, 3), pdf (n))."""
+        lib = _load(False)
+        wo = np.ascontiguousarray(wo, dtype=np.float64)
+        wi, pdf = np.empty_like(wo), np.empty(len(wo))
+        assert states.dtype == np.uint64 and states.flags.c_contiguous
+        lib.of_phase_sample_batch(float(g), len(wo), _ptr(wo), states.ctypes.data, _ptr(wi),
+                                  _ptr(pdf))
+        return wi, pdf
+
+    @staticmethod
+    def phase_pdf(g, wo, wi):
+        lib = _load(False)
+        wo = np.ascontiguousarray(wo, dtype=np.float64)
+        wi = np.ascontiguousarray(wi, dtype=np.float64)
+        pdf = np.empty(len(wo))
+        lib.of_phase_pdf_batch(float(g), len(wo), _ptr(wo), _ptr(wi), _ptr(pdf))
+        return pdf
+
+    @staticmethod
+    def normalize(v):
+        """Vec3::normalize (math.rs:58-65) of each row: Ray::new's dir."""
+        lib = _load(False)
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        out = np.empty_like(v)
+        lib.of_normalize_batch(len(v), _ptr(v), _ptr(out))
+        return out
 
     @staticmethod
     def num_threads(omp: bool = False) -> int:

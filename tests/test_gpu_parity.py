@@ -1374,3 +1374,123 @@ def test_adjoint_fp64_rollout_64_real_size_matches_central_differences():
     print("max fp64", max(e64), "max fp32", max(e32), "median fp32", np.median(e32), flush=True)
     assert max(e64) <= 1e-5, e64
     assert max(e32) <= 1e-4, e32
+
+
+def _volume_rays(n, seed):
+    """n seeded rays for the volumetric samplers: origins around and inside
+    the box [-1, 1]^3, directions normalised as Ray::new does (the oracle's
+    Vec3::normalize), with edge cases: axis-parallel directions (infinite
+    inv_dir), origins on a slab plane (0 * inf), far limits before the entry
+    and t_min past the box."""
+    from oracle.oracle import OracleFluid
+    rng = np.random.default_rng(seed)
+    org = rng.uniform(-2.0, 2.0, (n, 3))
+    d = rng.normal(size=(n, 3))
+    d[0:200, 1:] = 0.0          # along +-x
+    d[200:400, :2] = 0.0        # along +-z
+    org[400:600, 0] = -1.0      # on the x = -1 plane
+    d[400:600, 0] = 0.0         # ... and parallel to it
+    org[600:800] = rng.uniform(-0.9, 0.9, (200, 3))  # inside
+    d = OracleFluid.normalize(d)
+    t_min = rng.uniform(0.0, 0.05, n)
+    t_min[800:900] = 50.0       # past the box
+    t_far = rng.uniform(0.5, 6.0, n)
+    t_far[900:1000] = 1e-3      # before the entry
+    lam = rng.uniform(380.0, 750.0, (n, 4))
+    return org, d, t_min, t_far, lam
+
+
+def _volume_pair(precision, empty=False):
+    import aether_fluid as af
+    from oracle.oracle import OracleFluid
+    dims = (24, 20, 28)
+    rng = np.random.default_rng(9)
+    k, j, i = np.meshgrid(np.arange(dims[2]), np.arange(dims[1]), np.arange(dims[0]),
+                          indexing="ij")
+    dens = (0.7 + 0.3 * np.sin(0.4 * i) * np.cos(0.3 * j) * np.cos(0.25 * k)
+            + 0.1 * rng.random(i.shape)).ravel()
+    temp = (1.2 * np.exp(-((i - 12) ** 2 + (j - 6) ** 2 + (k - 14) ** 2) / 40.0)).ravel()
+    if empty:
+        dens[:] = 0.0
+    s = af.FluidSolver3D(*dims, precision=precision)
+    s.upload("density", dens)
+    s.upload("temperature", temp)
+    o = OracleFluid(*dims)
+    # the medium snapshots the solver's storage: the oracle gets those values
+    o.field("density")[...] = s.download("density").reshape(dims[2], dims[1], dims[0])
+    o.field("temperature")[...] = s.download("temperature").reshape(dims[2], dims[1], dims[0])
+    return s, o
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_volumetric_sampling_vs_oracle(precision):
+    """FluidMedium's path-tracer queries on the device (src/fluid.rs:793-910)
+    against the oracle's restatement over 10^5 seeded rays, each with its own
+    Xoshiro256++ state: delta tracking (hit or None, t, point, albedo,
+    emission), ratio tracking, and HG phase sampling and pdf.  The final RNG
+    states are equal, so every ray made the reference's draws; values agree
+    to the device/host libm (log, exp, sin, cos) rounding, rtol 1e-12."""
+    import aether_fluid as af
+    from oracle.oracle import OracleFluid
+    _announce(f"volumetric sampling vs oracle ({precision}), 1e5 rays")
+    s, o = _volume_pair(precision)
+    bounds = ((-1.0, -1.0, -1.0), (1.0, 1.0, 1.0))
+    scale, g = 2.5, 0.55
+    med = af.FluidMedium.from_solver(s, bounds, scale, g)
+    assert med.max_density == o.max_density()
+    n = 100000
+    org, d, t_min, t_far, lam = _volume_rays(n, 17)
+    rays = af.FluidMedium.make_rays(org, d, t_min, t_far)
+    from oracle.oracle import splitmix64_stream
+    st0 = np.array(splitmix64_stream(0xAE7E0005, 4 * n), dtype=np.uint64).reshape(n, 4)
+
+    st_g, st_o = st0.copy(), st0.copy()
+    got = med.sample_interaction(rays, lam, st_g)
+    exp = o.medium_interaction(*bounds, scale, g, org, d, t_min, t_far, lam, st_o)
+    np.testing.assert_array_equal(st_g, st_o)
+    np.testing.assert_array_equal(got["hit"], exp[:, 0].astype(np.int32))
+    h = got["hit"] == 1
+    assert 0.05 < h.mean() < 0.95, h.mean()  # both outcomes well represented
+    np.testing.assert_allclose(got["t"][h], exp[h, 1], rtol=1e-12)
+    np.testing.assert_allclose(got["point"][h], exp[h, 2:5], rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(got["albedo"][h], exp[h, 5:9], rtol=1e-12)
+    np.testing.assert_allclose(got["emission"][h], exp[h, 9:13], rtol=1e-12, atol=1e-300)
+    assert (got["emission"][h] > 0).any() and (got["emission"][h] == 0).any()
+
+    st_g, st_o = st0.copy(), st0.copy()
+    tg = med.transmittance(rays, st_g)
+    to = o.medium_transmittance(*bounds, scale, g, org, d, t_min, t_far, st_o)
+    np.testing.assert_array_equal(st_g, st_o)
+    np.testing.assert_allclose(tg, to, rtol=1e-12, atol=1e-300)
+    assert (tg == 0).any() and (tg == 1).any() and ((tg > 0) & (tg < 1)).any()
+
+    wo = OracleFluid.normalize(np.random.default_rng(4).normal(size=(n, 3)))
+    for gg in (g, 0.0):
+        med_g = af.FluidMedium.from_solver(s, bounds, scale, gg)
+        st_g, st_o = st0.copy(), st0.copy()
+        wi_g, pdf_g = med_g.sample_phase_hg(wo, st_g)
+        wi_o, pdf_o = OracleFluid.phase_sample(gg, wo, st_o)
+        np.testing.assert_array_equal(st_g, st_o)
+        np.testing.assert_allclose(wi_g, wi_o, rtol=1e-12, atol=1e-13)
+        np.testing.assert_allclose(pdf_g, pdf_o, rtol=1e-12)
+        np.testing.assert_allclose(med_g.phase_hg_pdf(wo, wi_o),
+                                   OracleFluid.phase_pdf(gg, wo, wi_o), rtol=1e-14)
+        med_g.close()
+    med.close()
+
+
+def test_volumetric_sampling_empty_medium_draws_nothing():
+    """max_density <= 1e-6 (src/fluid.rs:799, 842): no interaction,
+    transmittance 1, and no RNG draw."""
+    import aether_fluid as af
+    _announce("volumetric sampling, empty medium")
+    s, o = _volume_pair("fp64", empty=True)
+    med = af.FluidMedium.from_solver(s, ((-1, -1, -1), (1, 1, 1)), 2.5, 0.5)
+    n = 1000
+    org, d, t_min, t_far, lam = _volume_rays(n, 3)
+    rays = af.FluidMedium.make_rays(org, d, t_min, t_far)
+    st = np.arange(4 * n, dtype=np.uint64).reshape(n, 4) + 1
+    before = st.copy()
+    assert not med.sample_interaction(rays, lam, st)["hit"].any()
+    np.testing.assert_array_equal(med.transmittance(rays, st), 1.0)
+    np.testing.assert_array_equal(st, before)

@@ -501,3 +501,148 @@ def test_oracle_under_address_and_undefined_behaviour_sanitizers():
     assert res.returncode == 0, res.stderr[-4000:]
     assert "runtime error" not in res.stderr and "AddressSanitizer" not in res.stderr
     assert "known answer 2.5751e+00 6.7289e-03" in res.stdout
+
+
+# --- FluidMedium volumetric sampling (src/fluid.rs:793-910) -----------------
+# The oracle restates delta tracking, ratio tracking and Henyey-Greenstein
+# sampling; these checks pin it by what the mathematics fixes: ratio tracking
+# is unbiased (E[T] = exp(-tau)), delta tracking's free flights follow
+# P(no interaction before s) = exp(-tau(s)), HG samples follow the HG law
+# (<cos> = g, its CDF), the pdf integrates to 1, and the spectral curves
+# reach their channel values at the ends of the visible range.
+
+def _vol_medium(seed=3):
+    from oracle.oracle import OracleFluid
+    o = OracleFluid(12, 10, 14)
+    rng = np.random.default_rng(seed)
+    k, j, i = np.meshgrid(np.arange(14), np.arange(10), np.arange(12), indexing="ij")
+    o.field("density")[...] = 0.6 + 0.3 * np.sin(0.7 * i) * np.cos(0.5 * j) + 0.05 * rng.random(
+        (14, 10, 12))
+    o.field("temperature")[...] = 0.8 * np.exp(-((i - 6) ** 2 + (k - 7) ** 2) / 18.0)
+    return o
+
+
+def _tau(o, bmin, bmax, scale, org, d, t0, t1, n=40001):
+    """Optical depth along the ray by Simpson's rule on the pinned
+    sample_density_and_temp."""
+    ts = np.linspace(t0, t1, n)
+    pts = org[None, :] + d[None, :] * ts[:, None]
+    sig = o.medium_sample(bmin, bmax, pts)[:, 0] * scale
+    w = np.ones(n)
+    w[1:-1:2], w[2:-1:2] = 4, 2
+    return (t1 - t0) / (3 * (n - 1)) * float(w @ sig)
+
+
+def _seeded_states(n, seed):
+    from oracle.oracle import splitmix64_stream
+    return np.array(splitmix64_stream(seed, 4 * n), dtype=np.uint64).reshape(n, 4)
+
+
+def test_oracle_ratio_tracking_is_unbiased():
+    from oracle.oracle import OracleFluid
+    o = _vol_medium()
+    bmin, bmax, scale = (-1.0, -1.0, -1.0), (1.0, 1.0, 1.0), 1.5
+    org = np.array([-1.5, -0.2, 0.1])
+    d = OracleFluid.normalize(np.array([[1.0, 0.15, -0.1]]))[0]
+    n = 40000
+    states = _seeded_states(n, 11)
+    far = 10.0
+    tr = o.medium_transmittance(bmin, bmax, scale, 0.0, np.tile(org, (n, 1)), np.tile(d, (n, 1)),
+                                np.zeros(n), np.full(n, far), states)
+    # the ray enters at x = -1 and leaves at x = 1 (t = (x - ox) / dx)
+    t0, t1 = (-1.0 - org[0]) / d[0], (1.0 - org[0]) / d[0]
+    expect = np.exp(-_tau(o, bmin, bmax, scale, org, d, t0, t1))
+    se = tr.std() / np.sqrt(n)
+    assert 0.1 < expect < 0.9
+    assert abs(tr.mean() - expect) < 5 * se, (tr.mean(), expect, se)
+    assert tr.min() >= 0.0 and tr.max() <= 1.0
+    # a ray that misses the box, and a medium without density, draw nothing
+    st2 = _seeded_states(3, 12)
+    before = st2.copy()
+    miss = o.medium_transmittance(bmin, bmax, scale, 0.0, np.tile([5.0, 5.0, 5.0], (3, 1)),
+                                  np.tile(d, (3, 1)), np.zeros(3), np.full(3, far), st2)
+    np.testing.assert_array_equal(miss, 1.0)
+    np.testing.assert_array_equal(st2, before)
+
+
+def test_oracle_delta_tracking_free_flight_law():
+    from oracle.oracle import OracleFluid
+    o = _vol_medium()
+    bmin, bmax, scale = (-1.0, -1.0, -1.0), (1.0, 1.0, 1.0), 1.5
+    org = np.array([-1.5, -0.2, 0.1])
+    d = OracleFluid.normalize(np.array([[1.0, 0.15, -0.1]]))[0]
+    n = 40000
+    states = _seeded_states(n, 21)
+    lam = np.tile([400.0, 500.0, 600.0, 700.0], (n, 1))
+    t0, t1 = (-1.0 - org[0]) / d[0], (1.0 - org[0]) / d[0]
+    for surface in (10.0, 0.5 * (t0 + t1)):
+        out = o.medium_interaction(bmin, bmax, scale, 0.0, np.tile(org, (n, 1)),
+                                   np.tile(d, (n, 1)), np.zeros(n), np.full(n, surface), lam,
+                                   states)
+        hit = out[:, 0] == 1.0
+        lim = min(t1, surface)
+        p_none = np.exp(-_tau(o, bmin, bmax, scale, org, d, t0, lim))
+        se = np.sqrt(p_none * (1 - p_none) / n)
+        assert abs((~hit).mean() - p_none) < 5 * se, ((~hit).mean(), p_none)
+        th = out[hit, 1]
+        assert th.min() >= t0 and th.max() < lim
+        np.testing.assert_allclose(out[hit, 2:5], org + d * th[:, None], rtol=0, atol=1e-12)
+        # the CDF of the interaction distance at the middle of the segment
+        s = 0.5 * (t0 + lim)
+        p_before = 1.0 - np.exp(-_tau(o, bmin, bmax, scale, org, d, t0, s))
+        se2 = np.sqrt(p_before * (1 - p_before) / n)
+        assert abs((hit & (out[:, 1] < s)).mean() - p_before) < 5 * se2
+
+
+def test_oracle_emission_and_albedo_curves():
+    """Scatter albedo SmoothRgb(0.84, 0.90, 0.96): blue weight at 380 nm, red
+    at 750 nm, green between; emission is 0 where the temperature is at most
+    0.35 and peaks at the 486 nm line where the core is hot."""
+    o = _vol_medium()
+    o.field("temperature")[...] = 0.0
+    o.field("temperature")[:, :, :6] = 1.0  # hot half x < 0 (glow (1 - 0.35)^2 0.95)
+    bmin, bmax = (-1.0, -1.0, -1.0), (1.0, 1.0, 1.0)
+    n = 4000
+    states = _seeded_states(n, 31)
+    lam = np.tile([380.0, 540.0, 486.0, 750.0], (n, 1))
+    out = o.medium_interaction(bmin, bmax, 50.0, 0.0, np.tile([-1.5, 0.0, 0.0], (n, 1)),
+                               np.tile([1.0, 0.0, 0.0], (n, 1)), np.zeros(n), np.full(n, 10.0),
+                               lam, states)
+    hit = out[:, 0] == 1.0
+    assert hit.mean() > 0.99
+    alb = out[hit][0, 5:9]
+    assert abs(alb[0] - 0.96) < 1e-3 and abs(alb[3] - 0.84) < 1e-3 and abs(alb[1] - 0.90) < 1e-2
+    em = out[hit, 9:13]
+    hot = em[:, 2] > 0
+    assert hot.any() and (~hot).any()
+    # at 486 nm the line (amp 1.6) dominates the continuum of the hot core
+    assert np.all(em[hot, 2] > em[hot, 1])
+
+
+def test_oracle_henyey_greenstein_law():
+    from oracle.oracle import OracleFluid
+    n = 60000
+    rng = np.random.default_rng(5)
+    wo = OracleFluid.normalize(rng.normal(size=(n, 3)))
+    for g in (0.6, -0.3, 0.0):
+        states = _seeded_states(n, 41)
+        wi, pdf = OracleFluid.phase_sample(g, wo, states)
+        np.testing.assert_allclose(np.linalg.norm(wi, axis=1), 1.0, atol=1e-12)
+        mu = np.einsum("ij,ij->i", -wo, wi)
+        assert abs(mu.mean() - g) < 5 * mu.std() / np.sqrt(n), (g, mu.mean())
+        np.testing.assert_allclose(pdf, OracleFluid.phase_pdf(g, wo, wi), rtol=1e-12)
+        if g != 0.0:
+            # HG CDF of cos(theta): (1-g^2)/(2g) [(1+g^2-2g mu)^-1/2 - (1+g)^-1]
+            xs = np.sort(mu)
+            cdf = (1 - g * g) / (2 * g) * ((1 + g * g - 2 * g * xs) ** -0.5 - 1 / (1 + g))
+            ks = np.max(np.abs(cdf - np.arange(1, n + 1) / n))
+            assert ks < 1.63 / np.sqrt(n), (g, ks)  # 1% level of the KS statistic
+        else:
+            np.testing.assert_allclose(pdf, 1 / (4 * np.pi))
+    # the pdf integrates to 1 over the sphere (2 pi int p(mu) dmu)
+    for g in (0.85, -0.5, 0.2):
+        mus = np.linspace(-1, 1, 200001)
+        w_o = np.tile([0.0, 0.0, -1.0], (len(mus), 1))  # -wo = +z
+        w_i = np.stack([np.sqrt(1 - mus ** 2), np.zeros_like(mus), mus], axis=1)
+        p = OracleFluid.phase_pdf(g, w_o, w_i)
+        assert abs(2 * np.pi * np.trapezoid(p, mus) - 1.0) < 1e-6
