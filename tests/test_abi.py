@@ -103,3 +103,22 @@ def test_noalloc_telemetry_rejects_bad_arguments_before_device_use():
     # ceil(4160/64) * ceil(256/4) = 65 * 64 = 4160 columns, one partial each
     assert lib.aether_fluid_compute_telemetry_noalloc(*args(0, 4160, 256, 6)) == einval
     assert b"4096" in lib.aether_fluid_last_error()
+
+
+def test_rust_ffi_block_matches_header():
+    """The Rust binding (aether-core-engine_amd/rust/src/ffi.rs) declares
+    every entry point of include/aether_fluid.h with the same number of
+    parameters, and nothing else.  (No Rust toolchain exists in this image,
+    so the crate itself is not compiled here.)"""
+    text = open(HEADER).read()
+    c_protos = {}
+    for m in re.finditer(r"^(?:int|const char \*)\s*(aether_\w+)\s*\(([^;]*)\);", text,
+                         re.M | re.S):
+        args = " ".join(m.group(2).split())
+        c_protos[m.group(1)] = 0 if args in ("void", "") else args.count(",") + 1
+    rs = open(os.path.join(ROOT, "aether-core-engine_amd", "rust", "src", "ffi.rs")).read()
+    rs_protos = {}
+    for m in re.finditer(r"pub fn (aether_\w+)\s*\(([^)]*)\)", rs, re.S):
+        args = " ".join(m.group(2).split())
+        rs_protos[m.group(1)] = 0 if not args else args.count(",") + 1
+    assert rs_protos == c_protos
