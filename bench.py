@@ -136,53 +136,139 @@ def cpu_baseline(seconds_budget: float = 20.0):
     return out
 
 
-def bench_mixed(steps: int, warmup: int) -> dict:
-    """256^3, dt 0.005: FP32 advection/stencils with the FP64-refined FP32
-    CG projection (inner tol 1e-3, refine to max|div u| <= 1e-7), against
-    the all-FP64 GPU projection (FP64 CG to relative tolerance 1e-14)."""
+def _mixed_solver(prec: str, cg_tol: float = 0.0, cg_iters: int = 4000):
     import aether_fluid as af
     from aether_fluid.scenarios import SMOKE_CONFIGS, seeded_config_state
     cfg = SMOKE_CONFIGS["mixed_256"]
-    nvox = int(np.prod(cfg["dims"]))
-    state = seeded_config_state("mixed_256")
-    out = {"workload": "mixed-precision projection 256^3", "steps": steps, "warmup": warmup}
-    for prec in ("fp32_ir64", "fp64"):
-        s = af.FluidSolver3D(*cfg["dims"], h=cfg["h"], precision=prec)
-        for n, a in state.items():
-            s.upload(n, a)
-        s.gravity_alpha, s.buoyancy_beta, s.vorticity_eps = cfg["alpha"], cfg["beta"], cfg["eps"]
-        s.set_sphere_source(cfg["source"]["center"], cfg["source"]["radius"])
-        s.solver_kind, s.solver_iters = 0, 4000
-        if prec == "fp64":
-            s.cg_rel_tol = 1e-14
-        else:
-            s.solver_iters = 1000
-            s.ir_inner_rel_tol, s.ir_div_tol, s.ir_max_outer = 1e-3, 1e-7, 30
-        dt = cfg["dt"]
-        times = [dt * i for i in range(warmup + steps)]
-        if warmup:
-            s.run(dt, times[:warmup])
+    s = af.FluidSolver3D(*cfg["dims"], h=cfg["h"], precision=prec)
+    for n, a in seeded_config_state("mixed_256").items():
+        s.upload(n, a)
+    s.gravity_alpha, s.buoyancy_beta, s.vorticity_eps = cfg["alpha"], cfg["beta"], cfg["eps"]
+    s.set_sphere_source(cfg["source"]["center"], cfg["source"]["radius"])
+    s.solver_kind, s.solver_iters, s.cg_rel_tol = 0, cg_iters, cg_tol
+    if prec == "fp32_ir64":
+        s.solver_iters = 1000
+        s.ir_inner_rel_tol, s.ir_div_tol, s.ir_max_outer = 1e-3, 1e-7, 30
+    return cfg, s
+
+
+def _timed_steps(s, dt, t0_index, steps):
+    """Runs `steps` steps one call each; returns (seconds, per-step max|div u|
+    of the projection's FP64 residual or of the stored velocity, per-step CG
+    iterations)."""
+    el, divs, iters = 0.0, [], []
+    for q in range(steps):
         t0 = time.perf_counter()
-        s.run(dt, times[warmup:])
-        el = time.perf_counter() - t0
+        s.run(dt, [dt * (t0_index + q)])
+        el += time.perf_counter() - t0
         st = s.last_solve_stats()
-        # FP64 storage: the post-projection max|div u| is the solve's max
-        # residual / h^2 (the projection identity), so the two coincide
-        res_div = st["div_max"] if prec != "fp64" else s.divergence_max()
-        out[prec] = {"voxel_steps_per_s": nvox * steps / el, "ms_per_step": el * 1e3 / steps,
-                     "cg_iters_per_step": s.run_iters(), "outer_sweeps_last": st["outer"],
-                     "solve_div_max": res_div, "velocity_div_max": s.divergence_max()}
-        out[prec + "_state"] = [s.download(n) for n in ("u", "v", "w", "density")]
-    a, b = out.pop("fp32_ir64_state"), out.pop("fp64_state")
-    out["rel_l2_mixed_vs_fp64"] = {n: float(np.linalg.norm(x - y) / max(np.linalg.norm(y), 1e-300))
-                                   for n, x, y in zip(("u", "v", "w", "density"), a, b)}
-    out["speedup_mixed_over_fp64"] = (out["fp32_ir64"]["voxel_steps_per_s"] /
-                                      out["fp64"]["voxel_steps_per_s"])
+        divs.append(st["div_max"] if st["outer"] > 0 else s.divergence_max())
+        iters.append(s.run_iters()[0])
+    return el, divs, iters
+
+
+def bench_mixed(steps: int, warmup: int) -> dict:
+    """256^3, dt 0.005: FP32 advection/stencils with the FP64-refined FP32
+    CG projection (inner tol 1e-3, refined to max|div u| <= 1e-7 in FP64),
+    against the all-FP64 projection stopped at the same divergence: FP64 CG
+    to the loosest relative tolerance whose first step reaches
+    max|div u| <= 1e-7 (found by trying 1e-8, 3e-9, 1e-9, ... on that step,
+    outside the timed region), the same tolerance for every timed step."""
+    cfg, _ = None, None
+    nvox = 256 ** 3
+    out = {"workload": "mixed-precision projection 256^3 against FP64 at the same max|div u|",
+           "steps": steps, "warmup": warmup, "div_target": 1e-7}
+    cfg, s = _mixed_solver("fp32_ir64")
+    dt = cfg["dt"]
+    if warmup:
+        s.run(dt, [dt * q for q in range(warmup)])
+    el, divs, its = _timed_steps(s, dt, warmup, steps)
+    mixed_state = [s.download(n) for n in ("u", "v", "w", "density")]
+    out["fp32_ir64"] = {"voxel_steps_per_s": nvox * steps / el, "ms_per_step": el * 1e3 / steps,
+                        "solve_div_max": divs, "velocity_div_max": s.divergence_max(),
+                        "inner_cg_iters": its}
+    s.close()
+    tau = None
+    for t in (1e-8, 3e-9, 1e-9, 3e-10, 1e-10, 3e-11, 1e-11):
+        _, c = _mixed_solver("fp64", cg_tol=t)
+        c.run(dt, [0.0])
+        ok = c.divergence_max() <= 1e-7
+        c.close()
+        if ok:
+            tau = t
+            break
+    out["fp64_rel_tol"] = tau
+    _, s = _mixed_solver("fp64", cg_tol=tau or 1e-11)
+    if warmup:
+        s.run(dt, [dt * q for q in range(warmup)])
+    el64, divs64, its64 = _timed_steps(s, dt, warmup, steps)
+    out["fp64"] = {"voxel_steps_per_s": nvox * steps / el64, "ms_per_step": el64 * 1e3 / steps,
+                   "solve_div_max": divs64, "cg_iters": its64}
+    fp64_state = [s.download(n) for n in ("u", "v", "w", "density")]
+    s.close()
+    out["rel_l2_mixed_vs_fp64"] = {
+        n: float(np.linalg.norm(x - y) / max(np.linalg.norm(y), 1e-300))
+        for n, x, y in zip(("u", "v", "w", "density"), mixed_state, fp64_state)}
+    out["speedup_mixed_over_fp64"] = el64 / el
     return out
 
 
+def precision_vs_fp64(steps: int, warmup: int, fp32_solver) -> dict:
+    """The timed FP32 run against the FP64 instantiation (the reference's
+    arithmetic; pinned to the f64 oracle bit for bit or to 1e-12) over the
+    same steps from the same state: per-field relative L2 and max|div u|
+    after the last projection of each.  Outside the timed region."""
+    import aether_fluid as af
+    from aether_fluid.scenarios import SMOKE_CONFIGS, configure, seeded_config_state
+    cfg = SMOKE_CONFIGS[WORKLOAD]
+    s64 = af.FluidSolver3D(*cfg["dims"], h=cfg["h"], precision="fp64")
+    configure(s64, WORKLOAD, "cg")
+    for name, arr in seeded_config_state(WORKLOAD).items():
+        s64.upload(name, arr)
+    dt = cfg["dt"]
+    s64.run(dt, [dt * i for i in range(warmup + steps)])
+    out = {"reference": "FP64 instantiation of the same kernels, same steps",
+           "steps": warmup + steps, "rel_l2_fp32_vs_fp64": {}}
+    for n in ("u", "v", "w", "density", "temperature"):
+        a, b = fp32_solver.download(n), s64.download(n)
+        out["rel_l2_fp32_vs_fp64"][n] = float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+    out["div_max_fp32"] = fp32_solver.divergence_max()
+    out["div_max_fp64"] = s64.divergence_max()
+    out["cg_iters_fp64_last"] = s64.run_iters()[-1]
+    s64.close()
+    return out
+
+
+def cg256_roofline(iters: int = 20) -> dict:
+    """The HBM-bound case: one CG iteration at 256^3 FP32 (64 MiB per array,
+    past the 256 MB Infinity Cache), in each layout, timed by the CG
+    autotuner with HIP events over `iters` whole iterations (so the
+    iteration time includes every launch of it: the direction and update
+    kernels and, past 1024 partials, the k_partials_total pre-sum).
+    Algorithmic bytes: 8 FP32 passes of N per iteration."""
+    import aether_fluid as af
+    n = 256
+    s = af.FluidSolver3D(n, n, n, h=1.0 / n, precision="fp32")
+    tune = s.autotune_cg(iters)
+    s.close()
+    bytes_it = 8 * 4.0 * n ** 3
+    per = {k: {"us_per_iter": v, "gbps": bytes_it / (v * 1e-6) / 1e9}
+           for k, v in tune["us_per_iter"].items()}
+    best = min(per, key=lambda k: per[k]["us_per_iter"])
+    return {"bound": "hbm", "kernel": f"CG iteration 256^3 ({best}: k_cg_dir4 + k_cg_update4 "
+            "+ k_partials_total)" if best.startswith("x4") else f"CG iteration 256^3 ({best})",
+            "achieved": per[best]["gbps"], "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+            "frac": per[best]["gbps"] / HBM_PEAK_GBPS, "algorithmic_bytes_per_iter": bytes_it,
+            "layouts": per}
+
+
+FD_EPS = (("fd_1e-3", 1e-3), ("fd_1e-4", 1e-4), ("fd_1e-5", 1e-5), ("fd_1e-6", 1e-6),
+          ("fd_1e-7", 1e-7))
+
+
 def bench_rollout(n: int, steps: int, ckpt: int, ndirs: int = 0, reps: int = 2,
-                  init_state=None, cg_tol: float = 1e-5, cg_iters: int = 200) -> dict:
+                  init_state=None, cg_tol: float = 1e-5, cg_iters: int = 200,
+                  fd_eps=FD_EPS) -> dict:
     """Differentiable rollout at n^3: FP32 forward, FP32 adjoint with FP64
     reductions, checkpoint every `ckpt` steps; inflow-patch controls and
     buoyancy beta; loss = sum (rho_T - target)^2.  Times the forward-only
@@ -233,9 +319,6 @@ def bench_rollout(n: int, steps: int, ckpt: int, ndirs: int = 0, reps: int = 2,
         out["fp32_grad_rel_l2_vs_fp64"] = float(np.linalg.norm(gc - gc64) / np.linalg.norm(gc64))
         out["fp32_grad_beta_rel_vs_fp64"] = float(abs(gb - gb64) / abs(gb64))
         rng = np.random.default_rng(0xAE7E0003)
-        # the specified eps = 1e-3, then smaller steps down to 1e-7
-        fd_eps = (("fd_1e-3", 1e-3), ("fd_1e-4", 1e-4), ("fd_1e-5", 1e-5), ("fd_1e-6", 1e-6),
-                  ("fd_1e-7", 1e-7))
         errs = {"fp64_adjoint": [], **{k: [] for k, _ in fd_eps}}
         fd64 = {k: [] for k, _ in fd_eps}
         for _ in range(ndirs):
@@ -332,6 +415,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the fwd+adjoint rollout")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the precision, 256^3 roofline, mixed-precision, 64^3 rollout "
+                         "and batched sections of the default line")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--workload", default="plain_128", choices=["plain_128", "mixed_256", "batched_64", "rollout_64"],
                     help="plain_128: the headline line; mixed_256: mixed-precision "
@@ -369,11 +455,16 @@ def main():
         print("probe done", flush=True)
         return
 
+    def note(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    note("headline 128^3 FP32 CG")
     cfg, s, elapsed, frames, run_iters = time_gpu("cg", args.steps, args.warmup)
     nvox = int(np.prod(cfg["dims"]))
     stats = s.last_solve_stats()
     value = nvox * args.steps / elapsed
     ms = elapsed * 1e3 / args.steps
+    precision = None if args.no_extras else precision_vs_fp64(args.steps, args.warmup, s)
 
     # RBGS variant of the same workload (omega 1.8, 60 sweeps)
     _, _, el_rb, _, _ = time_gpu("rbgs", args.steps, args.warmup)
@@ -405,8 +496,19 @@ def main():
                  "step_algorithmic_gbps": algorithmic_bytes_per_rbgs_step(
                      nvox, cfg["rbgs_sweeps"]) * args.steps / el_rb / 1e9},
         "roofline": roofline,
+        "precision": precision,
         "fwd_adjoint": None,  # filled below
     }
+    if not args.no_extras:
+        note("256^3 CG iteration roofline")
+        line["roofline_hbm_256"] = cg256_roofline()
+        note("mixed precision 256^3 vs FP64")
+        line["mixed_256"] = bench_mixed(3, 1)
+        note("rollout 64^3 x 40")
+        line["rollout_64"] = bench_rollout(64, 40, 8, ndirs=4, cg_tol=1e-6, cg_iters=1000,
+                                           fd_eps=(("fd_1e-3", 1e-3), ("fd_1e-7", 1e-7)))
+        note("batched 64 grids")
+        line["batched_64"] = bench_batched(3, 1)
     if not args.no_adjoint:
         from aether_fluid.scenarios import seeded_config_state
         ro = bench_rollout(128, 8, 8, ndirs=0, reps=2,
@@ -422,6 +524,7 @@ def main():
         line["step_algorithmic_gbps"] = (sum(algorithmic_bytes_per_step(nvox, k)
                                              for k in run_iters) / elapsed / 1e9)
     if not args.no_cpu:
+        note("CPU baseline")
         cb = cpu_baseline(args.cpu_seconds)
         omp = cb["omp"]
         line["cpu_baseline"] = {
