@@ -1340,7 +1340,7 @@ class Solver final : public SolverBase {
         for (int a = 0; a < 3; ++a) cur_[a] = save[a];
     }
 
-    double adjoint_step(double dt, StepRec &r, const aether_rollout_desc &d) {
+    void adjoint_step(double dt, StepRec &r, const aether_rollout_desc &d, int step_slot) {
         // a) density / temperature advection (velocity u3)
         zero(adj_b_[0].p);
         zero(adj_b_[1].p);
@@ -1366,7 +1366,9 @@ class Solver final : public SolverBase {
             adj_b_[1].p, adj_part_.p);
         LAUNCH_CHECK();
         const int ncell = (int)(cell_grid_.x * cell_grid_.y * cell_grid_.z);
-        k_partials_final<<<1, kThreads, 0, st_>>>(adj_part_.p, ncell, 0, scalar_.p);
+        // this step's d loss / d beta, kept on the device (summed over the
+        // steps in reverse order at the end: no host synchronisation per step)
+        k_partials_final<<<1, kThreads, 0, st_>>>(adj_part_.p, ncell, 0, gbeta_.p + step_slot);
         LAUNCH_CHECK();
         k_adj_confine2<T><<<cell_grid_, cell_blk_, 0, st_>>>(
             g_, (T)inv_2h, r.u1.p, r.v1.p, r.w1.p, curl_.p, adj_lg_[0].p, adj_lg_[1].p,
@@ -1394,7 +1396,6 @@ class Solver final : public SolverBase {
         for (int a = 0; a < 3; ++a) std::swap(adj_a_[a].p, adj_d_[a].p);
         std::swap(adj_ar_.p, adj_b_[0].p);
         std::swap(adj_at_.p, adj_b_[1].p);
-        return read_scalar();  // this step's d loss / d beta
     }
 
     // Checks a rollout description and loads its controls; sets the patch
@@ -1461,15 +1462,21 @@ class Solver final : public SolverBase {
         zero(adj_p_.p);
         k_fill<double><<<blocks_for(cnt), kThreads, 0, st_>>>(grad_c_.p, (long)cnt, 0.0);
         LAUNCH_CHECK();
-        double gbeta = 0.0;
+        if ((int)gbeta_.n < d.nsteps) gbeta_.alloc(d.nsteps);
         for (int seg = nseg - 1; seg >= 0; --seg) {
             const int t0 = seg * K, t1 = std::min(d.nsteps, t0 + K);
             if (seg != nseg - 1) {
                 load_ckpt(*ckpts_[seg]);
                 for (int t = t0; t < t1; ++t) step_recorded(d.dt, *recs_[t - t0]);
             }
-            for (int t = t1 - 1; t >= t0; --t) gbeta += adjoint_step(d.dt, *recs_[t - t0], d);
+            for (int t = t1 - 1; t >= t0; --t) adjoint_step(d.dt, *recs_[t - t0], d, t);
         }
+        std::vector<double> per_step(d.nsteps);
+        HIPT(hipMemcpyAsync(per_step.data(), gbeta_.p, d.nsteps * sizeof(double),
+                            hipMemcpyDeviceToHost, st_));
+        sync();
+        double gbeta = 0.0;
+        for (int t = d.nsteps - 1; t >= 0; --t) gbeta += per_step[t];  // reverse-sweep order
         return gbeta;
     }
 
@@ -1623,6 +1630,7 @@ class Solver final : public SolverBase {
         adj_lw_[3], adj_lg_[3], adj_p_;
     DevBuf<fixacc_t> acc_src_, acc_fwd_;  // fixed-point scatter accumulators of the adjoint
     DevBuf<FixScale> fix_;
+    DevBuf<double> gbeta_;  // per-step d loss / d beta of the reverse sweep
     // the tape between rollout_forward and rollout_backward
     bool tape_valid_ = false;
     aether_rollout_desc tape_desc_{};

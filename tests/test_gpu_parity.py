@@ -1336,8 +1336,9 @@ def test_adjoint_fp64_rollout_64_real_size_matches_central_differences():
     of the discrete forward) along all 16 seeded directions, relative error
     <= 1e-5; the FP32 adjoint (FP64 reductions, fixed-point scatters)
     against the same differences, relative error <= 1e-4 in every
-    direction, and against the FP64 adjoint, whole-gradient relative L2
-    <= 1e-3.
+    direction (also with the adjoint's projections stopped at 1e-5, the
+    benchmark's setting), and against the FP64 adjoint, whole-gradient
+    relative L2 <= 1e-3.
 
     The FP32 rollout projects to relative residual 1e-6 (the workload's
     tight tolerance; FP32 CG attains a true residual of ~6e-7 here).  The
@@ -1357,11 +1358,14 @@ def test_adjoint_fp64_rollout_64_real_size_matches_central_differences():
     assert len(pb["controls"]) == 32 * 4 * 32
     loss, gc, gb = s64.rollout(controls=pb["controls"], beta=1.0, **args)
     _, gc32, gb32 = s32.rollout(controls=pb["controls"], beta=1.0, **args)
+    # the benchmark's setting: adjoint projections to 1e-5 only
+    _, gc32a, gb32a = s32.rollout(controls=pb["controls"], beta=1.0, adjoint_cg_rel_tol=1e-5,
+                                  **args)
     assert loss > 0 and np.all(np.isfinite(gc))
     assert np.linalg.norm(gc32 - gc) <= 1e-3 * np.linalg.norm(gc), np.linalg.norm(gc32 - gc)
     assert abs(gb32 - gb) <= 1e-3 * abs(gb)
     eps = 1e-7
-    e64, e32 = [], []
+    e64, e32, e32a = [], [], []
     for dc, db in _directions(16, len(pb["controls"]), seed=0xAE7E0003):
         lp, _, _ = s64.rollout(controls=pb["controls"] + eps * dc, beta=1.0 + eps * db,
                                grad=False, **args)
@@ -1370,10 +1374,14 @@ def test_adjoint_fp64_rollout_64_real_size_matches_central_differences():
         fd = (lp - lm) / (2 * eps)
         e64.append(abs(float(gc @ dc + gb * db) - fd) / abs(fd))
         e32.append(abs(float(gc32 @ dc + gb32 * db) - fd) / abs(fd))
-        print(f"64^3 dir fd {fd:.9e} fp64 {e64[-1]:.2e} fp32 {e32[-1]:.2e}", flush=True)
-    print("max fp64", max(e64), "max fp32", max(e32), "median fp32", np.median(e32), flush=True)
+        e32a.append(abs(float(gc32a @ dc + gb32a * db) - fd) / abs(fd))
+        print(f"64^3 dir fd {fd:.9e} fp64 {e64[-1]:.2e} fp32 {e32[-1]:.2e} "
+              f"(adjoint 1e-5: {e32a[-1]:.2e})", flush=True)
+    print("max fp64", max(e64), "max fp32", max(e32), "median fp32", np.median(e32),
+          "max fp32 adjoint 1e-5", max(e32a), flush=True)
     assert max(e64) <= 1e-5, e64
     assert max(e32) <= 1e-4, e32
+    assert max(e32a) <= 1e-4, e32a
 
 
 def _volume_rays(n, seed):
