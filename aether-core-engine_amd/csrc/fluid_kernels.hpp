@@ -795,20 +795,10 @@ __device__ __forceinline__ T lap7(T xc, T xm, T xp, T ym, T yp, T zm, T zp, bool
     }
 }
 
-// Store for CG vectors.  -DAETHER_NT_STORES=1 selects streaming
-// (non-temporal) stores; measured at 128^3 they are ~2% slower than plain
-// stores, which stay the default.
-#ifndef AETHER_NT_STORES
-#define AETHER_NT_STORES 0
-#endif
+// Store for CG vectors: a plain store (non-temporal stores measured ~2%
+// slower at 128^3, as the next launch re-reads the vector from L2/MALL).
 template <typename T>
-__device__ __forceinline__ void st_stream(T *p, T v) {
-#if AETHER_NT_STORES
-    __builtin_nontemporal_store(v, p);
-#else
-    *p = v;
-#endif
-}
+__device__ __forceinline__ void st_stream(T *p, T v) { *p = v; }
 
 // Column geometry shared by the CG kernels.
 struct Col {
