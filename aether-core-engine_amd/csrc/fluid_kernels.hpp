@@ -1346,16 +1346,29 @@ __device__ __forceinline__ void body_telemetry_part(int i, int j, int kb, int ke
     const double inv_2h = 0.5 / h;
     double acc[6] = {0, 0, 0, 0, 0, 0};
     double peak = 0.0;
-    if (i < g.nx && j < g.ny) {
-        for (int k = kb; k < ke; ++k) {
-            if (!is_interior(g, i, j, k)) continue;
-            const int id = (k * g.ny + j) * g.nx + i;
-            const double uc = u[id], vc = v[id], wc = w[id];
+    // interior cells only (src/fluid.rs:622-624): whole columns of them, and
+    // planes 1 .. nz-2 of the chunk, so every neighbour index is in the grid.
+    // Marching in z, the z-neighbours of u and v rotate through registers.
+    if (i >= 1 && i <= g.nx - 2 && j >= 1 && j <= g.ny - 2) {
+        const int k0 = max(kb, 1), k1 = min(ke, g.nz - 1);
+        const double fx = 2.0 * ((double)i / (double)(g.nx - 1)) - 1.0;
+        const double fy = 2.0 * ((double)j / (double)(g.ny - 1)) - 1.0;
+        int id = (k0 * g.ny + j) * g.nx + i;
+        double um = 0.0, uc = 0.0, vm = 0.0, vc = 0.0;
+        if (k0 < k1) {
+            um = u[id - g.sxy];
+            uc = u[id];
+            vm = v[id - g.sxy];
+            vc = v[id];
+        }
+#pragma unroll 2
+        for (int k = k0; k < k1; ++k, id += g.sxy) {
+            const double up = u[id + g.sxy], vp = v[id + g.sxy], wc = w[id];
             const double u2 = uc * uc + vc * vc + wc * wc;
             acc[0] += 0.5 * u2 * dv;
             const double dw_dy = ((double)w[id + g.nx] - (double)w[id - g.nx]) * inv_2h;
-            const double dv_dz = ((double)v[id + g.sxy] - (double)v[id - g.sxy]) * inv_2h;
-            const double du_dz = ((double)u[id + g.sxy] - (double)u[id - g.sxy]) * inv_2h;
+            const double dv_dz = (vp - vm) * inv_2h;
+            const double du_dz = (up - um) * inv_2h;
             const double dw_dx = ((double)w[id + 1] - (double)w[id - 1]) * inv_2h;
             const double dv_dx = ((double)v[id + 1] - (double)v[id - 1]) * inv_2h;
             const double du_dy = ((double)u[id + g.nx] - (double)u[id - g.nx]) * inv_2h;
@@ -1367,13 +1380,18 @@ __device__ __forceinline__ void body_telemetry_part(int i, int j, int kb, int ke
             // a non-finite velocity or density reads as a non-finite |u|^2,
             // |w|^2 or rho (inf - inf and NaN alike)
             bad |= !isfinite(u2) || !isfinite(w2) || !isfinite(r);
-            if (r > 1e-8) {
-                const double m = r * dv;
-                acc[2] += m;
-                acc[3] += (2.0 * ((double)i / (double)(g.nx - 1)) - 1.0) * m;
-                acc[4] += (2.0 * ((double)j / (double)(g.ny - 1)) - 1.0) * m;
-                acc[5] += (2.0 * ((double)k / (double)(g.nz - 1)) - 1.0) * m;
-            }
+            // cells at or below the threshold add +0.0, which leaves the
+            // sums' bits as skipping them would
+            const bool mass = r > 1e-8;
+            const double m = mass ? r * dv : 0.0;
+            acc[2] += m;
+            acc[3] += mass ? fx * m : 0.0;
+            acc[4] += mass ? fy * m : 0.0;
+            acc[5] += mass ? (2.0 * ((double)k / (double)(g.nz - 1)) - 1.0) * m : 0.0;
+            um = uc;
+            uc = up;
+            vm = vc;
+            vc = vp;
         }
     }
     // the device flag of src/fluid.rs's silent NaN propagation: an integer
