@@ -3,8 +3,9 @@
 aether_fluid_compute_telemetry_noalloc reduces fields the caller holds on the
 device with its partial sums in a library-owned static device array.  It runs
 the same kernels on the same reduction tiling as the solver's own telemetry,
-so the frames must be equal bit for bit.  Two calls in a row share the array:
-the second, on a smaller grid, must not see the first call's partial sums.
+so the frames must be equal bit for bit.  Calls in a row share the array:
+each, on a smaller grid than the one before, must not see the earlier calls'
+partial sums.  The first grid fills the array to its stated maximum.
 
 The test plays the caller: it allocates its device buffers itself, through
 the HIP runtime the library is linked against (the same process-wide
@@ -75,9 +76,10 @@ def test_noalloc_telemetry_equals_solver_telemetry(precision):
     stream = C.c_void_p()
     _ok(hip.hipStreamCreate(C.byref(stream)))
     try:
-        # the first grid needs 1100 partial slots (more than the ~1024 of
-        # most grids); the second is smaller
-        for dims, t in (((128, 2200, 6), 0.25), ((20, 18, 16), 1.5)):
+        # each grid smaller than the one before: 1024 x 1024 uses all 4096
+        # partial slots of the array (16 x 256 reduction columns), 128 x 2200
+        # needs 1100 (more than the ~1024 of most grids), 20 x 18 x 16 few
+        for dims, t in (((1024, 1024, 6), 0.5), ((128, 2200, 6), 0.25), ((20, 18, 16), 1.5)):
             print(f"[case] noalloc telemetry {precision} {dims}", flush=True)
             s = af.FluidSolver3D(*dims, precision=precision)
             st = random_state(*dims, seed=sum(dims))
