@@ -414,6 +414,81 @@ def bench_batched(steps: int, warmup: int, dt: float = 0.01) -> dict:
             "speedup": el_1 / el_b}
 
 
+def _medium_rays(n: int, seed: int):
+    """n seeded rays from a sphere of radius 3 towards uniform points of the
+    box [-1, 1]^3 (directions normalised), wavelengths in 380-750 nm and
+    Xoshiro256++ states (any nonzero words)."""
+    rng = np.random.default_rng(seed)
+    org = rng.normal(size=(n, 3))
+    org *= 3.0 / np.linalg.norm(org, axis=1)[:, None]
+    d = rng.uniform(-1.0, 1.0, (n, 3)) - org
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    lam = rng.uniform(380.0, 750.0, (n, 4))
+    states = rng.integers(1, 1 << 62, size=(n, 4), dtype=np.uint64)
+    return org, d, lam, states
+
+
+MEDIUM_BOUNDS = ((-1.0, -1.0, -1.0), (1.0, 1.0, 1.0))
+MEDIUM_SCALE, MEDIUM_G = 2.5, 0.55
+
+
+def bench_medium(s, nrays: int = 1 << 20) -> dict:
+    """FluidMedium (src/fluid.rs:675-910) on the headline solver's final
+    128^3 state: the device snapshot (from_solver), point sampling, and the
+    path tracer's queries over nrays seeded rays, each with its own
+    Xoshiro256++ state.  The API takes host arrays, so every rate includes
+    the host <-> device copies of its inputs and results."""
+    import aether_fluid as af
+    t0 = time.perf_counter()
+    med = af.FluidMedium.from_solver(s, MEDIUM_BOUNDS, MEDIUM_SCALE, MEDIUM_G)
+    t_snap = time.perf_counter() - t0
+    org, d, lam, st0 = _medium_rays(nrays, 0xAE7E0006)
+    rays = af.FluidMedium.make_rays(org, d, 0.0, 10.0)
+    out = {"workload": f"FluidMedium of the 128^3 state after the timed steps, bounds [-1, 1]^3, "
+                       f"sigma_t scale {MEDIUM_SCALE}, g {MEDIUM_G}; {nrays} rays from radius 3 "
+                       f"towards the box", "rays": nrays,
+           "note": "rates include host<->device copies (the API takes host arrays)",
+           "from_solver_ms": t_snap * 1e3, "max_density": med.max_density}
+    pts = np.random.default_rng(7).uniform(-1.1, 1.1, (nrays, 3))
+
+    def rate(fn):
+        fn()  # first call: module load and allocation
+        t = time.perf_counter()
+        r = fn()
+        return nrays / (time.perf_counter() - t), r
+
+    out["sample_density_and_temp_points_per_s"], _ = rate(lambda: med.sample_density_and_temp(pts))
+    out["sample_interaction_rays_per_s"], it = rate(
+        lambda: med.sample_interaction(rays, lam, st0.copy()))
+    out["transmittance_rays_per_s"], tr = rate(lambda: med.transmittance(rays, st0.copy()))
+    out["sample_phase_hg_per_s"], _ = rate(lambda: med.sample_phase_hg(-d, st0.copy()))
+    out["interaction_fraction"] = float(np.mean(it["hit"] == 1))
+    out["mean_transmittance"] = float(np.mean(tr))
+    med.close()
+    return out
+
+
+def medium_cpu_baseline(s, nrays: int = 1 << 16) -> dict:
+    """The f64 oracle's FluidMedium queries on the first nrays rays of the
+    same set, on one host thread, with the device snapshot's fields."""
+    from oracle.oracle import OracleFluid
+    nx, ny, nz = s.nx, s.ny, s.nz
+    o = OracleFluid(nx, ny, nz)
+    for f in ("density", "temperature"):
+        o.field(f)[...] = s.download(f).reshape(nz, ny, nx)
+    org, d, lam, st0 = _medium_rays(1 << 20, 0xAE7E0006)
+    org, d, lam, st0 = org[:nrays], d[:nrays], lam[:nrays], st0[:nrays].copy()
+    z, far = np.zeros(nrays), np.full(nrays, 10.0)
+    t = time.perf_counter()
+    o.medium_interaction(*MEDIUM_BOUNDS, MEDIUM_SCALE, MEDIUM_G, org, d, z, far, lam, st0.copy())
+    t_i = time.perf_counter() - t
+    t = time.perf_counter()
+    o.medium_transmittance(*MEDIUM_BOUNDS, MEDIUM_SCALE, MEDIUM_G, org, d, z, far, st0.copy())
+    t_t = time.perf_counter() - t
+    return {"sample_interaction_rays_per_s": nrays / t_i, "transmittance_rays_per_s": nrays / t_t,
+            "cores": 1, "kind": "port", "sample": f"the first {nrays} rays, f64 C oracle"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -427,9 +502,11 @@ def main():
                     help="skip the precision, 256^3 roofline, mixed-precision, 64^3 rollout "
                          "and batched sections of the default line")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
-    ap.add_argument("--workload", default="plain_128", choices=["plain_128", "mixed_256", "batched_64", "rollout_64"],
+    ap.add_argument("--workload", default="plain_128",
+                    choices=["plain_128", "mixed_256", "batched_64", "rollout_64", "medium_128"],
                     help="plain_128: the headline line; mixed_256: mixed-precision "
-                         "projection at 256^3 against the all-FP64 projection")
+                         "projection at 256^3 against the all-FP64 projection; medium_128: "
+                         "FluidMedium sampling of the 128^3 state after --steps steps")
     ap.add_argument("--probe", action="store_true",
                     help="counter-collection mode: 2 steps of the workload with each CG "
                          "layout and with RB-SOR, and a 256 MiB stream copy; no timing output")
@@ -450,6 +527,10 @@ def main():
         return
     if args.workload == "mixed_256":
         print(json.dumps(bench_mixed(args.steps, args.warmup)), flush=True)
+        return
+    if args.workload == "medium_128":
+        _, s, _, _, _ = time_gpu("cg", args.steps, args.warmup)
+        print(json.dumps(bench_medium(s)), flush=True)
         return
     if args.probe:  # short run for counter collection: both CG layouts, RB-SOR, the copy
         _, s = make_solver("cg")
@@ -522,6 +603,14 @@ def main():
                                            adj_tol=1e-5, reps=3)
         note("batched 64 grids")
         line["batched_64"] = bench_batched(3, 1)
+        note("FluidMedium sampling")
+        line["medium_128"] = bench_medium(s)
+        if not args.no_cpu:
+            cpu_m = medium_cpu_baseline(s)
+            line["medium_128"]["cpu_baseline"] = cpu_m
+            line["medium_128"]["gpu_over_cpu_interaction"] = (
+                line["medium_128"]["sample_interaction_rays_per_s"] /
+                cpu_m["sample_interaction_rays_per_s"])
     if not args.no_adjoint:
         from aether_fluid.scenarios import seeded_config_state
         ro = bench_rollout(128, 8, 8, ndirs=0, reps=2,
