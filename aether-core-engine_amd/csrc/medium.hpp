@@ -9,6 +9,28 @@
 namespace aether {
 namespace {
 
+// Device scratch of the medium's queries: one block, grown when a call needs
+// more and reused by later calls (each call waits for its copies before it
+// returns, and a handle is not shared between threads).  Sub-buffers start
+// on 256-byte boundaries.
+struct Arena {
+    DevBuf<unsigned char> buf;
+    size_t off = 0;
+    static size_t round_up(size_t b) { return (b + 255) & ~size_t(255); }
+    template <typename... B>
+    void reserve(B... bytes) {
+        const size_t need = (round_up(bytes) + ...);
+        if (buf.n < need) buf.alloc(need);
+        off = 0;
+    }
+    template <typename U>
+    U *take(size_t count) {
+        U *q = reinterpret_cast<U *>(buf.p + off);
+        off += round_up(count * sizeof(U));
+        return q;
+    }
+};
+
 template <typename T>
 class Medium final : public MediumBase {
    public:
@@ -34,12 +56,13 @@ class Medium final : public MediumBase {
     }
     void sample(const double *xyz, size_t npts, double *out) override {
         if (npts == 0) return;
-        DevBuf<double> dxyz(npts * 3), dout(npts * 2);
-        HIPT(hipMemcpyAsync(dxyz.p, xyz, npts * 3 * sizeof(double), hipMemcpyHostToDevice, st_));
+        scratch_.reserve(npts * 3 * sizeof(double), npts * 2 * sizeof(double));
+        double *dxyz = scratch_.take<double>(npts * 3), *dout = scratch_.take<double>(npts * 2);
+        HIPT(hipMemcpyAsync(dxyz, xyz, npts * 3 * sizeof(double), hipMemcpyHostToDevice, st_));
         k_medium_sample<T><<<(unsigned)((npts + kThreads - 1) / kThreads), kThreads, 0, st_>>>(
-            g_, box_, rho_.p, temp_.p, dxyz.p, (long)npts, dout.p);
+            g_, box_, rho_.p, temp_.p, dxyz, (long)npts, dout);
         LAUNCH_CHECK();
-        HIPT(hipMemcpyAsync(out, dout.p, npts * 2 * sizeof(double), hipMemcpyDeviceToHost, st_));
+        HIPT(hipMemcpyAsync(out, dout, npts * 2 * sizeof(double), hipMemcpyDeviceToHost, st_));
         HIPT(hipStreamSynchronize(st_));
     }
 
@@ -60,18 +83,20 @@ class Medium final : public MediumBase {
     void interaction(const aether_medium_ray *rays, size_t n, const double *lambda,
                      uint64_t *rng, aether_medium_interaction *out) override {
         if (n == 0) return;
-        DevBuf<aether_medium_ray> dr(n);
-        DevBuf<double> dl(4 * n);
-        DevBuf<uint64_t> ds(4 * n);
-        DevBuf<aether_medium_interaction> dout(n);
-        HIPT(hipMemcpyAsync(dr.p, rays, n * sizeof(*rays), hipMemcpyHostToDevice, st_));
-        HIPT(hipMemcpyAsync(dl.p, lambda, 4 * n * sizeof(double), hipMemcpyHostToDevice, st_));
-        HIPT(hipMemcpyAsync(ds.p, rng, 4 * n * sizeof(uint64_t), hipMemcpyHostToDevice, st_));
-        k_medium_interaction<T><<<blocks(n), kThreads, 0, st_>>>(dev(), rho_.p, temp_.p, dr.p,
-                                                                 dl.p, (long)n, ds.p, dout.p);
+        scratch_.reserve(n * sizeof(*rays), 4 * n * sizeof(double), 4 * n * sizeof(uint64_t),
+                         n * sizeof(*out));
+        auto *dr = scratch_.take<aether_medium_ray>(n);
+        double *dl = scratch_.take<double>(4 * n);
+        uint64_t *ds = scratch_.take<uint64_t>(4 * n);
+        auto *dout = scratch_.take<aether_medium_interaction>(n);
+        HIPT(hipMemcpyAsync(dr, rays, n * sizeof(*rays), hipMemcpyHostToDevice, st_));
+        HIPT(hipMemcpyAsync(dl, lambda, 4 * n * sizeof(double), hipMemcpyHostToDevice, st_));
+        HIPT(hipMemcpyAsync(ds, rng, 4 * n * sizeof(uint64_t), hipMemcpyHostToDevice, st_));
+        k_medium_interaction<T><<<blocks(n), kThreads, 0, st_>>>(dev(), rho_.p, temp_.p, dr, dl,
+                                                                 (long)n, ds, dout);
         LAUNCH_CHECK();
-        HIPT(hipMemcpyAsync(out, dout.p, n * sizeof(*out), hipMemcpyDeviceToHost, st_));
-        HIPT(hipMemcpyAsync(rng, ds.p, 4 * n * sizeof(uint64_t), hipMemcpyDeviceToHost, st_));
+        HIPT(hipMemcpyAsync(out, dout, n * sizeof(*out), hipMemcpyDeviceToHost, st_));
+        HIPT(hipMemcpyAsync(rng, ds, 4 * n * sizeof(uint64_t), hipMemcpyDeviceToHost, st_));
         HIPT(hipStreamSynchronize(st_));
     }
 
@@ -79,28 +104,31 @@ class Medium final : public MediumBase {
     void transmittance(const aether_medium_ray *rays, size_t n, uint64_t *rng,
                        double *out) override {
         if (n == 0) return;
-        DevBuf<aether_medium_ray> dr(n);
-        DevBuf<uint64_t> ds(4 * n);
-        DevBuf<double> dout(n);
-        HIPT(hipMemcpyAsync(dr.p, rays, n * sizeof(*rays), hipMemcpyHostToDevice, st_));
-        HIPT(hipMemcpyAsync(ds.p, rng, 4 * n * sizeof(uint64_t), hipMemcpyHostToDevice, st_));
-        k_medium_transmittance<T><<<blocks(n), kThreads, 0, st_>>>(dev(), rho_.p, temp_.p, dr.p,
-                                                                   (long)n, ds.p, dout.p);
+        scratch_.reserve(n * sizeof(*rays), 4 * n * sizeof(uint64_t), n * sizeof(double));
+        auto *dr = scratch_.take<aether_medium_ray>(n);
+        uint64_t *ds = scratch_.take<uint64_t>(4 * n);
+        double *dout = scratch_.take<double>(n);
+        HIPT(hipMemcpyAsync(dr, rays, n * sizeof(*rays), hipMemcpyHostToDevice, st_));
+        HIPT(hipMemcpyAsync(ds, rng, 4 * n * sizeof(uint64_t), hipMemcpyHostToDevice, st_));
+        k_medium_transmittance<T><<<blocks(n), kThreads, 0, st_>>>(dev(), rho_.p, temp_.p, dr,
+                                                                   (long)n, ds, dout);
         LAUNCH_CHECK();
-        HIPT(hipMemcpyAsync(out, dout.p, n * sizeof(double), hipMemcpyDeviceToHost, st_));
-        HIPT(hipMemcpyAsync(rng, ds.p, 4 * n * sizeof(uint64_t), hipMemcpyDeviceToHost, st_));
+        HIPT(hipMemcpyAsync(out, dout, n * sizeof(double), hipMemcpyDeviceToHost, st_));
+        HIPT(hipMemcpyAsync(rng, ds, 4 * n * sizeof(uint64_t), hipMemcpyDeviceToHost, st_));
         HIPT(hipStreamSynchronize(st_));
     }
 
     // fluid.rs:878-888
     void phase_pdf(const double *wo, const double *wi, size_t n, double *pdf) override {
         if (n == 0) return;
-        DevBuf<double> dwo(3 * n), dwi(3 * n), dp(n);
-        HIPT(hipMemcpyAsync(dwo.p, wo, 3 * n * sizeof(double), hipMemcpyHostToDevice, st_));
-        HIPT(hipMemcpyAsync(dwi.p, wi, 3 * n * sizeof(double), hipMemcpyHostToDevice, st_));
-        k_phase_pdf<<<blocks(n), kThreads, 0, st_>>>(phase_g, dwo.p, dwi.p, (long)n, dp.p);
+        scratch_.reserve(3 * n * sizeof(double), 3 * n * sizeof(double), n * sizeof(double));
+        double *dwo = scratch_.take<double>(3 * n), *dwi = scratch_.take<double>(3 * n);
+        double *dp = scratch_.take<double>(n);
+        HIPT(hipMemcpyAsync(dwo, wo, 3 * n * sizeof(double), hipMemcpyHostToDevice, st_));
+        HIPT(hipMemcpyAsync(dwi, wi, 3 * n * sizeof(double), hipMemcpyHostToDevice, st_));
+        k_phase_pdf<<<blocks(n), kThreads, 0, st_>>>(phase_g, dwo, dwi, (long)n, dp);
         LAUNCH_CHECK();
-        HIPT(hipMemcpyAsync(pdf, dp.p, n * sizeof(double), hipMemcpyDeviceToHost, st_));
+        HIPT(hipMemcpyAsync(pdf, dp, n * sizeof(double), hipMemcpyDeviceToHost, st_));
         HIPT(hipStreamSynchronize(st_));
     }
 
@@ -108,16 +136,18 @@ class Medium final : public MediumBase {
     void phase_sample(const double *wo, size_t n, uint64_t *rng, double *wi,
                       double *pdf) override {
         if (n == 0) return;
-        DevBuf<double> dwo(3 * n), dwi(3 * n), dp(n);
-        DevBuf<uint64_t> ds(4 * n);
-        HIPT(hipMemcpyAsync(dwo.p, wo, 3 * n * sizeof(double), hipMemcpyHostToDevice, st_));
-        HIPT(hipMemcpyAsync(ds.p, rng, 4 * n * sizeof(uint64_t), hipMemcpyHostToDevice, st_));
-        k_phase_sample<<<blocks(n), kThreads, 0, st_>>>(phase_g, dwo.p, (long)n, ds.p, dwi.p,
-                                                        dp.p);
+        scratch_.reserve(3 * n * sizeof(double), 3 * n * sizeof(double), n * sizeof(double),
+                         4 * n * sizeof(uint64_t));
+        double *dwo = scratch_.take<double>(3 * n), *dwi = scratch_.take<double>(3 * n);
+        double *dp = scratch_.take<double>(n);
+        uint64_t *ds = scratch_.take<uint64_t>(4 * n);
+        HIPT(hipMemcpyAsync(dwo, wo, 3 * n * sizeof(double), hipMemcpyHostToDevice, st_));
+        HIPT(hipMemcpyAsync(ds, rng, 4 * n * sizeof(uint64_t), hipMemcpyHostToDevice, st_));
+        k_phase_sample<<<blocks(n), kThreads, 0, st_>>>(phase_g, dwo, (long)n, ds, dwi, dp);
         LAUNCH_CHECK();
-        HIPT(hipMemcpyAsync(wi, dwi.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost, st_));
-        HIPT(hipMemcpyAsync(pdf, dp.p, n * sizeof(double), hipMemcpyDeviceToHost, st_));
-        HIPT(hipMemcpyAsync(rng, ds.p, 4 * n * sizeof(uint64_t), hipMemcpyDeviceToHost, st_));
+        HIPT(hipMemcpyAsync(wi, dwi, 3 * n * sizeof(double), hipMemcpyDeviceToHost, st_));
+        HIPT(hipMemcpyAsync(pdf, dp, n * sizeof(double), hipMemcpyDeviceToHost, st_));
+        HIPT(hipMemcpyAsync(rng, ds, 4 * n * sizeof(uint64_t), hipMemcpyDeviceToHost, st_));
         HIPT(hipStreamSynchronize(st_));
     }
 
@@ -125,6 +155,7 @@ class Medium final : public MediumBase {
     Dims g_;
     Box3 box_;
     DevBuf<T> rho_, temp_;
+    Arena scratch_;
     hipStream_t st_ = nullptr;
 };
 
