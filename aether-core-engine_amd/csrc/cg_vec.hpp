@@ -418,7 +418,7 @@ __device__ __forceinline__ float4 sor_colour4(const Dims &g, int i0, int j, int 
         if (j + 2 < g.ny) { diag += 1.0f; nb += f4(yp, e); }
         if (k > 1) { diag += 1.0f; nb += f4(zm, e); }
         if (k + 2 < g.nz) { diag += 1.0f; nb += f4(zp, e); }
-        const float gs = (f4(bb, e) + nb) / diag;
+        const float gs = sor_quot(f4(bb, e) + nb, diag);
         o[e] = (1.0f - omega) * o[e] + omega * gs;
     }
     return make_float4(o[0], o[1], o[2], o[3]);

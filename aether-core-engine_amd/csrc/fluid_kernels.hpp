@@ -1130,6 +1130,16 @@ __global__ __launch_bounds__(kThreads) void k_cg_finalize(int max_iters, double 
 
 // Red-black SOR sweep of one colour (src/fluid.rs:400-437).  Same-colour
 // cells are independent, so this equals the reference's sequential loop.
+// (b + sum of neighbours) / diag of the Gauss-Seidel update
+// (src/fluid.rs:434).  FP64 divides, as the reference does.  FP32
+// multiplies by the hardware reciprocal of diag (an integer 1..6): within two
+// ulps of the quotient, for ~2 instructions instead of an IEEE division's ~10,
+// which made up a quarter of the FP32 sweep's vector instructions.
+__device__ __forceinline__ double sor_quot(double x, double diag) { return x / diag; }
+__device__ __forceinline__ float sor_quot(float x, float diag) {
+    return x * __builtin_amdgcn_rcpf(diag);
+}
+
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_rbgs(Dims g, int color, T omega,
                                                    const T *__restrict__ b, T *__restrict__ p) {
@@ -1146,7 +1156,7 @@ __global__ __launch_bounds__(kThreads) void k_rbgs(Dims g, int color, T omega,
     if (j + 2 < g.ny) { diag += T(1); nb += p[id + g.nx]; }
     if (k > 1) { diag += T(1); nb += p[id - g.sxy]; }
     if (k + 2 < g.nz) { diag += T(1); nb += p[id + g.sxy]; }
-    const T gs = (b[id] + nb) / diag;
+    const T gs = sor_quot(b[id] + nb, diag);
     p[id] = (T(1) - omega) * p[id] + omega * gs;
 }
 
