@@ -54,6 +54,8 @@ class Params(C.Structure):
         ("ir_div_tol", C.c_double),
         ("ir_max_outer", C.c_int32),
         ("batch_graph", C.c_int32),
+        ("cg_z_cells", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 
@@ -151,7 +153,8 @@ SIGNATURES = {
     "aether_fluid_nonfinite": (C.c_int, [_vp, C.POINTER(C.c_int)]),
     "aether_fluid_download": (C.c_int, [_vp, C.c_int, _pd, C.c_size_t]),
     "aether_fluid_max_density": (C.c_int, [_vp, _pd]),
-    "aether_fluid_batch_autotune_cg": (C.c_int, [_vp, C.c_int, _pd, C.POINTER(C.c_int)]),
+    "aether_fluid_batch_autotune_cg": (C.c_int, [_vp, C.c_int, _pd, C.POINTER(C.c_int),
+                                                 C.POINTER(C.c_int)]),
     "aether_fluid_autotune_cg": (C.c_int, [_vp, C.c_int, _pd, C.POINTER(C.c_int),
                                            C.POINTER(C.c_int)]),
     "aether_fluid_medium_create": (C.c_int, [_vp, _pd, _pd, C.c_double, C.c_double,

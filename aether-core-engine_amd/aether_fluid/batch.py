@@ -14,13 +14,15 @@ class FluidBatch:
     """instances: list of dicts with keys nx, ny, nz and optionally h,
     vorticity_eps, buoyancy_beta, gravity_alpha, source_center,
     source_radius, source_density, source_temperature.  Shared knobs
-    (solver_iters, cg_rel_tol, dissipation, cg_form) come from keyword
-    arguments; cg_form CG_AUTO runs the four-wide CG when every nx % 4 == 0
-    (FP32).  graph=True captures each step into a HIP graph and replays it."""
+    (solver_iters, cg_rel_tol, dissipation, cg_form, cg_z_cells) come from
+    keyword arguments; cg_form CG_AUTO runs the four-wide CG when every
+    nx % 4 == 0 (FP32), with cg_z_cells (2 or 4) z planes per thread, 4 when
+    0 and not tuned.  graph=True captures each step into a HIP graph and
+    replays it."""
 
     def __init__(self, instances, precision: str = "fp32", solver_iters: int = 200,
                  cg_rel_tol: float = 1e-5, dissipation=(0.996, 0.996, 0.985),
-                 cg_form: int = L.CG_AUTO, graph: bool = False):
+                 cg_form: int = L.CG_AUTO, graph: bool = False, cg_z_cells: int = 0):
         self._lib = L.lib()
         n = len(instances)
         descs = (L.InstanceDesc * n)()
@@ -39,6 +41,7 @@ class FluidBatch:
         L.check(self._lib.aether_fluid_default_params(C.byref(p)))
         p.solver_kind, p.solver_iters, p.cg_rel_tol = 0, solver_iters, cg_rel_tol
         p.cg_form = cg_form
+        p.cg_z_cells = cg_z_cells
         p.batch_graph = 1 if graph else 0
         (p.dissipation_velocity, p.dissipation_density,
          p.dissipation_temperature) = dissipation
@@ -84,15 +87,19 @@ class FluidBatch:
                 for s in range(len(ts))]
 
     def autotune_cg(self, iters: int = 20) -> dict:
-        """Times `iters` CG iterations of the batch in each layout on synthetic
-        right-hand sides and keeps the faster as the batch's CG_AUTO choice
-        (pressure and divergence are restored)."""
-        us = (C.c_double * 2)()
-        form = C.c_int()
+        """Times `iters` CG iterations of the batch in each candidate layout
+        and tiling (four-wide with 2 and 4 z planes per thread, one cell per
+        thread) on synthetic right-hand sides and keeps the fastest as the
+        batch's choice for CG_AUTO and cg_z_cells 0 (pressure and divergence
+        are restored)."""
+        us = (C.c_double * 3)()
+        form, z = C.c_int(), C.c_int()
         L.check(self._lib.aether_fluid_batch_autotune_cg(self._h, int(iters), us,
-                                                         C.byref(form)))
+                                                         C.byref(form), C.byref(z)))
         return {"choice": {L.CG_X4: "x4", L.CG_SCALAR: "scalar"}[form.value],
-                "us_per_iter": {k: round(us[a], 3) for a, k in enumerate(("x4", "scalar"))
+                "z_cells": z.value,
+                "us_per_iter": {k: round(us[a], 3)
+                                for a, k in enumerate(("x4_z2", "x4_z4", "scalar_z8"))
                                 if us[a] >= 0}}
 
     def iterations(self) -> np.ndarray:

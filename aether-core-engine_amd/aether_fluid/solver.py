@@ -51,7 +51,7 @@ _PARAM_NAMES = ("vorticity_eps", "buoyancy_beta", "gravity_alpha", "solver_kind"
                 "solver_iters", "sor_omega", "cg_rel_tol", "dissipation_velocity",
                 "dissipation_density", "dissipation_temperature", "source_kind",
                 "source_center", "source_radius", "source_density", "source_temperature",
-                "ir_inner_rel_tol", "ir_div_tol", "ir_max_outer", "cg_form")
+                "ir_inner_rel_tol", "ir_div_tol", "ir_max_outer", "cg_form", "cg_z_cells")
 
 
 def _dptr(a: np.ndarray):
@@ -101,7 +101,7 @@ class FluidSolver3D:
             if name == "source_center":
                 p.source_center = (C.c_double * 3)(*[float(x) for x in val])
             elif name in ("solver_kind", "solver_iters", "source_kind", "ir_max_outer",
-                          "cg_form"):
+                          "cg_form", "cg_z_cells"):
                 setattr(p, name, int(val))
             else:
                 setattr(p, name, float(val))

@@ -22,7 +22,8 @@ class Batch final : public BatchBase {
         double h;
         size_t n;
         int gx, gy, gz, nb;
-        VTiling vt = {0, 0, 0, 0, 0, 1, 1};  // four-wide CG tiling (nx % 4 == 0)
+        // four-wide CG tilings (nx % 4 == 0) with 2 and 4 z planes per thread
+        VTiling vtz[2] = {{0, 0, 0, 0, 0, 1, 1}, {0, 0, 0, 0, 0, 1, 1}};
         aether_fluid_instance_desc d;
         DevBuf<T> f[2][5], fwd0, fwd1, curl, p, b, r, dd[2];
         DevBuf<double> pa, pb, pc, ptel, tel, staging;
@@ -34,7 +35,7 @@ class Batch final : public BatchBase {
     Batch(const aether_fluid_instance_desc *desc, int count, const aether_fluid_params &shared,
           hipStream_t stream)
         : params_(shared), st_(stream) {
-        std::vector<BItem> cells, cg, cg4;
+        std::vector<BItem> cells, cg, cg4[2];
         std::vector<int2> src;
         // four-wide CG: FP32 and every nx % 4 == 0
         x4_ok_ = sizeof(T) == 4;
@@ -56,18 +57,21 @@ class Batch final : public BatchBase {
             in->gy = (d.ny + kBY - 1) / kBY;
             in->gz = (d.nz + kBatchZU - 1) / kBatchZU;
             in->nb = in->gx * in->gy * in->gz;
-            if (x4_ok_) {  // as the single-grid four-wide tiling (4 z cells per thread)
-                VTiling &v = in->vt;
-                v.txv = std::min(d.nx / 4, 64);
-                v.ty = kThreads / v.txv;
-                v.gx = (d.nx / 4 + v.txv - 1) / v.txv;
-                v.gy = (d.ny + v.ty - 1) / v.ty;
-                v.gz = (d.nz + 3) / 4;
-                v.nb = v.gx * v.gy * v.gz;
-                v.remap = (v.nb % 8 == 0) ? 1 : 0;
-                wide_ = wide_ || std::max(in->nb, v.nb) > 4 * kThreads;
+            if (x4_ok_) {  // as the single-grid four-wide tilings
+                for (int zi = 0; zi < 2; ++zi) {
+                    VTiling &v = in->vtz[zi];
+                    const int vz = zi == 0 ? 2 : 4;
+                    v.txv = std::min(d.nx / 4, 64);
+                    v.ty = kThreads / v.txv;
+                    v.gx = (d.nx / 4 + v.txv - 1) / v.txv;
+                    v.gy = (d.ny + v.ty - 1) / v.ty;
+                    v.gz = (d.nz + vz - 1) / vz;
+                    v.nb = v.gx * v.gy * v.gz;
+                    v.remap = (v.nb % 8 == 0) ? 1 : 0;
+                    wide_[zi] = wide_[zi] || std::max(in->nb, v.nb) > 4 * kThreads;
+                }
             }
-            const int npart = std::max(in->nb, in->vt.nb);
+            const int npart = std::max(in->nb, std::max(in->vtz[0].nb, in->vtz[1].nb));
             for (int a = 0; a < 2; ++a)
                 for (int f = 0; f < 5; ++f) in->f[a][f].alloc(in->n);
             for (DevBuf<T> *b : {&in->fwd0, &in->fwd1, &in->curl, &in->p, &in->b, &in->r,
@@ -96,14 +100,15 @@ class Batch final : public BatchBase {
                 const int bx = t % in->gx, rest = t / in->gx;
                 cg.push_back({q, bx, rest % in->gy, rest / in->gy, b, {0, 0, 0}});
             }
-            if (x4_ok_) {
-                const VTiling &v = in->vt;
-                for (int b = 0; b < v.nb; ++b) {
-                    const int t = v.remap ? (b & 7) * (v.nb >> 3) + (b >> 3) : b;
-                    const int bx = t % v.gx, rest = t / v.gx;
-                    cg4.push_back({q, bx, rest % v.gy, rest / v.gy, b, {0, 0, 0}});
+            if (x4_ok_)
+                for (int zi = 0; zi < 2; ++zi) {
+                    const VTiling &v = in->vtz[zi];
+                    for (int b = 0; b < v.nb; ++b) {
+                        const int t = v.remap ? (b & 7) * (v.nb >> 3) + (b >> 3) : b;
+                        const int bx = t % v.gx, rest = t / v.gx;
+                        cg4[zi].push_back({q, bx, rest % v.gy, rest / v.gy, b, {0, 0, 0}});
+                    }
                 }
-            }
             // sphere source box (interior cells)
             if (d.source_radius > 0.0) {
                 const int nn[3] = {d.nx, d.ny, d.nz};
@@ -140,11 +145,13 @@ class Batch final : public BatchBase {
         nsrc_ = (int)src.size();
         cell_items_.alloc(ncell_);
         cg_items_.alloc(ncg_);
-        ncg4_ = (int)cg4.size();
-        if (ncg4_) {
-            cg4_items_.alloc(ncg4_);
-            HIPT(hipMemcpyAsync(cg4_items_.p, cg4.data(), cg4.size() * sizeof(BItem),
-                                hipMemcpyHostToDevice, st_));
+        for (int zi = 0; zi < 2; ++zi) {
+            ncg4_[zi] = (int)cg4[zi].size();
+            if (ncg4_[zi]) {
+                cg4_items_[zi].alloc(ncg4_[zi]);
+                HIPT(hipMemcpyAsync(cg4_items_[zi].p, cg4[zi].data(),
+                                    cg4[zi].size() * sizeof(BItem), hipMemcpyHostToDevice, st_));
+            }
         }
         HIPT(hipMemcpyAsync(cell_items_.p, cells.data(), cells.size() * sizeof(BItem),
                             hipMemcpyHostToDevice, st_));
@@ -249,8 +256,8 @@ class Batch final : public BatchBase {
             t.stemp = in.d.source_temperature;
             t.tel = in.tel.p;
             t.iters = in.iters.p;
-            t.vt = in.vt;
-            t.nbv = in.vt.nb;
+            t.vt = in.vtz[vzi()];
+            t.nbv = t.vt.nb;
         }
         HIPT(hipMemcpyAsync(table_.p, tab.data(), tab.size() * sizeof(BInst<T>),
                             hipMemcpyHostToDevice, st_));
@@ -266,8 +273,22 @@ class Batch final : public BatchBase {
         *slot[f] ^= 1;
     }
 
+    // The four-wide tiling in use (0: 2 z planes per thread, 1: 4): the
+    // autotuner's trial, else cg_z_cells, else the autotuner's choice, else 4.
+    int vzi() const {
+        int vz = 4;
+        if (trial_vz_)
+            vz = trial_vz_;
+        else if (params_.cg_z_cells == 2 || params_.cg_z_cells == 4)
+            vz = params_.cg_z_cells;
+        else if (tuned_vz_)
+            vz = tuned_vz_;
+        return vz == 2 ? 0 : 1;
+    }
+
     // The batch's CG layout for the current knobs: four-wide or one cell per thread.
     bool use_x4() const {
+        if (trial_form_) return trial_form_ == AETHER_CG_X4;
         switch (params_.cg_form) {
         case AETHER_CG_SCALAR:
             return false;
@@ -277,25 +298,23 @@ class Batch final : public BatchBase {
         case AETHER_CG_FUSED:
             raise(AETHER_EINVAL, "AETHER_CG_FUSED is not available for batches");
         default:
-            if (trial_form_) return trial_form_ == AETHER_CG_X4;
             if (tuned_form_) return tuned_form_ == AETHER_CG_X4;
             return x4_ok_;
         }
     }
 
-    // Times `iters` CG iterations of the batch in each layout (four-wide when
-    // every nx % 4 == 0, one cell per thread) on synthetic right-hand sides
-    // (every instance's pressure and divergence are saved and restored) and
-    // keeps the faster as the AETHER_CG_AUTO choice.  us[2]: time per
-    // iteration of the whole batch, four-wide then one cell per thread (-1
-    // when not applicable).
-    void autotune(int iters, double *us, int *form_out) override {
+    // Times `iters` CG iterations of the batch in each candidate (four-wide
+    // with 2 and 4 z planes per thread when every nx % 4 == 0, one cell per
+    // thread with 8) on synthetic right-hand sides (every instance's pressure
+    // and divergence are saved and restored) and keeps the fastest as the
+    // choice for cg_form AETHER_CG_AUTO and cg_z_cells 0.  us[3]: time per
+    // iteration of the whole batch, in that order (-1 when not applicable).
+    void autotune(int iters, double *us, int *form_out, int *z_out) override {
         if (iters < 1) raise(AETHER_EINVAL, "iters must be >= 1");
         for (auto &in : inst_) {  // finalize logs iteration 0's count
             if (in->iters.n < 1) in->iters.alloc(1);
             if (in->tel.n < 8) in->tel.alloc(8);
         }
-        write_table();
         std::vector<std::unique_ptr<DevBuf<T>>> saved;
         for (auto &in : inst_) {
             for (DevBuf<T> *src : {&in->p, &in->b}) {
@@ -307,25 +326,31 @@ class Batch final : public BatchBase {
             LAUNCH_CHECK();
         }
         const aether_fluid_params keep = params_;
-        params_.cg_rel_tol = 0.0;  // absolute guards only: each layout runs `iters`
+        params_.cg_rel_tol = 0.0;  // absolute guards only: each candidate runs `iters`
         params_.solver_iters = iters;
         hipEvent_t e0, e1;
         HIPT(hipEventCreate(&e0));
         HIPT(hipEventCreate(&e1));
-        const int forms[2] = {AETHER_CG_X4, AETHER_CG_SCALAR};
+        struct Cand {
+            int form, z;
+        };
+        const Cand cands[3] = {{AETHER_CG_X4, 2}, {AETHER_CG_X4, 4}, {AETHER_CG_SCALAR, kBatchZU}};
         double best = 1e300;
-        int pick = AETHER_CG_SCALAR;
+        Cand pick = {AETHER_CG_SCALAR, kBatchZU};
         auto cleanup = [&] {
             trial_form_ = 0;
+            trial_vz_ = 0;
             params_ = keep;
             (void)hipEventDestroy(e0);
             (void)hipEventDestroy(e1);
         };
         try {
-            for (int a = 0; a < 2; ++a) {
+            for (int a = 0; a < 3; ++a) {
                 if (us) us[a] = -1.0;
-                if (forms[a] == AETHER_CG_X4 && !x4_ok_) continue;
-                trial_form_ = forms[a];
+                if (cands[a].form == AETHER_CG_X4 && !x4_ok_) continue;
+                trial_form_ = cands[a].form;
+                trial_vz_ = cands[a].form == AETHER_CG_X4 ? cands[a].z : 0;
+                write_table();  // the table holds the four-wide tiling in use
                 float ms = 0.0f;
                 for (int rep = 0; rep < 2; ++rep) {  // the first run warms up
                     for (auto &in : inst_) fill(in->p.p, in->n);
@@ -342,7 +367,7 @@ class Batch final : public BatchBase {
                 if (us) us[a] = t;
                 if (t < best) {
                     best = t;
-                    pick = forms[a];
+                    pick = cands[a];
                 }
             }
         } catch (...) {
@@ -350,7 +375,9 @@ class Batch final : public BatchBase {
             throw;
         }
         cleanup();
-        tuned_form_ = pick;
+        tuned_form_ = pick.form;
+        if (pick.form == AETHER_CG_X4) tuned_vz_ = pick.z;
+        write_table();
         size_t k = 0;
         for (auto &in : inst_)
             for (DevBuf<T> *dst : {&in->p, &in->b}) {
@@ -359,38 +386,48 @@ class Batch final : public BatchBase {
                 ++k;
             }
         HIPT(hipStreamSynchronize(st_));
-        if (form_out) *form_out = pick;
+        if (form_out) *form_out = pick.form;
+        if (z_out) *z_out = pick.z;
     }
 
     void launch_cg4(int it, double rel2, int solve_id) {
         if constexpr (sizeof(T) == 4) {
             // p is brought up to date on odd iterations (kb_cg_update4)
             const bool odd = it & 1;
-            const dim3 g(ncg4_), b(kThreads);
-#define KB_DIR4(NQ_)                                                                       \
-    do {                                                                                   \
-        if (it == 0)                                                                       \
-            kb_cg_dir4<true, NQ_, false><<<g, b, 0, st_>>>(table_.p, cg4_items_.p, it,     \
-                                                           rel2, done_dev_, solve_id);     \
-        else if (odd)                                                                      \
-            kb_cg_dir4<false, NQ_, true><<<g, b, 0, st_>>>(table_.p, cg4_items_.p, it,     \
-                                                           rel2, done_dev_, solve_id);     \
-        else                                                                               \
-            kb_cg_dir4<false, NQ_, false><<<g, b, 0, st_>>>(table_.p, cg4_items_.p, it,    \
-                                                            rel2, done_dev_, solve_id);    \
-        LAUNCH_CHECK();                                                                    \
-        if (odd)                                                                           \
-            kb_cg_update4<NQ_, kPBoth><<<g, b, 0, st_>>>(table_.p, cg4_items_.p, it,       \
-                                                         done_dev_, solve_id);             \
-        else                                                                               \
-            kb_cg_update4<NQ_, kPSkip><<<g, b, 0, st_>>>(table_.p, cg4_items_.p, it,       \
-                                                         done_dev_, solve_id);             \
-        LAUNCH_CHECK();                                                                    \
+            const int zi = vzi();
+            const dim3 g(ncg4_[zi]), b(kThreads);
+            const BItem *items = cg4_items_[zi].p;
+#define KB_DIR4(VZ_, NQ_)                                                                    \
+    do {                                                                                     \
+        if (it == 0)                                                                         \
+            kb_cg_dir4<VZ_, true, NQ_, false><<<g, b, 0, st_>>>(table_.p, items, it, rel2,   \
+                                                                done_dev_, solve_id);        \
+        else if (odd)                                                                        \
+            kb_cg_dir4<VZ_, false, NQ_, true><<<g, b, 0, st_>>>(table_.p, items, it, rel2,   \
+                                                                done_dev_, solve_id);        \
+        else                                                                                 \
+            kb_cg_dir4<VZ_, false, NQ_, false><<<g, b, 0, st_>>>(table_.p, items, it, rel2,  \
+                                                                 done_dev_, solve_id);       \
+        LAUNCH_CHECK();                                                                      \
+        if (odd)                                                                             \
+            kb_cg_update4<VZ_, NQ_, kPBoth><<<g, b, 0, st_>>>(table_.p, items, it,           \
+                                                              done_dev_, solve_id);          \
+        else                                                                                 \
+            kb_cg_update4<VZ_, NQ_, kPSkip><<<g, b, 0, st_>>>(table_.p, items, it,           \
+                                                              done_dev_, solve_id);          \
+        LAUNCH_CHECK();                                                                      \
     } while (0)
-            if (wide_)
-                KB_DIR4(16);
-            else
-                KB_DIR4(4);
+            if (zi == 0) {
+                if (wide_[0])
+                    KB_DIR4(2, 16);
+                else
+                    KB_DIR4(2, 4);
+            } else {
+                if (wide_[1])
+                    KB_DIR4(4, 16);
+                else
+                    KB_DIR4(4, 4);
+            }
 #undef KB_DIR4
         }
     }
@@ -400,6 +437,11 @@ class Batch final : public BatchBase {
     void cg_solve(int step, bool graph = false) {
         const int max_iters = params_.solver_iters;
         const bool x4 = use_x4();
+        const int zc = params_.cg_z_cells;
+        if (!trial_form_ && zc && (x4 ? zc != 2 && zc != 4 : zc != kBatchZU))
+            raise(AETHER_EINVAL, x4 ? "the batch's four-wide CG takes cg_z_cells 2 or 4"
+                                    : "the batch's one-cell-per-thread CG runs 8 z cells per "
+                                      "thread (cg_z_cells 0 or 8)");
         const double rel2 = params_.cg_rel_tol > 0 ? params_.cg_rel_tol * params_.cg_rel_tol : 0.0;
         const int solve_id = ++solve_seq_;
         const int chunk = 8;
@@ -439,7 +481,13 @@ class Batch final : public BatchBase {
         }
         if constexpr (sizeof(T) == 4) {
             if (x4 && (max_iters & 1)) {  // an odd cap leaves the last alpha d pending
-                kb_cg_p_tail<<<ncg4_, kThreads, 0, st_>>>(table_.p, cg4_items_.p, max_iters - 1);
+                const int zi = vzi();
+                if (zi == 0)
+                    kb_cg_p_tail<2><<<ncg4_[0], kThreads, 0, st_>>>(table_.p, cg4_items_[0].p,
+                                                                   max_iters - 1);
+                else
+                    kb_cg_p_tail<4><<<ncg4_[1], kThreads, 0, st_>>>(table_.p, cg4_items_[1].p,
+                                                                   max_iters - 1);
                 LAUNCH_CHECK();
             }
         }
@@ -583,15 +631,17 @@ class Batch final : public BatchBase {
     std::vector<std::unique_ptr<Inst>> inst_;
     std::vector<int> box_lo_, box_n_;
     DevBuf<BInst<T>> table_;
-    DevBuf<BItem> cell_items_, cg_items_, cg4_items_;
+    DevBuf<BItem> cell_items_, cg_items_, cg4_items_[2];
     DevBuf<int2> src_items_;
-    int ncell_ = 0, ncg_ = 0, nsrc_ = 0, ncg4_ = 0;
+    int ncell_ = 0, ncg_ = 0, nsrc_ = 0, ncg4_[2] = {0, 0};
     bool x4_ok_ = false;  // FP32 and every nx % 4 == 0: four-wide CG available
     int tuned_form_ = 0;  // the autotuner's AETHER_CG_AUTO choice (0: none)
     int trial_form_ = 0;  // layout forced while the autotuner times it
     DevBuf<int> step_dev_;          // device step index of captured steps
     DevBuf<double> times_dev_;      // their time axis
-    bool wide_ = false;   // some instance has more than 1024 CG partials
+    bool wide_[2] = {false, false};  // some instance has more than 1024 CG partials
+    int tuned_vz_ = 0;    // the autotuner's z planes per thread of the four-wide tiling
+    int trial_vz_ = 0;    // planes forced while the autotuner times them
     Slots sl_ = {0, 0, 0, 0, 0};
     int *done_host_ = nullptr, *done_dev_ = nullptr;
     int solve_seq_ = 0;

@@ -189,29 +189,30 @@ __global__ __launch_bounds__(kThreads) void kb_cg_update(const BInst<T> *__restr
 // computes bit for bit what the instances compute one at a time.
 // PEND / PM: p is brought up to date on odd iterations only, as in the
 // single-grid four-wide CG (same arithmetic, so batches stay bit-identical
-// to the instances run one at a time).
-template <bool FIRST, int NQ, bool PEND>
+// to the instances run one at a time).  VZ: z planes per thread of the
+// tiling (the table's vt).
+template <int VZ, bool FIRST, int NQ, bool PEND>
 __global__ __launch_bounds__(kThreads) void kb_cg_dir4(const BInst<float> *__restrict__ I,
                                                        const BItem *__restrict__ items, int iter,
                                                        double rel_tol2, int *done_mirror,
                                                        int solve_id) {
     const BItem it = items[blockIdx.x];
     const BInst<float> &in = I[it.inst];
-    body_cg_dir4<4, FIRST, NQ, kStWT, PEND>(vcol_at<4>(in.g, in.vt, it.bx, it.by, it.bz),
+    body_cg_dir4<VZ, FIRST, NQ, kStWT, PEND>(vcol_at<VZ>(in.g, in.vt, it.bx, it.by, it.bz),
                                             it.local, in.g, iter, rel_tol2,
                                             FIRST ? in.nb : in.nbv, in.pb, in.pc, in.st, in.r,
                                             in.d[(iter + 1) & 1], in.d[iter & 1], in.pa,
                                             done_mirror + it.inst, solve_id, in.p);
 }
 
-template <int NQ, int PM>
+template <int VZ, int NQ, int PM>
 __global__ __launch_bounds__(kThreads) void kb_cg_update4(const BInst<float> *__restrict__ I,
                                                           const BItem *__restrict__ items,
                                                           int iter, int *done_mirror,
                                                           int solve_id) {
     const BItem it = items[blockIdx.x];
     const BInst<float> &in = I[it.inst];
-    body_cg_update4<4, NQ, kStWT, PM>(vcol_at<4>(in.g, in.vt, it.bx, it.by, it.bz), it.local,
+    body_cg_update4<VZ, NQ, kStWT, PM>(vcol_at<VZ>(in.g, in.vt, it.bx, it.by, it.bz), it.local,
                                       in.nbv, in.g, iter, in.pa, in.st, in.d[iter & 1], in.p,
                                       in.r, in.pb, done_mirror + it.inst, solve_id,
                                       in.d[(iter + 1) & 1]);
@@ -219,18 +220,19 @@ __global__ __launch_bounds__(kThreads) void kb_cg_update4(const BInst<float> *__
 
 // Closes, per instance, a four-wide solve that an odd iteration cap ended
 // after an even launch: p += alpha_last d_last over the instance's tiles.
+template <int VZ>
 __global__ __launch_bounds__(kThreads) void kb_cg_p_tail(const BInst<float> *__restrict__ I,
                                                          const BItem *__restrict__ items,
                                                          int last_it) {
     const BItem it = items[blockIdx.x];
     const BInst<float> &in = I[it.inst];
     if (in.st->done) return;
-    const VCol c = vcol_at<4>(in.g, in.vt, it.bx, it.by, it.bz);
+    const VCol c = vcol_at<VZ>(in.g, in.vt, it.bx, it.by, it.bz);
     const float ta = (float)in.st->cg_alpha[last_it & 1];
     float4 *p4 = reinterpret_cast<float4 *>(in.p);
     const float4 *d4 = reinterpret_cast<const float4 *>(in.d[last_it & 1]);
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
+    for (int a = 0; a < VZ; ++a) {
         const int k = c.k0 + a;
         if (c.valid && k < in.g.nz) {
             const int id4 = (k * in.g.sxy + c.off) >> 2;

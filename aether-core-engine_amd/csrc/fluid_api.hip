@@ -315,11 +315,11 @@ int aether_fluid_max_density(aether_fluid_t s, double *out) {
 }
 
 int aether_fluid_batch_autotune_cg(aether_fluid_batch_t b, int iters, double *us_per_iter,
-                                   int *form) {
+                                   int *form, int *z_cells) {
     return guarded([&] {
         if (!b) raise(AETHER_EINVAL, "null batch handle");
         HIPT(hipSetDevice(b->device));
-        b->impl->autotune(iters, us_per_iter, form);
+        b->impl->autotune(iters, us_per_iter, form, z_cells);
     });
 }
 

@@ -141,6 +141,9 @@ inline void check_params(const aether_fluid_params &p) {
     if (p.source_kind < AETHER_SOURCE_JETS || p.source_kind > AETHER_SOURCE_NONE)
         raise(AETHER_EINVAL, "unknown source_kind");
     if (!(p.cg_rel_tol >= 0.0)) raise(AETHER_EINVAL, "cg_rel_tol must be >= 0");
+    if (p.cg_z_cells != 0 && p.cg_z_cells != 2 && p.cg_z_cells != 4 && p.cg_z_cells != 8 &&
+        p.cg_z_cells != 16)
+        raise(AETHER_EINVAL, "cg_z_cells must be 0, 2, 4, 8 or 16");
     if (p.source_kind == AETHER_SOURCE_SPHERE &&
         !(std::isfinite(p.source_center[0]) && std::isfinite(p.source_center[1]) &&
           std::isfinite(p.source_center[2]) && std::isfinite(p.source_radius) &&
@@ -243,7 +246,7 @@ class SolverBase {
 class BatchBase {
    public:
     virtual ~BatchBase() = default;
-    virtual void autotune(int iters, double *us, int *form) = 0;
+    virtual void autotune(int iters, double *us, int *form, int *z) = 0;
     virtual void upload(int inst, int field, const double *src, size_t n) = 0;
     virtual void download(int inst, int field, double *dst, size_t n) = 0;
     virtual void run(double dt, const double *times, int nsteps, aether_fluid_telemetry *out) = 0;

@@ -46,6 +46,7 @@ class Solver final : public SolverBase {
             const int z = std::atoi(e);
             if (z == 4 || z == 8 || z == 16) zu_ = z;
         }
+        zu_default_ = zu_;
         tl_.gx = gx;
         tl_.gy = gy;
         tl_.gz = (nz + zu_ - 1) / zu_;
@@ -426,6 +427,18 @@ class Solver final : public SolverBase {
         }
     }
 
+    // Tilings of the next solve: cg_z_cells when it is set (the four-wide
+    // layout takes 2 or 4, the one-cell-per-thread layout 4, 8 or 16), else
+    // the autotuner's choice, else the defaults chosen at creation.
+    void sync_tilings() {
+        const int zc = params.cg_z_cells;
+        int zu = tuned_zu_ ? tuned_zu_ : zu_default_;
+        int vzu = tuned_vzu_ ? tuned_vzu_ : 4;
+        if (zc == 2 || zc == 4) vzu = zc;
+        if (zc == 4 || zc == 8 || zc == 16) zu = zc;
+        if (zu != zu_ || (tlv_.nb > 0 && vzu != vzu_)) set_tilings(zu, vzu);
+    }
+
     // Times `iters` CG iterations of every candidate layout on a synthetic
     // right-hand side of this grid (p and b are saved and restored) and
     // keeps the fastest as this handle's AETHER_CG_AUTO choice.  us[6]
@@ -499,6 +512,8 @@ class Solver final : public SolverBase {
         set_tilings(pick.form == AETHER_CG_SCALAR ? pick.z : zu0,
                     pick.form == AETHER_CG_X4 ? pick.z : vzu0);
         tuned_form_ = fp32 ? pick.form : 0;
+        tuned_zu_ = zu_;
+        tuned_vzu_ = vzu_;
         HIPT(hipMemcpyAsync(p_.p, saved_p.p, n_ * sizeof(T), hipMemcpyDeviceToDevice, st_));
         HIPT(hipMemcpyAsync(b_.p, saved_b.p, n_ * sizeof(T), hipMemcpyDeviceToDevice, st_));
         sync();
@@ -852,6 +867,7 @@ class Solver final : public SolverBase {
     // advection pass that follows (enqueue_advect_scalars with_gradient);
     // returns whether it was left (never for the mixed-precision projection)
     bool enqueue_project(int step_index, bool defer_gradient = false) {
+        sync_tilings();
         if (ir64_) {
             project_ir(step_index);
             return false;
@@ -885,6 +901,7 @@ class Solver final : public SolverBase {
 
     void solve_cg(int iters) override {
         if (iters < 0) raise(AETHER_EINVAL, "iterations must be >= 0");
+        sync_tilings();
         ZU_DISPATCH(zu_, (k_div_cginit<T, ZU><<<tl_.nb, cell_blk_, 0, st_>>>(
                               g_, tl_, (T)h_, 0, cur_[0], cur_[1], cur_[2], p_.p, b_.p, r_.p,
                               partB_.p, partC_.p, cg_.p)));
@@ -1607,6 +1624,8 @@ class Solver final : public SolverBase {
     Tiling tl_;
     int zu_ = 4;
     int tuned_form_ = 0;  // the autotuner's AETHER_CG_AUTO choice (0: none)
+    int tuned_zu_ = 0, tuned_vzu_ = 0;  // its tilings (0: none)
+    int zu_default_ = 4;  // z cells per thread chosen at creation
     int trial_form_ = 0;  // layout forced while the autotuner times a candidate
     int form_ = AETHER_CG_SCALAR;  // layout of the solve being enqueued
     VTiling tlv_ = {0, 0, 0, 0, 0, 1, 1};  // four-wide CG tiling (nx % 4 == 0)

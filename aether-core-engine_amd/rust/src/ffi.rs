@@ -77,6 +77,8 @@ pub struct aether_fluid_params {
     pub ir_div_tol: f64,
     pub ir_max_outer: i32,
     pub batch_graph: i32,
+    pub cg_z_cells: i32,
+    pub reserved: i32,
 }
 
 #[repr(C)]
@@ -253,7 +255,8 @@ extern "C" {
     pub fn aether_fluid_batch_iterations(b: aether_fluid_batch_t, out: *mut i32, n: c_int,
                                          count: *mut c_int) -> c_int;
     pub fn aether_fluid_batch_autotune_cg(b: aether_fluid_batch_t, iters: c_int,
-                                          us_per_iter: *mut f64, form: *mut c_int) -> c_int;
+                                          us_per_iter: *mut f64, form: *mut c_int,
+                                          z_cells: *mut c_int) -> c_int;
 
     // differentiable rollouts
     pub fn aether_fluid_rollout(s: aether_fluid_t, d: *const aether_rollout_desc,

@@ -371,7 +371,8 @@ def bench_batched(steps: int, warmup: int, dt: float = 0.01) -> dict:
     rho_b = b.download(len(insts) - 1, "density")
     b.close()
     # the same batch with every step captured into a HIP graph and replayed
-    bg = af.FluidBatch(insts, solver_iters=200, cg_rel_tol=1e-5, graph=True, cg_form=form_b)
+    bg = af.FluidBatch(insts, solver_iters=200, cg_rel_tol=1e-5, graph=True, cg_form=form_b,
+                       cg_z_cells=tune_b["z_cells"])
     if warmup:
         bg.run(dt, times[:warmup])
     t0 = time.perf_counter()

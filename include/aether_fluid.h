@@ -97,6 +97,11 @@ typedef struct {
     int32_t batch_graph;            /* batches only: 1 = capture each step into a HIP graph
                                        and replay it (all solver_iters CG launches are in
                                        the graph; converged instances return at once) */
+    int32_t cg_z_cells;             /* z cells per thread of the CG tiling: 0 = the
+                                       layout's default or the autotuner's choice; the
+                                       four-wide two-launch layout takes 2 or 4, the
+                                       one-cell-per-thread layout 4, 8 or 16 (batches: 8) */
+    int32_t reserved;
 } aether_fluid_params;
 
 /* src/fluid.rs:6-15 FluidTelemetryFrame (center_of_mass flattened) */
@@ -314,14 +319,16 @@ int aether_fluid_batch_run(aether_fluid_batch_t b, double dt, const double *time
                            aether_fluid_telemetry *out);
 /* CG iterations of the last run, nsteps*count values, step-major. */
 int aether_fluid_batch_iterations(aether_fluid_batch_t b, int32_t *out, int n, int *count);
-/* CG layout autotuner of a batch: times `iters` CG iterations of the whole
- * batch in the four-wide layout (FP32, every nx % 4 == 0) and in the
- * one-cell-per-thread layout, on synthetic right-hand sides (every
- * instance's pressure and divergence are saved and restored), and keeps the
- * faster as the batch's AETHER_CG_AUTO choice.  us_per_iter[2] (may be
- * NULL): four-wide, one cell per thread (-1 when not applicable). */
+/* CG layout and tile autotuner of a batch: times `iters` CG iterations of
+ * the whole batch in the four-wide layout (FP32, every nx % 4 == 0) with 2
+ * and with 4 z planes per thread, and in the one-cell-per-thread layout (8 z
+ * cells per thread), on synthetic right-hand sides (every instance's
+ * pressure and divergence are saved and restored), and keeps the fastest as
+ * the batch's choice for cg_form AETHER_CG_AUTO and cg_z_cells 0.
+ * us_per_iter[3] (may be NULL): four-wide z2, four-wide z4, one cell per
+ * thread (-1 when not applicable).  *form / *z_cells: the choice. */
 int aether_fluid_batch_autotune_cg(aether_fluid_batch_t b, int iters, double *us_per_iter,
-                                   int *form);
+                                   int *form, int *z_cells);
 
 /* Differentiable rollouts (no counterpart in the reference).  Starting from
  * the solver's current state (which is left unchanged), run `nsteps` steps

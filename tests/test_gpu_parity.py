@@ -990,7 +990,7 @@ def test_autotune_keeps_state_and_results(precision):
 def test_batch_autotune_keeps_state_and_results():
     """The batch autotuner restores every instance's pressure and divergence;
     the steps after it match an untuned batch (bit for bit when it keeps the
-    four-wide layout, to the FP32 bar otherwise)."""
+    four-wide layout with 4 planes per thread, to the FP32 bar otherwise)."""
     _announce("batch autotune")
     import aether_fluid as af
     insts = [dict(nx=32, ny=24, nz=20, vorticity_eps=0.3, buoyancy_beta=1.0, gravity_alpha=0.05,
@@ -1006,7 +1006,8 @@ def test_batch_autotune_keeps_state_and_results():
     before = [bs[0].download(q, n) for q in range(2) for n in ("pressure", "divergence")]
     r = bs[0].autotune_cg(10)
     print("batch autotune", r)
-    assert set(r["us_per_iter"]) == {"x4", "scalar"}
+    assert set(r["us_per_iter"]) == {"x4_z2", "x4_z4", "scalar_z8"}
+    assert (r["choice"], r["z_cells"]) in (("x4", 2), ("x4", 4), ("scalar", 8))
     after = [bs[0].download(q, n) for q in range(2) for n in ("pressure", "divergence")]
     for x, y in zip(before, after):
         np.testing.assert_array_equal(x, y)
@@ -1015,7 +1016,7 @@ def test_batch_autotune_keeps_state_and_results():
     for q in range(2):
         for n in ("u", "v", "w", "density"):
             x, y = bs[0].download(q, n), bs[1].download(q, n)
-            if r["choice"] == "x4":
+            if r["choice"] == "x4" and r["z_cells"] == 4:
                 np.testing.assert_array_equal(x, y)
             else:
                 assert rel_l2(x, y) <= 1e-5, (q, n)
@@ -1054,14 +1055,15 @@ def test_batched_graph_replay_equals_stream_launches():
         np.testing.assert_array_equal(x, y)
 
 
-def _batched_vs_single(insts, single_form, case, cap=120, tol=1e-5):
+def _batched_vs_single(insts, single_form, case, cap=120, tol=1e-5, z_cells=0):
     """A batch of grids of different sizes gives, bit for bit, the fields and
     CG iteration counts of single-grid solvers run one at a time with the
-    same CG layout (and 8 z cells per thread in the initialisation tiling)."""
+    same CG layout and tiling (z_cells; 8 z cells per thread in the
+    initialisation tiling)."""
     _announce(case)
     import os
     import aether_fluid as af
-    b = af.FluidBatch(insts, solver_iters=cap, cg_rel_tol=tol)
+    b = af.FluidBatch(insts, solver_iters=cap, cg_rel_tol=tol, cg_z_cells=z_cells)
     states = []
     for q, d in enumerate(insts):
         st = random_state(d["nx"], d["ny"], d["nz"], seed=40 + q, vel=0.5)
@@ -1080,6 +1082,7 @@ def _batched_vs_single(insts, single_form, case, cap=120, tol=1e-5):
             s.set_sphere_source(d["source_center"], d["source_radius"])
             s.solver_iters, s.cg_rel_tol = cap, tol
             s.cg_form = single_form
+            s.cg_z_cells = z_cells
             for n in ("u", "v", "w", "density", "temperature"):
                 s.upload(n, states[q][n])
             single = s.run(0.02, times)
@@ -1125,6 +1128,28 @@ def test_batched_four_wide_equals_one_at_a_time_bitwise():
     # whose alpha d term the batch's tail kernel applies
     _batched_vs_single(insts, af.CG_X4, "batched vs one-at-a-time (four-wide CG, cap 7)",
                        cap=7, tol=0.0)
+
+
+def test_batched_four_wide_two_planes_equals_one_at_a_time_bitwise():
+    """The four-wide CG with 2 z planes per thread (cg_z_cells = 2, a tiling
+    the batch autotuner also times): the batch equals single grids pinned to
+    the same layout and tiling, also with an odd cap (the tail kernel); a
+    tiling the layout does not take is rejected."""
+    import aether_fluid as af
+    insts = [dict(nx=32, ny=32, nz=32, vorticity_eps=0.3, buoyancy_beta=1.0, gravity_alpha=0.05,
+                  source_center=(16.0, 4.0, 16.0), source_radius=3.5),
+             dict(nx=68, ny=20, nz=19, vorticity_eps=0.45, buoyancy_beta=1.0, gravity_alpha=0.03,
+                  source_center=(30.0, 3.0, 9.0), source_radius=2.5),
+             dict(nx=16, ny=16, nz=40, vorticity_eps=0.1, buoyancy_beta=1.0, gravity_alpha=0.1,
+                  source_center=(8.0, 3.0, 20.0), source_radius=2.0)]
+    _batched_vs_single(insts, af.CG_X4, "batched vs one-at-a-time (four-wide CG, 2 planes)",
+                       z_cells=2)
+    _batched_vs_single(insts, af.CG_X4, "batched vs one-at-a-time (four-wide CG, 2 planes, cap 7)",
+                       cap=7, tol=0.0, z_cells=2)
+    b = af.FluidBatch(insts, solver_iters=10, cg_z_cells=16)
+    with pytest.raises(af.AetherFluidError, match="cg_z_cells"):
+        b.run(0.02, [0.0])
+    b.close()
 
 
 def test_batched_64_workload_equals_one_at_a_time_bitwise():
