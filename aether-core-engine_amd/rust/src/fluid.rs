@@ -119,6 +119,8 @@ impl FluidSolver3D {
             Precision::Fp32 => ffi::AETHER_FP32,
             Precision::Fp32Refined => ffi::AETHER_FP32_IR64,
         };
+        let abi = unsafe { ffi::aether_fluid_abi_version() };
+        assert_eq!(abi, ffi::AETHER_FLUID_ABI_VERSION, "libaether_fluid_hip.so has another ABI version");
         let mut dev: ffi::aether_fluid_t = std::ptr::null_mut();
         check(unsafe { ffi::aether_fluid_create(nx as i32, ny as i32, nz as i32, 0.0, prec, &mut dev) });
         let mut params = ffi::aether_fluid_params::default();

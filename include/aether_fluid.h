@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define AETHER_FLUID_ABI_VERSION 1
+#define AETHER_FLUID_ABI_VERSION 2
 
 /* status codes */
 #define AETHER_OK 0
