@@ -35,6 +35,15 @@
 // Array passes per iteration: r, r_(it-1), w read and r', w' written (5),
 // plus r_(it+1-K) .. r_(it-2), d, p read and d, p written by the K-th
 // launch: 5 + (K + 2) / K on average.
+//
+// The kernel is bound by its instruction issue as much as by its bytes (two
+// waves per SIMD, one generation, every wave loading, reducing, computing
+// and storing in step), so it is written for few instructions: the FP32
+// recurrences (s, r', d, p) are fused multiply-adds, the FP64 dot products
+// fused multiply-adds, loads take 32-bit offsets from a scalar base, and a
+// regular tiling (REG: rows that fill whole wavefront segments, dimensions
+// that divide evenly) drops the edge loads, the validity tests and the
+// integer divisions of the tile coordinates.
 #pragma once
 #include <utility>
 
@@ -54,19 +63,19 @@ struct CgRing {
 // dd = d_(it-M-1) (unused when it == M: d_0 = r_0) and the ring's residuals,
 // and returns d_(it-1) in dd and the updated p in pv.  fa/fb[m]: alpha and
 // beta of launch it - m.
-template <int K, int PH, int M>
+template <int K, int PH, int M, int ST>
 __device__ __forceinline__ void cgs4_pending(const CgRing &ring, int it, int oc,
                                              const float (&fa)[K], const float (&fb)[K],
                                              float4 &dd, float4 &pv) {
 #pragma unroll
     for (int m = M; m >= 1; --m) {
-        const float4 rj = reinterpret_cast<const float4 *>(ring.r[(PH - m + K) % K])[oc];
-        dd = it == m ? rj : axpy4(rj, fb[m], dd);
-        pv = axpy4(pv, fa[m], dd);
+        const float4 rj = ld4<ST>(ring.r[(PH - m + K) % K], oc);
+        dd = it == m ? rj : fma4(fb[m], dd, rj);
+        pv = fma4(fa[m], dd, pv);
     }
 }
 
-template <int ZU, bool FIRST, int NQ, int ST, int K, int PH>
+template <int ZU, bool FIRST, int NQ, int ST, int K, int PH, bool REG>
 __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
     Dims g, VTiling vt, int it, double rel_tol2, int nb_in, const double *__restrict__ pg_in,
     const double *__restrict__ pd_in, const double *__restrict__ pbb, double *__restrict__ pg_out,
@@ -77,9 +86,14 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
     static_assert(K >= 3 && K <= kCgRingMax && PH >= 0 && PH < K && (!FIRST || PH == 0),
                   "one-launch CG cycle");
     constexpr int kMask = kCgRingMax - 1;
-    int bx, by, bz;
-    vtile_coords(vt, bx, by, bz);
-    const VCol c = vcol_at<ZU>(g, vt, bx, by, bz);
+    VCol c;
+    if constexpr (REG) {
+        c = vcol_regular<ZU>(g, vt);
+    } else {
+        int bx, by, bz;
+        vtile_coords(vt, bx, by, bz);
+        c = vcol_at<ZU>(g, vt, bx, by, bz);
+    }
     // separate slots for each reduction, so none needs a trailing barrier
     __shared__ double lds[8], lds_b[8], lds_out[8];
     __shared__ int flag_lds[1];
@@ -107,11 +121,6 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
     const float *__restrict__ r_in = ring.r[PH];
     float *__restrict__ r_out = ring.r[(PH + 1) % K];
     const float *__restrict__ r_prev = ring.r[(PH + K - 1) % K];
-    const float4 *r4 = reinterpret_cast<const float4 *>(r_in);
-    const float4 *ro1 = reinterpret_cast<const float4 *>(r_prev);
-    const float4 *w4 = reinterpret_cast<const float4 *>(w_in);
-    float4 *d4 = reinterpret_cast<float4 *>(d);
-    float4 *p4 = reinterpret_cast<float4 *>(p);
     const long nbytes = (long)g.sxy * g.nz * 4;
     const Out4<ST> o_r(r_out, nbytes), o_w(w_out, nbytes), o_d(d, nbytes), o_p(p, nbytes);
     const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -121,31 +130,31 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
 #pragma unroll
     for (int a = 0; a < ZU + 2; ++a) {
         const int o = (min(max(c.k0 - 1 + a, 0), g.nz - 1) * g.sxy + c.off) >> 2;
-        rz[a] = r4[o];
-        wz[a] = w4[o];
-        qz[a] = FIRST ? z4 : ro1[o];
+        rz[a] = ld4<ST>(r_in, o);
+        wz[a] = ld4<ST>(w_in, o);
+        qz[a] = FIRST ? z4 : ld4<ST>(r_prev, o);
     }
 #pragma unroll
     for (int a = 0; a < ZU; ++a) {
         const int base = min(c.k0 + a, g.nz - 1) * g.sxy;
         const int om = (base + c.offym) >> 2, op = (base + c.offyp) >> 2;
-        ry[a][0] = r4[om];
-        ry[a][1] = r4[op];
-        wy[a][0] = w4[om];
-        wy[a][1] = w4[op];
-        qy[a][0] = FIRST ? z4 : ro1[om];
-        qy[a][1] = FIRST ? z4 : ro1[op];
+        ry[a][0] = ld4<ST>(r_in, om);
+        ry[a][1] = ld4<ST>(r_in, op);
+        wy[a][0] = ld4<ST>(w_in, om);
+        wy[a][1] = ld4<ST>(w_in, op);
+        qy[a][0] = FIRST ? z4 : ld4<ST>(r_prev, om);
+        qy[a][1] = FIRST ? z4 : ld4<ST>(r_prev, op);
 #pragma unroll
         for (int e = 0; e < 2; ++e) er[a][e] = ew[a][e] = eq[a][e] = 0.0f;
-        if (c.needL) {
-            er[a][0] = r_in[base + c.off - 1];
-            ew[a][0] = w_in[base + c.off - 1];
-            if (!FIRST) eq[a][0] = r_prev[base + c.off - 1];
+        if (!REG && c.needL) {
+            er[a][0] = ld1<ST>(r_in, base + c.off - 1);
+            ew[a][0] = ld1<ST>(w_in, base + c.off - 1);
+            if (!FIRST) eq[a][0] = ld1<ST>(r_prev, base + c.off - 1);
         }
-        if (c.needR) {
-            er[a][1] = r_in[base + c.off + 4];
-            ew[a][1] = w_in[base + c.off + 4];
-            if (!FIRST) eq[a][1] = r_prev[base + c.off + 4];
+        if (!REG && c.needR) {
+            er[a][1] = ld1<ST>(r_in, base + c.off + 4);
+            ew[a][1] = ld1<ST>(w_in, base + c.off + 4);
+            if (!FIRST) eq[a][1] = ld1<ST>(r_prev, base + c.off + 4);
         }
     }
 
@@ -161,7 +170,7 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
     if (flag_lds[0]) return;
     const double gam = tot[0], del = tot[1];
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
-    double beta = 0.0, mu = del;
+    double beta = 0.0, mu = del, tcd = 0.0;  // tcd: beta / alpha_(it-1)
     bool stop;
     if (FIRST) {
         b2 = eb.total(pbb, nb_in, lds_b);
@@ -173,7 +182,8 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
     } else {
         stop = gam < 1e-20 || (rel_tol2 > 0.0 && gam <= rel_tol2 * b2);
         beta = gam / gprev;
-        mu = del - beta * gam / aprev;
+        tcd = beta / aprev;
+        mu = del - tcd * gam;
     }
     if (!stop && fabs(mu) < 1e-28) stop = true;
     if (stop) {
@@ -188,10 +198,10 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
 #pragma unroll
             for (int a = 0; a < ZU; ++a) {
                 const int k = c.k0 + a;
-                if (c.valid && k < g.nz) {
+                if (REG || (c.valid && k < g.nz)) {
                     const int oc = (k * g.sxy + c.off) >> 2;
-                    float4 dd = d4[oc], pv = p4[oc];
-                    cgs4_pending<K, PH, PH>(ring, it, oc, fa, fb, dd, pv);
+                    float4 dd = ld4<ST>(d, oc), pv = ld4<ST>(p, oc);
+                    cgs4_pending<K, PH, PH, ST>(ring, it, oc, fa, fb, dd, pv);
                     o_p.put(oc, pv);
                 }
             }
@@ -206,7 +216,7 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
     }
     const float tb = (float)beta, ta = (float)alpha;
     // beta / alpha_(it-1): scales r_(it-1) - r into beta A d_(it-1)
-    const float tc = FIRST ? 0.0f : (float)(beta / aprev);
+    const float tc = (float)tcd;
 
     // ---- s = w + (beta / alpha_(it-1)) (r_(it-1) - r) and r' = r - alpha s
     // on the radius-1 neighbourhood
@@ -214,19 +224,22 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
     float rne[ZU][2];
 #pragma unroll
     for (int a = 0; a < ZU + 2; ++a) {
-        const float4 sv = FIRST ? wz[a] : axpy4(wz[a], tc, sub4(qz[a], rz[a]));
-        rn[a] = make_float4(rz[a].x - ta * sv.x, rz[a].y - ta * sv.y, rz[a].z - ta * sv.z,
-                            rz[a].w - ta * sv.w);
+        const float4 sv = FIRST ? wz[a] : fma4(tc, sub4(qz[a], rz[a]), wz[a]);
+        rn[a] = fma4(-ta, sv, rz[a]);
     }
 #pragma unroll
     for (int a = 0; a < ZU; ++a) {
 #pragma unroll
         for (int e = 0; e < 2; ++e) {
-            const float4 sv = FIRST ? wy[a][e] : axpy4(wy[a][e], tc, sub4(qy[a][e], ry[a][e]));
-            rny[a][e] = make_float4(ry[a][e].x - ta * sv.x, ry[a][e].y - ta * sv.y,
-                                    ry[a][e].z - ta * sv.z, ry[a][e].w - ta * sv.w);
-            const float se = FIRST ? ew[a][e] : ew[a][e] + tc * (eq[a][e] - er[a][e]);
-            rne[a][e] = er[a][e] - ta * se;
+            const float4 sv = FIRST ? wy[a][e] : fma4(tc, sub4(qy[a][e], ry[a][e]), wy[a][e]);
+            rny[a][e] = fma4(-ta, sv, ry[a][e]);
+            if (!REG) {
+                const float se =
+                    FIRST ? ew[a][e] : __builtin_fmaf(tc, eq[a][e] - er[a][e], ew[a][e]);
+                rne[a][e] = __builtin_fmaf(-ta, se, er[a][e]);
+            } else {
+                rne[a][e] = 0.0f;
+            }
         }
     }
 
@@ -239,12 +252,12 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
 #pragma unroll
         for (int a = 0; a < ZU; ++a) {
             const int k = c.k0 + a;
-            const int oc = (min(k, g.nz - 1) * g.sxy + c.off) >> 2;
-            float4 dd = d4[oc], pv = p4[oc];
-            cgs4_pending<K, PH, K - 1>(ring, it, oc, fa, fb, dd, pv);
-            dd = axpy4(rz[a + 1], tb, dd);  // d_it
-            pv = axpy4(pv, ta, dd);
-            if (c.valid && k < g.nz) {
+            const int oc = ((REG ? k : min(k, g.nz - 1)) * g.sxy + c.off) >> 2;
+            float4 dd = ld4<ST>(d, oc), pv = ld4<ST>(p, oc);
+            cgs4_pending<K, PH, K - 1, ST>(ring, it, oc, fa, fb, dd, pv);
+            dd = fma4(tb, dd, rz[a + 1]);  // d_it
+            pv = fma4(ta, dd, pv);
+            if (REG || (c.valid && k < g.nz)) {
                 o_d.put(oc, dd);
                 o_p.put(oc, pv);
             }
@@ -258,17 +271,17 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
     for (int a = 0; a < ZU; ++a) {
         const int k = c.k0 + a;
         const float sl = lane_value(rn[a + 1].w, srcL), sr = lane_value(rn[a + 1].x, srcR);
-        const float xl = c.needL ? rne[a][0] : sl;
-        const float xrv = c.needR ? rne[a][1] : sr;
-        if (c.valid && k < g.nz) {
+        const float xl = !REG && c.needL ? rne[a][0] : sl;
+        const float xrv = !REG && c.needR ? rne[a][1] : sr;
+        if (REG || (c.valid && k < g.nz)) {
             float q[4];
             lap4(g, c, k, rn[a + 1], xl, xrv, rny[a][0], rny[a][1], rn[a], rn[a + 2], q);
             const float4 wn = make_float4(q[0], q[1], q[2], q[3]);
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const double rv = (double)f4(rn[a + 1], e);
-                acc[0] += rv * rv;
-                acc[1] += rv * (double)q[e];
+                acc[0] = __builtin_fma(rv, rv, acc[0]);
+                acc[1] = __builtin_fma(rv, (double)q[e], acc[1]);
             }
             const int id4 = (k * g.sxy + c.off) >> 2;
             o_r.put(id4, rn[a + 1]);
@@ -310,8 +323,8 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4_tail(const CgState *st, in
         for (int q = 0; q < K - 1; ++q) {
             if (q < npend) {
                 const float4 rj = reinterpret_cast<const float4 *>(pend.r[q])[a];
-                dd = j0 + q == 0 ? rj : axpy4(rj, fb[q], dd);
-                pv = axpy4(pv, fa[q], dd);
+                dd = j0 + q == 0 ? rj : fma4(fb[q], dd, rj);
+                pv = fma4(fa[q], dd, pv);
             }
         }
         p[a] = pv;
@@ -325,24 +338,29 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4_tail(const CgState *st, in
 // passes and a tail; the 128^3 step measured ~1% slower with 3 or 4).
 constexpr int kCgsCycle = 8;
 
-template <int NQ, int ST, int... PH>
+template <int NQ, int ST, bool REG, int... PH>
 inline const void *cgs4_phase(int ph, std::integer_sequence<int, PH...>) {
-    const void *fns[] = {(const void *)k_cg_cgs4<4, false, NQ, ST, kCgsCycle, PH>...};
+    const void *fns[] = {(const void *)k_cg_cgs4<4, false, NQ, ST, kCgsCycle, PH, REG>...};
     return fns[ph];
 }
 
-template <int NQ, int ST>
+template <int NQ, int ST, bool REG>
 inline const void *cgs4_pick(bool first, int ph) {
-    if (first) return (const void *)k_cg_cgs4<4, true, NQ, ST, kCgsCycle, 0>;
-    return cgs4_phase<NQ, ST>(ph, std::make_integer_sequence<int, kCgsCycle>{});
+    if (first) return (const void *)k_cg_cgs4<4, true, NQ, ST, kCgsCycle, 0, REG>;
+    return cgs4_phase<NQ, ST, REG>(ph, std::make_integer_sequence<int, kCgsCycle>{});
 }
 
 // The k_cg_cgs4 instantiation for launch phase ph (= it mod kCgsCycle).
 // wide: more than 1024 partial sums per launch (16 per thread); wt:
-// write-through stores (arrays under 2 GiB), else plain stores.
-inline const void *cgs4_kernel(bool first, int ph, bool wide, bool wt) {
-    if (wide) return wt ? cgs4_pick<16, kStWT>(first, ph) : cgs4_pick<16, kStPlain>(first, ph);
-    return wt ? cgs4_pick<4, kStWT>(first, ph) : cgs4_pick<4, kStPlain>(first, ph);
+// write-through stores and 32-bit load offsets (arrays under 2 GiB), else
+// plain stores; reg: a regular tiling (vtiling_regular, write-through only).
+inline const void *cgs4_kernel(bool first, int ph, bool wide, bool wt, bool reg) {
+    if (wt && reg)
+        return wide ? cgs4_pick<16, kStWT, true>(first, ph) : cgs4_pick<4, kStWT, true>(first, ph);
+    if (wide)
+        return wt ? cgs4_pick<16, kStWT, false>(first, ph)
+                  : cgs4_pick<16, kStPlain, false>(first, ph);
+    return wt ? cgs4_pick<4, kStWT, false>(first, ph) : cgs4_pick<4, kStPlain, false>(first, ph);
 }
 
 inline void cgs4_tail_launch(unsigned nb, hipStream_t st, const CgState *cg, int last,

@@ -64,6 +64,36 @@ __device__ __forceinline__ VCol vcol_at(const Dims &g, const VTiling &vt, int bx
     return c;
 }
 
+// A tiling is regular when one workgroup row spans the whole x extent
+// (gx == 1), a wavefront holds whole rows (txv a power of two dividing 64,
+// txv * ty == 256), and the rows and planes divide evenly (ny % ty == 0,
+// nz % ZU == 0).  Then every thread owns four in-grid cells on every plane,
+// and every x-neighbour inside the row is in the adjacent lane of the same
+// wavefront (the row ends are walls, which the stencil masks): no edge
+// loads, no validity tests, and the coordinates come from shifts.
+inline bool vtiling_regular(const VTiling &t, int ny, int nz, int zu) {
+    return t.gx == 1 && t.txv > 0 && (t.txv & (t.txv - 1)) == 0 && 64 % t.txv == 0 &&
+           t.txv * t.ty == 256 && ny % t.ty == 0 && nz % zu == 0;
+}
+
+template <int ZU>
+__device__ __forceinline__ VCol vcol_regular(const Dims &g, const VTiling &vt) {
+    int b = blockIdx.x;
+    if (vt.remap) b = (b & 7) * (vt.nb >> 3) + (b >> 3);
+    const int t = threadIdx.x, sh = __builtin_ctz(vt.txv);
+    VCol c;
+    c.i0 = 4 * (t & (vt.txv - 1));
+    c.j = (b % vt.gy) * vt.ty + (t >> sh);
+    c.k0 = (b / vt.gy) * ZU;
+    c.valid = true;
+    c.off = c.j * g.nx + c.i0;
+    c.offym = max(c.j - 1, 0) * g.nx + c.i0;
+    c.offyp = min(c.j + 1, g.ny - 1) * g.nx + c.i0;
+    c.lane = t & 63;
+    c.needL = c.needR = false;
+    return c;
+}
+
 // Output array of a four-wide kernel under one store policy.  ST 0: plain
 // stores; 1: non-temporal; 2: write-through, through a buffer resource with
 // the sc1 cache-policy bit, so that the stores leave no dirty lines in L2 for
@@ -98,6 +128,33 @@ struct Out4 {
     }
 };
 
+// Loads of a four-wide kernel under one store policy.  With write-through
+// stores (arrays under 2 GiB) the byte offset fits 32 bits, so the address is
+// the array base (scalar registers) plus a zero-extended 32-bit offset: one
+// shift per load instead of 64-bit address arithmetic.
+template <int ST>
+__device__ __forceinline__ float4 ld4(const float *__restrict__ base, int o4) {
+    if constexpr (ST == kStWT)
+        return *reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(base) +
+                                                 (size_t)((unsigned)o4 << 4));
+    else
+        return reinterpret_cast<const float4 *>(base)[o4];
+}
+template <int ST>
+__device__ __forceinline__ float ld1(const float *__restrict__ base, int o) {
+    if constexpr (ST == kStWT)
+        return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) +
+                                                (size_t)((unsigned)o << 2));
+    else
+        return base[o];
+}
+
+// a x + y per component, each one fused multiply-add
+__device__ __forceinline__ float4 fma4(float a, const float4 &x, const float4 &y) {
+    return make_float4(__builtin_fmaf(a, x.x, y.x), __builtin_fmaf(a, x.y, y.y),
+                       __builtin_fmaf(a, x.z, y.z), __builtin_fmaf(a, x.w, y.w));
+}
+
 __device__ __forceinline__ float f4(const float4 &v, int e) {
     return e == 0 ? v.x : e == 1 ? v.y : e == 2 ? v.z : v.w;
 }
@@ -112,19 +169,21 @@ __device__ __forceinline__ float lane_value(float v, int src) {
     return __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(v)));
 }
 
-// A x on the four cells of plane k of a column (lap7 per cell, interior only);
-// returns the products x.Ax summed when `dot` is set.
+// A x on the four cells of plane k of a column (lap7 per cell, interior only).
+// Every cell's stencil is formed and the non-interior ones are then zeroed by
+// a select, not skipped by a branch: the stencil reads registers only, and a
+// branch per cell would also pull the neighbours' arithmetic into it.
 __device__ __forceinline__ void lap4(const Dims &g, const VCol &c, int k, const float4 &x,
                                      float xl, float xr, const float4 &ym, const float4 &yp,
                                      const float4 &zm, const float4 &zp, float (&q)[4]) {
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
         const int i = c.i0 + e;
-        q[e] = 0.0f;
-        if (is_interior(g, i, c.j, k))
-            q[e] = lap7<float>(f4(x, e), e == 0 ? xl : f4(x, e - 1), e == 3 ? xr : f4(x, e + 1),
-                               f4(ym, e), f4(yp, e), f4(zm, e), f4(zp, e), i > 1, i + 2 < g.nx,
-                               c.j > 1, c.j + 2 < g.ny, k > 1, k + 2 < g.nz);
+        const float v =
+            lap7<float>(f4(x, e), e == 0 ? xl : f4(x, e - 1), e == 3 ? xr : f4(x, e + 1),
+                        f4(ym, e), f4(yp, e), f4(zm, e), f4(zp, e), i > 1, i + 2 < g.nx, c.j > 1,
+                        c.j + 2 < g.ny, k > 1, k + 2 < g.nz);
+        q[e] = is_interior(g, i, c.j, k) ? v : 0.0f;
     }
 }
 
