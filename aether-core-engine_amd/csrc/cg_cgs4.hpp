@@ -58,6 +58,52 @@ struct CgRing {
     float *r[kCgRingMax];
 };
 
+// Wall masks of a regular tiling (vtiling_regular) as 0/1 factors.  With the
+// row's four-cell groups one per lane, an x-neighbour of an interior cell can
+// only be a wall at cell 1 of the row's first lane (mx1) and cell 2 of its
+// last lane (mx2); cells 0 and 3 of those lanes are walls themselves (xin0,
+// xin3).  The y factors hold for the thread's whole column.
+struct RegMasks {
+    float mx1, mx2, mym, myp;
+    bool xin0, xin3, jin;
+};
+
+__device__ __forceinline__ RegMasks reg_masks(const Dims &g, const VCol &c) {
+    RegMasks m;
+    m.xin0 = c.i0 > 0;
+    m.xin3 = c.i0 + 4 < g.nx;
+    m.mx1 = m.xin0 ? 1.0f : 0.0f;
+    m.mx2 = m.xin3 ? 1.0f : 0.0f;
+    m.mym = c.j > 1 ? 1.0f : 0.0f;
+    m.myp = c.j + 2 < g.ny ? 1.0f : 0.0f;
+    m.jin = c.j >= 1 && c.j <= g.ny - 2;
+    return m;
+}
+
+// lap4 for a regular tiling: the same sum of (x_c - x_nb) over the valid
+// neighbours in the same order (x-, x+, y-, y+, z-, z+), with each term that
+// can be masked added as a multiply-add by its 0/1 factor.  Equal to lap4 up
+// to the sign of a zero result.
+__device__ __forceinline__ void lap4_reg(const Dims &g, const RegMasks &m, int k, const float4 &x,
+                                         float xl, float xr, const float4 &ym, const float4 &yp,
+                                         const float4 &zm, const float4 &zp, float (&q)[4]) {
+    const float mzm = k > 1 ? 1.0f : 0.0f, mzp = k + 2 < g.nz ? 1.0f : 0.0f;
+    const bool kin = k >= 1 && k <= g.nz - 2 && m.jin;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const float xc = f4(x, e);
+        const float xm = e == 0 ? xl : f4(x, e - 1), xp = e == 3 ? xr : f4(x, e + 1);
+        float acc = e == 1 ? m.mx1 * (xc - xm) : xc - xm;
+        acc = e == 2 ? __builtin_fmaf(m.mx2, xc - xp, acc) : acc + (xc - xp);
+        acc = __builtin_fmaf(m.mym, xc - f4(ym, e), acc);
+        acc = __builtin_fmaf(m.myp, xc - f4(yp, e), acc);
+        acc = __builtin_fmaf(mzm, xc - f4(zm, e), acc);
+        acc = __builtin_fmaf(mzp, xc - f4(zp, e), acc);
+        const bool in = kin && (e == 0 ? m.xin0 : e == 3 ? m.xin3 : true);
+        q[e] = in ? acc : 0.0f;
+    }
+}
+
 // Applies at cell `oc` the p terms of the M launches before launch `it`
 // (it mod K = PH), in launch order: rebuilds d_(it-M) .. d_(it-1) from
 // dd = d_(it-M-1) (unused when it == M: d_0 = r_0) and the ring's residuals,
@@ -124,7 +170,9 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
     const long nbytes = (long)g.sxy * g.nz * 4;
     const Out4<ST> o_r(r_out, nbytes), o_w(w_out, nbytes), o_d(d, nbytes), o_p(p, nbytes);
     const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-    // r, w and r_(it-1) on the radius-1 neighbourhood (z column, y rows, x edges)
+    // r, w and r_(it-1) on the radius-1 neighbourhood (z column, y rows, x
+    // edges), issued plane by plane: z plane a, then the y rows of plane
+    // a - 1, so that the stencil of a plane waits only for the loads it reads
     float4 rz[ZU + 2], wz[ZU + 2], qz[ZU + 2], ry[ZU][2], wy[ZU][2], qy[ZU][2];
     float er[ZU][2], ew[ZU][2], eq[ZU][2];
 #pragma unroll
@@ -133,28 +181,27 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
         rz[a] = ld4<ST>(r_in, o);
         wz[a] = ld4<ST>(w_in, o);
         qz[a] = FIRST ? z4 : ld4<ST>(r_prev, o);
-    }
-#pragma unroll
-    for (int a = 0; a < ZU; ++a) {
-        const int base = min(c.k0 + a, g.nz - 1) * g.sxy;
+        if (a < 1 || a > ZU) continue;
+        const int b = a - 1;
+        const int base = min(c.k0 + b, g.nz - 1) * g.sxy;
         const int om = (base + c.offym) >> 2, op = (base + c.offyp) >> 2;
-        ry[a][0] = ld4<ST>(r_in, om);
-        ry[a][1] = ld4<ST>(r_in, op);
-        wy[a][0] = ld4<ST>(w_in, om);
-        wy[a][1] = ld4<ST>(w_in, op);
-        qy[a][0] = FIRST ? z4 : ld4<ST>(r_prev, om);
-        qy[a][1] = FIRST ? z4 : ld4<ST>(r_prev, op);
+        ry[b][0] = ld4<ST>(r_in, om);
+        ry[b][1] = ld4<ST>(r_in, op);
+        wy[b][0] = ld4<ST>(w_in, om);
+        wy[b][1] = ld4<ST>(w_in, op);
+        qy[b][0] = FIRST ? z4 : ld4<ST>(r_prev, om);
+        qy[b][1] = FIRST ? z4 : ld4<ST>(r_prev, op);
 #pragma unroll
-        for (int e = 0; e < 2; ++e) er[a][e] = ew[a][e] = eq[a][e] = 0.0f;
+        for (int e = 0; e < 2; ++e) er[b][e] = ew[b][e] = eq[b][e] = 0.0f;
         if (!REG && c.needL) {
-            er[a][0] = ld1<ST>(r_in, base + c.off - 1);
-            ew[a][0] = ld1<ST>(w_in, base + c.off - 1);
-            if (!FIRST) eq[a][0] = ld1<ST>(r_prev, base + c.off - 1);
+            er[b][0] = ld1<ST>(r_in, base + c.off - 1);
+            ew[b][0] = ld1<ST>(w_in, base + c.off - 1);
+            if (!FIRST) eq[b][0] = ld1<ST>(r_prev, base + c.off - 1);
         }
         if (!REG && c.needR) {
-            er[a][1] = ld1<ST>(r_in, base + c.off + 4);
-            ew[a][1] = ld1<ST>(w_in, base + c.off + 4);
-            if (!FIRST) eq[a][1] = ld1<ST>(r_prev, base + c.off + 4);
+            er[b][1] = ld1<ST>(r_in, base + c.off + 4);
+            ew[b][1] = ld1<ST>(w_in, base + c.off + 4);
+            if (!FIRST) eq[b][1] = ld1<ST>(r_prev, base + c.off + 4);
         }
     }
 
@@ -222,17 +269,14 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
     // on the radius-1 neighbourhood
     float4 rn[ZU + 2], rny[ZU][2];
     float rne[ZU][2];
-#pragma unroll
-    for (int a = 0; a < ZU + 2; ++a) {
-        const float4 sv = FIRST ? wz[a] : fma4(tc, sub4(qz[a], rz[a]), wz[a]);
-        rn[a] = fma4(-ta, sv, rz[a]);
-    }
-#pragma unroll
-    for (int a = 0; a < ZU; ++a) {
+    auto r_new = [&](const float4 &rv, const float4 &wv, const float4 &qv) {
+        const float4 sv = FIRST ? wv : fma4(tc, sub4(qv, rv), wv);
+        return fma4(-ta, sv, rv);
+    };
+    auto r_new_y = [&](int a) {  // the y rows (and row-segment edges) of plane a
 #pragma unroll
         for (int e = 0; e < 2; ++e) {
-            const float4 sv = FIRST ? wy[a][e] : fma4(tc, sub4(qy[a][e], ry[a][e]), wy[a][e]);
-            rny[a][e] = fma4(-ta, sv, ry[a][e]);
+            rny[a][e] = r_new(ry[a][e], wy[a][e], qy[a][e]);
             if (!REG) {
                 const float se =
                     FIRST ? ew[a][e] : __builtin_fmaf(tc, eq[a][e] - er[a][e], ew[a][e]);
@@ -241,13 +285,55 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
                 rne[a][e] = 0.0f;
             }
         }
-    }
+    };
 
-    // ---- every K-th launch: d_(it+1-K) .. d_it and p, finished and stored
-    // at once.  The ring's residuals, d and p are loaded only now, after the
-    // halo registers above are consumed, and held by nothing below, so that
-    // the launch keeps two waves per SIMD.
-    if (PH == K - 1) {
+    // ---- w' = A r' and the partials of r'.r', r'.w' on plane a ------------
+    const int srcL = (c.lane + 63) & 63, srcR = (c.lane + 1) & 63;
+    const RegMasks rm = reg_masks(g, c);
+    double acc[2] = {0.0, 0.0};
+    auto plane = [&](int a) {
+        const int k = c.k0 + a;
+        const float sl = lane_value(rn[a + 1].w, srcL), sr = lane_value(rn[a + 1].x, srcR);
+        const float xl = !REG && c.needL ? rne[a][0] : sl;
+        const float xrv = !REG && c.needR ? rne[a][1] : sr;
+        if (REG || (c.valid && k < g.nz)) {
+            float q[4];
+            if constexpr (REG)
+                lap4_reg(g, rm, k, rn[a + 1], xl, xrv, rny[a][0], rny[a][1], rn[a], rn[a + 2], q);
+            else
+                lap4(g, c, k, rn[a + 1], xl, xrv, rny[a][0], rny[a][1], rn[a], rn[a + 2], q);
+            const float4 wn = make_float4(q[0], q[1], q[2], q[3]);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const double rv = (double)f4(rn[a + 1], e);
+                acc[0] = __builtin_fma(rv, rv, acc[0]);
+                acc[1] = __builtin_fma(rv, (double)q[e], acc[1]);
+            }
+            const int id4 = (k * g.sxy + c.off) >> 2;
+            o_r.put(id4, rn[a + 1]);
+            o_w.put(id4, wn);
+        }
+    };
+
+    if constexpr (PH != K - 1) {
+        // plane by plane, in the order the loads were issued
+        rn[0] = r_new(rz[0], wz[0], qz[0]);
+        rn[1] = r_new(rz[1], wz[1], qz[1]);
+#pragma unroll
+        for (int a = 0; a < ZU; ++a) {
+            rn[a + 2] = r_new(rz[a + 2], wz[a + 2], qz[a + 2]);
+            r_new_y(a);
+            plane(a);
+        }
+    } else {
+        // every K-th launch: r' on the whole neighbourhood first, so that the
+        // halo registers are consumed before the ring's residuals, d and p
+        // are loaded; then d_(it+1-K) .. d_it and p, finished and stored at
+        // once (held by nothing after, so the launch keeps two waves per SIMD)
+#pragma unroll
+        for (int a = 0; a < ZU + 2; ++a) rn[a] = r_new(rz[a], wz[a], qz[a]);
+#pragma unroll
+        for (int a = 0; a < ZU; ++a) r_new_y(a);
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int a = 0; a < ZU; ++a) {
@@ -262,31 +348,8 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
                 o_p.put(oc, pv);
             }
         }
-    }
-
-    // ---- w' = A r' and the partials of r'.r', r'.w' ----------------------
-    const int srcL = (c.lane + 63) & 63, srcR = (c.lane + 1) & 63;
-    double acc[2] = {0.0, 0.0};
 #pragma unroll
-    for (int a = 0; a < ZU; ++a) {
-        const int k = c.k0 + a;
-        const float sl = lane_value(rn[a + 1].w, srcL), sr = lane_value(rn[a + 1].x, srcR);
-        const float xl = !REG && c.needL ? rne[a][0] : sl;
-        const float xrv = !REG && c.needR ? rne[a][1] : sr;
-        if (REG || (c.valid && k < g.nz)) {
-            float q[4];
-            lap4(g, c, k, rn[a + 1], xl, xrv, rny[a][0], rny[a][1], rn[a], rn[a + 2], q);
-            const float4 wn = make_float4(q[0], q[1], q[2], q[3]);
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const double rv = (double)f4(rn[a + 1], e);
-                acc[0] = __builtin_fma(rv, rv, acc[0]);
-                acc[1] = __builtin_fma(rv, (double)q[e], acc[1]);
-            }
-            const int id4 = (k * g.sxy + c.off) >> 2;
-            o_r.put(id4, rn[a + 1]);
-            o_w.put(id4, wn);
-        }
+        for (int a = 0; a < ZU; ++a) plane(a);
     }
     block_sum<2, false>(acc, lds_out);
     if (threadIdx.x == 0) {
@@ -351,12 +414,14 @@ inline const void *cgs4_pick(bool first, int ph) {
 }
 
 // The k_cg_cgs4 instantiation for launch phase ph (= it mod kCgsCycle).
-// wide: more than 1024 partial sums per launch (16 per thread); wt:
+// nparts: partial sums read or written per launch (2, 4 or 16 per thread); wt:
 // write-through stores and 32-bit load offsets (arrays under 2 GiB), else
 // plain stores; reg: a regular tiling (vtiling_regular, write-through only).
-inline const void *cgs4_kernel(bool first, int ph, bool wide, bool wt, bool reg) {
-    if (wt && reg)
-        return wide ? cgs4_pick<16, kStWT, true>(first, ph) : cgs4_pick<4, kStWT, true>(first, ph);
+inline const void *cgs4_kernel(bool first, int ph, int nparts, bool wt, bool reg) {
+    const bool wide = nparts > 4 * kThreads;
+    if (wt && reg && !wide)
+        return nparts <= 2 * kThreads ? cgs4_pick<2, kStWT, true>(first, ph)
+                                      : cgs4_pick<4, kStWT, true>(first, ph);
     if (wide)
         return wt ? cgs4_pick<16, kStWT, false>(first, ph)
                   : cgs4_pick<16, kStPlain, false>(first, ph);

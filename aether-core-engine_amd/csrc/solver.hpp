@@ -354,10 +354,9 @@ class Solver final : public SolverBase {
                 for (int q = 0; q < kCgsCycle; ++q) ring.r[q] = ring_at(q);
                 void *args[] = {&g,      &vt,     &itv, &rt,   &nbv,  &pg_in, &pd_in, &pbb, &pg_out,
                                 &pd_out, &st,     &ring, &w_in, &w_out, &dd,   &pp,    &dm,  &sid};
-                const bool wide = std::max(nb_in, vt.nb) > 4 * kThreads;
                 // write-through stores for arrays under 2 GiB
-                const void *fn = cgs4_kernel(it == 0, it % kCgsCycle, wide, wt_ok(),
-                                             vtiling_regular(vt, g.ny, g.nz, 4));
+                const void *fn = cgs4_kernel(it == 0, it % kCgsCycle, std::max(nb_in, vt.nb),
+                                             wt_ok(), vtiling_regular(vt, g.ny, g.nz, 4));
                 HIPT(hipExtLaunchKernel(fn, dim3(vt.nb), dim3(kThreads), args, 0, st_, e0, e1, 0));
                 return;
             }
