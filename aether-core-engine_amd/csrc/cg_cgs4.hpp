@@ -315,41 +315,77 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
         }
     };
 
-    if constexpr (PH != K - 1) {
-        // plane by plane, in the order the loads were issued
-        rn[0] = r_new(rz[0], wz[0], qz[0]);
-        rn[1] = r_new(rz[1], wz[1], qz[1]);
+    // the K-th launch rebuilds d and p from residuals the ring holds; the one
+    // in this launch's output array (r_(it+1-K)) is read at the thread's own
+    // cells before the planes below overwrite them
+    float4 rold[ZU];
+    if constexpr (PH == K - 1) {
 #pragma unroll
         for (int a = 0; a < ZU; ++a) {
-            rn[a + 2] = r_new(rz[a + 2], wz[a + 2], qz[a + 2]);
-            r_new_y(a);
-            plane(a);
+            const int k = REG ? c.k0 + a : min(c.k0 + a, g.nz - 1);
+            rold[a] = ld4<ST>(r_out, (k * g.sxy + c.off) >> 2);
         }
-    } else {
-        // every K-th launch: r' on the whole neighbourhood first, so that the
-        // halo registers are consumed before the ring's residuals, d and p
-        // are loaded; then d_(it+1-K) .. d_it and p, finished and stored at
-        // once (held by nothing after, so the launch keeps two waves per SIMD)
+        // keep these loads ahead of the buffer stores into the same array
+        __asm__ volatile("" ::: "memory");
+    }
+
+    // plane by plane, in the order the loads were issued
+    rn[0] = r_new(rz[0], wz[0], qz[0]);
+    rn[1] = r_new(rz[1], wz[1], qz[1]);
 #pragma unroll
-        for (int a = 0; a < ZU + 2; ++a) rn[a] = r_new(rz[a], wz[a], qz[a]);
-#pragma unroll
-        for (int a = 0; a < ZU; ++a) r_new_y(a);
+    for (int a = 0; a < ZU; ++a) {
+        rn[a + 2] = r_new(rz[a + 2], wz[a + 2], qz[a + 2]);
+        r_new_y(a);
+        plane(a);
+    }
+
+    // ---- every K-th launch: d_(it+1-K) .. d_it and p, finished and stored
+    // at once, after the halo registers are dead (only r_it at the thread's
+    // own cells is kept), so that the loads of all planes can be in flight
+    // together at two waves per SIMD
+    if constexpr (PH == K - 1) {
         __builtin_amdgcn_sched_barrier(0);
+        float4 dd[ZU], pv[ZU];
+        int oc[ZU];
 #pragma unroll
         for (int a = 0; a < ZU; ++a) {
-            const int k = c.k0 + a;
-            const int oc = ((REG ? k : min(k, g.nz - 1)) * g.sxy + c.off) >> 2;
-            float4 dd = ld4<ST>(d, oc), pv = ld4<ST>(p, oc);
-            cgs4_pending<K, PH, K - 1, ST>(ring, it, oc, fa, fb, dd, pv);
-            dd = fma4(tb, dd, rz[a + 1]);  // d_it
-            pv = fma4(ta, dd, pv);
-            if (REG || (c.valid && k < g.nz)) {
-                o_d.put(oc, dd);
-                o_p.put(oc, pv);
+            const int k = REG ? c.k0 + a : min(c.k0 + a, g.nz - 1);
+            oc[a] = (k * g.sxy + c.off) >> 2;
+            dd[a] = ld4<ST>(d, oc[a]);
+            pv[a] = ld4<ST>(p, oc[a]);
+        }
+        // launches it+1-K .. it-1 in order: d_j = r_j + beta_j d_(j-1)
+        // (d_0 = r_0), p += alpha_j d_j; launch it - m read ring[(PH - m) % K].
+        // Only the solve's first K-th launch starts from d_0 = r_0; that one
+        // decision is taken before the loop, so that the loop's loads share
+        // one basic block and are all in flight together.
+        if (it == K - 1) {
+#pragma unroll
+            for (int a = 0; a < ZU; ++a) dd[a] = rold[a];
+        } else {
+#pragma unroll
+            for (int a = 0; a < ZU; ++a) dd[a] = fma4(fb[K - 1], dd[a], rold[a]);
+        }
+#pragma unroll
+        for (int a = 0; a < ZU; ++a) pv[a] = fma4(fa[K - 1], dd[a], pv[a]);
+#pragma unroll
+        for (int m = K - 2; m >= 1; --m) {
+#pragma unroll
+            for (int a = 0; a < ZU; ++a) {
+                const float4 rj = ld4<ST>(ring.r[(PH - m + K) % K], oc[a]);
+                dd[a] = fma4(fb[m], dd[a], rj);
+                pv[a] = fma4(fa[m], dd[a], pv[a]);
             }
         }
 #pragma unroll
-        for (int a = 0; a < ZU; ++a) plane(a);
+        for (int a = 0; a < ZU; ++a) {
+            dd[a] = fma4(tb, dd[a], rz[a + 1]);  // d_it
+            pv[a] = fma4(ta, dd[a], pv[a]);
+            if (REG || (c.valid && c.k0 + a < g.nz)) {
+                o_d.put(oc[a], dd[a]);
+                o_p.put(oc[a], pv[a]);
+            }
+        }
     }
     block_sum<2, false>(acc, lds_out);
     if (threadIdx.x == 0) {
