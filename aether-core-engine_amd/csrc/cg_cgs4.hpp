@@ -315,36 +315,14 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
         }
     };
 
-    // the K-th launch rebuilds d and p from residuals the ring holds; the one
-    // in this launch's output array (r_(it+1-K)) is read at the thread's own
-    // cells before the planes below overwrite them
-    float4 rold[ZU];
-    if constexpr (PH == K - 1) {
-#pragma unroll
-        for (int a = 0; a < ZU; ++a) {
-            const int k = REG ? c.k0 + a : min(c.k0 + a, g.nz - 1);
-            rold[a] = ld4<ST>(r_out, (k * g.sxy + c.off) >> 2);
-        }
-        // keep these loads ahead of the buffer stores into the same array
-        __asm__ volatile("" ::: "memory");
-    }
-
-    // plane by plane, in the order the loads were issued
-    rn[0] = r_new(rz[0], wz[0], qz[0]);
-    rn[1] = r_new(rz[1], wz[1], qz[1]);
-#pragma unroll
-    for (int a = 0; a < ZU; ++a) {
-        rn[a + 2] = r_new(rz[a + 2], wz[a + 2], qz[a + 2]);
-        r_new_y(a);
-        plane(a);
-    }
-
     // ---- every K-th launch: d_(it+1-K) .. d_it and p, finished and stored
-    // at once, after the halo registers are dead (only r_it at the thread's
-    // own cells is kept), so that the loads of all planes can be in flight
-    // together at two waves per SIMD
-    if constexpr (PH == K - 1) {
-        __builtin_amdgcn_sched_barrier(0);
+    // at once (the ring's residuals are those of the K - 1 launches before;
+    // r_(it+1-K) sits in this launch's output array and is read here, before
+    // the planes below overwrite it).  r' is formed on the whole
+    // neighbourhood first, so that the halo registers are dead while the
+    // ring's residuals, d and p are in flight; the planes' r' and w' are
+    // stored last, so that the next launch finds them in L2.
+    auto rebuild_dp = [&]() {
         float4 dd[ZU], pv[ZU];
         int oc[ZU];
 #pragma unroll
@@ -359,6 +337,9 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
         // Only the solve's first K-th launch starts from d_0 = r_0; that one
         // decision is taken before the loop, so that the loop's loads share
         // one basic block and are all in flight together.
+        float4 rold[ZU];
+#pragma unroll
+        for (int a = 0; a < ZU; ++a) rold[a] = ld4<ST>(r_out, oc[a]);
         if (it == K - 1) {
 #pragma unroll
             for (int a = 0; a < ZU; ++a) dd[a] = rold[a];
@@ -386,6 +367,28 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
                 o_p.put(oc[a], pv[a]);
             }
         }
+    };
+
+    if constexpr (PH != K - 1) {
+        // plane by plane, in the order the loads were issued
+        rn[0] = r_new(rz[0], wz[0], qz[0]);
+        rn[1] = r_new(rz[1], wz[1], qz[1]);
+#pragma unroll
+        for (int a = 0; a < ZU; ++a) {
+            rn[a + 2] = r_new(rz[a + 2], wz[a + 2], qz[a + 2]);
+            r_new_y(a);
+            plane(a);
+        }
+    } else {
+#pragma unroll
+        for (int a = 0; a < ZU + 2; ++a) rn[a] = r_new(rz[a], wz[a], qz[a]);
+#pragma unroll
+        for (int a = 0; a < ZU; ++a) r_new_y(a);
+        __builtin_amdgcn_sched_barrier(0);
+        rebuild_dp();
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int a = 0; a < ZU; ++a) plane(a);
     }
     block_sum<2, false>(acc, lds_out);
     if (threadIdx.x == 0) {
