@@ -293,7 +293,10 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
     double acc[2] = {0.0, 0.0};
     auto plane = [&](int a) {
         const int k = c.k0 + a;
-        const float sl = lane_value(rn[a + 1].w, srcL), sr = lane_value(rn[a + 1].x, srcR);
+        // x-neighbours from the adjacent lanes (a regular tiling's rows lie
+        // within one wavefront, so a whole-wave shift serves)
+        const float sl = REG ? lane_prev(rn[a + 1].w) : lane_value(rn[a + 1].w, srcL);
+        const float sr = REG ? lane_next(rn[a + 1].x) : lane_value(rn[a + 1].x, srcR);
         const float xl = !REG && c.needL ? rne[a][0] : sl;
         const float xrv = !REG && c.needR ? rne[a][1] : sr;
         if (REG || (c.valid && k < g.nz)) {

@@ -168,6 +168,16 @@ __device__ __forceinline__ float4 sub4(const float4 &x, const float4 &y) {
 __device__ __forceinline__ float lane_value(float v, int src) {
     return __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(v)));
 }
+// value of lane - 1 and of lane + 1 through a whole-wavefront DPP shift (no
+// LDS round trip); lane 0 and lane 63 keep their own value
+__device__ __forceinline__ float lane_prev(float v) {
+    const int b = __float_as_int(v);
+    return __int_as_float(__builtin_amdgcn_update_dpp(b, b, 0x138 /* wave_shr:1 */, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float lane_next(float v) {
+    const int b = __float_as_int(v);
+    return __int_as_float(__builtin_amdgcn_update_dpp(b, b, 0x130 /* wave_shl:1 */, 0xf, 0xf, false));
+}
 
 // A x on the four cells of plane k of a column (lap7 per cell, interior only).
 // Every cell's stencil is formed and the non-interior ones are then zeroed by
