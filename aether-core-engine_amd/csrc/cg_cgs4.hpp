@@ -121,13 +121,18 @@ __device__ __forceinline__ void cgs4_pending(const CgRing &ring, int it, int oc,
     }
 }
 
-template <int ZU, bool FIRST, int NQ, int ST, int K, int PH, bool REG>
+// RG: 0 any tiling; 1 a regular tiling (vtiling_regular); 2 a regular tiling
+// with two rows per wavefront (32 float4 lanes per row), where each lane takes
+// the y-neighbour row inside its wavefront from the other half of the wave and
+// loads only the row outside it.
+template <int ZU, bool FIRST, int NQ, int ST, int K, int PH, int RG>
 __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
     Dims g, VTiling vt, int it, double rel_tol2, int nb_in, const double *__restrict__ pg_in,
     const double *__restrict__ pd_in, const double *__restrict__ pbb, double *__restrict__ pg_out,
     double *__restrict__ pd_out, CgState *st, CgRing ring, const float *__restrict__ w_in,
     float *__restrict__ w_out, float *__restrict__ d, float *__restrict__ p, int *done_mirror,
     int solve_id) {
+    constexpr bool REG = RG != 0;
     // K >= 3: the array a launch overwrites is neither its r_it nor its r_(it-1)
     static_assert(K >= 3 && K <= kCgRingMax && PH >= 0 && PH < K && (!FIRST || PH == 0),
                   "one-launch CG cycle");
@@ -175,6 +180,7 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
     // a - 1, so that the stencil of a plane waits only for the loads it reads
     float4 rz[ZU + 2], wz[ZU + 2], qz[ZU + 2], ry[ZU][2], wy[ZU][2], qy[ZU][2];
     float er[ZU][2], ew[ZU][2], eq[ZU][2];
+    const bool hi = c.lane >= 32;  // RG 2: the wavefront's second row
 #pragma unroll
     for (int a = 0; a < ZU + 2; ++a) {
         const int o = (min(max(c.k0 - 1 + a, 0), g.nz - 1) * g.sxy + c.off) >> 2;
@@ -184,6 +190,14 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
         if (a < 1 || a > ZU) continue;
         const int b = a - 1;
         const int base = min(c.k0 + b, g.nz - 1) * g.sxy;
+        if constexpr (RG == 2) {  // only the row outside the wavefront: [0]
+            const int oo = (base + (hi ? c.offyp : c.offym)) >> 2;
+            ry[b][0] = ld4<ST>(r_in, oo);
+            wy[b][0] = ld4<ST>(w_in, oo);
+            qy[b][0] = FIRST ? z4 : ld4<ST>(r_prev, oo);
+            ry[b][1] = wy[b][1] = qy[b][1] = z4;
+            continue;
+        }
         const int om = (base + c.offym) >> 2, op = (base + c.offyp) >> 2;
         ry[b][0] = ld4<ST>(r_in, om);
         ry[b][1] = ld4<ST>(r_in, op);
@@ -275,7 +289,7 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
     };
     auto r_new_y = [&](int a) {  // the y rows (and row-segment edges) of plane a
 #pragma unroll
-        for (int e = 0; e < 2; ++e) {
+        for (int e = 0; e < (RG == 2 ? 1 : 2); ++e) {
             rny[a][e] = r_new(ry[a][e], wy[a][e], qy[a][e]);
             if (!REG) {
                 const float se =
@@ -289,7 +303,12 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
 
     // ---- w' = A r' and the partials of r'.r', r'.w' on plane a ------------
     const int srcL = (c.lane + 63) & 63, srcR = (c.lane + 1) & 63;
-    const RegMasks rm = reg_masks(g, c);
+    RegMasks rm = reg_masks(g, c);
+    if constexpr (RG == 2) {  // the stencil's y terms in the order (outside row, inside row)
+        const float fm = rm.mym, fp = rm.myp;
+        rm.mym = hi ? fp : fm;
+        rm.myp = hi ? fm : fp;
+    }
     double acc[2] = {0.0, 0.0};
     auto plane = [&](int a) {
         const int k = c.k0 + a;
@@ -301,7 +320,12 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
         const float xrv = !REG && c.needR ? rne[a][1] : sr;
         if (REG || (c.valid && k < g.nz)) {
             float q[4];
-            if constexpr (REG)
+            if constexpr (RG == 2) {
+                const float4 &x = rn[a + 1];
+                const float4 yin = make_float4(other_half(x.x, hi), other_half(x.y, hi),
+                                               other_half(x.z, hi), other_half(x.w, hi));
+                lap4_reg(g, rm, k, x, xl, xrv, rny[a][0], yin, rn[a], rn[a + 2], q);
+            } else if constexpr (REG)
                 lap4_reg(g, rm, k, rn[a + 1], xl, xrv, rny[a][0], rny[a][1], rn[a], rn[a + 2], q);
             else
                 lap4(g, c, k, rn[a + 1], xl, xrv, rny[a][0], rny[a][1], rn[a], rn[a + 2], q);
@@ -443,31 +467,33 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4_tail(const CgState *st, in
 // passes and a tail; the 128^3 step measured ~1% slower with 3 or 4).
 constexpr int kCgsCycle = 8;
 
-template <int NQ, int ST, bool REG, int... PH>
+template <int NQ, int ST, int RG, int... PH>
 inline const void *cgs4_phase(int ph, std::integer_sequence<int, PH...>) {
-    const void *fns[] = {(const void *)k_cg_cgs4<4, false, NQ, ST, kCgsCycle, PH, REG>...};
+    const void *fns[] = {(const void *)k_cg_cgs4<4, false, NQ, ST, kCgsCycle, PH, RG>...};
     return fns[ph];
 }
 
-template <int NQ, int ST, bool REG>
+template <int NQ, int ST, int RG>
 inline const void *cgs4_pick(bool first, int ph) {
-    if (first) return (const void *)k_cg_cgs4<4, true, NQ, ST, kCgsCycle, 0, REG>;
-    return cgs4_phase<NQ, ST, REG>(ph, std::make_integer_sequence<int, kCgsCycle>{});
+    if (first) return (const void *)k_cg_cgs4<4, true, NQ, ST, kCgsCycle, 0, RG>;
+    return cgs4_phase<NQ, ST, RG>(ph, std::make_integer_sequence<int, kCgsCycle>{});
 }
 
 // The k_cg_cgs4 instantiation for launch phase ph (= it mod kCgsCycle).
 // nparts: partial sums read or written per launch (2, 4 or 16 per thread); wt:
 // write-through stores and 32-bit load offsets (arrays under 2 GiB), else
-// plain stores; reg: a regular tiling (vtiling_regular, write-through only).
-inline const void *cgs4_kernel(bool first, int ph, int nparts, bool wt, bool reg) {
+// plain stores; rg: the tiling class (RG of k_cg_cgs4; 1 and 2 need wt and at
+// most 1024 partials).
+inline const void *cgs4_kernel(bool first, int ph, int nparts, bool wt, int rg) {
     const bool wide = nparts > 4 * kThreads;
-    if (wt && reg && !wide)
-        return nparts <= 2 * kThreads ? cgs4_pick<2, kStWT, true>(first, ph)
-                                      : cgs4_pick<4, kStWT, true>(first, ph);
+    if (wt && rg != 0 && !wide) {
+        const bool two = nparts <= 2 * kThreads;
+        if (rg == 2) return two ? cgs4_pick<2, kStWT, 2>(first, ph) : cgs4_pick<4, kStWT, 2>(first, ph);
+        return two ? cgs4_pick<2, kStWT, 1>(first, ph) : cgs4_pick<4, kStWT, 1>(first, ph);
+    }
     if (wide)
-        return wt ? cgs4_pick<16, kStWT, false>(first, ph)
-                  : cgs4_pick<16, kStPlain, false>(first, ph);
-    return wt ? cgs4_pick<4, kStWT, false>(first, ph) : cgs4_pick<4, kStPlain, false>(first, ph);
+        return wt ? cgs4_pick<16, kStWT, 0>(first, ph) : cgs4_pick<16, kStPlain, 0>(first, ph);
+    return wt ? cgs4_pick<4, kStWT, 0>(first, ph) : cgs4_pick<4, kStPlain, 0>(first, ph);
 }
 
 inline void cgs4_tail_launch(unsigned nb, hipStream_t st, const CgState *cg, int last,

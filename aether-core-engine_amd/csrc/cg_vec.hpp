@@ -168,6 +168,13 @@ __device__ __forceinline__ float4 sub4(const float4 &x, const float4 &y) {
 __device__ __forceinline__ float lane_value(float v, int src) {
     return __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(v)));
 }
+// value of lane ^ 32 (the other half of the wavefront) by v_permlane32_swap:
+// its first result holds lanes 0-31 in both halves, its second lanes 32-63
+__device__ __forceinline__ float other_half(float v, bool hi) {
+    const unsigned b = __float_as_uint(v);
+    const auto r = __builtin_amdgcn_permlane32_swap(b, b, false, false);
+    return __uint_as_float(hi ? r[0] : r[1]);
+}
 // value of lane - 1 and of lane + 1 through a whole-wavefront DPP shift (no
 // LDS round trip); lane 0 and lane 63 keep their own value
 __device__ __forceinline__ float lane_prev(float v) {

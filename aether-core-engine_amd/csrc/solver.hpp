@@ -355,8 +355,9 @@ class Solver final : public SolverBase {
                 void *args[] = {&g,      &vt,     &itv, &rt,   &nbv,  &pg_in, &pd_in, &pbb, &pg_out,
                                 &pd_out, &st,     &ring, &w_in, &w_out, &dd,   &pp,    &dm,  &sid};
                 // write-through stores for arrays under 2 GiB
+                const int rg = !vtiling_regular(vt, g.ny, g.nz, 4) ? 0 : vt.txv == 32 ? 2 : 1;
                 const void *fn = cgs4_kernel(it == 0, it % kCgsCycle, std::max(nb_in, vt.nb),
-                                             wt_ok(), vtiling_regular(vt, g.ny, g.nz, 4));
+                                             wt_ok(), rg);
                 HIPT(hipExtLaunchKernel(fn, dim3(vt.nb), dim3(kThreads), args, 0, st_, e0, e1, 0));
                 return;
             }
