@@ -405,6 +405,7 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
             rn[a + 2] = r_new(rz[a + 2], wz[a + 2], qz[a + 2]);
             r_new_y(a);
             plane(a);
+            __builtin_amdgcn_sched_barrier(0);  // this plane's stores go out now
         }
     } else {
 #pragma unroll
