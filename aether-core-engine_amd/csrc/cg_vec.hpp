@@ -569,8 +569,6 @@ __global__ __launch_bounds__(kThreads) void k_rbsor4(Dims g, VTiling vt, float o
     int bx, by, bz;
     vtile_coords(vt, bx, by, bz);
     const VCol c = vcol_at<ZU>(g, vt, bx, by, bz);
-    const float4 *p4 = reinterpret_cast<const float4 *>(p_in);
-    const float4 *b4 = reinterpret_cast<const float4 *>(b);
     const Out4<ST> o_p(p_out, (long)g.sxy * g.nz * 4);
     const int t = threadIdx.x, ty = t / vt.txv;
     const bool first_row = ty == 0, last_row = ty == vt.ty - 1;
@@ -581,21 +579,36 @@ __global__ __launch_bounds__(kThreads) void k_rbsor4(Dims g, VTiling vt, float o
     float4 Y1[2][ZU + 2];  // rows j-1, j+1, planes k0-1 .. k0+ZU
     float4 BC[ZU + 2];     // b on row j, planes k0-1 .. k0+ZU
 #pragma unroll
-    for (int a = 0; a < ZU + 4; ++a) C[a] = p4[(plane(c.k0 - 2 + a) + c.off) >> 2];
+    for (int a = 0; a < ZU + 4; ++a) C[a] = ld4<ST>(p_in, (plane(c.k0 - 2 + a) + c.off) >> 2);
 #pragma unroll
     for (int a = 0; a < ZU + 2; ++a) {
         const int base = plane(c.k0 - 1 + a);
-        Y1[0][a] = p4[(base + c.offym) >> 2];
-        Y1[1][a] = p4[(base + c.offyp) >> 2];
-        BC[a] = b4[(base + c.off) >> 2];
+        Y1[0][a] = ld4<ST>(p_in, (base + c.offym) >> 2);
+        Y1[1][a] = ld4<ST>(p_in, (base + c.offyp) >> 2);
+        BC[a] = ld4<ST>(b, (base + c.off) >> 2);
     }
-    const int srcL = (c.lane + 63) & 63, srcR = (c.lane + 1) & 63;
+    // the first and last row of the tile also need p two rows away and b on
+    // the row beyond the tile; loaded with the rest, so that the colour-0
+    // values of that row cost no second memory round trip
+    const int dj_edge = first_row ? -1 : last_row ? 1 : 0;
+    float4 FAR[ZU], BB[ZU];
+    if (dj_edge != 0) {
+        const int rfar = row(2 * dj_edge), rnear = dj_edge < 0 ? c.offym : c.offyp;
+#pragma unroll
+        for (int a = 0; a < ZU; ++a) {
+            const int base = plane(c.k0 + a);
+            FAR[a] = ld4<ST>(p_in, (base + rfar) >> 2);
+            BB[a] = ld4<ST>(b, (base + rnear) >> 2);
+        }
+    }
     bool any;
-    // colour 0 on row j, planes k0-1 .. k0+ZU; the thread's own planes to LDS
+    // colour 0 on row j, planes k0-1 .. k0+ZU; the thread's own planes to LDS.
+    // x-neighbours from the adjacent lanes by whole-wave shifts: the launch
+    // condition puts whole rows in each wavefront (row ends are walls)
     float4 RC[ZU + 2];
 #pragma unroll
     for (int a = 0; a < ZU + 2; ++a) {
-        const float sl = lane_value(C[a + 1].w, srcL), sr = lane_value(C[a + 1].x, srcR);
+        const float sl = lane_prev(C[a + 1].w), sr = lane_next(C[a + 1].x);
         RC[a] = sor_colour4(g, c.i0, c.j, c.k0 - 1 + a, 0, omega, C[a + 1], sl, sr, Y1[0][a],
                             Y1[1][a], C[a], C[a + 2], BC[a], any);
     }
@@ -607,8 +620,8 @@ __global__ __launch_bounds__(kThreads) void k_rbsor4(Dims g, VTiling vt, float o
     for (int s2 = 0; s2 < 2; ++s2)
 #pragma unroll
         for (int a = 0; a < ZU; ++a) {
-            yl[s2][a] = lane_value(Y1[s2][a + 1].w, srcL);
-            yr[s2][a] = lane_value(Y1[s2][a + 1].x, srcR);
+            yl[s2][a] = lane_prev(Y1[s2][a + 1].w);
+            yr[s2][a] = lane_next(Y1[s2][a + 1].x);
         }
     // colour 0 on the row beyond the tile (first and last tile rows only)
     float4 RE[ZU];
@@ -618,15 +631,11 @@ __global__ __launch_bounds__(kThreads) void k_rbsor4(Dims g, VTiling vt, float o
     // arrays stay in registers)
     auto edge_row = [&](const float4(&y)[ZU + 2], const float(&l)[ZU], const float(&r)[ZU],
                         int dj) {
-        const int rfar = row(2 * dj), rnear = dj < 0 ? c.offym : c.offyp;
 #pragma unroll
-        for (int a = 0; a < ZU; ++a) {
-            const int base = plane(c.k0 + a);
-            const float4 far = p4[(base + rfar) >> 2], bb = b4[(base + rnear) >> 2];
+        for (int a = 0; a < ZU; ++a)
             RE[a] = sor_colour4(g, c.i0, c.j + dj, c.k0 + a, 0, omega, y[a + 1], l[a], r[a],
-                                dj < 0 ? far : C[a + 2], dj < 0 ? C[a + 2] : far, y[a],
-                                y[a + 2], bb, any);
-        }
+                                dj < 0 ? FAR[a] : C[a + 2], dj < 0 ? C[a + 2] : FAR[a], y[a],
+                                y[a + 2], BB[a], any);
     };
     if (first_row) edge_row(Y1[0], yl[0], yr[0], -1);
     if (last_row) edge_row(Y1[1], yl[1], yr[1], 1);
@@ -635,7 +644,7 @@ __global__ __launch_bounds__(kThreads) void k_rbsor4(Dims g, VTiling vt, float o
 #pragma unroll
     for (int a = 0; a < ZU; ++a) {
         const int k = c.k0 + a;
-        const float sl = lane_value(RC[a + 1].w, srcL), sr = lane_value(RC[a + 1].x, srcR);
+        const float sl = lane_prev(RC[a + 1].w), sr = lane_next(RC[a + 1].x);
         // (indices kept in range on the rows that take RE instead)
         const float4 ym = first_row ? RE[a] : lds_rc[a][max(t - vt.txv, 0)];
         const float4 yp = last_row ? RE[a] : lds_rc[a][min(t + vt.txv, kThreads - 1)];
