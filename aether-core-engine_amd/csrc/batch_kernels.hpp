@@ -236,7 +236,7 @@ __global__ __launch_bounds__(kThreads) void kb_cg_p_tail(const BInst<float> *__r
         const int k = c.k0 + a;
         if (c.valid && k < in.g.nz) {
             const int id4 = (k * in.g.sxy + c.off) >> 2;
-            p4[id4] = axpy4(p4[id4], ta, d4[id4]);
+            p4[id4] = fma4(ta, d4[id4], p4[id4]);
         }
     }
 }

@@ -302,7 +302,6 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
     };
 
     // ---- w' = A r' and the partials of r'.r', r'.w' on plane a ------------
-    const int srcL = (c.lane + 63) & 63, srcR = (c.lane + 1) & 63;
     RegMasks rm = reg_masks(g, c);
     if constexpr (RG == 2) {  // the stencil's y terms in the order (outside row, inside row)
         const float fm = rm.mym, fp = rm.myp;
@@ -312,10 +311,9 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
     double acc[2] = {0.0, 0.0};
     auto plane = [&](int a) {
         const int k = c.k0 + a;
-        // x-neighbours from the adjacent lanes (a regular tiling's rows lie
-        // within one wavefront, so a whole-wave shift serves)
-        const float sl = REG ? lane_prev(rn[a + 1].w) : lane_value(rn[a + 1].w, srcL);
-        const float sr = REG ? lane_next(rn[a + 1].x) : lane_value(rn[a + 1].x, srcR);
+        // x-neighbours from the adjacent lanes by whole-wave shifts (lanes 0
+        // and 63 sit at a row start or end, or take the edge loads)
+        const float sl = lane_prev(rn[a + 1].w), sr = lane_next(rn[a + 1].x);
         const float xl = !REG && c.needL ? rne[a][0] : sl;
         const float xrv = !REG && c.needR ? rne[a][1] : sr;
         if (REG || (c.valid && k < g.nz)) {

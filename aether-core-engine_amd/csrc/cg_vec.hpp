@@ -1,11 +1,13 @@
 // cg_vec.hpp -- the two-launch CG iteration (and the red-black SOR pass) with
 // four x-cells per thread.
 //
-// Same arithmetic as body_cg_dir / body_cg_update (src/fluid.rs:463-487),
-// laid out for the memory pipeline: every thread owns a 4 (x) x ZU (z)
-// block, moves it with 16-byte loads and stores, and takes its x-neighbours
-// from the adjacent lanes (cross-lane permute) instead of from memory; only
-// lanes at a row-segment or wavefront edge load their x-neighbour.  A
+// The recurrences of body_cg_dir / body_cg_update (src/fluid.rs:463-487),
+// with each a x + y a fused multiply-add, laid out for the memory pipeline:
+// every thread owns a 4 (x) x ZU (z) block, moves it with 16-byte loads and
+// stores, and takes its x-neighbours from the adjacent lanes (whole-wave DPP
+// shifts) instead of from memory; only lanes at a row-segment or wavefront
+// edge load their x-neighbour.  The red-black SOR kernels keep the exact
+// per-cell arithmetic of k_rbgs.  A
 // one-cell-per-thread stencil issues ~10 four-byte memory instructions per
 // cell and is limited by the address unit, not by HBM; here it is ~2
 // sixteen-byte ones per four cells.  Needs nx % 4 == 0 (rows start
@@ -158,15 +160,8 @@ __device__ __forceinline__ float4 fma4(float a, const float4 &x, const float4 &y
 __device__ __forceinline__ float f4(const float4 &v, int e) {
     return e == 0 ? v.x : e == 1 ? v.y : e == 2 ? v.z : v.w;
 }
-__device__ __forceinline__ float4 axpy4(const float4 &x, float a, const float4 &y) {
-    return make_float4(x.x + a * y.x, x.y + a * y.y, x.z + a * y.z, x.w + a * y.w);
-}
 __device__ __forceinline__ float4 sub4(const float4 &x, const float4 &y) {
     return make_float4(x.x - y.x, x.y - y.y, x.z - y.z, x.w - y.w);
-}
-// value of lane `src` (every lane of the wave must execute this)
-__device__ __forceinline__ float lane_value(float v, int src) {
-    return __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(v)));
 }
 // value of lane ^ 32 (the other half of the wavefront) by v_permlane32_swap:
 // its first result holds lanes 0-31 in both halves, its second lanes 32-63
@@ -223,32 +218,30 @@ __device__ __forceinline__ void body_cg_dir4(
     if (FIRST) pbb.load(part_bb, nb_in);
     const double delta_prev = FIRST ? 0.0 : st->delta[(it - 1) & 1];
     const double b2_prev = FIRST ? 0.0 : st->b2;
-    const float4 *r4 = reinterpret_cast<const float4 *>(r);
-    const float4 *d4 = reinterpret_cast<const float4 *>(dold);
     const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
     float4 rz[ZU + 2], dz[ZU + 2], ry[ZU][2], dy[ZU][2];
     float xr[ZU][2], xd[ZU][2];
 #pragma unroll
     for (int a = 0; a < ZU + 2; ++a) {
         const int base = min(max(c.k0 - 1 + a, 0), g.nz - 1) * g.sxy;
-        rz[a] = r4[(base + c.off) >> 2];
-        dz[a] = FIRST ? z4 : d4[(base + c.off) >> 2];
+        rz[a] = ld4<ST>(r, (base + c.off) >> 2);
+        dz[a] = FIRST ? z4 : ld4<ST>(dold, (base + c.off) >> 2);
     }
 #pragma unroll
     for (int a = 0; a < ZU; ++a) {
         const int base = min(c.k0 + a, g.nz - 1) * g.sxy;
-        ry[a][0] = r4[(base + c.offym) >> 2];
-        ry[a][1] = r4[(base + c.offyp) >> 2];
-        dy[a][0] = FIRST ? z4 : d4[(base + c.offym) >> 2];
-        dy[a][1] = FIRST ? z4 : d4[(base + c.offyp) >> 2];
+        ry[a][0] = ld4<ST>(r, (base + c.offym) >> 2);
+        ry[a][1] = ld4<ST>(r, (base + c.offyp) >> 2);
+        dy[a][0] = FIRST ? z4 : ld4<ST>(dold, (base + c.offym) >> 2);
+        dy[a][1] = FIRST ? z4 : ld4<ST>(dold, (base + c.offyp) >> 2);
         xr[a][0] = xr[a][1] = xd[a][0] = xd[a][1] = 0.0f;
         if (c.needL) {
-            xr[a][0] = r[base + c.off - 1];
-            if (!FIRST) xd[a][0] = dold[base + c.off - 1];
+            xr[a][0] = ld1<ST>(r, base + c.off - 1);
+            if (!FIRST) xd[a][0] = ld1<ST>(dold, base + c.off - 1);
         }
         if (c.needR) {
-            xr[a][1] = r[base + c.off + 4];
-            if (!FIRST) xd[a][1] = dold[base + c.off + 4];
+            xr[a][1] = ld1<ST>(r, base + c.off + 4);
+            if (!FIRST) xd[a][1] = ld1<ST>(dold, base + c.off + 4);
         }
     }
     // stopping rules (every workgroup reaches the same decision); the stop
@@ -297,7 +290,7 @@ __device__ __forceinline__ void body_cg_dir4(
                     const int k = c.k0 + a;
                     if (c.valid && k < g.nz) {
                         const int id4 = (k * g.sxy + c.off) >> 2;
-                        p4[id4] = axpy4(p4[id4], tap, dz[a + 1]);
+                        p4[id4] = fma4(tap, dz[a + 1], p4[id4]);
                     }
                 }
             }
@@ -308,23 +301,23 @@ __device__ __forceinline__ void body_cg_dir4(
     const float tb = (float)beta;
     float4 dc[ZU + 2];
 #pragma unroll
-    for (int a = 0; a < ZU + 2; ++a) dc[a] = FIRST ? rz[a] : axpy4(rz[a], tb, dz[a]);
-    const int srcL = (c.lane + 63) & 63, srcR = (c.lane + 1) & 63;
+    for (int a = 0; a < ZU + 2; ++a) dc[a] = FIRST ? rz[a] : fma4(tb, dz[a], rz[a]);
     double acc[1] = {0.0};
     const Out4<ST> o_d(dnew, (long)g.sxy * g.nz * 4);
 #pragma unroll
     for (int a = 0; a < ZU; ++a) {
         const int k = c.k0 + a;
-        const float4 ym = FIRST ? ry[a][0] : axpy4(ry[a][0], tb, dy[a][0]);
-        const float4 yp = FIRST ? ry[a][1] : axpy4(ry[a][1], tb, dy[a][1]);
-        const float sl = lane_value(dc[a + 1].w, srcL), sr = lane_value(dc[a + 1].x, srcR);
-        const float xl = c.needL ? (FIRST ? xr[a][0] : xr[a][0] + tb * xd[a][0]) : sl;
-        const float xrv = c.needR ? (FIRST ? xr[a][1] : xr[a][1] + tb * xd[a][1]) : sr;
+        const float4 ym = FIRST ? ry[a][0] : fma4(tb, dy[a][0], ry[a][0]);
+        const float4 yp = FIRST ? ry[a][1] : fma4(tb, dy[a][1], ry[a][1]);
+        const float sl = lane_prev(dc[a + 1].w), sr = lane_next(dc[a + 1].x);
+        const float xl = c.needL ? (FIRST ? xr[a][0] : __builtin_fmaf(tb, xd[a][0], xr[a][0])) : sl;
+        const float xrv = c.needR ? (FIRST ? xr[a][1] : __builtin_fmaf(tb, xd[a][1], xr[a][1])) : sr;
         if (c.valid && k < g.nz) {
             float q[4];
             lap4(g, c, k, dc[a + 1], xl, xrv, ym, yp, dc[a], dc[a + 2], q);
 #pragma unroll
-            for (int e = 0; e < 4; ++e) acc[0] += (double)f4(dc[a + 1], e) * (double)q[e];
+            for (int e = 0; e < 4; ++e)
+                acc[0] = __builtin_fma((double)f4(dc[a + 1], e), (double)q[e], acc[0]);
             o_d.put((k * g.sxy + c.off) >> 2, dc[a + 1]);
         }
     }
@@ -366,9 +359,6 @@ __device__ __forceinline__ void body_cg_update4(
     EarlyPartialsT<NQ> pdq;
     pdq.load(part_dq, nb);
     const double delta = st->delta[it & 1];
-    const float4 *d4 = reinterpret_cast<const float4 *>(d);
-    float4 *p4 = reinterpret_cast<float4 *>(p);
-    float4 *r4 = reinterpret_cast<float4 *>(r);
     const Out4<ST> o_p(p, (long)g.sxy * g.nz * 4), o_r(r, (long)g.sxy * g.nz * 4);
     const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
     float4 dz[ZU + 2], dy[ZU][2], pc[ZU], rc[ZU], dp[ZU];
@@ -376,19 +366,19 @@ __device__ __forceinline__ void body_cg_update4(
 #pragma unroll
     for (int a = 0; a < ZU + 2; ++a) {
         const int base = min(max(c.k0 - 1 + a, 0), g.nz - 1) * g.sxy;
-        dz[a] = d4[(base + c.off) >> 2];
+        dz[a] = ld4<ST>(d, (base + c.off) >> 2);
     }
 #pragma unroll
     for (int a = 0; a < ZU; ++a) {
         const int base = min(c.k0 + a, g.nz - 1) * g.sxy;
-        dy[a][0] = d4[(base + c.offym) >> 2];
-        dy[a][1] = d4[(base + c.offyp) >> 2];
-        pc[a] = PM == kPSkip ? z4 : p4[(base + c.off) >> 2];
-        dp[a] = PM == kPBoth ? reinterpret_cast<const float4 *>(dprev)[(base + c.off) >> 2] : z4;
-        rc[a] = r4[(base + c.off) >> 2];
+        dy[a][0] = ld4<ST>(d, (base + c.offym) >> 2);
+        dy[a][1] = ld4<ST>(d, (base + c.offyp) >> 2);
+        pc[a] = PM == kPSkip ? z4 : ld4<ST>(p, (base + c.off) >> 2);
+        dp[a] = PM == kPBoth ? ld4<ST>(dprev, (base + c.off) >> 2) : z4;
+        rc[a] = ld4<ST>(r, (base + c.off) >> 2);
         xd[a][0] = xd[a][1] = 0.0f;
-        if (c.needL) xd[a][0] = d[base + c.off - 1];
-        if (c.needR) xd[a][1] = d[base + c.off + 4];
+        if (c.needL) xd[a][0] = ld1<ST>(d, base + c.off - 1);
+        if (c.needR) xd[a][1] = ld1<ST>(d, base + c.off + 4);
     }
     const bool lead = slot == 0 && threadIdx.x == 0;
     // the stop word holds the stopping launch + 1 (launch 2 it + 1 here)
@@ -408,7 +398,7 @@ __device__ __forceinline__ void body_cg_update4(
 #pragma unroll
             for (int a = 0; a < ZU; ++a) {
                 const int k = c.k0 + a;
-                if (c.valid && k < g.nz) o_p.put((k * g.sxy + c.off) >> 2, axpy4(pc[a], tap, dp[a]));
+                if (c.valid && k < g.nz) o_p.put((k * g.sxy + c.off) >> 2, fma4(tap, dp[a], pc[a]));
             }
         }
         return;
@@ -416,25 +406,23 @@ __device__ __forceinline__ void body_cg_update4(
     const double alpha = delta / dq;
     if (lead) st->cg_alpha[it & 1] = alpha;
     const float ta = (float)alpha;
-    const int srcL = (c.lane + 63) & 63, srcR = (c.lane + 1) & 63;
     double acc[1] = {0.0};
 #pragma unroll
     for (int a = 0; a < ZU; ++a) {
         const int k = c.k0 + a;
-        const float sl = lane_value(dz[a + 1].w, srcL), sr = lane_value(dz[a + 1].x, srcR);
+        const float sl = lane_prev(dz[a + 1].w), sr = lane_next(dz[a + 1].x);
         const float xl = c.needL ? xd[a][0] : sl;
         const float xrv = c.needR ? xd[a][1] : sr;
         if (c.valid && k < g.nz) {
             float q[4];
             lap4(g, c, k, dz[a + 1], xl, xrv, dy[a][0], dy[a][1], dz[a], dz[a + 2], q);
-            const float4 pn = PM == kPBoth ? axpy4(axpy4(pc[a], tap, dp[a]), ta, dz[a + 1])
-                                           : axpy4(pc[a], ta, dz[a + 1]);
-            const float4 rn = make_float4(rc[a].x - ta * q[0], rc[a].y - ta * q[1],
-                                          rc[a].z - ta * q[2], rc[a].w - ta * q[3]);
-            acc[0] += (double)rn.x * (double)rn.x;
-            acc[0] += (double)rn.y * (double)rn.y;
-            acc[0] += (double)rn.z * (double)rn.z;
-            acc[0] += (double)rn.w * (double)rn.w;
+            const float4 pn = PM == kPBoth ? fma4(ta, dz[a + 1], fma4(tap, dp[a], pc[a]))
+                                           : fma4(ta, dz[a + 1], pc[a]);
+            const float4 rn = fma4(-ta, make_float4(q[0], q[1], q[2], q[3]), rc[a]);
+            acc[0] = __builtin_fma((double)rn.x, (double)rn.x, acc[0]);
+            acc[0] = __builtin_fma((double)rn.y, (double)rn.y, acc[0]);
+            acc[0] = __builtin_fma((double)rn.z, (double)rn.z, acc[0]);
+            acc[0] = __builtin_fma((double)rn.w, (double)rn.w, acc[0]);
             const int id4 = (k * g.sxy + c.off) >> 2;
             if (PM != kPSkip) o_p.put(id4, pn);
             o_r.put(id4, rn);
@@ -467,7 +455,7 @@ __global__ __launch_bounds__(kThreads) void k_cg_p_tail(const CgState *st, int l
     if (st->done) return;
     const float ta = (float)st->cg_alpha[last_it & 1];
     for (long a = (long)blockIdx.x * kThreads + threadIdx.x; a < n4; a += (long)gridDim.x * kThreads)
-        p[a] = axpy4(p[a], ta, d[a]);
+        p[a] = fma4(ta, d[a], p[a]);
 }
 
 // Red-black SOR update of the colour-`colour` interior cells among four
@@ -512,31 +500,28 @@ __global__ __launch_bounds__(kThreads) void k_rbgs4(Dims g, VTiling vt, int colo
     int bx, by, bz;
     vtile_coords(vt, bx, by, bz);
     const VCol c = vcol_at<ZU>(g, vt, bx, by, bz);
-    const float4 *p4 = reinterpret_cast<const float4 *>(p);
     const Out4<ST> o_p(p, (long)g.sxy * g.nz * 4);
-    const float4 *b4 = reinterpret_cast<const float4 *>(b);
     float4 pz[ZU + 2], py[ZU][2], bc[ZU];
     float xl[ZU], xr[ZU];
 #pragma unroll
     for (int a = 0; a < ZU + 2; ++a) {
         const int base = min(max(c.k0 - 1 + a, 0), g.nz - 1) * g.sxy;
-        pz[a] = p4[(base + c.off) >> 2];
+        pz[a] = ld4<ST>(p, (base + c.off) >> 2);
     }
 #pragma unroll
     for (int a = 0; a < ZU; ++a) {
         const int base = min(c.k0 + a, g.nz - 1) * g.sxy;
-        py[a][0] = p4[(base + c.offym) >> 2];
-        py[a][1] = p4[(base + c.offyp) >> 2];
-        bc[a] = b4[(base + c.off) >> 2];
+        py[a][0] = ld4<ST>(p, (base + c.offym) >> 2);
+        py[a][1] = ld4<ST>(p, (base + c.offyp) >> 2);
+        bc[a] = ld4<ST>(b, (base + c.off) >> 2);
         xl[a] = xr[a] = 0.0f;
         if (c.needL) xl[a] = p[base + c.off - 1];
         if (c.needR) xr[a] = p[base + c.off + 4];
     }
-    const int srcL = (c.lane + 63) & 63, srcR = (c.lane + 1) & 63;
 #pragma unroll
     for (int a = 0; a < ZU; ++a) {
         const int k = c.k0 + a;
-        const float sl = lane_value(pz[a + 1].w, srcL), sr = lane_value(pz[a + 1].x, srcR);
+        const float sl = lane_prev(pz[a + 1].w), sr = lane_next(pz[a + 1].x);
         const float left = c.needL ? xl[a] : sl, right = c.needR ? xr[a] : sr;
         if (c.valid && k < g.nz) {
             bool any;
