@@ -23,7 +23,7 @@ AETHER_EHIP = -2
 AETHER_ENOMEM = -3
 AETHER_ENODEV = -4
 AETHER_EINTERNAL = -5
-ABI_VERSION = 2  # include/aether_fluid.h AETHER_FLUID_ABI_VERSION
+ABI_VERSION = 3  # include/aether_fluid.h AETHER_FLUID_ABI_VERSION
 
 FP32, FP64, FP32_IR64 = 0, 1, 2
 SOLVER_CG, SOLVER_RBGS = 0, 1
@@ -218,7 +218,7 @@ def lib() -> C.CDLL:
         fn = getattr(h, name)
         fn.restype = res
         fn.argtypes = args
-    if h.aether_fluid_abi_version() != ABI_VERSION:  # the structs below follow version 2
+    if h.aether_fluid_abi_version() != ABI_VERSION:  # the structs below follow version 3
         raise AetherFluidError(AETHER_EINTERNAL,
                                f"{LIB_PATH} has ABI version {h.aether_fluid_abi_version()}, "
                                f"this binding expects {ABI_VERSION}: rebuild it")

@@ -17,8 +17,9 @@ class FluidBatch:
     (solver_iters, cg_rel_tol, dissipation, cg_form, cg_z_cells) come from
     keyword arguments; cg_form CG_AUTO runs the four-wide CG when every
     nx % 4 == 0 (FP32), with cg_z_cells (2 or 4) z planes per thread, 4 when
-    0 and not tuned.  graph=True captures each step into a HIP graph and
-    replays it."""
+    0 and not tuned, or the autotuner's choice; CG_FUSED runs the one-launch
+    CG solved group by group (FP32, every nx % 4 == 0).  graph=True captures
+    each step into a HIP graph and replays it."""
 
     def __init__(self, instances, precision: str = "fp32", solver_iters: int = 200,
                  cg_rel_tol: float = 1e-5, dissipation=(0.996, 0.996, 0.985),
@@ -89,17 +90,18 @@ class FluidBatch:
     def autotune_cg(self, iters: int = 20) -> dict:
         """Times `iters` CG iterations of the batch in each candidate layout
         and tiling (four-wide with 2 and 4 z planes per thread, one cell per
-        thread) on synthetic right-hand sides and keeps the fastest as the
-        batch's choice for CG_AUTO and cg_z_cells 0 (pressure and divergence
-        are restored)."""
-        us = (C.c_double * 3)()
+        thread, one launch per iteration solved group by group) on synthetic
+        right-hand sides and keeps the fastest as the batch's choice for
+        CG_AUTO and cg_z_cells 0 (pressure and divergence are restored)."""
+        us = (C.c_double * 4)()
         form, z = C.c_int(), C.c_int()
         L.check(self._lib.aether_fluid_batch_autotune_cg(self._h, int(iters), us,
                                                          C.byref(form), C.byref(z)))
-        return {"choice": {L.CG_X4: "x4", L.CG_SCALAR: "scalar"}[form.value],
+        return {"choice": {L.CG_X4: "x4", L.CG_SCALAR: "scalar",
+                           L.CG_FUSED: "fused"}[form.value],
                 "z_cells": z.value,
                 "us_per_iter": {k: round(us[a], 3)
-                                for a, k in enumerate(("x4_z2", "x4_z4", "scalar_z8"))
+                                for a, k in enumerate(("x4_z2", "x4_z4", "scalar_z8", "fused_z4"))
                                 if us[a] >= 0}}
 
     def iterations(self) -> np.ndarray:

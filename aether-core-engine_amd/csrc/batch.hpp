@@ -1,6 +1,8 @@
 // batch.hpp -- many smoke grids advanced together, one launch per stage.
 // Included by batch.hip only.
 #pragma once
+#include <deque>
+
 #include "host_common.hpp"
 #include "cg_vec.hpp"
 #include "batch_kernels.hpp"
@@ -27,6 +29,10 @@ class Batch final : public BatchBase {
         aether_fluid_instance_desc d;
         DevBuf<T> f[2][5], fwd0, fwd1, curl, p, b, r, dd[2];
         DevBuf<double> pa, pb, pc, ptel, tel, staging;
+        // one-launch CG (allocated on first use): ring[1..7] beside r, w pair,
+        // delta partials
+        DevBuf<T> ring[kCgRingMax - 1], w[2];
+        DevBuf<double> pe, pf;
         DevBuf<CgState> st;
         DevBuf<int> iters;
     };
@@ -258,6 +264,14 @@ class Batch final : public BatchBase {
             t.iters = in.iters.p;
             t.vt = in.vtz[vzi()];
             t.nbv = t.vt.nb;
+            t.ring[0] = in.r.p;
+            for (int q = 1; q < kCgRingMax; ++q) t.ring[q] = in.ring[q - 1].p;
+            t.w[0] = in.w[0].p;
+            t.w[1] = in.w[1].p;
+            t.pe = in.pe.p;
+            t.pf = in.pf.p;
+            t.vt4 = in.vtz[1];
+            t.nb4 = in.vtz[1].nb;
         }
         HIPT(hipMemcpyAsync(table_.p, tab.data(), tab.size() * sizeof(BInst<T>),
                             hipMemcpyHostToDevice, st_));
@@ -286,29 +300,100 @@ class Batch final : public BatchBase {
         return vz == 2 ? 0 : 1;
     }
 
-    // The batch's CG layout for the current knobs: four-wide or one cell per thread.
-    bool use_x4() const {
-        if (trial_form_) return trial_form_ == AETHER_CG_X4;
+    // The batch's CG layout for the current knobs: AETHER_CG_X4 (four-wide,
+    // two launches per iteration over every instance), AETHER_CG_FUSED (one
+    // launch per iteration, solved group by group) or AETHER_CG_SCALAR.
+    int form() const {
+        if (trial_form_) return trial_form_;
         switch (params_.cg_form) {
         case AETHER_CG_SCALAR:
-            return false;
+            return AETHER_CG_SCALAR;
         case AETHER_CG_X4:
             if (!x4_ok_) raise(AETHER_EINVAL, "AETHER_CG_X4 needs FP32 and nx % 4 == 0 in every instance");
-            return true;
+            return AETHER_CG_X4;
         case AETHER_CG_FUSED:
-            raise(AETHER_EINVAL, "AETHER_CG_FUSED is not available for batches");
+            if (!x4_ok_)
+                raise(AETHER_EINVAL, "AETHER_CG_FUSED for batches needs FP32 and nx % 4 == 0 "
+                                     "in every instance");
+            return AETHER_CG_FUSED;
         default:
-            if (tuned_form_) return tuned_form_ == AETHER_CG_X4;
-            return x4_ok_;
+            if (tuned_form_) return tuned_form_;
+            return x4_ok_ ? AETHER_CG_X4 : AETHER_CG_SCALAR;
         }
+    }
+    bool use_x4() const { return form() == AETHER_CG_X4; }
+
+    // The one-launch CG's buffers, and its groups: instances whose launches
+    // share a kernel (the same tiling class and partial-sum width, for the
+    // first launch and for the others), packed in instance order into groups
+    // of at most kOneLaunchGroupCells cells, so that a group's CG working set
+    // stays in the Infinity Cache through its whole solve.
+    static constexpr size_t kOneLaunchGroupCells = 3200000;
+    void ensure_fused() {
+        if (fused_ready_) return;
+        if (!x4_ok_)
+            raise(AETHER_EINVAL, "AETHER_CG_FUSED for batches needs FP32 and nx % 4 == 0 in "
+                                 "every instance");
+        for (auto &in : inst_) {
+            for (DevBuf<T> &b : in->ring) {
+                b.alloc(in->n);
+                fill(b.p, in->n);
+            }
+            for (DevBuf<T> &b : in->w) {
+                b.alloc(in->n);
+                fill(b.p, in->n);
+            }
+            in->pe.alloc(in->pa.n);
+            in->pf.alloc(in->pa.n);
+        }
+        groups_.clear();
+        for (int q = 0; q < count(); ++q) {
+            const Inst &in = *inst_[q];
+            const VTiling &v = in.vtz[1];
+            const int rg = !vtiling_regular(v, in.g.ny, in.g.nz, 4) ? 0 : v.txv == 32 ? 2 : 1;
+            const int c0 = kb_cgs4_class(std::max(in.nb, v.nb), rg), c1 = kb_cgs4_class(v.nb, rg);
+            Group *gp = nullptr;
+            for (Group &gr : groups_)
+                if (gr.open && gr.cls_first == c0 && gr.cls == c1) gp = &gr;
+            if (gp && gp->cells + in.n > kOneLaunchGroupCells) {
+                gp->open = false;
+                gp = nullptr;
+            }
+            if (!gp) {
+                groups_.emplace_back();
+                gp = &groups_.back();
+                gp->cls_first = c0;
+                gp->cls = c1;
+            }
+            gp->insts.push_back(q);
+            gp->cells += in.n;
+        }
+        for (Group &gr : groups_) {
+            std::vector<BItem> items;
+            for (int q : gr.insts) {
+                const VTiling &v = inst_[q]->vtz[1];
+                for (int b = 0; b < v.nb; ++b) {  // the single-grid launch order
+                    const int t = v.remap ? (b & 7) * (v.nb >> 3) + (b >> 3) : b;
+                    const int bx = t % v.gx, rest = t / v.gx;
+                    items.push_back({q, bx, rest % v.gy, rest / v.gy, b, {0, 0, 0}});
+                }
+            }
+            gr.nitems = (int)items.size();
+            gr.items.alloc(items.size());
+            HIPT(hipMemcpyAsync(gr.items.p, items.data(), items.size() * sizeof(BItem),
+                                hipMemcpyHostToDevice, st_));
+        }
+        HIPT(hipStreamSynchronize(st_));
+        fused_ready_ = true;
     }
 
     // Times `iters` CG iterations of the batch in each candidate (four-wide
     // with 2 and 4 z planes per thread when every nx % 4 == 0, one cell per
-    // thread with 8) on synthetic right-hand sides (every instance's pressure
-    // and divergence are saved and restored) and keeps the fastest as the
-    // choice for cg_form AETHER_CG_AUTO and cg_z_cells 0.  us[3]: time per
-    // iteration of the whole batch, in that order (-1 when not applicable).
+    // thread with 8, and the one-launch form solved group by group) on
+    // synthetic right-hand sides (every instance's pressure and divergence
+    // are saved and restored) and keeps the fastest as the choice for
+    // cg_form AETHER_CG_AUTO and cg_z_cells 0.  us[4]: time per iteration of
+    // the whole batch, in that order (-1 when not applicable).
     void autotune(int iters, double *us, int *form_out, int *z_out) override {
         if (iters < 1) raise(AETHER_EINVAL, "iters must be >= 1");
         for (auto &in : inst_) {  // finalize logs iteration 0's count
@@ -334,7 +419,10 @@ class Batch final : public BatchBase {
         struct Cand {
             int form, z;
         };
-        const Cand cands[3] = {{AETHER_CG_X4, 2}, {AETHER_CG_X4, 4}, {AETHER_CG_SCALAR, kBatchZU}};
+        const Cand cands[4] = {{AETHER_CG_X4, 2},
+                               {AETHER_CG_X4, 4},
+                               {AETHER_CG_SCALAR, kBatchZU},
+                               {AETHER_CG_FUSED, 4}};
         double best = 1e300;
         Cand pick = {AETHER_CG_SCALAR, kBatchZU};
         auto cleanup = [&] {
@@ -345,9 +433,10 @@ class Batch final : public BatchBase {
             (void)hipEventDestroy(e1);
         };
         try {
-            for (int a = 0; a < 3; ++a) {
+            for (int a = 0; a < 4; ++a) {
                 if (us) us[a] = -1.0;
-                if (cands[a].form == AETHER_CG_X4 && !x4_ok_) continue;
+                if (cands[a].form != AETHER_CG_SCALAR && !x4_ok_) continue;
+                if (cands[a].form == AETHER_CG_FUSED) ensure_fused();
                 trial_form_ = cands[a].form;
                 trial_vz_ = cands[a].form == AETHER_CG_X4 ? cands[a].z : 0;
                 write_table();  // the table holds the four-wide tiling in use
@@ -436,12 +525,23 @@ class Batch final : public BatchBase {
     // host look-ahead) and the step index comes from the device.
     void cg_solve(int step, bool graph = false) {
         const int max_iters = params_.solver_iters;
-        const bool x4 = use_x4();
+        const int fm = form();
+        const bool x4 = fm == AETHER_CG_X4;
         const int zc = params_.cg_z_cells;
-        if (!trial_form_ && zc && (x4 ? zc != 2 && zc != 4 : zc != kBatchZU))
-            raise(AETHER_EINVAL, x4 ? "the batch's four-wide CG takes cg_z_cells 2 or 4"
-                                    : "the batch's one-cell-per-thread CG runs 8 z cells per "
-                                      "thread (cg_z_cells 0 or 8)");
+        if (!trial_form_ && zc) {
+            if (fm == AETHER_CG_X4 && zc != 2 && zc != 4)
+                raise(AETHER_EINVAL, "the batch's four-wide CG takes cg_z_cells 2 or 4");
+            if (fm == AETHER_CG_FUSED && zc != 4)
+                raise(AETHER_EINVAL, "the batch's one-launch CG runs 4 z planes per thread "
+                                     "(cg_z_cells 0 or 4)");
+            if (fm == AETHER_CG_SCALAR && zc != kBatchZU)
+                raise(AETHER_EINVAL, "the batch's one-cell-per-thread CG runs 8 z cells per "
+                                     "thread (cg_z_cells 0 or 8)");
+        }
+        if (fm == AETHER_CG_FUSED) {
+            cg_solve_fused(step, graph);
+            return;
+        }
         const double rel2 = params_.cg_rel_tol > 0 ? params_.cg_rel_tol * params_.cg_rel_tol : 0.0;
         const int solve_id = ++solve_seq_;
         const int chunk = 8;
@@ -494,6 +594,85 @@ class Batch final : public BatchBase {
         kb_cg_finalize<T><<<count(), blk, 0, st_>>>(table_.p, max_iters, rel2, step, x4 ? 1 : 0,
                                                     graph ? step_dev_.p : nullptr);
         LAUNCH_CHECK();
+    }
+
+    // The one-launch CG (cg_cgs4.hpp), solved group by group: every launch of
+    // a group's solve runs before the next group starts, so that the group's
+    // ring, w, d and p stay in the Infinity Cache.  Within a group the
+    // launches are paced as the other forms (chunks of 8, two ahead, stop
+    // once every instance of the group has published this solve as finished);
+    // an instance that converged first returns from each later launch before
+    // loading anything.
+    void cg_solve_fused(int step, bool graph) {
+        if constexpr (sizeof(T) == 4) {
+            ensure_fused();
+            const int max_iters = params_.solver_iters;
+            const double rel2 =
+                params_.cg_rel_tol > 0 ? params_.cg_rel_tol * params_.cg_rel_tol : 0.0;
+            const int solve_id = ++solve_seq_;
+            const int chunk = 8;
+            const int nchunks = (max_iters + chunk - 1) / chunk;
+            while ((int)chunk_events_.size() < nchunks) {
+                hipEvent_t e;
+                HIPT(hipEventCreateWithFlags(&e, hipEventDisableTiming | hipEventDisableSystemFence));
+                chunk_events_.push_back(e);
+            }
+            const dim3 blk(kBX, kBY);
+            if (max_iters > 0) {  // delta_0 partials and w_0 = A r_0 of every instance
+                kb_cg_rar<T, kBatchZU><<<ncg_, blk, 0, st_>>>(table_.p, cg_items_.p);
+                LAUNCH_CHECK();
+            }
+            const BInst<T> *tab = table_.p;
+            int *dm = done_dev_;
+            for (Group &gr : groups_) {
+                const BItem *items = gr.items.p;
+                for (int c = 0; c < nchunks; ++c) {
+                    if (c >= 2 && !graph) {
+                        HIPT(hipEventSynchronize(chunk_events_[c - 2]));
+                        bool all = true;
+                        for (int q : gr.insts)
+                            if (__atomic_load_n(done_host_ + q, __ATOMIC_ACQUIRE) != solve_id) {
+                                all = false;
+                                break;
+                            }
+                        if (all) break;
+                    }
+                    for (int it = c * chunk; it < std::min(max_iters, (c + 1) * chunk); ++it) {
+                        int itv = it, sid = solve_id;
+                        double rt = rel2;
+                        void *args[] = {&tab, &items, &itv, &rt, &dm, &sid};
+                        const void *fn = kb_cgs4_kernel(it == 0, it % kCgsCycle,
+                                                        it == 0 ? gr.cls_first : gr.cls);
+                        HIPT(hipLaunchKernel(fn, dim3(gr.nitems), dim3(kThreads), args, 0, st_));
+                    }
+                    if (!graph) HIPT(hipEventRecord(chunk_events_[c], st_));
+                }
+            }
+            // a cap that ends mid-cycle leaves directions unbuilt and p terms
+            // pending (the residuals launch j read sit in ring[j % K])
+            if (max_iters > 0 && max_iters % kCgsCycle != 0) {
+                const int last = max_iters - 1, j0 = last - last % kCgsCycle;
+                for (auto &in : inst_) {
+                    CgRing pend{};
+                    for (int q = 0; q <= last % kCgsCycle; ++q) {
+                        const int slot = (j0 + q) % kCgsCycle;
+                        pend.r[q] = slot == 0 ? in->r.p : in->ring[slot - 1].p;
+                    }
+                    const long n4 = (long)(in->n / 4);
+                    const unsigned nbt =
+                        (unsigned)std::min<long>((n4 + kThreads - 1) / kThreads, 4096);
+                    cgs4_tail_launch(nbt, st_, in->st.p, last, pend,
+                                     reinterpret_cast<const float4 *>(in->dd[0].p),
+                                     reinterpret_cast<float4 *>(in->p.p), n4);
+                    LAUNCH_CHECK();
+                }
+            }
+            kb_cg_finalize<T><<<count(), blk, 0, st_>>>(table_.p, max_iters, rel2, step, 2,
+                                                        graph ? step_dev_.p : nullptr);
+            LAUNCH_CHECK();
+        } else {
+            raise(AETHER_EINVAL, "AETHER_CG_FUSED for batches needs FP32");
+        }
     }
 
     // src/fluid.rs:598-611 for every instance at once.  graph: the launches
@@ -594,6 +773,7 @@ class Batch final : public BatchBase {
             if (in->tel.n < (size_t)nsteps * 8) in->tel.alloc((size_t)nsteps * 8);
             if (in->iters.n < (size_t)nsteps) in->iters.alloc(nsteps);
         }
+        if (form() == AETHER_CG_FUSED) ensure_fused();
         write_table();
         if (params_.batch_graph)
             run_graphs(dt, times, nsteps);
@@ -647,6 +827,16 @@ class Batch final : public BatchBase {
     int solve_seq_ = 0;
     std::vector<hipEvent_t> chunk_events_;
     std::vector<int> iters_;
+    struct Group {
+        std::vector<int> insts;
+        size_t cells = 0;
+        int cls_first = 0, cls = 0;  // kb_cgs4_class of the first launch and of the others
+        bool open = true;            // still taking instances
+        DevBuf<BItem> items;
+        int nitems = 0;
+    };
+    std::deque<Group> groups_;  // (a deque: groups hold device buffers and never move)
+    bool fused_ready_ = false;
 };
 
 }  // namespace

@@ -9,6 +9,8 @@
 // partial sums and stop flag: a converged instance's tiles return at once
 // while the others keep iterating.
 #pragma once
+#include "cg_cgs4.hpp"
+#include "cg_fused.hpp"
 #include "cg_vec.hpp"
 
 namespace aether {
@@ -39,6 +41,15 @@ struct BInst {
     VTiling vt;                           // four-wide CG tiling (FP32, nx % 4 == 0)
     int nbv;                              // its tile count (= partial sums)
     int pad2;
+    // one-launch CG (FP32, nx % 4 == 0): the residual ring (ring[0] = r), the
+    // ping-pong pair of w = A r, the delta partials (ping-pong with pe/pf as
+    // gamma with pb/pa), and its tiling (four z planes per thread)
+    T *ring[kCgRingMax];
+    T *w[2];
+    double *pe, *pf;
+    VTiling vt4;
+    int nb4;
+    int pad3;
 };
 
 struct Slots {
@@ -218,6 +229,80 @@ __global__ __launch_bounds__(kThreads) void kb_cg_update4(const BInst<float> *__
                                       in.d[(iter + 1) & 1]);
 }
 
+// delta_0 = r_0 . A r_0 partials and w_0 = A r_0 of every instance (as
+// k_cg_rar on the initialisation tiling).
+template <typename T, int ZU>
+__global__ __launch_bounds__(kThreads) void kb_cg_rar(const BInst<T> *__restrict__ I,
+                                                      const BItem *__restrict__ items) {
+    const BItem it = items[blockIdx.x];
+    const BInst<T> &in = I[it.inst];
+    body_cg_rar<T, ZU>(col_at<ZU>(in.g, it.bx, it.by, it.bz), it.local, in.g, in.r, in.pe,
+                       in.w[0]);
+}
+
+// One-launch CG iteration `iter` (cg_cgs4.hpp) over the items of a group of
+// instances whose tilings share the launch's class RG and partial-sum width
+// NQ: the same body, buffers and partial-sum slots as the single-grid
+// launch, so each instance computes bit for bit what it computes alone.  A
+// converged instance's tiles return before loading anything (its stop word
+// holds the stopping launch + 1; the stopping launch itself never sees a
+// value at or below its own index).
+template <bool FIRST, int NQ, int PH, int RG>
+__global__ __launch_bounds__(kThreads) void kb_cg_cgs4(const BInst<float> *__restrict__ I,
+                                                       const BItem *__restrict__ items,
+                                                       int iter, double rel_tol2,
+                                                       int *done_mirror, int solve_id) {
+    const BItem it = items[blockIdx.x];
+    const BInst<float> &in = I[it.inst];
+    if (!FIRST) {
+        const int done = in.st->done;
+        if (done != 0 && done <= iter) return;
+    }
+    VCol c;
+    if constexpr (RG != 0)
+        c = vcol_regular_at<4>(in.g, in.vt4, it.by, it.bz);
+    else
+        c = vcol_at<4>(in.g, in.vt4, it.bx, it.by, it.bz);
+    CgRing ring;
+#pragma unroll
+    for (int q = 0; q < kCgRingMax; ++q) ring.r[q] = in.ring[q];
+    const bool odd = iter & 1;
+    body_cg_cgs4<4, FIRST, NQ, kStWT, kCgsCycle, PH, RG>(
+        c, it.local, in.g, iter, rel_tol2, FIRST ? in.nb : in.nb4, odd ? in.pa : in.pb,
+        odd ? in.pf : in.pe, in.pc, odd ? in.pb : in.pa, odd ? in.pe : in.pf, in.st, ring,
+        in.w[iter & 1], in.w[(iter + 1) & 1], in.d[0], in.p, done_mirror + it.inst, solve_id);
+}
+
+template <int NQ, int RG, int... PH>
+inline const void *kb_cgs4_phase(int ph, std::integer_sequence<int, PH...>) {
+    const void *fns[] = {(const void *)kb_cg_cgs4<false, NQ, PH, RG>...};
+    return fns[ph];
+}
+
+template <int NQ, int RG>
+inline const void *kb_cgs4_pick(bool first, int ph) {
+    if (first) return (const void *)kb_cg_cgs4<true, NQ, 0, RG>;
+    return kb_cgs4_phase<NQ, RG>(ph, std::make_integer_sequence<int, kCgsCycle>{});
+}
+
+// The launch's class as the single grid chooses it (cgs4_kernel with
+// write-through stores): nparts partial sums, tiling class rg.
+inline int kb_cgs4_class(int nparts, int rg) {
+    if (rg != 0 && nparts <= 4 * kThreads) return 10 * rg + (nparts <= 2 * kThreads ? 2 : 4);
+    return nparts > 4 * kThreads ? 16 : 4;
+}
+
+inline const void *kb_cgs4_kernel(bool first, int ph, int cls) {
+    switch (cls) {
+    case 12: return kb_cgs4_pick<2, 1>(first, ph);
+    case 14: return kb_cgs4_pick<4, 1>(first, ph);
+    case 22: return kb_cgs4_pick<2, 2>(first, ph);
+    case 24: return kb_cgs4_pick<4, 2>(first, ph);
+    case 16: return kb_cgs4_pick<16, 0>(first, ph);
+    default: return kb_cgs4_pick<4, 0>(first, ph);
+    }
+}
+
 // Closes, per instance, a four-wide solve that an odd iteration cap ended
 // after an even launch: p += alpha_last d_last over the instance's tiles.
 template <int VZ>
@@ -243,19 +328,23 @@ __global__ __launch_bounds__(kThreads) void kb_cg_p_tail(const BInst<float> *__r
 
 // One workgroup per instance: closes solves that ran out of iterations and
 // logs the iteration count of step `step` (of *step_dev when it is given: a
-// step captured in a graph reads its index from the device).  x4: the solve
-// ran the four-wide launches (their r.r partials are nbv wide).
+// step captured in a graph reads its index from the device).  form: the
+// solve's launches, 0 one cell per thread, 1 four-wide (its r.r partials are
+// nbv wide), 2 one-launch (nb4 wide, in pa after an odd count of launches).
 template <typename T>
 __global__ __launch_bounds__(kThreads) void kb_cg_finalize(const BInst<T> *__restrict__ I,
                                                            int max_iters, double rel_tol2,
-                                                           int step, int x4,
+                                                           int step, int form,
                                                            const int *__restrict__ step_dev) {
     const BInst<T> &in = I[blockIdx.x];
     __shared__ double lds[8];
     __shared__ int flag_lds[1];
     CgState *st = in.st;
     if (!block_flag(&st->done, flag_lds)) {
-        const double rr = sum_partials(in.pb, (x4 && max_iters > 0) ? in.nbv : in.nb, lds);
+        const bool ran = max_iters > 0;
+        const double *prr = (form == 2 && ran && (max_iters & 1)) ? in.pa : in.pb;
+        const int nrr = !ran ? in.nb : form == 1 ? in.nbv : form == 2 ? in.nb4 : in.nb;
+        const double rr = sum_partials(prr, nrr, lds);
         double bb = 0.0;
         if (max_iters == 0) bb = sum_partials(in.pc, in.nb, lds);
         if (tid_lin() == 0) {

@@ -125,26 +125,20 @@ __device__ __forceinline__ void cgs4_pending(const CgRing &ring, int it, int oc,
 // with two rows per wavefront (32 float4 lanes per row), where each lane takes
 // the y-neighbour row inside its wavefront from the other half of the wave and
 // loads only the row outside it.
+// The work of one workgroup on the tile `c`; `slot` is its partial-sum slot
+// (0 leads: it publishes the CG scalars).
 template <int ZU, bool FIRST, int NQ, int ST, int K, int PH, int RG>
-__global__ __launch_bounds__(kThreads) void k_cg_cgs4(
-    Dims g, VTiling vt, int it, double rel_tol2, int nb_in, const double *__restrict__ pg_in,
-    const double *__restrict__ pd_in, const double *__restrict__ pbb, double *__restrict__ pg_out,
-    double *__restrict__ pd_out, CgState *st, CgRing ring, const float *__restrict__ w_in,
-    float *__restrict__ w_out, float *__restrict__ d, float *__restrict__ p, int *done_mirror,
-    int solve_id) {
+__device__ __forceinline__ void body_cg_cgs4(
+    const VCol &c, int slot, const Dims &g, int it, double rel_tol2, int nb_in,
+    const double *__restrict__ pg_in, const double *__restrict__ pd_in,
+    const double *__restrict__ pbb, double *__restrict__ pg_out, double *__restrict__ pd_out,
+    CgState *st, const CgRing &ring, const float *__restrict__ w_in, float *__restrict__ w_out,
+    float *__restrict__ d, float *__restrict__ p, int *done_mirror, int solve_id) {
     constexpr bool REG = RG != 0;
     // K >= 3: the array a launch overwrites is neither its r_it nor its r_(it-1)
     static_assert(K >= 3 && K <= kCgRingMax && PH >= 0 && PH < K && (!FIRST || PH == 0),
                   "one-launch CG cycle");
     constexpr int kMask = kCgRingMax - 1;
-    VCol c;
-    if constexpr (REG) {
-        c = vcol_regular<ZU>(g, vt);
-    } else {
-        int bx, by, bz;
-        vtile_coords(vt, bx, by, bz);
-        c = vcol_at<ZU>(g, vt, bx, by, bz);
-    }
     // separate slots for each reduction, so none needs a trailing barrier
     __shared__ double lds[8], lds_b[8], lds_out[8];
     __shared__ int flag_lds[1];
@@ -230,7 +224,7 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
     block_sum<2, false>(tot, lds);  // its barrier publishes flag_lds
     if (flag_lds[0]) return;
     const double gam = tot[0], del = tot[1];
-    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    const bool lead = slot == 0 && threadIdx.x == 0;
     double beta = 0.0, mu = del, tcd = 0.0;  // tcd: beta / alpha_(it-1)
     bool stop;
     if (FIRST) {
@@ -418,9 +412,29 @@ __global__ __launch_bounds__(kThreads) void k_cg_cgs4(
     }
     block_sum<2, false>(acc, lds_out);
     if (threadIdx.x == 0) {
-        pg_out[blockIdx.x] = acc[0];
-        pd_out[blockIdx.x] = acc[1];
+        pg_out[slot] = acc[0];
+        pd_out[slot] = acc[1];
     }
+}
+
+template <int ZU, bool FIRST, int NQ, int ST, int K, int PH, int RG>
+__global__ __launch_bounds__(kThreads) void k_cg_cgs4(
+    Dims g, VTiling vt, int it, double rel_tol2, int nb_in, const double *__restrict__ pg_in,
+    const double *__restrict__ pd_in, const double *__restrict__ pbb, double *__restrict__ pg_out,
+    double *__restrict__ pd_out, CgState *st, CgRing ring, const float *__restrict__ w_in,
+    float *__restrict__ w_out, float *__restrict__ d, float *__restrict__ p, int *done_mirror,
+    int solve_id) {
+    VCol c;
+    if constexpr (RG != 0) {
+        c = vcol_regular<ZU>(g, vt);
+    } else {
+        int bx, by, bz;
+        vtile_coords(vt, bx, by, bz);
+        c = vcol_at<ZU>(g, vt, bx, by, bz);
+    }
+    body_cg_cgs4<ZU, FIRST, NQ, ST, K, PH, RG>(c, blockIdx.x, g, it, rel_tol2, nb_in, pg_in,
+                                              pd_in, pbb, pg_out, pd_out, st, ring, w_in, w_out,
+                                              d, p, done_mirror, solve_id);
 }
 
 // Closes a solve whose iteration cap ended it after launch `last_it` of

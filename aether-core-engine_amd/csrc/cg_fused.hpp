@@ -45,11 +45,10 @@ constexpr int kFusedPF = 4;   // planes of loads in flight ahead of their use
 // half of the first iteration's reduction; gamma_0 comes with r_0).  When
 // w_out is set, A r_0 is also stored there (0 off the interior).
 template <typename T, int ZU>
-__global__ __launch_bounds__(kThreads) void k_cg_rar(Dims g, Tiling tl, const T *__restrict__ r,
-                                                     double *__restrict__ part,
-                                                     T *__restrict__ w_out) {
+__device__ __forceinline__ void body_cg_rar(const Col &c, int slot, const Dims &g,
+                                            const T *__restrict__ r, double *__restrict__ part,
+                                            T *__restrict__ w_out) {
     __shared__ double lds[8];
-    const Col c = col_of<ZU>(g, tl);
     T rz[ZU + 2], rn[ZU][4];
 #pragma unroll
     for (int a = 0; a < ZU + 2; ++a) rz[a] = r[col_id(g, c, c.k0 - 1 + a)];
@@ -75,7 +74,14 @@ __global__ __launch_bounds__(kThreads) void k_cg_rar(Dims g, Tiling tl, const T 
         if (w_out) w_out[col_id(g, c, k)] = q;
     }
     block_sum<1>(acc, lds);
-    if (tid_lin() == 0) part[blockIdx.x] = acc[0];
+    if (tid_lin() == 0) part[slot] = acc[0];
+}
+
+template <typename T, int ZU>
+__global__ __launch_bounds__(kThreads) void k_cg_rar(Dims g, Tiling tl, const T *__restrict__ r,
+                                                     double *__restrict__ part,
+                                                     T *__restrict__ w_out) {
+    body_cg_rar<T, ZU>(col_of<ZU>(g, tl), blockIdx.x, g, r, part, w_out);
 }
 
 // Iteration `it` of the single-reduction CG.  pg_in/pd_in: partials of

@@ -78,15 +78,15 @@ inline bool vtiling_regular(const VTiling &t, int ny, int nz, int zu) {
            t.txv * t.ty == 256 && ny % t.ty == 0 && nz % zu == 0;
 }
 
+// the tile (by, bz) of a regular tiling (its bx is 0)
 template <int ZU>
-__device__ __forceinline__ VCol vcol_regular(const Dims &g, const VTiling &vt) {
-    int b = blockIdx.x;
-    if (vt.remap) b = (b & 7) * (vt.nb >> 3) + (b >> 3);
+__device__ __forceinline__ VCol vcol_regular_at(const Dims &g, const VTiling &vt, int by,
+                                                int bz) {
     const int t = threadIdx.x, sh = __builtin_ctz(vt.txv);
     VCol c;
     c.i0 = 4 * (t & (vt.txv - 1));
-    c.j = (b % vt.gy) * vt.ty + (t >> sh);
-    c.k0 = (b / vt.gy) * ZU;
+    c.j = by * vt.ty + (t >> sh);
+    c.k0 = bz * ZU;
     c.valid = true;
     c.off = c.j * g.nx + c.i0;
     c.offym = max(c.j - 1, 0) * g.nx + c.i0;
@@ -94,6 +94,14 @@ __device__ __forceinline__ VCol vcol_regular(const Dims &g, const VTiling &vt) {
     c.lane = t & 63;
     c.needL = c.needR = false;
     return c;
+}
+
+// the tile of this workgroup (renumbered per XCD slab as vtile_coords)
+template <int ZU>
+__device__ __forceinline__ VCol vcol_regular(const Dims &g, const VTiling &vt) {
+    int b = blockIdx.x;
+    if (vt.remap) b = (b & 7) * (vt.nb >> 3) + (b >> 3);
+    return vcol_regular_at<ZU>(g, vt, b % vt.gy, b / vt.gy);
 }
 
 // Output array of a four-wide kernel under one store policy.  ST 0: plain

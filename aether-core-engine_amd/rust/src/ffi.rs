@@ -8,7 +8,7 @@
 use std::os::raw::{c_char, c_int, c_long, c_void};
 
 /// `AETHER_FLUID_ABI_VERSION` of include/aether_fluid.h that the structs below follow.
-pub const AETHER_FLUID_ABI_VERSION: c_int = 2;
+pub const AETHER_FLUID_ABI_VERSION: c_int = 3;
 
 pub const AETHER_OK: c_int = 0;
 pub const AETHER_EINVAL: c_int = -1;

@@ -361,7 +361,7 @@ def bench_batched(steps: int, warmup: int, dt: float = 0.01) -> dict:
     times = [dt * i for i in range(warmup + steps)]
     b = af.FluidBatch(insts, solver_iters=200, cg_rel_tol=1e-5)
     tune_b = b.autotune_cg(20)  # CG layout of the batch, chosen by timing
-    form_b = af.CG_X4 if tune_b["choice"] == "x4" else af.CG_SCALAR
+    form_b = {"x4": af.CG_X4, "scalar": af.CG_SCALAR, "fused": af.CG_FUSED}[tune_b["choice"]]
     if warmup:
         b.run(dt, times[:warmup])
     t0 = time.perf_counter()

@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define AETHER_FLUID_ABI_VERSION 2
+#define AETHER_FLUID_ABI_VERSION 3
 
 /* status codes */
 #define AETHER_OK 0
@@ -52,7 +52,9 @@ extern "C" {
                                 aether_fluid_autotune_cg choice */
 #define AETHER_CG_SCALAR 1   /* two launches per iteration, one cell per thread */
 #define AETHER_CG_X4 2       /* two launches per iteration, four x-cells per thread (FP32, nx % 4 == 0) */
-#define AETHER_CG_FUSED 3    /* one launch per iteration, single-reduction form (FP32) */
+#define AETHER_CG_FUSED 3    /* one launch per iteration, single-reduction form (FP32);
+                                batches (every nx % 4 == 0): solved group by group, each
+                                group at most 3.2 M cells */
 
 /* source kinds */
 #define AETHER_SOURCE_JETS 0   /* src/fluid.rs:532-595 inject_cyberpunk_sources */
@@ -100,7 +102,8 @@ typedef struct {
     int32_t cg_z_cells;             /* z cells per thread of the CG tiling: 0 = the
                                        layout's default or the autotuner's choice; the
                                        four-wide two-launch layout takes 2 or 4, the
-                                       one-cell-per-thread layout 4, 8 or 16 (batches: 8) */
+                                       one-cell-per-thread layout 4, 8 or 16 (batches: 8),
+                                       the batches' one-launch layout 4 */
     int32_t reserved;
 } aether_fluid_params;
 
@@ -321,12 +324,14 @@ int aether_fluid_batch_run(aether_fluid_batch_t b, double dt, const double *time
 int aether_fluid_batch_iterations(aether_fluid_batch_t b, int32_t *out, int n, int *count);
 /* CG layout and tile autotuner of a batch: times `iters` CG iterations of
  * the whole batch in the four-wide layout (FP32, every nx % 4 == 0) with 2
- * and with 4 z planes per thread, and in the one-cell-per-thread layout (8 z
- * cells per thread), on synthetic right-hand sides (every instance's
+ * and with 4 z planes per thread, in the one-cell-per-thread layout (8 z
+ * cells per thread), and in the one-launch layout solved group by group
+ * (FP32, every nx % 4 == 0), on synthetic right-hand sides (every instance's
  * pressure and divergence are saved and restored), and keeps the fastest as
  * the batch's choice for cg_form AETHER_CG_AUTO and cg_z_cells 0.
- * us_per_iter[3] (may be NULL): four-wide z2, four-wide z4, one cell per
- * thread (-1 when not applicable).  *form / *z_cells: the choice. */
+ * us_per_iter[4] (may be NULL): four-wide z2, four-wide z4, one cell per
+ * thread, one launch (-1 when not applicable).  *form / *z_cells: the
+ * choice. */
 int aether_fluid_batch_autotune_cg(aether_fluid_batch_t b, int iters, double *us_per_iter,
                                    int *form, int *z_cells);
 

@@ -35,7 +35,7 @@ def test_header_symbols_exported_and_bound():
 
 def test_abi_version():
     import aether_fluid as af
-    assert af.lib().aether_fluid_abi_version() == 2 == af._lib.ABI_VERSION
+    assert af.lib().aether_fluid_abi_version() == 3 == af._lib.ABI_VERSION
     header = int(re.search(r"#define AETHER_FLUID_ABI_VERSION (\d+)", open(HEADER).read()).group(1))
     ffi = open(os.path.join(ROOT, "aether-core-engine_amd", "rust", "src", "ffi.rs")).read()
     rust = int(re.search(r"pub const AETHER_FLUID_ABI_VERSION: c_int = (\d+);", ffi).group(1))

@@ -1014,8 +1014,8 @@ def test_batch_autotune_keeps_state_and_results():
     before = [bs[0].download(q, n) for q in range(2) for n in ("pressure", "divergence")]
     r = bs[0].autotune_cg(10)
     print("batch autotune", r)
-    assert set(r["us_per_iter"]) == {"x4_z2", "x4_z4", "scalar_z8"}
-    assert (r["choice"], r["z_cells"]) in (("x4", 2), ("x4", 4), ("scalar", 8))
+    assert set(r["us_per_iter"]) == {"x4_z2", "x4_z4", "scalar_z8", "fused_z4"}
+    assert (r["choice"], r["z_cells"]) in (("x4", 2), ("x4", 4), ("scalar", 8), ("fused", 4))
     after = [bs[0].download(q, n) for q in range(2) for n in ("pressure", "divergence")]
     for x, y in zip(before, after):
         np.testing.assert_array_equal(x, y)
@@ -1063,15 +1063,16 @@ def test_batched_graph_replay_equals_stream_launches():
         np.testing.assert_array_equal(x, y)
 
 
-def _batched_vs_single(insts, single_form, case, cap=120, tol=1e-5, z_cells=0):
+def _batched_vs_single(insts, single_form, case, cap=120, tol=1e-5, z_cells=0, batch_form=0):
     """A batch of grids of different sizes gives, bit for bit, the fields and
     CG iteration counts of single-grid solvers run one at a time with the
     same CG layout and tiling (z_cells; 8 z cells per thread in the
-    initialisation tiling)."""
+    initialisation tiling).  batch_form: the batch's cg_form (0: AUTO)."""
     _announce(case)
     import os
     import aether_fluid as af
-    b = af.FluidBatch(insts, solver_iters=cap, cg_rel_tol=tol, cg_z_cells=z_cells)
+    b = af.FluidBatch(insts, solver_iters=cap, cg_rel_tol=tol, cg_z_cells=z_cells,
+                      cg_form=batch_form)
     states = []
     for q, d in enumerate(insts):
         st = random_state(d["nx"], d["ny"], d["nz"], seed=40 + q, vel=0.5)
@@ -1160,16 +1161,46 @@ def test_batched_four_wide_two_planes_equals_one_at_a_time_bitwise():
     b.close()
 
 
-def test_batched_64_workload_equals_one_at_a_time_bitwise():
+@pytest.mark.parametrize("form", ["x4", "fused"])
+def test_batched_64_workload_equals_one_at_a_time_bitwise(form):
     """The batched workload's own 64 seeded grids (sizes from 32^3, 48^3,
     64^3, 96^3 and 128x96x128; 31.3 M voxels; seeded sources and forcing),
-    three steps with CG to relative tolerance 1e-5: the batch (four-wide CG)
-    equals the grids run one at a time with the same layout, bit for bit."""
+    three steps with CG to relative tolerance 1e-5: the batch (four-wide CG,
+    or the one-launch CG solved group by group) equals the grids run one at
+    a time with the same layout, bit for bit."""
     import aether_fluid as af
     insts = af.batched_instances(64)
     assert sum(d["nx"] * d["ny"] * d["nz"] for d in insts) > 30_000_000
-    _batched_vs_single(insts, af.CG_X4, "batched vs one-at-a-time, the 64-grid workload",
-                       cap=200, tol=1e-5)
+    f = af.CG_X4 if form == "x4" else af.CG_FUSED
+    _batched_vs_single(insts, f, f"batched vs one-at-a-time, the 64-grid workload ({form})",
+                       cap=200, tol=1e-5, batch_form=af.CG_AUTO if form == "x4" else f)
+
+
+def test_batched_one_launch_equals_one_at_a_time_bitwise():
+    """The one-launch CG for batches (cg_form CG_FUSED), solved group by
+    group: instances of every tiling class (two 128-cell rows per wavefront,
+    regular tilings of 32 and 64 cells per row, irregular rows of 6 and 17
+    float4 lanes) equal single grids pinned to the one-launch layout, bit for
+    bit, with a tolerance that stops instances of one group at different
+    launches (the converged ones return early from the rest), and with a cap
+    of 13 that ends every solve mid-cycle (the per-instance tail)."""
+    import aether_fluid as af
+    insts = [dict(nx=128, ny=16, nz=16, vorticity_eps=0.3, buoyancy_beta=1.0, gravity_alpha=0.05,
+                  source_center=(64.0, 4.0, 8.0), source_radius=3.0),
+             dict(nx=32, ny=32, nz=32, vorticity_eps=0.3, buoyancy_beta=1.0, gravity_alpha=0.05,
+                  source_center=(16.0, 4.0, 16.0), source_radius=3.5),
+             dict(nx=24, ny=40, nz=20, vorticity_eps=0.2, buoyancy_beta=1.0, gravity_alpha=0.08,
+                  source_center=(12.0, 5.0, 10.0), source_radius=3.0),
+             dict(nx=68, ny=20, nz=18, vorticity_eps=0.45, buoyancy_beta=1.0, gravity_alpha=0.03,
+                  source_center=(30.0, 3.0, 9.0), source_radius=2.5),
+             dict(nx=64, ny=16, nz=24, vorticity_eps=0.1, buoyancy_beta=1.0, gravity_alpha=0.1,
+                  source_center=(32.0, 3.0, 12.0), source_radius=2.0),
+             dict(nx=32, ny=32, nz=16, vorticity_eps=0.25, buoyancy_beta=1.0, gravity_alpha=0.06,
+                  source_center=(16.0, 4.0, 8.0), source_radius=2.5)]
+    _batched_vs_single(insts, af.CG_FUSED, "batched one-launch CG vs one-at-a-time",
+                       batch_form=af.CG_FUSED)
+    _batched_vs_single(insts, af.CG_FUSED, "batched one-launch CG vs one-at-a-time (cap 13)",
+                       cap=13, tol=0.0, batch_form=af.CG_FUSED)
 
 
 def _rollout_setup(n, precision, tol=1e-12, iters=600):
