@@ -169,7 +169,10 @@ class Batch final : public BatchBase {
                                 hipMemcpyHostToDevice, st_));
         }
         table_.alloc(count);
-        HIPT(hipHostMalloc((void **)&done_host_, sizeof(int) * count, hipHostMallocMapped));
+        // per instance: id of the last solve a CG launch stopped; coherent, so
+        // the device's writes are seen by the host's pacing polls without a flush
+        HIPT(hipHostMalloc((void **)&done_host_, sizeof(int) * count,
+                           hipHostMallocMapped | hipHostMallocCoherent));
         for (int q = 0; q < count; ++q) done_host_[q] = 0;
         HIPT(hipHostGetDevicePointer((void **)&done_dev_, done_host_, 0));
         HIPT(hipStreamSynchronize(st_));
