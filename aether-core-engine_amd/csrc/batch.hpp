@@ -278,6 +278,7 @@ class Batch final : public BatchBase {
         }
         HIPT(hipMemcpyAsync(table_.p, tab.data(), tab.size() * sizeof(BInst<T>),
                             hipMemcpyHostToDevice, st_));
+        HIPT(hipStreamSynchronize(st_));  // tab is pageable and ends with this scope
     }
 
     void advect1(int f, double dt, double diss) {
@@ -371,8 +372,9 @@ class Batch final : public BatchBase {
             gp->insts.push_back(q);
             gp->cells += in.n;
         }
+        std::deque<std::vector<BItem>> host;  // kept until the copies below have completed
         for (Group &gr : groups_) {
-            std::vector<BItem> items;
+            std::vector<BItem> &items = host.emplace_back();
             for (int q : gr.insts) {
                 const VTiling &v = inst_[q]->vtz[1];
                 for (int b = 0; b < v.nb; ++b) {  // the single-grid launch order

@@ -350,18 +350,45 @@ def bench_rollout(n: int, steps: int, ckpt: int, ndirs: int = 0, reps: int = 2,
     return out
 
 
-def bench_batched(steps: int, warmup: int, dt: float = 0.01) -> dict:
+def _batched_graph(af, insts, form, z, dt, times, warmup, steps, nvox, rho_ref, it_ref):
+    """The batch with every step captured into a HIP graph and replayed,
+    checked bit for bit against the stream-launched run (rho_ref, it_ref)."""
+    bg = af.FluidBatch(insts, solver_iters=200, cg_rel_tol=1e-5, graph=True, cg_form=form,
+                       cg_z_cells=z)
+    if warmup:
+        bg.run(dt, times[:warmup])
+    t0 = time.perf_counter()
+    bg.run(dt, times[warmup:])
+    el_g = time.perf_counter() - t0
+    same = bool(np.array_equal(bg.download(len(insts) - 1, "density"), rho_ref) and
+                np.array_equal(bg.iterations(), it_ref))
+    bg.close()
+    return {"voxel_steps_per_s": nvox * steps / el_g, "ms_per_step": el_g * 1e3 / steps,
+            "bitwise_equal_to_stream_launches": same,
+            "note": "includes capturing two step graphs per run"}
+
+
+def bench_batched(steps: int, warmup: int, dt: float = 0.01, cg: str = "auto",
+                  graph: bool = True) -> dict:
     """64 grids drawn from {32^3, 48^3, 64^3, 96^3, 128x96x128}: one launch
     per stage for all of them, against the same grids stepped one at a time
     with the single-grid solver.  FP32, CG rel tol 1e-5 (max 200), zero
-    initial state, sphere sources in the bottom quarter."""
+    initial state, sphere sources in the bottom quarter.  cg: the batch's CG
+    form, "auto" for the batch autotuner's choice, else x4, scalar or fused
+    (the one-launch form by groups).  graph=False skips the HIP-graph replay."""
     import aether_fluid as af
     insts = af.batched_instances(64)
     nvox = sum(d["nx"] * d["ny"] * d["nz"] for d in insts)
     times = [dt * i for i in range(warmup + steps)]
     b = af.FluidBatch(insts, solver_iters=200, cg_rel_tol=1e-5)
     tune_b = b.autotune_cg(20)  # CG layout of the batch, chosen by timing
-    form_b = {"x4": af.CG_X4, "scalar": af.CG_SCALAR, "fused": af.CG_FUSED}[tune_b["choice"]]
+    forms = {"x4": af.CG_X4, "scalar": af.CG_SCALAR, "fused": af.CG_FUSED}
+    form_b = forms[tune_b["choice"] if cg == "auto" else cg]
+    z_b = tune_b["z_cells"] if cg == "auto" else (4 if form_b != af.CG_SCALAR else 0)
+    if cg != "auto":
+        b.close()
+        b = af.FluidBatch(insts, solver_iters=200, cg_rel_tol=1e-5, cg_form=form_b,
+                          cg_z_cells=z_b)
     if warmup:
         b.run(dt, times[:warmup])
     t0 = time.perf_counter()
@@ -370,17 +397,10 @@ def bench_batched(steps: int, warmup: int, dt: float = 0.01) -> dict:
     it_b = b.iterations()
     rho_b = b.download(len(insts) - 1, "density")
     b.close()
-    # the same batch with every step captured into a HIP graph and replayed
-    bg = af.FluidBatch(insts, solver_iters=200, cg_rel_tol=1e-5, graph=True, cg_form=form_b,
-                       cg_z_cells=tune_b["z_cells"])
-    if warmup:
-        bg.run(dt, times[:warmup])
-    t0 = time.perf_counter()
-    bg.run(dt, times[warmup:])
-    el_g = time.perf_counter() - t0
-    same = bool(np.array_equal(bg.download(len(insts) - 1, "density"), rho_b) and
-                np.array_equal(bg.iterations(), it_b))
-    bg.close()
+    out_graph = None
+    if graph:  # the same batch with every step captured into a HIP graph and replayed
+        out_graph = _batched_graph(af, insts, form_b, z_b, dt, times, warmup, steps, nvox,
+                                   rho_b, it_b)
     el_1 = 0.0
     it_1 = []
     single_choices = {}
@@ -403,11 +423,9 @@ def bench_batched(steps: int, warmup: int, dt: float = 0.01) -> dict:
     return {"workload": "64 batched smoke grids, FP32, CG rel tol 1e-5", "instances": 64,
             "total_voxels": nvox, "steps": steps, "warmup": warmup,
             "batched": {"voxel_steps_per_s": nvox * steps / el_b, "ms_per_step": el_b * 1e3 / steps,
-                        "cg_iters_mean": float(np.mean(it_b)), "autotune": tune_b},
-            "batched_graph": {"voxel_steps_per_s": nvox * steps / el_g,
-                              "ms_per_step": el_g * 1e3 / steps,
-                              "bitwise_equal_to_stream_launches": same,
-                              "note": "includes capturing two step graphs per run"},
+                        "cg_iters_mean": float(np.mean(it_b)), "autotune": tune_b,
+                        "cg_form": {v: k for k, v in forms.items()}[form_b]},
+            "batched_graph": out_graph,
             "one_at_a_time": {"voxel_steps_per_s": nvox * steps / el_1,
                               "ms_per_step": el_1 * 1e3 / steps,
                               "cg_iters_mean": float(np.mean(it_1)),
@@ -508,6 +526,10 @@ def main():
                     help="plain_128: the headline line; mixed_256: mixed-precision "
                          "projection at 256^3 against the all-FP64 projection; medium_128: "
                          "FluidMedium sampling of the 128^3 state after --steps steps")
+    ap.add_argument("--batch-cg", default="auto", choices=["auto", "x4", "scalar", "fused"],
+                    help="batched_64: the batch's CG form (auto: the batch autotuner's choice)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="batched_64: skip the HIP-graph replay (stream launches only)")
     ap.add_argument("--probe", action="store_true",
                     help="counter-collection mode: 2 steps of the workload with each CG "
                          "layout and with RB-SOR, and a 256 MiB stream copy; no timing output")
@@ -524,7 +546,8 @@ def main():
         print(json.dumps(bench_rollout(64, 40, 8, ndirs=16)), flush=True)
         return
     if args.workload == "batched_64":
-        print(json.dumps(bench_batched(args.steps, args.warmup)), flush=True)
+        print(json.dumps(bench_batched(args.steps, args.warmup, cg=args.batch_cg,
+                                       graph=not args.no_graph)), flush=True)
         return
     if args.workload == "mixed_256":
         print(json.dumps(bench_mixed(args.steps, args.warmup)), flush=True)
