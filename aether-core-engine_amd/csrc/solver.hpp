@@ -6,6 +6,7 @@
 #include "medium.hpp"
 #include "cg_fused.hpp"
 #include "cg_vec.hpp"
+#include "cg_vec2.hpp"
 #include "cg_cgs4.hpp"
 #include "adjoint_kernels.hpp"
 
@@ -14,6 +15,9 @@ namespace {
 
 template <typename T>
 class Solver final : public SolverBase {
+    // x-cells per thread of the 16-byte-lane CG layout (AETHER_CG_X4)
+    static constexpr int kLaneCells = sizeof(T) == 4 ? 4 : 2;
+
    public:
     Solver(int nx, int ny, int nz, double h, hipStream_t stream, bool ir64 = false)
         : h_(h), st_(stream), ir64_(ir64) {
@@ -52,10 +56,11 @@ class Solver final : public SolverBase {
         tl_.gz = (nz + zu_ - 1) / zu_;
         tl_.nb = gx * gy * tl_.gz;
         tl_.remap = (tl_.nb % 8 == 0) ? 1 : 0;
-        // four-wide two-launch CG: 256-thread groups of txv float4 lanes x ty
-        // rows x vzu planes (vzu = 4 measured faster than 2 at 128^3 and 256^3)
-        if (nx % 4 == 0) {
-            const int w4 = nx / 4;
+        // two-launch CG with 16-byte lanes (four FP32 or two FP64 x-cells per
+        // thread): 256-thread groups of txv lanes x ty rows x vzu planes (vzu
+        // = 4 measured faster than 2 at 128^3 and 256^3 in FP32)
+        if (nx % kLaneCells == 0) {
+            const int w4 = nx / kLaneCells;
             tlv_.txv = std::min(w4, 64);
             tlv_.ty = kThreads / tlv_.txv;
             tlv_.gx = (w4 + tlv_.txv - 1) / tlv_.txv;
@@ -65,9 +70,9 @@ class Solver final : public SolverBase {
             tlv_.nb = tlv_.gx * tlv_.gy * tlv_.gz;
             tlv_.remap = (tlv_.nb % 8 == 0) ? 1 : 0;
         }
-        // one-launch SOR sweep (k_rbsor4): the four-wide tiling with 2 planes
-        // per thread (0.79 ms per 128^3 RB-SOR step against 0.87 with 4)
-        if (nx % 4 == 0) {
+        // one-launch SOR sweep (k_rbsor4, FP32): the four-wide tiling with 2
+        // planes per thread (0.79 ms per 128^3 RB-SOR step against 0.87 with 4)
+        if (sizeof(T) == 4 && nx % 4 == 0) {
             tls_ = tlv_;
             tls_.gz = (nz + 1) / 2;
             tls_.nb = tls_.gx * tls_.gy * tls_.gz;
@@ -389,17 +394,24 @@ class Solver final : public SolverBase {
     static constexpr size_t kOneLaunchMaxCells = 3200000;  // 32 N bytes <= ~100 MB
 
     int effective_form() const {
+        if (trial_form_) return trial_form_;  // the autotuner's candidate
+        int f = params.cg_form;
         if constexpr (sizeof(T) != 4) {
-            return AETHER_CG_SCALAR;
+            // FP64: two cells per 16-byte lane when nx % 2 == 0 (measured 1.33x
+            // faster per iteration than one cell per thread at 128^3, 1.77x at
+            // 256^3); the one-launch form is built for FP32
+            if (f == AETHER_CG_AUTO)
+                f = tuned_form_ ? tuned_form_ : tlv_.nb > 0 ? AETHER_CG_X4 : AETHER_CG_SCALAR;
+            if (f == AETHER_CG_FUSED) f = AETHER_CG_SCALAR;
+            if (f == AETHER_CG_X4 && tlv_.nb == 0)
+                raise(AETHER_EINVAL, "AETHER_CG_X4 in FP64 needs nx % 2 == 0");
         } else {
-            if (trial_form_) return trial_form_;  // the autotuner's candidate
-            int f = params.cg_form;
             if (f == AETHER_CG_AUTO)
                 f = tuned_form_ ? tuned_form_ : default_form();
             if (f == AETHER_CG_X4 && g_.nx % 4 != 0)
                 raise(AETHER_EINVAL, "AETHER_CG_X4 needs nx % 4 == 0");
-            return f;
         }
+        return f;
     }
 
     // Partial-sum buffers hold at least n slots (their contents are per solve).
@@ -472,12 +484,13 @@ class Solver final : public SolverBase {
             for (int a = 0; a < 6; ++a) {
                 const Cand c = cands[a];
                 const bool ok = c.form == AETHER_CG_SCALAR ||
-                                (fp32 && (c.form == AETHER_CG_FUSED || tlv_.nb > 0));
+                                (c.form == AETHER_CG_X4 && tlv_.nb > 0) ||
+                                (fp32 && c.form == AETHER_CG_FUSED);
                 if (us) us[a] = -1.0;
                 if (!ok) continue;
                 set_tilings(c.form == AETHER_CG_SCALAR ? c.z : zu0,
                             c.form == AETHER_CG_X4 ? c.z : vzu0);
-                trial_form_ = fp32 ? c.form : 0;
+                trial_form_ = c.form;
                 float ms = 0.0f;
                 for (int rep = 0; rep < 2; ++rep) {  // the first run warms up
                     zero(p_.p);
@@ -512,7 +525,7 @@ class Solver final : public SolverBase {
         (void)hipEventDestroy(e1);
         set_tilings(pick.form == AETHER_CG_SCALAR ? pick.z : zu0,
                     pick.form == AETHER_CG_X4 ? pick.z : vzu0);
-        tuned_form_ = fp32 ? pick.form : 0;
+        tuned_form_ = pick.form;
         tuned_zu_ = zu_;
         tuned_vzu_ = vzu_;
         HIPT(hipMemcpyAsync(p_.p, saved_p.p, n_ * sizeof(T), hipMemcpyDeviceToDevice, st_));
@@ -619,7 +632,82 @@ class Solver final : public SolverBase {
             }));
 #undef X4_PICK
         } else {
-            raise(AETHER_EINVAL, "the four-wide CG is built for FP32 only");
+            // FP64: two x-cells per thread (cg_vec2.hpp), the same buffer and
+            // partial-sum roles and the same p deferral as the FP32 kernels
+            const int nb_in = it == 0 ? tl_.nb : tlv_.nb;
+            hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
+            if (timer.enabled) {
+                e0 = timer.get();
+                e1 = timer.get();
+                e2 = timer.get();
+                e3 = timer.get();
+                timer.recs.push_back({0, step_index, it, e0, e1});
+                timer.recs.push_back({1, step_index, it, e2, e3});
+            }
+            Dims g = g_;
+            VTiling vt = tlv_;
+            int itv = it, nbv = nb_in, sid = solve_id;
+            double rt = rel2;
+            const double *prr = partB_.p, *pbb = partC_.p, *pdq_in = partA_.p;
+            double *pdq = partA_.p, *prr_out = partB_.p;
+            CgState *st = cg_.p;
+            const T *r = r_.p, *dold = d_[(it + 1) & 1].p, *dcur = d_[it & 1].p;
+            T *dnew = d_[it & 1].p, *pp = p_.p, *rw = r_.p;
+            int *dm = done_dev_;
+            // Past 1024 partials one workgroup sums the previous launch's
+            // first, as in the FP32 form; past 4096 the d.q partials too (two
+            // cells per thread make twice the FP32 form's workgroups: 8192 at
+            // 256^3 with 4 planes per thread).
+            const bool wide = std::max(nb_in, vt.nb) > 4 * kThreads;
+            const bool wide_u = vt.nb > 16 * kThreads;
+            int nbu = vt.nb;
+            if (wide) {
+                k_partials_total<<<1, kThreads, 0, st_>>>(prr, nb_in, tot_.p);
+                if (it == 0) k_partials_total<<<1, kThreads, 0, st_>>>(pbb, nb_in, tot_.p + 1);
+                LAUNCH_CHECK();
+                prr = tot_.p;
+                pbb = tot_.p + 1;
+                nbv = 1;
+            }
+            const bool wt = n_ * sizeof(T) < (size_t(1) << 31);  // 32-bit byte offsets
+            const bool odd = it & 1;
+            const T *dprev = dold;  // d_(it-1), for the odd update
+#define X2_PICK(NQD, ST_)                                                                  \
+    do {                                                                                   \
+        f1 = it == 0 ? (const void *)k_cg_dir2<VZU, true, NQD, ST_, false>                 \
+             : odd   ? (const void *)k_cg_dir2<VZU, false, NQD, ST_, true>                 \
+                     : (const void *)k_cg_dir2<VZU, false, NQD, ST_, false>;               \
+        if (nbu > 4 * kThreads)                                                            \
+            f2 = odd ? (const void *)k_cg_update2<VZU, 16, ST_, kPBoth>                    \
+                     : (const void *)k_cg_update2<VZU, 16, ST_, kPSkip>;                   \
+        else                                                                               \
+            f2 = odd ? (const void *)k_cg_update2<VZU, 4, ST_, kPBoth>                     \
+                     : (const void *)k_cg_update2<VZU, 4, ST_, kPSkip>;                    \
+    } while (0)
+            VZU_DISPATCH(vzu_, ({
+                const void *f1, *f2;
+                if (wide_u) nbu = 1;  // summed below, after the direction launch
+                if (wide && wt)
+                    X2_PICK(1, kStWT);
+                else if (wide)
+                    X2_PICK(1, kStPlain);
+                else if (wt)
+                    X2_PICK(4, kStWT);
+                else
+                    X2_PICK(4, kStPlain);
+                void *a1[] = {&g, &vt, &itv, &rt, &nbv, &prr, &pbb, &st, &r, &dold, &dnew, &pdq,
+                              &dm, &sid, &pp};
+                HIPT(hipExtLaunchKernel(f1, dim3(vt.nb), dim3(kThreads), a1, 0, st_, e0, e1, 0));
+                if (wide_u) {
+                    k_partials_total<<<1, kThreads, 0, st_>>>(pdq, vt.nb, tot_.p + 2);
+                    LAUNCH_CHECK();
+                    pdq_in = tot_.p + 2;
+                }
+                void *a2[] = {&g, &vt, &itv, &nbu, &pdq_in, &st, &dcur, &pp, &rw, &prr_out, &dm,
+                              &sid, &dprev};
+                HIPT(hipExtLaunchKernel(f2, dim3(vt.nb), dim3(kThreads), a2, 0, st_, e2, e3, 0));
+            }));
+#undef X2_PICK
         }
     }
 
@@ -748,6 +836,15 @@ class Solver final : public SolverBase {
                 }
                 LAUNCH_CHECK();
             }
+        } else if (form_ == AETHER_CG_X4 && max_iters > 0 && (max_iters & 1)) {
+            // the FP64 two-wide form defers p as the FP32 one
+            const long n2 = (long)(n_ / 2);
+            const unsigned nbt = (unsigned)std::min<long>((n2 + kThreads - 1) / kThreads, 4096);
+            const int last = max_iters - 1;
+            k_cg_p_tail2<<<nbt, kThreads, 0, st_>>>(
+                cg_.p, last, reinterpret_cast<const double2 *>(d_[last & 1].p),
+                reinterpret_cast<double2 *>(p_.p), n2);
+            LAUNCH_CHECK();
         }
         const double *part_rr = partB_.p;
         int nb_rr = tl_.nb;

@@ -45,14 +45,19 @@ extern "C" {
 #define AETHER_SOLVER_CG 0
 #define AETHER_SOLVER_RBGS 1
 
-/* CG iteration layout (all follow src/fluid.rs:463-487; the FP64
- * instantiation always uses AETHER_CG_SCALAR, the reference's evaluation order) */
+/* CG iteration layout (all follow src/fluid.rs:463-487; in FP64 every layout
+ * keeps the reference's per-cell evaluation order, and the layouts differ only
+ * in the order their dot products are summed) */
 #define AETHER_CG_AUTO 0     /* FP32: nx % 4 != 0: AETHER_CG_SCALAR; else AETHER_CG_FUSED up to
-                                3.2 M cells (cache-resident), AETHER_CG_X4 above; or the
-                                aether_fluid_autotune_cg choice */
+                                3.2 M cells (cache-resident), AETHER_CG_X4 above.  FP64:
+                                AETHER_CG_X4 when nx % 2 == 0, else AETHER_CG_SCALAR.  Or
+                                the aether_fluid_autotune_cg choice */
 #define AETHER_CG_SCALAR 1   /* two launches per iteration, one cell per thread */
-#define AETHER_CG_X4 2       /* two launches per iteration, four x-cells per thread (FP32, nx % 4 == 0) */
-#define AETHER_CG_FUSED 3    /* one launch per iteration, single-reduction form (FP32);
+#define AETHER_CG_X4 2       /* two launches per iteration, one 16-byte lane of x-cells per
+                                thread: four in FP32 (nx % 4 == 0), two in FP64
+                                (nx % 2 == 0) */
+#define AETHER_CG_FUSED 3    /* one launch per iteration, single-reduction form (FP32; FP64
+                                runs AETHER_CG_SCALAR);
                                 batches (every nx % 4 == 0): solved group by group, each
                                 group at most 3.2 M cells */
 
@@ -101,7 +106,7 @@ typedef struct {
                                        the graph; converged instances return at once) */
     int32_t cg_z_cells;             /* z cells per thread of the CG tiling: 0 = the
                                        layout's default or the autotuner's choice; the
-                                       four-wide two-launch layout takes 2 or 4, the
+                                       16-byte-lane two-launch layout takes 2 or 4, the
                                        one-cell-per-thread layout 4, 8 or 16 (batches: 8),
                                        the batches' one-launch layout 4 */
     int32_t reserved;
@@ -225,9 +230,9 @@ int aether_fluid_max_density(aether_fluid_t s, double *out);
  * this grid (pressure and divergence are saved and restored; the last-solve
  * statistics then describe the tuning solves) and makes the fastest the
  * AETHER_CG_AUTO choice of this handle.  us_per_iter (6 entries, may be
- * NULL): four-wide with 2 and 4 z cells per thread, one cell per thread with
- * 4, 8 and 16, the one-launch form; -1 where a candidate does not apply (FP64
- * tunes the one-cell-per-thread tiling only).  *form / *z_cells: the choice. */
+ * NULL): the 16-byte-lane layout with 2 and 4 z cells per thread, one cell
+ * per thread with 4, 8 and 16, the one-launch form; -1 where a candidate does
+ * not apply (FP64 has no one-launch form).  *form / *z_cells: the choice. */
 int aether_fluid_autotune_cg(aether_fluid_t s, int iters, double *us_per_iter, int *form,
                              int *z_cells);
 

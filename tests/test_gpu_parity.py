@@ -140,6 +140,73 @@ def test_fp64_cg_rel_tol_iteration_count():
     assert max(errs.values()) <= 1e-9, errs
 
 
+@pytest.mark.parametrize("dims,zc", [((18, 22, 18), 4), ((64, 48, 40), 4), ((64, 48, 40), 2),
+                                     ((36, 37, 41), 4), ((320, 24, 20), 4), ((6, 6, 6), 2),
+                                     ((128, 20, 12), 4)])
+def test_fp64_two_wide_cg_vs_oracle(dims, zc):
+    """FP64 CG with two x-cells per thread (cg_form CG_X4 in FP64): the
+    one-cell-per-thread form's per-cell arithmetic with the dot products
+    summed in another order.  Rows of 3, 9, 18, 32, 64 and 160 lanes (two
+    wavefront-wide segments and a partial one), 2 and 4 planes per thread.
+    Odd and even iteration caps (an odd cap leaves p's last term to the tail
+    kernel, a stop before it applies the term itself): iteration counts
+    equal to the oracle's and fields within
+    FP64_TIGHT; a tolerance stop: iterations within 1 and fields within 1e-9,
+    as the one-cell-per-thread form."""
+    import aether_fluid as af
+    from pairs import load_state
+    st = random_state(*dims, seed=7)
+    gpu, orc = make_pair(*dims, state=st)
+    gpu.cg_form = af.CG_X4
+    gpu.cg_z_cells = zc
+    for iters, tol in ((25, 0.0), (40, 0.0), (3000, 1e-5)):
+        _announce(f"fp64 two-wide CG {dims} z{zc} iters {iters} tol {tol}")
+        load_state(gpu, orc, st)
+        set_knobs(gpu, orc, kind=0, iters=iters, rel_tol=tol)
+        gpu.project()
+        orc.project()
+        gs, os_ = gpu.last_solve_stats(), orc.stats()
+        names = ("u", "v", "w", "pressure", "divergence")
+        errs = compare_fields(gpu, orc, names)
+        if tol == 0.0:
+            # (the 6^3 system reaches the absolute guard before either cap)
+            assert gs["iters"] == os_["iters"], (gs, os_)
+            assert max(errs.values()) <= FP64_TIGHT, errs
+        else:
+            assert os_["converged"] and gs["converged"]
+            assert abs(gs["iters"] - os_["iters"]) <= 1, (gs, os_)
+            assert max(errs.values()) <= 1e-9, errs
+
+
+def test_fp64_two_wide_cg_past_4096_partials():
+    """256 x 256 x 136 in FP64 (8.9 M cells): the two-wide tiling has 4352
+    workgroups, so both launches read pre-summed partials.  Against the
+    one-cell-per-thread FP64 form on the same state: equal iteration counts
+    (an odd cap and a tolerance stop) and fields within 1e-12 relative."""
+    import aether_fluid as af
+    dims = (256, 256, 136)
+    st = random_state(*dims, seed=9)
+    out = {}
+    for form in (af.CG_SCALAR, af.CG_X4):
+        _announce(f"fp64 CG {dims} form {form}")
+        s = af.FluidSolver3D(*dims, precision="fp64")
+        s.solver_kind, s.cg_form = 0, form
+        res = []
+        for iters, tol in ((21, 0.0), (400, 1e-3)):
+            for n, v in st.items():
+                s.upload(n, v)
+            s.solver_iters, s.cg_rel_tol = iters, tol
+            s.project()
+            res.append((s.last_solve_stats()["iters"],
+                        {n: s.download(n) for n in ("u", "v", "w", "pressure")}))
+        out[form] = res
+        s.close()
+    for (ia, fa), (ib, fb) in zip(out[af.CG_SCALAR], out[af.CG_X4]):
+        assert ia == ib, (ia, ib)
+        for n in fa:
+            assert rel_l2(fb[n], fa[n]) <= 1e-12, n
+
+
 def test_cg_zero_rhs_does_nothing():
     _announce("CG zero rhs")
     import aether_fluid as af
@@ -982,8 +1049,9 @@ def test_autotune_keeps_state_and_results(precision):
     print("autotune", r)
     np.testing.assert_array_equal(a.download("pressure"), p0)
     np.testing.assert_array_equal(a.download("divergence"), d0)
-    if precision == "fp64":
-        assert set(r["us_per_iter"]) == {"scalar_z4", "scalar_z8", "scalar_z16"}
+    if precision == "fp64":  # nx % 2 == 0: the two-wide layout is a candidate too
+        assert set(r["us_per_iter"]) == {"x4_z2", "x4_z4", "scalar_z4", "scalar_z8",
+                                         "scalar_z16"}
     else:
         assert len(r["us_per_iter"]) == 6
     fa, fb = a.step(0.02, 0.0).as_array(), b.step(0.02, 0.0).as_array()
