@@ -1476,11 +1476,14 @@ __global__ __launch_bounds__(kThreads) void k_telemetry_final(const double *__re
 // ---------------------------------------------------------------------------
 // One workgroup: the total of nb partials in the order of
 // EarlyPartialsT<16>::total (so a kernel that reads this total gets the
-// value it would have summed itself), with every load in flight at once.
+// value it would have summed itself).  32 registers per thread keep every
+// load of up to 8192 partials in flight at once; EarlyPartialsT<32> adds
+// the same terms in the same order as <16> (its registers in index order,
+// then the rest in index order).
 __global__ __launch_bounds__(kThreads) void k_partials_total(const double *__restrict__ part,
                                                              int nb, double *__restrict__ out) {
     __shared__ double lds[8];
-    EarlyPartialsT<16> v;
+    EarlyPartialsT<32> v;
     v.load(part, nb);
     const double s = v.total(part, nb, lds);
     if (tid_lin() == 0) *out = s;
