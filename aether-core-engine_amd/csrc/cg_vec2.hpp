@@ -43,12 +43,6 @@ __device__ __forceinline__ VCol vcol2_at(const Dims &g, const VTiling &vt, int b
     return c;
 }
 
-__device__ __forceinline__ VCol vcol2_of(const Dims &g, const VTiling &vt, int zu) {
-    int bx, by, bz;
-    vtile_coords(vt, bx, by, bz);
-    return zu == 2 ? vcol2_at<2>(g, vt, bx, by, bz) : vcol2_at<4>(g, vt, bx, by, bz);
-}
-
 // Loads and stores of a two-wide kernel.  With write-through stores (arrays
 // under 2 GiB) the byte offset fits 32 bits, as in ld4 and Out4.
 template <int ST>
@@ -350,6 +344,83 @@ __global__ __launch_bounds__(kThreads) void k_cg_update2(
     }
     block_sum<1>(acc, lds);
     if (threadIdx.x == 0) part_rr[slot] = acc[0];
+}
+
+// Red-black SOR update of the colour-`colour` interior cells among two
+// x-cells (src/fluid.rs:400-437), with k_rbgs's per-cell FP64 arithmetic:
+// left/right are the x-neighbours outside the double2, ym/yp/zm/zp the rows
+// and planes beside them, bb the right-hand side.  Cells of the other
+// colour or off the interior keep their bits.
+__device__ __forceinline__ double2 sor_colour2(const Dims &g, int i0, int j, int k, int colour,
+                                               double omega, const double2 &ctr, double left,
+                                               double right, const double2 &ym,
+                                               const double2 &yp, const double2 &zm,
+                                               const double2 &zp, const double2 &bb, bool &any) {
+    double o[2] = {ctr.x, ctr.y};
+    const double vym[2] = {ym.x, ym.y}, vyp[2] = {yp.x, yp.y};
+    const double vzm[2] = {zm.x, zm.y}, vzp[2] = {zp.x, zp.y}, vb[2] = {bb.x, bb.y};
+    const double xm[2] = {left, ctr.x}, xp[2] = {ctr.y, right};
+    any = false;
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+        const int i = i0 + e;
+        if (!is_interior(g, i, j, k) || ((i + j + k) & 1) != colour) continue;
+        any = true;
+        double diag = 0.0, nb = 0.0;
+        if (i > 1) { diag += 1.0; nb += xm[e]; }
+        if (i + 2 < g.nx) { diag += 1.0; nb += xp[e]; }
+        if (j > 1) { diag += 1.0; nb += vym[e]; }
+        if (j + 2 < g.ny) { diag += 1.0; nb += vyp[e]; }
+        if (k > 1) { diag += 1.0; nb += vzm[e]; }
+        if (k + 2 < g.nz) { diag += 1.0; nb += vzp[e]; }
+        const double gs = sor_quot(vb[e] + nb, diag);
+        o[e] = (1.0 - omega) * o[e] + omega * gs;
+    }
+    return make_double2(o[0], o[1]);
+}
+
+// Red-black SOR pass of one colour on the two-wide FP64 layout, as k_rbgs4
+// on the four-wide FP32 one: a cell's neighbours all have the other colour,
+// which this pass leaves unchanged, so every thread reads consistent values
+// and writes back its whole double2 (other-colour cells with their bits).
+// Bit-identical to k_rbgs.
+template <int ZU, int ST>
+__global__ __launch_bounds__(kThreads) void k_rbgs2(Dims g, VTiling vt, int color, double omega,
+                                                    const double *__restrict__ b,
+                                                    double *__restrict__ p) {
+    int bx, by, bz;
+    vtile_coords(vt, bx, by, bz);
+    const VCol c = vcol2_at<ZU>(g, vt, bx, by, bz);
+    const Out2<ST> o_p(p, (long)g.sxy * g.nz * 8);
+    double2 pz[ZU + 2], py[ZU][2], bc[ZU];
+    double xl[ZU], xr[ZU];
+#pragma unroll
+    for (int a = 0; a < ZU + 2; ++a) {
+        const int base = min(max(c.k0 - 1 + a, 0), g.nz - 1) * g.sxy;
+        pz[a] = ld2<ST>(p, (base + c.off) >> 1);
+    }
+#pragma unroll
+    for (int a = 0; a < ZU; ++a) {
+        const int base = min(c.k0 + a, g.nz - 1) * g.sxy;
+        py[a][0] = ld2<ST>(p, (base + c.offym) >> 1);
+        py[a][1] = ld2<ST>(p, (base + c.offyp) >> 1);
+        bc[a] = ld2<ST>(b, (base + c.off) >> 1);
+        xl[a] = xr[a] = 0.0;
+        if (c.needL) xl[a] = ld1d<ST>(p, base + c.off - 1);
+        if (c.needR) xr[a] = ld1d<ST>(p, base + c.off + 2);
+    }
+#pragma unroll
+    for (int a = 0; a < ZU; ++a) {
+        const int k = c.k0 + a;
+        const double sl = lane_prev_d(pz[a + 1].y), sr = lane_next_d(pz[a + 1].x);
+        const double left = c.needL ? xl[a] : sl, right = c.needR ? xr[a] : sr;
+        if (c.valid && k < g.nz) {
+            bool any;
+            const double2 o = sor_colour2(g, c.i0, c.j, k, color, omega, pz[a + 1], left, right,
+                                          py[a][0], py[a][1], pz[a], pz[a + 2], bc[a], any);
+            if (any) o_p.put((k * g.sxy + c.off) >> 1, o);
+        }
+    }
 }
 
 // Closes a two-wide solve whose iteration cap ended it after an even launch

@@ -908,6 +908,27 @@ class Solver final : public SolverBase {
             last_sweeps_ = sweeps;
             return;
         }
+        if constexpr (sizeof(T) == 8) {
+            // FP64 colour passes on the two-wide layout (k_rbgs2: k_rbgs's
+            // per-cell arithmetic, bit-identical)
+            if (tlv_.nb > 0 && params.cg_form != AETHER_CG_SCALAR) {
+                const bool wt = n_ * sizeof(T) < (size_t(1) << 31);
+                for (int it = 0; it < sweeps; ++it)
+                    for (int color = 0; color < 2; ++color) {
+                        if (wt)
+                            VZU_DISPATCH(vzu_, (k_rbgs2<VZU, kStWT><<<tlv_.nb, kThreads, 0, st_>>>(
+                                                   g_, tlv_, color, params.sor_omega, b_.p, p_.p)));
+                        else
+                            VZU_DISPATCH(vzu_, (k_rbgs2<VZU, kStPlain><<<tlv_.nb, kThreads, 0,
+                                                                         st_>>>(
+                                                   g_, tlv_, color, params.sor_omega, b_.p, p_.p)));
+                        LAUNCH_CHECK();
+                    }
+                last_kind_ = AETHER_SOLVER_RBGS;
+                last_sweeps_ = sweeps;
+                return;
+            }
+        }
         for (int it = 0; it < sweeps; ++it)
             for (int color = 0; color < 2; ++color) {
                 k_rbgs<T><<<cell_grid_, cell_blk_, 0, st_>>>(g_, color, (T)params.sor_omega,

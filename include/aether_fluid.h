@@ -52,7 +52,8 @@ extern "C" {
                                 3.2 M cells (cache-resident), AETHER_CG_X4 above.  FP64:
                                 AETHER_CG_X4 when nx % 2 == 0, else AETHER_CG_SCALAR.  Or
                                 the aether_fluid_autotune_cg choice */
-#define AETHER_CG_SCALAR 1   /* two launches per iteration, one cell per thread */
+#define AETHER_CG_SCALAR 1   /* two launches per iteration, one cell per thread (also keeps
+                                the red-black SOR passes at one cell per thread) */
 #define AETHER_CG_X4 2       /* two launches per iteration, one 16-byte lane of x-cells per
                                 thread: four in FP32 (nx % 4 == 0), two in FP64
                                 (nx % 2 == 0) */
