@@ -423,6 +423,106 @@ __global__ __launch_bounds__(kThreads) void k_rbgs2(Dims g, VTiling vt, int colo
     }
 }
 
+// One whole red-black SOR sweep in one launch on the two-wide FP64 layout,
+// from p_in to p_out, as k_rbsor4 on the four-wide FP32 one: each thread
+// forms the colour-0 values of its column and one plane past each end,
+// shares its own planes through LDS (the first and last tile rows also form
+// the row beyond the tile), then updates its colour-1 cells and writes its
+// whole column.  The colour-0 values are what the colour-0 pass would store
+// (same inputs, same arithmetic), so the sweep is bit-identical to two
+// k_rbgs2 passes (and to k_rbgs).  Needs every grid row in one wave with no
+// lane past its end (nx = 2 w2, w2 dividing 64).
+template <int ZU, int ST>
+__global__ __launch_bounds__(kThreads) void k_rbsor2(Dims g, VTiling vt, double omega,
+                                                     const double *__restrict__ b,
+                                                     const double *__restrict__ p_in,
+                                                     double *__restrict__ p_out) {
+    __shared__ double2 lds_rc[ZU][kThreads];
+    int bx, by, bz;
+    vtile_coords(vt, bx, by, bz);
+    const VCol c = vcol2_at<ZU>(g, vt, bx, by, bz);
+    const Out2<ST> o_p(p_out, (long)g.sxy * g.nz * 8);
+    const int t = threadIdx.x, ty = t / vt.txv;
+    const bool first_row = ty == 0, last_row = ty == vt.ty - 1;
+    const int jc = min(c.j, g.ny - 1), ic0 = c.off - jc * g.nx;
+    auto row = [&](int dj) { return min(max(c.j + dj, 0), g.ny - 1) * g.nx + ic0; };
+    auto plane = [&](int k) { return min(max(k, 0), g.nz - 1) * g.sxy; };
+    double2 C[ZU + 4];      // row j, planes k0-2 .. k0+ZU+1
+    double2 Y1[2][ZU + 2];  // rows j-1, j+1, planes k0-1 .. k0+ZU
+    double2 BC[ZU + 2];     // b on row j, planes k0-1 .. k0+ZU
+#pragma unroll
+    for (int a = 0; a < ZU + 4; ++a) C[a] = ld2<ST>(p_in, (plane(c.k0 - 2 + a) + c.off) >> 1);
+#pragma unroll
+    for (int a = 0; a < ZU + 2; ++a) {
+        const int base = plane(c.k0 - 1 + a);
+        Y1[0][a] = ld2<ST>(p_in, (base + c.offym) >> 1);
+        Y1[1][a] = ld2<ST>(p_in, (base + c.offyp) >> 1);
+        BC[a] = ld2<ST>(b, (base + c.off) >> 1);
+    }
+    // the first and last tile rows also need p two rows away and b on the
+    // row beyond the tile, loaded with the rest
+    const int dj_edge = first_row ? -1 : last_row ? 1 : 0;
+    double2 FAR[ZU], BB[ZU];
+    if (dj_edge != 0) {
+        const int rfar = row(2 * dj_edge), rnear = dj_edge < 0 ? c.offym : c.offyp;
+#pragma unroll
+        for (int a = 0; a < ZU; ++a) {
+            const int base = plane(c.k0 + a);
+            FAR[a] = ld2<ST>(p_in, (base + rfar) >> 1);
+            BB[a] = ld2<ST>(b, (base + rnear) >> 1);
+        }
+    }
+    bool any;
+    // colour 0 on row j, planes k0-1 .. k0+ZU; the thread's own planes to LDS
+    double2 RC[ZU + 2];
+#pragma unroll
+    for (int a = 0; a < ZU + 2; ++a) {
+        const double sl = lane_prev_d(C[a + 1].y), sr = lane_next_d(C[a + 1].x);
+        RC[a] = sor_colour2(g, c.i0, c.j, c.k0 - 1 + a, 0, omega, C[a + 1], sl, sr, Y1[0][a],
+                            Y1[1][a], C[a], C[a + 2], BC[a], any);
+    }
+#pragma unroll
+    for (int a = 0; a < ZU; ++a) lds_rc[a][t] = RC[a + 1];
+    // x-neighbours on rows j-1 and j+1 (every lane takes part in the exchange)
+    double yl[2][ZU], yr[2][ZU];
+#pragma unroll
+    for (int s2 = 0; s2 < 2; ++s2)
+#pragma unroll
+        for (int a = 0; a < ZU; ++a) {
+            yl[s2][a] = lane_prev_d(Y1[s2][a + 1].y);
+            yr[s2][a] = lane_next_d(Y1[s2][a + 1].x);
+        }
+    // colour 0 on the row beyond the tile (first and last tile rows only)
+    double2 RE[ZU];
+#pragma unroll
+    for (int a = 0; a < ZU; ++a) RE[a] = make_double2(0.0, 0.0);
+    auto edge_row = [&](const double2(&y)[ZU + 2], const double(&l)[ZU], const double(&r)[ZU],
+                        int dj) {
+#pragma unroll
+        for (int a = 0; a < ZU; ++a)
+            RE[a] = sor_colour2(g, c.i0, c.j + dj, c.k0 + a, 0, omega, y[a + 1], l[a], r[a],
+                                dj < 0 ? FAR[a] : C[a + 2], dj < 0 ? C[a + 2] : FAR[a], y[a],
+                                y[a + 2], BB[a], any);
+    };
+    if (first_row) edge_row(Y1[0], yl[0], yr[0], -1);
+    if (last_row) edge_row(Y1[1], yl[1], yr[1], 1);
+    __syncthreads();
+    // colour 1 on the thread's own cells, from the colour-0 values above
+#pragma unroll
+    for (int a = 0; a < ZU; ++a) {
+        const int k = c.k0 + a;
+        const double sl = lane_prev_d(RC[a + 1].y), sr = lane_next_d(RC[a + 1].x);
+        // (indices kept in range on the rows that take RE instead)
+        const double2 ym = first_row ? RE[a] : lds_rc[a][max(t - vt.txv, 0)];
+        const double2 yp = last_row ? RE[a] : lds_rc[a][min(t + vt.txv, kThreads - 1)];
+        if (c.valid && k < g.nz) {
+            const double2 o = sor_colour2(g, c.i0, c.j, k, 1, omega, RC[a + 1], sl, sr, ym, yp,
+                                          RC[a], RC[a + 2], BC[a + 1], any);
+            o_p.put((k * g.sxy + c.off) >> 1, o);  // every cell: p_out holds the whole sweep
+        }
+    }
+}
+
 // Closes a two-wide solve whose iteration cap ended it after an even launch
 // (that launch left alpha_it d_it pending): p += alpha d.  Skipped when a
 // launch stopped the solve (that launch applied the pending term).

@@ -913,6 +913,32 @@ class Solver final : public SolverBase {
             // per-cell arithmetic, bit-identical)
             if (tlv_.nb > 0 && params.cg_form != AETHER_CG_SCALAR) {
                 const bool wt = n_ * sizeof(T) < (size_t(1) << 31);
+                if (tlv_.gx == 1 && 64 % tlv_.txv == 0 && sweeps > 0) {
+                    // one launch per sweep (k_rbsor2), 2 planes per thread,
+                    // from p to a second array and back
+                    VTiling ts = tlv_;
+                    ts.gz = (g_.nz + 1) / 2;
+                    ts.nb = ts.gx * ts.gy * ts.gz;
+                    ts.remap = (ts.nb % 8 == 0) ? 1 : 0;
+                    if (!p2_.p) p2_.alloc(n_);
+                    for (int it = 0; it < sweeps; ++it) {
+                        const T *pin = (it & 1) ? p2_.p : p_.p;
+                        T *pout = (it & 1) ? p_.p : p2_.p;
+                        if (wt)
+                            k_rbsor2<2, kStWT><<<ts.nb, kThreads, 0, st_>>>(
+                                g_, ts, params.sor_omega, b_.p, pin, pout);
+                        else
+                            k_rbsor2<2, kStPlain><<<ts.nb, kThreads, 0, st_>>>(
+                                g_, ts, params.sor_omega, b_.p, pin, pout);
+                        LAUNCH_CHECK();
+                    }
+                    if (sweeps & 1)
+                        HIPT(hipMemcpyAsync(p_.p, p2_.p, n_ * sizeof(T), hipMemcpyDeviceToDevice,
+                                            st_));
+                    last_kind_ = AETHER_SOLVER_RBGS;
+                    last_sweeps_ = sweeps;
+                    return;
+                }
                 for (int it = 0; it < sweeps; ++it)
                     for (int color = 0; color < 2; ++color) {
                         if (wt)

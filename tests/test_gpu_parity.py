@@ -228,20 +228,22 @@ def test_fp64_rbgs_bitexact():
         np.testing.assert_array_equal(gpu.download(n), orc.field(n).ravel())
 
 
-@pytest.mark.parametrize("dims,sweeps", [((32, 24, 20), 7), ((32, 24, 20), 8),
-                                         ((256, 10, 9), 5)])
-def test_one_launch_sor_sweep_equals_colour_passes(dims, sweeps):
-    """FP32 RB-SOR on rows of 8 and 64 float4 lanes, where a sweep runs as
-    one launch (colour-0 values formed where the colour-1 cells read them),
-    with odd and even sweep counts: the projection is bitwise equal to the
-    one-cell-per-thread colour passes, and within the FP32 bar of the
-    oracle."""
-    _announce(f"one-launch SOR sweep {dims} x{sweeps}")
+@pytest.mark.parametrize("precision,dims,sweeps", [
+    ("fp32", (32, 24, 20), 7), ("fp32", (32, 24, 20), 8), ("fp32", (256, 10, 9), 5),
+    ("fp64", (32, 24, 20), 7), ("fp64", (128, 12, 10), 8), ("fp64", (256, 10, 9), 5)])
+def test_one_launch_sor_sweep_equals_colour_passes(precision, dims, sweeps):
+    """RB-SOR where a sweep runs as one launch (colour-0 values formed where
+    the colour-1 cells read them): FP32 rows of 8 and 64 float4 lanes, FP64
+    rows of 16 and 64 double2 lanes (FP64 rows of 128 lanes run the two-wide
+    colour passes), odd and even sweep counts.  The projection is bitwise
+    equal to the one-cell-per-thread colour passes; FP32 is within its bar of
+    the oracle, FP64 equal to it bit for bit."""
+    _announce(f"one-launch SOR sweep {precision} {dims} x{sweeps}")
     import aether_fluid as af
     st = random_state(*dims, seed=41)
-    a, orc = make_pair(*dims, precision="fp32", state=st)
+    a, orc = make_pair(*dims, precision=precision, state=st)
     set_knobs(a, orc, kind=1, iters=sweeps, omega=1.8)
-    b = af.FluidSolver3D(*dims, precision="fp32")
+    b = af.FluidSolver3D(*dims, precision=precision)
     for n, v in st.items():
         b.upload(n, v)
     b.solver_kind, b.solver_iters, b.sor_omega = 1, sweeps, 1.8
@@ -251,6 +253,8 @@ def test_one_launch_sor_sweep_equals_colour_passes(dims, sweeps):
     orc.project()
     for n in ("u", "v", "w", "pressure"):
         np.testing.assert_array_equal(a.download(n), b.download(n))
+        if precision == "fp64":
+            np.testing.assert_array_equal(a.download(n), orc.field(n).ravel(), err_msg=n)
     errs = compare_fields(a, orc, ("u", "v", "w", "pressure"))
     assert max(errs.values()) <= 1e-5, errs
 
