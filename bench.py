@@ -236,6 +236,24 @@ def precision_vs_fp64(steps: int, warmup: int, fp32_solver) -> dict:
     out["div_max_fp64"] = s64.divergence_max()
     out["cg_iters_fp64_last"] = s64.run_iters()[-1]
     s64.close()
+    # one step from the seeded state in both precisions (the FP32 solver
+    # starts from the FP32 rounding of it): the per-step difference, before
+    # the flow's sensitivity amplifies it
+    one = {}
+    for prec in ("fp32", "fp64"):
+        s1 = af.FluidSolver3D(*cfg["dims"], h=cfg["h"], precision=prec)
+        configure(s1, WORKLOAD, "cg")
+        for name, arr in seeded_config_state(WORKLOAD).items():
+            s1.upload(name, arr)
+        s1.run(dt, [0.0])
+        one[prec] = ({n: s1.download(n) for n in ("u", "v", "w", "density", "temperature")},
+                     s1.divergence_max())
+        s1.close()
+    out["one_step"] = {
+        "rel_l2_fp32_vs_fp64": {n: float(np.linalg.norm(one["fp32"][0][n] - one["fp64"][0][n]) /
+                                         max(np.linalg.norm(one["fp64"][0][n]), 1e-300))
+                                for n in one["fp64"][0]},
+        "div_max_fp32": one["fp32"][1], "div_max_fp64": one["fp64"][1]}
     return out
 
 
