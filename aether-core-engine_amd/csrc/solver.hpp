@@ -397,9 +397,9 @@ class Solver final : public SolverBase {
         if (trial_form_) return trial_form_;  // the autotuner's candidate
         int f = params.cg_form;
         if constexpr (sizeof(T) != 4) {
-            // FP64: two cells per 16-byte lane when nx % 2 == 0 (measured 1.33x
-            // faster per iteration than one cell per thread at 128^3, 1.77x at
-            // 256^3); the one-launch form is built for FP32
+            // FP64: two cells per 16-byte lane when nx % 2 == 0 (measured 1.38x
+            // faster per iteration than the best one-cell-per-thread tiling at
+            // 128^3, 1.89x at 256^3); the one-launch form is built for FP32
             if (f == AETHER_CG_AUTO)
                 f = tuned_form_ ? tuned_form_ : tlv_.nb > 0 ? AETHER_CG_X4 : AETHER_CG_SCALAR;
             if (f == AETHER_CG_FUSED) f = AETHER_CG_SCALAR;

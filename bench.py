@@ -469,6 +469,64 @@ MEDIUM_BOUNDS = ((-1.0, -1.0, -1.0), (1.0, 1.0, 1.0))
 MEDIUM_SCALE, MEDIUM_G = 2.5, 0.55
 
 
+def bench_fp64(steps: int, warmup: int) -> dict:
+    """The 128^3 workload in FP64, the reference's own precision: ms per step
+    with the reference's default CG budget (32 iterations, src/fluid.rs:83)
+    and with the workload's 200; the CG layouts' time per iteration (the
+    autotuner's HIP-event timing of 20 whole iterations) at 128^3 and 256^3;
+    red-black SOR time per sweep (60 sweeps, omega 1.8) in the default
+    layout and one cell per thread, with a bitwise comparison of the two."""
+    import aether_fluid as af
+    from aether_fluid.scenarios import SMOKE_CONFIGS, configure, seeded_config_state
+    cfg = SMOKE_CONFIGS[WORKLOAD]
+    dt = cfg["dt"]
+    state = seeded_config_state(WORKLOAD)
+    out = {"workload": "plain smoke 128^3 in FP64", "steps": steps, "warmup": warmup}
+
+    def solver(n=None):
+        dims = cfg["dims"] if n is None else (n, n, n)
+        s = af.FluidSolver3D(*dims, h=cfg["h"] if n is None else 1.0 / n, precision="fp64")
+        if n is None:
+            configure(s, WORKLOAD, "cg")
+            for name, arr in state.items():
+                s.upload(name, arr)
+        return s
+
+    for iters in (32, 200):
+        s = solver()
+        s.solver_iters = iters
+        t_axis = [dt * i for i in range(warmup + steps)]
+        if warmup:
+            s.run(dt, t_axis[:warmup])
+        t0 = time.perf_counter()
+        s.run(dt, t_axis[warmup:])
+        el = time.perf_counter() - t0
+        out[f"cg_{iters}"] = {"ms_per_step": el * 1e3 / steps,
+                              "voxel_steps_per_s": float(np.prod(cfg["dims"])) * steps / el}
+        s.close()
+    out["cg_us_per_iter"] = {}
+    for n in (128, 256):
+        s = solver(n)
+        out["cg_us_per_iter"][f"{n}^3"] = s.autotune_cg(20)["us_per_iter"]
+        s.close()
+    sweeps = {}
+    pres = {}
+    for name, form in (("default", af.CG_AUTO), ("one_cell_per_thread", af.CG_SCALAR)):
+        s = solver()
+        s.solver_kind, s.solver_iters, s.sor_omega, s.cg_form = 1, 60, 1.8, form
+        s.project()  # warm-up
+        for k, arr in state.items():
+            s.upload(k, arr)
+        t0 = time.perf_counter()
+        s.project()
+        sweeps[name] = (time.perf_counter() - t0) * 1e6 / 60
+        pres[name] = s.download("pressure")
+        s.close()
+    out["rbsor_us_per_sweep"] = sweeps
+    out["rbsor_bitwise_equal"] = bool(np.array_equal(pres["default"], pres["one_cell_per_thread"]))
+    return out
+
+
 def bench_medium(s, nrays: int = 1 << 20) -> dict:
     """FluidMedium (src/fluid.rs:675-910) on the headline solver's final
     128^3 state: the device snapshot (from_solver), point sampling, and the
@@ -540,10 +598,13 @@ def main():
                          "and batched sections of the default line")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--workload", default="plain_128",
-                    choices=["plain_128", "mixed_256", "batched_64", "rollout_64", "medium_128"],
+                    choices=["plain_128", "mixed_256", "batched_64", "rollout_64", "medium_128",
+                             "fp64_128"],
                     help="plain_128: the headline line; mixed_256: mixed-precision "
                          "projection at 256^3 against the all-FP64 projection; medium_128: "
-                         "FluidMedium sampling of the 128^3 state after --steps steps")
+                         "FluidMedium sampling of the 128^3 state after --steps steps; "
+                         "fp64_128: the 128^3 workload in FP64 (step times, CG and RB-SOR "
+                         "layouts)")
     ap.add_argument("--batch-cg", default="auto", choices=["auto", "x4", "scalar", "fused"],
                     help="batched_64: the batch's CG form (auto: the batch autotuner's choice)")
     ap.add_argument("--no-graph", action="store_true",
@@ -573,6 +634,9 @@ def main():
     if args.workload == "medium_128":
         _, s, _, _, _ = time_gpu("cg", args.steps, args.warmup)
         print(json.dumps(bench_medium(s)), flush=True)
+        return
+    if args.workload == "fp64_128":
+        print(json.dumps(bench_fp64(args.steps, args.warmup)), flush=True)
         return
     if args.probe:  # short run for counter collection: both CG layouts, RB-SOR, the copy
         _, s = make_solver("cg")
