@@ -88,11 +88,13 @@ class FluidBatch:
                 for s in range(len(ts))]
 
     def autotune_cg(self, iters: int = 20) -> dict:
-        """Times `iters` CG iterations of the batch in each candidate layout
-        and tiling (four-wide with 2 and 4 z planes per thread, one cell per
-        thread, one launch per iteration solved group by group) on synthetic
-        right-hand sides and keeps the fastest as the batch's choice for
-        CG_AUTO and cg_z_cells 0 (pressure and divergence are restored)."""
+        """Times the batch's CG in each candidate layout and tiling (four-wide
+        with 2 and 4 z planes per thread, one cell per thread, one launch per
+        iteration solved group by group) and keeps the fastest as the batch's
+        choice for CG_AUTO and cg_z_cells 0 (pressure and divergence are
+        restored).  iters > 0: that many iterations on synthetic right-hand
+        sides; iters 0: whole solves of the last step's right-hand sides at
+        the batch's tolerance and cap (after at least one run() step)."""
         us = (C.c_double * 4)()
         form, z = C.c_int(), C.c_int()
         L.check(self._lib.aether_fluid_batch_autotune_cg(self._h, int(iters), us,

@@ -330,9 +330,12 @@ class Batch final : public BatchBase {
     // The one-launch CG's buffers, and its groups: instances whose launches
     // share a kernel (the same tiling class and partial-sum width, for the
     // first launch and for the others), packed in instance order into groups
-    // of at most kOneLaunchGroupCells cells, so that a group's CG working set
-    // stays in the Infinity Cache through its whole solve.
-    static constexpr size_t kOneLaunchGroupCells = 3200000;
+    // of at most kOneLaunchGroupCells cells.  Measured on the 64-grid workload
+    // (ms per step at caps of 3.2 / 4.8 / 6.4 / 9.6 / 16 M cells and no cap:
+    // 39.5 / 37.3 / 36.0 / 36.6 / 38.4 / 38.2): larger launches pay the launch
+    // floor less often until a group's arrays (~50 bytes per cell) spill out
+    // of the 256 MB Infinity Cache between the launches that reuse them.
+    static constexpr size_t kOneLaunchGroupCells = 6400000;
     void ensure_fused() {
         if (fused_ready_) return;
         if (!x4_ok_)
@@ -392,15 +395,24 @@ class Batch final : public BatchBase {
         fused_ready_ = true;
     }
 
-    // Times `iters` CG iterations of the batch in each candidate (four-wide
-    // with 2 and 4 z planes per thread when every nx % 4 == 0, one cell per
-    // thread with 8, and the one-launch form solved group by group) on
-    // synthetic right-hand sides (every instance's pressure and divergence
-    // are saved and restored) and keeps the fastest as the choice for
-    // cg_form AETHER_CG_AUTO and cg_z_cells 0.  us[4]: time per iteration of
-    // the whole batch, in that order (-1 when not applicable).
+    // Times each candidate (four-wide with 2 and 4 z planes per thread when
+    // every nx % 4 == 0, one cell per thread with 8, and the one-launch form
+    // solved group by group) and keeps the fastest as the choice for cg_form
+    // AETHER_CG_AUTO and cg_z_cells 0; every instance's pressure and
+    // divergence are saved and restored.  iters > 0: `iters` iterations on
+    // synthetic right-hand sides with no tolerance stop.  iters 0: whole
+    // solves, from zero pressure, of the last step's right-hand sides at the
+    // batch's tolerance and iteration cap, so that instances converging at
+    // different iterations weigh in as in the batch's own steps.  us[4]: time
+    // per iteration of the whole batch (iters 0: per iteration of the
+    // instance that ran longest), in that order (-1 when not applicable).
     void autotune(int iters, double *us, int *form_out, int *z_out) override {
-        if (iters < 1) raise(AETHER_EINVAL, "iters must be >= 1");
+        if (iters < 0) raise(AETHER_EINVAL, "iters must be >= 0");
+        // iters 0: whole solves of the last step's right-hand side
+        const bool real = iters == 0;
+        if (real && !rhs_ready_)
+            raise(AETHER_EINVAL, "autotune with iters 0 solves the last step's right-hand "
+                                 "side: run a step first");
         for (auto &in : inst_) {  // finalize logs iteration 0's count
             if (in->iters.n < 1) in->iters.alloc(1);
             if (in->tel.n < 8) in->tel.alloc(8);
@@ -412,12 +424,17 @@ class Batch final : public BatchBase {
                 HIPT(hipMemcpyAsync(saved.back()->p, src->p, in->n * sizeof(T),
                                     hipMemcpyDeviceToDevice, st_));
             }
-            k_fill_hash<T><<<4096, kThreads, 0, st_>>>(in->b.p, (long)in->n);
-            LAUNCH_CHECK();
+            if (!real) {
+                k_fill_hash<T><<<4096, kThreads, 0, st_>>>(in->b.p, (long)in->n);
+                LAUNCH_CHECK();
+            }
         }
         const aether_fluid_params keep = params_;
-        params_.cg_rel_tol = 0.0;  // absolute guards only: each candidate runs `iters`
-        params_.solver_iters = iters;
+        if (!real) {
+            params_.cg_rel_tol = 0.0;  // absolute guards only: each candidate runs `iters`
+            params_.solver_iters = iters;
+        }
+        std::vector<int> its(count());
         hipEvent_t e0, e1;
         HIPT(hipEventCreate(&e0));
         HIPT(hipEventCreate(&e1));
@@ -447,7 +464,7 @@ class Batch final : public BatchBase {
                 write_table();  // the table holds the four-wide tiling in use
                 float ms = 0.0f;
                 for (int rep = 0; rep < 2; ++rep) {  // the first run warms up
-                    for (auto &in : inst_) fill(in->p.p, in->n);
+                    for (auto &in : inst_) fill(in->p.p, in->n);  // from p = 0
                     kb_div_cginit<T, kBatchZU><<<ncg_, dim3(kBX, kBY), 0, st_>>>(
                         table_.p, cg_items_.p, sl_, 0);
                     LAUNCH_CHECK();
@@ -457,7 +474,16 @@ class Batch final : public BatchBase {
                     HIPT(hipEventSynchronize(e1));
                     HIPT(hipEventElapsedTime(&ms, e0, e1));
                 }
-                const double t = 1e3 * (double)ms / iters;
+                int nit = iters;
+                if (real) {  // per iteration of the instance that ran longest
+                    nit = 1;
+                    for (int q = 0; q < count(); ++q) {
+                        HIPT(hipMemcpy(&its[q], inst_[q]->iters.p, sizeof(int),
+                                       hipMemcpyDeviceToHost));
+                        nit = std::max(nit, its[q]);
+                    }
+                }
+                const double t = 1e3 * (double)ms / nit;
                 if (us) us[a] = t;
                 if (t < best) {
                     best = t;
@@ -699,6 +725,7 @@ class Batch final : public BatchBase {
         sl_.v ^= 1;
         sl_.w ^= 1;
         kb_div_cginit<T, kBatchZU><<<ncg_, blk, 0, st_>>>(table_.p, cg_items_.p, sl_, 1);
+        rhs_ready_ = true;
         LAUNCH_CHECK();
         cg_solve(s, graph);
         kb_gradient_bc<T><<<ncell_, blk, 0, st_>>>(table_.p, cell_items_.p, sl_);
@@ -842,6 +869,7 @@ class Batch final : public BatchBase {
     };
     std::deque<Group> groups_;  // (a deque: groups hold device buffers and never move)
     bool fused_ready_ = false;
+    bool rhs_ready_ = false;  // b holds a step's right-hand side (autotune with iters 0)
 };
 
 }  // namespace

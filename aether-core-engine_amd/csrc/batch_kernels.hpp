@@ -209,6 +209,11 @@ __global__ __launch_bounds__(kThreads) void kb_cg_dir4(const BInst<float> *__res
                                                        int solve_id) {
     const BItem it = items[blockIdx.x];
     const BInst<float> &in = I[it.inst];
+    if (!FIRST) {  // an instance that stopped returns before loading anything (the
+                   // body's own test: stop word <= 2 iter, written by an earlier launch)
+        const int done = in.st->done;
+        if (done != 0 && done <= 2 * iter) return;
+    }
     body_cg_dir4<VZ, FIRST, NQ, kStWT, PEND>(vcol_at<VZ>(in.g, in.vt, it.bx, it.by, it.bz),
                                             it.local, in.g, iter, rel_tol2,
                                             FIRST ? in.nb : in.nbv, in.pb, in.pc, in.st, in.r,
@@ -223,6 +228,10 @@ __global__ __launch_bounds__(kThreads) void kb_cg_update4(const BInst<float> *__
                                                           int solve_id) {
     const BItem it = items[blockIdx.x];
     const BInst<float> &in = I[it.inst];
+    {  // an instance that stopped returns before loading anything (stop word <= 2 iter + 1)
+        const int done = in.st->done;
+        if (done != 0 && done <= 2 * iter + 1) return;
+    }
     body_cg_update4<VZ, NQ, kStWT, PM>(vcol_at<VZ>(in.g, in.vt, it.bx, it.by, it.bz), it.local,
                                       in.nbv, in.g, iter, in.pa, in.st, in.d[iter & 1], in.p,
                                       in.r, in.pb, done_mirror + it.inst, solve_id,

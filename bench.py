@@ -397,18 +397,24 @@ def bench_batched(steps: int, warmup: int, dt: float = 0.01, cg: str = "auto",
     import aether_fluid as af
     insts = af.batched_instances(64)
     nvox = sum(d["nx"] * d["ny"] * d["nz"] for d in insts)
+    warmup = max(warmup, 1)  # the autotuner solves a warm-up step's right-hand sides
     times = [dt * i for i in range(warmup + steps)]
-    b = af.FluidBatch(insts, solver_iters=200, cg_rel_tol=1e-5)
-    tune_b = b.autotune_cg(20)  # CG layout of the batch, chosen by timing
     forms = {"x4": af.CG_X4, "scalar": af.CG_SCALAR, "fused": af.CG_FUSED}
-    form_b = forms[tune_b["choice"] if cg == "auto" else cg]
-    z_b = tune_b["z_cells"] if cg == "auto" else (4 if form_b != af.CG_SCALAR else 0)
-    if cg != "auto":
-        b.close()
-        b = af.FluidBatch(insts, solver_iters=200, cg_rel_tol=1e-5, cg_form=form_b,
-                          cg_z_cells=z_b)
-    if warmup:
-        b.run(dt, times[:warmup])
+    # CG layout of the batch, chosen by timing whole solves of a warm-up
+    # step's right-hand sides, on a batch of its own, so that the timed batch
+    # runs one layout from its first step (a forced form skips the tuning)
+    tune_b = None
+    if cg == "auto":
+        tb = af.FluidBatch(insts, solver_iters=200, cg_rel_tol=1e-5)
+        tb.run(dt, times[:warmup])
+        tune_b = tb.autotune_cg(0)
+        tb.close()
+        form_b, z_b = forms[tune_b["choice"]], tune_b["z_cells"]
+    else:
+        form_b, z_b = forms[cg], (4 if cg != "scalar" else 0)
+    b = af.FluidBatch(insts, solver_iters=200, cg_rel_tol=1e-5, cg_form=form_b,
+                      cg_z_cells=z_b)
+    b.run(dt, times[:warmup])
     t0 = time.perf_counter()
     b.run(dt, times[warmup:])
     el_b = time.perf_counter() - t0

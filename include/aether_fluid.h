@@ -60,7 +60,7 @@ extern "C" {
 #define AETHER_CG_FUSED 3    /* one launch per iteration, single-reduction form (FP32; FP64
                                 runs AETHER_CG_SCALAR);
                                 batches (every nx % 4 == 0): solved group by group, each
-                                group at most 3.2 M cells */
+                                group at most 6.4 M cells */
 
 /* source kinds */
 #define AETHER_SOURCE_JETS 0   /* src/fluid.rs:532-595 inject_cyberpunk_sources */
@@ -328,16 +328,20 @@ int aether_fluid_batch_run(aether_fluid_batch_t b, double dt, const double *time
                            aether_fluid_telemetry *out);
 /* CG iterations of the last run, nsteps*count values, step-major. */
 int aether_fluid_batch_iterations(aether_fluid_batch_t b, int32_t *out, int n, int *count);
-/* CG layout and tile autotuner of a batch: times `iters` CG iterations of
- * the whole batch in the four-wide layout (FP32, every nx % 4 == 0) with 2
- * and with 4 z planes per thread, in the one-cell-per-thread layout (8 z
- * cells per thread), and in the one-launch layout solved group by group
- * (FP32, every nx % 4 == 0), on synthetic right-hand sides (every instance's
- * pressure and divergence are saved and restored), and keeps the fastest as
- * the batch's choice for cg_form AETHER_CG_AUTO and cg_z_cells 0.
- * us_per_iter[4] (may be NULL): four-wide z2, four-wide z4, one cell per
- * thread, one launch (-1 when not applicable).  *form / *z_cells: the
- * choice. */
+/* CG layout and tile autotuner of a batch: times the whole batch's CG in the
+ * four-wide layout (FP32, every nx % 4 == 0) with 2 and with 4 z planes per
+ * thread, in the one-cell-per-thread layout (8 z cells per thread), and in
+ * the one-launch layout solved group by group (FP32, every nx % 4 == 0), and
+ * keeps the fastest as the batch's choice for cg_form AETHER_CG_AUTO and
+ * cg_z_cells 0 (every instance's pressure and divergence are saved and
+ * restored).  iters > 0: `iters` iterations on synthetic right-hand sides,
+ * no tolerance stop.  iters == 0: whole solves, from zero pressure, of the
+ * last step's right-hand sides at the batch's tolerance and iteration cap,
+ * so that instances converging at different iterations count as in the
+ * batch's steps (AETHER_EINVAL before the first step).  us_per_iter[4] (may
+ * be NULL): time per iteration (iters 0: per iteration of the instance that
+ * ran longest) of four-wide z2, four-wide z4, one cell per thread, one launch
+ * (-1 when not applicable).  *form / *z_cells: the choice. */
 int aether_fluid_batch_autotune_cg(aether_fluid_batch_t b, int iters, double *us_per_iter,
                                    int *form, int *z_cells);
 

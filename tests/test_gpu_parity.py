@@ -1102,6 +1102,46 @@ def test_batch_autotune_keeps_state_and_results():
                 assert rel_l2(x, y) <= 1e-5, (q, n)
 
 
+def test_batch_autotune_on_step_right_hand_sides():
+    """autotune_cg(0) times whole solves of the last step's right-hand sides
+    at the batch's tolerance and cap: refused before the first step; after
+    it, pressure and divergence are restored, and the next step matches an
+    untuned batch run alongside (bit for bit when it keeps the untuned
+    four-wide layout with 4 planes per thread, to the FP32 bar otherwise)."""
+    _announce("batch autotune on step right-hand sides")
+    import aether_fluid as af
+    insts = [dict(nx=32, ny=24, nz=20, vorticity_eps=0.3, buoyancy_beta=1.0, gravity_alpha=0.05,
+                  source_center=(16.0, 4.0, 10.0), source_radius=3.0),
+             dict(nx=20, ny=28, nz=16, vorticity_eps=0.2, buoyancy_beta=1.0, gravity_alpha=0.08,
+                  source_center=(10.0, 4.0, 8.0), source_radius=2.5)]
+    bs = [af.FluidBatch(insts, solver_iters=80, cg_rel_tol=1e-5) for _ in range(2)]
+    for b in bs:
+        for q, d in enumerate(insts):
+            st = random_state(d["nx"], d["ny"], d["nz"], seed=70 + q, vel=0.5)
+            for n in ("u", "v", "w", "density", "temperature", "pressure"):
+                b.upload(q, n, st[n])
+    with pytest.raises(af.AetherFluidError):
+        bs[0].autotune_cg(0)
+    for b in bs:
+        b.run(0.02, [0.0])
+    before = [bs[0].download(q, n) for q in range(2) for n in ("pressure", "divergence")]
+    r = bs[0].autotune_cg(0)
+    print("batch autotune (step right-hand sides)", r)
+    assert set(r["us_per_iter"]) == {"x4_z2", "x4_z4", "scalar_z8", "fused_z4"}
+    after = [bs[0].download(q, n) for q in range(2) for n in ("pressure", "divergence")]
+    for x, y in zip(before, after):
+        np.testing.assert_array_equal(x, y)
+    for b in bs:
+        b.run(0.02, [0.02])
+    for q in range(2):
+        for n in ("u", "v", "w", "density"):
+            x, y = bs[0].download(q, n), bs[1].download(q, n)
+            if r["choice"] == "x4" and r["z_cells"] == 4:
+                np.testing.assert_array_equal(x, y)
+            else:
+                assert rel_l2(x, y) <= 1e-5, (q, n)
+
+
 def test_batched_graph_replay_equals_stream_launches():
     """Steps captured into HIP graphs and replayed give, bit for bit, what the
     same batch gives launched step by step: odd step counts (the density /
