@@ -1142,19 +1142,23 @@ def test_batch_autotune_on_step_right_hand_sides():
                 assert rel_l2(x, y) <= 1e-5, (q, n)
 
 
-def test_batched_graph_replay_equals_stream_launches():
+@pytest.mark.parametrize("form", ["auto", "fused"])
+def test_batched_graph_replay_equals_stream_launches(form):
     """Steps captured into HIP graphs and replayed give, bit for bit, what the
     same batch gives launched step by step: odd step counts (the density /
-    temperature slot parity) and a second run continuing the first."""
-    _announce("batched graph replay vs stream launches")
+    temperature slot parity) and a second run continuing the first; with the
+    untuned layout and with the one-launch form by groups (whose captured
+    solve holds every group's launches, 60 of them, with a mid-cycle cap)."""
+    _announce(f"batched graph replay vs stream launches ({form})")
     import aether_fluid as af
+    cg_form = af.CG_AUTO if form == "auto" else af.CG_FUSED
     insts = [dict(nx=32, ny=24, nz=20, vorticity_eps=0.3, buoyancy_beta=1.0, gravity_alpha=0.05,
                   source_center=(16.0, 4.0, 10.0), source_radius=3.0),
              dict(nx=20, ny=28, nz=16, vorticity_eps=0.2, buoyancy_beta=1.0, gravity_alpha=0.08,
                   source_center=(10.0, 4.0, 8.0), source_radius=2.5)]
     runs = []
     for graph in (False, True):
-        b = af.FluidBatch(insts, solver_iters=60, cg_rel_tol=1e-5, graph=graph)
+        b = af.FluidBatch(insts, solver_iters=60, cg_rel_tol=1e-5, graph=graph, cg_form=cg_form)
         for q, d in enumerate(insts):
             st = random_state(d["nx"], d["ny"], d["nz"], seed=90 + q, vel=0.5)
             for n in ("u", "v", "w", "density", "temperature"):
