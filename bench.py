@@ -273,11 +273,20 @@ def cg256_roofline(iters: int = 20) -> dict:
     per = {k: {"us_per_iter": v, "gbps": bytes_it / (v * 1e-6) / 1e9}
            for k, v in tune["us_per_iter"].items()}
     best = min(per, key=lambda k: per[k]["us_per_iter"])
+    # HBM-side bytes of one iteration of the four-wide pair with 4 planes per
+    # thread, from the FETCH_SIZE / WRITE_SIZE passes in profiles/traffic.json
+    traffic = None
+    if best == "x4_z4":
+        try:
+            with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+                traffic = json.load(f).get("cg256_iteration", {}).get("bytes_per_iteration")
+        except (OSError, ValueError):
+            traffic = None
     return {"bound": "hbm", "kernel": f"CG iteration 256^3 ({best}: k_cg_dir4 + k_cg_update4 "
             "+ k_partials_total)" if best.startswith("x4") else f"CG iteration 256^3 ({best})",
             "achieved": per[best]["gbps"], "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-            "frac": per[best]["gbps"] / HBM_PEAK_GBPS, "algorithmic_bytes_per_iter": bytes_it,
-            "layouts": per}
+            "frac": per[best]["gbps"] / HBM_PEAK_GBPS, "traffic": traffic,
+            "algorithmic_bytes_per_iter": bytes_it, "layouts": per}
 
 
 FD_EPS = (("fd_1e-3", 1e-3), ("fd_1e-4", 1e-4), ("fd_1e-5", 1e-5), ("fd_1e-6", 1e-6),
